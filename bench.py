@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Throughput of the explicit residual/update path on one MI355X.
+
+Metric: cell-updates/s per RK stage (real cells x stages / wall time of the time-step
+loop: dt + all stages), FP64 and mixed precision, with the mixed run's error against
+the FP64 GPU run per conserved variable, and the dominant kernel's HBM roofline
+fraction.  Workload (default): 80^3 box split into 6 Kuhn tetrahedra per box cell
+(3,072,000 tets), interior nodes jittered +-8%, Mach-2 planar shock into quiescent gas
+with 1% seeded density noise, second-order HLLC with WLSQ gradients and the
+Venkatakrishnan limiter, Wray RK3 (the reference's three-stage scheme), slip walls.
+A step is one time step (dt + 3 stages) over the whole mesh; inputs are resident in
+HBM before the timed region.
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+KERNELS = ["face_flux", "update", "gradient", "blend", "time_step", "bface_flux"]
+
+
+def algorithmic_bytes(m, num, kernel: str) -> float:
+    """Compulsory bytes of one launch: every array element the kernel needs, once, at
+    its stored width (double4 geometry counted at 32 B)."""
+    N, F, B = int(m.n_cells), int(m.n_faces), int(m.n_bfaces)
+    S = int(m.c2cs_idx.size) if num.order == 2 else 0
+    K = int(m.c2f_idx.size)               # cell-to-face entries (2F + B)
+    fb = 4 if num.precision in (1, 3) else 8   # flux buffer width
+    gb = 4 if num.precision in (1, 2) else 8   # gradient width
+    if kernel == "gradient":
+        b = N * (40 + 32 + 48 + 4 + 8 + 4) + 4 * S + 15 * gb * N
+        if num.reconst == 11:
+            b += 4 * K + F * (8 + 32)     # c2f, f2c and face centroids of the limiter
+        return float(b)
+    if kernel == "face_flux":
+        per_face = 8 + 32 + (32 if num.order == 2 else 0) + 5 * fb
+        per_cell = (40 + 32 + 15 * gb) if num.order == 2 else (40 + 24)
+        return float(F * per_face + N * per_cell)
+    if kernel == "update":
+        # c2f, flux (each face value once), cv in/out, rhsSum in/out, vol, dv out,
+        # prim out (order 2)
+        b = 4 * (N + 1) + 4 * K + 5 * fb * (F + B) + N * (40 + 40 + 40 + 40 + 8 + 24)
+        if num.order == 2:
+            b += 40 * N
+        return float(b)
+    if kernel == "time_step":
+        return float(4 * (N + 1) + 4 * K + F * (8 + 32) + B * 32 + N * (40 + 24 + 8))
+    return 0.0
+
+
+def build_case(args, precision):
+    from rocflu_amd import cases
+    from rocflu_amd.solver import Numerics
+    num = Numerics(discr=args.discr, order=2, reconst=args.reconst, rk=2, cfl=0.5,
+                   dt_imposed=1e-4, dt_min_limit=0.0, precision=precision)
+    return num
+
+
+def cpu_baseline(args):
+    """The oracle (a single-threaded C restatement of the reference loops) on a bounded
+    sample of the same workload: a 20^3 x 6-tet mesh, same scheme, a few steps."""
+    from rocflu_amd import cases
+    from oracle.oracle import Oracle
+    n = args.cpu_n
+    m = cases.tet_box(n=n, jitter=0.08, seed=2)
+    num = build_case(args, 0)
+    cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
+    o = Oracle(m, num)
+    o.run(cv0, 1)  # warm-up
+    steps = args.cpu_steps
+    t0 = time.perf_counter()
+    _, _, st = o.run(cv0, steps)
+    el = time.perf_counter() - t0
+    return {"value": m.n_cells * 3 * steps / el, "unit": "cell-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{n}^3x6 = {m.n_cells} tets, {steps} Wray steps of the same scheme, "
+                      f"{el:.1f} s, single thread"}
+
+
+def run(args):
+    from rocflu_amd import cases
+    from rocflu_amd.solver import GpuRegion
+    import torch  # device memory and streams are owned by the C-ABI library
+
+    t0 = time.time()
+    m = cases.tet_box(n=args.n, jitter=0.08, seed=2)
+    print(f"# mesh {args.n}^3x6: {m.n_cells} tets, {m.n_faces} faces, "
+          f"{m.c2cs_idx.size} stencil entries, built in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    results = {}
+    finals = {}
+    for prec in ([0, 1] if args.precision == "both" else [0 if args.precision == "fp64" else 1]):
+        num = build_case(args, prec)
+        cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
+        g = GpuRegion(m, num)
+        g.upload(cv0)
+        g.time_steps(args.warmup)
+        g.synchronize()
+        torch.cuda.synchronize()
+        g.set_profiling(True)
+        t1 = time.perf_counter()
+        g.time_steps(args.steps)
+        g.synchronize()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t1
+        kt = g.kernel_times()
+        g.set_profiling(False)
+        # graph replay of the same loop (launch overhead removed)
+        g.upload(cv0)
+        g.time_steps(args.warmup, graph=True)
+        g.synchronize()
+        t2 = time.perf_counter()
+        g.time_steps(args.steps, graph=True)
+        g.synchronize()
+        elg = time.perf_counter() - t2
+        # accuracy leg: identical step count from the same state for both precisions
+        g.upload(cv0)
+        g.time_steps(args.err_steps, graph=True)
+        finals[prec] = g.download()[0]
+        st = g.stats()
+        g.close()
+        n_upd = m.n_cells * 3 * args.steps
+        results[prec] = {"value": n_upd / el, "ms_per_step": 1e3 * el / args.steps,
+                         "graph_value": n_upd / elg, "kernels": kt, "num": num, "stats": st}
+        print(f"# prec {prec}: {n_upd / el:.4g} cell-upd/s ({1e3 * el / args.steps:.3f} ms/step), "
+              f"graph {n_upd / elg:.4g}; kernels {kt}", file=sys.stderr, flush=True)
+    return m, results, finals
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=80, help="box cells per side (6 tets each)")
+    ap.add_argument("--discr", type=int, default=3, help="1 Roe, 3 HLLC, 4 AUSM+")
+    ap.add_argument("--reconst", type=int, default=11)
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--err-steps", type=int, default=200)
+    ap.add_argument("--cpu-n", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design note only)"}))
+        sys.exit(2)
+    ws = int(os.environ.get("WORLD_SIZE", "1"))
+    if ws != 1:
+        print(json.dumps({"error": "run with one rank"}))
+        sys.exit(2)
+
+    m, res, finals = run(args)
+    main_prec = 0 if 0 in res else 1
+    r = res[main_prec]
+    kt = r["kernels"]
+    dom = max(KERNELS, key=lambda k: kt[k][0])
+    ms_launch = kt[dom][0] / max(kt[dom][1], 1)
+    ab = algorithmic_bytes(m, r["num"], dom)
+    achieved = ab / (ms_launch * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_tet{args.n}_prec{main_prec}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    out = {
+        "metric": "cell-updates/s per RK stage, FP64 vs mixed at 1e-6 rel L2; % of HBM peak",
+        "value": r["value"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if main_prec == 0 else "f32/f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.n}^3 x 6 Kuhn tets ({m.n_cells} cells, {m.n_faces} faces), "
+                               f"jitter 8%, Mach-2 shock + 1% density noise, "
+                               f"{ {1: 'Roe', 3: 'HLLC', 4: 'AUSM+'}[args.discr]} order 2, "
+                               f"WLSQ + {'Venkatakrishnan' if args.reconst == 11 else 'WENO-XYZ'}, "
+                               f"Wray RK3, slip walls",
+                   "cells": int(m.n_cells), "stages_per_step": 3},
+        "graph_value": r["graph_value"],
+        "kernel_ms": {k: [v[0], v[1]] for k, v in kt.items()},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": ab, "ms_per_launch": ms_launch},
+    }
+    if 1 in res and 0 in res:
+        a, b = finals[1], finals[0]
+        errs = [float(np.linalg.norm(a[:, k] - b[:, k]) / np.linalg.norm(b[:, k])) for k in range(5)]
+        mom = np.linalg.norm(b[:, 1:4])
+        errs_mom = [float(np.linalg.norm(a[:, k] - b[:, k]) / mom) for k in (1, 2, 3)]
+        out["mixed"] = {"value": res[1]["value"], "ms_per_step": res[1]["ms_per_step"],
+                        "graph_value": res[1]["graph_value"],
+                        "speedup_vs_fp64": res[1]["value"] / res[0]["value"],
+                        "err_steps": args.err_steps,
+                        "rel_l2_per_variable": errs,
+                        "rel_l2_momentum_vs_momentum_norm": errs_mom}
+    if not args.no_cpu:
+        cb = cpu_baseline(args)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = r["value"] / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,181 @@
+/*
+ * rocflu_amd.h -- C-ABI of the MI355X explicit residual/update path.
+ *
+ * The reference has no plugin system: its boundary is the set of Fortran subroutines
+ * RungeKuttaMP calls once per stage for each region (modfloflu/ModInterfaces.F90:
+ * 85-263, called from libfloflu/RungeKuttaMP.F90:216,220,274,348,432,437) and the
+ * time-step pair called from rocflu/RFLU_TimeStepping.F90:429,444.  Each entry point
+ * below replaces one of them for one region; a Fortran ISO_C_BINDING module
+ * (rocflu_amd/fortran/rflu_gpu.f90) binds them so the reference host keeps its
+ * t_region data and input files.
+ *
+ * Conventions: plain pointers and sizes, column-major arrays with the component index
+ * fastest (cv(5,nCells) is double cv[5*ic+iv]), cell/face indices with the base given
+ * in the region descriptor (1 for Fortran arrays).  Every call returns 0 or one of the
+ * RFLU_ERR_* codes; rflu_gpu_last_error() describes the last failure.  Calls on one
+ * region are ordered on that region's HIP stream and return without waiting, except
+ * those that hand a value back to the host (documented per call).
+ */
+#ifndef ROCFLU_AMD_H
+#define ROCFLU_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  RFLU_ERR_INVALID_VALUE and RFLU_ERR_NEGATIVE_POSDEF are the
+ * reference's ERR_INVALID_VALUE (NaN, libflu/RFLU_CheckValidity.F90:165-222) and
+ * ERR_NEGATIVE_POSDEF (libflu/RFLU_CheckPositivity.F90:219-253). */
+#define RFLU_OK 0
+#define RFLU_ERR_INVALID_VALUE 1
+#define RFLU_ERR_NEGATIVE_POSDEF 2
+#define RFLU_ERR_ARG 3
+#define RFLU_ERR_HIP 4
+#define RFLU_ERR_NOMEM 5
+#define RFLU_ERR_HANDLE 6
+
+/* Flux schemes (DISCR, modfloflu/ModParameters.F90:662-667). */
+#define RFLU_DISCR_ROE 1
+#define RFLU_DISCR_HLLC 3
+#define RFLU_DISCR_AUSMPLUS 4
+/* Gradient modifiers (RECONST, modfloflu/ModParameters.F90:678-683). */
+#define RFLU_RECONST_NONE 0
+#define RFLU_RECONST_WENO_XYZ 2
+#define RFLU_RECONST_LIM_VENKAT 11
+/* Time integrators (RKSCHEME, modfloflu/ModParameters.F90:615-618); 4 is Shu-Osher
+ * SSP-RK3, which the reference does not have. */
+#define RFLU_RK4_CLASSICAL 1
+#define RFLU_RK3_WRAY 2
+#define RFLU_RK1_EULER 3
+#define RFLU_RK3_SSP 4
+/* Arithmetic: FP64 everywhere, or FP32 gradients and fluxes with FP64 residual
+ * accumulation, state, limiter and positivity checks. */
+#define RFLU_PREC_FP64 0
+#define RFLU_PREC_MIXED 1
+/* Diagnostic splits of the mixed mode: FP32 only in the gradients, or only in the
+ * interior fluxes. */
+#define RFLU_PREC_FP32_GRAD 2
+#define RFLU_PREC_FP32_FLUX 3
+
+/*
+ * One region's grid and numerics -- the t_grid/t_patch/t_mixt_input fields the hot
+ * path reads (modfloflu/ModGrid.F90:95-224, ModBndPatch.F90:113-148,
+ * ModMixture.F90:97-105).  Boundary faces of all patches are concatenated patch by
+ * patch; bpatch_ptr holds 0-based offsets into them.  The stencil arrays flatten
+ * c2cs(icg)%cellMembs into a CSR (c2cs_ptr has n_cells+1 0-based offsets) and
+ * xyz_moms is c2cs(icg)%xyzMoms (10 per cell); they are read only for order 2.
+ * The library copies everything it needs during rflu_gpu_create_region.
+ */
+typedef struct {
+  int32_t index_base;        /* 0 or 1: base of f2c, bf2c and c2cs_idx values      */
+  int32_t n_cells;           /* nCells (no virtual cells on a single region)         */
+  int32_t n_faces;           /* nFaces                                               */
+  int32_t n_patches;
+  const int32_t* f2c;        /* (2, n_faces)                                         */
+  const double* fn;          /* (4, n_faces) unit normal and |A|                     */
+  const double* fc;          /* (3, n_faces) face centroid                           */
+  const double* cofg;        /* (3, n_cells)                                         */
+  const double* vol;         /* (n_cells)                                            */
+  const int32_t* bpatch_ptr; /* (n_patches+1)                                        */
+  const int32_t* bf2c;       /* (nBFaces)                                            */
+  const double* bfn;         /* (4, nBFaces)                                         */
+  const double* bfc;         /* (3, nBFaces)                                         */
+  const int32_t* patch_bc;   /* (n_patches) 20 outflow, 60 slip wall                 */
+  const int32_t* patch_opt;  /* (n_patches) outflow option 0 sup., 1 sub., 2 mixed   */
+  const double* patch_pout;  /* (n_patches) outflow static pressure                  */
+  const int32_t* patch_order;/* (n_patches) 1 or 2; NULL = all 1 (reference default) */
+  const int32_t* c2cs_ptr;   /* (n_cells+1), order 2 only                            */
+  const int32_t* c2cs_idx;
+  const double* xyz_moms;    /* (10, n_cells), order 2 only                          */
+  double gamma, cp;          /* global%refGamma, global%refCp                        */
+  int32_t discr, order, reconst, rk, precision;
+  double cfl, dt_imposed, dt_min_limit, eps_entr, diss_fact;
+} rflu_region_desc;
+
+/* Per-region counters of the after-update checks, accumulated since the last reset. */
+typedef struct {
+  int64_t n_invalid;      /* cells with NaN (ERR_INVALID_VALUE)                     */
+  int64_t n_negative;     /* cells with rho <= 0 or p <= 0 after the energy floor   */
+  int64_t n_bounds_fix;   /* RFLU_EnforceBounds clamps                              */
+  int64_t n_pos_fix;      /* RFLU_CheckPositivity energy floors                     */
+  double mass_in, mass_out; /* outflow mass bookkeeping of stage 1 (global%massIn/Out) */
+  double time, dt;        /* global%currentTime and the last global%dtMin           */
+  int64_t steps;          /* completed time steps                                   */
+} rflu_gpu_stats;
+
+/* Number of visible GPUs (0 when none); never fails. */
+int rflu_gpu_device_count(void);
+
+/* Builds the device copy of one region (RFLU_InitFlowSolver's upload point). */
+int rflu_gpu_create_region(const rflu_region_desc* desc, int32_t* handle);
+int rflu_gpu_destroy_region(int32_t handle);
+
+/* cv (5,n_cells) conservative state; dv (3,n_cells) p,T,c or NULL to compute it
+ * (RFLU_SetDependentVars).  Also sets global%currentTime. */
+int rflu_gpu_upload_state(int32_t handle, const double* cv, const double* dv, double time);
+/* Copies the state back; waits for the region's stream.  dv may be NULL. */
+int rflu_gpu_download_state(int32_t handle, double* cv, double* dv);
+
+/* RFLU_TimeStepInviscid + RFLU_MinimumTimeStep (rocflu/RFLU_TimeStepInviscid.F90:
+ * 153-334, rocflu/RFLU_MinimumTimeStep.F90:126-245) for a single region at the host's
+ * global%currentTime: sets the device dtMin (and the device time to current_time).
+ * If dt_out is not NULL, waits and returns it. */
+int rflu_gpu_time_step(int32_t handle, double current_time, double* dt_out);
+
+/* RKInitMP (libfloflu/RkInitGeneric.F90:128-160). */
+int rflu_gpu_rk_init(int32_t handle, int32_t istage);
+/* CellGradientsMP (libfloflu/CellGradientsMP.F90:160-296); no-op at order 1. */
+int rflu_gpu_cell_gradients(int32_t handle);
+/* ConvectiveFluxesMP (libfloflu/ConvectiveFluxes.F90:133-159): fluxes of all faces. */
+int rflu_gpu_convective_fluxes(int32_t handle, int32_t istage);
+/* RKUpdateMP (libfloflu/RkUpdateGeneric.F90:255-325) with the residual gathered per
+ * cell over the cell-to-face list in the reference's summation order. */
+int rflu_gpu_rk_update(int32_t handle, int32_t istage);
+/* RFLU_SetVarsContWrapper over all cells (libflu/RFLU_SetDependentVars.F90:255-284). */
+int rflu_gpu_set_dependent_vars(int32_t handle);
+/* AfterUpdateMP (libfloflu/AfterUpdateMP.F90:134-136); waits and returns
+ * RFLU_ERR_INVALID_VALUE / RFLU_ERR_NEGATIVE_POSDEF like the reference's ErrorStop. */
+int rflu_gpu_after_update(int32_t handle, int32_t istage);
+
+/* Residual R(cv) of the current state (gradients included for order 2), gathered
+ * into rhs (5,n_cells); waits.  For inspection and tests. */
+int rflu_gpu_residual(int32_t handle, double* rhs);
+
+/* RFLU_TimeStepping's loop body (rocflu/RFLU_TimeStepping.F90:353-640) nsteps times:
+ * time step, all RungeKuttaMP stages with the update, dependent variables and checks
+ * fused, and the time advance.  With use_graph = 1 one step is captured in a HIP
+ * graph and replayed.  Stops advancing (state frozen) at the first failed check and
+ * returns its code; waits. */
+int rflu_gpu_time_steps(int32_t handle, int64_t nsteps, int32_t use_graph);
+
+int rflu_gpu_get_stats(int32_t handle, rflu_gpu_stats* stats);
+int rflu_gpu_reset_stats(int32_t handle);
+
+/* Waits for the region's stream. */
+int rflu_gpu_synchronize(int32_t handle);
+
+/* Launch tuning: threads per block for the face, cell and gradient kernels (64..1024,
+ * multiple of 64; 0 keeps the current value). */
+int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
+                        int32_t grad_block);
+
+/* Times `reps` launches of one kernel of the stage loop on the region's stream with
+ * HIP events and returns the average milliseconds per launch (which: 0 face flux,
+ * 1 update, 2 gradient, 3 limiter/blend, 4 time step, 5 fused stage).  Waits. */
+int rflu_gpu_time_kernel(int32_t handle, int32_t which, int32_t reps, double* ms);
+
+/* Per-kernel timing of rflu_gpu_time_steps: when on, every launch is bracketed by HIP
+ * events on the region's stream (graphs are not used) and the milliseconds are
+ * accumulated per kernel class (0 face flux, 1 update, 2 gradient, 3 blend, 4 time
+ * step, 5 boundary flux).  Turning it on or off clears the totals. */
+int rflu_gpu_set_profiling(int32_t handle, int32_t on);
+int rflu_gpu_get_kernel_times(int32_t handle, double* ms, int64_t* counts, int32_t n);
+
+const char* rflu_gpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

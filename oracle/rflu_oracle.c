@@ -1,0 +1,854 @@
+/*
+ * rflu_oracle.c -- CPU restatement of RocfluMP's explicit residual/update path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker of the GPU path; see rflu_oracle.h).
+ *
+ * FP64 throughout, built with -ffp-contract=off so every expression rounds as the
+ * reference's gfortran build does (no FMA contraction, CMakeLists.txt:44,
+ * machines/dane.sh:6).  Loops follow the reference's order: residuals are scattered
+ * face by face, interior faces first and then patch by patch, which fixes the
+ * per-cell summation order the GPU gather reproduces.  Expressions keep the
+ * reference's operand order and parenthesisation.
+ */
+#include "rflu_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define BC_OUTFLOW 20
+#define BC_SLIPWALL 60
+#define BCOPT_SUPERSONIC 0
+#define BCOPT_SUBSONIC 1
+#define BCOPT_MIXED 2
+
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+
+/* Gas constants of the dependent-variable path: gv(CP) = refCp and gv(MOL) = M(R)
+ * (libflu/RFLU_SetGasVars.F90:172-178), read back as R' = 8314.3/M and
+ * gamma' = cp/(cp - R') (libfloflu/MixtPerf_R.F90:93, MixtPerf_G.F90:78). */
+static void gas_prime(const orc_region* r, double* mm, double* rgas, double* gam) {
+  double gc0 = r->cp - r->cp / r->gamma;     /* MixtPerf_R_CpG */
+  double m = 8314.3 / gc0;                   /* MixtPerf_M_R  */
+  double gc = 8314.3 / m;                    /* MixtPerf_R_M  */
+  *mm = m;
+  *rgas = gc;
+  *gam = r->cp / (r->cp - gc);               /* MixtPerf_G_CpR */
+}
+
+/* libflu/RFLU_SetDependentVars.F90:255-284 (TCPERF, explicit solver). */
+void orc_set_dependent_vars(const orc_region* r, const double* cv, double* dv, int beg, int end) {
+  double mm, gc, g;
+  gas_prime(r, &mm, &gc, &g);
+  for (int icg = beg; icg < end; ++icg) {
+    const double* c = cv + 5 * (size_t)icg;
+    double rr = c[0];
+    double ir = 1.0 / rr;
+    double Eo = ir * c[4];
+    double Vm2 = ir * ir * (c[1] * c[1] + c[2] * c[2] + c[3] * c[3]);
+    double p = (g - 1.0) * rr * (Eo - 0.5 * Vm2);  /* MixtPerf_P_DEoGVm2 */
+    double t = p / (rr * gc);                       /* MixtPerf_T_DPR */
+    dv[3 * (size_t)icg + 0] = p;
+    dv[3 * (size_t)icg + 1] = t;
+    dv[3 * (size_t)icg + 2] = sqrt(g * gc * t);     /* MixtPerf_C_GRT */
+  }
+}
+
+/* ---------------------------------------------------------------------------------
+ * Face flux functions.  Inputs are the left/right states the reference computes in
+ * its face loop; outputs flx(1:5) (already multiplied by |A|).
+ * ------------------------------------------------------------------------------- */
+
+/* HLLC (Batten): modflu/RFLU_ModHLLCFlux.F90:1064-1117 (order 1), 2310-2363 (order 2). */
+static void hllc_core(double g, double nx, double ny, double nz, double nm, double fs,
+                      double rl, double ul, double vl, double wl, double pl, double al,
+                      double el, double Hl, double rr, double ur, double vr, double wr,
+                      double pr, double ar, double er, double Hr, double* flx) {
+  double ql = ul * nx + vl * ny + wl * nz - fs;
+  double qr = ur * nx + vr * ny + wr * nz - fs;
+  double wt1 = sqrt(rr / rl);
+  double wt2 = 1.0 / (1.0 + wt1);
+  double us = wt2 * (ul + wt1 * ur);
+  double vs = wt2 * (vl + wt1 * vr);
+  double ws = wt2 * (wl + wt1 * wr);
+  double qs = wt2 * (ql + wt1 * qr);
+  double Hs = wt2 * (Hl + wt1 * Hr);
+  double Vms = us * us + vs * vs + ws * ws;
+  double as = sqrt((g - 1.0) * (Hs - 0.5 * Vms)); /* MixtPerf_C_GHoVm2 */
+  double sl = dmin(ql - al, qs - as);
+  double sr = dmax(qr + ar, qs + as);
+  if (sl > 0.0) {
+    flx[0] = ql * rl * nm;
+    flx[1] = (ql * rl * ul + pl * nx) * nm;
+    flx[2] = (ql * rl * vl + pl * ny) * nm;
+    flx[3] = (ql * rl * wl + pl * nz) * nm;
+    flx[4] = (ql * (rl * el + pl) + pl * fs) * nm;
+  } else if (sr < 0.0) {
+    flx[0] = qr * rr * nm;
+    flx[1] = (qr * rr * ur + pr * nx) * nm;
+    flx[2] = (qr * rr * vr + pr * ny) * nm;
+    flx[3] = (qr * rr * wr + pr * nz) * nm;
+    flx[4] = (qr * (rr * er + pr) + pr * fs) * nm;
+  } else {
+    double sm = (rr * qr * (sr - qr) - rl * ql * (sl - ql) - pr + pl) /
+                (rr * (sr - qr) - rl * (sl - ql));
+    double ps = pl + rl * (ql - sl) * (ql - sm);
+    double term, rs, rus, rvs, rws, res;
+    if (sm > 0.0) {
+      term = 1.0 / (sl - sm);
+      rs = term * (sl - ql) * rl;
+      rus = term * ((sl - ql) * rl * ul + (ps - pl) * nx);
+      rvs = term * ((sl - ql) * rl * vl + (ps - pl) * ny);
+      rws = term * ((sl - ql) * rl * wl + (ps - pl) * nz);
+      res = term * ((sl - ql) * rl * el + (ps * sm - pl * ql));
+    } else {
+      term = 1.0 / (sr - sm);
+      rs = term * (sr - qr) * rr;
+      rus = term * ((sr - qr) * rr * ur + (ps - pr) * nx);
+      rvs = term * ((sr - qr) * rr * vr + (ps - pr) * ny);
+      rws = term * ((sr - qr) * rr * wr + (ps - pr) * nz);
+      res = term * ((sr - qr) * rr * er + (ps * sm - pr * qr));
+    }
+    flx[0] = sm * rs * nm;
+    flx[1] = (sm * rus + ps * nx) * nm;
+    flx[2] = (sm * rvs + ps * ny) * nm;
+    flx[3] = (sm * rws + ps * nz) * nm;
+    flx[4] = (sm * (res + ps) - ps * fs) * nm;
+  }
+}
+
+/* Harten-Hyman entropy fix, modflu/RFLU_ModEntropyFixes.F90:95-106. */
+static inline double ef_hh(double l, double d) { return l > d ? l : (l * l + d * d) / (2.0 * d); }
+
+/* Roe FDS: modflu/RFLU_ModRoeFlux.F90:1838-1894 (order 1), 2762-2818 (order 2). */
+static void roe_core(double g, double epsentr, double dissFact, double nx, double ny, double nz,
+                     double nm, double fs, double rl, double ul, double vl, double wl, double pl,
+                     double Hl, double rr, double ur, double vr, double wr, double pr, double Hr,
+                     double* flx) {
+  double ql = ul * nx + vl * ny + wl * nz - fs;
+  double qr = ur * nx + vr * ny + wr * nz - fs;
+  double rh = sqrt(rl * rr);
+  double wt = rl / (rl + rh);
+  double uh = wt * ul + (1.0 - wt) * ur;
+  double vh = wt * vl + (1.0 - wt) * vr;
+  double wh = wt * wl + (1.0 - wt) * wr;
+  double Hh = wt * Hl + (1.0 - wt) * Hr;
+  double qh = uh * nx + vh * ny + wh * nz - fs;
+  double sh = 0.5 * (uh * uh + vh * vh + wh * wh);
+  double ah = sqrt((g - 1.0) * (Hh - 0.5 * (2.0 * sh)));
+  double l1 = fabs(qh - ah);
+  double l2 = fabs(qh);
+  double l5 = fabs(qh + ah);
+  double efc = epsentr * l5;
+  l1 = ef_hh(l1, efc);
+  l2 = ef_hh(l2, efc);
+  l5 = ef_hh(l5, efc);
+  double dr = rr - rl;
+  double du = ur - ul;
+  double de = vr - vl;
+  double dw = wr - wl;
+  double dp = pr - pl;
+  double dq = du * nx + de * ny + dw * nz;
+  double dv1 = (dp - rh * ah * dq) / (2.0 * ah * ah);
+  double dv5 = (dp + rh * ah * dq) / (2.0 * ah * ah);
+  double dv2 = dr - dp / (ah * ah);
+  double t1 = l1 * dv1;
+  double t2 = l2 * dv2;
+  double t3 = l2 * rh;
+  double t5 = l5 * dv5;
+  double fc1 = 0.5 * (ql * rl + qr * rr) * nm;
+  double fc2 = 0.5 * (ql * rl * ul + pl * nx + qr * rr * ur + pr * nx) * nm;
+  double fc3 = 0.5 * (ql * rl * vl + pl * ny + qr * rr * vr + pr * ny) * nm;
+  double fc4 = 0.5 * (ql * rl * wl + pl * nz + qr * rr * wr + pr * nz) * nm;
+  double fc5 = 0.5 * (ql * rl * Hl + pl * fs + qr * rr * Hr + pr * fs) * nm;
+  double fd1 = 0.5 * dissFact * (t1 + t2 + t5) * nm;
+  double fd2 = 0.5 * dissFact * (t1 * (uh - nx * ah) + t2 * uh + t3 * (du - nx * dq) + t5 * (uh + nx * ah)) * nm;
+  double fd3 = 0.5 * dissFact * (t1 * (vh - ny * ah) + t2 * vh + t3 * (de - ny * dq) + t5 * (vh + ny * ah)) * nm;
+  double fd4 = 0.5 * dissFact * (t1 * (wh - nz * ah) + t2 * wh + t3 * (dw - nz * dq) + t5 * (wh + nz * ah)) * nm;
+  double fd5 = 0.5 * dissFact * (t1 * (Hh - qh * ah) + t2 * sh + t3 * (uh * du + vh * de + wh * dw - qh * dq) + t5 * (Hh + qh * ah)) * nm;
+  flx[0] = fc1 - fd1;
+  flx[1] = fc2 - fd2;
+  flx[2] = fc3 - fd3;
+  flx[3] = fc4 - fd4;
+  flx[4] = fc5 - fd5;
+}
+
+/* AUSM+ flux function, modflu/RFLU_ModAUSMFlux.F90:282-367. */
+static void ausm_core(double nx, double ny, double nz, double nm, double fs, double rl, double ul,
+                      double vl, double wl, double pl, double Hl, double al, double rr, double ur,
+                      double vr, double wr, double pr, double Hr, double ar, double* flx) {
+  double ql = ul * nx + vl * ny + wl * nz - fs;
+  double qr = ur * nx + vr * ny + wr * nz - fs;
+  double af = 0.5 * (al + ar);
+  double ml = ql / af, mla = fabs(ml);
+  double mr = qr / af, mra = fabs(mr);
+  double mlp, wtl, mrm, wtr;
+  if (mla <= 1.0) {
+    mlp = 0.25 * (ml + 1.0) * (ml + 1.0) + 0.125 * (ml * ml - 1.0) * (ml * ml - 1.0);
+    wtl = 0.25 * (ml + 1.0) * (ml + 1.0) * (2.0 - ml) + 0.1875 * ml * (ml * ml - 1.0) * (ml * ml - 1.0);
+  } else {
+    mlp = 0.5 * (ml + mla);
+    wtl = 0.5 * (1.0 + ml / mla);
+  }
+  if (mra <= 1.0) {
+    mrm = -0.25 * (mr - 1.0) * (mr - 1.0) - 0.125 * (mr * mr - 1.0) * (mr * mr - 1.0);
+    wtr = 0.25 * (mr - 1.0) * (mr - 1.0) * (2.0 + mr) - 0.1875 * mr * (mr * mr - 1.0) * (mr * mr - 1.0);
+  } else {
+    mrm = 0.5 * (mr - mra);
+    wtr = 0.5 * (1.0 - mr / mra);
+  }
+  double mf = mlp + mrm;
+  double mfa = fabs(mf);
+  double mfp = 0.5 * (mf + mfa);
+  double mfm = 0.5 * (mf - mfa);
+  double pf = wtl * pl + wtr * pr;
+  flx[0] = (af * (mfp * rl + mfm * rr)) * nm;
+  flx[1] = (af * (mfp * rl * ul + mfm * rr * ur) + pf * nx) * nm;
+  flx[2] = (af * (mfp * rl * vl + mfm * rr * vr) + pf * ny) * nm;
+  flx[3] = (af * (mfp * rl * wl + mfm * rr * wr) + pf * nz) * nm;
+  flx[4] = (af * (mfp * rl * Hl + mfm * rr * Hr) + pf * fs) * nm;
+}
+
+/* Outflow boundary state, libfloflu/BcondOutflowPerf.F90:111-157. */
+static void bcond_outflow_perf(int bcOpt, double pout, double sxn, double syn, double szn,
+                               double cpgas, double mol, double rho, double rhou, double rhov,
+                               double rhow, double rhoe, double press, double* rhob,
+                               double* rhoub, double* rhovb, double* rhowb, double* rhoeb) {
+  double rgas = 8314.3 / mol;
+  double gamma = cpgas / (cpgas - rgas);
+  double u = rhou / rho, v = rhov / rho, w = rhow / rho;
+  double csound = sqrt(gamma * press / rho);
+  double mach = sqrt(u * u + v * v + w * w) / csound;
+  if (mach < 1.0 && (bcOpt == BCOPT_SUBSONIC || bcOpt == BCOPT_MIXED)) {
+    double rrhoc = 1.0 / (rho * csound);
+    double deltp = press - pout;
+    double rb = rho - deltp / (csound * csound);
+    double ub = u + sxn * deltp * rrhoc;
+    double vb = v + syn * deltp * rrhoc;
+    double wb = w + szn * deltp * rrhoc;
+    double vnd = ub * sxn + vb * syn + wb * szn;
+    if (vnd < 0.0) {
+      ub = copysign(1.0, u) * dmax(fabs(ub), fabs(u));
+      vb = copysign(1.0, v) * dmax(fabs(vb), fabs(v));
+      wb = copysign(1.0, w) * dmax(fabs(wb), fabs(w));
+    }
+    *rhob = rb;
+    *rhoub = rb * ub;
+    *rhovb = rb * vb;
+    *rhowb = rb * wb;
+    *rhoeb = rb * (pout / (rb * (gamma - 1.0)) + 0.5 * (ub * ub + vb * vb + wb * wb));
+  } else {
+    *rhob = rho; *rhoub = rhou; *rhovb = rhov; *rhowb = rhow; *rhoeb = rhoe;
+  }
+}
+
+/* Central flux of an outflow face from the interior and boundary states,
+ * rocflu/RFLU_CentralFirstPatch.F90:676-686 (same form in CentralSecondPatch). */
+static void outflow_flux(double nx, double ny, double nz, double nm, double fs, double rl,
+                         double rul, double rvl, double rwl, double rel, double pl, double rr,
+                         double rur, double rvr, double rwr, double rer, double pr, double* flx) {
+  double ql = (rul * nx + rvl * ny + rwl * nz) / rl - fs;
+  double qr = (rur * nx + rvr * ny + rwr * nz) / rr - fs;
+  flx[0] = 0.5 * (ql * rl + qr * rr) * nm;
+  flx[1] = 0.5 * (ql * rul + pl * nx + qr * rur + pr * nx) * nm;
+  flx[2] = 0.5 * (ql * rvl + pl * ny + qr * rvr + pr * ny) * nm;
+  flx[3] = 0.5 * (ql * rwl + pl * nz + qr * rwr + pr * nz) * nm;
+  flx[4] = 0.5 * (ql * (rel + pl) + pl * fs + qr * (rer + pr) + pr * fs) * nm;
+}
+
+/* Left/right states of an interior face for the flux functions.  Order 2 adds the
+ * linear reconstruction at the face centroid (e.g. RFLU_ModHLLCFlux.F90:2231-2262). */
+typedef struct { double r, u, v, w, p, a, e, H, ir; } face_state;
+
+static void reconstruct(const orc_region* R, const double* cv, const double* dv,
+                        const double* grad, int c, const double* xf, face_state* s) {
+  const double* q = cv + 5 * (size_t)c;
+  double r = q[0];
+  double ir = 1.0 / r;
+  double u = q[1] * ir, v = q[2] * ir, w = q[3] * ir;
+  double p = dv[3 * (size_t)c];
+  const double* gr = grad + 15 * (size_t)c;   /* grad(3,5,N) */
+  double dx = xf[0] - R->cofg[3 * (size_t)c];
+  double dy = xf[1] - R->cofg[3 * (size_t)c + 1];
+  double dz = xf[2] - R->cofg[3 * (size_t)c + 2];
+  s->r = r + gr[0] * dx + gr[1] * dy + gr[2] * dz;
+  s->u = u + gr[3] * dx + gr[4] * dy + gr[5] * dz;
+  s->v = v + gr[6] * dx + gr[7] * dy + gr[8] * dz;
+  s->w = w + gr[9] * dx + gr[10] * dy + gr[11] * dz;
+  s->p = p + gr[12] * dx + gr[13] * dy + gr[14] * dz;
+  s->ir = ir;
+}
+
+/* Interior face flux, dispatched on DISCR and order (rocflu/RFLU_ComputeFluxInv.F90:123-140). */
+static void interior_flux(const orc_region* R, const double* cv, const double* dv,
+                          const double* grad, int f, double* flx) {
+  int c1 = R->f2c[2 * (size_t)f], c2 = R->f2c[2 * (size_t)f + 1];
+  const double* n = R->fn + 4 * (size_t)f;
+  double nx = n[0], ny = n[1], nz = n[2], nm = n[3];
+  const double fs = 0.0; /* static grid: gs(indGs*ifg) with indGs = 0 */
+  const double g = R->gamma, cp = R->cp;
+  const double gc = cp - cp / g;  /* MixtPerf_R_CpG */
+  if (R->order == 1) {
+    const double* L = cv + 5 * (size_t)c1;
+    const double* Rr = cv + 5 * (size_t)c2;
+    double rl = L[0], irl = 1.0 / rl;
+    double ul = L[1] * irl, vl = L[2] * irl, wl = L[3] * irl;
+    double pl = dv[3 * (size_t)c1], tl = dv[3 * (size_t)c1 + 1], al = dv[3 * (size_t)c1 + 2];
+    double rr = Rr[0], irr = 1.0 / rr;
+    double ur = Rr[1] * irr, vr = Rr[2] * irr, wr = Rr[3] * irr;
+    double pr = dv[3 * (size_t)c2], tr = dv[3 * (size_t)c2 + 1], ar = dv[3 * (size_t)c2 + 2];
+    if (R->discr == 3) {
+      double el = L[4] * irl, er = Rr[4] * irr;
+      double Hl = el + irl * pl, Hr = er + irr * pr;
+      hllc_core(g, nx, ny, nz, nm, fs, rl, ul, vl, wl, pl, al, el, Hl, rr, ur, vr, wr, pr, ar,
+                er, Hr, flx);
+    } else if (R->discr == 1) {
+      double Hl = cp * tl + 0.5 * (ul * ul + vl * vl + wl * wl);  /* MixtPerf_Ho_CpTUVW */
+      double Hr = cp * tr + 0.5 * (ur * ur + vr * vr + wr * wr);
+      roe_core(g, R->eps_entr, R->diss_fact, nx, ny, nz, nm, fs, rl, ul, vl, wl, pl, Hl, rr, ur,
+               vr, wr, pr, Hr, flx);
+    } else {
+      /* RFLU_ModAUSMFlux.F90:668,713: H = E/rho + p/rho */
+      double Hl = L[4] * irl + pl / rl, Hr = Rr[4] * irr + pr / rr;
+      ausm_core(nx, ny, nz, nm, fs, rl, ul, vl, wl, pl, Hl, al, rr, ur, vr, wr, pr, Hr, ar, flx);
+    }
+    return;
+  }
+  const double* xf = R->fc + 3 * (size_t)f;
+  face_state sl, sr;
+  reconstruct(R, cv, dv, grad, c1, xf, &sl);
+  reconstruct(R, cv, dv, grad, c2, xf, &sr);
+  if (R->discr == 3) {
+    /* RFLU_ModHLLCFlux.F90:2262-2267: E from the face state, H mixes the cell 1/rho. */
+    double el = sl.p / (sl.r * (g - 1.0)) + 0.5 * (sl.u * sl.u + sl.v * sl.v + sl.w * sl.w);
+    double al = sqrt(g * sl.p / sl.r);
+    double Hl = el + sl.ir * sl.p;
+    double er = sr.p / (sr.r * (g - 1.0)) + 0.5 * (sr.u * sr.u + sr.v * sr.v + sr.w * sr.w);
+    double ar = sqrt(g * sr.p / sr.r);
+    double Hr = er + sr.ir * sr.p;
+    hllc_core(g, nx, ny, nz, nm, fs, sl.r, sl.u, sl.v, sl.w, sl.p, al, el, Hl, sr.r, sr.u, sr.v,
+              sr.w, sr.p, ar, er, Hr, flx);
+  } else if (R->discr == 1) {
+    double tl = sl.p / (sl.r * gc), tr = sr.p / (sr.r * gc);
+    double Hl = cp * tl + 0.5 * (sl.u * sl.u + sl.v * sl.v + sl.w * sl.w);
+    double Hr = cp * tr + 0.5 * (sr.u * sr.u + sr.v * sr.v + sr.w * sr.w);
+    roe_core(g, R->eps_entr, R->diss_fact, nx, ny, nz, nm, fs, sl.r, sl.u, sl.v, sl.w, sl.p, Hl,
+             sr.r, sr.u, sr.v, sr.w, sr.p, Hr, flx);
+  } else {
+    double tl = sl.p / (sl.r * gc), tr = sr.p / (sr.r * gc);
+    double al = sqrt(g * gc * tl), ar = sqrt(g * gc * tr);
+    double Hl = cp * tl + 0.5 * (sl.u * sl.u + sl.v * sl.v + sl.w * sl.w);
+    double Hr = cp * tr + 0.5 * (sr.u * sr.u + sr.v * sr.v + sr.w * sr.w);
+    ausm_core(nx, ny, nz, nm, fs, sl.r, sl.u, sl.v, sl.w, sl.p, Hl, al, sr.r, sr.u, sr.v, sr.w,
+              sr.p, Hr, ar, flx);
+  }
+}
+
+/* Boundary-face flux of patch face b (rocflu/RFLU_CentralFirstPatch.F90:571-906,
+ * rocflu/RFLU_CentralSecondPatch.F90:634-1024).  Boundary patches run at first order
+ * unless the patch asks for second order; this restatement uses the region's
+ * patch's order (patch_order[p] == 2 selects second order; the reference default,
+ * spaceOrderBFaces = 1, is at libflu/RFLU_InitInputValues.F90:195 and the per-patch
+ * ORDER key at modflu/RFLU_ModReadBcInputFile.F90:301-309).  A first-order patch
+ * in a second-order run reads cv after CellGradientsMP's conversion, whose slot 5
+ * holds p (see orc_cell_gradients), exactly as the reference's outflow patch does. */
+static void boundary_flux(const orc_region* R, const double* cv, const double* dv,
+                          const double* grad, int patch, int b, int patch_order2, double* flx,
+                          int* has_mass) {
+  int c1 = R->bf2c[b];
+  const double* n = R->bfn + 4 * (size_t)b;
+  double nx = n[0], ny = n[1], nz = n[2], nm = n[3];
+  const double fs = 0.0;
+  double mm, rgas, gam;
+  gas_prime(R, &mm, &rgas, &gam);
+  const double* q = cv + 5 * (size_t)c1;
+  int bc = R->patch_bc[patch];
+  *has_mass = 0;
+  if (!patch_order2) {
+    double rl = q[0], rul = q[1], rvl = q[2], rwl = q[3], rel = q[4];
+    double pl = dv[3 * (size_t)c1];
+    if (bc == BC_SLIPWALL) {
+      flx[0] = 0.0;
+      flx[1] = pl * nx * nm;
+      flx[2] = pl * ny * nm;
+      flx[3] = pl * nz * nm;
+      flx[4] = pl * fs * nm;
+      return;
+    }
+    int opt = R->patch_opt[patch];
+    double pr = opt != BCOPT_SUPERSONIC ? R->patch_pout[patch] : pl;
+    double rr, rur, rvr, rwr, rer;
+    bcond_outflow_perf(opt, pr, nx, ny, nz, R->cp, mm, rl, rul, rvl, rwl, rel, pl, &rr, &rur,
+                       &rvr, &rwr, &rer);
+    outflow_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, rr, rur, rvr, rwr, rer, pr, flx);
+    *has_mass = 1;
+    return;
+  }
+  double rl = q[0], irl = 1.0 / rl;
+  const double* xf = R->bfc + 3 * (size_t)b;
+  double dx = xf[0] - R->cofg[3 * (size_t)c1];
+  double dy = xf[1] - R->cofg[3 * (size_t)c1 + 1];
+  double dz = xf[2] - R->cofg[3 * (size_t)c1 + 2];
+  double ul = q[1] * irl, vl = q[2] * irl, wl = q[3] * irl;
+  double pl = dv[3 * (size_t)c1];
+  const double* gr = grad + 15 * (size_t)c1;
+  rl = rl + gr[0] * dx + gr[1] * dy + gr[2] * dz;
+  ul = ul + gr[3] * dx + gr[4] * dy + gr[5] * dz;
+  vl = vl + gr[6] * dx + gr[7] * dy + gr[8] * dz;
+  wl = wl + gr[9] * dx + gr[10] * dy + gr[11] * dz;
+  pl = pl + gr[12] * dx + gr[13] * dy + gr[14] * dz;
+  double rul = rl * ul, rvl = rl * vl, rwl = rl * wl;
+  if (bc == BC_SLIPWALL) {
+    flx[0] = 0.0;
+    flx[1] = pl * nx * nm;
+    flx[2] = pl * ny * nm;
+    flx[3] = pl * nz * nm;
+    flx[4] = pl * fs * nm;
+    return;
+  }
+  int opt = R->patch_opt[patch];
+  double pr = opt != BCOPT_SUPERSONIC ? R->patch_pout[patch] : pl;
+  double rel = rl * (pl / (rl * (gam - 1.0)) + 0.5 * (ul * ul + vl * vl + wl * wl));
+  double rr, rur, rvr, rwr, rer;
+  bcond_outflow_perf(opt, pr, nx, ny, nz, R->cp, mm, rl, rul, rvl, rwl, rel, pl, &rr, &rur, &rvr,
+                     &rwr, &rer);
+  outflow_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, rr, rur, rvr, rwr, rer, pr, flx);
+  *has_mass = 1;
+}
+
+/* ConvectiveFluxesMP -> RFLU_ComputeFluxInv: rhs = -diss (= 0 here), then the interior
+ * face scatter in ascending face order and the patch scatter
+ * (libfloflu/ConvectiveFluxes.F90:133-159, modflu/RFLU_ModHLLCFlux.F90:2207-2425). */
+void orc_fluxes(const orc_region* R, const double* cv, const double* dv, const double* grad,
+                double* rhs, orc_stats* st, int stage) {
+  size_t N = (size_t)R->n_cells;
+  for (size_t i = 0; i < 5 * N; ++i) rhs[i] = -0.0;
+  double flx[5];
+  for (int f = 0; f < R->n_faces; ++f) {
+    interior_flux(R, cv, dv, grad, f, flx);
+    int c1 = R->f2c[2 * (size_t)f], c2 = R->f2c[2 * (size_t)f + 1];
+    for (int k = 0; k < 5; ++k) rhs[5 * (size_t)c1 + k] = rhs[5 * (size_t)c1 + k] + flx[k];
+    for (int k = 0; k < 5; ++k) rhs[5 * (size_t)c2 + k] = rhs[5 * (size_t)c2 + k] - flx[k];
+  }
+  for (int p = 0; p < R->n_patches; ++p) {
+    int o2 = R->order == 2 && R->patch_order && R->patch_order[p] == 2;
+    for (int b = R->bpatch_ptr[p]; b < R->bpatch_ptr[p + 1]; ++b) {
+      int has_mass;
+      boundary_flux(R, cv, dv, grad, p, b, o2, flx, &has_mass);
+      int c1 = R->bf2c[b];
+      if (R->patch_bc[p] == BC_SLIPWALL) {
+        for (int k = 1; k < 5; ++k) rhs[5 * (size_t)c1 + k] = rhs[5 * (size_t)c1 + k] + flx[k];
+      } else {
+        for (int k = 0; k < 5; ++k) rhs[5 * (size_t)c1 + k] = rhs[5 * (size_t)c1 + k] + flx[k];
+        if (st && stage == 1) {
+          st->mass_in = st->mass_in - dmin(flx[0], 0.0);
+          st->mass_out = st->mass_out + dmax(flx[0], 0.0);
+        }
+      }
+    }
+  }
+}
+
+/* RFLU_ComputeGradCellsFast_3D, modflu/RFLU_ModDifferentiationCells.F90:931-1144.
+ * var (5,N) holds (rho,u,v,w,p); grad (3,5,N). */
+void orc_gradients(const orc_region* R, const double* var, double* grad) {
+  for (int icg = 0; icg < R->n_cells; ++icg) {
+    const double* mo = R->xyz_moms + 10 * (size_t)icg;
+    double r11 = mo[0], r12 = mo[1], r22 = mo[2], r13 = mo[3], r23 = mo[4], r33 = mo[5];
+    double c11 = 1.0 / r11, c22 = 1.0 / r22, c33 = 1.0 / r33;
+    double c12 = -c11 * r12;
+    double c13 = -(c11 * r13 + c12 * c22 * r23);
+    double c23 = -c22 * r23;
+    double* g = grad + 15 * (size_t)icg;
+    for (int k = 0; k < 15; ++k) g[k] = 0.0;
+    double xc = R->cofg[3 * (size_t)icg], yc = R->cofg[3 * (size_t)icg + 1],
+           zc = R->cofg[3 * (size_t)icg + 2];
+    double c1111 = c11 * c11, c2222 = c22 * c22, c3333 = c33 * c33;
+    const double* vi = var + 5 * (size_t)icg;
+    for (int s = R->c2cs_ptr[icg]; s < R->c2cs_ptr[icg + 1]; ++s) {
+      int j = R->c2cs_idx[s];
+      double dx = R->cofg[3 * (size_t)j] - xc;
+      double dy = R->cofg[3 * (size_t)j + 1] - yc;
+      double dz = R->cofg[3 * (size_t)j + 2] - zc;
+      double term = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+      dx = term * dx;
+      dy = term * dy;
+      dz = term * dz;
+      double term1 = c1111 * (dx);
+      double term2 = c2222 * (dy + c12 * dx);
+      double term3 = c3333 * (dz + c23 * dy + c13 * dx);
+      double wtx = term * (term1 + c12 * term2 + c13 * term3);
+      double wty = term * (term2 + c23 * term3);
+      double wtz = term * (term3);
+      const double* vj = var + 5 * (size_t)j;
+      for (int iv = 0; iv < 5; ++iv) {
+        double dVar = vj[iv] - vi[iv];
+        g[3 * iv + 0] = g[3 * iv + 0] + wtx * dVar;
+        g[3 * iv + 1] = g[3 * iv + 1] + wty * dVar;
+        g[3 * iv + 2] = g[3 * iv + 2] + wtz * dVar;
+      }
+    }
+  }
+}
+
+/* RFLU_WENOGradCellsXYZFast_3D, modflu/RFLU_ModWENO.F90:1437-1508: per component, blend
+ * the own and member gradients with weights 1/(g^2 + 1e-15), own value first. */
+void orc_weno_xyz(const orc_region* R, double* grad) {
+  size_t N = (size_t)R->n_cells;
+  double* eno = (double*)malloc(15 * N * sizeof(double));
+  for (size_t icg = 0; icg < N; ++icg) {
+    for (int k = 0; k < 15; ++k) {
+      double term = grad[15 * icg + k];
+      double sumw = 1.0 / (term * term + 1.0E-15);
+      double loc = sumw * term;
+      for (int s = R->c2cs_ptr[icg]; s < R->c2cs_ptr[icg + 1]; ++s) {
+        double t = grad[15 * (size_t)R->c2cs_idx[s] + k];
+        double w = 1.0 / (t * t + 1.0E-15);
+        sumw = sumw + w;
+        loc = loc + w * t;
+      }
+      eno[15 * icg + k] = loc / sumw;
+    }
+  }
+  memcpy(grad, eno, 15 * N * sizeof(double));
+  free(eno);
+}
+
+/* RFLU_LimitGradCellsSimple, modflu/RFLU_ModLimiters.F90:1199-1267: zero the gradient of
+ * a positive variable (rho, p) when its extrapolation to any face centroid is <= 0. */
+void orc_limit_simple(const orc_region* R, const double* var, double* grad) {
+  static const int pos_vars[2] = {0, 4};
+  for (int icg = 0; icg < R->n_cells; ++icg) {
+    double xc = R->cofg[3 * (size_t)icg], yc = R->cofg[3 * (size_t)icg + 1],
+           zc = R->cofg[3 * (size_t)icg + 2];
+    for (int q = 0; q < 2; ++q) {
+      int iv = pos_vars[q];
+      double* g = grad + 15 * (size_t)icg + 3 * iv;
+      double varc = var[5 * (size_t)icg + iv];
+      for (int k = R->c2f_ptr[icg]; k < R->c2f_ptr[icg + 1]; ++k) {
+        int e = R->c2f_idx[k];
+        const double* xf;
+        if (e > R->n_faces || e < -R->n_faces) xf = R->bfc + 3 * (size_t)(e - R->n_faces - 1);
+        else xf = R->fc + 3 * (size_t)((e > 0 ? e : -e) - 1);
+        double dx = xf[0] - xc, dy = xf[1] - yc, dz = xf[2] - zc;
+        double varf = varc + g[0] * dx + g[1] * dy + g[2] * dz;
+        if (varf <= 0.0) {
+          g[0] = 0.0; g[1] = 0.0; g[2] = 0.0;
+          break;
+        }
+      }
+    }
+  }
+}
+
+/* RFLU_ComputeLimiterVenkat + RFLU_LimitGradCells,
+ * modflu/RFLU_ModLimiters.F90:407-522,1100-1106.  Note dx1 = cofg(c1) - fc (the
+ * reference's sign) and eps^2 = (K*vol^(1/3))^3 with K = 1. */
+void orc_venkat(const orc_region* R, const double* var, double* grad) {
+  const double THRD = 1.0 / 3.0, TINY = 1.0E-12, K = 1.0;
+  size_t N = (size_t)R->n_cells;
+  double* vmin = (double*)malloc(5 * N * sizeof(double));
+  double* vmax = (double*)malloc(5 * N * sizeof(double));
+  double* lim = (double*)malloc(5 * N * sizeof(double));
+  for (size_t i = 0; i < 5 * N; ++i) { vmin[i] = var[i]; vmax[i] = var[i]; lim[i] = 1.0; }
+  for (int f = 0; f < R->n_faces; ++f) {
+    int c1 = R->f2c[2 * (size_t)f], c2 = R->f2c[2 * (size_t)f + 1];
+    for (int iv = 0; iv < 5; ++iv) {
+      double v1 = var[5 * (size_t)c1 + iv], v2 = var[5 * (size_t)c2 + iv];
+      vmin[5 * (size_t)c1 + iv] = dmin(vmin[5 * (size_t)c1 + iv], v2);
+      vmin[5 * (size_t)c2 + iv] = dmin(vmin[5 * (size_t)c2 + iv], v1);
+      vmax[5 * (size_t)c1 + iv] = dmax(vmax[5 * (size_t)c1 + iv], v2);
+      vmax[5 * (size_t)c2 + iv] = dmax(vmax[5 * (size_t)c2 + iv], v1);
+    }
+  }
+  for (int f = 0; f < R->n_faces; ++f) {
+    int c[2] = {R->f2c[2 * (size_t)f], R->f2c[2 * (size_t)f + 1]};
+    const double* xf = R->fc + 3 * (size_t)f;
+    for (int side = 0; side < 2; ++side) {
+      int ci = c[side];
+      double dx = R->cofg[3 * (size_t)ci] - xf[0];
+      double dy = R->cofg[3 * (size_t)ci + 1] - xf[1];
+      double dz = R->cofg[3 * (size_t)ci + 2] - xf[2];
+      double ds = pow(R->vol[ci], THRD);
+      double epsq = (K * ds) * (K * ds) * (K * ds);
+      for (int iv = 0; iv < 5; ++iv) {
+        double vi = var[5 * (size_t)ci + iv];
+        double dmn = vmin[5 * (size_t)ci + iv] - vi;
+        double dmx = vmax[5 * (size_t)ci + iv] - vi;
+        const double* g = grad + 15 * (size_t)ci + 3 * iv;
+        double d2 = g[0] * dx + g[1] * dy + g[2] * dz;
+        double* l = lim + 5 * (size_t)ci + iv;
+        if (d2 > 0.0) {
+          double numer = (dmx * dmx + epsq) * d2 + 2 * d2 * d2 * dmx;
+          double denom = d2 * (dmx * dmx + 2 * d2 * d2 + dmx * d2 + epsq);
+          double term = numer / (denom + TINY);
+          *l = dmin(term, *l);
+        } else if (d2 < 0.0) {
+          double numer = (dmn * dmn + epsq) * d2 + 2 * d2 * d2 * dmn;
+          double denom = d2 * (dmn * dmn + 2 * d2 * d2 + dmn * d2 + epsq);
+          double term = numer / (denom + TINY);
+          *l = dmin(term, *l);
+        }
+      }
+    }
+  }
+  for (size_t i = 0; i < N; ++i)
+    for (int iv = 0; iv < 5; ++iv)
+      for (int d = 0; d < 3; ++d)
+        grad[15 * i + 3 * iv + d] = lim[5 * i + iv] * grad[15 * i + 3 * iv + d];
+  free(vmin); free(vmax); free(lim);
+}
+
+/* CellGradientsMP, libfloflu/CellGradientsMP.F90:160-296: cons -> (rho,u,v,w,p) in
+ * place, gradients, reconstruction modifier, and back to conservative form.  The
+ * back-conversion of the default build leaves p in slot 5
+ * (modflu/RFLU_ModConvertCv.F90:560-574), which this restatement keeps. */
+void orc_cell_gradients(const orc_region* R, double* cv, const double* dv, double* grad) {
+  size_t N = (size_t)R->n_cells;
+  for (size_t i = 0; i < N; ++i) {
+    double ir = 1.0 / cv[5 * i];
+    cv[5 * i + 1] = ir * cv[5 * i + 1];
+    cv[5 * i + 2] = ir * cv[5 * i + 2];
+    cv[5 * i + 3] = ir * cv[5 * i + 3];
+    cv[5 * i + 4] = dv[3 * i];
+  }
+  orc_gradients(R, cv, grad);
+  if (R->reconst == 2) {
+    orc_weno_xyz(R, grad);
+    orc_limit_simple(R, cv, grad);
+  } else if (R->reconst == 11) {
+    orc_venkat(R, cv, grad);
+  }
+  for (size_t i = 0; i < N; ++i) {
+    double r = cv[5 * i];
+    cv[5 * i + 1] = r * cv[5 * i + 1];
+    cv[5 * i + 2] = r * cv[5 * i + 2];
+    cv[5 * i + 3] = r * cv[5 * i + 3];
+  }
+}
+
+int orc_nstages(int rk) { return rk == 1 ? 4 : rk == 2 ? 3 : rk == 3 ? 1 : rk == 4 ? 3 : 0; }
+
+/* RkUpdateGeneric, libfloflu/RkUpdateGeneric.F90:255-325 (static grid); ark/grk from
+ * libflu/RFLU_DerivedInputValues.F90:347-396.  rk = 4 is the Shu-Osher SSP-RK3, which the
+ * reference does not have: u1 = u0 - dt/V R0, u2 = 3/4 u0 + 1/4 (u1 - dt/V R1),
+ * u3 = 1/3 u0 + 2/3 (u2 - dt/V R2), with cvOld = u0 kept from stage 1. */
+void orc_rk_update(const orc_region* R, int stage, double dt, double* cv, const double* cvOld,
+                   const double* rhs, double* rhsSum) {
+  static const double ark4[4] = {0.5, 0.5, 1.0, 1.0 / 6.0};
+  static const double grk4[4] = {1.0, 2.0, 2.0, 1.0};
+  static const double ark3[3] = {8.0 / 15.0, 5.0 / 12.0, 0.75};
+  static const double grk3[3] = {0.0, 17.0 / 25.0, 5.0 / 9.0};
+  size_t N = (size_t)R->n_cells;
+  int s = stage - 1;
+  if (R->rk == 1) {
+    double fac = ark4[s] * dt;
+    for (size_t ic = 0; ic < N; ++ic) {
+      double adtv = fac / R->vol[ic];
+      for (int iv = 0; iv < 5; ++iv) {
+        size_t k = 5 * ic + iv;
+        if (stage == 1) {
+          cv[k] = cvOld[k] - adtv * rhs[k];
+          rhsSum[k] = rhs[k];
+        } else if (stage == 4) {
+          cv[k] = cvOld[k] - adtv * (rhs[k] + rhsSum[k]);
+        } else {
+          cv[k] = cvOld[k] - adtv * rhs[k];
+          rhsSum[k] = rhsSum[k] + grk4[s] * rhs[k];
+        }
+      }
+    }
+  } else if (R->rk == 2) {
+    double fac = ark3[s] * dt;
+    for (size_t ic = 0; ic < N; ++ic) {
+      double adtv = fac / R->vol[ic];
+      for (int iv = 0; iv < 5; ++iv) {
+        size_t k = 5 * ic + iv;
+        if (stage == 1) {
+          cv[k] = cvOld[k] - adtv * rhs[k];
+          rhsSum[k] = rhs[k];
+        } else if (stage == 2) {
+          cv[k] = cvOld[k] - adtv * (rhs[k] - grk3[s] * rhsSum[k]);
+          rhsSum[k] = rhs[k];
+        } else {
+          cv[k] = cvOld[k] - adtv * (rhs[k] - grk3[s] * rhsSum[k]);
+        }
+      }
+    }
+  } else if (R->rk == 3) {
+    double fac = 1.0 * dt;
+    for (size_t ic = 0; ic < N; ++ic) {
+      double adtv = fac / R->vol[ic];
+      for (int iv = 0; iv < 5; ++iv) {
+        size_t k = 5 * ic + iv;
+        cv[k] = cvOld[k] - adtv * rhs[k];
+      }
+    }
+  } else {
+    for (size_t ic = 0; ic < N; ++ic) {
+      double adtv = dt / R->vol[ic];
+      for (int iv = 0; iv < 5; ++iv) {
+        size_t k = 5 * ic + iv;
+        /* rhsSum carries the stage's starting state u_{s-1} (set by the driver). */
+        double us = rhsSum[k] - adtv * rhs[k];
+        if (stage == 1) cv[k] = us;
+        else if (stage == 2) cv[k] = 0.75 * cvOld[k] + 0.25 * us;
+        else cv[k] = (1.0 / 3.0) * cvOld[k] + (2.0 / 3.0) * us;
+      }
+    }
+  }
+}
+
+/* AfterUpdateMP, libfloflu/AfterUpdateMP.F90:134-136: CheckValidity
+ * (libflu/RFLU_CheckValidity.F90:141-222), EnforceBounds (libflu/RFLU_EnforceBounds.F90:
+ * 169-231) and CheckPositivity (libflu/RFLU_CheckPositivity.F90:147-253). */
+int orc_after_update(const orc_region* R, double* cv, double* dv, orc_stats* st) {
+  size_t N = (size_t)R->n_cells;
+  double mm, rgas, gamma;
+  gas_prime(R, &mm, &rgas, &gamma);
+  for (size_t i = 0; i < N; ++i) {
+    double rho = cv[5 * i], rrho = 1.0 / rho;
+    double u = rrho * cv[5 * i + 1], v = rrho * cv[5 * i + 2], w = rrho * cv[5 * i + 3];
+    double p = dv[3 * i], t = dv[3 * i + 1];
+    if (isnan(rho) || isnan(u) || isnan(v) || isnan(w) || isnan(p) || isnan(t))
+      return ORC_ERR_INVALID_VALUE;
+  }
+  const double pTol = 1.0E-12;
+  for (size_t i = 0; i < N; ++i) {
+    double r = cv[5 * i], ir = 1.0 / r;
+    double u = ir * cv[5 * i + 1], v = ir * cv[5 * i + 2], w = ir * cv[5 * i + 3];
+    double Eo = ir * cv[5 * i + 4];
+    double Vm2 = u * u + v * v + w * w;
+    double p = (gamma - 1.0) * r * (Eo - 0.5 * Vm2);
+    if (p <= 0.0 && fabs(p) <= pTol) {
+      if (st) st->n_bounds_fix++;
+      p = pTol;
+      cv[5 * i + 4] = r * (p / (r * (gamma - 1.0)) + 0.5 * (u * u + v * v + w * w));
+    }
+  }
+  for (size_t i = 0; i < N; ++i) {
+    double rho = cv[5 * i], rrho = 1.0 / rho;
+    double u = rrho * cv[5 * i + 1], v = rrho * cv[5 * i + 2], w = rrho * cv[5 * i + 3];
+    double Vm2 = u * u + v * v + w * w;
+    double p = dv[3 * i];
+    if (p <= 0.0) {
+      if (st) st->n_pos_fix++;
+      double Eo = 1.0e4 + 0.5 * Vm2;
+      cv[5 * i + 4] = rho * Eo;
+      orc_set_dependent_vars(R, cv, dv, (int)i, (int)i + 1);
+      p = dv[3 * i];
+    }
+    if (rho <= 0.0 || p <= 0.0) return ORC_ERR_NEGATIVE_POSDEF;
+  }
+  return ORC_OK;
+}
+
+/* RFLU_TimeStepInviscid, rocflu/RFLU_TimeStepInviscid.F90:153-334: spectral-radius
+ * sums over interior faces and boundary faces, dt = vol/sum, minimum over cells. */
+double orc_local_dt(const orc_region* R, const double* cv, const double* dv) {
+  size_t N = (size_t)R->n_cells;
+  double* dt = (double*)malloc(N * sizeof(double));
+  for (size_t i = 0; i < N; ++i) dt[i] = 0.0;
+  const double fsu = 0.0;
+  for (int f = 0; f < R->n_faces; ++f) {
+    int c1 = R->f2c[2 * (size_t)f], c2 = R->f2c[2 * (size_t)f + 1];
+    const double* n = R->fn + 4 * (size_t)f;
+    double ir1 = 1.0 / cv[5 * (size_t)c1], ir2 = 1.0 / cv[5 * (size_t)c2];
+    double u1 = cv[5 * (size_t)c1 + 1] * ir1, v1 = cv[5 * (size_t)c1 + 2] * ir1,
+           w1 = cv[5 * (size_t)c1 + 3] * ir1, a1 = dv[3 * (size_t)c1 + 2];
+    double u2 = cv[5 * (size_t)c2 + 1] * ir2, v2 = cv[5 * (size_t)c2 + 2] * ir2,
+           w2 = cv[5 * (size_t)c2 + 3] * ir2, a2 = dv[3 * (size_t)c2 + 2];
+    double vc = 0.5 * ((u1 + u2) * n[0] + (v1 + v2) * n[1] + (w1 + w2) * n[2]);
+    double as = 0.5 * (a1 + a2);
+    double srad = (fabs(vc - fsu) + as) * n[3];
+    dt[c1] = dt[c1] + srad;
+    dt[c2] = dt[c2] + srad;
+  }
+  for (int p = 0; p < R->n_patches; ++p)
+    for (int b = R->bpatch_ptr[p]; b < R->bpatch_ptr[p + 1]; ++b) {
+      int c1 = R->bf2c[b];
+      const double* n = R->bfn + 4 * (size_t)b;
+      double ir1 = 1.0 / cv[5 * (size_t)c1];
+      double u1 = cv[5 * (size_t)c1 + 1] * ir1, v1 = cv[5 * (size_t)c1 + 2] * ir1,
+             w1 = cv[5 * (size_t)c1 + 3] * ir1, a1 = dv[3 * (size_t)c1 + 2];
+      double vc = u1 * n[0] + v1 * n[1] + w1 * n[2];
+      dt[c1] = dt[c1] + (fabs(vc - fsu) + a1) * n[3];
+    }
+  double mn = HUGE_VAL;
+  for (size_t i = 0; i < N; ++i) {
+    dt[i] = R->vol[i] / dt[i];
+    if (dt[i] < mn) mn = dt[i];
+  }
+  free(dt);
+  return mn;
+}
+
+/* RFLU_MinimumTimeStep, rocflu/RFLU_MinimumTimeStep.F90:126-245: CFL scaling, the
+ * imposed step while currentTime <= 1e-8 (formed as 1/(1/dtImposed + 0/dt)), the
+ * non-positive guard and the DTMINLIMIT floor. */
+double orc_min_timestep(const orc_region* R, double dt_local_min, double current_time) {
+  double wI = 0.0, wC = 0.0;
+  double d = R->cfl * dt_local_min;
+  if (current_time <= 1.0E-08) wI = 1.0;
+  else wC = 1.0;
+  d = wI / R->dt_imposed + wC / d;
+  d = 1.0 / d;
+  double dtMin = HUGE_VAL;
+  if (d < dtMin) dtMin = d;
+  if (dtMin <= 0.0) dtMin = 6.0E-12;
+  dtMin = dmax(dtMin, R->dt_min_limit);
+  return dtMin;
+}
+
+/* One RK stage of RungeKuttaMP (libfloflu/RungeKuttaMP.F90:148-508) for one region. */
+static int orc_stage(const orc_region* R, int stage, double dt, double* cv, double* dv,
+                     double* cvOld, double* rhs, double* rhsSum, double* grad, orc_stats* st) {
+  size_t N = (size_t)R->n_cells;
+  /* RKInitMP -> RkInitGeneric.F90:128-160 */
+  if (R->rk == 1) {
+    if (stage == 1) memcpy(cvOld, cv, 5 * N * sizeof(double));
+  } else if (R->rk == 4) {
+    if (stage == 1) memcpy(cvOld, cv, 5 * N * sizeof(double));
+    memcpy(rhsSum, cv, 5 * N * sizeof(double)); /* stage start state */
+  } else {
+    memcpy(cvOld, cv, 5 * N * sizeof(double));
+  }
+  if (R->order == 2) orc_cell_gradients(R, cv, dv, grad);
+  orc_fluxes(R, cv, dv, grad, rhs, st, stage);
+  orc_rk_update(R, stage, dt, cv, cvOld, rhs, rhsSum);
+  orc_set_dependent_vars(R, cv, dv, 0, R->n_cells);
+  return orc_after_update(R, cv, dv, st);
+}
+
+int orc_run(const orc_region* R, double* cv, double* dv, int64_t nsteps, orc_stats* st) {
+  size_t N = (size_t)R->n_cells;
+  int ns = orc_nstages(R->rk);
+  if (ns == 0 || (R->order != 1 && R->order != 2)) return ORC_ERR_ARG;
+  double* cvOld = (double*)malloc(5 * N * sizeof(double));
+  double* rhs = (double*)malloc(5 * N * sizeof(double));
+  double* rhsSum = (double*)calloc(5 * N, sizeof(double));
+  double* grad = (double*)calloc(15 * N, sizeof(double));
+  int rc = ORC_OK;
+  for (int64_t step = 0; step < nsteps && rc == ORC_OK; ++step) {
+    double dtl = orc_local_dt(R, cv, dv);
+    double dt = orc_min_timestep(R, dtl, st->time);
+    for (int s = 1; s <= ns && rc == ORC_OK; ++s)
+      rc = orc_stage(R, s, dt, cv, dv, cvOld, rhs, rhsSum, grad, st);
+    st->time = st->time + dt;
+    st->dt_last = dt;
+    st->steps++;
+  }
+  free(cvOld); free(rhs); free(rhsSum); free(grad);
+  return rc;
+}
+
+void orc_residual(const orc_region* R, const double* cv_in, const double* dv, double* rhs) {
+  size_t N = (size_t)R->n_cells;
+  double* cv = (double*)malloc(5 * N * sizeof(double));
+  double* grad = (double*)calloc(15 * N, sizeof(double));
+  memcpy(cv, cv_in, 5 * N * sizeof(double));
+  if (R->order == 2) orc_cell_gradients(R, cv, dv, grad);
+  orc_fluxes(R, cv, dv, grad, rhs, NULL, 0);
+  free(cv); free(grad);
+}

@@ -1,0 +1,86 @@
+/*
+ * rflu_oracle.h -- CPU restatement of RocfluMP's explicit residual/update path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker for the GPU path: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and never as
+ * the thing measured or shipped.  The product (rocflu_amd) never calls it.
+ *
+ * Plain C, FP64, built with -ffp-contract=off.  Loops run in the reference's order
+ * (face scatter, patch by patch), each function citing the reference lines it follows.
+ * Arrays use the reference's component-fastest layout with 0-based indices.
+ */
+#ifndef RFLU_ORACLE_H
+#define RFLU_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int32_t n_cells, n_faces, n_patches;
+  const int32_t* f2c;        /* (2,F) */
+  const double* fn;          /* (4,F) unit normal + |A| */
+  const double* fc;          /* (3,F) */
+  const double* cofg;        /* (3,N) */
+  const double* vol;         /* (N) */
+  const int32_t* bpatch_ptr; /* (P+1) boundary faces of patch p: [ptr[p], ptr[p+1]) */
+  const int32_t* bf2c;       /* (B) */
+  const double* bfn;         /* (4,B) */
+  const double* bfc;         /* (3,B) */
+  const int32_t* patch_bc;   /* (P) 20 outflow, 60 slip wall */
+  const int32_t* patch_opt;  /* (P) outflow option 0/1/2 */
+  const double* patch_pout;  /* (P) outflow static pressure */
+  const int32_t* patch_order;/* (P) 1 or 2 (boundary-face order); NULL = all first order */
+  const int32_t* c2cs_ptr;   /* (N+1) stencil CSR, order 2 */
+  const int32_t* c2cs_idx;
+  const double* xyz_moms;    /* (10,N) R-factors */
+  const int32_t* c2f_ptr;    /* (N+1) faces of each cell (signed, see rflu_mesh.h) */
+  const int32_t* c2f_idx;
+  /* gas model and numerics */
+  double gamma, cp;          /* global%refGamma, global%refCp */
+  int32_t discr;             /* 1 Roe, 3 HLLC, 4 AUSM+ */
+  int32_t order;             /* 1 or 2 */
+  int32_t reconst;           /* 0 none, 2 WENO-XYZ + clip, 11 Venkatakrishnan */
+  int32_t rk;                /* 1 RK4, 2 Wray RK3, 3 Euler, 4 SSP-RK3 */
+  double cfl, dt_imposed, dt_min_limit, eps_entr, diss_fact;
+} orc_region;
+
+typedef struct {
+  double time;          /* global%currentTime */
+  double dt_last;       /* last dtMin */
+  int64_t steps;
+  int64_t n_bounds_fix; /* EnforceBounds clamps */
+  int64_t n_pos_fix;    /* CheckPositivity energy floors */
+  double mass_in, mass_out;
+} orc_stats;
+
+/* Error codes (mirror of ModError's ERR_INVALID_VALUE / ERR_NEGATIVE_POSDEF). */
+#define ORC_OK 0
+#define ORC_ERR_INVALID_VALUE 1
+#define ORC_ERR_NEGATIVE_POSDEF 2
+#define ORC_ERR_ARG 3
+
+void orc_set_dependent_vars(const orc_region* r, const double* cv, double* dv, int beg, int end);
+void orc_fluxes(const orc_region* r, const double* cv, const double* dv, const double* grad,
+                double* rhs, orc_stats* st, int stage);
+void orc_gradients(const orc_region* r, const double* var, double* grad);
+void orc_weno_xyz(const orc_region* r, double* grad);
+void orc_limit_simple(const orc_region* r, const double* var, double* grad);
+void orc_venkat(const orc_region* r, const double* var, double* grad);
+void orc_cell_gradients(const orc_region* r, double* cv, const double* dv, double* grad);
+void orc_rk_update(const orc_region* r, int stage, double dt, double* cv, const double* cvOld,
+                   const double* rhs, double* rhsSum);
+int orc_after_update(const orc_region* r, double* cv, double* dv, orc_stats* st);
+double orc_local_dt(const orc_region* r, const double* cv, const double* dv);
+double orc_min_timestep(const orc_region* r, double dt_local_min, double current_time);
+int orc_nstages(int rk);
+/* Full time loop: nsteps of (dt, stages).  cv (5,N) and dv (3,N) are updated in place. */
+int orc_run(const orc_region* r, double* cv, double* dv, int64_t nsteps, orc_stats* st);
+/* One residual evaluation R(cv) for the current state (gradients included for order 2). */
+void orc_residual(const orc_region* r, const double* cv, const double* dv, double* rhs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

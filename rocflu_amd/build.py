@@ -1,0 +1,84 @@
+"""Build recipes of the native libraries (in-tree, for gfx950).
+
+- ``librocflu_amd.so``: HIP kernels + C-ABI (hipcc, gfx950, FMA contraction off so
+  FP64 results round like the reference's gfortran build).
+- ``librflu_mesh.so``: host region builder (g++).
+- ``fortran/librflu_gpu_f.so`` + ``fortran/rflu_driver``: the ISO_C_BINDING module and
+  a Fortran driver that calls the C-ABI in RungeKuttaMP's order (amdflang, when present).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+INC = os.path.join(ROOT, "include")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off",
+    # packed FP32 instructions are kept out of generated code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    p = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in p.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if out.strip():
+        print(out)
+    if p.returncode != 0:
+        raise RuntimeError(f"build step failed ({p.returncode}): {' '.join(cmd)}")
+
+
+def build_hip():
+    hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
+    src = os.path.join(HERE, "csrc", "capi.hip")
+    out = os.path.join(HERE, "librocflu_amd.so")
+    tmp = out + ".tmp"
+    _run([hipcc] + HIPCC_FLAGS + ["-I" + INC, "-I" + os.path.join(HERE, "csrc"), src, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_mesh():
+    src = os.path.join(HERE, "csrc", "mesh.cpp")
+    out = os.path.join(HERE, "librflu_mesh.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall", "-Wextra",
+          "-I" + INC, src, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_fortran():
+    fc = shutil.which("amdflang") or os.path.join(ROCM, "bin", "amdflang")
+    if not os.path.exists(fc):
+        print("amdflang not found; skipping the Fortran binding", flush=True)
+        return None
+    fdir = os.path.join(HERE, "fortran")
+    mod = os.path.join(fdir, "rflu_gpu.f90")
+    drv = os.path.join(fdir, "rflu_driver.f90")
+    if not (os.path.exists(mod) and os.path.exists(drv)):
+        return None
+    out = os.path.join(fdir, "rflu_driver")
+    _run([fc, "-O2", "-J", fdir, mod, drv, "-L" + HERE, "-lrocflu_amd", "-lrflu_mesh",
+          "-Wl,-rpath," + HERE, "-o", out], cwd=fdir)
+    return out
+
+
+def build_all(fortran: bool = True):
+    build_mesh()
+    build_hip()
+    if fortran:
+        build_fortran()
+
+
+if __name__ == "__main__":
+    build_all(fortran="--no-fortran" not in sys.argv)

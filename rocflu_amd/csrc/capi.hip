@@ -1,0 +1,832 @@
+// C-ABI of the explicit residual/update path (include/rocflu_amd.h): region
+// registry, upload of the t_region arrays into the device layout, and the launch
+// sequences of RungeKuttaMP's callees.
+#include "rocflu_amd.h"
+#include "kernels.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace rflu;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(RFLU_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+enum KernelId { KID_FACE = 0, KID_UPDATE = 1, KID_GRAD = 2, KID_BLEND = 3, KID_DT = 4, KID_BFACE = 5, KID_N = 6 };
+
+struct Region {
+  int N = 0, F = 0, B = 0, P = 0;
+  int order = 1, discr = 3, reconst = 0, rk = 2, prec = 0;
+  double cfl = 0.5, dt_imposed = 0, dt_min_limit = 0;
+  Gas gas{};
+  PatchTable patches{};
+  bool has_outflow = false;
+  hipStream_t stream = nullptr;
+  // device buffers
+  int2* f2c = nullptr;
+  double4 *fn = nullptr, *fc = nullptr, *cell = nullptr, *bfn = nullptr, *bfc = nullptr;
+  int *bf2c = nullptr, *bpatch = nullptr, *c2f_ptr = nullptr, *c2f_idx = nullptr;
+  int *sten_ptr = nullptr, *sten_idx = nullptr;
+  double *moms = nullptr, *epsq = nullptr;
+  double *cv = nullptr, *cv_old = nullptr, *rhs_sum = nullptr, *dv = nullptr, *prim = nullptr;
+  void *grad = nullptr, *grad2 = nullptr, *flux = nullptr;
+  double* rhs = nullptr;
+  double* dt_part = nullptr;
+  Scalars* sc = nullptr;
+  int n_dt_parts = 0;
+  int face_block = 256, cell_block = 256, grad_block = 256;
+  hipGraphExec_t graph = nullptr;
+  bool graph_dirty = true;
+  // kernel timing
+  bool profiling = false;
+  struct Ev { int kid; hipEvent_t a, b; };
+  std::vector<Ev> events;
+  double kern_ms[KID_N] = {};
+  long long kern_n[KID_N] = {};
+
+  Mesh mesh() const {
+    Mesh m;
+    m.N = N; m.F = F; m.B = B;
+    m.f2c = f2c; m.fn = fn; m.fc = fc; m.cell = cell;
+    m.bf2c = bf2c; m.bfn = bfn; m.bfc = bfc; m.bpatch = bpatch;
+    m.c2f_ptr = c2f_ptr; m.c2f_idx = c2f_idx;
+    m.sten_ptr = sten_ptr; m.sten_idx = sten_idx; m.moms = moms; m.epsq = epsq;
+    return m;
+  }
+  State state() const {
+    State s;
+    s.cv = cv; s.cv_old = cv_old; s.rhs_sum = rhs_sum; s.dv = dv; s.prim = prim;
+    return s;
+  }
+  const void* grad_final() const { return reconst == RFLU_RECONST_WENO_XYZ ? grad2 : grad; }
+};
+
+std::mutex g_mu;
+std::vector<Region*> g_regions;
+
+Region* get(int32_t h) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (h < 1 || h > (int32_t)g_regions.size()) return nullptr;
+  return g_regions[h - 1];
+}
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(RFLU_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return 0;
+}
+template <typename T>
+int dput(T* d, const std::vector<T>& h, hipStream_t s) {
+  if (h.empty()) return 0;
+  HIP_TRY(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+void free_region(Region* R) {
+  if (!R) return;
+  if (R->stream) (void)hipStreamSynchronize(R->stream);
+  if (R->graph) (void)hipGraphExecDestroy(R->graph);
+  for (auto& e : R->events) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+  void* bufs[] = {R->f2c, R->fn, R->fc, R->cell, R->bfn, R->bfc, R->bf2c, R->bpatch, R->c2f_ptr,
+                  R->c2f_idx, R->sten_ptr, R->sten_idx, R->moms, R->epsq, R->cv, R->cv_old,
+                  R->rhs_sum, R->dv, R->prim, R->grad, R->grad2, R->flux, R->rhs, R->dt_part,
+                  R->sc};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (R->stream) (void)hipStreamDestroy(R->stream);
+  delete R;
+}
+
+inline unsigned nblk(int n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ---- kernel dispatch ------------------------------------------------------------
+
+template <typename T, typename GT, typename FT>
+int launch_fluxes_t(Region* R) {
+  Mesh M = R->mesh();
+  State S = R->state();
+  const GT* g = (const GT*)R->grad_final();
+  FT* fl = (FT*)R->flux;
+  dim3 fb(R->face_block);
+  if (R->F > 0) {
+#define RFLU_FACE(D, O)                                                                          \
+  hipLaunchKernelGGL((k_face_flux<D, O, T, GT, FT>), dim3(nblk(R->F, R->face_block)), fb, 0,   \
+                     R->stream, M, S, R->gas, g, fl, R->sc)
+    if (R->order == 1) {
+      if (R->discr == 1) RFLU_FACE(1, 1);
+      else if (R->discr == 3) RFLU_FACE(3, 1);
+      else RFLU_FACE(4, 1);
+    } else {
+      if (R->discr == 1) RFLU_FACE(1, 2);
+      else if (R->discr == 3) RFLU_FACE(3, 2);
+      else RFLU_FACE(4, 2);
+    }
+#undef RFLU_FACE
+  }
+  return 0;
+}
+
+template <typename GT, typename FT>
+int launch_bfaces_t(Region* R) {
+  if (R->B == 0) return 0;
+  Mesh M = R->mesh();
+  State S = R->state();
+  const GT* g = (const GT*)R->grad_final();
+  if (R->order == 1)
+    hipLaunchKernelGGL((k_bface_flux<1, GT, FT>), dim3(nblk(R->B, 256)), dim3(256), 0, R->stream,
+                       M, S, R->gas, R->patches, g, (FT*)R->flux, R->sc);
+  else
+    hipLaunchKernelGGL((k_bface_flux<2, GT, FT>), dim3(nblk(R->B, 256)), dim3(256), 0, R->stream,
+                       M, S, R->gas, R->patches, g, (FT*)R->flux, R->sc);
+  return 0;
+}
+
+template <typename GT>
+int launch_gradients_t(Region* R) {
+  Mesh M = R->mesh();
+  State S = R->state();
+  dim3 gb(R->grad_block);
+  dim3 gg(nblk(R->N, R->grad_block));
+  if (R->reconst == RFLU_RECONST_LIM_VENKAT)
+    hipLaunchKernelGGL((k_gradient<GT, true>), gg, gb, 0, R->stream, M, S, (GT*)R->grad, R->sc);
+  else
+    hipLaunchKernelGGL((k_gradient<GT, false>), gg, gb, 0, R->stream, M, S, (GT*)R->grad, R->sc);
+  return 0;
+}
+
+template <typename GT>
+int launch_blend_t(Region* R) {
+  if (R->reconst != RFLU_RECONST_WENO_XYZ) return 0;
+  Mesh M = R->mesh();
+  State S = R->state();
+  hipLaunchKernelGGL((k_weno_clip<GT>), dim3(nblk(R->N, R->grad_block)), dim3(R->grad_block), 0,
+                     R->stream, M, S, (const GT*)R->grad, (GT*)R->grad2, R->sc);
+  return 0;
+}
+
+template <typename FT, bool FUSED, bool PRIM>
+int launch_update_rk(Region* R, int stage, int last) {
+  Mesh M = R->mesh();
+  State S = R->state();
+  dim3 cb(R->cell_block), cg(nblk(R->N, R->cell_block));
+  const FT* fl = (const FT*)R->flux;
+  switch (R->rk) {
+    case 1: hipLaunchKernelGGL((k_update<1, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last); break;
+    case 2: hipLaunchKernelGGL((k_update<2, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last); break;
+    case 3: hipLaunchKernelGGL((k_update<3, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last); break;
+    default: hipLaunchKernelGGL((k_update<4, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last); break;
+  }
+  return 0;
+}
+
+// Event bracketing for per-kernel timing.
+struct Timed {
+  Region* R;
+  int kid;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timed(Region* r, int k) : R(r), kid(k) {
+    if (R->profiling) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, R->stream);
+    }
+  }
+  ~Timed() {
+    if (R->profiling) {
+      (void)hipEventRecord(b, R->stream);
+      R->events.push_back({kid, a, b});
+    }
+  }
+};
+
+void collect_events(Region* R) {
+  for (auto& e : R->events) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+      R->kern_ms[e.kid] += ms;
+      R->kern_n[e.kid] += 1;
+    }
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  R->events.clear();
+}
+
+// FP32 gradient storage/arithmetic for the mixed and FP32-gradient modes.
+inline bool grad32(const Region* R) { return R->prec == RFLU_PREC_MIXED || R->prec == RFLU_PREC_FP32_GRAD; }
+// FP32 interior-flux arithmetic and flux buffer for the mixed and FP32-flux modes.
+inline bool flux32(const Region* R) { return R->prec == RFLU_PREC_MIXED || R->prec == RFLU_PREC_FP32_FLUX; }
+
+int launch_gradients(Region* R) {
+  if (R->order != 2) return 0;
+  int rc;
+  {
+    Timed t(R, KID_GRAD);
+    rc = grad32(R) ? launch_gradients_t<float>(R) : launch_gradients_t<double>(R);
+  }
+  if (rc) return rc;
+  if (R->reconst == RFLU_RECONST_WENO_XYZ) {
+    Timed t(R, KID_BLEND);
+    rc = grad32(R) ? launch_blend_t<float>(R) : launch_blend_t<double>(R);
+  }
+  return rc;
+}
+
+int launch_fluxes(Region* R, int stage) {
+  int rc;
+  {
+    Timed t(R, KID_FACE);
+    switch (R->prec) {
+      case RFLU_PREC_FP64: rc = launch_fluxes_t<double, double, double>(R); break;
+      case RFLU_PREC_MIXED: rc = launch_fluxes_t<float, float, float>(R); break;
+      case RFLU_PREC_FP32_GRAD: rc = launch_fluxes_t<double, float, double>(R); break;
+      default: rc = launch_fluxes_t<float, double, float>(R); break;
+    }
+  }
+  if (rc) return rc;
+  {
+    Timed t(R, KID_BFACE);
+    switch (R->prec) {
+      case RFLU_PREC_FP64: rc = launch_bfaces_t<double, double>(R); break;
+      case RFLU_PREC_MIXED: rc = launch_bfaces_t<float, float>(R); break;
+      case RFLU_PREC_FP32_GRAD: rc = launch_bfaces_t<float, double>(R); break;
+      default: rc = launch_bfaces_t<double, float>(R); break;
+    }
+  }
+  if (rc) return rc;
+  if (stage == 1 && R->has_outflow) {
+    Mesh M = R->mesh();
+    if (!flux32(R))
+      hipLaunchKernelGGL((k_mass_inout<double>), dim3(1), dim3(256), 0, R->stream, M, R->patches,
+                         (const double*)R->flux, R->sc);
+    else
+      hipLaunchKernelGGL((k_mass_inout<float>), dim3(1), dim3(256), 0, R->stream, M, R->patches,
+                         (const float*)R->flux, R->sc);
+  }
+  return 0;
+}
+
+int launch_update(Region* R, int stage, bool fused) {
+  Timed t(R, KID_UPDATE);
+  int last = stage == (R->rk == 1 ? 4 : R->rk == 3 ? 1 : 3);
+  bool prim = R->order == 2;
+  if (!flux32(R)) {
+    if (fused) return prim ? launch_update_rk<double, true, true>(R, stage, last)
+                           : launch_update_rk<double, true, false>(R, stage, last);
+    return launch_update_rk<double, false, false>(R, stage, last);
+  }
+  if (fused) return prim ? launch_update_rk<float, true, true>(R, stage, last)
+                         : launch_update_rk<float, true, false>(R, stage, last);
+  return launch_update_rk<float, false, false>(R, stage, last);
+}
+
+int launch_time_step(Region* R, double time_in = -1.0) {
+  Timed t(R, KID_DT);
+  Mesh M = R->mesh();
+  State S = R->state();
+  hipLaunchKernelGGL(k_local_dt, dim3(R->n_dt_parts), dim3(256), 0, R->stream, M, S, R->dt_part, R->sc);
+  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, R->stream, (const double*)R->dt_part,
+                     R->n_dt_parts, R->sc, R->cfl, R->dt_imposed, R->dt_min_limit, time_in);
+  return 0;
+}
+
+int nstages(int rk) { return rk == 1 ? 4 : rk == 2 ? 3 : rk == 3 ? 1 : 3; }
+
+int launch_step(Region* R) {
+  int rc = launch_time_step(R);
+  if (rc) return rc;
+  for (int s = 1; s <= nstages(R->rk); ++s) {
+    if ((rc = launch_gradients(R))) return rc;
+    if ((rc = launch_fluxes(R, s))) return rc;
+    if ((rc = launch_update(R, s, true))) return rc;
+  }
+  return 0;
+}
+
+int read_scalars(Region* R, Scalars* h) {
+  HIP_TRY(hipMemcpyAsync(h, R->sc, sizeof(Scalars), hipMemcpyDeviceToHost, R->stream));
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  return 0;
+}
+
+int status_from(const Scalars& s) {
+  if (s.n_invalid) return fail(RFLU_ERR_INVALID_VALUE, "invalid (NaN) values detected");
+  if (s.n_negative) return fail(RFLU_ERR_NEGATIVE_POSDEF, "negative positive-definite variables detected");
+  return RFLU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rflu_gpu_last_error(void) { return g_err.c_str(); }
+
+int rflu_gpu_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int rflu_gpu_create_region(const rflu_region_desc* d, int32_t* handle) {
+  if (!d || !handle) return fail(RFLU_ERR_ARG, "null argument");
+  *handle = 0;
+  const int N = d->n_cells, F = d->n_faces, P = d->n_patches;
+  const int base = d->index_base;
+  if (N <= 0 || F < 0 || P < 0 || P > MAX_PATCHES) return fail(RFLU_ERR_ARG, "bad region sizes");
+  if (base != 0 && base != 1) return fail(RFLU_ERR_ARG, "index_base must be 0 or 1");
+  if (!d->f2c || !d->fn || !d->cofg || !d->vol || (P > 0 && (!d->bpatch_ptr || !d->patch_bc)))
+    return fail(RFLU_ERR_ARG, "missing grid arrays");
+  if (d->order != 1 && d->order != 2) return fail(RFLU_ERR_ARG, "order must be 1 or 2");
+  if (d->discr != 1 && d->discr != 3 && d->discr != 4) return fail(RFLU_ERR_ARG, "discr must be 1, 3 or 4");
+  if (d->rk < 1 || d->rk > 4) return fail(RFLU_ERR_ARG, "rk must be 1..4");
+  if (d->precision < 0 || d->precision > 3) return fail(RFLU_ERR_ARG, "precision must be 0..3");
+  if (d->order == 2) {
+    if (!d->fc || !d->c2cs_ptr || !d->c2cs_idx || !d->xyz_moms)
+      return fail(RFLU_ERR_ARG, "order 2 needs fc, stencils and xyz_moms");
+    if (d->reconst != 0 && d->reconst != 2 && d->reconst != 11)
+      return fail(RFLU_ERR_ARG, "reconst must be 0, 2 or 11");
+  }
+  if (!(d->gamma > 1.0) || !(d->cp > 0.0)) return fail(RFLU_ERR_ARG, "bad gas constants");
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+    return fail(RFLU_ERR_HIP, "no GPU visible");
+  const int B = P > 0 ? d->bpatch_ptr[P] : 0;
+  if (B > 0 && (!d->bf2c || !d->bfn || !d->bfc)) return fail(RFLU_ERR_ARG, "missing boundary arrays");
+  if (d->bpatch_ptr && d->bpatch_ptr[0] != 0) return fail(RFLU_ERR_ARG, "bpatch_ptr[0] must be 0");
+
+  Region* R = new Region();
+  R->N = N; R->F = F; R->B = B; R->P = P;
+  R->order = d->order; R->discr = d->discr; R->reconst = d->order == 2 ? d->reconst : 0;
+  R->rk = d->rk; R->prec = d->precision;
+  R->cfl = d->cfl; R->dt_imposed = d->dt_imposed; R->dt_min_limit = d->dt_min_limit;
+  {
+    Gas& G = R->gas;
+    G.g = d->gamma; G.cp = d->cp; G.gc = d->cp - d->cp / d->gamma;
+    double m = 8314.3 / G.gc;          // MixtPerf_M_R (libflu/RFLU_SetGasVars.F90:172-178)
+    G.mm = m;
+    G.rgasp = 8314.3 / m;              // MixtPerf_R_M
+    G.gp = d->cp / (d->cp - G.rgasp);  // MixtPerf_G_CpR
+    G.eps_entr = d->eps_entr; G.diss_fact = d->diss_fact;
+  }
+  R->patches.n = P;
+  for (int p = 0; p < P; ++p) {
+    R->patches.bc[p] = d->patch_bc[p];
+    R->patches.opt[p] = d->patch_opt ? d->patch_opt[p] : 0;
+    R->patches.order2[p] = d->patch_order ? (d->patch_order[p] == 2) : 0;
+    R->patches.pout[p] = d->patch_pout ? d->patch_pout[p] : 0.0;
+    if (d->patch_bc[p] != BC_OUTFLOW && d->patch_bc[p] != BC_SLIPWALL) {
+      free_region(R);
+      return fail(RFLU_ERR_ARG, "only slip-wall (60) and outflow (20) patches are supported");
+    }
+    if (d->patch_bc[p] == BC_OUTFLOW) R->has_outflow = true;
+  }
+
+  // ---- host-side packing ----
+  std::vector<int2> hf2c(F);
+  std::vector<double4> hfn(F), hfc(F), hcell(N), hbfn(B), hbfc(B);
+  std::vector<int> hbf2c(B), hbpatch(B);
+  for (int f = 0; f < F; ++f) {
+    int c1 = d->f2c[2 * (size_t)f] - base, c2 = d->f2c[2 * (size_t)f + 1] - base;
+    if (c1 < 0 || c1 >= N || c2 < 0 || c2 >= N || c1 == c2) {
+      free_region(R);
+      return fail(RFLU_ERR_ARG, "f2c out of range at face " + std::to_string(f));
+    }
+    hf2c[f] = make_int2(c1, c2);
+    hfn[f] = make_double4(d->fn[4 * (size_t)f], d->fn[4 * (size_t)f + 1], d->fn[4 * (size_t)f + 2], d->fn[4 * (size_t)f + 3]);
+    if (d->fc) hfc[f] = make_double4(d->fc[3 * (size_t)f], d->fc[3 * (size_t)f + 1], d->fc[3 * (size_t)f + 2], 0.0);
+    else hfc[f] = make_double4(0, 0, 0, 0);
+  }
+  for (int c = 0; c < N; ++c) {
+    hcell[c] = make_double4(d->cofg[3 * (size_t)c], d->cofg[3 * (size_t)c + 1], d->cofg[3 * (size_t)c + 2], d->vol[c]);
+    if (!(d->vol[c] > 0.0)) { free_region(R); return fail(RFLU_ERR_ARG, "non-positive volume"); }
+  }
+  for (int p = 0; p < P; ++p)
+    for (int b = d->bpatch_ptr[p]; b < d->bpatch_ptr[p + 1]; ++b) {
+      if (b < 0 || b >= B) { free_region(R); return fail(RFLU_ERR_ARG, "bad bpatch_ptr"); }
+      int c1 = d->bf2c[b] - base;
+      if (c1 < 0 || c1 >= N) { free_region(R); return fail(RFLU_ERR_ARG, "bf2c out of range"); }
+      hbf2c[b] = c1;
+      hbpatch[b] = p;
+      hbfn[b] = make_double4(d->bfn[4 * (size_t)b], d->bfn[4 * (size_t)b + 1], d->bfn[4 * (size_t)b + 2], d->bfn[4 * (size_t)b + 3]);
+      hbfc[b] = make_double4(d->bfc[3 * (size_t)b], d->bfc[3 * (size_t)b + 1], d->bfc[3 * (size_t)b + 2], 0.0);
+    }
+  // cell -> face list in the reference's scatter order
+  std::vector<int> hc2f_ptr(N + 1, 0), hc2f_idx;
+  for (int f = 0; f < F; ++f) { hc2f_ptr[hf2c[f].x + 1]++; hc2f_ptr[hf2c[f].y + 1]++; }
+  for (int b = 0; b < B; ++b) hc2f_ptr[hbf2c[b] + 1]++;
+  for (int c = 0; c < N; ++c) hc2f_ptr[c + 1] += hc2f_ptr[c];
+  hc2f_idx.resize(hc2f_ptr[N]);
+  {
+    std::vector<int> fill(hc2f_ptr.begin(), hc2f_ptr.end() - 1);
+    for (int f = 0; f < F; ++f) {
+      hc2f_idx[fill[hf2c[f].x]++] = f + 1;
+      hc2f_idx[fill[hf2c[f].y]++] = -(f + 1);
+    }
+    for (int b = 0; b < B; ++b) hc2f_idx[fill[hbf2c[b]]++] = F + b + 1;
+  }
+  std::vector<int> hsten_ptr, hsten_idx;
+  std::vector<double> hmoms, hepsq;
+  if (R->order == 2) {
+    hsten_ptr.assign(d->c2cs_ptr, d->c2cs_ptr + N + 1);
+    if (hsten_ptr[0] != 0) { free_region(R); return fail(RFLU_ERR_ARG, "c2cs_ptr[0] must be 0"); }
+    for (int c = 0; c < N; ++c)
+      if (hsten_ptr[c + 1] < hsten_ptr[c]) { free_region(R); return fail(RFLU_ERR_ARG, "c2cs_ptr not monotone"); }
+    size_t S = (size_t)hsten_ptr[N];
+    hsten_idx.resize(S);
+    for (size_t s = 0; s < S; ++s) {
+      int j = d->c2cs_idx[s] - base;
+      if (j < 0 || j >= N) { free_region(R); return fail(RFLU_ERR_ARG, "c2cs_idx out of range"); }
+      hsten_idx[s] = j;
+    }
+    hmoms.resize((size_t)6 * N);
+    for (int c = 0; c < N; ++c)
+      for (int k = 0; k < 6; ++k) hmoms[(size_t)k * N + c] = d->xyz_moms[10 * (size_t)c + k];
+    // Venkatakrishnan eps^2 = (K*vol^(1/3))^3, K = 1 (modflu/RFLU_ModLimiters.F90:473-476)
+    hepsq.resize(N);
+    const double THRD = 1.0 / 3.0, K = 1.0;
+    for (int c = 0; c < N; ++c) {
+      double ds = std::pow(d->vol[c], THRD);
+      hepsq[c] = (K * ds) * (K * ds) * (K * ds);
+    }
+  }
+
+  // ---- device allocation ----
+  int rc = 0;
+  if (hipStreamCreate(&R->stream) != hipSuccess) { free_region(R); return fail(RFLU_ERR_HIP, "hipStreamCreate"); }
+  const size_t FB = (size_t)F + B;
+  const size_t fsz = flux32(R) ? sizeof(float) : sizeof(double);
+  const size_t gsz = grad32(R) ? sizeof(float) : sizeof(double);
+  R->n_dt_parts = (int)nblk(N, 256);
+  if ((rc = dalloc(&R->f2c, F)) || (rc = dalloc(&R->fn, F)) || (rc = dalloc(&R->fc, F)) ||
+      (rc = dalloc(&R->cell, N)) || (rc = dalloc(&R->bfn, B)) || (rc = dalloc(&R->bfc, B)) ||
+      (rc = dalloc(&R->bf2c, B)) || (rc = dalloc(&R->bpatch, B)) ||
+      (rc = dalloc(&R->c2f_ptr, N + 1)) || (rc = dalloc(&R->c2f_idx, hc2f_idx.size())) ||
+      (rc = dalloc(&R->cv, 5 * (size_t)N)) || (rc = dalloc(&R->rhs_sum, 5 * (size_t)N)) ||
+      (rc = dalloc(&R->dv, 3 * (size_t)N)) || (rc = dalloc(&R->rhs, 5 * (size_t)N)) ||
+      (rc = dalloc(&R->dt_part, R->n_dt_parts)) || (rc = dalloc(&R->sc, 1))) {
+    free_region(R);
+    return rc;
+  }
+  if (R->rk == 1 || R->rk == 4) {
+    if ((rc = dalloc(&R->cv_old, 5 * (size_t)N))) { free_region(R); return rc; }
+  }
+  {
+    hipError_t e = hipMalloc(&R->flux, 5 * FB * fsz + 16);
+    if (e != hipSuccess) { free_region(R); return fail(RFLU_ERR_NOMEM, "hipMalloc flux"); }
+  }
+  if (R->order == 2) {
+    if ((rc = dalloc(&R->prim, 5 * (size_t)N)) || (rc = dalloc(&R->sten_ptr, N + 1)) ||
+        (rc = dalloc(&R->sten_idx, hsten_idx.size())) || (rc = dalloc(&R->moms, 6 * (size_t)N)) ||
+        (rc = dalloc(&R->epsq, N))) {
+      free_region(R);
+      return rc;
+    }
+    hipError_t e = hipMalloc(&R->grad, 15 * (size_t)N * gsz);
+    if (e == hipSuccess && R->reconst == RFLU_RECONST_WENO_XYZ) e = hipMalloc(&R->grad2, 15 * (size_t)N * gsz);
+    if (e != hipSuccess) { free_region(R); return fail(RFLU_ERR_NOMEM, "hipMalloc grad"); }
+  } else {
+    // never read at order 1, but kernels take a valid pointer
+    if ((rc = dalloc(&R->prim, 5))) { free_region(R); return rc; }
+    hipError_t e = hipMalloc(&R->grad, 64);
+    if (e != hipSuccess) { free_region(R); return fail(RFLU_ERR_NOMEM, "hipMalloc grad"); }
+  }
+  if ((rc = dput(R->f2c, hf2c, R->stream)) || (rc = dput(R->fn, hfn, R->stream)) ||
+      (rc = dput(R->fc, hfc, R->stream)) || (rc = dput(R->cell, hcell, R->stream)) ||
+      (rc = dput(R->bfn, hbfn, R->stream)) || (rc = dput(R->bfc, hbfc, R->stream)) ||
+      (rc = dput(R->bf2c, hbf2c, R->stream)) || (rc = dput(R->bpatch, hbpatch, R->stream)) ||
+      (rc = dput(R->c2f_ptr, hc2f_ptr, R->stream)) || (rc = dput(R->c2f_idx, hc2f_idx, R->stream))) {
+    free_region(R);
+    return rc;
+  }
+  if (R->order == 2) {
+    if ((rc = dput(R->sten_ptr, hsten_ptr, R->stream)) || (rc = dput(R->sten_idx, hsten_idx, R->stream)) ||
+        (rc = dput(R->moms, hmoms, R->stream)) || (rc = dput(R->epsq, hepsq, R->stream))) {
+      free_region(R);
+      return rc;
+    }
+  }
+  hipLaunchKernelGGL(k_fill_scalars, dim3(1), dim3(64), 0, R->stream, R->sc, 0.0);
+  if (hipStreamSynchronize(R->stream) != hipSuccess) {
+    free_region(R);
+    return fail(RFLU_ERR_HIP, "region upload failed");
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_regions.push_back(R);
+  *handle = (int32_t)g_regions.size();
+  return RFLU_OK;
+}
+
+int rflu_gpu_destroy_region(int32_t h) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_regions[h - 1] = nullptr;
+  }
+  free_region(R);
+  return RFLU_OK;
+}
+
+int rflu_gpu_upload_state(int32_t h, const double* cv, const double* dv, double time) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!cv) return fail(RFLU_ERR_ARG, "null cv");
+  const int N = R->N;
+  std::vector<double> hc((size_t)5 * N);
+  for (int c = 0; c < N; ++c)
+    for (int k = 0; k < 5; ++k) hc[(size_t)k * N + c] = cv[5 * (size_t)c + k];
+  HIP_TRY(hipMemcpyAsync(R->cv, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, R->stream));
+  if (dv) {
+    std::vector<double> hd((size_t)3 * N);
+    for (int c = 0; c < N; ++c)
+      for (int k = 0; k < 3; ++k) hd[(size_t)k * N + c] = dv[3 * (size_t)c + k];
+    HIP_TRY(hipMemcpyAsync(R->dv, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice, R->stream));
+    HIP_TRY(hipStreamSynchronize(R->stream));
+  } else {
+    Mesh M = R->mesh();
+    State S = R->state();
+    hipLaunchKernelGGL(k_set_dv, dim3(nblk(N, 256)), dim3(256), 0, R->stream, M, S, R->gas);
+  }
+  if (R->order == 2) {
+    Mesh M = R->mesh();
+    State S = R->state();
+    hipLaunchKernelGGL(k_set_prim, dim3(nblk(N, 256)), dim3(256), 0, R->stream, M, S);
+  }
+  hipLaunchKernelGGL(k_fill_scalars, dim3(1), dim3(64), 0, R->stream, R->sc, time);
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  return RFLU_OK;
+}
+
+int rflu_gpu_download_state(int32_t h, double* cv, double* dv) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  const int N = R->N;
+  std::vector<double> hc((size_t)5 * N), hd((size_t)3 * N);
+  HIP_TRY(hipMemcpyAsync(hc.data(), R->cv, hc.size() * sizeof(double), hipMemcpyDeviceToHost, R->stream));
+  HIP_TRY(hipMemcpyAsync(hd.data(), R->dv, hd.size() * sizeof(double), hipMemcpyDeviceToHost, R->stream));
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  if (cv)
+    for (int c = 0; c < N; ++c)
+      for (int k = 0; k < 5; ++k) cv[5 * (size_t)c + k] = hc[(size_t)k * N + c];
+  if (dv)
+    for (int c = 0; c < N; ++c)
+      for (int k = 0; k < 3; ++k) dv[3 * (size_t)c + k] = hd[(size_t)k * N + c];
+  return RFLU_OK;
+}
+
+int rflu_gpu_time_step(int32_t h, double current_time, double* dt_out) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!(current_time >= 0.0)) return fail(RFLU_ERR_ARG, "current_time must be >= 0");
+  int rc = launch_time_step(R, current_time);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());
+  if (dt_out) {
+    Scalars s;
+    if ((rc = read_scalars(R, &s))) return rc;
+    *dt_out = s.dt;
+  }
+  return RFLU_OK;
+}
+
+int rflu_gpu_rk_init(int32_t h, int32_t istage) {
+  // cvOld and rhsSum are kept by the update kernel: the stage-start state is the
+  // current cv, and RK4 / SSP-RK3 save the step-start state at stage 1.  diss is zero
+  // for inviscid runs, so rhs starts at zero inside the gather.
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (istage < 1 || istage > nstages(R->rk)) return fail(RFLU_ERR_ARG, "bad stage");
+  return RFLU_OK;
+}
+
+int rflu_gpu_cell_gradients(int32_t h) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  int rc = launch_gradients(R);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());
+  return RFLU_OK;
+}
+
+int rflu_gpu_convective_fluxes(int32_t h, int32_t istage) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (istage < 1 || istage > nstages(R->rk)) return fail(RFLU_ERR_ARG, "bad stage");
+  int rc = launch_fluxes(R, istage);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());
+  return RFLU_OK;
+}
+
+int rflu_gpu_rk_update(int32_t h, int32_t istage) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (istage < 1 || istage > nstages(R->rk)) return fail(RFLU_ERR_ARG, "bad stage");
+  int rc = launch_update(R, istage, false);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());
+  return RFLU_OK;
+}
+
+int rflu_gpu_set_dependent_vars(int32_t h) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  Mesh M = R->mesh();
+  State S = R->state();
+  hipLaunchKernelGGL(k_set_dv, dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S, R->gas);
+  if (R->order == 2)
+    hipLaunchKernelGGL(k_set_prim, dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S);
+  HIP_TRY(hipGetLastError());
+  return RFLU_OK;
+}
+
+int rflu_gpu_after_update(int32_t h, int32_t istage) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (istage < 1 || istage > nstages(R->rk)) return fail(RFLU_ERR_ARG, "bad stage");
+  Mesh M = R->mesh();
+  State S = R->state();
+  if (R->order == 2)
+    hipLaunchKernelGGL((k_after_update<true>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S, R->gas, R->sc);
+  else
+    hipLaunchKernelGGL((k_after_update<false>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S, R->gas, R->sc);
+  HIP_TRY(hipGetLastError());
+  Scalars s;
+  int rc = read_scalars(R, &s);
+  if (rc) return rc;
+  return status_from(s);
+}
+
+int rflu_gpu_residual(int32_t h, double* rhs) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!rhs) return fail(RFLU_ERR_ARG, "null rhs");
+  int rc;
+  if ((rc = launch_gradients(R)) || (rc = launch_fluxes(R, 0))) return rc;
+  Mesh M = R->mesh();
+  if (!flux32(R))
+    hipLaunchKernelGGL((k_gather_rhs<double>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, (const double*)R->flux, R->rhs);
+  else
+    hipLaunchKernelGGL((k_gather_rhs<float>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, (const float*)R->flux, R->rhs);
+  HIP_TRY(hipGetLastError());
+  const int N = R->N;
+  std::vector<double> hr((size_t)5 * N);
+  HIP_TRY(hipMemcpyAsync(hr.data(), R->rhs, hr.size() * sizeof(double), hipMemcpyDeviceToHost, R->stream));
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  for (int c = 0; c < N; ++c)
+    for (int k = 0; k < 5; ++k) rhs[5 * (size_t)c + k] = hr[(size_t)k * N + c];
+  return RFLU_OK;
+}
+
+int rflu_gpu_time_steps(int32_t h, int64_t nsteps, int32_t use_graph) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (nsteps < 0) return fail(RFLU_ERR_ARG, "nsteps < 0");
+  int rc;
+  if (use_graph && !R->profiling) {
+    if (R->graph_dirty || !R->graph) {
+      if (R->graph) { (void)hipGraphExecDestroy(R->graph); R->graph = nullptr; }
+      hipGraph_t g;
+      HIP_TRY(hipStreamBeginCapture(R->stream, hipStreamCaptureModeThreadLocal));
+      rc = launch_step(R);
+      hipError_t ce = hipStreamEndCapture(R->stream, &g);
+      if (rc) return rc;
+      if (ce != hipSuccess) return fail(RFLU_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+      hipError_t ie = hipGraphInstantiate(&R->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) return fail(RFLU_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+      R->graph_dirty = false;
+    }
+    for (int64_t s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(R->graph, R->stream));
+  } else {
+    for (int64_t s = 0; s < nsteps; ++s)
+      if ((rc = launch_step(R))) return rc;
+  }
+  HIP_TRY(hipGetLastError());
+  Scalars sc;
+  if ((rc = read_scalars(R, &sc))) return rc;
+  if (R->profiling) collect_events(R);
+  return status_from(sc);
+}
+
+int rflu_gpu_get_stats(int32_t h, rflu_gpu_stats* st) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!st) return fail(RFLU_ERR_ARG, "null stats");
+  Scalars s;
+  int rc = read_scalars(R, &s);
+  if (rc) return rc;
+  st->n_invalid = (int64_t)s.n_invalid;
+  st->n_negative = (int64_t)s.n_negative;
+  st->n_bounds_fix = (int64_t)s.n_bounds_fix;
+  st->n_pos_fix = (int64_t)s.n_pos_fix;
+  st->mass_in = s.mass_in;
+  st->mass_out = s.mass_out;
+  st->time = s.time;
+  st->dt = s.dt;
+  st->steps = (int64_t)s.steps;
+  return RFLU_OK;
+}
+
+int rflu_gpu_reset_stats(int32_t h) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, R->stream, R->sc);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  return RFLU_OK;
+}
+
+int rflu_gpu_synchronize(int32_t h) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  return RFLU_OK;
+}
+
+int rflu_gpu_set_launch(int32_t h, int32_t face_block, int32_t cell_block, int32_t grad_block) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  auto ok = [](int b) { return b == 0 || (b >= 64 && b <= 1024 && b % 64 == 0); };
+  if (!ok(face_block) || !ok(cell_block) || !ok(grad_block))
+    return fail(RFLU_ERR_ARG, "block sizes must be multiples of 64 in [64, 1024]");
+  if (face_block) R->face_block = face_block;
+  if (cell_block) R->cell_block = cell_block;
+  if (grad_block) R->grad_block = grad_block;
+  R->graph_dirty = true;
+  return RFLU_OK;
+}
+
+int rflu_gpu_time_kernel(int32_t h, int32_t which, int32_t reps, double* ms) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!ms || reps < 1 || which < 0 || which > 5) return fail(RFLU_ERR_ARG, "bad arguments");
+  hipEvent_t a, b;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  bool prof = R->profiling;
+  R->profiling = false;
+  int rc = 0;
+  HIP_TRY(hipEventRecord(a, R->stream));
+  for (int r = 0; r < reps && !rc; ++r) {
+    switch (which) {
+      case 0: rc = launch_fluxes(R, 2); break;
+      case 1: rc = launch_update(R, 1, false); break;
+      case 2: rc = launch_gradients(R); break;
+      case 3: rc = grad32(R) ? launch_blend_t<float>(R) : launch_blend_t<double>(R); break;
+      case 4: rc = launch_time_step(R); break;
+      default:
+        if (!(rc = launch_gradients(R)) && !(rc = launch_fluxes(R, 2))) rc = launch_update(R, 1, true);
+        break;
+    }
+  }
+  R->profiling = prof;
+  HIP_TRY(hipEventRecord(b, R->stream));
+  HIP_TRY(hipEventSynchronize(b));
+  float t = 0;
+  HIP_TRY(hipEventElapsedTime(&t, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (rc) return rc;
+  *ms = t / reps;
+  return RFLU_OK;
+}
+
+// Per-kernel timing of the launches made by rflu_gpu_time_steps (HIP events on the
+// region's stream around each launch; graphs are not used while on).
+int rflu_gpu_set_profiling(int32_t h, int32_t on) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  R->profiling = on != 0;
+  for (int k = 0; k < KID_N; ++k) { R->kern_ms[k] = 0; R->kern_n[k] = 0; }
+  return RFLU_OK;
+}
+
+int rflu_gpu_get_kernel_times(int32_t h, double* ms, int64_t* counts, int32_t n) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!ms || !counts || n < 1) return fail(RFLU_ERR_ARG, "bad arguments");
+  for (int k = 0; k < n && k < KID_N; ++k) { ms[k] = R->kern_ms[k]; counts[k] = R->kern_n[k]; }
+  return RFLU_OK;
+}
+
+}  // extern "C"

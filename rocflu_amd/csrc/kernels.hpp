@@ -1,0 +1,1009 @@
+// Device kernels of the explicit residual/update path for gfx950.
+//
+// Layout in HBM (all structure-of-arrays so a wavefront's 64 lanes read 64
+// consecutive cells or faces): cv[5][N], dv[3][N], prim[5][N] = (rho,u,v,w,p),
+// grad[15][N] with component index 3*var+axis, flux[5][F+B].  Face geometry is
+// packed as double4 (nx,ny,nz,|A|) and double4 (xc,yc,zc,-); cells as double4
+// (x,y,z,vol).  Interior faces come first in the flux buffer, then the boundary
+// faces patch by patch.  The cell-to-face list c2f stores +(f+1) where the cell is
+// f2c(1,f) or the face is a boundary face, and -(f+1) where it is f2c(2,f), in the
+// reference's scatter order, so the per-cell gather sums in that order.
+//
+// Arithmetic follows the reference's expressions operand by operand; the library
+// is compiled with FMA contraction off so FP64 results match the reference's
+// gfortran build rounding for rounding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rflu {
+
+constexpr int BC_OUTFLOW = 20;
+constexpr int BC_SLIPWALL = 60;
+constexpr int BCOPT_SUPERSONIC = 0;
+constexpr int BCOPT_SUBSONIC = 1;
+constexpr int BCOPT_MIXED = 2;
+constexpr int MAX_PATCHES = 32;
+
+struct PatchTable {
+  int n;
+  int bc[MAX_PATCHES];
+  int opt[MAX_PATCHES];
+  int order2[MAX_PATCHES];
+  double pout[MAX_PATCHES];
+};
+
+struct Gas {
+  double g, cp, gc;            // refGamma, refCp, R = cp - cp/g (flux functions)
+  double gp, rgasp, mm;        // gamma', R' and M of the dependent-variable path
+  double eps_entr, diss_fact;
+};
+
+// Device-resident scalars of a region.
+struct Scalars {
+  double time, dt;
+  double dt_local_min;
+  double mass_in, mass_out;
+  unsigned long long n_invalid, n_negative, n_bounds_fix, n_pos_fix;
+  unsigned long long steps;
+  int abort;
+  int pad;
+};
+
+struct Mesh {
+  int N, F, B;
+  const int2* f2c;
+  const double4* fn;
+  const double4* fc;
+  const double4* cell;     // cofg + vol
+  const int* bf2c;
+  const double4* bfn;
+  const double4* bfc;
+  const int* bpatch;
+  const int* c2f_ptr;
+  const int* c2f_idx;
+  const int* sten_ptr;
+  const int* sten_idx;
+  const double* moms;      // [6][N]: r11 r12 r22 r13 r23 r33
+  const double* epsq;      // [N] Venkatakrishnan eps^2 = (vol^(1/3))^3
+};
+
+struct State {
+  double* cv;      // [5][N]
+  double* cv_old;  // [5][N]  step-start state (RK4, SSP-RK3)
+  double* rhs_sum; // [5][N]
+  double* dv;      // [3][N]
+  double* prim;    // [5][N]
+};
+
+template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+__device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float tsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double tabs(double x) { return fabs(x); }
+__device__ __forceinline__ float tabs(float x) { return fabsf(x); }
+
+// ---------------------------------------------------------------------------------
+// Flux functions (interior faces).  See the oracle for the line-by-line citations:
+// HLLC modflu/RFLU_ModHLLCFlux.F90:1064-1117, Roe modflu/RFLU_ModRoeFlux.F90:
+// 1838-1894 with RFLU_ModEntropyFixes.F90:95-106, AUSM+ RFLU_ModAUSMFlux.F90:282-367.
+// ---------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void hllc(T g, T nx, T ny, T nz, T nm, T rl, T ul, T vl, T wl, T pl,
+                                     T al, T el, T Hl, T rr, T ur, T vr, T wr, T pr, T ar, T er,
+                                     T Hr, T* flx) {
+  const T one = T(1), half = T(0.5), zero = T(0);
+  T ql = ul * nx + vl * ny + wl * nz;
+  T qr = ur * nx + vr * ny + wr * nz;
+  T wt1 = tsqrt(rr / rl);
+  T wt2 = one / (one + wt1);
+  T us = wt2 * (ul + wt1 * ur);
+  T vs = wt2 * (vl + wt1 * vr);
+  T ws = wt2 * (wl + wt1 * wr);
+  T qs = wt2 * (ql + wt1 * qr);
+  T Hs = wt2 * (Hl + wt1 * Hr);
+  T Vms = us * us + vs * vs + ws * ws;
+  T as = tsqrt((g - one) * (Hs - half * Vms));
+  T sl = tmin(ql - al, qs - as);
+  T sr = tmax(qr + ar, qs + as);
+  if (sl > zero) {
+    flx[0] = ql * rl * nm;
+    flx[1] = (ql * rl * ul + pl * nx) * nm;
+    flx[2] = (ql * rl * vl + pl * ny) * nm;
+    flx[3] = (ql * rl * wl + pl * nz) * nm;
+    flx[4] = (ql * (rl * el + pl)) * nm;
+  } else if (sr < zero) {
+    flx[0] = qr * rr * nm;
+    flx[1] = (qr * rr * ur + pr * nx) * nm;
+    flx[2] = (qr * rr * vr + pr * ny) * nm;
+    flx[3] = (qr * rr * wr + pr * nz) * nm;
+    flx[4] = (qr * (rr * er + pr)) * nm;
+  } else {
+    T sm = (rr * qr * (sr - qr) - rl * ql * (sl - ql) - pr + pl) /
+           (rr * (sr - qr) - rl * (sl - ql));
+    T ps = pl + rl * (ql - sl) * (ql - sm);
+    T term, rs, rus, rvs, rws, res;
+    if (sm > zero) {
+      term = one / (sl - sm);
+      rs = term * (sl - ql) * rl;
+      rus = term * ((sl - ql) * rl * ul + (ps - pl) * nx);
+      rvs = term * ((sl - ql) * rl * vl + (ps - pl) * ny);
+      rws = term * ((sl - ql) * rl * wl + (ps - pl) * nz);
+      res = term * ((sl - ql) * rl * el + (ps * sm - pl * ql));
+    } else {
+      term = one / (sr - sm);
+      rs = term * (sr - qr) * rr;
+      rus = term * ((sr - qr) * rr * ur + (ps - pr) * nx);
+      rvs = term * ((sr - qr) * rr * vr + (ps - pr) * ny);
+      rws = term * ((sr - qr) * rr * wr + (ps - pr) * nz);
+      res = term * ((sr - qr) * rr * er + (ps * sm - pr * qr));
+    }
+    flx[0] = sm * rs * nm;
+    flx[1] = (sm * rus + ps * nx) * nm;
+    flx[2] = (sm * rvs + ps * ny) * nm;
+    flx[3] = (sm * rws + ps * nz) * nm;
+    flx[4] = (sm * (res + ps)) * nm;
+  }
+}
+
+template <typename T> __device__ __forceinline__ T ef_hh(T l, T d) {
+  return l > d ? l : (l * l + d * d) / (T(2) * d);
+}
+
+template <typename T>
+__device__ __forceinline__ void roe(T g, T epsentr, T dissFact, T nx, T ny, T nz, T nm, T rl,
+                                    T ul, T vl, T wl, T pl, T Hl, T rr, T ur, T vr, T wr, T pr,
+                                    T Hr, T* flx) {
+  const T one = T(1), half = T(0.5), two = T(2);
+  T ql = ul * nx + vl * ny + wl * nz;
+  T qr = ur * nx + vr * ny + wr * nz;
+  T rh = tsqrt(rl * rr);
+  T wt = rl / (rl + rh);
+  T uh = wt * ul + (one - wt) * ur;
+  T vh = wt * vl + (one - wt) * vr;
+  T wh = wt * wl + (one - wt) * wr;
+  T Hh = wt * Hl + (one - wt) * Hr;
+  T qh = uh * nx + vh * ny + wh * nz;
+  T sh = half * (uh * uh + vh * vh + wh * wh);
+  T ah = tsqrt((g - one) * (Hh - half * (two * sh)));
+  T l1 = tabs(qh - ah);
+  T l2 = tabs(qh);
+  T l5 = tabs(qh + ah);
+  T efc = epsentr * l5;
+  l1 = ef_hh(l1, efc);
+  l2 = ef_hh(l2, efc);
+  l5 = ef_hh(l5, efc);
+  T dr = rr - rl, du = ur - ul, de = vr - vl, dw = wr - wl, dp = pr - pl;
+  T dq = du * nx + de * ny + dw * nz;
+  T dv1 = (dp - rh * ah * dq) / (two * ah * ah);
+  T dv5 = (dp + rh * ah * dq) / (two * ah * ah);
+  T dv2 = dr - dp / (ah * ah);
+  T t1 = l1 * dv1, t2 = l2 * dv2, t3 = l2 * rh, t5 = l5 * dv5;
+  T hd = half * dissFact;
+  T fc1 = half * (ql * rl + qr * rr) * nm;
+  T fc2 = half * (ql * rl * ul + pl * nx + qr * rr * ur + pr * nx) * nm;
+  T fc3 = half * (ql * rl * vl + pl * ny + qr * rr * vr + pr * ny) * nm;
+  T fc4 = half * (ql * rl * wl + pl * nz + qr * rr * wr + pr * nz) * nm;
+  T fc5 = half * (ql * rl * Hl + qr * rr * Hr) * nm;
+  T fd1 = hd * (t1 + t2 + t5) * nm;
+  T fd2 = hd * (t1 * (uh - nx * ah) + t2 * uh + t3 * (du - nx * dq) + t5 * (uh + nx * ah)) * nm;
+  T fd3 = hd * (t1 * (vh - ny * ah) + t2 * vh + t3 * (de - ny * dq) + t5 * (vh + ny * ah)) * nm;
+  T fd4 = hd * (t1 * (wh - nz * ah) + t2 * wh + t3 * (dw - nz * dq) + t5 * (wh + nz * ah)) * nm;
+  T fd5 = hd * (t1 * (Hh - qh * ah) + t2 * sh + t3 * (uh * du + vh * de + wh * dw - qh * dq) +
+                t5 * (Hh + qh * ah)) * nm;
+  flx[0] = fc1 - fd1;
+  flx[1] = fc2 - fd2;
+  flx[2] = fc3 - fd3;
+  flx[3] = fc4 - fd4;
+  flx[4] = fc5 - fd5;
+}
+
+template <typename T>
+__device__ __forceinline__ void ausm(T nx, T ny, T nz, T nm, T rl, T ul, T vl, T wl, T pl, T Hl,
+                                     T al, T rr, T ur, T vr, T wr, T pr, T Hr, T ar, T* flx) {
+  const T one = T(1), half = T(0.5), q4 = T(0.25), e8 = T(0.125), c3 = T(0.1875), two = T(2);
+  T ql = ul * nx + vl * ny + wl * nz;
+  T qr = ur * nx + vr * ny + wr * nz;
+  T af = half * (al + ar);
+  T ml = ql / af, mla = tabs(ml);
+  T mr = qr / af, mra = tabs(mr);
+  T mlp, wtl, mrm, wtr;
+  if (mla <= one) {
+    mlp = q4 * (ml + one) * (ml + one) + e8 * (ml * ml - one) * (ml * ml - one);
+    wtl = q4 * (ml + one) * (ml + one) * (two - ml) + c3 * ml * (ml * ml - one) * (ml * ml - one);
+  } else {
+    mlp = half * (ml + mla);
+    wtl = half * (one + ml / mla);
+  }
+  if (mra <= one) {
+    mrm = -q4 * (mr - one) * (mr - one) - e8 * (mr * mr - one) * (mr * mr - one);
+    wtr = q4 * (mr - one) * (mr - one) * (two + mr) - c3 * mr * (mr * mr - one) * (mr * mr - one);
+  } else {
+    mrm = half * (mr - mra);
+    wtr = half * (one - mr / mra);
+  }
+  T mf = mlp + mrm;
+  T mfa = tabs(mf);
+  T mfp = half * (mf + mfa);
+  T mfm = half * (mf - mfa);
+  T pf = wtl * pl + wtr * pr;
+  flx[0] = (af * (mfp * rl + mfm * rr)) * nm;
+  flx[1] = (af * (mfp * rl * ul + mfm * rr * ur) + pf * nx) * nm;
+  flx[2] = (af * (mfp * rl * vl + mfm * rr * vr) + pf * ny) * nm;
+  flx[3] = (af * (mfp * rl * wl + mfm * rr * wr) + pf * nz) * nm;
+  flx[4] = (af * (mfp * rl * Hl + mfm * rr * Hr)) * nm;
+}
+
+// Face state of cell c for a second-order flux: the flux loop reads cv after
+// CellGradientsMP's cons->prim->cons round trip, so momentum is rho*(u) with
+// u = (1/rho)*m, and velocities are that momentum times 1/rho again.
+template <typename T, typename GT>
+__device__ __forceinline__ void recon_state(const State& S, const Mesh& M, const GT* grad, int c,
+                                            double xf, double yf, double zf, T& r, T& u, T& v,
+                                            T& w, T& p, T& ir) {
+  const int N = M.N;
+  double rd = S.prim[c];
+  double ird = 1.0 / rd;
+  double ud = (rd * S.prim[N + c]) * ird;
+  double vd = (rd * S.prim[2 * N + c]) * ird;
+  double wd = (rd * S.prim[3 * N + c]) * ird;
+  double pd = S.prim[4 * N + c];
+  double4 cc = M.cell[c];
+  T dx = T(xf - cc.x), dy = T(yf - cc.y), dz = T(zf - cc.z);
+  const GT* g = grad + c;
+  r = T(rd) + T(g[0]) * dx + T(g[(size_t)N]) * dy + T(g[2 * (size_t)N]) * dz;
+  u = T(ud) + T(g[3 * (size_t)N]) * dx + T(g[4 * (size_t)N]) * dy + T(g[5 * (size_t)N]) * dz;
+  v = T(vd) + T(g[6 * (size_t)N]) * dx + T(g[7 * (size_t)N]) * dy + T(g[8 * (size_t)N]) * dz;
+  w = T(wd) + T(g[9 * (size_t)N]) * dx + T(g[10 * (size_t)N]) * dy + T(g[11 * (size_t)N]) * dz;
+  p = T(pd) + T(g[12 * (size_t)N]) * dx + T(g[13 * (size_t)N]) * dy + T(g[14 * (size_t)N]) * dz;
+  ir = T(ird);
+}
+
+// Flux of interior face f (computed with f2c(1,f) as the left state).
+template <int DISCR, int ORDER, typename T, typename GT>
+__device__ __forceinline__ void interior_face_flux(const Mesh& M, const State& S, const Gas& G,
+                                                   const GT* grad, int f, T* flx) {
+  const int N = M.N;
+  int2 cc = M.f2c[f];
+  int c1 = cc.x, c2 = cc.y;
+  double4 n = M.fn[f];
+  T nx = T(n.x), ny = T(n.y), nz = T(n.z), nm = T(n.w);
+  const T g = T(G.g), cp = T(G.cp), gc = T(G.gc), one = T(1), half = T(0.5);
+  if (ORDER == 1) {
+    double rl = S.cv[c1], rr = S.cv[c2];
+    double irl = 1.0 / rl, irr = 1.0 / rr;
+    double ul = S.cv[N + c1] * irl, vl = S.cv[2 * N + c1] * irl, wl = S.cv[3 * N + c1] * irl;
+    double ur = S.cv[N + c2] * irr, vr = S.cv[2 * N + c2] * irr, wr = S.cv[3 * N + c2] * irr;
+    double pl = S.dv[c1], pr = S.dv[c2];
+    if (DISCR == 3) {
+      double al = S.dv[2 * N + c1], ar = S.dv[2 * N + c2];
+      double el = S.cv[4 * N + c1] * irl, er = S.cv[4 * N + c2] * irr;
+      double Hl = el + irl * pl, Hr = er + irr * pr;
+      hllc<T>(g, nx, ny, nz, nm, T(rl), T(ul), T(vl), T(wl), T(pl), T(al), T(el), T(Hl), T(rr),
+              T(ur), T(vr), T(wr), T(pr), T(ar), T(er), T(Hr), flx);
+    } else if (DISCR == 1) {
+      double tl = S.dv[N + c1], tr = S.dv[N + c2];
+      double Hl = G.cp * tl + 0.5 * (ul * ul + vl * vl + wl * wl);
+      double Hr = G.cp * tr + 0.5 * (ur * ur + vr * vr + wr * wr);
+      roe<T>(g, T(G.eps_entr), T(G.diss_fact), nx, ny, nz, nm, T(rl), T(ul), T(vl), T(wl), T(pl),
+             T(Hl), T(rr), T(ur), T(vr), T(wr), T(pr), T(Hr), flx);
+    } else {
+      double al = S.dv[2 * N + c1], ar = S.dv[2 * N + c2];
+      double Hl = S.cv[4 * N + c1] * irl + pl / rl;
+      double Hr = S.cv[4 * N + c2] * irr + pr / rr;
+      ausm<T>(nx, ny, nz, nm, T(rl), T(ul), T(vl), T(wl), T(pl), T(Hl), T(al), T(rr), T(ur),
+              T(vr), T(wr), T(pr), T(Hr), T(ar), flx);
+    }
+  } else {
+    double4 xf = M.fc[f];
+    T rl, ul, vl, wl, pl, irl, rr, ur, vr, wr, pr, irr;
+    recon_state<T, GT>(S, M, grad, c1, xf.x, xf.y, xf.z, rl, ul, vl, wl, pl, irl);
+    recon_state<T, GT>(S, M, grad, c2, xf.x, xf.y, xf.z, rr, ur, vr, wr, pr, irr);
+    if (DISCR == 3) {
+      // RFLU_ModHLLCFlux.F90:2262-2267: E of the face state, H with the cell 1/rho.
+      T el = pl / (rl * (g - one)) + half * (ul * ul + vl * vl + wl * wl);
+      T al = tsqrt(g * pl / rl);
+      T Hl = el + irl * pl;
+      T er = pr / (rr * (g - one)) + half * (ur * ur + vr * vr + wr * wr);
+      T ar = tsqrt(g * pr / rr);
+      T Hr = er + irr * pr;
+      hllc<T>(g, nx, ny, nz, nm, rl, ul, vl, wl, pl, al, el, Hl, rr, ur, vr, wr, pr, ar, er, Hr,
+              flx);
+    } else if (DISCR == 1) {
+      T tl = pl / (rl * gc), tr = pr / (rr * gc);
+      T Hl = cp * tl + half * (ul * ul + vl * vl + wl * wl);
+      T Hr = cp * tr + half * (ur * ur + vr * vr + wr * wr);
+      roe<T>(g, T(G.eps_entr), T(G.diss_fact), nx, ny, nz, nm, rl, ul, vl, wl, pl, Hl, rr, ur, vr,
+             wr, pr, Hr, flx);
+    } else {
+      T tl = pl / (rl * gc), tr = pr / (rr * gc);
+      T al = tsqrt(g * gc * tl), ar = tsqrt(g * gc * tr);
+      T Hl = cp * tl + half * (ul * ul + vl * vl + wl * wl);
+      T Hr = cp * tr + half * (ur * ur + vr * vr + wr * wr);
+      ausm<T>(nx, ny, nz, nm, rl, ul, vl, wl, pl, Hl, al, rr, ur, vr, wr, pr, Hr, ar, flx);
+    }
+  }
+}
+
+template <int DISCR, int ORDER, typename T, typename GT, typename FT>
+__global__ void k_face_flux(Mesh M, State S, Gas G, const GT* __restrict__ grad,
+                            FT* __restrict__ flux, const Scalars* __restrict__ sc) {
+  if (sc->abort) return;
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= M.F) return;
+  T flx[5];
+  interior_face_flux<DISCR, ORDER, T, GT>(M, S, G, grad, f, flx);
+  const size_t FB = (size_t)M.F + M.B;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) flux[k * FB + f] = FT(flx[k]);
+}
+
+// Outflow boundary state (libfloflu/BcondOutflowPerf.F90:111-157), FP64.
+__device__ __forceinline__ void bcond_outflow(int bcOpt, double pout, double sxn, double syn,
+                                              double szn, double gamma, double rho, double rhou,
+                                              double rhov, double rhow, double rhoe,
+                                              double press, double* b) {
+  double u = rhou / rho, v = rhov / rho, w = rhow / rho;
+  double csound = sqrt(gamma * press / rho);
+  double mach = sqrt(u * u + v * v + w * w) / csound;
+  if (mach < 1.0 && (bcOpt == BCOPT_SUBSONIC || bcOpt == BCOPT_MIXED)) {
+    double rrhoc = 1.0 / (rho * csound);
+    double deltp = press - pout;
+    double rb = rho - deltp / (csound * csound);
+    double ub = u + sxn * deltp * rrhoc;
+    double vb = v + syn * deltp * rrhoc;
+    double wb = w + szn * deltp * rrhoc;
+    double vnd = ub * sxn + vb * syn + wb * szn;
+    if (vnd < 0.0) {
+      ub = copysign(1.0, u) * fmax(fabs(ub), fabs(u));
+      vb = copysign(1.0, v) * fmax(fabs(vb), fabs(v));
+      wb = copysign(1.0, w) * fmax(fabs(wb), fabs(w));
+    }
+    b[0] = rb;
+    b[1] = rb * ub;
+    b[2] = rb * vb;
+    b[3] = rb * wb;
+    b[4] = rb * (pout / (rb * (gamma - 1.0)) + 0.5 * (ub * ub + vb * vb + wb * wb));
+  } else {
+    b[0] = rho; b[1] = rhou; b[2] = rhov; b[3] = rhow; b[4] = rhoe;
+  }
+}
+
+// Boundary faces (rocflu/RFLU_CentralFirstPatch.F90:571-906,
+// rocflu/RFLU_CentralSecondPatch.F90:634-1024).  Slip walls store a zero mass flux.
+// Boundary fluxes stay FP64 in every precision mode (they are few).
+template <int ORDER, typename GT, typename FT>
+__global__ void k_bface_flux(Mesh M, State S, Gas G, PatchTable P, const GT* __restrict__ grad,
+                             FT* __restrict__ flux, const Scalars* __restrict__ sc) {
+  if (sc->abort) return;
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= M.B) return;
+  const int N = M.N;
+  int c1 = M.bf2c[b];
+  int p = M.bpatch[b];
+  double4 n = M.bfn[b];
+  double nx = n.x, ny = n.y, nz = n.z, nm = n.w;
+  double flx[5];
+  int bc = P.bc[p];
+  double rl, rul, rvl, rwl, rel, pl;
+  bool o2 = ORDER == 2 && P.order2[p];
+  if (!o2) {
+    if (ORDER == 2) {
+      // First-order patch in a second-order run: cv as left by CellGradientsMP,
+      // momentum rho*((1/rho)*m) and slot 5 holding p
+      // (modflu/RFLU_ModConvertCv.F90:560-574).
+      rl = S.prim[c1];
+      rul = rl * S.prim[N + c1];
+      rvl = rl * S.prim[2 * N + c1];
+      rwl = rl * S.prim[3 * N + c1];
+      rel = S.prim[4 * N + c1];
+      pl = S.prim[4 * N + c1];
+    } else {
+      rl = S.cv[c1];
+      rul = S.cv[N + c1];
+      rvl = S.cv[2 * N + c1];
+      rwl = S.cv[3 * N + c1];
+      rel = S.cv[4 * N + c1];
+      pl = S.dv[c1];
+    }
+  } else {
+    double4 xf = M.bfc[b];
+    double4 cc = M.cell[c1];
+    double dx = xf.x - cc.x, dy = xf.y - cc.y, dz = xf.z - cc.z;
+    double r0 = S.prim[c1];
+    double ir = 1.0 / r0;
+    double ul = (r0 * S.prim[N + c1]) * ir;
+    double vl = (r0 * S.prim[2 * N + c1]) * ir;
+    double wl = (r0 * S.prim[3 * N + c1]) * ir;
+    double p0 = S.prim[4 * N + c1];
+    const GT* g = grad + c1;
+    rl = r0 + double(g[0]) * dx + double(g[(size_t)N]) * dy + double(g[2 * (size_t)N]) * dz;
+    ul = ul + double(g[3 * (size_t)N]) * dx + double(g[4 * (size_t)N]) * dy + double(g[5 * (size_t)N]) * dz;
+    vl = vl + double(g[6 * (size_t)N]) * dx + double(g[7 * (size_t)N]) * dy + double(g[8 * (size_t)N]) * dz;
+    wl = wl + double(g[9 * (size_t)N]) * dx + double(g[10 * (size_t)N]) * dy + double(g[11 * (size_t)N]) * dz;
+    pl = p0 + double(g[12 * (size_t)N]) * dx + double(g[13 * (size_t)N]) * dy + double(g[14 * (size_t)N]) * dz;
+    rul = rl * ul;
+    rvl = rl * vl;
+    rwl = rl * wl;
+    rel = rl * (pl / (rl * (G.gp - 1.0)) + 0.5 * (ul * ul + vl * vl + wl * wl));
+  }
+  if (bc == BC_SLIPWALL) {
+    flx[0] = 0.0;
+    flx[1] = pl * nx * nm;
+    flx[2] = pl * ny * nm;
+    flx[3] = pl * nz * nm;
+    flx[4] = 0.0;
+  } else {
+    int opt = P.opt[p];
+    double pr = opt != BCOPT_SUPERSONIC ? P.pout[p] : pl;
+    double bs[5];
+    bcond_outflow(opt, pr, nx, ny, nz, G.gp, rl, rul, rvl, rwl, rel, pl, bs);
+    double ql = (rul * nx + rvl * ny + rwl * nz) / rl;
+    double qr = (bs[1] * nx + bs[2] * ny + bs[3] * nz) / bs[0];
+    flx[0] = 0.5 * (ql * rl + qr * bs[0]) * nm;
+    flx[1] = 0.5 * (ql * rul + pl * nx + qr * bs[1] + pr * nx) * nm;
+    flx[2] = 0.5 * (ql * rvl + pl * ny + qr * bs[2] + pr * ny) * nm;
+    flx[3] = 0.5 * (ql * rwl + pl * nz + qr * bs[3] + pr * nz) * nm;
+    flx[4] = 0.5 * (ql * (rel + pl) + qr * (bs[4] + pr)) * nm;
+  }
+  const size_t FB = (size_t)M.F + M.B;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) flux[k * FB + M.F + b] = FT(flx[k]);
+}
+
+// Dependent variables p, T, c (libflu/RFLU_SetDependentVars.F90:255-284).
+__device__ __forceinline__ void dep_vars(const Gas& G, double r, double mx, double my, double mz,
+                                         double E, double& p, double& t, double& a) {
+  double ir = 1.0 / r;
+  double Eo = ir * E;
+  double Vm2 = ir * ir * (mx * mx + my * my + mz * mz);
+  p = (G.gp - 1.0) * r * (Eo - 0.5 * Vm2);
+  t = p / (r * G.rgasp);
+  a = sqrt(G.gp * G.rgasp * t);
+}
+
+// Gathers the residual of cell i over its cell-to-face list.
+template <typename FT>
+__device__ __forceinline__ void gather_residual(const Mesh& M, const FT* __restrict__ flux, int i,
+                                                double* acc) {
+  const size_t FB = (size_t)M.F + M.B;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) acc[k] = -0.0;
+  int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
+  for (int e = e0; e < e1; ++e) {
+    int s = M.c2f_idx[e];
+    if (s > 0) {
+      size_t f = (size_t)(s - 1);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) acc[k] = acc[k] + double(flux[k * FB + f]);
+    } else {
+      size_t f = (size_t)(-s - 1);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) acc[k] = acc[k] - double(flux[k * FB + f]);
+    }
+  }
+}
+
+// Wavefront-level OR/ADD helpers for the rare after-update events.
+__device__ __forceinline__ void count_event(unsigned long long* ctr) { atomicAdd(ctr, 1ull); }
+
+// After-update checks of one cell (libfloflu/AfterUpdateMP.F90:134-136).  dv must be
+// the dependent variables of the updated cv.  Returns the final p for prim.
+__device__ __forceinline__ void after_update_cell(const Gas& G, double* q, double& p, double& t,
+                                                  double& a, Scalars* sc) {
+  // RFLU_CheckValidity.F90:141-222
+  {
+    double rrho = 1.0 / q[0];
+    double u = rrho * q[1], v = rrho * q[2], w = rrho * q[3];
+    if (isnan(q[0]) || isnan(u) || isnan(v) || isnan(w) || isnan(p) || isnan(t)) {
+      count_event(&sc->n_invalid);
+      sc->abort = 1;
+    }
+  }
+  // RFLU_EnforceBounds.F90:169-231 (p recomputed with its own rounding, dv untouched)
+  {
+    double r = q[0], ir = 1.0 / r;
+    double u = ir * q[1], v = ir * q[2], w = ir * q[3];
+    double Eo = ir * q[4];
+    double Vm2 = u * u + v * v + w * w;
+    double pe = (G.gp - 1.0) * r * (Eo - 0.5 * Vm2);
+    if (pe <= 0.0 && fabs(pe) <= 1.0E-12) {
+      count_event(&sc->n_bounds_fix);
+      pe = 1.0E-12;
+      q[4] = r * (pe / (r * (G.gp - 1.0)) + 0.5 * (u * u + v * v + w * w));
+    }
+  }
+  // RFLU_CheckPositivity.F90:147-253
+  {
+    double rho = q[0], rrho = 1.0 / rho;
+    double u = rrho * q[1], v = rrho * q[2], w = rrho * q[3];
+    double Vm2 = u * u + v * v + w * w;
+    if (p <= 0.0) {
+      count_event(&sc->n_pos_fix);
+      double Eo = 1.0e4 + 0.5 * Vm2;
+      q[4] = rho * Eo;
+      dep_vars(G, q[0], q[1], q[2], q[3], q[4], p, t, a);
+    }
+    if (rho <= 0.0 || p <= 0.0) {
+      count_event(&sc->n_negative);
+      sc->abort = 1;
+    }
+  }
+}
+
+// RK update (libfloflu/RkUpdateGeneric.F90:255-325) of one cell given its residual.
+// q holds the stage-start state on entry and the updated state on return.
+template <int RK>
+__device__ __forceinline__ void rk_update_cell(const State& S, int N, int i, int stage, double dt,
+                                               double vol, const double* acc, double* q) {
+  if (RK == 1) {
+    const double ark[4] = {0.5, 0.5, 1.0, 1.0 / 6.0};
+    const double grk[4] = {1.0, 2.0, 2.0, 1.0};
+    double fac = ark[stage - 1] * dt;
+    double adtv = fac / vol;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      size_t o = (size_t)k * N + i;
+      double old = stage == 1 ? q[k] : S.cv_old[o];
+      if (stage == 1) {
+        S.cv_old[o] = q[k];
+        q[k] = old - adtv * acc[k];
+        S.rhs_sum[o] = acc[k];
+      } else if (stage == 4) {
+        q[k] = old - adtv * (acc[k] + S.rhs_sum[o]);
+      } else {
+        q[k] = old - adtv * acc[k];
+        S.rhs_sum[o] = S.rhs_sum[o] + grk[stage - 1] * acc[k];
+      }
+    }
+  } else if (RK == 2) {
+    const double ark[3] = {8.0 / 15.0, 5.0 / 12.0, 0.75};
+    const double grk[3] = {0.0, 17.0 / 25.0, 5.0 / 9.0};
+    double fac = ark[stage - 1] * dt;
+    double adtv = fac / vol;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      size_t o = (size_t)k * N + i;
+      if (stage == 1) {
+        q[k] = q[k] - adtv * acc[k];
+        S.rhs_sum[o] = acc[k];
+      } else if (stage == 2) {
+        q[k] = q[k] - adtv * (acc[k] - grk[1] * S.rhs_sum[o]);
+        S.rhs_sum[o] = acc[k];
+      } else {
+        q[k] = q[k] - adtv * (acc[k] - grk[2] * S.rhs_sum[o]);
+      }
+    }
+  } else if (RK == 3) {
+    double fac = 1.0 * dt;
+    double adtv = fac / vol;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) q[k] = q[k] - adtv * acc[k];
+  } else {
+    // Shu-Osher SSP-RK3 (no counterpart in the reference)
+    double adtv = dt / vol;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      size_t o = (size_t)k * N + i;
+      double us = q[k] - adtv * acc[k];
+      if (stage == 1) {
+        S.cv_old[o] = q[k];
+        q[k] = us;
+      } else if (stage == 2) {
+        q[k] = 0.75 * S.cv_old[o] + 0.25 * us;
+      } else {
+        q[k] = (1.0 / 3.0) * S.cv_old[o] + (2.0 / 3.0) * us;
+      }
+    }
+  }
+}
+
+// Residual gather + RK update (+ dependent variables + after-update checks when
+// FUSED) for every cell.  Stage-start state is the current cv, so Wray and Euler
+// need no separate cvOld copy.
+template <int RK, typename FT, bool FUSED, bool WRITE_PRIM>
+__global__ void k_update(Mesh M, State S, Gas G, const FT* __restrict__ flux, Scalars* sc,
+                         int stage, int last_stage) {
+  if (sc->abort) return;
+  const int N = M.N;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double dt = sc->dt;
+  if (last_stage && FUSED && i == 0) {
+    sc->time = sc->time + dt;
+    sc->steps = sc->steps + 1;
+  }
+  if (i >= N) return;
+  double acc[5];
+  gather_residual<FT>(M, flux, i, acc);
+  double q[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) q[k] = S.cv[(size_t)k * N + i];
+  rk_update_cell<RK>(S, N, i, stage, dt, M.cell[i].w, acc, q);
+  if (FUSED) {
+    double p, t, a;
+    dep_vars(G, q[0], q[1], q[2], q[3], q[4], p, t, a);
+    after_update_cell(G, q, p, t, a, sc);
+    S.dv[i] = p;
+    S.dv[(size_t)N + i] = t;
+    S.dv[2 * (size_t)N + i] = a;
+    if (WRITE_PRIM) {
+      double ir = 1.0 / q[0];
+      S.prim[i] = q[0];
+      S.prim[(size_t)N + i] = ir * q[1];
+      S.prim[2 * (size_t)N + i] = ir * q[2];
+      S.prim[3 * (size_t)N + i] = ir * q[3];
+      S.prim[4 * (size_t)N + i] = p;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) S.cv[(size_t)k * N + i] = q[k];
+}
+
+// Dependent variables (and the primitive copy used by second-order kernels).
+__global__ void k_set_dv(Mesh M, State S, Gas G) {
+  const int N = M.N;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double p, t, a;
+  double r = S.cv[i];
+  dep_vars(G, r, S.cv[(size_t)N + i], S.cv[2 * (size_t)N + i], S.cv[3 * (size_t)N + i],
+           S.cv[4 * (size_t)N + i], p, t, a);
+  S.dv[i] = p;
+  S.dv[(size_t)N + i] = t;
+  S.dv[2 * (size_t)N + i] = a;
+}
+
+// Primitive copy (rho, u, v, w, p) from cv and dv (CellGradientsMP's conversion,
+// modflu/RFLU_ModConvertCv.F90:217-227).
+__global__ void k_set_prim(Mesh M, State S) {
+  const int N = M.N;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double r = S.cv[i];
+  double ir = 1.0 / r;
+  S.prim[i] = r;
+  S.prim[(size_t)N + i] = ir * S.cv[(size_t)N + i];
+  S.prim[2 * (size_t)N + i] = ir * S.cv[2 * (size_t)N + i];
+  S.prim[3 * (size_t)N + i] = ir * S.cv[3 * (size_t)N + i];
+  S.prim[4 * (size_t)N + i] = S.dv[i];
+}
+
+// After-update checks as a separate pass (the unfused entry points).
+template <bool WRITE_PRIM>
+__global__ void k_after_update(Mesh M, State S, Gas G, Scalars* sc) {
+  const int N = M.N;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double q[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) q[k] = S.cv[(size_t)k * N + i];
+  double p = S.dv[i], t = S.dv[(size_t)N + i], a = S.dv[2 * (size_t)N + i];
+  after_update_cell(G, q, p, t, a, sc);
+  S.cv[4 * (size_t)N + i] = q[4];
+  S.dv[i] = p;
+  S.dv[(size_t)N + i] = t;
+  S.dv[2 * (size_t)N + i] = a;
+  if (WRITE_PRIM) S.prim[4 * (size_t)N + i] = p;
+}
+
+// Residual gather alone (for rflu_gpu_residual).
+template <typename FT>
+__global__ void k_gather_rhs(Mesh M, const FT* __restrict__ flux, double* __restrict__ rhs) {
+  const int N = M.N;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double acc[5];
+  gather_residual<FT>(M, flux, i, acc);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) rhs[(size_t)k * N + i] = acc[k];
+}
+
+// ---------------------------------------------------------------------------------
+// Cell gradients: inverse-distance weighted least squares with R-factor weights
+// recomputed per stencil member (RFLU_ComputeGradCellsFast_3D,
+// modflu/RFLU_ModDifferentiationCells.F90:931-1144), optionally followed in the
+// same pass by the Venkatakrishnan limiter (modflu/RFLU_ModLimiters.F90:407-522,
+// 1100-1106), which needs only this cell's gradient and its face neighbours' values.
+// ---------------------------------------------------------------------------------
+template <typename GT, bool VENKAT>
+__global__ void k_gradient(Mesh M, State S, GT* __restrict__ grad, const Scalars* __restrict__ sc) {
+  if (sc->abort) return;
+  const int N = M.N;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  typedef GT T;
+  double r11 = M.moms[i], r12 = M.moms[(size_t)N + i], r22 = M.moms[2 * (size_t)N + i];
+  double r13 = M.moms[3 * (size_t)N + i], r23 = M.moms[4 * (size_t)N + i],
+         r33 = M.moms[5 * (size_t)N + i];
+  double c11d = 1.0 / r11, c22d = 1.0 / r22, c33d = 1.0 / r33;
+  double c12d = -c11d * r12;
+  double c13d = -(c11d * r13 + c12d * c22d * r23);
+  double c23d = -c22d * r23;
+  T c12 = T(c12d), c13 = T(c13d), c23 = T(c23d);
+  T c1111 = T(c11d * c11d), c2222 = T(c22d * c22d), c3333 = T(c33d * c33d);
+  double4 ci = M.cell[i];
+  double vi[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) vi[k] = S.prim[(size_t)k * N + i];
+  T g[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) g[k] = T(0);
+  int s0 = M.sten_ptr[i], s1 = M.sten_ptr[i + 1];
+  for (int s = s0; s < s1; ++s) {
+    int j = M.sten_idx[s];
+    double4 cj = M.cell[j];
+    T dx = T(cj.x - ci.x), dy = T(cj.y - ci.y), dz = T(cj.z - ci.z);
+    T term = T(1) / tsqrt(dx * dx + dy * dy + dz * dz);
+    dx = term * dx;
+    dy = term * dy;
+    dz = term * dz;
+    T term1 = c1111 * (dx);
+    T term2 = c2222 * (dy + c12 * dx);
+    T term3 = c3333 * (dz + c23 * dy + c13 * dx);
+    T wtx = term * (term1 + c12 * term2 + c13 * term3);
+    T wty = term * (term2 + c23 * term3);
+    T wtz = term * (term3);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      T dVar = T(S.prim[(size_t)k * N + j] - vi[k]);
+      g[3 * k + 0] = g[3 * k + 0] + wtx * dVar;
+      g[3 * k + 1] = g[3 * k + 1] + wty * dVar;
+      g[3 * k + 2] = g[3 * k + 2] + wtz * dVar;
+    }
+  }
+  if (VENKAT) {
+    // Limiter in FP64: varMin/varMax over interior-face neighbours, then the
+    // face-wise limiter with dx = cofg - fc (the reference's sign).
+    const double TINY = 1.0E-12;
+    double vmn[5], vmx[5], lim[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) { vmn[k] = vi[k]; vmx[k] = vi[k]; lim[k] = 1.0; }
+    int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
+    for (int e = e0; e < e1; ++e) {
+      int sgn = M.c2f_idx[e];
+      int f = (sgn > 0 ? sgn : -sgn) - 1;
+      if (f >= M.F) continue;
+      int2 cc = M.f2c[f];
+      int o = sgn > 0 ? cc.y : cc.x;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        double vo = S.prim[(size_t)k * N + o];
+        vmn[k] = vmn[k] < vo ? vmn[k] : vo;
+        vmx[k] = vmx[k] > vo ? vmx[k] : vo;
+      }
+    }
+    double epsq = M.epsq[i];
+    for (int e = e0; e < e1; ++e) {
+      int sgn = M.c2f_idx[e];
+      int f = (sgn > 0 ? sgn : -sgn) - 1;
+      if (f >= M.F) continue;
+      double4 xf = M.fc[f];
+      double dx = ci.x - xf.x, dy = ci.y - xf.y, dz = ci.z - xf.z;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        double dmn = vmn[k] - vi[k];
+        double dmx = vmx[k] - vi[k];
+        double d2 = double(g[3 * k]) * dx + double(g[3 * k + 1]) * dy + double(g[3 * k + 2]) * dz;
+        if (d2 > 0.0) {
+          double numer = (dmx * dmx + epsq) * d2 + 2 * d2 * d2 * dmx;
+          double denom = d2 * (dmx * dmx + 2 * d2 * d2 + dmx * d2 + epsq);
+          double term = numer / (denom + TINY);
+          lim[k] = term < lim[k] ? term : lim[k];
+        } else if (d2 < 0.0) {
+          double numer = (dmn * dmn + epsq) * d2 + 2 * d2 * d2 * dmn;
+          double denom = d2 * (dmn * dmn + 2 * d2 * d2 + dmn * d2 + epsq);
+          double term = numer / (denom + TINY);
+          lim[k] = term < lim[k] ? term : lim[k];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) g[3 * k + d] = T(lim[k] * double(g[3 * k + d]));
+  }
+#pragma unroll
+  for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
+}
+
+// WENO-XYZ blend (modflu/RFLU_ModWENO.F90:1437-1508) followed by the positivity clip
+// of rho and p (modflu/RFLU_ModLimiters.F90:1199-1267).
+template <typename GT>
+__global__ void k_weno_clip(Mesh M, State S, const GT* __restrict__ gin, GT* __restrict__ gout,
+                            const Scalars* __restrict__ sc) {
+  if (sc->abort) return;
+  const int N = M.N;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  typedef GT T;
+  const T eps = T(1.0E-15);
+  int s0 = M.sten_ptr[i], s1 = M.sten_ptr[i + 1];
+  T g[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) {
+    T term = gin[(size_t)k * N + i];
+    T sumw = T(1) / (term * term + eps);
+    T loc = sumw * term;
+    for (int s = s0; s < s1; ++s) {
+      T t = gin[(size_t)k * N + M.sten_idx[s]];
+      T w = T(1) / (t * t + eps);
+      sumw = sumw + w;
+      loc = loc + w * t;
+    }
+    g[k] = loc / sumw;
+  }
+  double4 ci = M.cell[i];
+  const int pos[2] = {0, 4};
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    int iv = pos[q];
+    double varc = S.prim[(size_t)iv * N + i];
+    int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
+    for (int e = e0; e < e1; ++e) {
+      int sgn = M.c2f_idx[e];
+      int f = (sgn > 0 ? sgn : -sgn) - 1;
+      double4 xf = f < M.F ? M.fc[f] : M.bfc[f - M.F];
+      double dx = xf.x - ci.x, dy = xf.y - ci.y, dz = xf.z - ci.z;
+      double varf = varc + double(g[3 * iv]) * dx + double(g[3 * iv + 1]) * dy +
+                    double(g[3 * iv + 2]) * dz;
+      if (varf <= 0.0) {
+        g[3 * iv] = T(0);
+        g[3 * iv + 1] = T(0);
+        g[3 * iv + 2] = T(0);
+        break;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 15; ++k) gout[(size_t)k * N + i] = g[k];
+}
+
+// ---------------------------------------------------------------------------------
+// Time step: per-cell spectral-radius sums gathered over the cell-to-face list in
+// the reference's order (rocflu/RFLU_TimeStepInviscid.F90:165-304), block minimum,
+// then one block applies RFLU_MinimumTimeStep's rules (:126-245).
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+__global__ void k_local_dt(Mesh M, State S, double* __restrict__ part, const Scalars* __restrict__ sc) {
+  __shared__ double red[16];
+  const int N = M.N;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double dtc = HUGE_VAL;
+  if (i < N && !sc->abort) {
+    double sum = 0.0;
+    int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
+    for (int e = e0; e < e1; ++e) {
+      int sgn = M.c2f_idx[e];
+      int f = (sgn > 0 ? sgn : -sgn) - 1;
+      if (f < M.F) {
+        int2 cc = M.f2c[f];
+        int c1 = cc.x, c2 = cc.y;
+        double4 n = M.fn[f];
+        double ir1 = 1.0 / S.cv[c1], ir2 = 1.0 / S.cv[c2];
+        double u1 = S.cv[(size_t)N + c1] * ir1, v1 = S.cv[2 * (size_t)N + c1] * ir1,
+               w1 = S.cv[3 * (size_t)N + c1] * ir1, a1 = S.dv[2 * (size_t)N + c1];
+        double u2 = S.cv[(size_t)N + c2] * ir2, v2 = S.cv[2 * (size_t)N + c2] * ir2,
+               w2 = S.cv[3 * (size_t)N + c2] * ir2, a2 = S.dv[2 * (size_t)N + c2];
+        double vc = 0.5 * ((u1 + u2) * n.x + (v1 + v2) * n.y + (w1 + w2) * n.z);
+        double as = 0.5 * (a1 + a2);
+        double srad = (fabs(vc) + as) * n.w;
+        sum = sum + srad;
+      } else {
+        double4 n = M.bfn[f - M.F];
+        double ir1 = 1.0 / S.cv[i];
+        double u1 = S.cv[(size_t)N + i] * ir1, v1 = S.cv[2 * (size_t)N + i] * ir1,
+               w1 = S.cv[3 * (size_t)N + i] * ir1, a1 = S.dv[2 * (size_t)N + i];
+        double vc = u1 * n.x + v1 * n.y + w1 * n.z;
+        sum = sum + (fabs(vc) + a1) * n.w;
+      }
+    }
+    dtc = M.cell[i].w / sum;
+  }
+  dtc = wave_min(dtc);
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = dtc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmin(m, red[w]);
+    part[blockIdx.x] = m;
+  }
+}
+
+// time_in >= 0 sets global%currentTime from the host first (the unfused entry point).
+__global__ void k_dt_finalize(const double* __restrict__ part, int nparts, Scalars* sc,
+                              double cfl, double dt_imposed, double dt_min_limit, double time_in) {
+  __shared__ double red[16];
+  double m = HUGE_VAL;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) m = fmin(m, part[k]);
+  m = wave_min(m);
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mm = fmin(mm, red[w]);
+    sc->dt_local_min = mm;
+    if (time_in >= 0.0) sc->time = time_in;
+    double wI = 0.0, wC = 0.0;
+    double d = cfl * mm;
+    if (sc->time <= 1.0E-08) wI = 1.0;
+    else wC = 1.0;
+    d = wI / dt_imposed + wC / d;
+    d = 1.0 / d;
+    double dtMin = HUGE_VAL;
+    if (d < dtMin) dtMin = d;
+    if (dtMin <= 0.0) dtMin = 6.0E-12;
+    dtMin = dtMin > dt_min_limit ? dtMin : dt_min_limit;
+    if (!sc->abort) sc->dt = dtMin;
+  }
+}
+
+// Outflow mass bookkeeping of stage 1 (rocflu/RFLU_CentralFirstPatch.F90:697-700):
+// one block sums the outflow faces' mass fluxes in face order per thread stride.
+template <typename FT>
+__global__ void k_mass_inout(Mesh M, PatchTable P, const FT* __restrict__ flux, Scalars* sc) {
+  __shared__ double sin_[256], sout[256];
+  const size_t FB = (size_t)M.F + M.B;
+  double a = 0.0, b = 0.0;
+  if (!sc->abort) {
+    for (int k = threadIdx.x; k < M.B; k += blockDim.x) {
+      int p = M.bpatch[k];
+      if (P.bc[p] != BC_OUTFLOW) continue;
+      double m = double(flux[M.F + k]);
+      a = a - fmin(m, 0.0);
+      b = b + fmax(m, 0.0);
+    }
+  }
+  sin_[threadIdx.x] = a;
+  sout[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sin_[threadIdx.x] += sin_[threadIdx.x + s];
+      sout[threadIdx.x] += sout[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  (void)FB;
+  if (threadIdx.x == 0 && !sc->abort) {
+    sc->mass_in = sc->mass_in + sin_[0];
+    sc->mass_out = sc->mass_out + sout[0];
+  }
+}
+
+__global__ void k_fill_scalars(Scalars* sc, double time) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sc->time = time;
+    sc->dt = 0.0;
+    sc->dt_local_min = 0.0;
+    sc->mass_in = 0.0;
+    sc->mass_out = 0.0;
+    sc->n_invalid = 0;
+    sc->n_negative = 0;
+    sc->n_bounds_fix = 0;
+    sc->n_pos_fix = 0;
+    sc->steps = 0;
+    sc->abort = 0;
+  }
+}
+
+__global__ void k_reset_counters(Scalars* sc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sc->mass_in = 0.0;
+    sc->mass_out = 0.0;
+    sc->n_invalid = 0;
+    sc->n_negative = 0;
+    sc->n_bounds_fix = 0;
+    sc->n_pos_fix = 0;
+    sc->abort = 0;
+  }
+}
+
+}  // namespace rflu

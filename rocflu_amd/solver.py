@@ -1,0 +1,248 @@
+"""Host-side mirror of the reference's region and stage loop over the C-ABI.
+
+``GpuRegion`` owns one region's device copy (the t_region the reference keeps per
+MPI rank) and exposes RungeKuttaMP's callees one by one (``rk_init``,
+``cell_gradients``, ``convective_fluxes``, ``rk_update``, ``set_dependent_vars``,
+``after_update``; libfloflu/RungeKuttaMP.F90:148-508) as well as the fused time loop
+(``time_steps``; rocflu/RFLU_TimeStepping.F90:353-640).  Errors come back as
+``RfluError`` with the reference's error names (ERR_INVALID_VALUE,
+ERR_NEGATIVE_POSDEF).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as nat
+from .mesh import Mesh
+
+DISCR_ROE, DISCR_HLLC, DISCR_AUSMPLUS = 1, 3, 4
+RECONST_NONE, RECONST_WENO_XYZ, RECONST_LIM_VENKAT = 0, 2, 11
+RK4_CLASSICAL, RK3_WRAY, RK1_EULER, RK3_SSP = 1, 2, 3, 4
+PREC_FP64, PREC_MIXED, PREC_FP32_GRAD, PREC_FP32_FLUX = 0, 1, 2, 3
+
+
+def n_stages(rk: int) -> int:
+    return {1: 4, 2: 3, 3: 1, 4: 3}[rk]
+
+
+@dataclass
+class Numerics:
+    """The t_mixt_input / global fields of the hot path (defaults from
+    libflu/RFLU_InitInputValues.F90:182-215 and RFLU_InitGlobal.F90:351-352)."""
+    discr: int = DISCR_HLLC
+    order: int = 1
+    reconst: int = RECONST_NONE
+    rk: int = RK3_WRAY
+    precision: int = PREC_FP64
+    cfl: float = 0.5
+    dt_imposed: float = 1.0e-9
+    dt_min_limit: float = 0.0
+    eps_entr: float = 0.05
+    diss_fact: float = 1.0
+    gamma: float = 1.4
+    cp: float = 1004.5
+    patch_order: Optional[Sequence[int]] = None
+
+    def gas_prime(self):
+        """(M, R', gamma') of the dependent-variable path (libflu/RFLU_SetGasVars.F90:172-178)."""
+        gc0 = self.cp - self.cp / self.gamma
+        m = 8314.3 / gc0
+        rg = 8314.3 / m
+        return m, rg, self.cp / (self.cp - rg)
+
+
+def region_arrays(mesh: Mesh, num: Numerics) -> dict:
+    """Contiguous arrays of one region, kept alive while a descriptor points at them."""
+    a = mesh.a
+    keep = {
+        "f2c": np.ascontiguousarray(a["f2c"], dtype=np.int32).reshape(-1),
+        "fn": np.ascontiguousarray(a["fn"], dtype=np.float64).reshape(-1),
+        "fc": np.ascontiguousarray(a["fc"], dtype=np.float64).reshape(-1),
+        "cofg": np.ascontiguousarray(a["cofg"], dtype=np.float64).reshape(-1),
+        "vol": np.ascontiguousarray(a["vol"], dtype=np.float64).reshape(-1),
+        "bpatch_ptr": np.ascontiguousarray(a["bpatch_ptr"], dtype=np.int32),
+        "bf2c": np.ascontiguousarray(a["bf2c"], dtype=np.int32).reshape(-1),
+        "bfn": np.ascontiguousarray(a["bfn"], dtype=np.float64).reshape(-1),
+        "bfc": np.ascontiguousarray(a["bfc"], dtype=np.float64).reshape(-1),
+        "patch_bc": np.ascontiguousarray(a["patch_bc"], dtype=np.int32),
+        "patch_opt": np.ascontiguousarray(a["patch_opt"], dtype=np.int32),
+        "patch_pout": np.ascontiguousarray(a["patch_pout"], dtype=np.float64),
+        "c2f_ptr": np.ascontiguousarray(a["c2f_ptr"], dtype=np.int32),
+        "c2f_idx": np.ascontiguousarray(a["c2f_idx"], dtype=np.int32),
+    }
+    po = num.patch_order if num.patch_order is not None else [1] * int(mesh.n_patches)
+    keep["patch_order"] = np.ascontiguousarray(po, dtype=np.int32)
+    if num.order == 2:
+        if not mesh.has_stencils:
+            raise ValueError("second-order runs need a mesh built with stencils=True")
+        keep["c2cs_ptr"] = np.ascontiguousarray(a["c2cs_ptr"], dtype=np.int32)
+        keep["c2cs_idx"] = np.ascontiguousarray(a["c2cs_idx"], dtype=np.int32)
+        keep["xyz_moms"] = np.ascontiguousarray(a["xyz_moms"], dtype=np.float64).reshape(-1)
+    return keep
+
+
+class GpuRegion:
+    """One region on the GPU behind the C-ABI."""
+
+    def __init__(self, mesh: Mesh, num: Numerics):
+        self.mesh = mesh
+        self.num = num
+        self.n = int(mesh.n_cells)
+        L = nat.lib()
+        k = region_arrays(mesh, num)
+        i32, f64 = ctypes.c_int32, ctypes.c_double
+        d = nat.RegionDesc()
+        d.index_base = 0
+        d.n_cells = self.n
+        d.n_faces = int(mesh.n_faces)
+        d.n_patches = int(mesh.n_patches)
+        d.f2c = nat.ptr(k["f2c"], i32)
+        d.fn = nat.ptr(k["fn"], f64)
+        d.fc = nat.ptr(k["fc"], f64)
+        d.cofg = nat.ptr(k["cofg"], f64)
+        d.vol = nat.ptr(k["vol"], f64)
+        d.bpatch_ptr = nat.ptr(k["bpatch_ptr"], i32)
+        d.bf2c = nat.ptr(k["bf2c"], i32)
+        d.bfn = nat.ptr(k["bfn"], f64)
+        d.bfc = nat.ptr(k["bfc"], f64)
+        d.patch_bc = nat.ptr(k["patch_bc"], i32)
+        d.patch_opt = nat.ptr(k["patch_opt"], i32)
+        d.patch_pout = nat.ptr(k["patch_pout"], f64)
+        d.patch_order = nat.ptr(k["patch_order"], i32)
+        if num.order == 2:
+            d.c2cs_ptr = nat.ptr(k["c2cs_ptr"], i32)
+            d.c2cs_idx = nat.ptr(k["c2cs_idx"], i32)
+            d.xyz_moms = nat.ptr(k["xyz_moms"], f64)
+        d.gamma, d.cp = num.gamma, num.cp
+        d.discr, d.order, d.reconst, d.rk, d.precision = (
+            num.discr, num.order, num.reconst, num.rk, num.precision)
+        d.cfl, d.dt_imposed, d.dt_min_limit = num.cfl, num.dt_imposed, num.dt_min_limit
+        d.eps_entr, d.diss_fact = num.eps_entr, num.diss_fact
+        h = ctypes.c_int32(0)
+        nat.check(L.rflu_gpu_create_region(ctypes.byref(d), ctypes.byref(h)),
+                  "rflu_gpu_create_region")
+        self.handle = h.value
+        self._lib = L
+
+    # -- state ---------------------------------------------------------------
+    def upload(self, cv: np.ndarray, dv: Optional[np.ndarray] = None, time: float = 0.0):
+        cv = np.ascontiguousarray(cv, dtype=np.float64).reshape(self.n, 5)
+        dvp = None
+        if dv is not None:
+            dv = np.ascontiguousarray(dv, dtype=np.float64).reshape(self.n, 3)
+            dvp = nat.ptr(dv, ctypes.c_double)
+        nat.check(self._lib.rflu_gpu_upload_state(self.handle, nat.ptr(cv, ctypes.c_double),
+                                                  dvp, float(time)), "rflu_gpu_upload_state")
+        self.time = float(time)
+
+    def download(self):
+        cv = np.empty((self.n, 5))
+        dv = np.empty((self.n, 3))
+        nat.check(self._lib.rflu_gpu_download_state(self.handle, nat.ptr(cv, ctypes.c_double),
+                                                    nat.ptr(dv, ctypes.c_double)),
+                  "rflu_gpu_download_state")
+        return cv, dv
+
+    # -- RungeKuttaMP's callees (libfloflu/RungeKuttaMP.F90:216-437) -----------
+    def time_step(self, current_time: Optional[float] = None) -> float:
+        """dtMin at global%currentTime (the host's time unless given)."""
+        t = self.time if current_time is None else float(current_time)
+        dt = ctypes.c_double()
+        nat.check(self._lib.rflu_gpu_time_step(self.handle, t, ctypes.byref(dt)),
+                  "rflu_gpu_time_step")
+        return dt.value
+
+    def rk_init(self, stage: int):
+        nat.check(self._lib.rflu_gpu_rk_init(self.handle, stage), "rflu_gpu_rk_init")
+
+    def cell_gradients(self):
+        nat.check(self._lib.rflu_gpu_cell_gradients(self.handle), "rflu_gpu_cell_gradients")
+
+    def convective_fluxes(self, stage: int):
+        nat.check(self._lib.rflu_gpu_convective_fluxes(self.handle, stage),
+                  "rflu_gpu_convective_fluxes")
+
+    def rk_update(self, stage: int):
+        nat.check(self._lib.rflu_gpu_rk_update(self.handle, stage), "rflu_gpu_rk_update")
+
+    def set_dependent_vars(self):
+        nat.check(self._lib.rflu_gpu_set_dependent_vars(self.handle),
+                  "rflu_gpu_set_dependent_vars")
+
+    def after_update(self, stage: int):
+        nat.check(self._lib.rflu_gpu_after_update(self.handle, stage), "rflu_gpu_after_update")
+
+    def runge_kutta(self):
+        """One time step through the separate entry points: RFLU_TimeStepping's dt, the
+        stages in RungeKuttaMP's order, and the host-side time advance
+        (rocflu/RFLU_TimeStepping.F90:429-640)."""
+        dt = self.time_step()
+        for s in range(1, n_stages(self.num.rk) + 1):
+            self.rk_init(s)
+            self.cell_gradients()
+            self.convective_fluxes(s)
+            self.rk_update(s)
+            self.set_dependent_vars()
+            self.after_update(s)
+        self.time = self.time + dt
+        return dt
+
+    # -- fused paths ---------------------------------------------------------------
+    def time_steps(self, nsteps: int, graph: bool = False):
+        nat.check(self._lib.rflu_gpu_time_steps(self.handle, int(nsteps), 1 if graph else 0),
+                  "rflu_gpu_time_steps")
+        self.time = self.stats()["time"]
+
+    def residual(self) -> np.ndarray:
+        rhs = np.empty((self.n, 5))
+        nat.check(self._lib.rflu_gpu_residual(self.handle, nat.ptr(rhs, ctypes.c_double)),
+                  "rflu_gpu_residual")
+        return rhs
+
+    def stats(self) -> dict:
+        s = nat.GpuStats()
+        nat.check(self._lib.rflu_gpu_get_stats(self.handle, ctypes.byref(s)), "rflu_gpu_get_stats")
+        return {name: getattr(s, name) for name, _ in nat.GpuStats._fields_}
+
+    def reset_stats(self):
+        nat.check(self._lib.rflu_gpu_reset_stats(self.handle), "rflu_gpu_reset_stats")
+
+    def synchronize(self):
+        nat.check(self._lib.rflu_gpu_synchronize(self.handle), "rflu_gpu_synchronize")
+
+    def set_launch(self, face_block=0, cell_block=0, grad_block=0):
+        nat.check(self._lib.rflu_gpu_set_launch(self.handle, face_block, cell_block, grad_block),
+                  "rflu_gpu_set_launch")
+
+    def time_kernel(self, which: int, reps: int = 10) -> float:
+        ms = ctypes.c_double()
+        nat.check(self._lib.rflu_gpu_time_kernel(self.handle, which, reps, ctypes.byref(ms)),
+                  "rflu_gpu_time_kernel")
+        return ms.value
+
+    def set_profiling(self, on: bool):
+        nat.check(self._lib.rflu_gpu_set_profiling(self.handle, 1 if on else 0),
+                  "rflu_gpu_set_profiling")
+
+    def kernel_times(self):
+        ms = (ctypes.c_double * 6)()
+        cnt = (ctypes.c_int64 * 6)()
+        nat.check(self._lib.rflu_gpu_get_kernel_times(self.handle, ms, cnt, 6),
+                  "rflu_gpu_get_kernel_times")
+        names = ["face_flux", "update", "gradient", "blend", "time_step", "bface_flux"]
+        return {names[k]: (ms[k], cnt[k]) for k in range(6)}
+
+    def close(self):
+        if getattr(self, "handle", 0):
+            nat.check(self._lib.rflu_gpu_destroy_region(self.handle), "rflu_gpu_destroy_region")
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,182 @@
+"""GPU parity of the HIP path against the CPU oracle on the same seeded inputs.
+
+Bar: FP64 runs agree with the oracle to 1e-12 relative L2 per conserved variable
+(the FP64 tolerance of the build's target); mixed precision is compared with the
+FP64 GPU run.  All GPU calls go through the C-ABI (librocflu_amd.so).
+"""
+import numpy as np
+import pytest
+
+from rocflu_amd import cases, mesh as rmesh
+from rocflu_amd.solver import GpuRegion, Numerics
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-12
+
+
+def rel_l2(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    out = []
+    for k in range(a.shape[1]):
+        den = np.linalg.norm(b[:, k])
+        num = np.linalg.norm(a[:, k] - b[:, k])
+        out.append(num / den if den > 0 else num)
+    return np.array(out)
+
+
+def run_both(m, num, cv0, nsteps, mode="fused", graph=False):
+    print(f"case: cells={m.n_cells} discr={num.discr} order={num.order} reconst={num.reconst} "
+          f"rk={num.rk} prec={num.precision} steps={nsteps} mode={mode}", flush=True)
+    o = Oracle(m, num)
+    cvo, dvo, st = o.run(cv0, nsteps)
+    assert st["rc"] == 0
+    g = GpuRegion(m, num)
+    try:
+        g.upload(cv0)
+        if mode == "fused":
+            g.time_steps(nsteps, graph=graph)
+        else:
+            for _ in range(nsteps):
+                g.runge_kutta()
+        cvg, dvg = g.download()
+        gst = g.stats()
+    finally:
+        g.close()
+    return cvo, dvo, st, cvg, dvg, gst
+
+
+SOD_NUM = dict(order=1, rk=2, cfl=0.5, dt_imposed=1e-3, dt_min_limit=0.0)
+
+
+@pytest.mark.parametrize("discr", [1, 3, 4])
+def test_sod_first_order_100_steps(gpu, discr):
+    """Config 1: 64x16x16 jittered hex box, Sod along x, slip walls + outflow ends."""
+    m = cases.sod_box()
+    num = Numerics(discr=discr, **SOD_NUM)
+    cv0 = cases.sod(m, num, 32.0)
+    cvo, dvo, st, cvg, dvg, gst = run_both(m, num, cv0, 100)
+    err = rel_l2(cvg, cvo)
+    print("rel L2 per variable:", err, "max abs:", np.abs(cvg - cvo).max(), flush=True)
+    assert np.all(err <= TOL_FP64), err
+    assert gst["steps"] == 100
+    assert abs(gst["time"] - st["time"]) <= 1e-12 * st["time"]
+
+
+def test_sod_entry_points_match_fused(gpu):
+    """The per-callee entry points (RungeKuttaMP order) give the fused loop's state."""
+    m = cases.sod_box(nx=32, ny=8, nz=8, subsonic=True)
+    num = Numerics(discr=3, **SOD_NUM)
+    cv0 = cases.sod(m, num, 16.0)
+    cvo, _, _, cvg, _, _ = run_both(m, num, cv0, 20, mode="steps")
+    g = GpuRegion(m, num)
+    g.upload(cv0)
+    g.time_steps(20)
+    cvf, _ = g.download()
+    g.close()
+    assert np.array_equal(cvg, cvf)
+    assert np.all(rel_l2(cvg, cvo) <= TOL_FP64)
+
+
+def test_graph_replay_matches_eager(gpu):
+    m = cases.sod_box(nx=32, ny=8, nz=8)
+    num = Numerics(discr=1, **SOD_NUM)
+    cv0 = cases.sod(m, num, 16.0)
+    res = []
+    for graph in (False, True):
+        g = GpuRegion(m, num)
+        g.upload(cv0)
+        for _ in range(3):  # several replays with host work between them
+            g.time_steps(5, graph=graph)
+            _ = g.stats()
+        res.append(g.download()[0])
+        g.close()
+    assert np.array_equal(res[0], res[1])
+
+
+@pytest.mark.parametrize("rk", [1, 3, 4])
+def test_other_integrators(gpu, rk):
+    m = cases.sod_box(nx=32, ny=8, nz=8, subsonic=True)
+    num = Numerics(discr=3, order=1, rk=rk, cfl=0.5 if rk != 3 else 0.2, dt_imposed=1e-3)
+    cv0 = cases.sod(m, num, 16.0)
+    cvo, _, _, cvg, _, _ = run_both(m, num, cv0, 30)
+    assert np.all(rel_l2(cvg, cvo) <= TOL_FP64)
+
+
+def small_tet(n=8, bc=rmesh.BC_SLIPWALL, seed=5):
+    return cases.tet_box(n=n, jitter=0.08, seed=seed, bc=bc)
+
+
+@pytest.mark.parametrize("discr,reconst", [(3, 11), (4, 11), (1, 11), (3, 2), (3, 0)])
+def test_second_order_tets(gpu, discr, reconst):
+    """Config 2's scheme on a reduced 8^3 x 6 Kuhn mesh (3,072 tets), Mach-2 shock."""
+    m = small_tet()
+    num = Numerics(discr=discr, order=2, reconst=reconst, rk=2, cfl=0.5, dt_imposed=1e-3)
+    cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
+    cvo, _, _, cvg, _, _ = run_both(m, num, cv0, 20)
+    err = rel_l2(cvg, cvo)
+    print("rel L2 per variable:", err, flush=True)
+    assert np.all(err <= TOL_FP64), err
+
+
+def test_second_order_hex_weno_outflow(gpu):
+    """WENO-XYZ + clip on hexes with subsonic outflow ends, first- and second-order patches."""
+    m = cases.sod_box(nx=24, ny=6, nz=6, subsonic=True, stencils=True)
+    for po in ([1] * 6, [2, 2, 1, 1, 1, 1]):
+        num = Numerics(discr=3, order=2, reconst=2, rk=2, cfl=0.4, dt_imposed=1e-3,
+                       patch_order=po)
+        cv0 = cases.sod(m, num, 12.0)
+        cvo, _, _, cvg, _, _ = run_both(m, num, cv0, 25)
+        err = rel_l2(cvg, cvo)
+        print("patch order", po, "rel L2:", err, flush=True)
+        assert np.all(err <= TOL_FP64), err
+
+
+def test_residual_matches_oracle(gpu):
+    m = small_tet(n=6)
+    for order, reconst in ((1, 0), (2, 11), (2, 2)):
+        num = Numerics(discr=3, order=order, reconst=reconst, rk=2)
+        cv0 = cases.planar_shock(m, num, 0.5 * m.dims[0], mach=2.0, noise=0.01, seed=3)
+        o = Oracle(m, num)
+        ro = o.residual(cv0)
+        g = GpuRegion(m, num)
+        g.upload(cv0)
+        rg = g.residual()
+        g.close()
+        scale = np.abs(ro).max(axis=0)
+        assert np.all(np.abs(rg - ro).max(axis=0) <= 1e-13 * scale), (order, reconst)
+
+
+def test_time_step_matches_oracle(gpu):
+    m = cases.sod_box(nx=16, ny=8, nz=8)
+    num = Numerics(discr=3, order=1, rk=2, cfl=0.5, dt_imposed=1e-3)
+    cv0 = cases.sod(m, num, 8.0)
+    o = Oracle(m, num)
+    g = GpuRegion(m, num)
+    # first step: imposed dt (time <= 1e-8); later: CFL dt
+    for t in (0.0, 1.0):
+        g.upload(cv0)
+        dtg = g.time_step(t)
+        dto = o.min_timestep(o.local_dt(cv0), t)
+        assert dtg == dto, (t, dtg, dto)
+    g.close()
+
+
+def test_mixed_precision_error_vs_fp64(gpu):
+    """FP32 gradients and fluxes, FP64 accumulation/state: per-variable error vs FP64."""
+    m = small_tet(n=10)
+    res = {}
+    for prec in (0, 1):
+        num = Numerics(discr=3, order=2, reconst=11, rk=2, cfl=0.5, dt_imposed=1e-3,
+                       precision=prec)
+        cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
+        g = GpuRegion(m, num)
+        g.upload(cv0)
+        g.time_steps(20)
+        res[prec] = g.download()[0]
+        g.close()
+    err = rel_l2(res[1], res[0])
+    print("mixed vs FP64 rel L2 per variable:", err, flush=True)
+    assert np.all(err <= 1e-5), err
