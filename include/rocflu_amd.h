@@ -103,6 +103,8 @@ typedef struct {
   double mass_in, mass_out; /* outflow mass bookkeeping of stage 1 (global%massIn/Out) */
   double time, dt;        /* global%currentTime and the last global%dtMin           */
   int64_t steps;          /* completed time steps                                   */
+  int32_t finished;       /* the fused loop reached its end time                    */
+  int32_t pad;
 } rflu_gpu_stats;
 
 /* Number of visible GPUs (0 when none); never fails. */
@@ -131,8 +133,9 @@ int rflu_gpu_cell_gradients(int32_t handle);
 /* ConvectiveFluxesMP (libfloflu/ConvectiveFluxes.F90:133-159): fluxes of all faces. */
 int rflu_gpu_convective_fluxes(int32_t handle, int32_t istage);
 /* RKUpdateMP (libfloflu/RkUpdateGeneric.F90:255-325) with the residual gathered per
- * cell over the cell-to-face list in the reference's summation order. */
-int rflu_gpu_rk_update(int32_t handle, int32_t istage);
+ * cell over the cell-to-face list in the reference's summation order; dt_min is the
+ * host's global%dtMin (after its end-time clamp). */
+int rflu_gpu_rk_update(int32_t handle, int32_t istage, double dt_min);
 /* RFLU_SetVarsContWrapper over all cells (libflu/RFLU_SetDependentVars.F90:255-284). */
 int rflu_gpu_set_dependent_vars(int32_t handle);
 /* AfterUpdateMP (libfloflu/AfterUpdateMP.F90:134-136); waits and returns
@@ -149,6 +152,10 @@ int rflu_gpu_residual(int32_t handle, double* rhs);
  * graph and replayed.  Stops advancing (state frozen) at the first failed check and
  * returns its code; waits. */
 int rflu_gpu_time_steps(int32_t handle, int64_t nsteps, int32_t use_graph);
+
+/* global%maxTime of rflu_gpu_time_steps: the step that would pass it is shortened to
+ * end at it and later steps do nothing (rocflu/RFLU_TimeStepping.F90:445-448). */
+int rflu_gpu_set_end_time(int32_t handle, double end_time);
 
 int rflu_gpu_get_stats(int32_t handle, rflu_gpu_stats* stats);
 int rflu_gpu_reset_stats(int32_t handle);

@@ -62,6 +62,9 @@ def lib():
         L.orc_run.argtypes = [R, _f64p, _f64p, ctypes.c_int64, ctypes.POINTER(OrcStats)]
         L.orc_run.restype = ctypes.c_int
         L.orc_residual.argtypes = [R, _f64p, _f64p, _f64p]
+        L.orc_run_until.argtypes = [R, _f64p, _f64p, ctypes.c_double, ctypes.c_int64,
+                                    ctypes.POINTER(OrcStats)]
+        L.orc_run_until.restype = ctypes.c_int
         L.orc_set_dependent_vars.argtypes = [R, _f64p, _f64p, ctypes.c_int, ctypes.c_int]
         L.orc_gradients.argtypes = [R, _f64p, _f64p]
         L.orc_venkat.argtypes = [R, _f64p, _f64p]
@@ -126,6 +129,18 @@ class Oracle:
         st.time = time
         rc = self.L.orc_run(ctypes.byref(self.r), _p(cv, ctypes.c_double),
                             _p(dv, ctypes.c_double), int(nsteps), ctypes.byref(st))
+        stats = {name: getattr(st, name) for name, _ in OrcStats._fields_}
+        stats["rc"] = rc
+        return cv, dv, stats
+
+    def run_until(self, cv, end_time, max_steps=10**9, time=0.0, dv=None):
+        cv = np.array(cv, dtype=np.float64, copy=True, order="C")
+        dv = self.dv(cv) if dv is None else np.array(dv, dtype=np.float64, copy=True, order="C")
+        st = OrcStats()
+        st.time = time
+        rc = self.L.orc_run_until(ctypes.byref(self.r), _p(cv, ctypes.c_double),
+                                  _p(dv, ctypes.c_double), float(end_time), int(max_steps),
+                                  ctypes.byref(st))
         stats = {name: getattr(st, name) for name, _ in OrcStats._fields_}
         stats["rc"] = rc
         return cv, dv, stats

@@ -843,6 +843,36 @@ int orc_run(const orc_region* R, double* cv, double* dv, int64_t nsteps, orc_sta
   return rc;
 }
 
+/* RFLU_TimeStepping's unsteady loop up to MAXTIME (rocflu/RFLU_TimeStepping.F90:
+ * 429-448,628-640): dt, the end-time clamp dt = endTime - currentTime, the stages, and
+ * the time advance. */
+int orc_run_until(const orc_region* R, double* cv, double* dv, double end_time,
+                  int64_t max_steps, orc_stats* st) {
+  size_t N = (size_t)R->n_cells;
+  int ns = orc_nstages(R->rk);
+  if (ns == 0 || (R->order != 1 && R->order != 2)) return ORC_ERR_ARG;
+  double* cvOld = (double*)malloc(5 * N * sizeof(double));
+  double* rhs = (double*)malloc(5 * N * sizeof(double));
+  double* rhsSum = (double*)calloc(5 * N, sizeof(double));
+  double* grad = (double*)calloc(15 * N, sizeof(double));
+  int rc = ORC_OK, finished = 0;
+  while (!finished && st->steps < max_steps && rc == ORC_OK) {
+    double dtl = orc_local_dt(R, cv, dv);
+    double dt = orc_min_timestep(R, dtl, st->time);
+    if (st->time + dt > end_time) {
+      dt = end_time - st->time;
+      finished = 1;
+    }
+    for (int s = 1; s <= ns && rc == ORC_OK; ++s)
+      rc = orc_stage(R, s, dt, cv, dv, cvOld, rhs, rhsSum, grad, st);
+    st->time = st->time + dt;
+    st->dt_last = dt;
+    st->steps++;
+  }
+  free(cvOld); free(rhs); free(rhsSum); free(grad);
+  return rc;
+}
+
 void orc_residual(const orc_region* R, const double* cv_in, const double* dv, double* rhs) {
   size_t N = (size_t)R->n_cells;
   double* cv = (double*)malloc(5 * N * sizeof(double));

@@ -77,6 +77,9 @@ double orc_min_timestep(const orc_region* r, double dt_local_min, double current
 int orc_nstages(int rk);
 /* Full time loop: nsteps of (dt, stages).  cv (5,N) and dv (3,N) are updated in place. */
 int orc_run(const orc_region* r, double* cv, double* dv, int64_t nsteps, orc_stats* st);
+/* Time loop to end_time with RFLU_TimeStepping's end-time clamp (at most max_steps). */
+int orc_run_until(const orc_region* r, double* cv, double* dv, double end_time,
+                  int64_t max_steps, orc_stats* st);
 /* One residual evaluation R(cv) for the current state (gradients included for order 2). */
 void orc_residual(const orc_region* r, const double* cv, const double* dv, double* rhs);
 

@@ -53,6 +53,7 @@ class GpuStats(ctypes.Structure):
         ("n_bounds_fix", ctypes.c_int64), ("n_pos_fix", ctypes.c_int64),
         ("mass_in", ctypes.c_double), ("mass_out", ctypes.c_double),
         ("time", ctypes.c_double), ("dt", ctypes.c_double), ("steps", ctypes.c_int64),
+        ("finished", ctypes.c_int32), ("pad", ctypes.c_int32),
     ]
 
 
@@ -67,7 +68,8 @@ SIGNATURES = {
     "rflu_gpu_rk_init": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32]),
     "rflu_gpu_cell_gradients": (ctypes.c_int, [ctypes.c_int32]),
     "rflu_gpu_convective_fluxes": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32]),
-    "rflu_gpu_rk_update": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32]),
+    "rflu_gpu_rk_update": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double]),
+    "rflu_gpu_set_end_time": (ctypes.c_int, [ctypes.c_int32, ctypes.c_double]),
     "rflu_gpu_set_dependent_vars": (ctypes.c_int, [ctypes.c_int32]),
     "rflu_gpu_after_update": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32]),
     "rflu_gpu_residual": (ctypes.c_int, [ctypes.c_int32, _f64p]),

@@ -4,6 +4,7 @@
 #include "rocflu_amd.h"
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -35,6 +36,7 @@ struct Region {
   int N = 0, F = 0, B = 0, P = 0;
   int order = 1, discr = 3, reconst = 0, rk = 2, prec = 0;
   double cfl = 0.5, dt_imposed = 0, dt_min_limit = 0;
+  double end_time = HUGE_VAL;  // global%maxTime of the fused loop
   Gas gas{};
   PatchTable patches{};
   bool has_outflow = false;
@@ -50,6 +52,13 @@ struct Region {
   double* rhs = nullptr;
   double* dt_part = nullptr;
   Scalars* sc = nullptr;
+  // internal cell numbering: device cell i holds the caller's cell new2old[i]
+  std::vector<int> new2old;
+  // stencil tiles staged in LDS by the gradient kernel
+  int *tile_uptr = nullptr, *tile_uidx = nullptr, *tile_lptr = nullptr;
+  unsigned short* tile_loc = nullptr;
+  int n_tiles = 0, tile_cap = 0;
+  bool use_lds = false;
   int n_dt_parts = 0;
   int face_block = 256, cell_block = 256, grad_block = 256;
   hipGraphExec_t graph = nullptr;
@@ -109,7 +118,7 @@ void free_region(Region* R) {
   void* bufs[] = {R->f2c, R->fn, R->fc, R->cell, R->bfn, R->bfc, R->bf2c, R->bpatch, R->c2f_ptr,
                   R->c2f_idx, R->sten_ptr, R->sten_idx, R->moms, R->epsq, R->cv, R->cv_old,
                   R->rhs_sum, R->dv, R->prim, R->grad, R->grad2, R->flux, R->rhs, R->dt_part,
-                  R->sc};
+                  R->sc, R->tile_uptr, R->tile_uidx, R->tile_lptr, R->tile_loc};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (R->stream) (void)hipStreamDestroy(R->stream);
@@ -117,6 +126,165 @@ void free_region(Region* R) {
 }
 
 inline unsigned nblk(int n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ---- renumbering -------------------------------------------------------------------
+
+inline uint64_t spread3(uint64_t x) {
+  x &= 0x1fffffULL;
+  x = (x | (x << 32)) & 0x1f00000000ffffULL;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffULL;
+  x = (x | (x << 8)) & 0x100f00f00f00f00fULL;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ULL;
+  x = (x | (x << 2)) & 0x1249249249249249ULL;
+  return x;
+}
+
+// Cells sorted along a Morton curve of their centroids (ties keep the caller's order),
+// so that consecutive cells -- one wavefront, one LDS tile -- are compact in space.
+std::vector<int> morton_order(const std::vector<double4>& cell) {
+  const int N = (int)cell.size();
+  double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (const double4& c : cell) {
+    lo[0] = std::min(lo[0], c.x); hi[0] = std::max(hi[0], c.x);
+    lo[1] = std::min(lo[1], c.y); hi[1] = std::max(hi[1], c.y);
+    lo[2] = std::min(lo[2], c.z); hi[2] = std::max(hi[2], c.z);
+  }
+  std::vector<std::pair<uint64_t, int>> key(N);
+  for (int i = 0; i < N; ++i) {
+    const double p[3] = {cell[i].x, cell[i].y, cell[i].z};
+    uint64_t q[3];
+    for (int d = 0; d < 3; ++d) {
+      double ext = hi[d] - lo[d];
+      double s = ext > 0 ? (p[d] - lo[d]) / ext : 0.0;
+      q[d] = (uint64_t)(s * 2097151.0);
+    }
+    key[i] = {spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2), i};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int> n2o(N);
+  for (int i = 0; i < N; ++i) n2o[i] = key[i].second;
+  return n2o;
+}
+
+void renumber(Region* R, std::vector<double4>& hcell, std::vector<int2>& hf2c,
+              std::vector<double4>& hfn, std::vector<double4>& hfc, std::vector<int>& hbf2c,
+              std::vector<int>& hc2f_ptr, std::vector<int>& hc2f_idx, std::vector<int>& hsten_ptr,
+              std::vector<int>& hsten_idx, std::vector<double>& hmoms, std::vector<double>& hepsq) {
+  const int N = R->N, F = R->F;
+  std::vector<int> n2o = morton_order(hcell), o2n(N);
+  for (int i = 0; i < N; ++i) o2n[n2o[i]] = i;
+  {
+    std::vector<double4> c2(N);
+    for (int i = 0; i < N; ++i) c2[i] = hcell[n2o[i]];
+    hcell.swap(c2);
+  }
+  // faces ordered by their lower new cell index
+  std::vector<int> fn2o(F), fo2n(F);
+  {
+    std::vector<std::pair<int64_t, int>> key(F);
+    for (int f = 0; f < F; ++f) {
+      int a = o2n[hf2c[f].x], b = o2n[hf2c[f].y];
+      key[f] = {((int64_t)std::min(a, b) << 32) | (int64_t)std::max(a, b), f};
+    }
+    std::sort(key.begin(), key.end());
+    for (int k = 0; k < F; ++k) { fn2o[k] = key[k].second; fo2n[key[k].second] = k; }
+    std::vector<int2> f2(F);
+    std::vector<double4> n2(F), c2(F);
+    for (int k = 0; k < F; ++k) {
+      int2 o = hf2c[fn2o[k]];
+      f2[k] = make_int2(o2n[o.x], o2n[o.y]);
+      n2[k] = hfn[fn2o[k]];
+      c2[k] = hfc[fn2o[k]];
+    }
+    hf2c.swap(f2); hfn.swap(n2); hfc.swap(c2);
+  }
+  for (int& c : hbf2c) c = o2n[c];
+  {
+    std::vector<int> p2(N + 1, 0), i2(hc2f_idx.size());
+    for (int i = 0; i < N; ++i) p2[i + 1] = p2[i] + (hc2f_ptr[n2o[i] + 1] - hc2f_ptr[n2o[i]]);
+    for (int i = 0; i < N; ++i) {
+      int o = n2o[i];
+      for (int e = hc2f_ptr[o], k = p2[i]; e < hc2f_ptr[o + 1]; ++e, ++k) {
+        int s = hc2f_idx[e];
+        if (s > F) i2[k] = s;                          // boundary face: unchanged
+        else if (s > 0) i2[k] = fo2n[s - 1] + 1;
+        else i2[k] = -(fo2n[-s - 1] + 1);
+      }
+    }
+    hc2f_ptr.swap(p2); hc2f_idx.swap(i2);
+  }
+  if (R->order == 2) {
+    std::vector<int> p2(N + 1, 0), i2(hsten_idx.size());
+    for (int i = 0; i < N; ++i) p2[i + 1] = p2[i] + (hsten_ptr[n2o[i] + 1] - hsten_ptr[n2o[i]]);
+    for (int i = 0; i < N; ++i) {
+      int o = n2o[i];
+      for (int e = hsten_ptr[o], k = p2[i]; e < hsten_ptr[o + 1]; ++e, ++k) i2[k] = o2n[hsten_idx[e]];
+    }
+    hsten_ptr.swap(p2); hsten_idx.swap(i2);
+    std::vector<double> m2(hmoms.size()), e2(N);
+    for (int k = 0; k < 6; ++k)
+      for (int i = 0; i < N; ++i) m2[(size_t)k * N + i] = hmoms[(size_t)k * N + n2o[i]];
+    for (int i = 0; i < N; ++i) e2[i] = hepsq[n2o[i]];
+    hmoms.swap(m2); hepsq.swap(e2);
+  }
+  R->new2old.swap(n2o);
+}
+
+// LDS tiles of the gradient kernel: tile t covers cells [t*TS, t*TS+TS); its union of
+// stencil members is staged in LDS, and loc[lptr[t] + s*TS + lane] is the LDS slot of
+// the lane's s-th stencil member (member order unchanged).
+constexpr int GRAD_TILE = 128;
+constexpr int GRAD_TILE_CAP = 1024;   // records of 8 doubles: 64 KiB of LDS
+
+void build_tiles(Region* R, const std::vector<int>& sp, const std::vector<int>& si,
+                 std::vector<int>& uptr, std::vector<int>& uidx, std::vector<int>& lptr,
+                 std::vector<unsigned short>& loc) {
+  const int N = R->N, TS = GRAD_TILE;
+  const int nt = (N + TS - 1) / TS;
+  uptr.assign(nt + 1, 0);
+  lptr.assign(nt + 1, 0);
+  uidx.clear();
+  loc.clear();
+  int cap = 0;
+  std::vector<int> u;
+  for (int t = 0; t < nt; ++t) {
+    int c0 = t * TS, c1 = std::min(N, c0 + TS);
+    u.clear();
+    int smax = 0;
+    for (int i = c0; i < c1; ++i) {
+      smax = std::max(smax, sp[i + 1] - sp[i]);
+      for (int e = sp[i]; e < sp[i + 1]; ++e) u.push_back(si[e]);
+    }
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    if ((int)u.size() > GRAD_TILE_CAP) {   // does not fit: gradient kernel reads global memory
+      R->use_lds = false;
+      uptr.clear(); uidx.clear(); lptr.clear(); loc.clear();
+      return;
+    }
+    cap = std::max(cap, (int)u.size());
+    uidx.insert(uidx.end(), u.begin(), u.end());
+    uptr[t + 1] = (int)uidx.size();
+    size_t base = loc.size();
+    loc.resize(base + (size_t)smax * TS, (unsigned short)0);
+    for (int i = c0; i < c1; ++i) {
+      int lane = i - c0;
+      for (int e = sp[i], s = 0; e < sp[i + 1]; ++e, ++s) {
+        int k = (int)(std::lower_bound(u.begin(), u.end(), si[e]) - u.begin());
+        loc[base + (size_t)s * TS + lane] = (unsigned short)k;
+      }
+    }
+    if (loc.size() > (size_t)0x7fffffff) {
+      R->use_lds = false;
+      uptr.clear(); uidx.clear(); lptr.clear(); loc.clear();
+      return;
+    }
+    lptr[t + 1] = (int)loc.size();
+  }
+  R->n_tiles = nt;
+  R->tile_cap = cap;
+  R->use_lds = true;
+}
 
 // ---- kernel dispatch ------------------------------------------------------------
 
@@ -164,6 +332,17 @@ template <typename GT>
 int launch_gradients_t(Region* R) {
   Mesh M = R->mesh();
   State S = R->state();
+  if (R->use_lds) {
+    Tiles T;
+    T.uptr = R->tile_uptr; T.uidx = R->tile_uidx; T.lptr = R->tile_lptr; T.loc = R->tile_loc;
+    T.ts = GRAD_TILE; T.cap = R->tile_cap;
+    size_t shm = (size_t)8 * R->tile_cap * sizeof(double);
+    if (R->reconst == RFLU_RECONST_LIM_VENKAT)
+      hipLaunchKernelGGL((k_gradient_lds<GT, true>), dim3(R->n_tiles), dim3(GRAD_TILE), shm, R->stream, M, S, T, (GT*)R->grad, R->sc);
+    else
+      hipLaunchKernelGGL((k_gradient_lds<GT, false>), dim3(R->n_tiles), dim3(GRAD_TILE), shm, R->stream, M, S, T, (GT*)R->grad, R->sc);
+    return 0;
+  }
   dim3 gb(R->grad_block);
   dim3 gg(nblk(R->N, R->grad_block));
   if (R->reconst == RFLU_RECONST_LIM_VENKAT)
@@ -184,16 +363,16 @@ int launch_blend_t(Region* R) {
 }
 
 template <typename FT, bool FUSED, bool PRIM>
-int launch_update_rk(Region* R, int stage, int last) {
+int launch_update_rk(Region* R, int stage, int last, double dt_in) {
   Mesh M = R->mesh();
   State S = R->state();
   dim3 cb(R->cell_block), cg(nblk(R->N, R->cell_block));
   const FT* fl = (const FT*)R->flux;
   switch (R->rk) {
-    case 1: hipLaunchKernelGGL((k_update<1, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last); break;
-    case 2: hipLaunchKernelGGL((k_update<2, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last); break;
-    case 3: hipLaunchKernelGGL((k_update<3, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last); break;
-    default: hipLaunchKernelGGL((k_update<4, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last); break;
+    case 1: hipLaunchKernelGGL((k_update<1, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last, dt_in); break;
+    case 2: hipLaunchKernelGGL((k_update<2, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last, dt_in); break;
+    case 3: hipLaunchKernelGGL((k_update<3, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last, dt_in); break;
+    default: hipLaunchKernelGGL((k_update<4, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last, dt_in); break;
   }
   return 0;
 }
@@ -285,34 +464,35 @@ int launch_fluxes(Region* R, int stage) {
   return 0;
 }
 
-int launch_update(Region* R, int stage, bool fused) {
+int launch_update(Region* R, int stage, bool fused, double dt_in = -1.0) {
   Timed t(R, KID_UPDATE);
   int last = stage == (R->rk == 1 ? 4 : R->rk == 3 ? 1 : 3);
   bool prim = R->order == 2;
   if (!flux32(R)) {
-    if (fused) return prim ? launch_update_rk<double, true, true>(R, stage, last)
-                           : launch_update_rk<double, true, false>(R, stage, last);
-    return launch_update_rk<double, false, false>(R, stage, last);
+    if (fused) return prim ? launch_update_rk<double, true, true>(R, stage, last, dt_in)
+                           : launch_update_rk<double, true, false>(R, stage, last, dt_in);
+    return launch_update_rk<double, false, false>(R, stage, last, dt_in);
   }
-  if (fused) return prim ? launch_update_rk<float, true, true>(R, stage, last)
-                         : launch_update_rk<float, true, false>(R, stage, last);
-  return launch_update_rk<float, false, false>(R, stage, last);
+  if (fused) return prim ? launch_update_rk<float, true, true>(R, stage, last, dt_in)
+                         : launch_update_rk<float, true, false>(R, stage, last, dt_in);
+  return launch_update_rk<float, false, false>(R, stage, last, dt_in);
 }
 
-int launch_time_step(Region* R, double time_in = -1.0) {
+int launch_time_step(Region* R, double time_in = -1.0, double end_time = HUGE_VAL) {
   Timed t(R, KID_DT);
   Mesh M = R->mesh();
   State S = R->state();
   hipLaunchKernelGGL(k_local_dt, dim3(R->n_dt_parts), dim3(256), 0, R->stream, M, S, R->dt_part, R->sc);
   hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, R->stream, (const double*)R->dt_part,
-                     R->n_dt_parts, R->sc, R->cfl, R->dt_imposed, R->dt_min_limit, time_in);
+                     R->n_dt_parts, R->sc, R->cfl, R->dt_imposed, R->dt_min_limit, time_in,
+                     end_time);
   return 0;
 }
 
 int nstages(int rk) { return rk == 1 ? 4 : rk == 2 ? 3 : rk == 3 ? 1 : 3; }
 
 int launch_step(Region* R) {
-  int rc = launch_time_step(R);
+  int rc = launch_time_step(R, -1.0, R->end_time);
   if (rc) return rc;
   for (int s = 1; s <= nstages(R->rk); ++s) {
     if ((rc = launch_gradients(R))) return rc;
@@ -469,6 +649,15 @@ int rflu_gpu_create_region(const rflu_region_desc* d, int32_t* handle) {
     }
   }
 
+  // ---- internal renumbering (Morton order of cell centroids) ----
+  // Storage order only: every per-cell face list keeps the reference's summation
+  // order (original face order) and every stencil keeps its member order, so the
+  // arithmetic is the same as in the original numbering.
+  renumber(R, hcell, hf2c, hfn, hfc, hbf2c, hc2f_ptr, hc2f_idx, hsten_ptr, hsten_idx, hmoms, hepsq);
+  std::vector<int> htile_uptr, htile_uidx, htile_lptr;
+  std::vector<unsigned short> htile_loc;
+  if (R->order == 2) build_tiles(R, hsten_ptr, hsten_idx, htile_uptr, htile_uidx, htile_lptr, htile_loc);
+
   // ---- device allocation ----
   int rc = 0;
   if (hipStreamCreate(&R->stream) != hipSuccess) { free_region(R); return fail(RFLU_ERR_HIP, "hipStreamCreate"); }
@@ -523,6 +712,15 @@ int rflu_gpu_create_region(const rflu_region_desc* d, int32_t* handle) {
       free_region(R);
       return rc;
     }
+    if (R->use_lds) {
+      if ((rc = dalloc(&R->tile_uptr, htile_uptr.size())) || (rc = dalloc(&R->tile_uidx, htile_uidx.size())) ||
+          (rc = dalloc(&R->tile_lptr, htile_lptr.size())) || (rc = dalloc(&R->tile_loc, htile_loc.size())) ||
+          (rc = dput(R->tile_uptr, htile_uptr, R->stream)) || (rc = dput(R->tile_uidx, htile_uidx, R->stream)) ||
+          (rc = dput(R->tile_lptr, htile_lptr, R->stream)) || (rc = dput(R->tile_loc, htile_loc, R->stream))) {
+        free_region(R);
+        return rc;
+      }
+    }
   }
   hipLaunchKernelGGL(k_fill_scalars, dim3(1), dim3(64), 0, R->stream, R->sc, 0.0);
   if (hipStreamSynchronize(R->stream) != hipSuccess) {
@@ -553,12 +751,12 @@ int rflu_gpu_upload_state(int32_t h, const double* cv, const double* dv, double 
   const int N = R->N;
   std::vector<double> hc((size_t)5 * N);
   for (int c = 0; c < N; ++c)
-    for (int k = 0; k < 5; ++k) hc[(size_t)k * N + c] = cv[5 * (size_t)c + k];
+    for (int k = 0; k < 5; ++k) hc[(size_t)k * N + c] = cv[5 * (size_t)R->new2old[c] + k];
   HIP_TRY(hipMemcpyAsync(R->cv, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, R->stream));
   if (dv) {
     std::vector<double> hd((size_t)3 * N);
     for (int c = 0; c < N; ++c)
-      for (int k = 0; k < 3; ++k) hd[(size_t)k * N + c] = dv[3 * (size_t)c + k];
+      for (int k = 0; k < 3; ++k) hd[(size_t)k * N + c] = dv[3 * (size_t)R->new2old[c] + k];
     HIP_TRY(hipMemcpyAsync(R->dv, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice, R->stream));
     HIP_TRY(hipStreamSynchronize(R->stream));
   } else {
@@ -586,10 +784,10 @@ int rflu_gpu_download_state(int32_t h, double* cv, double* dv) {
   HIP_TRY(hipStreamSynchronize(R->stream));
   if (cv)
     for (int c = 0; c < N; ++c)
-      for (int k = 0; k < 5; ++k) cv[5 * (size_t)c + k] = hc[(size_t)k * N + c];
+      for (int k = 0; k < 5; ++k) cv[5 * (size_t)R->new2old[c] + k] = hc[(size_t)k * N + c];
   if (dv)
     for (int c = 0; c < N; ++c)
-      for (int k = 0; k < 3; ++k) dv[3 * (size_t)c + k] = hd[(size_t)k * N + c];
+      for (int k = 0; k < 3; ++k) dv[3 * (size_t)R->new2old[c] + k] = hd[(size_t)k * N + c];
   return RFLU_OK;
 }
 
@@ -637,11 +835,12 @@ int rflu_gpu_convective_fluxes(int32_t h, int32_t istage) {
   return RFLU_OK;
 }
 
-int rflu_gpu_rk_update(int32_t h, int32_t istage) {
+int rflu_gpu_rk_update(int32_t h, int32_t istage, double dt_min) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
   if (istage < 1 || istage > nstages(R->rk)) return fail(RFLU_ERR_ARG, "bad stage");
-  int rc = launch_update(R, istage, false);
+  if (!(dt_min > 0.0)) return fail(RFLU_ERR_ARG, "dt_min must be > 0");
+  int rc = launch_update(R, istage, false, dt_min);
   if (rc) return rc;
   HIP_TRY(hipGetLastError());
   return RFLU_OK;
@@ -693,7 +892,7 @@ int rflu_gpu_residual(int32_t h, double* rhs) {
   HIP_TRY(hipMemcpyAsync(hr.data(), R->rhs, hr.size() * sizeof(double), hipMemcpyDeviceToHost, R->stream));
   HIP_TRY(hipStreamSynchronize(R->stream));
   for (int c = 0; c < N; ++c)
-    for (int k = 0; k < 5; ++k) rhs[5 * (size_t)c + k] = hr[(size_t)k * N + c];
+    for (int k = 0; k < 5; ++k) rhs[5 * (size_t)R->new2old[c] + k] = hr[(size_t)k * N + c];
   return RFLU_OK;
 }
 
@@ -728,6 +927,15 @@ int rflu_gpu_time_steps(int32_t h, int64_t nsteps, int32_t use_graph) {
   return status_from(sc);
 }
 
+int rflu_gpu_set_end_time(int32_t h, double end_time) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!(end_time > 0.0)) return fail(RFLU_ERR_ARG, "end_time must be > 0");
+  R->end_time = end_time;
+  R->graph_dirty = true;
+  return RFLU_OK;
+}
+
 int rflu_gpu_get_stats(int32_t h, rflu_gpu_stats* st) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
@@ -744,6 +952,7 @@ int rflu_gpu_get_stats(int32_t h, rflu_gpu_stats* st) {
   st->time = s.time;
   st->dt = s.dt;
   st->steps = (int64_t)s.steps;
+  st->finished = s.finished;
   return RFLU_OK;
 }
 
@@ -790,7 +999,7 @@ int rflu_gpu_time_kernel(int32_t h, int32_t which, int32_t reps, double* ms) {
   for (int r = 0; r < reps && !rc; ++r) {
     switch (which) {
       case 0: rc = launch_fluxes(R, 2); break;
-      case 1: rc = launch_update(R, 1, false); break;
+      case 1: rc = launch_update(R, 1, false, 1.0e-30); break;
       case 2: rc = launch_gradients(R); break;
       case 3: rc = grad32(R) ? launch_blend_t<float>(R) : launch_blend_t<double>(R); break;
       case 4: rc = launch_time_step(R); break;

@@ -47,9 +47,13 @@ struct Scalars {
   double mass_in, mass_out;
   unsigned long long n_invalid, n_negative, n_bounds_fix, n_pos_fix;
   unsigned long long steps;
-  int abort;
+  int abort;      // a failed check: every later launch of the region returns at once
+  int last_step;  // the end-time clamp shortened this step (RFLU_TimeStepping.F90:445-448)
+  int finished;   // set by the next step's time-step kernel after the last step
   int pad;
 };
+
+__device__ __forceinline__ bool halted(const Scalars* sc) { return sc->abort || sc->finished; }
 
 struct Mesh {
   int N, F, B;
@@ -329,7 +333,7 @@ __device__ __forceinline__ void interior_face_flux(const Mesh& M, const State& S
 template <int DISCR, int ORDER, typename T, typename GT, typename FT>
 __global__ void k_face_flux(Mesh M, State S, Gas G, const GT* __restrict__ grad,
                             FT* __restrict__ flux, const Scalars* __restrict__ sc) {
-  if (sc->abort) return;
+  if (halted(sc)) return;
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= M.F) return;
   T flx[5];
@@ -376,7 +380,7 @@ __device__ __forceinline__ void bcond_outflow(int bcOpt, double pout, double sxn
 template <int ORDER, typename GT, typename FT>
 __global__ void k_bface_flux(Mesh M, State S, Gas G, PatchTable P, const GT* __restrict__ grad,
                              FT* __restrict__ flux, const Scalars* __restrict__ sc) {
-  if (sc->abort) return;
+  if (halted(sc)) return;
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= M.B) return;
   const int N = M.N;
@@ -604,11 +608,11 @@ __device__ __forceinline__ void rk_update_cell(const State& S, int N, int i, int
 // need no separate cvOld copy.
 template <int RK, typename FT, bool FUSED, bool WRITE_PRIM>
 __global__ void k_update(Mesh M, State S, Gas G, const FT* __restrict__ flux, Scalars* sc,
-                         int stage, int last_stage) {
-  if (sc->abort) return;
+                         int stage, int last_stage, double dt_in) {
+  if (halted(sc)) return;
   const int N = M.N;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double dt = sc->dt;
+  double dt = dt_in > 0.0 ? dt_in : sc->dt;
   if (last_stage && FUSED && i == 0) {
     sc->time = sc->time + dt;
     sc->steps = sc->steps + 1;
@@ -706,103 +710,192 @@ __global__ void k_gather_rhs(Mesh M, const FT* __restrict__ flux, double* __rest
 // same pass by the Venkatakrishnan limiter (modflu/RFLU_ModLimiters.F90:407-522,
 // 1100-1106), which needs only this cell's gradient and its face neighbours' values.
 // ---------------------------------------------------------------------------------
-template <typename GT, bool VENKAT>
-__global__ void k_gradient(Mesh M, State S, GT* __restrict__ grad, const Scalars* __restrict__ sc) {
-  if (sc->abort) return;
+template <typename T>
+struct GradCoef {
+  T c12, c13, c23, c1111, c2222, c3333;
+};
+
+template <typename T>
+__device__ __forceinline__ GradCoef<T> grad_coef(const Mesh& M, int i) {
   const int N = M.N;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  typedef GT T;
   double r11 = M.moms[i], r12 = M.moms[(size_t)N + i], r22 = M.moms[2 * (size_t)N + i];
   double r13 = M.moms[3 * (size_t)N + i], r23 = M.moms[4 * (size_t)N + i],
          r33 = M.moms[5 * (size_t)N + i];
-  double c11d = 1.0 / r11, c22d = 1.0 / r22, c33d = 1.0 / r33;
-  double c12d = -c11d * r12;
-  double c13d = -(c11d * r13 + c12d * c22d * r23);
-  double c23d = -c22d * r23;
-  T c12 = T(c12d), c13 = T(c13d), c23 = T(c23d);
-  T c1111 = T(c11d * c11d), c2222 = T(c22d * c22d), c3333 = T(c33d * c33d);
+  double c11 = 1.0 / r11, c22 = 1.0 / r22, c33 = 1.0 / r33;
+  double c12 = -c11 * r12;
+  double c13 = -(c11 * r13 + c12 * c22 * r23);
+  double c23 = -c22 * r23;
+  GradCoef<T> c;
+  c.c12 = T(c12); c.c13 = T(c13); c.c23 = T(c23);
+  c.c1111 = T(c11 * c11); c.c2222 = T(c22 * c22); c.c3333 = T(c33 * c33);
+  return c;
+}
+
+// One stencil member's contribution (xj - xi, var_j - var_i).
+template <typename T>
+__device__ __forceinline__ void grad_member(const GradCoef<T>& c, double ddx, double ddy,
+                                            double ddz, const double* dvar, T* g) {
+  T dx = T(ddx), dy = T(ddy), dz = T(ddz);
+  T term = T(1) / tsqrt(dx * dx + dy * dy + dz * dz);
+  dx = term * dx;
+  dy = term * dy;
+  dz = term * dz;
+  T term1 = c.c1111 * (dx);
+  T term2 = c.c2222 * (dy + c.c12 * dx);
+  T term3 = c.c3333 * (dz + c.c23 * dy + c.c13 * dx);
+  T wtx = term * (term1 + c.c12 * term2 + c.c13 * term3);
+  T wty = term * (term2 + c.c23 * term3);
+  T wtz = term * (term3);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    T dVar = T(dvar[k]);
+    g[3 * k + 0] = g[3 * k + 0] + wtx * dVar;
+    g[3 * k + 1] = g[3 * k + 1] + wty * dVar;
+    g[3 * k + 2] = g[3 * k + 2] + wtz * dVar;
+  }
+}
+
+// Venkatakrishnan limiter of cell i in FP64: varMin/varMax over interior-face
+// neighbours, then the face-wise limiter with dx = cofg - fc (the reference's sign).
+template <typename T>
+__device__ __forceinline__ void venkat_limit(const Mesh& M, const State& S, int i, double4 ci,
+                                             const double* vi, T* g) {
+  const int N = M.N;
+  const double TINY = 1.0E-12;
+  double vmn[5], vmx[5], lim[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) { vmn[k] = vi[k]; vmx[k] = vi[k]; lim[k] = 1.0; }
+  int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
+  for (int e = e0; e < e1; ++e) {
+    int sgn = M.c2f_idx[e];
+    int f = (sgn > 0 ? sgn : -sgn) - 1;
+    if (f >= M.F) continue;
+    int2 cc = M.f2c[f];
+    int o = sgn > 0 ? cc.y : cc.x;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      double vo = S.prim[(size_t)k * N + o];
+      vmn[k] = vmn[k] < vo ? vmn[k] : vo;
+      vmx[k] = vmx[k] > vo ? vmx[k] : vo;
+    }
+  }
+  double epsq = M.epsq[i];
+  for (int e = e0; e < e1; ++e) {
+    int sgn = M.c2f_idx[e];
+    int f = (sgn > 0 ? sgn : -sgn) - 1;
+    if (f >= M.F) continue;
+    double4 xf = M.fc[f];
+    double dx = ci.x - xf.x, dy = ci.y - xf.y, dz = ci.z - xf.z;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      double dmn = vmn[k] - vi[k];
+      double dmx = vmx[k] - vi[k];
+      double d2 = double(g[3 * k]) * dx + double(g[3 * k + 1]) * dy + double(g[3 * k + 2]) * dz;
+      if (d2 > 0.0) {
+        double numer = (dmx * dmx + epsq) * d2 + 2 * d2 * d2 * dmx;
+        double denom = d2 * (dmx * dmx + 2 * d2 * d2 + dmx * d2 + epsq);
+        double term = numer / (denom + TINY);
+        lim[k] = term < lim[k] ? term : lim[k];
+      } else if (d2 < 0.0) {
+        double numer = (dmn * dmn + epsq) * d2 + 2 * d2 * d2 * dmn;
+        double denom = d2 * (dmn * dmn + 2 * d2 * d2 + dmn * d2 + epsq);
+        double term = numer / (denom + TINY);
+        lim[k] = term < lim[k] ? term : lim[k];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g[3 * k + d] = T(lim[k] * double(g[3 * k + d]));
+}
+
+// One thread per cell, members read from global memory (L2/MALL).
+template <typename GT, bool VENKAT>
+__global__ void k_gradient(Mesh M, State S, GT* __restrict__ grad, const Scalars* __restrict__ sc) {
+  if (halted(sc)) return;
+  const int N = M.N;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  GradCoef<GT> c = grad_coef<GT>(M, i);
   double4 ci = M.cell[i];
   double vi[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) vi[k] = S.prim[(size_t)k * N + i];
-  T g[15];
+  GT g[15];
 #pragma unroll
-  for (int k = 0; k < 15; ++k) g[k] = T(0);
+  for (int k = 0; k < 15; ++k) g[k] = GT(0);
   int s0 = M.sten_ptr[i], s1 = M.sten_ptr[i + 1];
   for (int s = s0; s < s1; ++s) {
     int j = M.sten_idx[s];
     double4 cj = M.cell[j];
-    T dx = T(cj.x - ci.x), dy = T(cj.y - ci.y), dz = T(cj.z - ci.z);
-    T term = T(1) / tsqrt(dx * dx + dy * dy + dz * dz);
-    dx = term * dx;
-    dy = term * dy;
-    dz = term * dz;
-    T term1 = c1111 * (dx);
-    T term2 = c2222 * (dy + c12 * dx);
-    T term3 = c3333 * (dz + c23 * dy + c13 * dx);
-    T wtx = term * (term1 + c12 * term2 + c13 * term3);
-    T wty = term * (term2 + c23 * term3);
-    T wtz = term * (term3);
+    double dvar[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      T dVar = T(S.prim[(size_t)k * N + j] - vi[k]);
-      g[3 * k + 0] = g[3 * k + 0] + wtx * dVar;
-      g[3 * k + 1] = g[3 * k + 1] + wty * dVar;
-      g[3 * k + 2] = g[3 * k + 2] + wtz * dVar;
-    }
+    for (int k = 0; k < 5; ++k) dvar[k] = S.prim[(size_t)k * N + j] - vi[k];
+    grad_member<GT>(c, cj.x - ci.x, cj.y - ci.y, cj.z - ci.z, dvar, g);
   }
-  if (VENKAT) {
-    // Limiter in FP64: varMin/varMax over interior-face neighbours, then the
-    // face-wise limiter with dx = cofg - fc (the reference's sign).
-    const double TINY = 1.0E-12;
-    double vmn[5], vmx[5], lim[5];
+  if (VENKAT) venkat_limit<GT>(M, S, i, ci, vi, g);
 #pragma unroll
-    for (int k = 0; k < 5; ++k) { vmn[k] = vi[k]; vmx[k] = vi[k]; lim[k] = 1.0; }
-    int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
-    for (int e = e0; e < e1; ++e) {
-      int sgn = M.c2f_idx[e];
-      int f = (sgn > 0 ? sgn : -sgn) - 1;
-      if (f >= M.F) continue;
-      int2 cc = M.f2c[f];
-      int o = sgn > 0 ? cc.y : cc.x;
+  for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
+}
+
+// Stencil tiles: the union of the stencil members of TS consecutive cells is staged
+// in LDS once per tile (centroid and primitive state, 8 doubles per member), and
+// each lane then walks its own stencil, in the reference's member order, reading
+// LDS slots through a coalesced [member][lane] table of 16-bit slot numbers.
+struct Tiles {
+  const int* uptr;
+  const int* uidx;
+  const int* lptr;
+  const unsigned short* loc;
+  int ts;
+  int cap;
+};
+
+template <typename GT, bool VENKAT>
+__global__ __launch_bounds__(128) void k_gradient_lds(Mesh M, State S, Tiles T,
+                                                      GT* __restrict__ grad,
+                                                      const Scalars* __restrict__ sc) {
+  extern __shared__ double lds[];
+  if (halted(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
+  const int N = M.N;
+  const int cap = T.cap;
+  double* lx = lds;
+  double* ly = lds + cap;
+  double* lz = lds + 2 * cap;
+  double* lv = lds + 3 * cap;  // 5 arrays: rho, u, v, w, p
+  const int t = blockIdx.x;
+  const int u0 = T.uptr[t], nu = T.uptr[t + 1] - u0;
+  for (int k = threadIdx.x; k < nu; k += blockDim.x) {
+    int j = T.uidx[u0 + k];
+    double4 cj = M.cell[j];
+    lx[k] = cj.x;
+    ly[k] = cj.y;
+    lz[k] = cj.z;
 #pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        double vo = S.prim[(size_t)k * N + o];
-        vmn[k] = vmn[k] < vo ? vmn[k] : vo;
-        vmx[k] = vmx[k] > vo ? vmx[k] : vo;
-      }
-    }
-    double epsq = M.epsq[i];
-    for (int e = e0; e < e1; ++e) {
-      int sgn = M.c2f_idx[e];
-      int f = (sgn > 0 ? sgn : -sgn) - 1;
-      if (f >= M.F) continue;
-      double4 xf = M.fc[f];
-      double dx = ci.x - xf.x, dy = ci.y - xf.y, dz = ci.z - xf.z;
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        double dmn = vmn[k] - vi[k];
-        double dmx = vmx[k] - vi[k];
-        double d2 = double(g[3 * k]) * dx + double(g[3 * k + 1]) * dy + double(g[3 * k + 2]) * dz;
-        if (d2 > 0.0) {
-          double numer = (dmx * dmx + epsq) * d2 + 2 * d2 * d2 * dmx;
-          double denom = d2 * (dmx * dmx + 2 * d2 * d2 + dmx * d2 + epsq);
-          double term = numer / (denom + TINY);
-          lim[k] = term < lim[k] ? term : lim[k];
-        } else if (d2 < 0.0) {
-          double numer = (dmn * dmn + epsq) * d2 + 2 * d2 * d2 * dmn;
-          double denom = d2 * (dmn * dmn + 2 * d2 * d2 + dmn * d2 + epsq);
-          double term = numer / (denom + TINY);
-          lim[k] = term < lim[k] ? term : lim[k];
-        }
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) g[3 * k + d] = T(lim[k] * double(g[3 * k + d]));
+    for (int q = 0; q < 5; ++q) lv[q * cap + k] = S.prim[(size_t)q * N + j];
   }
+  __syncthreads();
+  const int i = t * T.ts + (int)threadIdx.x;
+  if (i >= N) return;
+  GradCoef<GT> c = grad_coef<GT>(M, i);
+  double4 ci = M.cell[i];
+  double vi[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) vi[k] = S.prim[(size_t)k * N + i];
+  GT g[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) g[k] = GT(0);
+  const int cnt = M.sten_ptr[i + 1] - M.sten_ptr[i];
+  const unsigned short* L = T.loc + T.lptr[t] + threadIdx.x;
+  for (int s = 0; s < cnt; ++s) {
+    int k = L[(size_t)s * T.ts];
+    double dvar[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) dvar[q] = lv[q * cap + k] - vi[q];
+    grad_member<GT>(c, lx[k] - ci.x, ly[k] - ci.y, lz[k] - ci.z, dvar, g);
+  }
+  if (VENKAT) venkat_limit<GT>(M, S, i, ci, vi, g);
 #pragma unroll
   for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
 }
@@ -812,7 +905,7 @@ __global__ void k_gradient(Mesh M, State S, GT* __restrict__ grad, const Scalars
 template <typename GT>
 __global__ void k_weno_clip(Mesh M, State S, const GT* __restrict__ gin, GT* __restrict__ gout,
                             const Scalars* __restrict__ sc) {
-  if (sc->abort) return;
+  if (halted(sc)) return;
   const int N = M.N;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -875,7 +968,7 @@ __global__ void k_local_dt(Mesh M, State S, double* __restrict__ part, const Sca
   const int N = M.N;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double dtc = HUGE_VAL;
-  if (i < N && !sc->abort) {
+  if (i < N && !halted(sc)) {
     double sum = 0.0;
     int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
     for (int e = e0; e < e1; ++e) {
@@ -918,7 +1011,8 @@ __global__ void k_local_dt(Mesh M, State S, double* __restrict__ part, const Sca
 
 // time_in >= 0 sets global%currentTime from the host first (the unfused entry point).
 __global__ void k_dt_finalize(const double* __restrict__ part, int nparts, Scalars* sc,
-                              double cfl, double dt_imposed, double dt_min_limit, double time_in) {
+                              double cfl, double dt_imposed, double dt_min_limit, double time_in,
+                              double end_time) {
   __shared__ double red[16];
   double m = HUGE_VAL;
   for (int k = threadIdx.x; k < nparts; k += blockDim.x) m = fmin(m, part[k]);
@@ -941,7 +1035,14 @@ __global__ void k_dt_finalize(const double* __restrict__ part, int nparts, Scala
     if (d < dtMin) dtMin = d;
     if (dtMin <= 0.0) dtMin = 6.0E-12;
     dtMin = dtMin > dt_min_limit ? dtMin : dt_min_limit;
-    if (!sc->abort) sc->dt = dtMin;
+    if (sc->last_step) sc->finished = 1;
+    if (!halted(sc)) {
+      if (sc->time + dtMin > end_time) {
+        dtMin = end_time - sc->time;
+        sc->last_step = 1;
+      }
+      sc->dt = dtMin;
+    }
   }
 }
 
@@ -991,6 +1092,8 @@ __global__ void k_fill_scalars(Scalars* sc, double time) {
     sc->n_pos_fix = 0;
     sc->steps = 0;
     sc->abort = 0;
+    sc->last_step = 0;
+    sc->finished = 0;
   }
 }
 
@@ -1003,6 +1106,8 @@ __global__ void k_reset_counters(Scalars* sc) {
     sc->n_bounds_fix = 0;
     sc->n_pos_fix = 0;
     sc->abort = 0;
+    sc->last_step = 0;
+    sc->finished = 0;
   }
 }
 

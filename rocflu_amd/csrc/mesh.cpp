@@ -176,6 +176,53 @@ bool stencil_singular(const rflu_mesh& m, int icg, const std::vector<int32_t>& m
   return false;
 }
 
+// Sorts a ascending carrying b along, by the iterative partition-exchange sort of
+// QuickSortRFREALInteger (modfloflu/ModSortSearch.F90): middle-element pivot, the
+// larger partition pushed on an explicit stack, the smaller one processed next.
+// Equal keys end in the same relative order as in the reference.
+void rank_sort(std::vector<double>& a, std::vector<int32_t>& b) {
+  const int n = (int)a.size();
+  if (n < 2) return;
+  int stl[64], str[64];
+  int s = 0;
+  stl[0] = 1;
+  str[0] = n;
+  // 1-based indices as in the reference: element k is a[k-1]
+  auto A = [&](int k) -> double& { return a[k - 1]; };
+  auto Bv = [&](int k) -> int32_t& { return b[k - 1]; };
+  s = 1;
+  while (s != 0) {
+    int l = stl[s - 1], r = str[s - 1];
+    --s;
+    for (;;) {
+      int i = l, j = r;
+      int k = (l + r) / 2;
+      double x = A(k);
+      for (;;) {
+        while (A(i) < x) ++i;
+        while (x < A(j)) --j;
+        if (i <= j) {
+          std::swap(A(i), A(j));
+          std::swap(Bv(i), Bv(j));
+          ++i;
+          --j;
+          if (i > j) break;
+        } else {
+          break;
+        }
+      }
+      if (j - l >= r - i) {
+        if (l < j) { stl[s] = l; str[s] = j; ++s; }
+        l = i;
+      } else {
+        if (i < r) { stl[s] = i; str[s] = r; ++s; }
+        r = j;
+      }
+      if (!(l < r)) break;
+    }
+  }
+}
+
 void build_stencils(rflu_mesh& m) {
   const int N = m.n_cells, nv = m.nvpc;
   // vertex -> cells, ascending cell order
@@ -189,10 +236,11 @@ void build_stencils(rflu_mesh& m) {
     for (int c = 0; c < N; ++c)
       for (int k = 0; k < nv; ++k) v2c[fill[m.c2v[nv * c + k]]++] = c;
   }
-  // Stencil size rules (modflu/RFLU_ModStencilsCells.F90:2816-2909,
-  // libflu/RFLU_ComputeStencilSize.F90:118-122): min = 3*(1+1)(1+2)(1+3)/6 = 12,
-  // max = 10*min, at most 6 layers.
-  const int smin = 12, smax = 120, nlayers_max = 6;
+  // Stencil size rules: RFLU_SetInfoC2CStencil calls the four-argument
+  // RFLU_ComputeStencilSize(global, dimens=3, factor=1, order=1)
+  // (modflu/RFLU_ModStencilsCells.F90:2870-2873, RFLU_ModStencilsUtils.F90:1093-1130),
+  // so min = (1+1)(1+2)(1+3)/6 = 4, max = 10*min = 40, at most 6 layers.
+  const int smin = 4, smax = 40, nlayers_max = 6;
   m.stencil_min = smin;
   m.stencil_max = smax;
   m.c2cs_ptr.assign(N + 1, 0);
@@ -203,7 +251,8 @@ void build_stencils(rflu_mesh& m) {
 #pragma omp parallel
   {
   std::vector<int32_t> tmp, memb, help;
-  std::vector<std::pair<double, int32_t>> frac;
+  std::vector<double> frac;
+  std::vector<int32_t> fid;
 #pragma omp for schedule(dynamic, 256)
   for (int icg = 0; icg < N; ++icg) {
     // Basic stencil: vertex neighbours, ranked by shared-vertex fraction, best
@@ -218,6 +267,7 @@ void build_stencils(rflu_mesh& m) {
     tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
     tmp.erase(std::remove(tmp.begin(), tmp.end(), icg), tmp.end());
     frac.clear();
+    fid.clear();
     int vs[8];
     for (int k = 0; k < nv; ++k) vs[k] = m.c2v[nv * icg + k];
     std::sort(vs, vs + nv);
@@ -231,15 +281,15 @@ void build_stencils(rflu_mesh& m) {
         else if (vs[a] < vs2[b]) ++a;
         else ++b;
       }
-      frac.push_back({common * (1.0 / nv), c2});
+      frac.push_back(common * (1.0 / nv));
+      fid.push_back(c2);
     }
-    std::stable_sort(frac.begin(), frac.end(),
-                     [](const std::pair<double, int32_t>& x, const std::pair<double, int32_t>& y) {
-                       return x.first < y.first;
-                     });
+    // The ranking sort decides which tied candidates survive the cut at smax, so it
+    // keeps the reference's partition-exchange order.
+    rank_sort(frac, fid);
     memb.clear();
     for (int i = (int)frac.size() - 1; i >= std::max((int)frac.size() - smax, 0); --i)
-      memb.push_back(frac[i].second);
+      memb.push_back(fid[i]);
     std::sort(memb.begin(), memb.end());
     int degr = (int)memb.size();
     int lay_beg = 0, lay_end = degr;  // last layer [beg, end)

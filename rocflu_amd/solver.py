@@ -166,8 +166,9 @@ class GpuRegion:
         nat.check(self._lib.rflu_gpu_convective_fluxes(self.handle, stage),
                   "rflu_gpu_convective_fluxes")
 
-    def rk_update(self, stage: int):
-        nat.check(self._lib.rflu_gpu_rk_update(self.handle, stage), "rflu_gpu_rk_update")
+    def rk_update(self, stage: int, dt_min: float):
+        nat.check(self._lib.rflu_gpu_rk_update(self.handle, stage, float(dt_min)),
+                  "rflu_gpu_rk_update")
 
     def set_dependent_vars(self):
         nat.check(self._lib.rflu_gpu_set_dependent_vars(self.handle),
@@ -176,20 +177,28 @@ class GpuRegion:
     def after_update(self, stage: int):
         nat.check(self._lib.rflu_gpu_after_update(self.handle, stage), "rflu_gpu_after_update")
 
-    def runge_kutta(self):
-        """One time step through the separate entry points: RFLU_TimeStepping's dt, the
-        stages in RungeKuttaMP's order, and the host-side time advance
-        (rocflu/RFLU_TimeStepping.F90:429-640)."""
+    def runge_kutta(self, end_time: Optional[float] = None):
+        """One time step through the separate entry points: RFLU_TimeStepping's dt and
+        end-time clamp, the stages in RungeKuttaMP's order, and the host-side time
+        advance (rocflu/RFLU_TimeStepping.F90:429-640).  Returns (dt, finished)."""
         dt = self.time_step()
+        finished = False
+        if end_time is not None and self.time + dt > end_time:
+            dt = end_time - self.time
+            finished = True
         for s in range(1, n_stages(self.num.rk) + 1):
             self.rk_init(s)
             self.cell_gradients()
             self.convective_fluxes(s)
-            self.rk_update(s)
+            self.rk_update(s, dt)
             self.set_dependent_vars()
             self.after_update(s)
         self.time = self.time + dt
-        return dt
+        return dt, finished
+
+    def set_end_time(self, end_time: float):
+        nat.check(self._lib.rflu_gpu_set_end_time(self.handle, float(end_time)),
+                  "rflu_gpu_set_end_time")
 
     # -- fused paths ---------------------------------------------------------------
     def time_steps(self, nsteps: int, graph: bool = False):

@@ -180,3 +180,35 @@ def test_mixed_precision_error_vs_fp64(gpu):
     err = rel_l2(res[1], res[0])
     print("mixed vs FP64 rel L2 per variable:", err, flush=True)
     assert np.all(err <= 1e-5), err
+
+
+def test_gpu_matches_reference_golden_gas_jump(gpu):
+    """The GPU path against the reference's golden (box/gas-jump/closed-creconst0):
+    fused loop with the end-time clamp, and the per-callee entry points."""
+    from tests import gasjump
+    g = gasjump.load()
+    m, num, cv0 = gasjump.case()
+    r = GpuRegion(m, num)
+    r.upload(cv0)
+    r.set_end_time(gasjump.END_TIME)
+    r.time_steps(700, graph=True)
+    st = r.stats()
+    cvf, _ = r.download()
+    r.close()
+    assert st["finished"] == 1 and st["steps"] == 569, st
+    assert abs(st["time"] - g["time"]) <= 1e-9 * g["time"]
+    worst, fails = gasjump.compare(m, cvf, g)
+    print("fused: worst error / field scale:", worst, flush=True)
+    assert not fails, fails[:5]
+    r = GpuRegion(m, num)
+    r.upload(cv0)
+    n = 0
+    while True:
+        _, done = r.runge_kutta(end_time=gasjump.END_TIME)
+        n += 1
+        if done:
+            break
+    cve, _ = r.download()
+    r.close()
+    assert n == 569
+    assert np.array_equal(cve, cvf)
