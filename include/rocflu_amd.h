@@ -168,6 +168,16 @@ int rflu_gpu_synchronize(int32_t handle);
 int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
                         int32_t grad_block);
 
+/* Tuning options (set by the autotuner, rocflu_amd/autotune.py): whether the gradient
+ * kernel stages each 128-cell tile's stencil members in LDS (1) or reads them from
+ * global memory (0), how many members' weights it forms together (1, 2 or 4), and
+ * (read only) the LDS records per tile.  None changes results. */
+#define RFLU_OPT_GRAD_LDS 1
+#define RFLU_OPT_GRAD_UNROLL 2
+#define RFLU_OPT_TILE_CAP 3
+int rflu_gpu_set_option(int32_t handle, int32_t option, int32_t value);
+int rflu_gpu_get_option(int32_t handle, int32_t option, int32_t* value);
+
 /* Times `reps` launches of one kernel of the stage loop on the region's stream with
  * HIP events and returns the average milliseconds per launch (which: 0 face flux,
  * 1 update, 2 gradient, 3 limiter/blend, 4 time step, 5 fused stage).  Waits. */

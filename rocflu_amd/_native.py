@@ -82,6 +82,8 @@ SIGNATURES = {
     "rflu_gpu_set_profiling": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32]),
     "rflu_gpu_get_kernel_times": (ctypes.c_int, [ctypes.c_int32, _f64p,
                                                  ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
+    "rflu_gpu_set_option": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "rflu_gpu_get_option": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _i32p]),
     "rflu_gpu_last_error": (ctypes.c_char_p, []),
 }
 

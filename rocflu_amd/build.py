@@ -69,7 +69,7 @@ def build_fortran():
         return None
     out = os.path.join(fdir, "rflu_driver")
     _run([fc, "-O2", "-J", fdir, mod, drv, "-L" + HERE, "-lrocflu_amd", "-lrflu_mesh",
-          "-Wl,-rpath," + HERE, "-o", out], cwd=fdir)
+          "-Wl,-rpath,$ORIGIN/..", "-o", out], cwd=fdir)
     return out
 
 

@@ -58,7 +58,9 @@ struct Region {
   int *tile_uptr = nullptr, *tile_uidx = nullptr, *tile_lptr = nullptr;
   unsigned short* tile_loc = nullptr;
   int n_tiles = 0, tile_cap = 0;
-  bool use_lds = false;
+  bool use_lds = false;       // tiles were built (every tile fits in LDS)
+  bool grad_lds = true;       // tuning: stage stencils in LDS when tiles exist
+  int grad_unroll = 4;        // tuning: stencil members whose weights are formed together
   int n_dt_parts = 0;
   int face_block = 256, cell_block = 256, grad_block = 256;
   hipGraphExec_t graph = nullptr;
@@ -328,27 +330,31 @@ int launch_bfaces_t(Region* R) {
   return 0;
 }
 
-template <typename GT>
-int launch_gradients_t(Region* R) {
-  Mesh M = R->mesh();
-  State S = R->state();
-  if (R->use_lds) {
+template <typename GT, bool V, int U>
+void launch_gradient_kernel(Region* R, const Mesh& M, const State& S) {
+  if (R->use_lds && R->grad_lds) {
     Tiles T;
     T.uptr = R->tile_uptr; T.uidx = R->tile_uidx; T.lptr = R->tile_lptr; T.loc = R->tile_loc;
     T.ts = GRAD_TILE; T.cap = R->tile_cap;
     size_t shm = (size_t)8 * R->tile_cap * sizeof(double);
-    if (R->reconst == RFLU_RECONST_LIM_VENKAT)
-      hipLaunchKernelGGL((k_gradient_lds<GT, true>), dim3(R->n_tiles), dim3(GRAD_TILE), shm, R->stream, M, S, T, (GT*)R->grad, R->sc);
-    else
-      hipLaunchKernelGGL((k_gradient_lds<GT, false>), dim3(R->n_tiles), dim3(GRAD_TILE), shm, R->stream, M, S, T, (GT*)R->grad, R->sc);
-    return 0;
+    hipLaunchKernelGGL((k_gradient_lds<GT, V, U>), dim3(R->n_tiles), dim3(GRAD_TILE), shm,
+                       R->stream, M, S, T, (GT*)R->grad, R->sc);
+  } else {
+    hipLaunchKernelGGL((k_gradient<GT, V, U>), dim3(nblk(R->N, R->grad_block)), dim3(R->grad_block),
+                       0, R->stream, M, S, (GT*)R->grad, R->sc);
   }
-  dim3 gb(R->grad_block);
-  dim3 gg(nblk(R->N, R->grad_block));
-  if (R->reconst == RFLU_RECONST_LIM_VENKAT)
-    hipLaunchKernelGGL((k_gradient<GT, true>), gg, gb, 0, R->stream, M, S, (GT*)R->grad, R->sc);
-  else
-    hipLaunchKernelGGL((k_gradient<GT, false>), gg, gb, 0, R->stream, M, S, (GT*)R->grad, R->sc);
+}
+
+template <typename GT>
+int launch_gradients_t(Region* R) {
+  Mesh M = R->mesh();
+  State S = R->state();
+  bool v = R->reconst == RFLU_RECONST_LIM_VENKAT;
+  switch (R->grad_unroll) {
+    case 1: v ? launch_gradient_kernel<GT, true, 1>(R, M, S) : launch_gradient_kernel<GT, false, 1>(R, M, S); break;
+    case 2: v ? launch_gradient_kernel<GT, true, 2>(R, M, S) : launch_gradient_kernel<GT, false, 2>(R, M, S); break;
+    default: v ? launch_gradient_kernel<GT, true, 4>(R, M, S) : launch_gradient_kernel<GT, false, 4>(R, M, S); break;
+  }
   return 0;
 }
 
@@ -1017,6 +1023,38 @@ int rflu_gpu_time_kernel(int32_t h, int32_t which, int32_t reps, double* ms) {
   (void)hipEventDestroy(b);
   if (rc) return rc;
   *ms = t / reps;
+  return RFLU_OK;
+}
+
+int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  switch (option) {
+    case RFLU_OPT_GRAD_LDS:
+      if (value != 0 && value != 1) return fail(RFLU_ERR_ARG, "GRAD_LDS is 0 or 1");
+      R->grad_lds = value != 0;
+      break;
+    case RFLU_OPT_GRAD_UNROLL:
+      if (value != 1 && value != 2 && value != 4) return fail(RFLU_ERR_ARG, "GRAD_UNROLL is 1, 2 or 4");
+      R->grad_unroll = value;
+      break;
+    default:
+      return fail(RFLU_ERR_ARG, "unknown option");
+  }
+  R->graph_dirty = true;
+  return RFLU_OK;
+}
+
+int rflu_gpu_get_option(int32_t h, int32_t option, int32_t* value) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!value) return fail(RFLU_ERR_ARG, "null value");
+  switch (option) {
+    case RFLU_OPT_GRAD_LDS: *value = (R->use_lds && R->grad_lds) ? 1 : 0; break;
+    case RFLU_OPT_GRAD_UNROLL: *value = R->grad_unroll; break;
+    case RFLU_OPT_TILE_CAP: *value = R->tile_cap; break;
+    default: return fail(RFLU_ERR_ARG, "unknown option");
+  }
   return RFLU_OK;
 }
 

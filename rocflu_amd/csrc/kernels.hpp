@@ -731,10 +731,10 @@ __device__ __forceinline__ GradCoef<T> grad_coef(const Mesh& M, int i) {
   return c;
 }
 
-// One stencil member's contribution (xj - xi, var_j - var_i).
+// Weights of one stencil member (xj - xi) and its contribution (var_j - var_i).
 template <typename T>
-__device__ __forceinline__ void grad_member(const GradCoef<T>& c, double ddx, double ddy,
-                                            double ddz, const double* dvar, T* g) {
+__device__ __forceinline__ void grad_weights(const GradCoef<T>& c, double ddx, double ddy,
+                                             double ddz, T* w) {
   T dx = T(ddx), dy = T(ddy), dz = T(ddz);
   T term = T(1) / tsqrt(dx * dx + dy * dy + dz * dz);
   dx = term * dx;
@@ -743,16 +743,28 @@ __device__ __forceinline__ void grad_member(const GradCoef<T>& c, double ddx, do
   T term1 = c.c1111 * (dx);
   T term2 = c.c2222 * (dy + c.c12 * dx);
   T term3 = c.c3333 * (dz + c.c23 * dy + c.c13 * dx);
-  T wtx = term * (term1 + c.c12 * term2 + c.c13 * term3);
-  T wty = term * (term2 + c.c23 * term3);
-  T wtz = term * (term3);
+  w[0] = term * (term1 + c.c12 * term2 + c.c13 * term3);
+  w[1] = term * (term2 + c.c23 * term3);
+  w[2] = term * (term3);
+}
+
+template <typename T>
+__device__ __forceinline__ void grad_accum(const T* w, const double* dvar, T* g) {
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
     T dVar = T(dvar[k]);
-    g[3 * k + 0] = g[3 * k + 0] + wtx * dVar;
-    g[3 * k + 1] = g[3 * k + 1] + wty * dVar;
-    g[3 * k + 2] = g[3 * k + 2] + wtz * dVar;
+    g[3 * k + 0] = g[3 * k + 0] + w[0] * dVar;
+    g[3 * k + 1] = g[3 * k + 1] + w[1] * dVar;
+    g[3 * k + 2] = g[3 * k + 2] + w[2] * dVar;
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void grad_member(const GradCoef<T>& c, double ddx, double ddy,
+                                            double ddz, const double* dvar, T* g) {
+  T w[3];
+  grad_weights<T>(c, ddx, ddy, ddz, w);
+  grad_accum<T>(w, dvar, g);
 }
 
 // Venkatakrishnan limiter of cell i in FP64: varMin/varMax over interior-face
@@ -811,7 +823,7 @@ __device__ __forceinline__ void venkat_limit(const Mesh& M, const State& S, int 
 }
 
 // One thread per cell, members read from global memory (L2/MALL).
-template <typename GT, bool VENKAT>
+template <typename GT, bool VENKAT, int U>
 __global__ void k_gradient(Mesh M, State S, GT* __restrict__ grad, const Scalars* __restrict__ sc) {
   if (halted(sc)) return;
   const int N = M.N;
@@ -826,7 +838,22 @@ __global__ void k_gradient(Mesh M, State S, GT* __restrict__ grad, const Scalars
 #pragma unroll
   for (int k = 0; k < 15; ++k) g[k] = GT(0);
   int s0 = M.sten_ptr[i], s1 = M.sten_ptr[i + 1];
-  for (int s = s0; s < s1; ++s) {
+  int s = s0;
+  for (; s + U <= s1; s += U) {
+    GT w[U][3];
+    double dvar[U][5];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      int j = M.sten_idx[s + u];
+      double4 cj = M.cell[j];
+      grad_weights<GT>(c, cj.x - ci.x, cj.y - ci.y, cj.z - ci.z, w[u]);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) dvar[u][k] = S.prim[(size_t)k * N + j] - vi[k];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) grad_accum<GT>(w[u], dvar[u], g);
+  }
+  for (; s < s1; ++s) {
     int j = M.sten_idx[s];
     double4 cj = M.cell[j];
     double dvar[5];
@@ -852,7 +879,7 @@ struct Tiles {
   int cap;
 };
 
-template <typename GT, bool VENKAT>
+template <typename GT, bool VENKAT, int U>
 __global__ __launch_bounds__(128) void k_gradient_lds(Mesh M, State S, Tiles T,
                                                       GT* __restrict__ grad,
                                                       const Scalars* __restrict__ sc) {
@@ -888,7 +915,23 @@ __global__ __launch_bounds__(128) void k_gradient_lds(Mesh M, State S, Tiles T,
   for (int k = 0; k < 15; ++k) g[k] = GT(0);
   const int cnt = M.sten_ptr[i + 1] - M.sten_ptr[i];
   const unsigned short* L = T.loc + T.lptr[t] + threadIdx.x;
-  for (int s = 0; s < cnt; ++s) {
+  int s = 0;
+  // U members' weights are formed together (independent sqrt/divide chains), then
+  // accumulated one member after the other in stencil order.
+  for (; s + U <= cnt; s += U) {
+    GT w[U][3];
+    double dvar[U][5];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      int k = L[(size_t)(s + u) * T.ts];
+      grad_weights<GT>(c, lx[k] - ci.x, ly[k] - ci.y, lz[k] - ci.z, w[u]);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) dvar[u][q] = lv[q * cap + k] - vi[q];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) grad_accum<GT>(w[u], dvar[u], g);
+  }
+  for (; s < cnt; ++s) {
     int k = L[(size_t)s * T.ts];
     double dvar[5];
 #pragma unroll

@@ -1,0 +1,212 @@
+!******************************************************************************
+!
+! rflu_gpu -- ISO_C_BINDING interface of the MI355X explicit residual/update path
+! (include/rocflu_amd.h) and of the host region builder (include/rflu_mesh.h).
+!
+! A reference host keeps its t_region and replaces the bodies of RungeKuttaMP's
+! callees with these calls, one region handle per t_region:
+!   RKInitMP            -> rflu_gpu_rk_init              (libfloflu/RkInitMP.F90)
+!   CellGradientsMP     -> rflu_gpu_cell_gradients       (libfloflu/CellGradientsMP.F90)
+!   ConvectiveFluxesMP  -> rflu_gpu_convective_fluxes    (libfloflu/ConvectiveFluxesMP.F90)
+!   RKUpdateMP          -> rflu_gpu_rk_update            (libfloflu/RkUpdateMP.F90)
+!   RFLU_SetVarsContWrapper -> rflu_gpu_set_dependent_vars
+!   AfterUpdateMP       -> rflu_gpu_after_update         (libfloflu/AfterUpdateMP.F90)
+!   RFLU_TimeStepInviscid + RFLU_MinimumTimeStep -> rflu_gpu_time_step
+! Arrays are passed as they are stored in t_grid/t_patch (column-major, component
+! index fastest) with index_base = 1.
+!
+!******************************************************************************
+MODULE rflu_gpu
+
+  USE ISO_C_BINDING
+  IMPLICIT NONE
+
+  INTEGER(C_INT), PARAMETER :: RFLU_OK = 0, RFLU_ERR_INVALID_VALUE = 1, &
+                               RFLU_ERR_NEGATIVE_POSDEF = 2, RFLU_ERR_ARG = 3, &
+                               RFLU_ERR_HIP = 4, RFLU_ERR_NOMEM = 5, &
+                               RFLU_ERR_HANDLE = 6
+
+  TYPE, BIND(C) :: rflu_region_desc
+    INTEGER(C_INT32_T) :: index_base, n_cells, n_faces, n_patches
+    TYPE(C_PTR) :: f2c, fn, fc, cofg, vol
+    TYPE(C_PTR) :: bpatch_ptr, bf2c, bfn, bfc
+    TYPE(C_PTR) :: patch_bc, patch_opt, patch_pout, patch_order
+    TYPE(C_PTR) :: c2cs_ptr, c2cs_idx, xyz_moms
+    REAL(C_DOUBLE) :: gamma, cp
+    INTEGER(C_INT32_T) :: discr, order, reconst, rk, precision
+    REAL(C_DOUBLE) :: cfl, dt_imposed, dt_min_limit, eps_entr, diss_fact
+  END TYPE rflu_region_desc
+
+  TYPE, BIND(C) :: rflu_gpu_stats
+    INTEGER(C_INT64_T) :: n_invalid, n_negative, n_bounds_fix, n_pos_fix
+    REAL(C_DOUBLE) :: mass_in, mass_out, time, dt
+    INTEGER(C_INT64_T) :: steps
+    INTEGER(C_INT32_T) :: finished, pad
+  END TYPE rflu_gpu_stats
+
+  INTERFACE
+    INTEGER(C_INT) FUNCTION rflu_gpu_device_count() BIND(C, NAME='rflu_gpu_device_count')
+      IMPORT :: C_INT
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_create_region(desc, handle) &
+        BIND(C, NAME='rflu_gpu_create_region')
+      IMPORT :: C_INT, C_INT32_T, rflu_region_desc
+      TYPE(rflu_region_desc), INTENT(IN) :: desc
+      INTEGER(C_INT32_T), INTENT(OUT) :: handle
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_destroy_region(handle) &
+        BIND(C, NAME='rflu_gpu_destroy_region')
+      IMPORT :: C_INT, C_INT32_T
+      INTEGER(C_INT32_T), VALUE :: handle
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_upload_state(handle, cv, dv, time) &
+        BIND(C, NAME='rflu_gpu_upload_state')
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+      INTEGER(C_INT32_T), VALUE :: handle
+      REAL(C_DOUBLE), INTENT(IN) :: cv(*)
+      TYPE(C_PTR), VALUE :: dv
+      REAL(C_DOUBLE), VALUE :: time
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_download_state(handle, cv, dv) &
+        BIND(C, NAME='rflu_gpu_download_state')
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+      INTEGER(C_INT32_T), VALUE :: handle
+      REAL(C_DOUBLE), INTENT(OUT) :: cv(*), dv(*)
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_time_step(handle, current_time, dt_out) &
+        BIND(C, NAME='rflu_gpu_time_step')
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+      INTEGER(C_INT32_T), VALUE :: handle
+      REAL(C_DOUBLE), VALUE :: current_time
+      REAL(C_DOUBLE), INTENT(OUT) :: dt_out
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_rk_init(handle, istage) BIND(C, NAME='rflu_gpu_rk_init')
+      IMPORT :: C_INT, C_INT32_T
+      INTEGER(C_INT32_T), VALUE :: handle, istage
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_cell_gradients(handle) &
+        BIND(C, NAME='rflu_gpu_cell_gradients')
+      IMPORT :: C_INT, C_INT32_T
+      INTEGER(C_INT32_T), VALUE :: handle
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_convective_fluxes(handle, istage) &
+        BIND(C, NAME='rflu_gpu_convective_fluxes')
+      IMPORT :: C_INT, C_INT32_T
+      INTEGER(C_INT32_T), VALUE :: handle, istage
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_rk_update(handle, istage, dt_min) &
+        BIND(C, NAME='rflu_gpu_rk_update')
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+      INTEGER(C_INT32_T), VALUE :: handle, istage
+      REAL(C_DOUBLE), VALUE :: dt_min
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_set_dependent_vars(handle) &
+        BIND(C, NAME='rflu_gpu_set_dependent_vars')
+      IMPORT :: C_INT, C_INT32_T
+      INTEGER(C_INT32_T), VALUE :: handle
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_after_update(handle, istage) &
+        BIND(C, NAME='rflu_gpu_after_update')
+      IMPORT :: C_INT, C_INT32_T
+      INTEGER(C_INT32_T), VALUE :: handle, istage
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_residual(handle, rhs) BIND(C, NAME='rflu_gpu_residual')
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+      INTEGER(C_INT32_T), VALUE :: handle
+      REAL(C_DOUBLE), INTENT(OUT) :: rhs(*)
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_time_steps(handle, nsteps, use_graph) &
+        BIND(C, NAME='rflu_gpu_time_steps')
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T
+      INTEGER(C_INT32_T), VALUE :: handle
+      INTEGER(C_INT64_T), VALUE :: nsteps
+      INTEGER(C_INT32_T), VALUE :: use_graph
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_set_end_time(handle, end_time) &
+        BIND(C, NAME='rflu_gpu_set_end_time')
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+      INTEGER(C_INT32_T), VALUE :: handle
+      REAL(C_DOUBLE), VALUE :: end_time
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_get_stats(handle, stats) BIND(C, NAME='rflu_gpu_get_stats')
+      IMPORT :: C_INT, C_INT32_T, rflu_gpu_stats
+      INTEGER(C_INT32_T), VALUE :: handle
+      TYPE(rflu_gpu_stats), INTENT(OUT) :: stats
+    END FUNCTION
+
+    FUNCTION rflu_gpu_last_error() BIND(C, NAME='rflu_gpu_last_error')
+      IMPORT :: C_PTR
+      TYPE(C_PTR) :: rflu_gpu_last_error
+    END FUNCTION
+
+    ! host region builder (include/rflu_mesh.h)
+    INTEGER(C_INT) FUNCTION rflu_mesh_box(kind, nx, ny, nz, lx, ly, lz, x0, y0, z0, jitter, &
+                                          seed, bc_type, bc_opt, bc_pout, build_stencils, out) &
+        BIND(C, NAME='rflu_mesh_box')
+      IMPORT :: C_INT, C_DOUBLE, C_INT64_T, C_INT32_T, C_PTR
+      INTEGER(C_INT), VALUE :: kind, nx, ny, nz, build_stencils
+      REAL(C_DOUBLE), VALUE :: lx, ly, lz, x0, y0, z0, jitter
+      INTEGER(C_INT64_T), VALUE :: seed
+      INTEGER(C_INT32_T), INTENT(IN) :: bc_type(6), bc_opt(6)
+      REAL(C_DOUBLE), INTENT(IN) :: bc_pout(6)
+      TYPE(C_PTR), INTENT(OUT) :: out
+    END FUNCTION
+
+    SUBROUTINE rflu_mesh_destroy(m) BIND(C, NAME='rflu_mesh_destroy')
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: m
+    END SUBROUTINE
+
+    INTEGER(C_INT) FUNCTION rflu_mesh_sizes(m, out) BIND(C, NAME='rflu_mesh_sizes')
+      IMPORT :: C_INT, C_PTR, C_INT64_T
+      TYPE(C_PTR), VALUE :: m
+      INTEGER(C_INT64_T), INTENT(OUT) :: out(9)
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_mesh_array(m, name, ptr, count, dtype) &
+        BIND(C, NAME='rflu_mesh_array')
+      IMPORT :: C_INT, C_PTR, C_CHAR, C_INT64_T
+      TYPE(C_PTR), VALUE :: m
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: name(*)
+      TYPE(C_PTR), INTENT(OUT) :: ptr
+      INTEGER(C_INT64_T), INTENT(OUT) :: count
+      INTEGER(C_INT), INTENT(OUT) :: dtype
+    END FUNCTION
+  END INTERFACE
+
+CONTAINS
+
+  ! Stops with the library's message, as the reference's ErrorStop would.
+  SUBROUTINE rflu_gpu_check(rc, where)
+    INTEGER(C_INT), INTENT(IN) :: rc
+    CHARACTER(*), INTENT(IN) :: where
+    CHARACTER(KIND=C_CHAR), POINTER :: msg(:)
+    INTEGER :: n
+    IF ( rc /= RFLU_OK ) THEN
+      CALL C_F_POINTER(rflu_gpu_last_error(), msg, [512])
+      n = 0
+      DO WHILE ( n < 512 )
+        IF ( msg(n+1) == C_NULL_CHAR ) EXIT
+        n = n + 1
+      END DO
+      WRITE(*,'(A,A,I0,A,512A)') where, ': error ', rc, ': ', msg(1:n)
+      STOP 1
+    END IF
+  END SUBROUTINE rflu_gpu_check
+
+END MODULE rflu_gpu

@@ -233,6 +233,18 @@ class GpuRegion:
                   "rflu_gpu_time_kernel")
         return ms.value
 
+    OPT_GRAD_LDS, OPT_GRAD_UNROLL, OPT_TILE_CAP = 1, 2, 3
+
+    def set_option(self, option: int, value: int):
+        nat.check(self._lib.rflu_gpu_set_option(self.handle, int(option), int(value)),
+                  "rflu_gpu_set_option")
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int32()
+        nat.check(self._lib.rflu_gpu_get_option(self.handle, int(option), ctypes.byref(v)),
+                  "rflu_gpu_get_option")
+        return v.value
+
     def set_profiling(self, on: bool):
         nat.check(self._lib.rflu_gpu_set_profiling(self.handle, 1 if on else 0),
                   "rflu_gpu_set_profiling")
