@@ -7,9 +7,11 @@ the FP64 GPU run per conserved variable, and the dominant kernel's HBM roofline
 fraction.  Workload (default): 80^3 box split into 6 Kuhn tetrahedra per box cell
 (3,072,000 tets), interior nodes jittered +-8%, Mach-2 planar shock into quiescent gas
 with 1% seeded density noise, second-order HLLC with WLSQ gradients and the
-Venkatakrishnan limiter, Wray RK3 (the reference's three-stage scheme), slip walls.
-A step is one time step (dt + 3 stages) over the whole mesh; inputs are resident in
-HBM before the timed region.
+Venkatakrishnan limiter, SSP-RK3 (three stages, as the workload names it; the
+reference's own three-stage scheme, Wray RK3, runs the same kernels per stage), slip
+walls.  A step is one time step (dt + 3 stages) over the whole mesh; inputs are
+resident in HBM before the timed region.  Kernel variants (all bit-identical) are
+chosen by the project's autotuner before timing.
 
 Prints ONE JSON line.
 """
@@ -32,39 +34,49 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 KERNELS = ["face_flux", "update", "gradient", "blend", "time_step", "bface_flux"]
 
 
-def algorithmic_bytes(m, num, kernel: str) -> float:
+def algorithmic_bytes(m, num, kernel: str, tuning: dict) -> float:
     """Compulsory bytes of one launch: every array element the kernel needs, once, at
-    its stored width (double4 geometry counted at 32 B)."""
+    its stored width (double4 geometry counted at 32 B; re-reads of neighbours' data
+    through L2/LDS are not counted)."""
     N, F, B = int(m.n_cells), int(m.n_faces), int(m.n_bfaces)
     S = int(m.c2cs_idx.size) if num.order == 2 else 0
     K = int(m.c2f_idx.size)               # cell-to-face entries (2F + B)
     fb = 4 if num.precision in (1, 3) else 8   # flux buffer width
     gb = 4 if num.precision in (1, 2) else 8   # gradient width
     if kernel == "gradient":
-        b = N * (40 + 32 + 48 + 4 + 8 + 4) + 4 * S + 15 * gb * N
+        b = N * (40 + 15 * gb) + 4 * (N + 1)   # own state in, gradient out, stencil counts
+        if tuning.get("grad_lds"):
+            b += 2 * S                          # 16-bit LDS slot of every member
+        else:
+            b += 4 * S                          # member indices
+        if tuning.get("grad_weights"):
+            b += 3 * gb * S                     # stored stencil weights
+        else:
+            b += N * (32 + 48)                  # centroids, R-factor moments
         if num.reconst == 11:
-            b += 4 * K + F * (8 + 32)     # c2f, f2c and face centroids of the limiter
+            b += 8 * N + 2 * F * (24 + 2)       # eps^2; cofg - fc and slot per face side
         return float(b)
     if kernel == "face_flux":
         per_face = 8 + 32 + (32 if num.order == 2 else 0) + 5 * fb
         per_cell = (40 + 32 + 15 * gb) if num.order == 2 else (40 + 24)
         return float(F * per_face + N * per_cell)
     if kernel == "update":
-        # c2f, flux (each face value once), cv in/out, rhsSum in/out, vol, dv out,
-        # prim out (order 2)
-        b = 4 * (N + 1) + 4 * K + 5 * fb * (F + B) + N * (40 + 40 + 40 + 40 + 8 + 24)
+        # c2f, flux (each face value once), cv in/out, RK storage (SSP-RK3: cvOld once;
+        # Wray: rhsSum written, read+written, read over 3 stages), vol, dv out, prim out
+        rk_store = 40 if num.rk == 4 else 40 * 4 / 3
+        b = 4 * (N + 1) + 4 * K + 5 * fb * (F + B) + N * (40 + 40 + rk_store + 8 + 24)
         if num.order == 2:
             b += 40 * N
         return float(b)
     if kernel == "time_step":
-        return float(4 * (N + 1) + 4 * K + F * (8 + 32) + B * 32 + N * (40 + 24 + 8))
+        return float(4 * (N + 1) + 4 * K + 8 * (F + B) + 8 * N)
     return 0.0
 
 
 def build_case(args, precision):
     from rocflu_amd import cases
     from rocflu_amd.solver import Numerics
-    num = Numerics(discr=args.discr, order=2, reconst=args.reconst, rk=2, cfl=0.5,
+    num = Numerics(discr=args.discr, order=2, reconst=args.reconst, rk=4, cfl=0.5,
                    dt_imposed=1e-4, dt_min_limit=0.0, precision=precision)
     return num
 
@@ -86,13 +98,14 @@ def cpu_baseline(args):
     el = time.perf_counter() - t0
     return {"value": m.n_cells * 3 * steps / el, "unit": "cell-updates/s", "cores": 1,
             "kind": "port",
-            "sample": f"{n}^3x6 = {m.n_cells} tets, {steps} Wray steps of the same scheme, "
+            "sample": f"{n}^3x6 = {m.n_cells} tets, {steps} SSP-RK3 steps of the same scheme, "
                       f"{el:.1f} s, single thread"}
 
 
 def run(args):
     from rocflu_amd import cases
     from rocflu_amd.solver import GpuRegion
+    from rocflu_amd import autotune
     import torch  # device memory and streams are owned by the C-ABI library
 
     t0 = time.time()
@@ -106,6 +119,9 @@ def run(args):
         num = build_case(args, prec)
         cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
         g = GpuRegion(m, num)
+        g.upload(cv0)
+        tuning = autotune.tune(g)
+        print(f"# prec {prec}: tuning {tuning}", file=sys.stderr, flush=True)
         g.upload(cv0)
         g.time_steps(args.warmup)
         g.synchronize()
@@ -134,7 +150,8 @@ def run(args):
         g.close()
         n_upd = m.n_cells * 3 * args.steps
         results[prec] = {"value": n_upd / el, "ms_per_step": 1e3 * el / args.steps,
-                         "graph_value": n_upd / elg, "kernels": kt, "num": num, "stats": st}
+                         "graph_value": n_upd / elg, "kernels": kt, "num": num, "stats": st,
+                         "tuning": {k: v for k, v in tuning.items() if k != "timings_ms"}}
         print(f"# prec {prec}: {n_upd / el:.4g} cell-upd/s ({1e3 * el / args.steps:.3f} ms/step), "
               f"graph {n_upd / elg:.4g}; kernels {kt}", file=sys.stderr, flush=True)
     return m, results, finals
@@ -181,13 +198,16 @@ def main():
     kt = r["kernels"]
     dom = max(KERNELS, key=lambda k: kt[k][0])
     ms_launch = kt[dom][0] / max(kt[dom][1], 1)
-    ab = algorithmic_bytes(m, r["num"], dom)
+    ab = algorithmic_bytes(m, r["num"], dom, r["tuning"])
     achieved = ab / (ms_launch * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_tet{args.n}_prec{main_prec}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        # measured for the same kernel variant only
+        if dom != "gradient" or tj.get("variant") == "grad_lds{grad_lds}_u{grad_unroll}_w{grad_weights}".format(**r["tuning"]):
+            traffic = tj.get("bytes_per_launch")
     out = {
         "metric": "cell-updates/s per RK stage, FP64 vs mixed at 1e-6 rel L2; % of HBM peak",
         "value": r["value"],
@@ -205,10 +225,11 @@ def main():
                                f"jitter 8%, Mach-2 shock + 1% density noise, "
                                f"{ {1: 'Roe', 3: 'HLLC', 4: 'AUSM+'}[args.discr]} order 2, "
                                f"WLSQ + {'Venkatakrishnan' if args.reconst == 11 else 'WENO-XYZ'}, "
-                               f"Wray RK3, slip walls",
+                               f"SSP-RK3, slip walls",
                    "cells": int(m.n_cells), "stages_per_step": 3},
         "graph_value": r["graph_value"],
         "kernel_ms": {k: [v[0], v[1]] for k, v in kt.items()},
+        "tuning": r["tuning"],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": ab, "ms_per_launch": ms_launch},

@@ -1,0 +1,127 @@
+"""Autotuner of a region's launch variants.
+
+Every option it sets leaves results bit-for-bit unchanged (tested in
+tests/test_gpu_parity.py::test_tuning_options_do_not_change_results), so the choice is
+made on time alone: each candidate is launched on the region's own stream and timed
+with HIP events (``rflu_gpu_time_kernel``), and the fastest is kept.  Choices are cached
+per mesh/scheme signature, in memory and optionally in a JSON file named by
+``ROCFLU_AMD_TUNE_CACHE``.
+
+Timing launches run the kernels on the region's current state (the update kernel with a
+negligible dt), so upload the initial state after tuning.
+"""
+from __future__ import annotations
+
+import json
+import os
+from typing import Dict, List, Optional
+
+from . import _native as nat
+from .solver import GpuRegion
+
+K_FACE, K_UPDATE, K_GRAD = 0, 1, 2
+
+_cache: Dict[str, dict] = {}
+
+
+def signature(g: GpuRegion) -> str:
+    m, num = g.mesh, g.num
+    return (f"N{m.n_cells}-F{m.n_faces}-B{m.n_bfaces}-S{m.c2cs_idx.size if m.c2cs_idx is not None else 0}"
+            f"-d{num.discr}-o{num.order}-r{num.reconst}-k{num.rk}-p{num.precision}")
+
+
+def _load_file() -> dict:
+    path = os.environ.get("ROCFLU_AMD_TUNE_CACHE")
+    if not path or not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def _store_file(sig: str, choice: dict):
+    path = os.environ.get("ROCFLU_AMD_TUNE_CACHE")
+    if not path:
+        return
+    data = _load_file()
+    data[sig] = choice
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+
+
+def apply(g: GpuRegion, choice: dict):
+    """Sets the options of a tuning choice on a region."""
+    g.set_option(GpuRegion.OPT_XCD, choice["xcd"])
+    g.set_launch(face_block=choice["face_block"], cell_block=choice["cell_block"])
+    if g.num.order == 2:
+        g.set_option(GpuRegion.OPT_GRAD_LDS, choice["grad_lds"])
+        g.set_option(GpuRegion.OPT_GRAD_UNROLL, choice["grad_unroll"])
+        g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, choice["grad_weights"])
+
+
+def _time(g: GpuRegion, which: int, reps: int) -> float:
+    g.time_kernel(which, 2)  # warm-up (first launch of a variant loads its code object)
+    return min(g.time_kernel(which, reps) for _ in range(2))
+
+
+def gradient_candidates(g: GpuRegion, allow_weights: bool = True) -> List[dict]:
+    c = []
+    if g.get_option(GpuRegion.OPT_TILE_CAP) > 0:
+        for u in (2, 4):
+            c.append(dict(grad_lds=1, grad_unroll=u, grad_weights=0))
+        if allow_weights:
+            for u in (1, 2):
+                c.append(dict(grad_lds=1, grad_unroll=u, grad_weights=1))
+    c.append(dict(grad_lds=0, grad_unroll=4, grad_weights=0))
+    return c
+
+
+def tune(g: GpuRegion, reps: int = 10, allow_weights: bool = True, use_cache: bool = True,
+         verbose: bool = False) -> dict:
+    """Chooses and applies the fastest variant of each kernel of the stage for region g.
+    Returns the choice with the measured milliseconds of every candidate."""
+    sig = signature(g)
+    if use_cache:
+        if not _cache:
+            _cache.update(_load_file())
+        if sig in _cache:
+            apply(g, _cache[sig])
+            return dict(_cache[sig], cached=True)
+    choice = dict(xcd=8, face_block=256, cell_block=256, grad_lds=0, grad_unroll=4,
+                  grad_weights=0)
+    timings = {}
+    # XCD-aware block order and block sizes of the face and cell kernels
+    best = None
+    for xcd in (8, 1):
+        for blk in (256, 512):
+            g.set_option(GpuRegion.OPT_XCD, xcd)
+            g.set_launch(face_block=blk, cell_block=blk)
+            t = _time(g, K_FACE, reps) + _time(g, K_UPDATE, reps)
+            timings[f"xcd{xcd}_blk{blk}"] = t
+            if best is None or t < best[0]:
+                best = (t, xcd, blk)
+    choice.update(xcd=best[1], face_block=best[2], cell_block=best[2])
+    g.set_option(GpuRegion.OPT_XCD, choice["xcd"])
+    g.set_launch(face_block=choice["face_block"], cell_block=choice["cell_block"])
+    if g.num.order == 2:
+        best = None
+        for cand in gradient_candidates(g, allow_weights):
+            try:
+                g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, cand["grad_weights"])
+            except nat.RfluError as e:
+                if e.code == nat.RFLU_ERR_NOMEM:
+                    continue  # no room for the stored weights
+                raise
+            g.set_option(GpuRegion.OPT_GRAD_LDS, cand["grad_lds"])
+            g.set_option(GpuRegion.OPT_GRAD_UNROLL, cand["grad_unroll"])
+            t = _time(g, K_GRAD, reps)
+            key = "grad_lds{grad_lds}_u{grad_unroll}_w{grad_weights}".format(**cand)
+            timings[key] = t
+            if verbose:
+                print(f"autotune {sig}: {key} {t:.3f} ms", flush=True)
+            if best is None or t < best[0]:
+                best = (t, cand)
+        choice.update(best[1])
+        apply(g, choice)
+    _cache[sig] = dict(choice)
+    _store_file(sig, choice)
+    return dict(choice, timings_ms=timings, cached=False)

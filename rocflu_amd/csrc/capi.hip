@@ -47,6 +47,10 @@ struct Region {
   int *bf2c = nullptr, *bpatch = nullptr, *c2f_ptr = nullptr, *c2f_idx = nullptr;
   int *sten_ptr = nullptr, *sten_idx = nullptr;
   double *moms = nullptr, *epsq = nullptr;
+  int* vnb = nullptr;          // limiter neighbours [vq][N]
+  double* vgeo = nullptr;      // limiter cofg - fc [3*vq][N]
+  int vq = 0;
+  double* srad = nullptr;      // face spectral radii [F+B] of the time step
   double *cv = nullptr, *cv_old = nullptr, *rhs_sum = nullptr, *dv = nullptr, *prim = nullptr;
   void *grad = nullptr, *grad2 = nullptr, *flux = nullptr;
   double* rhs = nullptr;
@@ -57,10 +61,14 @@ struct Region {
   // stencil tiles staged in LDS by the gradient kernel
   int *tile_uptr = nullptr, *tile_uidx = nullptr, *tile_lptr = nullptr;
   unsigned short* tile_loc = nullptr;
+  unsigned short* tile_vloc = nullptr;
+  size_t tile_loc_len = 0;    // entries of the [member][lane] slot table
+  void* gw = nullptr;         // precomputed stencil weights [3][tile_loc_len] (GRAD_WEIGHTS)
   int n_tiles = 0, tile_cap = 0;
   bool use_lds = false;       // tiles were built (every tile fits in LDS)
   bool grad_lds = true;       // tuning: stage stencils in LDS when tiles exist
   int grad_unroll = 4;        // tuning: stencil members whose weights are formed together
+  int nxcd = 8;               // tuning: XCD-aware block order (MI355X: 8 XCDs); 1 = off
   int n_dt_parts = 0;
   int face_block = 256, cell_block = 256, grad_block = 256;
   hipGraphExec_t graph = nullptr;
@@ -79,6 +87,7 @@ struct Region {
     m.bf2c = bf2c; m.bfn = bfn; m.bfc = bfc; m.bpatch = bpatch;
     m.c2f_ptr = c2f_ptr; m.c2f_idx = c2f_idx;
     m.sten_ptr = sten_ptr; m.sten_idx = sten_idx; m.moms = moms; m.epsq = epsq;
+    m.vq = vq; m.vnb = vnb; m.vgeo = vgeo; m.nxcd = nxcd;
     return m;
   }
   State state() const {
@@ -120,7 +129,8 @@ void free_region(Region* R) {
   void* bufs[] = {R->f2c, R->fn, R->fc, R->cell, R->bfn, R->bfc, R->bf2c, R->bpatch, R->c2f_ptr,
                   R->c2f_idx, R->sten_ptr, R->sten_idx, R->moms, R->epsq, R->cv, R->cv_old,
                   R->rhs_sum, R->dv, R->prim, R->grad, R->grad2, R->flux, R->rhs, R->dt_part,
-                  R->sc, R->tile_uptr, R->tile_uidx, R->tile_lptr, R->tile_loc};
+                  R->sc, R->tile_uptr, R->tile_uidx, R->tile_lptr, R->tile_loc, R->tile_vloc,
+                  R->vnb, R->vgeo, R->srad, R->gw};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (R->stream) (void)hipStreamDestroy(R->stream);
@@ -233,20 +243,27 @@ void renumber(Region* R, std::vector<double4>& hcell, std::vector<int2>& hf2c,
 }
 
 // LDS tiles of the gradient kernel: tile t covers cells [t*TS, t*TS+TS); its union of
-// stencil members is staged in LDS, and loc[lptr[t] + s*TS + lane] is the LDS slot of
-// the lane's s-th stencil member (member order unchanged).
+// stencil members (and, with the limiter, face neighbours) is staged in LDS;
+// loc[lptr[t] + s*TS + lane] is the LDS slot of the lane's s-th stencil member (member
+// order unchanged) and vloc[(t*vq + q)*TS + lane] that of its q-th limiter neighbour.
 constexpr int GRAD_TILE = 128;
 constexpr int GRAD_TILE_CAP = 1024;   // records of 8 doubles: 64 KiB of LDS
 
 void build_tiles(Region* R, const std::vector<int>& sp, const std::vector<int>& si,
-                 std::vector<int>& uptr, std::vector<int>& uidx, std::vector<int>& lptr,
-                 std::vector<unsigned short>& loc) {
-  const int N = R->N, TS = GRAD_TILE;
+                 const std::vector<int>& vnb, bool venkat, std::vector<int>& uptr,
+                 std::vector<int>& uidx, std::vector<int>& lptr, std::vector<unsigned short>& loc,
+                 std::vector<unsigned short>& vloc) {
+  const int N = R->N, TS = GRAD_TILE, VQ = R->vq;
   const int nt = (N + TS - 1) / TS;
+  auto give_up = [&]() {
+    R->use_lds = false;
+    uptr.clear(); uidx.clear(); lptr.clear(); loc.clear(); vloc.clear();
+  };
   uptr.assign(nt + 1, 0);
   lptr.assign(nt + 1, 0);
   uidx.clear();
   loc.clear();
+  vloc.assign(venkat ? (size_t)nt * VQ * TS : 1, (unsigned short)0xFFFF);
   int cap = 0;
   std::vector<int> u;
   for (int t = 0; t < nt; ++t) {
@@ -256,14 +273,16 @@ void build_tiles(Region* R, const std::vector<int>& sp, const std::vector<int>& 
     for (int i = c0; i < c1; ++i) {
       smax = std::max(smax, sp[i + 1] - sp[i]);
       for (int e = sp[i]; e < sp[i + 1]; ++e) u.push_back(si[e]);
+      if (venkat)
+        for (int q = 0; q < VQ && vnb[(size_t)q * N + i] >= 0; ++q) u.push_back(vnb[(size_t)q * N + i]);
     }
     std::sort(u.begin(), u.end());
     u.erase(std::unique(u.begin(), u.end()), u.end());
     if ((int)u.size() > GRAD_TILE_CAP) {   // does not fit: gradient kernel reads global memory
-      R->use_lds = false;
-      uptr.clear(); uidx.clear(); lptr.clear(); loc.clear();
+      give_up();
       return;
     }
+    auto slot = [&](int j) { return (unsigned short)(std::lower_bound(u.begin(), u.end(), j) - u.begin()); };
     cap = std::max(cap, (int)u.size());
     uidx.insert(uidx.end(), u.begin(), u.end());
     uptr[t + 1] = (int)uidx.size();
@@ -271,14 +290,13 @@ void build_tiles(Region* R, const std::vector<int>& sp, const std::vector<int>& 
     loc.resize(base + (size_t)smax * TS, (unsigned short)0);
     for (int i = c0; i < c1; ++i) {
       int lane = i - c0;
-      for (int e = sp[i], s = 0; e < sp[i + 1]; ++e, ++s) {
-        int k = (int)(std::lower_bound(u.begin(), u.end(), si[e]) - u.begin());
-        loc[base + (size_t)s * TS + lane] = (unsigned short)k;
-      }
+      for (int e = sp[i], s = 0; e < sp[i + 1]; ++e, ++s) loc[base + (size_t)s * TS + lane] = slot(si[e]);
+      if (venkat)
+        for (int q = 0; q < VQ && vnb[(size_t)q * N + i] >= 0; ++q)
+          vloc[((size_t)t * VQ + q) * TS + lane] = slot(vnb[(size_t)q * N + i]);
     }
     if (loc.size() > (size_t)0x7fffffff) {
-      R->use_lds = false;
-      uptr.clear(); uidx.clear(); lptr.clear(); loc.clear();
+      give_up();
       return;
     }
     lptr[t + 1] = (int)loc.size();
@@ -288,9 +306,43 @@ void build_tiles(Region* R, const std::vector<int>& sp, const std::vector<int>& 
   R->use_lds = true;
 }
 
+// Limiter neighbourhood of every cell from its cell-to-face list: the cell across
+// each interior face (in list order) and cofg - fc of that face.
+void build_limiter_geometry(Region* R, const std::vector<int>& c2f_ptr,
+                            const std::vector<int>& c2f_idx, const std::vector<int2>& f2c,
+                            const std::vector<double4>& fc, const std::vector<double4>& cell,
+                            std::vector<int>& vnb, std::vector<double>& vgeo) {
+  const int N = R->N, F = R->F;
+  int vq = 1;
+  for (int i = 0; i < N; ++i) {
+    int n = 0;
+    for (int e = c2f_ptr[i]; e < c2f_ptr[i + 1]; ++e) {
+      int s = c2f_idx[e];
+      if ((s > 0 ? s : -s) <= F) ++n;
+    }
+    vq = std::max(vq, n);
+  }
+  R->vq = vq;
+  vnb.assign((size_t)vq * N, -1);
+  vgeo.assign((size_t)3 * vq * N, 0.0);
+  for (int i = 0; i < N; ++i) {
+    int q = 0;
+    for (int e = c2f_ptr[i]; e < c2f_ptr[i + 1]; ++e) {
+      int s = c2f_idx[e];
+      int f = (s > 0 ? s : -s) - 1;
+      if (f >= F) continue;
+      vnb[(size_t)q * N + i] = s > 0 ? f2c[f].y : f2c[f].x;
+      vgeo[(size_t)(3 * q) * N + i] = cell[i].x - fc[f].x;
+      vgeo[(size_t)(3 * q + 1) * N + i] = cell[i].y - fc[f].y;
+      vgeo[(size_t)(3 * q + 2) * N + i] = cell[i].z - fc[f].z;
+      ++q;
+    }
+  }
+}
+
 // ---- kernel dispatch ------------------------------------------------------------
 
-template <typename T, typename GT, typename FT>
+template <typename T, typename GT, typename FT, bool SR>
 int launch_fluxes_t(Region* R) {
   Mesh M = R->mesh();
   State S = R->state();
@@ -298,9 +350,9 @@ int launch_fluxes_t(Region* R) {
   FT* fl = (FT*)R->flux;
   dim3 fb(R->face_block);
   if (R->F > 0) {
-#define RFLU_FACE(D, O)                                                                          \
-  hipLaunchKernelGGL((k_face_flux<D, O, T, GT, FT>), dim3(nblk(R->F, R->face_block)), fb, 0,   \
-                     R->stream, M, S, R->gas, g, fl, R->sc)
+#define RFLU_FACE(D, O)                                                                            \
+  hipLaunchKernelGGL((k_face_flux<D, O, T, GT, FT, SR>), dim3(nblk(R->F, R->face_block)), fb, 0, \
+                     R->stream, M, S, R->gas, g, fl, R->srad, R->sc)
     if (R->order == 1) {
       if (R->discr == 1) RFLU_FACE(1, 1);
       else if (R->discr == 3) RFLU_FACE(3, 1);
@@ -315,27 +367,38 @@ int launch_fluxes_t(Region* R) {
   return 0;
 }
 
-template <typename GT, typename FT>
+template <typename GT, typename FT, bool SR>
 int launch_bfaces_t(Region* R) {
   if (R->B == 0) return 0;
   Mesh M = R->mesh();
   State S = R->state();
   const GT* g = (const GT*)R->grad_final();
   if (R->order == 1)
-    hipLaunchKernelGGL((k_bface_flux<1, GT, FT>), dim3(nblk(R->B, 256)), dim3(256), 0, R->stream,
-                       M, S, R->gas, R->patches, g, (FT*)R->flux, R->sc);
+    hipLaunchKernelGGL((k_bface_flux<1, GT, FT, SR>), dim3(nblk(R->B, 256)), dim3(256), 0, R->stream,
+                       M, S, R->gas, R->patches, g, (FT*)R->flux, R->srad, R->sc);
   else
-    hipLaunchKernelGGL((k_bface_flux<2, GT, FT>), dim3(nblk(R->B, 256)), dim3(256), 0, R->stream,
-                       M, S, R->gas, R->patches, g, (FT*)R->flux, R->sc);
+    hipLaunchKernelGGL((k_bface_flux<2, GT, FT, SR>), dim3(nblk(R->B, 256)), dim3(256), 0, R->stream,
+                       M, S, R->gas, R->patches, g, (FT*)R->flux, R->srad, R->sc);
   return 0;
+}
+
+Tiles region_tiles(const Region* R) {
+  Tiles T;
+  T.uptr = R->tile_uptr; T.uidx = R->tile_uidx; T.lptr = R->tile_lptr; T.loc = R->tile_loc;
+  T.vloc = R->tile_vloc;
+  T.ts = GRAD_TILE; T.cap = R->tile_cap;
+  return T;
 }
 
 template <typename GT, bool V, int U>
 void launch_gradient_kernel(Region* R, const Mesh& M, const State& S) {
-  if (R->use_lds && R->grad_lds) {
-    Tiles T;
-    T.uptr = R->tile_uptr; T.uidx = R->tile_uidx; T.lptr = R->tile_lptr; T.loc = R->tile_loc;
-    T.ts = GRAD_TILE; T.cap = R->tile_cap;
+  if (R->use_lds && R->grad_lds && R->gw) {
+    Tiles T = region_tiles(R);
+    size_t shm = (size_t)5 * R->tile_cap * sizeof(double);
+    hipLaunchKernelGGL((k_gradient_ldsw<GT, V, 4 * U>), dim3(R->n_tiles), dim3(GRAD_TILE), shm,
+                       R->stream, M, S, T, R->tile_loc_len, (const GT*)R->gw, (GT*)R->grad, R->sc);
+  } else if (R->use_lds && R->grad_lds) {
+    Tiles T = region_tiles(R);
     size_t shm = (size_t)8 * R->tile_cap * sizeof(double);
     hipLaunchKernelGGL((k_gradient_lds<GT, V, U>), dim3(R->n_tiles), dim3(GRAD_TILE), shm,
                        R->stream, M, S, T, (GT*)R->grad, R->sc);
@@ -390,8 +453,8 @@ struct Timed {
   hipEvent_t a = nullptr, b = nullptr;
   Timed(Region* r, int k) : R(r), kid(k) {
     if (R->profiling) {
-      (void)hipEventCreate(&a);
-      (void)hipEventCreate(&b);
+      (void)hipEventCreateWithFlags(&a, hipEventDisableSystemFence);
+      (void)hipEventCreateWithFlags(&b, hipEventDisableSystemFence);
       (void)hipEventRecord(a, R->stream);
     }
   }
@@ -436,27 +499,35 @@ int launch_gradients(Region* R) {
   return rc;
 }
 
-int launch_fluxes(Region* R, int stage) {
+template <bool SR>
+int launch_fluxes_p(Region* R) {
   int rc;
   {
     Timed t(R, KID_FACE);
     switch (R->prec) {
-      case RFLU_PREC_FP64: rc = launch_fluxes_t<double, double, double>(R); break;
-      case RFLU_PREC_MIXED: rc = launch_fluxes_t<float, float, float>(R); break;
-      case RFLU_PREC_FP32_GRAD: rc = launch_fluxes_t<double, float, double>(R); break;
-      default: rc = launch_fluxes_t<float, double, float>(R); break;
+      case RFLU_PREC_FP64: rc = launch_fluxes_t<double, double, double, SR>(R); break;
+      case RFLU_PREC_MIXED: rc = launch_fluxes_t<float, float, float, SR>(R); break;
+      case RFLU_PREC_FP32_GRAD: rc = launch_fluxes_t<double, float, double, SR>(R); break;
+      default: rc = launch_fluxes_t<float, double, float, SR>(R); break;
     }
   }
   if (rc) return rc;
   {
     Timed t(R, KID_BFACE);
     switch (R->prec) {
-      case RFLU_PREC_FP64: rc = launch_bfaces_t<double, double>(R); break;
-      case RFLU_PREC_MIXED: rc = launch_bfaces_t<float, float>(R); break;
-      case RFLU_PREC_FP32_GRAD: rc = launch_bfaces_t<float, double>(R); break;
-      default: rc = launch_bfaces_t<double, float>(R); break;
+      case RFLU_PREC_FP64: rc = launch_bfaces_t<double, double, SR>(R); break;
+      case RFLU_PREC_MIXED: rc = launch_bfaces_t<float, float, SR>(R); break;
+      case RFLU_PREC_FP32_GRAD: rc = launch_bfaces_t<float, double, SR>(R); break;
+      default: rc = launch_bfaces_t<double, float, SR>(R); break;
     }
   }
+  return rc;
+}
+
+// srad: also store the face spectral radii of the time step (first stage of a fused
+// step, whose face pass sees the step-start state).
+int launch_fluxes(Region* R, int stage, bool srad = false) {
+  int rc = srad ? launch_fluxes_p<true>(R) : launch_fluxes_p<false>(R);
   if (rc) return rc;
   if (stage == 1 && R->has_outflow) {
     Mesh M = R->mesh();
@@ -484,25 +555,43 @@ int launch_update(Region* R, int stage, bool fused, double dt_in = -1.0) {
   return launch_update_rk<float, false, false>(R, stage, last, dt_in);
 }
 
-int launch_time_step(Region* R, double time_in = -1.0, double end_time = HUGE_VAL) {
-  Timed t(R, KID_DT);
+// Minimum local time step from the face spectral radii already in R->srad, then
+// RFLU_MinimumTimeStep's rules.
+int launch_dt_reduce(Region* R, double time_in, double end_time) {
   Mesh M = R->mesh();
-  State S = R->state();
-  hipLaunchKernelGGL(k_local_dt, dim3(R->n_dt_parts), dim3(256), 0, R->stream, M, S, R->dt_part, R->sc);
+  hipLaunchKernelGGL(k_dt_gather, dim3(R->n_dt_parts), dim3(256), 0, R->stream, M,
+                     (const double*)R->srad, R->dt_part, R->sc);
   hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, R->stream, (const double*)R->dt_part,
                      R->n_dt_parts, R->sc, R->cfl, R->dt_imposed, R->dt_min_limit, time_in,
                      end_time);
   return 0;
 }
 
+// Time step as a pass of its own (the unfused entry point).
+int launch_time_step(Region* R, double time_in = -1.0, double end_time = HUGE_VAL) {
+  Timed t(R, KID_DT);
+  Mesh M = R->mesh();
+  State S = R->state();
+  const int nf = R->F + R->B;
+  if (nf > 0)
+    hipLaunchKernelGGL(k_face_srad, dim3(nblk(nf, 256)), dim3(256), 0, R->stream, M, S, R->srad, R->sc);
+  return launch_dt_reduce(R, time_in, end_time);
+}
+
 int nstages(int rk) { return rk == 1 ? 4 : rk == 2 ? 3 : rk == 3 ? 1 : 3; }
 
+// One fused time step.  The time step only enters at the first update, so it is
+// reduced after the first stage's face pass, which also stores the face spectral
+// radii of the step-start state.
 int launch_step(Region* R) {
-  int rc = launch_time_step(R, -1.0, R->end_time);
-  if (rc) return rc;
+  int rc;
   for (int s = 1; s <= nstages(R->rk); ++s) {
     if ((rc = launch_gradients(R))) return rc;
-    if ((rc = launch_fluxes(R, s))) return rc;
+    if ((rc = launch_fluxes(R, s, s == 1))) return rc;
+    if (s == 1) {
+      Timed t(R, KID_DT);
+      if ((rc = launch_dt_reduce(R, -1.0, R->end_time))) return rc;
+    }
     if ((rc = launch_update(R, s, true))) return rc;
   }
   return 0;
@@ -661,8 +750,14 @@ int rflu_gpu_create_region(const rflu_region_desc* d, int32_t* handle) {
   // arithmetic is the same as in the original numbering.
   renumber(R, hcell, hf2c, hfn, hfc, hbf2c, hc2f_ptr, hc2f_idx, hsten_ptr, hsten_idx, hmoms, hepsq);
   std::vector<int> htile_uptr, htile_uidx, htile_lptr;
-  std::vector<unsigned short> htile_loc;
-  if (R->order == 2) build_tiles(R, hsten_ptr, hsten_idx, htile_uptr, htile_uidx, htile_lptr, htile_loc);
+  std::vector<unsigned short> htile_loc, htile_vloc;
+  std::vector<int> hvnb;
+  std::vector<double> hvgeo;
+  const bool venkat = R->order == 2 && R->reconst == RFLU_RECONST_LIM_VENKAT;
+  if (venkat) build_limiter_geometry(R, hc2f_ptr, hc2f_idx, hf2c, hfc, hcell, hvnb, hvgeo);
+  if (R->order == 2)
+    build_tiles(R, hsten_ptr, hsten_idx, hvnb, venkat, htile_uptr, htile_uidx, htile_lptr,
+                htile_loc, htile_vloc);
 
   // ---- device allocation ----
   int rc = 0;
@@ -677,7 +772,8 @@ int rflu_gpu_create_region(const rflu_region_desc* d, int32_t* handle) {
       (rc = dalloc(&R->c2f_ptr, N + 1)) || (rc = dalloc(&R->c2f_idx, hc2f_idx.size())) ||
       (rc = dalloc(&R->cv, 5 * (size_t)N)) || (rc = dalloc(&R->rhs_sum, 5 * (size_t)N)) ||
       (rc = dalloc(&R->dv, 3 * (size_t)N)) || (rc = dalloc(&R->rhs, 5 * (size_t)N)) ||
-      (rc = dalloc(&R->dt_part, R->n_dt_parts)) || (rc = dalloc(&R->sc, 1))) {
+      (rc = dalloc(&R->dt_part, R->n_dt_parts)) || (rc = dalloc(&R->sc, 1)) ||
+      (rc = dalloc(&R->srad, FB))) {
     free_region(R);
     return rc;
   }
@@ -718,18 +814,28 @@ int rflu_gpu_create_region(const rflu_region_desc* d, int32_t* handle) {
       free_region(R);
       return rc;
     }
-    if (R->use_lds) {
-      if ((rc = dalloc(&R->tile_uptr, htile_uptr.size())) || (rc = dalloc(&R->tile_uidx, htile_uidx.size())) ||
-          (rc = dalloc(&R->tile_lptr, htile_lptr.size())) || (rc = dalloc(&R->tile_loc, htile_loc.size())) ||
-          (rc = dput(R->tile_uptr, htile_uptr, R->stream)) || (rc = dput(R->tile_uidx, htile_uidx, R->stream)) ||
-          (rc = dput(R->tile_lptr, htile_lptr, R->stream)) || (rc = dput(R->tile_loc, htile_loc, R->stream))) {
+    if (venkat) {
+      if ((rc = dalloc(&R->vnb, hvnb.size())) || (rc = dalloc(&R->vgeo, hvgeo.size())) ||
+          (rc = dput(R->vnb, hvnb, R->stream)) || (rc = dput(R->vgeo, hvgeo, R->stream))) {
         free_region(R);
         return rc;
       }
     }
+    if (R->use_lds) {
+      if ((rc = dalloc(&R->tile_uptr, htile_uptr.size())) || (rc = dalloc(&R->tile_uidx, htile_uidx.size())) ||
+          (rc = dalloc(&R->tile_lptr, htile_lptr.size())) || (rc = dalloc(&R->tile_loc, htile_loc.size())) ||
+          (rc = dput(R->tile_uptr, htile_uptr, R->stream)) || (rc = dput(R->tile_uidx, htile_uidx, R->stream)) ||
+          (rc = dput(R->tile_lptr, htile_lptr, R->stream)) || (rc = dput(R->tile_loc, htile_loc, R->stream)) ||
+          (rc = dalloc(&R->tile_vloc, htile_vloc.size())) || (rc = dput(R->tile_vloc, htile_vloc, R->stream))) {
+        free_region(R);
+        return rc;
+      }
+      R->tile_loc_len = htile_loc.size();
+    }
   }
   hipLaunchKernelGGL(k_fill_scalars, dim3(1), dim3(64), 0, R->stream, R->sc, 0.0);
-  if (hipStreamSynchronize(R->stream) != hipSuccess) {
+  if (hipMemsetAsync(R->srad, 0, FB * sizeof(double), R->stream) != hipSuccess ||
+      hipStreamSynchronize(R->stream) != hipSuccess) {
     free_region(R);
     return fail(RFLU_ERR_HIP, "region upload failed");
   }
@@ -1038,6 +1144,42 @@ int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
       if (value != 1 && value != 2 && value != 4) return fail(RFLU_ERR_ARG, "GRAD_UNROLL is 1, 2 or 4");
       R->grad_unroll = value;
       break;
+    case RFLU_OPT_XCD:
+      if (value < 1 || value > 64) return fail(RFLU_ERR_ARG, "XCD is 1..64");
+      R->nxcd = value;
+      break;
+    case RFLU_OPT_GRAD_WEIGHTS: {
+      if (value != 0 && value != 1) return fail(RFLU_ERR_ARG, "GRAD_WEIGHTS is 0 or 1");
+      if (value == 0) {
+        if (R->gw) {
+          HIP_TRY(hipStreamSynchronize(R->stream));
+          HIP_TRY(hipFree(R->gw));
+          R->gw = nullptr;
+        }
+        break;
+      }
+      if (R->gw) break;
+      if (R->order != 2 || !R->use_lds)
+        return fail(RFLU_ERR_ARG, "GRAD_WEIGHTS needs a second-order region with LDS tiles");
+      const size_t gsz = grad32(R) ? sizeof(float) : sizeof(double);
+      const size_t L = R->tile_loc_len;
+      hipError_t e = hipMalloc(&R->gw, 3 * L * gsz + 16);
+      if (e != hipSuccess) {
+        R->gw = nullptr;
+        return fail(RFLU_ERR_NOMEM, std::string("hipMalloc stencil weights: ") + hipGetErrorString(e));
+      }
+      Mesh M = R->mesh();
+      Tiles T = region_tiles(R);
+      if (grad32(R))
+        hipLaunchKernelGGL((k_grad_weights<float>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream,
+                           M, T, L, (float*)R->gw);
+      else
+        hipLaunchKernelGGL((k_grad_weights<double>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream,
+                           M, T, L, (double*)R->gw);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipStreamSynchronize(R->stream));
+      break;
+    }
     default:
       return fail(RFLU_ERR_ARG, "unknown option");
   }
@@ -1053,6 +1195,8 @@ int rflu_gpu_get_option(int32_t h, int32_t option, int32_t* value) {
     case RFLU_OPT_GRAD_LDS: *value = (R->use_lds && R->grad_lds) ? 1 : 0; break;
     case RFLU_OPT_GRAD_UNROLL: *value = R->grad_unroll; break;
     case RFLU_OPT_TILE_CAP: *value = R->tile_cap; break;
+    case RFLU_OPT_XCD: *value = R->nxcd; break;
+    case RFLU_OPT_GRAD_WEIGHTS: *value = R->gw ? 1 : 0; break;
     default: return fail(RFLU_ERR_ARG, "unknown option");
   }
   return RFLU_OK;

@@ -71,7 +71,25 @@ struct Mesh {
   const int* sten_idx;
   const double* moms;      // [6][N]: r11 r12 r22 r13 r23 r33
   const double* epsq;      // [N] Venkatakrishnan eps^2 = (vol^(1/3))^3
+  // Limiter neighbourhood: for the q-th interior face of cell i (cell-to-face order),
+  // vnb[q*N+i] is the cell across it (-1 past the last one) and vgeo[(3q+d)*N+i] the
+  // component d of cofg(i) - fc(face).
+  int vq;                  // slots per cell (largest interior-face count)
+  const int* vnb;
+  const double* vgeo;
+  int nxcd;                // blocks are spread over this many XCDs (1: no remap)
 };
+
+// Logical block index for an XCD-aware launch.  The dispatcher hands consecutive
+// workgroups to the XCDs in turn; remapping gives each XCD one contiguous range of
+// cells or faces, so neighbouring blocks share that XCD's L2.  A bijection on
+// [0, gridDim.x) for any grid size.
+__device__ __forceinline__ int xcd_block(int nxcd) {
+  const int b = (int)blockIdx.x, n = (int)gridDim.x;
+  if (nxcd <= 1) return b;
+  const int x = b % nxcd, l = b / nxcd, q = n / nxcd, r = n % nxcd;
+  return x < r ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
+}
 
 struct State {
   double* cv;      // [5][N]
@@ -264,6 +282,51 @@ __device__ __forceinline__ void recon_state(const State& S, const Mesh& M, const
   ir = T(ird);
 }
 
+// Spectral radius (|v.n| + c)|A| of every face, interior faces first, then boundary
+// faces (rocflu/RFLU_TimeStepInviscid.F90:165-304); velocities are m*(1/rho).
+__device__ __forceinline__ double srad_interior(double u1, double v1, double w1, double a1,
+                                                double u2, double v2, double w2, double a2,
+                                                double4 n) {
+  double vc = 0.5 * ((u1 + u2) * n.x + (v1 + v2) * n.y + (w1 + w2) * n.z);
+  double as = 0.5 * (a1 + a2);
+  return (fabs(vc) + as) * n.w;
+}
+__device__ __forceinline__ double srad_boundary(double u1, double v1, double w1, double a1,
+                                                double4 n) {
+  double vc = u1 * n.x + v1 * n.y + w1 * n.z;
+  return (fabs(vc) + a1) * n.w;
+}
+
+// Velocity of cell c from cv (order 1) or from the primitive copy (order 2), which
+// holds (1/rho)*m with the same rounding.
+template <int ORDER>
+__device__ __forceinline__ void cell_velocity(const State& S, int N, int c, double& u, double& v,
+                                              double& w) {
+  if (ORDER == 2) {
+    u = S.prim[(size_t)N + c]; v = S.prim[2 * (size_t)N + c]; w = S.prim[3 * (size_t)N + c];
+  } else {
+    double ir = 1.0 / S.cv[c];
+    u = S.cv[(size_t)N + c] * ir; v = S.cv[2 * (size_t)N + c] * ir; w = S.cv[3 * (size_t)N + c] * ir;
+  }
+}
+
+template <int ORDER>
+__device__ __forceinline__ double face_srad(const Mesh& M, const State& S, int f) {
+  const int N = M.N;
+  double u1, v1, w1, u2, v2, w2;
+  if (f < M.F) {
+    int2 cc = M.f2c[f];
+    cell_velocity<ORDER>(S, N, cc.x, u1, v1, w1);
+    cell_velocity<ORDER>(S, N, cc.y, u2, v2, w2);
+    return srad_interior(u1, v1, w1, S.dv[2 * (size_t)N + cc.x], u2, v2, w2,
+                         S.dv[2 * (size_t)N + cc.y], M.fn[f]);
+  }
+  int b = f - M.F;
+  int c = M.bf2c[b];
+  cell_velocity<ORDER>(S, N, c, u1, v1, w1);
+  return srad_boundary(u1, v1, w1, S.dv[2 * (size_t)N + c], M.bfn[b]);
+}
+
 // Flux of interior face f (computed with f2c(1,f) as the left state).
 template <int DISCR, int ORDER, typename T, typename GT>
 __device__ __forceinline__ void interior_face_flux(const Mesh& M, const State& S, const Gas& G,
@@ -330,12 +393,16 @@ __device__ __forceinline__ void interior_face_flux(const Mesh& M, const State& S
   }
 }
 
-template <int DISCR, int ORDER, typename T, typename GT, typename FT>
+// SRAD: also store the face's spectral radius for the time step (the first stage of
+// a fused step sees the step-start state, which is what the time step reads).
+template <int DISCR, int ORDER, typename T, typename GT, typename FT, bool SRAD>
 __global__ void k_face_flux(Mesh M, State S, Gas G, const GT* __restrict__ grad,
-                            FT* __restrict__ flux, const Scalars* __restrict__ sc) {
+                            FT* __restrict__ flux, double* __restrict__ srad,
+                            const Scalars* __restrict__ sc) {
   if (halted(sc)) return;
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  int f = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
   if (f >= M.F) return;
+  if (SRAD) srad[f] = face_srad<ORDER>(M, S, f);
   T flx[5];
   interior_face_flux<DISCR, ORDER, T, GT>(M, S, G, grad, f, flx);
   const size_t FB = (size_t)M.F + M.B;
@@ -377,12 +444,14 @@ __device__ __forceinline__ void bcond_outflow(int bcOpt, double pout, double sxn
 // Boundary faces (rocflu/RFLU_CentralFirstPatch.F90:571-906,
 // rocflu/RFLU_CentralSecondPatch.F90:634-1024).  Slip walls store a zero mass flux.
 // Boundary fluxes stay FP64 in every precision mode (they are few).
-template <int ORDER, typename GT, typename FT>
+template <int ORDER, typename GT, typename FT, bool SRAD>
 __global__ void k_bface_flux(Mesh M, State S, Gas G, PatchTable P, const GT* __restrict__ grad,
-                             FT* __restrict__ flux, const Scalars* __restrict__ sc) {
+                             FT* __restrict__ flux, double* __restrict__ srad,
+                             const Scalars* __restrict__ sc) {
   if (halted(sc)) return;
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= M.B) return;
+  if (SRAD) srad[M.F + b] = face_srad<ORDER>(M, S, M.F + b);
   const int N = M.N;
   int c1 = M.bf2c[b];
   int p = M.bpatch[b];
@@ -611,7 +680,7 @@ __global__ void k_update(Mesh M, State S, Gas G, const FT* __restrict__ flux, Sc
                          int stage, int last_stage, double dt_in) {
   if (halted(sc)) return;
   const int N = M.N;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
   double dt = dt_in > 0.0 ? dt_in : sc->dt;
   if (last_stage && FUSED && i == 0) {
     sc->time = sc->time + dt;
@@ -767,53 +836,45 @@ __device__ __forceinline__ void grad_member(const GradCoef<T>& c, double ddx, do
   grad_accum<T>(w, dvar, g);
 }
 
-// Venkatakrishnan limiter of cell i in FP64: varMin/varMax over interior-face
+// Venkatakrishnan limiter of cell i in FP64: varMin/varMax over the interior-face
 // neighbours, then the face-wise limiter with dx = cofg - fc (the reference's sign).
-template <typename T>
-__device__ __forceinline__ void venkat_limit(const Mesh& M, const State& S, int i, double4 ci,
-                                             const double* vi, T* g) {
+// nbr(q, v) loads the q-th neighbour's (rho,u,v,w,p) and returns false past the last
+// one.  MIN/MAX reductions do not depend on the face order, so the precomputed
+// neighbour slots give the reference's values exactly.
+template <typename T, typename Fetch>
+__device__ __forceinline__ void venkat_limit(const Mesh& M, int i, const double* vi, T* g,
+                                             Fetch nbr) {
   const int N = M.N;
   const double TINY = 1.0E-12;
   double vmn[5], vmx[5], lim[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) { vmn[k] = vi[k]; vmx[k] = vi[k]; lim[k] = 1.0; }
-  int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
-  for (int e = e0; e < e1; ++e) {
-    int sgn = M.c2f_idx[e];
-    int f = (sgn > 0 ? sgn : -sgn) - 1;
-    if (f >= M.F) continue;
-    int2 cc = M.f2c[f];
-    int o = sgn > 0 ? cc.y : cc.x;
+  int nv = 0;
+  for (int q = 0; q < M.vq; ++q) {
+    double vo[5];
+    if (!nbr(q, vo)) break;
+    ++nv;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      double vo = S.prim[(size_t)k * N + o];
-      vmn[k] = vmn[k] < vo ? vmn[k] : vo;
-      vmx[k] = vmx[k] > vo ? vmx[k] : vo;
+      vmn[k] = vmn[k] < vo[k] ? vmn[k] : vo[k];
+      vmx[k] = vmx[k] > vo[k] ? vmx[k] : vo[k];
     }
   }
   double epsq = M.epsq[i];
-  for (int e = e0; e < e1; ++e) {
-    int sgn = M.c2f_idx[e];
-    int f = (sgn > 0 ? sgn : -sgn) - 1;
-    if (f >= M.F) continue;
-    double4 xf = M.fc[f];
-    double dx = ci.x - xf.x, dy = ci.y - xf.y, dz = ci.z - xf.z;
+  for (int q = 0; q < nv; ++q) {
+    double dx = M.vgeo[(size_t)(3 * q) * N + i];
+    double dy = M.vgeo[(size_t)(3 * q + 1) * N + i];
+    double dz = M.vgeo[(size_t)(3 * q + 2) * N + i];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      double dmn = vmn[k] - vi[k];
-      double dmx = vmx[k] - vi[k];
       double d2 = double(g[3 * k]) * dx + double(g[3 * k + 1]) * dy + double(g[3 * k + 2]) * dz;
-      if (d2 > 0.0) {
-        double numer = (dmx * dmx + epsq) * d2 + 2 * d2 * d2 * dmx;
-        double denom = d2 * (dmx * dmx + 2 * d2 * d2 + dmx * d2 + epsq);
-        double term = numer / (denom + TINY);
-        lim[k] = term < lim[k] ? term : lim[k];
-      } else if (d2 < 0.0) {
-        double numer = (dmn * dmn + epsq) * d2 + 2 * d2 * d2 * dmn;
-        double denom = d2 * (dmn * dmn + 2 * d2 * d2 + dmn * d2 + epsq);
-        double term = numer / (denom + TINY);
-        lim[k] = term < lim[k] ? term : lim[k];
-      }
+      // the reference's two branches (d2 > 0 with varMax, d2 < 0 with varMin) share one
+      // expression; selecting the operand keeps the lanes of a wavefront together
+      double dm = d2 > 0.0 ? vmx[k] - vi[k] : vmn[k] - vi[k];
+      double numer = (dm * dm + epsq) * d2 + 2 * d2 * d2 * dm;
+      double denom = d2 * (dm * dm + 2 * d2 * d2 + dm * d2 + epsq);
+      double term = numer / (denom + TINY);
+      if (d2 != 0.0) lim[k] = term < lim[k] ? term : lim[k];
     }
   }
 #pragma unroll
@@ -827,7 +888,7 @@ template <typename GT, bool VENKAT, int U>
 __global__ void k_gradient(Mesh M, State S, GT* __restrict__ grad, const Scalars* __restrict__ sc) {
   if (halted(sc)) return;
   const int N = M.N;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
   if (i >= N) return;
   GradCoef<GT> c = grad_coef<GT>(M, i);
   double4 ci = M.cell[i];
@@ -861,23 +922,72 @@ __global__ void k_gradient(Mesh M, State S, GT* __restrict__ grad, const Scalars
     for (int k = 0; k < 5; ++k) dvar[k] = S.prim[(size_t)k * N + j] - vi[k];
     grad_member<GT>(c, cj.x - ci.x, cj.y - ci.y, cj.z - ci.z, dvar, g);
   }
-  if (VENKAT) venkat_limit<GT>(M, S, i, ci, vi, g);
+  if (VENKAT)
+    venkat_limit<GT>(M, i, vi, g, [&](int q, double* vo) {
+      int o = M.vnb[(size_t)q * N + i];
+      if (o < 0) return false;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) vo[k] = S.prim[(size_t)k * N + o];
+      return true;
+    });
 #pragma unroll
   for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
 }
 
-// Stencil tiles: the union of the stencil members of TS consecutive cells is staged
-// in LDS once per tile (centroid and primitive state, 8 doubles per member), and
-// each lane then walks its own stencil, in the reference's member order, reading
-// LDS slots through a coalesced [member][lane] table of 16-bit slot numbers.
+// Stencil tiles: the union of the stencil members and face neighbours of TS
+// consecutive cells is staged in LDS once per tile (centroid and primitive state, 8
+// doubles per member), and each lane then walks its own stencil, in the reference's
+// member order, reading LDS slots through a coalesced [member][lane] table of 16-bit
+// slot numbers; vloc[(q*TS) + lane] (per tile) holds the slot of the lane's q-th
+// limiter neighbour (0xFFFF past the last).
 struct Tiles {
   const int* uptr;
   const int* uidx;
   const int* lptr;
   const unsigned short* loc;
+  const unsigned short* vloc;
   int ts;
   int cap;
 };
+
+// Stages a tile's union records in LDS: each thread issues the loads of up to four
+// records before writing any of them, so the gathers overlap.  Layout: x, y, z (with
+// COORDS) then rho, u, v, w, p, each an array of cap doubles.
+template <bool COORDS>
+__device__ __forceinline__ void stage_tile(const Mesh& M, const State& S, const int* uidx, int nu,
+                                           int cap, double* lds) {
+  const int N = M.N;
+  double* lv = lds + (COORDS ? 3 * cap : 0);
+  for (int k0 = threadIdx.x; k0 < nu; k0 += 4 * blockDim.x) {
+    int jj[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int k = k0 + r * (int)blockDim.x;
+      jj[r] = k < nu ? uidx[k] : -1;
+    }
+    double4 cc[4];
+    double pp[4][5];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int j = jj[r] >= 0 ? jj[r] : 0;
+      if (COORDS) cc[r] = M.cell[j];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) pp[r][q] = S.prim[(size_t)q * N + j];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (jj[r] < 0) continue;
+      const int k = k0 + r * (int)blockDim.x;
+      if (COORDS) {
+        lds[k] = cc[r].x;
+        lds[cap + k] = cc[r].y;
+        lds[2 * cap + k] = cc[r].z;
+      }
+#pragma unroll
+      for (int q = 0; q < 5; ++q) lv[q * cap + k] = pp[r][q];
+    }
+  }
+}
 
 template <typename GT, bool VENKAT, int U>
 __global__ __launch_bounds__(128) void k_gradient_lds(Mesh M, State S, Tiles T,
@@ -887,43 +997,49 @@ __global__ __launch_bounds__(128) void k_gradient_lds(Mesh M, State S, Tiles T,
   if (halted(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
   const int N = M.N;
   const int cap = T.cap;
-  double* lx = lds;
-  double* ly = lds + cap;
-  double* lz = lds + 2 * cap;
-  double* lv = lds + 3 * cap;  // 5 arrays: rho, u, v, w, p
-  const int t = blockIdx.x;
-  const int u0 = T.uptr[t], nu = T.uptr[t + 1] - u0;
-  for (int k = threadIdx.x; k < nu; k += blockDim.x) {
-    int j = T.uidx[u0 + k];
-    double4 cj = M.cell[j];
-    lx[k] = cj.x;
-    ly[k] = cj.y;
-    lz[k] = cj.z;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) lv[q * cap + k] = S.prim[(size_t)q * N + j];
-  }
-  __syncthreads();
+  const double* lx = lds;
+  const double* ly = lds + cap;
+  const double* lz = lds + 2 * cap;
+  const double* lv = lds + 3 * cap;  // 5 arrays: rho, u, v, w, p
+  const int t = xcd_block(M.nxcd);
+  // the cell's own data and first slot numbers are requested before the staging
   const int i = t * T.ts + (int)threadIdx.x;
-  if (i >= N) return;
-  GradCoef<GT> c = grad_coef<GT>(M, i);
-  double4 ci = M.cell[i];
+  const bool live = i < N;
+  const int ic = live ? i : N - 1;
+  GradCoef<GT> c = grad_coef<GT>(M, ic);
+  const double4 ci = M.cell[ic];
   double vi[5];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) vi[k] = S.prim[(size_t)k * N + i];
+  for (int k = 0; k < 5; ++k) vi[k] = S.prim[(size_t)k * N + ic];
+  const int cnt = live ? M.sten_ptr[ic + 1] - M.sten_ptr[ic] : 0;
+  const unsigned short* L = T.loc + T.lptr[t] + threadIdx.x;
+  // Slot numbers are loaded one group of U members ahead of their use.
+  int nk[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) nk[u] = u < cnt ? (int)L[(size_t)u * T.ts] : 0;
+  const int u0 = T.uptr[t];
+  stage_tile<true>(M, S, T.uidx + u0, T.uptr[t + 1] - u0, cap, lds);
+  __syncthreads();
+  if (!live) return;
   GT g[15];
 #pragma unroll
   for (int k = 0; k < 15; ++k) g[k] = GT(0);
-  const int cnt = M.sten_ptr[i + 1] - M.sten_ptr[i];
-  const unsigned short* L = T.loc + T.lptr[t] + threadIdx.x;
   int s = 0;
   // U members' weights are formed together (independent sqrt/divide chains), then
   // accumulated one member after the other in stencil order.
   for (; s + U <= cnt; s += U) {
+    int kk[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      kk[u] = nk[u];
+      int sn = s + U + u;
+      nk[u] = sn < cnt ? (int)L[(size_t)sn * T.ts] : 0;
+    }
     GT w[U][3];
     double dvar[U][5];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      int k = L[(size_t)(s + u) * T.ts];
+      int k = kk[u];
       grad_weights<GT>(c, lx[k] - ci.x, ly[k] - ci.y, lz[k] - ci.z, w[u]);
 #pragma unroll
       for (int q = 0; q < 5; ++q) dvar[u][q] = lv[q * cap + k] - vi[q];
@@ -931,14 +1047,130 @@ __global__ __launch_bounds__(128) void k_gradient_lds(Mesh M, State S, Tiles T,
 #pragma unroll
     for (int u = 0; u < U; ++u) grad_accum<GT>(w[u], dvar[u], g);
   }
-  for (; s < cnt; ++s) {
-    int k = L[(size_t)s * T.ts];
-    double dvar[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) dvar[q] = lv[q * cap + k] - vi[q];
-    grad_member<GT>(c, lx[k] - ci.x, ly[k] - ci.y, lz[k] - ci.z, dvar, g);
+  for (int u = 0; u < U; ++u) {
+    if (s + u < cnt) {
+      int k = nk[u];
+      double dvar[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) dvar[q] = lv[q * cap + k] - vi[q];
+      grad_member<GT>(c, lx[k] - ci.x, ly[k] - ci.y, lz[k] - ci.z, dvar, g);
+    }
   }
-  if (VENKAT) venkat_limit<GT>(M, S, i, ci, vi, g);
+  if (VENKAT) {
+    const unsigned short* V = T.vloc + (size_t)t * M.vq * T.ts + threadIdx.x;
+    venkat_limit<GT>(M, i, vi, g, [&](int q, double* vo) {
+      int k = V[(size_t)q * T.ts];
+      if (k == 0xFFFF) return false;
+#pragma unroll
+      for (int r = 0; r < 5; ++r) vo[r] = lv[r * cap + k];
+      return true;
+    });
+  }
+#pragma unroll
+  for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
+}
+
+// Stencil weights of every (cell, member) pair, formed once per region with the same
+// arithmetic as the member loop (the weights depend on the grid only).  Stored in
+// the tiles' [member][lane] order: w[d*L + lptr[t] + s*TS + lane] for component d.
+template <typename GT>
+__global__ void k_grad_weights(Mesh M, Tiles T, size_t L, GT* __restrict__ w) {
+  const int N = M.N;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int t = i / T.ts, lane = i - t * T.ts;
+  GradCoef<GT> c = grad_coef<GT>(M, i);
+  double4 ci = M.cell[i];
+  const size_t base = (size_t)T.lptr[t] + lane;
+  const int s0 = M.sten_ptr[i], s1 = M.sten_ptr[i + 1];
+  for (int s = s0; s < s1; ++s) {
+    double4 cj = M.cell[M.sten_idx[s]];
+    GT ww[3];
+    grad_weights<GT>(c, cj.x - ci.x, cj.y - ci.y, cj.z - ci.z, ww);
+    size_t o = base + (size_t)(s - s0) * T.ts;
+    w[o] = ww[0];
+    w[L + o] = ww[1];
+    w[2 * L + o] = ww[2];
+  }
+}
+
+// Gradient with precomputed weights: only the members' primitive state is staged
+// in LDS (5 doubles per record), weights stream from HBM in [member][lane] order
+// through a ring of P members loaded ahead of their use, and the accumulation runs
+// in stencil order as in the member loop above.
+template <typename GT, bool VENKAT, int P>
+__global__ __launch_bounds__(128) void k_gradient_ldsw(Mesh M, State S, Tiles T, size_t L,
+                                                       const GT* __restrict__ w,
+                                                       GT* __restrict__ grad,
+                                                       const Scalars* __restrict__ sc) {
+  extern __shared__ double lds[];
+  if (halted(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
+  const int N = M.N;
+  const int cap = T.cap;
+  const double* lv = lds;  // 5 arrays: rho, u, v, w, p
+  const int t = xcd_block(M.nxcd);
+  const int i = t * T.ts + (int)threadIdx.x;
+  const bool live = i < N;
+  const int ic = live ? i : N - 1;
+  double vi[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) vi[k] = S.prim[(size_t)k * N + ic];
+  const int cnt = live ? M.sten_ptr[ic + 1] - M.sten_ptr[ic] : 0;
+  const size_t base = (size_t)T.lptr[t] + threadIdx.x;
+  const unsigned short* Lp = T.loc + base;
+  const GT* W = w + base;
+  int rk[P];
+  GT rw[P][3];
+  auto fetch = [&](int m, int sn) {
+    bool ok = sn < cnt;
+    size_t o = (size_t)(ok ? sn : 0) * T.ts;
+    rk[m] = ok ? (int)Lp[o] : 0;
+    rw[m][0] = ok ? W[o] : GT(0);
+    rw[m][1] = ok ? W[L + o] : GT(0);
+    rw[m][2] = ok ? W[2 * L + o] : GT(0);
+  };
+#pragma unroll
+  for (int m = 0; m < P; ++m) fetch(m, m);
+  const int u0 = T.uptr[t];
+  stage_tile<false>(M, S, T.uidx + u0, T.uptr[t + 1] - u0, cap, lds);
+  __syncthreads();
+  if (!live) return;
+  GT g[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) g[k] = GT(0);
+  int s = 0;
+  for (; s + P <= cnt; s += P) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      const int k = rk[m];
+      GT ww[3] = {rw[m][0], rw[m][1], rw[m][2]};
+      fetch(m, s + P + m);
+      double dvar[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) dvar[q] = lv[q * cap + k] - vi[q];
+      grad_accum<GT>(ww, dvar, g);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    if (s + m < cnt) {
+      double dvar[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) dvar[q] = lv[q * cap + rk[m]] - vi[q];
+      grad_accum<GT>(rw[m], dvar, g);
+    }
+  }
+  if (VENKAT) {
+    const unsigned short* V = T.vloc + (size_t)t * M.vq * T.ts + threadIdx.x;
+    venkat_limit<GT>(M, i, vi, g, [&](int q, double* vo) {
+      int k = V[(size_t)q * T.ts];
+      if (k == 0xFFFF) return false;
+#pragma unroll
+      for (int r = 0; r < 5; ++r) vo[r] = lv[r * cap + k];
+      return true;
+    });
+  }
 #pragma unroll
   for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
 }
@@ -950,7 +1182,7 @@ __global__ void k_weno_clip(Mesh M, State S, const GT* __restrict__ gin, GT* __r
                             const Scalars* __restrict__ sc) {
   if (halted(sc)) return;
   const int N = M.N;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
   if (i >= N) return;
   typedef GT T;
   const T eps = T(1.0E-15);
@@ -1006,38 +1238,30 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
-__global__ void k_local_dt(Mesh M, State S, double* __restrict__ part, const Scalars* __restrict__ sc) {
+// Face spectral radii as a pass of their own (the unfused time-step entry point).
+__global__ void k_face_srad(Mesh M, State S, double* __restrict__ srad,
+                            const Scalars* __restrict__ sc) {
+  if (halted(sc)) return;
+  int f = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
+  if (f >= M.F + M.B) return;
+  srad[f] = face_srad<1>(M, S, f);
+}
+
+// Local time step vol/sum(srad) per cell, summed over the cell-to-face list in the
+// reference's scatter order, block minimum into part[].
+__global__ void k_dt_gather(Mesh M, const double* __restrict__ srad, double* __restrict__ part,
+                            const Scalars* __restrict__ sc) {
   __shared__ double red[16];
   const int N = M.N;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int blk = xcd_block(M.nxcd);
+  int i = blk * blockDim.x + threadIdx.x;
   double dtc = HUGE_VAL;
   if (i < N && !halted(sc)) {
     double sum = 0.0;
     int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
     for (int e = e0; e < e1; ++e) {
       int sgn = M.c2f_idx[e];
-      int f = (sgn > 0 ? sgn : -sgn) - 1;
-      if (f < M.F) {
-        int2 cc = M.f2c[f];
-        int c1 = cc.x, c2 = cc.y;
-        double4 n = M.fn[f];
-        double ir1 = 1.0 / S.cv[c1], ir2 = 1.0 / S.cv[c2];
-        double u1 = S.cv[(size_t)N + c1] * ir1, v1 = S.cv[2 * (size_t)N + c1] * ir1,
-               w1 = S.cv[3 * (size_t)N + c1] * ir1, a1 = S.dv[2 * (size_t)N + c1];
-        double u2 = S.cv[(size_t)N + c2] * ir2, v2 = S.cv[2 * (size_t)N + c2] * ir2,
-               w2 = S.cv[3 * (size_t)N + c2] * ir2, a2 = S.dv[2 * (size_t)N + c2];
-        double vc = 0.5 * ((u1 + u2) * n.x + (v1 + v2) * n.y + (w1 + w2) * n.z);
-        double as = 0.5 * (a1 + a2);
-        double srad = (fabs(vc) + as) * n.w;
-        sum = sum + srad;
-      } else {
-        double4 n = M.bfn[f - M.F];
-        double ir1 = 1.0 / S.cv[i];
-        double u1 = S.cv[(size_t)N + i] * ir1, v1 = S.cv[2 * (size_t)N + i] * ir1,
-               w1 = S.cv[3 * (size_t)N + i] * ir1, a1 = S.dv[2 * (size_t)N + i];
-        double vc = u1 * n.x + v1 * n.y + w1 * n.z;
-        sum = sum + (fabs(vc) + a1) * n.w;
-      }
+      sum = sum + srad[(sgn > 0 ? sgn : -sgn) - 1];
     }
     dtc = M.cell[i].w / sum;
   }
@@ -1048,7 +1272,7 @@ __global__ void k_local_dt(Mesh M, State S, double* __restrict__ part, const Sca
   if (threadIdx.x == 0) {
     double m = red[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmin(m, red[w]);
-    part[blockIdx.x] = m;
+    part[blk] = m;
   }
 }
 

@@ -212,3 +212,45 @@ def test_gpu_matches_reference_golden_gas_jump(gpu):
     r.close()
     assert n == 569
     assert np.array_equal(cve, cvf)
+
+
+@pytest.mark.parametrize("reconst", [11, 2])
+def test_tuning_options_do_not_change_results(gpu, reconst):
+    """Every tuning option (LDS tiles or global gathers, member unroll, XCD-aware block
+    order, launch sizes) gives the same bits, and the same bits as the oracle's run."""
+    m = small_tet(n=10, bc=rmesh.BC_OUTFLOW)
+    num = Numerics(discr=3, order=2, reconst=reconst, rk=2, cfl=0.5, dt_imposed=1e-3)
+    cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=3)
+    o = Oracle(m, num)
+    cvo, _, st = o.run(cv0, 8)
+    assert st["rc"] == 0
+    ref = None
+    variants = [dict(lds=1, unroll=4, xcd=8), dict(lds=0, unroll=4, xcd=8),
+                dict(lds=1, unroll=1, xcd=1), dict(lds=1, unroll=2, xcd=3),
+                dict(lds=0, unroll=2, xcd=1, blocks=(64, 512, 128)),
+                dict(lds=1, unroll=4, xcd=8, weights=1), dict(lds=1, unroll=1, xcd=1, weights=1),
+                dict(lds=1, unroll=2, xcd=8, weights=1)]
+    for v in variants:
+        print("variant", v, flush=True)
+        g = GpuRegion(m, num)
+        try:
+            g.set_option(GpuRegion.OPT_GRAD_LDS, v["lds"])
+            g.set_option(GpuRegion.OPT_GRAD_UNROLL, v["unroll"])
+            g.set_option(GpuRegion.OPT_XCD, v["xcd"])
+            if "blocks" in v:
+                g.set_launch(*v["blocks"])
+            if v.get("weights"):
+                g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, 1)
+                assert g.get_option(GpuRegion.OPT_GRAD_WEIGHTS) == 1
+            g.upload(cv0)
+            g.time_steps(8, graph=True)
+            cvg, _ = g.download()
+        finally:
+            g.close()
+        if ref is None:
+            ref = cvg
+            err = rel_l2(cvg, cvo)
+            print("rel L2 vs oracle:", err, flush=True)
+            assert np.all(err <= TOL_FP64), err
+        else:
+            assert np.array_equal(cvg, ref), v
