@@ -114,6 +114,18 @@ int rflu_gpu_device_count(void);
 int rflu_gpu_create_region(const rflu_region_desc* desc, int32_t* handle);
 int rflu_gpu_destroy_region(int32_t handle);
 
+/* A batch of n_cases independent cases (descs[0..n_cases-1]) in one handle, the
+ * batched form of the reference's loop over regions (RungeKuttaMP.F90:148-508 runs
+ * each callee per region; here one launch covers every case).  Cases may differ in
+ * mesh, gas constants, CFL, dt controls and patches but share discr, order,
+ * reconst, rk and precision.  Each case keeps its own time, dt (its own
+ * RFLU_MinimumTimeStep), end time and checks: a case that fails a check or reaches
+ * its end time stops while the others go on.  State arrays of the handle are the
+ * cases' arrays concatenated in case order (cv(5, sum n_cells)).  Advanced by
+ * rflu_gpu_time_steps; rflu_gpu_time_step and rflu_gpu_rk_update are single-region
+ * calls.  No counterpart in the reference beyond its region loop. */
+int rflu_gpu_create_batch(int32_t n_cases, const rflu_region_desc* descs, int32_t* handle);
+
 /* cv (5,n_cells) conservative state; dv (3,n_cells) p,T,c or NULL to compute it
  * (RFLU_SetDependentVars).  Also sets global%currentTime. */
 int rflu_gpu_upload_state(int32_t handle, const double* cv, const double* dv, double time);
@@ -157,7 +169,11 @@ int rflu_gpu_time_steps(int32_t handle, int64_t nsteps, int32_t use_graph);
  * end at it and later steps do nothing (rocflu/RFLU_TimeStepping.F90:445-448). */
 int rflu_gpu_set_end_time(int32_t handle, double end_time);
 
+/* For a batch: counters and mass summed over the cases, the earliest time, the
+ * smallest dt and the fewest steps; finished when every case is. */
 int rflu_gpu_get_stats(int32_t handle, rflu_gpu_stats* stats);
+/* Statistics of one case of a batch (case 0 of a single region). */
+int rflu_gpu_get_case_stats(int32_t handle, int32_t icase, rflu_gpu_stats* stats);
 int rflu_gpu_reset_stats(int32_t handle);
 
 /* Waits for the region's stream. */

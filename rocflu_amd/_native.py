@@ -61,6 +61,7 @@ class GpuStats(ctypes.Structure):
 SIGNATURES = {
     "rflu_gpu_device_count": (ctypes.c_int, []),
     "rflu_gpu_create_region": (ctypes.c_int, [ctypes.POINTER(RegionDesc), _i32p]),
+    "rflu_gpu_create_batch": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(RegionDesc), _i32p]),
     "rflu_gpu_destroy_region": (ctypes.c_int, [ctypes.c_int32]),
     "rflu_gpu_upload_state": (ctypes.c_int, [ctypes.c_int32, _f64p, _f64p, ctypes.c_double]),
     "rflu_gpu_download_state": (ctypes.c_int, [ctypes.c_int32, _f64p, _f64p]),
@@ -75,6 +76,7 @@ SIGNATURES = {
     "rflu_gpu_residual": (ctypes.c_int, [ctypes.c_int32, _f64p]),
     "rflu_gpu_time_steps": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32]),
     "rflu_gpu_get_stats": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(GpuStats)]),
+    "rflu_gpu_get_case_stats": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(GpuStats)]),
     "rflu_gpu_reset_stats": (ctypes.c_int, [ctypes.c_int32]),
     "rflu_gpu_synchronize": (ctypes.c_int, [ctypes.c_int32]),
     "rflu_gpu_set_launch": (ctypes.c_int, [ctypes.c_int32] * 4),

@@ -25,9 +25,7 @@ _cache: Dict[str, dict] = {}
 
 
 def signature(g: GpuRegion) -> str:
-    m, num = g.mesh, g.num
-    return (f"N{m.n_cells}-F{m.n_faces}-B{m.n_bfaces}-S{m.c2cs_idx.size if m.c2cs_idx is not None else 0}"
-            f"-d{num.discr}-o{num.order}-r{num.reconst}-k{num.rk}-p{num.precision}")
+    return g.signature()
 
 
 def _load_file() -> dict:

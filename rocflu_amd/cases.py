@@ -99,3 +99,31 @@ def tet_box(n=80, jitter=0.08, seed=2, stencils=True, bc=BC_SLIPWALL):
     (modflu/RFLU_ModLimiters.F90:505-518) that a unit cube at n = 80 reaches."""
     return box(TET6, n, n, n, lengths=(float(n), float(n), float(n)), jitter=jitter, seed=seed,
                bc_type=(bc,) * 6, bc_opt=(0,) * 6, bc_pout=(0.0,) * 6, stencils=stencils)
+
+
+def ensemble(n_cases=64, min_cells=20_000, max_cells=500_000, mach=(1.2, 3.0),
+             gamma=(1.3, 1.67), seed=5, discr=3, order=2, reconst=11, rk=4, precision=1,
+             cfl=0.5, dt_imposed=1e-4):
+    """Independent shock-tube and blast cases for a batch: case k is a Kuhn-tet box
+    whose size is drawn so that its cell count is uniform in [min_cells, max_cells],
+    with its own shock Mach number and gamma drawn uniformly from the given ranges.
+    Even cases are planar shocks (x = 0.3 L), odd cases spherical blasts at the box
+    centre of radius 0.1 L with the pressure ratio of a normal shock at that Mach
+    number.  Returns [(mesh, numerics, cv0)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_cases):
+        ncell = rng.uniform(min_cells, max_cells)
+        n = max(2, int(round((ncell / 6.0) ** (1.0 / 3.0))))
+        ms = float(rng.uniform(*mach))
+        g = float(rng.uniform(*gamma))
+        m = tet_box(n=n, jitter=0.08, seed=100 + k)
+        num = Numerics(discr=discr, order=order, reconst=reconst, rk=rk, precision=precision,
+                       cfl=cfl, dt_imposed=dt_imposed, gamma=g)
+        if k % 2 == 0:
+            cv0 = planar_shock(m, num, 0.3 * n, mach=ms, noise=0.01, seed=200 + k)
+        else:
+            p2 = normal_shock(ms, num.gas_prime()[2])[2]
+            cv0 = blast(m, num, (0.5 * n,) * 3, 0.1 * n, p_in=p2, p_out=1.0)
+        out.append((m, num, cv0))
+    return out

@@ -37,9 +37,19 @@ struct Region {
   int order = 1, discr = 3, reconst = 0, rk = 2, prec = 0;
   double cfl = 0.5, dt_imposed = 0, dt_min_limit = 0;
   double end_time = HUGE_VAL;  // global%maxTime of the fused loop
-  Gas gas{};
-  PatchTable patches{};
+  Gas gas{};                   // case 0 (the only case of a single region)
   bool has_outflow = false;
+  // batched cases (K == 1: a single region; per-case tables still hold its values)
+  int K = 1;
+  std::vector<int> cell0_h, bface0_h;
+  std::vector<CaseParams> par_h;
+  unsigned short* cid = nullptr;
+  Gas* gas_tab = nullptr;
+  CaseParams* par = nullptr;
+  int *cell0 = nullptr, *bface0 = nullptr;
+  unsigned long long* dtmin = nullptr;
+  int *pbc = nullptr, *popt = nullptr, *porder2 = nullptr;
+  double* ppout = nullptr;
   hipStream_t stream = nullptr;
   // device buffers
   int2* f2c = nullptr;
@@ -96,6 +106,24 @@ struct Region {
     return s;
   }
   const void* grad_final() const { return reconst == RFLU_RECONST_WENO_XYZ ? grad2 : grad; }
+  Cases cases() const {
+    Cases c;
+    c.K = K; c.cid = K > 1 ? cid : nullptr; c.gas = gas_tab; c.sc = sc;
+    c.cell0 = cell0; c.bface0 = bface0; c.par = par; c.dtmin = dtmin;
+    return c;
+  }
+  Patches patches() const {
+    Patches p;
+    p.bc = pbc; p.opt = popt; p.order2 = porder2; p.pout = ppout;
+    return p;
+  }
+  // scalars that gate a whole launch: the single case's, none for a batch
+  const Scalars* gate_sc() const { return K == 1 ? sc : nullptr; }
+  CaseParams case0() const {
+    CaseParams c;
+    c.cfl = cfl; c.dt_imposed = dt_imposed; c.dt_min_limit = dt_min_limit; c.end_time = end_time;
+    return c;
+  }
 };
 
 std::mutex g_mu;
@@ -130,7 +158,8 @@ void free_region(Region* R) {
                   R->c2f_idx, R->sten_ptr, R->sten_idx, R->moms, R->epsq, R->cv, R->cv_old,
                   R->rhs_sum, R->dv, R->prim, R->grad, R->grad2, R->flux, R->rhs, R->dt_part,
                   R->sc, R->tile_uptr, R->tile_uidx, R->tile_lptr, R->tile_loc, R->tile_vloc,
-                  R->vnb, R->vgeo, R->srad, R->gw};
+                  R->vnb, R->vgeo, R->srad, R->gw, R->cid, R->gas_tab, R->par, R->cell0,
+                  R->bface0, R->dtmin, R->pbc, R->popt, R->porder2, R->ppout};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (R->stream) (void)hipStreamDestroy(R->stream);
@@ -151,31 +180,31 @@ inline uint64_t spread3(uint64_t x) {
   return x;
 }
 
-// Cells sorted along a Morton curve of their centroids (ties keep the caller's order),
-// so that consecutive cells -- one wavefront, one LDS tile -- are compact in space.
-std::vector<int> morton_order(const std::vector<double4>& cell) {
-  const int N = (int)cell.size();
+// Cells of one case [c0, c1) sorted along a Morton curve of their centroids (ties
+// keep the caller's order), so that consecutive cells -- one wavefront, one LDS tile
+// -- are compact in space.  Cases keep their ranges.
+void morton_order(const std::vector<double4>& cell, int c0, int c1, std::vector<int>& n2o) {
+  const int N = c1 - c0;
   double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
-  for (const double4& c : cell) {
+  for (int i = c0; i < c1; ++i) {
+    const double4& c = cell[i];
     lo[0] = std::min(lo[0], c.x); hi[0] = std::max(hi[0], c.x);
     lo[1] = std::min(lo[1], c.y); hi[1] = std::max(hi[1], c.y);
     lo[2] = std::min(lo[2], c.z); hi[2] = std::max(hi[2], c.z);
   }
   std::vector<std::pair<uint64_t, int>> key(N);
   for (int i = 0; i < N; ++i) {
-    const double p[3] = {cell[i].x, cell[i].y, cell[i].z};
+    const double p[3] = {cell[c0 + i].x, cell[c0 + i].y, cell[c0 + i].z};
     uint64_t q[3];
     for (int d = 0; d < 3; ++d) {
       double ext = hi[d] - lo[d];
       double s = ext > 0 ? (p[d] - lo[d]) / ext : 0.0;
       q[d] = (uint64_t)(s * 2097151.0);
     }
-    key[i] = {spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2), i};
+    key[i] = {spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2), c0 + i};
   }
   std::sort(key.begin(), key.end());
-  std::vector<int> n2o(N);
-  for (int i = 0; i < N; ++i) n2o[i] = key[i].second;
-  return n2o;
+  for (int i = 0; i < N; ++i) n2o[c0 + i] = key[i].second;
 }
 
 void renumber(Region* R, std::vector<double4>& hcell, std::vector<int2>& hf2c,
@@ -183,7 +212,8 @@ void renumber(Region* R, std::vector<double4>& hcell, std::vector<int2>& hf2c,
               std::vector<int>& hc2f_ptr, std::vector<int>& hc2f_idx, std::vector<int>& hsten_ptr,
               std::vector<int>& hsten_idx, std::vector<double>& hmoms, std::vector<double>& hepsq) {
   const int N = R->N, F = R->F;
-  std::vector<int> n2o = morton_order(hcell), o2n(N);
+  std::vector<int> n2o(N), o2n(N);
+  for (int c = 0; c < R->K; ++c) morton_order(hcell, R->cell0_h[c], R->cell0_h[c + 1], n2o);
   for (int i = 0; i < N; ++i) o2n[n2o[i]] = i;
   {
     std::vector<double4> c2(N);
@@ -352,7 +382,7 @@ int launch_fluxes_t(Region* R) {
   if (R->F > 0) {
 #define RFLU_FACE(D, O)                                                                            \
   hipLaunchKernelGGL((k_face_flux<D, O, T, GT, FT, SR>), dim3(nblk(R->F, R->face_block)), fb, 0, \
-                     R->stream, M, S, R->gas, g, fl, R->srad, R->sc)
+                     R->stream, M, S, R->gas, R->cases(), g, fl, R->srad, R->gate_sc())
     if (R->order == 1) {
       if (R->discr == 1) RFLU_FACE(1, 1);
       else if (R->discr == 3) RFLU_FACE(3, 1);
@@ -375,10 +405,10 @@ int launch_bfaces_t(Region* R) {
   const GT* g = (const GT*)R->grad_final();
   if (R->order == 1)
     hipLaunchKernelGGL((k_bface_flux<1, GT, FT, SR>), dim3(nblk(R->B, 256)), dim3(256), 0, R->stream,
-                       M, S, R->gas, R->patches, g, (FT*)R->flux, R->srad, R->sc);
+                       M, S, R->gas, R->cases(), R->patches(), g, (FT*)R->flux, R->srad, R->gate_sc());
   else
     hipLaunchKernelGGL((k_bface_flux<2, GT, FT, SR>), dim3(nblk(R->B, 256)), dim3(256), 0, R->stream,
-                       M, S, R->gas, R->patches, g, (FT*)R->flux, R->srad, R->sc);
+                       M, S, R->gas, R->cases(), R->patches(), g, (FT*)R->flux, R->srad, R->gate_sc());
   return 0;
 }
 
@@ -396,15 +426,15 @@ void launch_gradient_kernel(Region* R, const Mesh& M, const State& S) {
     Tiles T = region_tiles(R);
     size_t shm = (size_t)5 * R->tile_cap * sizeof(double);
     hipLaunchKernelGGL((k_gradient_ldsw<GT, V, 4 * U>), dim3(R->n_tiles), dim3(GRAD_TILE), shm,
-                       R->stream, M, S, T, R->tile_loc_len, (const GT*)R->gw, (GT*)R->grad, R->sc);
+                       R->stream, M, S, T, R->tile_loc_len, (const GT*)R->gw, (GT*)R->grad, R->gate_sc());
   } else if (R->use_lds && R->grad_lds) {
     Tiles T = region_tiles(R);
     size_t shm = (size_t)8 * R->tile_cap * sizeof(double);
     hipLaunchKernelGGL((k_gradient_lds<GT, V, U>), dim3(R->n_tiles), dim3(GRAD_TILE), shm,
-                       R->stream, M, S, T, (GT*)R->grad, R->sc);
+                       R->stream, M, S, T, (GT*)R->grad, R->gate_sc());
   } else {
     hipLaunchKernelGGL((k_gradient<GT, V, U>), dim3(nblk(R->N, R->grad_block)), dim3(R->grad_block),
-                       0, R->stream, M, S, (GT*)R->grad, R->sc);
+                       0, R->stream, M, S, (GT*)R->grad, R->gate_sc());
   }
 }
 
@@ -427,7 +457,7 @@ int launch_blend_t(Region* R) {
   Mesh M = R->mesh();
   State S = R->state();
   hipLaunchKernelGGL((k_weno_clip<GT>), dim3(nblk(R->N, R->grad_block)), dim3(R->grad_block), 0,
-                     R->stream, M, S, (const GT*)R->grad, (GT*)R->grad2, R->sc);
+                     R->stream, M, S, (const GT*)R->grad, (GT*)R->grad2, R->gate_sc());
   return 0;
 }
 
@@ -437,11 +467,12 @@ int launch_update_rk(Region* R, int stage, int last, double dt_in) {
   State S = R->state();
   dim3 cb(R->cell_block), cg(nblk(R->N, R->cell_block));
   const FT* fl = (const FT*)R->flux;
+  const Cases C = R->cases();
   switch (R->rk) {
-    case 1: hipLaunchKernelGGL((k_update<1, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last, dt_in); break;
-    case 2: hipLaunchKernelGGL((k_update<2, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last, dt_in); break;
-    case 3: hipLaunchKernelGGL((k_update<3, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last, dt_in); break;
-    default: hipLaunchKernelGGL((k_update<4, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, fl, R->sc, stage, last, dt_in); break;
+    case 1: hipLaunchKernelGGL((k_update<1, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, C, fl, R->sc, stage, last, dt_in); break;
+    case 2: hipLaunchKernelGGL((k_update<2, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, C, fl, R->sc, stage, last, dt_in); break;
+    case 3: hipLaunchKernelGGL((k_update<3, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, C, fl, R->sc, stage, last, dt_in); break;
+    default: hipLaunchKernelGGL((k_update<4, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, C, fl, R->sc, stage, last, dt_in); break;
   }
   return 0;
 }
@@ -532,11 +563,11 @@ int launch_fluxes(Region* R, int stage, bool srad = false) {
   if (stage == 1 && R->has_outflow) {
     Mesh M = R->mesh();
     if (!flux32(R))
-      hipLaunchKernelGGL((k_mass_inout<double>), dim3(1), dim3(256), 0, R->stream, M, R->patches,
-                         (const double*)R->flux, R->sc);
+      hipLaunchKernelGGL((k_mass_inout<double>), dim3(R->K), dim3(256), 0, R->stream, M, R->cases(),
+                         R->patches(), (const double*)R->flux);
     else
-      hipLaunchKernelGGL((k_mass_inout<float>), dim3(1), dim3(256), 0, R->stream, M, R->patches,
-                         (const float*)R->flux, R->sc);
+      hipLaunchKernelGGL((k_mass_inout<float>), dim3(R->K), dim3(256), 0, R->stream, M, R->cases(),
+                         R->patches(), (const float*)R->flux);
   }
   return 0;
 }
@@ -559,11 +590,18 @@ int launch_update(Region* R, int stage, bool fused, double dt_in = -1.0) {
 // RFLU_MinimumTimeStep's rules.
 int launch_dt_reduce(Region* R, double time_in, double end_time) {
   Mesh M = R->mesh();
+  if (R->K > 1) {
+    hipLaunchKernelGGL(k_dt_gather_cases, dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M,
+                       R->cases(), (const double*)R->srad);
+    hipLaunchKernelGGL(k_dt_finalize_cases, dim3(nblk(R->K, 64)), dim3(64), 0, R->stream, R->cases());
+    return 0;
+  }
+  CaseParams cp = R->case0();
+  cp.end_time = end_time;
   hipLaunchKernelGGL(k_dt_gather, dim3(R->n_dt_parts), dim3(256), 0, R->stream, M,
                      (const double*)R->srad, R->dt_part, R->sc);
   hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, R->stream, (const double*)R->dt_part,
-                     R->n_dt_parts, R->sc, R->cfl, R->dt_imposed, R->dt_min_limit, time_in,
-                     end_time);
+                     R->n_dt_parts, R->sc, cp, time_in);
   return 0;
 }
 
@@ -574,7 +612,7 @@ int launch_time_step(Region* R, double time_in = -1.0, double end_time = HUGE_VA
   State S = R->state();
   const int nf = R->F + R->B;
   if (nf > 0)
-    hipLaunchKernelGGL(k_face_srad, dim3(nblk(nf, 256)), dim3(256), 0, R->stream, M, S, R->srad, R->sc);
+    hipLaunchKernelGGL(k_face_srad, dim3(nblk(nf, 256)), dim3(256), 0, R->stream, M, S, R->srad, R->gate_sc());
   return launch_dt_reduce(R, time_in, end_time);
 }
 
@@ -597,9 +635,36 @@ int launch_step(Region* R) {
   return 0;
 }
 
-int read_scalars(Region* R, Scalars* h) {
-  HIP_TRY(hipMemcpyAsync(h, R->sc, sizeof(Scalars), hipMemcpyDeviceToHost, R->stream));
+int read_cases(Region* R, std::vector<Scalars>& v) {
+  v.resize(R->K);
+  HIP_TRY(hipMemcpyAsync(v.data(), R->sc, R->K * sizeof(Scalars), hipMemcpyDeviceToHost, R->stream));
   HIP_TRY(hipStreamSynchronize(R->stream));
+  return 0;
+}
+
+// The region's scalars; for a batch, counters and mass summed over the cases, the
+// earliest time, smallest dt and fewest steps, finished when every case is.
+int read_scalars(Region* R, Scalars* h) {
+  std::vector<Scalars> v;
+  int rc = read_cases(R, v);
+  if (rc) return rc;
+  *h = v[0];
+  for (int c = 1; c < R->K; ++c) {
+    const Scalars& s = v[c];
+    h->time = std::min(h->time, s.time);
+    h->dt = std::min(h->dt, s.dt);
+    h->dt_local_min = std::min(h->dt_local_min, s.dt_local_min);
+    h->mass_in += s.mass_in;
+    h->mass_out += s.mass_out;
+    h->n_invalid += s.n_invalid;
+    h->n_negative += s.n_negative;
+    h->n_bounds_fix += s.n_bounds_fix;
+    h->n_pos_fix += s.n_pos_fix;
+    h->steps = std::min(h->steps, s.steps);
+    h->abort |= s.abort;
+    h->last_step &= s.last_step;
+    h->finished &= s.finished;
+  }
   return 0;
 }
 
@@ -609,24 +674,12 @@ int status_from(const Scalars& s) {
   return RFLU_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char* rflu_gpu_last_error(void) { return g_err.c_str(); }
-
-int rflu_gpu_device_count(void) {
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
-  return n;
-}
-
-int rflu_gpu_create_region(const rflu_region_desc* d, int32_t* handle) {
-  if (!d || !handle) return fail(RFLU_ERR_ARG, "null argument");
-  *handle = 0;
+// Validates one descriptor (one case); ref, when given, is the batch's first case,
+// whose scheme every case must share.
+int check_desc(const rflu_region_desc* d, const rflu_region_desc* ref) {
   const int N = d->n_cells, F = d->n_faces, P = d->n_patches;
   const int base = d->index_base;
-  if (N <= 0 || F < 0 || P < 0 || P > MAX_PATCHES) return fail(RFLU_ERR_ARG, "bad region sizes");
+  if (N <= 0 || F < 0 || P < 0) return fail(RFLU_ERR_ARG, "bad region sizes");
   if (base != 0 && base != 1) return fail(RFLU_ERR_ARG, "index_base must be 0 or 1");
   if (!d->f2c || !d->fn || !d->cofg || !d->vol || (P > 0 && (!d->bpatch_ptr || !d->patch_bc)))
     return fail(RFLU_ERR_ARG, "missing grid arrays");
@@ -641,70 +694,149 @@ int rflu_gpu_create_region(const rflu_region_desc* d, int32_t* handle) {
       return fail(RFLU_ERR_ARG, "reconst must be 0, 2 or 11");
   }
   if (!(d->gamma > 1.0) || !(d->cp > 0.0)) return fail(RFLU_ERR_ARG, "bad gas constants");
-  int dev_count = 0;
-  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
-    return fail(RFLU_ERR_HIP, "no GPU visible");
   const int B = P > 0 ? d->bpatch_ptr[P] : 0;
   if (B > 0 && (!d->bf2c || !d->bfn || !d->bfc)) return fail(RFLU_ERR_ARG, "missing boundary arrays");
   if (d->bpatch_ptr && d->bpatch_ptr[0] != 0) return fail(RFLU_ERR_ARG, "bpatch_ptr[0] must be 0");
+  for (int p = 0; p < P; ++p)
+    if (d->patch_bc[p] != BC_OUTFLOW && d->patch_bc[p] != BC_SLIPWALL)
+      return fail(RFLU_ERR_ARG, "only slip-wall (60) and outflow (20) patches are supported");
+  if (ref && (d->order != ref->order || d->discr != ref->discr || d->rk != ref->rk ||
+              d->precision != ref->precision || (d->order == 2 && d->reconst != ref->reconst)))
+    return fail(RFLU_ERR_ARG, "every case of a batch needs the same discr, order, reconst, rk and precision");
+  return 0;
+}
+
+// Gas constants of one case (libflu/RFLU_SetGasVars.F90:172-178 for gamma').
+Gas gas_of(const rflu_region_desc* d) {
+  Gas G;
+  G.g = d->gamma; G.cp = d->cp; G.gc = d->cp - d->cp / d->gamma;
+  double m = 8314.3 / G.gc;          // MixtPerf_M_R
+  G.mm = m;
+  G.rgasp = 8314.3 / m;              // MixtPerf_R_M
+  G.gp = d->cp / (d->cp - G.rgasp);  // MixtPerf_G_CpR
+  G.eps_entr = d->eps_entr; G.diss_fact = d->diss_fact;
+  return G;
+}
+
+// Builds a region from K case descriptors (K == 1: one t_region).  The cases' cells,
+// faces, boundary faces and patches are concatenated in case order; every per-case
+// list keeps its own order.
+int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
+  if (!descs || !handle || K < 1 || K > 65535) return fail(RFLU_ERR_ARG, "bad arguments");
+  *handle = 0;
+  int rc = 0;
+  std::vector<long long> co(K + 1, 0), fo(K + 1, 0), bo(K + 1, 0), po(K + 1, 0), so(K + 1, 0);
+  for (int c = 0; c < K; ++c) {
+    const rflu_region_desc* d = descs + c;
+    if ((rc = check_desc(d, c ? descs : nullptr))) return rc;
+    co[c + 1] = co[c] + d->n_cells;
+    fo[c + 1] = fo[c] + d->n_faces;
+    bo[c + 1] = bo[c] + (d->n_patches > 0 ? d->bpatch_ptr[d->n_patches] : 0);
+    po[c + 1] = po[c] + d->n_patches;
+    so[c + 1] = so[c] + (d->order == 2 ? d->c2cs_ptr[d->n_cells] : 0);
+  }
+  if (co[K] > 0x7fffffff || fo[K] + bo[K] > 0x7fffffff || so[K] > 0x7fffffff)
+    return fail(RFLU_ERR_ARG, "cell, face and stencil counts must fit int32");
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+    return fail(RFLU_ERR_HIP, "no GPU visible");
+  const int N = (int)co[K], F = (int)fo[K], B = (int)bo[K], P = (int)po[K];
+  const rflu_region_desc* d0 = descs;
 
   Region* R = new Region();
-  R->N = N; R->F = F; R->B = B; R->P = P;
-  R->order = d->order; R->discr = d->discr; R->reconst = d->order == 2 ? d->reconst : 0;
-  R->rk = d->rk; R->prec = d->precision;
-  R->cfl = d->cfl; R->dt_imposed = d->dt_imposed; R->dt_min_limit = d->dt_min_limit;
-  {
-    Gas& G = R->gas;
-    G.g = d->gamma; G.cp = d->cp; G.gc = d->cp - d->cp / d->gamma;
-    double m = 8314.3 / G.gc;          // MixtPerf_M_R (libflu/RFLU_SetGasVars.F90:172-178)
-    G.mm = m;
-    G.rgasp = 8314.3 / m;              // MixtPerf_R_M
-    G.gp = d->cp / (d->cp - G.rgasp);  // MixtPerf_G_CpR
-    G.eps_entr = d->eps_entr; G.diss_fact = d->diss_fact;
-  }
-  R->patches.n = P;
-  for (int p = 0; p < P; ++p) {
-    R->patches.bc[p] = d->patch_bc[p];
-    R->patches.opt[p] = d->patch_opt ? d->patch_opt[p] : 0;
-    R->patches.order2[p] = d->patch_order ? (d->patch_order[p] == 2) : 0;
-    R->patches.pout[p] = d->patch_pout ? d->patch_pout[p] : 0.0;
-    if (d->patch_bc[p] != BC_OUTFLOW && d->patch_bc[p] != BC_SLIPWALL) {
-      free_region(R);
-      return fail(RFLU_ERR_ARG, "only slip-wall (60) and outflow (20) patches are supported");
+  R->N = N; R->F = F; R->B = B; R->P = P; R->K = K;
+  R->order = d0->order; R->discr = d0->discr; R->reconst = d0->order == 2 ? d0->reconst : 0;
+  R->rk = d0->rk; R->prec = d0->precision;
+  R->cfl = d0->cfl; R->dt_imposed = d0->dt_imposed; R->dt_min_limit = d0->dt_min_limit;
+  R->gas = gas_of(d0);
+  std::vector<Gas> hgas(K);
+  R->par_h.resize(K);
+  R->cell0_h.assign(co.begin(), co.end());
+  R->bface0_h.assign(bo.begin(), bo.end());
+  std::vector<int> hpbc(P), hpopt(P), hporder2(P);
+  std::vector<double> hppout(P);
+  for (int c = 0; c < K; ++c) {
+    const rflu_region_desc* d = descs + c;
+    hgas[c] = gas_of(d);
+    CaseParams& cp = R->par_h[c];
+    cp.cfl = d->cfl; cp.dt_imposed = d->dt_imposed; cp.dt_min_limit = d->dt_min_limit;
+    cp.end_time = HUGE_VAL;
+    for (int p = 0; p < d->n_patches; ++p) {
+      const size_t q = (size_t)po[c] + p;
+      hpbc[q] = d->patch_bc[p];
+      hpopt[q] = d->patch_opt ? d->patch_opt[p] : 0;
+      hporder2[q] = d->patch_order ? (d->patch_order[p] == 2) : 0;
+      hppout[q] = d->patch_pout ? d->patch_pout[p] : 0.0;
+      if (d->patch_bc[p] == BC_OUTFLOW) R->has_outflow = true;
     }
-    if (d->patch_bc[p] == BC_OUTFLOW) R->has_outflow = true;
   }
 
   // ---- host-side packing ----
   std::vector<int2> hf2c(F);
   std::vector<double4> hfn(F), hfc(F), hcell(N), hbfn(B), hbfc(B);
   std::vector<int> hbf2c(B), hbpatch(B);
-  for (int f = 0; f < F; ++f) {
-    int c1 = d->f2c[2 * (size_t)f] - base, c2 = d->f2c[2 * (size_t)f + 1] - base;
-    if (c1 < 0 || c1 >= N || c2 < 0 || c2 >= N || c1 == c2) {
-      free_region(R);
-      return fail(RFLU_ERR_ARG, "f2c out of range at face " + std::to_string(f));
-    }
-    hf2c[f] = make_int2(c1, c2);
-    hfn[f] = make_double4(d->fn[4 * (size_t)f], d->fn[4 * (size_t)f + 1], d->fn[4 * (size_t)f + 2], d->fn[4 * (size_t)f + 3]);
-    if (d->fc) hfc[f] = make_double4(d->fc[3 * (size_t)f], d->fc[3 * (size_t)f + 1], d->fc[3 * (size_t)f + 2], 0.0);
-    else hfc[f] = make_double4(0, 0, 0, 0);
+  std::vector<int> hsten_ptr, hsten_idx;
+  std::vector<double> hmoms, hepsq;
+  if (R->order == 2) {
+    hsten_ptr.assign(N + 1, 0);
+    hsten_idx.resize(so[K]);
+    hmoms.resize((size_t)6 * N);
+    hepsq.resize(N);
   }
-  for (int c = 0; c < N; ++c) {
-    hcell[c] = make_double4(d->cofg[3 * (size_t)c], d->cofg[3 * (size_t)c + 1], d->cofg[3 * (size_t)c + 2], d->vol[c]);
-    if (!(d->vol[c] > 0.0)) { free_region(R); return fail(RFLU_ERR_ARG, "non-positive volume"); }
-  }
-  for (int p = 0; p < P; ++p)
-    for (int b = d->bpatch_ptr[p]; b < d->bpatch_ptr[p + 1]; ++b) {
-      if (b < 0 || b >= B) { free_region(R); return fail(RFLU_ERR_ARG, "bad bpatch_ptr"); }
-      int c1 = d->bf2c[b] - base;
-      if (c1 < 0 || c1 >= N) { free_region(R); return fail(RFLU_ERR_ARG, "bf2c out of range"); }
-      hbf2c[b] = c1;
-      hbpatch[b] = p;
-      hbfn[b] = make_double4(d->bfn[4 * (size_t)b], d->bfn[4 * (size_t)b + 1], d->bfn[4 * (size_t)b + 2], d->bfn[4 * (size_t)b + 3]);
-      hbfc[b] = make_double4(d->bfc[3 * (size_t)b], d->bfc[3 * (size_t)b + 1], d->bfc[3 * (size_t)b + 2], 0.0);
+  for (int k = 0; k < K; ++k) {
+    const rflu_region_desc* d = descs + k;
+    const int n = d->n_cells, nf = d->n_faces, np = d->n_patches, base = d->index_base;
+    const int c0 = (int)co[k], f0 = (int)fo[k], b0 = (int)bo[k], p0 = (int)po[k];
+    const int nb = (int)(bo[k + 1] - bo[k]);
+    for (int f = 0; f < nf; ++f) {
+      int c1 = d->f2c[2 * (size_t)f] - base, c2 = d->f2c[2 * (size_t)f + 1] - base;
+      if (c1 < 0 || c1 >= n || c2 < 0 || c2 >= n || c1 == c2) {
+        free_region(R);
+        return fail(RFLU_ERR_ARG, "f2c out of range at face " + std::to_string(f) + " of case " + std::to_string(k));
+      }
+      hf2c[f0 + f] = make_int2(c0 + c1, c0 + c2);
+      hfn[f0 + f] = make_double4(d->fn[4 * (size_t)f], d->fn[4 * (size_t)f + 1], d->fn[4 * (size_t)f + 2], d->fn[4 * (size_t)f + 3]);
+      if (d->fc) hfc[f0 + f] = make_double4(d->fc[3 * (size_t)f], d->fc[3 * (size_t)f + 1], d->fc[3 * (size_t)f + 2], 0.0);
+      else hfc[f0 + f] = make_double4(0, 0, 0, 0);
     }
-  // cell -> face list in the reference's scatter order
+    for (int c = 0; c < n; ++c) {
+      hcell[c0 + c] = make_double4(d->cofg[3 * (size_t)c], d->cofg[3 * (size_t)c + 1], d->cofg[3 * (size_t)c + 2], d->vol[c]);
+      if (!(d->vol[c] > 0.0)) { free_region(R); return fail(RFLU_ERR_ARG, "non-positive volume"); }
+    }
+    for (int p = 0; p < np; ++p)
+      for (int b = d->bpatch_ptr[p]; b < d->bpatch_ptr[p + 1]; ++b) {
+        if (b < 0 || b >= nb) { free_region(R); return fail(RFLU_ERR_ARG, "bad bpatch_ptr"); }
+        int c1 = d->bf2c[b] - base;
+        if (c1 < 0 || c1 >= n) { free_region(R); return fail(RFLU_ERR_ARG, "bf2c out of range"); }
+        hbf2c[b0 + b] = c0 + c1;
+        hbpatch[b0 + b] = p0 + p;
+        hbfn[b0 + b] = make_double4(d->bfn[4 * (size_t)b], d->bfn[4 * (size_t)b + 1], d->bfn[4 * (size_t)b + 2], d->bfn[4 * (size_t)b + 3]);
+        hbfc[b0 + b] = make_double4(d->bfc[3 * (size_t)b], d->bfc[3 * (size_t)b + 1], d->bfc[3 * (size_t)b + 2], 0.0);
+      }
+    if (R->order == 2) {
+      if (d->c2cs_ptr[0] != 0) { free_region(R); return fail(RFLU_ERR_ARG, "c2cs_ptr[0] must be 0"); }
+      const int s0 = (int)so[k];
+      for (int c = 0; c < n; ++c) {
+        if (d->c2cs_ptr[c + 1] < d->c2cs_ptr[c]) { free_region(R); return fail(RFLU_ERR_ARG, "c2cs_ptr not monotone"); }
+        hsten_ptr[c0 + c + 1] = s0 + d->c2cs_ptr[c + 1];
+      }
+      for (int s = 0; s < d->c2cs_ptr[n]; ++s) {
+        int j = d->c2cs_idx[s] - base;
+        if (j < 0 || j >= n) { free_region(R); return fail(RFLU_ERR_ARG, "c2cs_idx out of range"); }
+        hsten_idx[(size_t)s0 + s] = c0 + j;
+      }
+      for (int c = 0; c < n; ++c)
+        for (int q = 0; q < 6; ++q) hmoms[(size_t)q * N + c0 + c] = d->xyz_moms[10 * (size_t)c + q];
+      // Venkatakrishnan eps^2 = (K*vol^(1/3))^3, K = 1 (modflu/RFLU_ModLimiters.F90:473-476)
+      const double THRD = 1.0 / 3.0, KV = 1.0;
+      for (int c = 0; c < n; ++c) {
+        double ds = std::pow(d->vol[c], THRD);
+        hepsq[c0 + c] = (KV * ds) * (KV * ds) * (KV * ds);
+      }
+    }
+  }
+  // cell -> face list in the reference's scatter order (interior faces in face order,
+  // then boundary faces patch by patch)
   std::vector<int> hc2f_ptr(N + 1, 0), hc2f_idx;
   for (int f = 0; f < F; ++f) { hc2f_ptr[hf2c[f].x + 1]++; hc2f_ptr[hf2c[f].y + 1]++; }
   for (int b = 0; b < B; ++b) hc2f_ptr[hbf2c[b] + 1]++;
@@ -718,37 +850,18 @@ int rflu_gpu_create_region(const rflu_region_desc* d, int32_t* handle) {
     }
     for (int b = 0; b < B; ++b) hc2f_idx[fill[hbf2c[b]]++] = F + b + 1;
   }
-  std::vector<int> hsten_ptr, hsten_idx;
-  std::vector<double> hmoms, hepsq;
-  if (R->order == 2) {
-    hsten_ptr.assign(d->c2cs_ptr, d->c2cs_ptr + N + 1);
-    if (hsten_ptr[0] != 0) { free_region(R); return fail(RFLU_ERR_ARG, "c2cs_ptr[0] must be 0"); }
-    for (int c = 0; c < N; ++c)
-      if (hsten_ptr[c + 1] < hsten_ptr[c]) { free_region(R); return fail(RFLU_ERR_ARG, "c2cs_ptr not monotone"); }
-    size_t S = (size_t)hsten_ptr[N];
-    hsten_idx.resize(S);
-    for (size_t s = 0; s < S; ++s) {
-      int j = d->c2cs_idx[s] - base;
-      if (j < 0 || j >= N) { free_region(R); return fail(RFLU_ERR_ARG, "c2cs_idx out of range"); }
-      hsten_idx[s] = j;
-    }
-    hmoms.resize((size_t)6 * N);
-    for (int c = 0; c < N; ++c)
-      for (int k = 0; k < 6; ++k) hmoms[(size_t)k * N + c] = d->xyz_moms[10 * (size_t)c + k];
-    // Venkatakrishnan eps^2 = (K*vol^(1/3))^3, K = 1 (modflu/RFLU_ModLimiters.F90:473-476)
-    hepsq.resize(N);
-    const double THRD = 1.0 / 3.0, K = 1.0;
-    for (int c = 0; c < N; ++c) {
-      double ds = std::pow(d->vol[c], THRD);
-      hepsq[c] = (K * ds) * (K * ds) * (K * ds);
-    }
-  }
 
-  // ---- internal renumbering (Morton order of cell centroids) ----
+  // ---- internal renumbering (Morton order of cell centroids within each case) ----
   // Storage order only: every per-cell face list keeps the reference's summation
   // order (original face order) and every stencil keeps its member order, so the
   // arithmetic is the same as in the original numbering.
   renumber(R, hcell, hf2c, hfn, hfc, hbf2c, hc2f_ptr, hc2f_idx, hsten_ptr, hsten_idx, hmoms, hepsq);
+  std::vector<unsigned short> hcid;
+  if (K > 1) {
+    hcid.resize(N);
+    for (int k = 0; k < K; ++k)
+      for (long long i = co[k]; i < co[k + 1]; ++i) hcid[i] = (unsigned short)k;
+  }
   std::vector<int> htile_uptr, htile_uidx, htile_lptr;
   std::vector<unsigned short> htile_loc, htile_vloc;
   std::vector<int> hvnb;
@@ -760,20 +873,24 @@ int rflu_gpu_create_region(const rflu_region_desc* d, int32_t* handle) {
                 htile_loc, htile_vloc);
 
   // ---- device allocation ----
-  int rc = 0;
   if (hipStreamCreate(&R->stream) != hipSuccess) { free_region(R); return fail(RFLU_ERR_HIP, "hipStreamCreate"); }
   const size_t FB = (size_t)F + B;
   const size_t fsz = flux32(R) ? sizeof(float) : sizeof(double);
   const size_t gsz = grad32(R) ? sizeof(float) : sizeof(double);
   R->n_dt_parts = (int)nblk(N, 256);
+  std::vector<int> hcell0(co.begin(), co.end()), hbface0(bo.begin(), bo.end());
   if ((rc = dalloc(&R->f2c, F)) || (rc = dalloc(&R->fn, F)) || (rc = dalloc(&R->fc, F)) ||
       (rc = dalloc(&R->cell, N)) || (rc = dalloc(&R->bfn, B)) || (rc = dalloc(&R->bfc, B)) ||
       (rc = dalloc(&R->bf2c, B)) || (rc = dalloc(&R->bpatch, B)) ||
       (rc = dalloc(&R->c2f_ptr, N + 1)) || (rc = dalloc(&R->c2f_idx, hc2f_idx.size())) ||
       (rc = dalloc(&R->cv, 5 * (size_t)N)) || (rc = dalloc(&R->rhs_sum, 5 * (size_t)N)) ||
       (rc = dalloc(&R->dv, 3 * (size_t)N)) || (rc = dalloc(&R->rhs, 5 * (size_t)N)) ||
-      (rc = dalloc(&R->dt_part, R->n_dt_parts)) || (rc = dalloc(&R->sc, 1)) ||
-      (rc = dalloc(&R->srad, FB))) {
+      (rc = dalloc(&R->dt_part, R->n_dt_parts)) || (rc = dalloc(&R->sc, K)) ||
+      (rc = dalloc(&R->srad, FB)) || (rc = dalloc(&R->gas_tab, K)) || (rc = dalloc(&R->par, K)) ||
+      (rc = dalloc(&R->cell0, K + 1)) || (rc = dalloc(&R->bface0, K + 1)) ||
+      (rc = dalloc(&R->dtmin, K)) || (rc = dalloc(&R->pbc, P)) || (rc = dalloc(&R->popt, P)) ||
+      (rc = dalloc(&R->porder2, P)) || (rc = dalloc(&R->ppout, P)) ||
+      (K > 1 && (rc = dalloc(&R->cid, N)))) {
     free_region(R);
     return rc;
   }
@@ -804,7 +921,12 @@ int rflu_gpu_create_region(const rflu_region_desc* d, int32_t* handle) {
       (rc = dput(R->fc, hfc, R->stream)) || (rc = dput(R->cell, hcell, R->stream)) ||
       (rc = dput(R->bfn, hbfn, R->stream)) || (rc = dput(R->bfc, hbfc, R->stream)) ||
       (rc = dput(R->bf2c, hbf2c, R->stream)) || (rc = dput(R->bpatch, hbpatch, R->stream)) ||
-      (rc = dput(R->c2f_ptr, hc2f_ptr, R->stream)) || (rc = dput(R->c2f_idx, hc2f_idx, R->stream))) {
+      (rc = dput(R->c2f_ptr, hc2f_ptr, R->stream)) || (rc = dput(R->c2f_idx, hc2f_idx, R->stream)) ||
+      (rc = dput(R->gas_tab, hgas, R->stream)) || (rc = dput(R->par, R->par_h, R->stream)) ||
+      (rc = dput(R->cell0, hcell0, R->stream)) || (rc = dput(R->bface0, hbface0, R->stream)) ||
+      (rc = dput(R->pbc, hpbc, R->stream)) || (rc = dput(R->popt, hpopt, R->stream)) ||
+      (rc = dput(R->porder2, hporder2, R->stream)) || (rc = dput(R->ppout, hppout, R->stream)) ||
+      (K > 1 && (rc = dput(R->cid, hcid, R->stream)))) {
     free_region(R);
     return rc;
   }
@@ -833,7 +955,7 @@ int rflu_gpu_create_region(const rflu_region_desc* d, int32_t* handle) {
       R->tile_loc_len = htile_loc.size();
     }
   }
-  hipLaunchKernelGGL(k_fill_scalars, dim3(1), dim3(64), 0, R->stream, R->sc, 0.0);
+  hipLaunchKernelGGL(k_fill_scalars, dim3(nblk(K, 64)), dim3(64), 0, R->stream, R->sc, K, 0.0, R->dtmin);
   if (hipMemsetAsync(R->srad, 0, FB * sizeof(double), R->stream) != hipSuccess ||
       hipStreamSynchronize(R->stream) != hipSuccess) {
     free_region(R);
@@ -843,6 +965,26 @@ int rflu_gpu_create_region(const rflu_region_desc* d, int32_t* handle) {
   g_regions.push_back(R);
   *handle = (int32_t)g_regions.size();
   return RFLU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rflu_gpu_last_error(void) { return g_err.c_str(); }
+
+int rflu_gpu_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int rflu_gpu_create_region(const rflu_region_desc* d, int32_t* handle) {
+  return create_cases(1, d, handle);
+}
+
+int rflu_gpu_create_batch(int32_t n_cases, const rflu_region_desc* descs, int32_t* handle) {
+  return create_cases(n_cases, descs, handle);
 }
 
 int rflu_gpu_destroy_region(int32_t h) {
@@ -874,14 +1016,14 @@ int rflu_gpu_upload_state(int32_t h, const double* cv, const double* dv, double 
   } else {
     Mesh M = R->mesh();
     State S = R->state();
-    hipLaunchKernelGGL(k_set_dv, dim3(nblk(N, 256)), dim3(256), 0, R->stream, M, S, R->gas);
+    hipLaunchKernelGGL(k_set_dv, dim3(nblk(N, 256)), dim3(256), 0, R->stream, M, S, R->gas, R->cases());
   }
   if (R->order == 2) {
     Mesh M = R->mesh();
     State S = R->state();
     hipLaunchKernelGGL(k_set_prim, dim3(nblk(N, 256)), dim3(256), 0, R->stream, M, S);
   }
-  hipLaunchKernelGGL(k_fill_scalars, dim3(1), dim3(64), 0, R->stream, R->sc, time);
+  hipLaunchKernelGGL(k_fill_scalars, dim3(nblk(R->K, 64)), dim3(64), 0, R->stream, R->sc, R->K, time, R->dtmin);
   HIP_TRY(hipStreamSynchronize(R->stream));
   return RFLU_OK;
 }
@@ -906,6 +1048,7 @@ int rflu_gpu_download_state(int32_t h, double* cv, double* dv) {
 int rflu_gpu_time_step(int32_t h, double current_time, double* dt_out) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (R->K > 1) return fail(RFLU_ERR_ARG, "a batch of cases advances through rflu_gpu_time_steps");
   if (!(current_time >= 0.0)) return fail(RFLU_ERR_ARG, "current_time must be >= 0");
   int rc = launch_time_step(R, current_time);
   if (rc) return rc;
@@ -950,6 +1093,7 @@ int rflu_gpu_convective_fluxes(int32_t h, int32_t istage) {
 int rflu_gpu_rk_update(int32_t h, int32_t istage, double dt_min) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (R->K > 1) return fail(RFLU_ERR_ARG, "a batch of cases advances through rflu_gpu_time_steps");
   if (istage < 1 || istage > nstages(R->rk)) return fail(RFLU_ERR_ARG, "bad stage");
   if (!(dt_min > 0.0)) return fail(RFLU_ERR_ARG, "dt_min must be > 0");
   int rc = launch_update(R, istage, false, dt_min);
@@ -963,7 +1107,7 @@ int rflu_gpu_set_dependent_vars(int32_t h) {
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
   Mesh M = R->mesh();
   State S = R->state();
-  hipLaunchKernelGGL(k_set_dv, dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S, R->gas);
+  hipLaunchKernelGGL(k_set_dv, dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S, R->gas, R->cases());
   if (R->order == 2)
     hipLaunchKernelGGL(k_set_prim, dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S);
   HIP_TRY(hipGetLastError());
@@ -977,9 +1121,9 @@ int rflu_gpu_after_update(int32_t h, int32_t istage) {
   Mesh M = R->mesh();
   State S = R->state();
   if (R->order == 2)
-    hipLaunchKernelGGL((k_after_update<true>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S, R->gas, R->sc);
+    hipLaunchKernelGGL((k_after_update<true>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S, R->gas, R->cases(), R->sc);
   else
-    hipLaunchKernelGGL((k_after_update<false>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S, R->gas, R->sc);
+    hipLaunchKernelGGL((k_after_update<false>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S, R->gas, R->cases(), R->sc);
   HIP_TRY(hipGetLastError());
   Scalars s;
   int rc = read_scalars(R, &s);
@@ -1044,6 +1188,10 @@ int rflu_gpu_set_end_time(int32_t h, double end_time) {
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
   if (!(end_time > 0.0)) return fail(RFLU_ERR_ARG, "end_time must be > 0");
   R->end_time = end_time;
+  for (CaseParams& cp : R->par_h) cp.end_time = end_time;
+  int rc = dput(R->par, R->par_h, R->stream);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(R->stream));
   R->graph_dirty = true;
   return RFLU_OK;
 }
@@ -1068,10 +1216,31 @@ int rflu_gpu_get_stats(int32_t h, rflu_gpu_stats* st) {
   return RFLU_OK;
 }
 
+int rflu_gpu_get_case_stats(int32_t h, int32_t icase, rflu_gpu_stats* st) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!st || icase < 0 || icase >= R->K) return fail(RFLU_ERR_ARG, "bad case or null stats");
+  std::vector<Scalars> v;
+  int rc = read_cases(R, v);
+  if (rc) return rc;
+  const Scalars& s = v[icase];
+  st->n_invalid = (int64_t)s.n_invalid;
+  st->n_negative = (int64_t)s.n_negative;
+  st->n_bounds_fix = (int64_t)s.n_bounds_fix;
+  st->n_pos_fix = (int64_t)s.n_pos_fix;
+  st->mass_in = s.mass_in;
+  st->mass_out = s.mass_out;
+  st->time = s.time;
+  st->dt = s.dt;
+  st->steps = (int64_t)s.steps;
+  st->finished = s.finished;
+  return RFLU_OK;
+}
+
 int rflu_gpu_reset_stats(int32_t h) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, R->stream, R->sc);
+  hipLaunchKernelGGL(k_reset_counters, dim3(nblk(R->K, 64)), dim3(64), 0, R->stream, R->sc, R->K);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(R->stream));
   return RFLU_OK;

@@ -24,14 +24,13 @@ constexpr int BC_SLIPWALL = 60;
 constexpr int BCOPT_SUPERSONIC = 0;
 constexpr int BCOPT_SUBSONIC = 1;
 constexpr int BCOPT_MIXED = 2;
-constexpr int MAX_PATCHES = 32;
 
-struct PatchTable {
-  int n;
-  int bc[MAX_PATCHES];
-  int opt[MAX_PATCHES];
-  int order2[MAX_PATCHES];
-  double pout[MAX_PATCHES];
+// Boundary patches of a region (all cases), indexed by bpatch[b].
+struct Patches {
+  const int* bc;
+  const int* opt;
+  const int* order2;
+  const double* pout;
 };
 
 struct Gas {
@@ -54,6 +53,28 @@ struct Scalars {
 };
 
 __device__ __forceinline__ bool halted(const Scalars* sc) { return sc->abort || sc->finished; }
+// Early exit of a whole launch: the single-case scalars, or none (nullptr) for a batch
+// of cases, whose kernels run every case and leave halted cases to the update.
+__device__ __forceinline__ bool gate(const Scalars* sc) { return sc && halted(sc); }
+
+// Per-case time-step controls (RFLU_MinimumTimeStep inputs and global%maxTime).
+struct CaseParams {
+  double cfl, dt_imposed, dt_min_limit, end_time;
+};
+
+// Cases of a batched region: K independent cases whose cells, faces and boundary
+// faces occupy consecutive ranges.  With cid == nullptr the region is one case and
+// the kernels use the by-value Gas and the scalars they are given.
+struct Cases {
+  int K;
+  const unsigned short* cid;   // [N] case of each cell, or nullptr
+  const Gas* gas;              // [K]
+  Scalars* sc;                 // [K]
+  const int* cell0;            // [K+1] first cell of each case
+  const int* bface0;           // [K+1] first boundary face of each case
+  const CaseParams* par;       // [K]
+  unsigned long long* dtmin;   // [K] minimum local dt of the step (bits of a positive double)
+};
 
 struct Mesh {
   int N, F, B;
@@ -396,14 +417,15 @@ __device__ __forceinline__ void interior_face_flux(const Mesh& M, const State& S
 // SRAD: also store the face's spectral radius for the time step (the first stage of
 // a fused step sees the step-start state, which is what the time step reads).
 template <int DISCR, int ORDER, typename T, typename GT, typename FT, bool SRAD>
-__global__ void k_face_flux(Mesh M, State S, Gas G, const GT* __restrict__ grad,
+__global__ void k_face_flux(Mesh M, State S, Gas G, Cases C, const GT* __restrict__ grad,
                             FT* __restrict__ flux, double* __restrict__ srad,
                             const Scalars* __restrict__ sc) {
-  if (halted(sc)) return;
+  if (gate(sc)) return;
   int f = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
   if (f >= M.F) return;
   if (SRAD) srad[f] = face_srad<ORDER>(M, S, f);
   T flx[5];
+  if (C.cid) G = C.gas[C.cid[M.f2c[f].x]];
   interior_face_flux<DISCR, ORDER, T, GT>(M, S, G, grad, f, flx);
   const size_t FB = (size_t)M.F + M.B;
 #pragma unroll
@@ -445,15 +467,16 @@ __device__ __forceinline__ void bcond_outflow(int bcOpt, double pout, double sxn
 // rocflu/RFLU_CentralSecondPatch.F90:634-1024).  Slip walls store a zero mass flux.
 // Boundary fluxes stay FP64 in every precision mode (they are few).
 template <int ORDER, typename GT, typename FT, bool SRAD>
-__global__ void k_bface_flux(Mesh M, State S, Gas G, PatchTable P, const GT* __restrict__ grad,
-                             FT* __restrict__ flux, double* __restrict__ srad,
-                             const Scalars* __restrict__ sc) {
-  if (halted(sc)) return;
+__global__ void k_bface_flux(Mesh M, State S, Gas G, Cases C, Patches P,
+                             const GT* __restrict__ grad, FT* __restrict__ flux,
+                             double* __restrict__ srad, const Scalars* __restrict__ sc) {
+  if (gate(sc)) return;
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= M.B) return;
   if (SRAD) srad[M.F + b] = face_srad<ORDER>(M, S, M.F + b);
   const int N = M.N;
   int c1 = M.bf2c[b];
+  if (C.cid) G = C.gas[C.cid[c1]];
   int p = M.bpatch[b];
   double4 n = M.bfn[b];
   double nx = n.x, ny = n.y, nz = n.z, nm = n.w;
@@ -676,17 +699,25 @@ __device__ __forceinline__ void rk_update_cell(const State& S, int N, int i, int
 // FUSED) for every cell.  Stage-start state is the current cv, so Wray and Euler
 // need no separate cvOld copy.
 template <int RK, typename FT, bool FUSED, bool WRITE_PRIM>
-__global__ void k_update(Mesh M, State S, Gas G, const FT* __restrict__ flux, Scalars* sc,
-                         int stage, int last_stage, double dt_in) {
-  if (halted(sc)) return;
+__global__ void k_update(Mesh M, State S, Gas G, Cases C, const FT* __restrict__ flux,
+                         Scalars* sc, int stage, int last_stage, double dt_in) {
+  if (!C.cid && halted(sc)) return;
   const int N = M.N;
   int i = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int first = 0;
+  if (C.cid) {
+    const int c = C.cid[i];
+    sc = C.sc + c;
+    if (halted(sc)) return;   // this case is done or failed: its state stays
+    G = C.gas[c];
+    first = C.cell0[c];
+  }
   double dt = dt_in > 0.0 ? dt_in : sc->dt;
-  if (last_stage && FUSED && i == 0) {
+  if (last_stage && FUSED && i == first) {
     sc->time = sc->time + dt;
     sc->steps = sc->steps + 1;
   }
-  if (i >= N) return;
   double acc[5];
   gather_residual<FT>(M, flux, i, acc);
   double q[5];
@@ -714,10 +745,11 @@ __global__ void k_update(Mesh M, State S, Gas G, const FT* __restrict__ flux, Sc
 }
 
 // Dependent variables (and the primitive copy used by second-order kernels).
-__global__ void k_set_dv(Mesh M, State S, Gas G) {
+__global__ void k_set_dv(Mesh M, State S, Gas G, Cases C) {
   const int N = M.N;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  if (C.cid) G = C.gas[C.cid[i]];
   double p, t, a;
   double r = S.cv[i];
   dep_vars(G, r, S.cv[(size_t)N + i], S.cv[2 * (size_t)N + i], S.cv[3 * (size_t)N + i],
@@ -744,10 +776,14 @@ __global__ void k_set_prim(Mesh M, State S) {
 
 // After-update checks as a separate pass (the unfused entry points).
 template <bool WRITE_PRIM>
-__global__ void k_after_update(Mesh M, State S, Gas G, Scalars* sc) {
+__global__ void k_after_update(Mesh M, State S, Gas G, Cases C, Scalars* sc) {
   const int N = M.N;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  if (C.cid) {
+    G = C.gas[C.cid[i]];
+    sc = C.sc + C.cid[i];
+  }
   double q[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) q[k] = S.cv[(size_t)k * N + i];
@@ -886,7 +922,7 @@ __device__ __forceinline__ void venkat_limit(const Mesh& M, int i, const double*
 // One thread per cell, members read from global memory (L2/MALL).
 template <typename GT, bool VENKAT, int U>
 __global__ void k_gradient(Mesh M, State S, GT* __restrict__ grad, const Scalars* __restrict__ sc) {
-  if (halted(sc)) return;
+  if (gate(sc)) return;
   const int N = M.N;
   int i = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -994,7 +1030,7 @@ __global__ __launch_bounds__(128) void k_gradient_lds(Mesh M, State S, Tiles T,
                                                       GT* __restrict__ grad,
                                                       const Scalars* __restrict__ sc) {
   extern __shared__ double lds[];
-  if (halted(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
+  if (gate(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
   const int N = M.N;
   const int cap = T.cap;
   const double* lx = lds;
@@ -1105,7 +1141,7 @@ __global__ __launch_bounds__(128) void k_gradient_ldsw(Mesh M, State S, Tiles T,
                                                        GT* __restrict__ grad,
                                                        const Scalars* __restrict__ sc) {
   extern __shared__ double lds[];
-  if (halted(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
+  if (gate(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
   const int N = M.N;
   const int cap = T.cap;
   const double* lv = lds;  // 5 arrays: rho, u, v, w, p
@@ -1180,7 +1216,7 @@ __global__ __launch_bounds__(128) void k_gradient_ldsw(Mesh M, State S, Tiles T,
 template <typename GT>
 __global__ void k_weno_clip(Mesh M, State S, const GT* __restrict__ gin, GT* __restrict__ gout,
                             const Scalars* __restrict__ sc) {
-  if (halted(sc)) return;
+  if (gate(sc)) return;
   const int N = M.N;
   int i = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -1241,7 +1277,7 @@ __device__ __forceinline__ double wave_min(double v) {
 // Face spectral radii as a pass of their own (the unfused time-step entry point).
 __global__ void k_face_srad(Mesh M, State S, double* __restrict__ srad,
                             const Scalars* __restrict__ sc) {
-  if (halted(sc)) return;
+  if (gate(sc)) return;
   int f = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
   if (f >= M.F + M.B) return;
   srad[f] = face_srad<1>(M, S, f);
@@ -1276,10 +1312,36 @@ __global__ void k_dt_gather(Mesh M, const double* __restrict__ srad, double* __r
   }
 }
 
-// time_in >= 0 sets global%currentTime from the host first (the unfused entry point).
+// RFLU_MinimumTimeStep's rules (rocflu/RFLU_MinimumTimeStep.F90:126-245) for one case
+// given its minimum local dt, then the end-time clamp of RFLU_TimeStepping
+// (:445-448).  time_in >= 0 sets global%currentTime from the host first (the unfused
+// entry point).
+__device__ __forceinline__ void finalize_dt(Scalars* sc, double mm, const CaseParams& cp,
+                                            double time_in) {
+  sc->dt_local_min = mm;
+  if (time_in >= 0.0) sc->time = time_in;
+  double wI = 0.0, wC = 0.0;
+  double d = cp.cfl * mm;
+  if (sc->time <= 1.0E-08) wI = 1.0;
+  else wC = 1.0;
+  d = wI / cp.dt_imposed + wC / d;
+  d = 1.0 / d;
+  double dtMin = HUGE_VAL;
+  if (d < dtMin) dtMin = d;
+  if (dtMin <= 0.0) dtMin = 6.0E-12;
+  dtMin = dtMin > cp.dt_min_limit ? dtMin : cp.dt_min_limit;
+  if (sc->last_step) sc->finished = 1;
+  if (!halted(sc)) {
+    if (sc->time + dtMin > cp.end_time) {
+      dtMin = cp.end_time - sc->time;
+      sc->last_step = 1;
+    }
+    sc->dt = dtMin;
+  }
+}
+
 __global__ void k_dt_finalize(const double* __restrict__ part, int nparts, Scalars* sc,
-                              double cfl, double dt_imposed, double dt_min_limit, double time_in,
-                              double end_time) {
+                              CaseParams cp, double time_in) {
   __shared__ double red[16];
   double m = HUGE_VAL;
   for (int k = threadIdx.x; k < nparts; k += blockDim.x) m = fmin(m, part[k]);
@@ -1290,38 +1352,46 @@ __global__ void k_dt_finalize(const double* __restrict__ part, int nparts, Scala
   if (threadIdx.x == 0) {
     double mm = red[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mm = fmin(mm, red[w]);
-    sc->dt_local_min = mm;
-    if (time_in >= 0.0) sc->time = time_in;
-    double wI = 0.0, wC = 0.0;
-    double d = cfl * mm;
-    if (sc->time <= 1.0E-08) wI = 1.0;
-    else wC = 1.0;
-    d = wI / dt_imposed + wC / d;
-    d = 1.0 / d;
-    double dtMin = HUGE_VAL;
-    if (d < dtMin) dtMin = d;
-    if (dtMin <= 0.0) dtMin = 6.0E-12;
-    dtMin = dtMin > dt_min_limit ? dtMin : dt_min_limit;
-    if (sc->last_step) sc->finished = 1;
-    if (!halted(sc)) {
-      if (sc->time + dtMin > end_time) {
-        dtMin = end_time - sc->time;
-        sc->last_step = 1;
-      }
-      sc->dt = dtMin;
-    }
+    finalize_dt(sc, mm, cp, time_in);
   }
 }
 
+// Batched cases: per-case minimum of vol/sum(srad) by an order-independent atomic
+// minimum on the bits of the (positive) local time steps.
+__global__ void k_dt_gather_cases(Mesh M, Cases C, const double* __restrict__ srad) {
+  const int N = M.N;
+  int i = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int c = C.cid[i];
+  if (halted(C.sc + c)) return;
+  double sum = 0.0;
+  int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
+  for (int e = e0; e < e1; ++e) {
+    int sgn = M.c2f_idx[e];
+    sum = sum + srad[(sgn > 0 ? sgn : -sgn) - 1];
+  }
+  double dtc = M.cell[i].w / sum;
+  atomicMin(C.dtmin + c, (unsigned long long)__double_as_longlong(dtc));
+}
+
+__global__ void k_dt_finalize_cases(Cases C) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C.K) return;
+  unsigned long long bits = C.dtmin[c];
+  C.dtmin[c] = 0x7FF0000000000000ULL;   // +inf for the next step
+  finalize_dt(C.sc + c, __longlong_as_double((long long)bits), C.par[c], -1.0);
+}
+
 // Outflow mass bookkeeping of stage 1 (rocflu/RFLU_CentralFirstPatch.F90:697-700):
-// one block sums the outflow faces' mass fluxes in face order per thread stride.
+// block c sums case c's outflow faces' mass fluxes in face order per thread stride.
 template <typename FT>
-__global__ void k_mass_inout(Mesh M, PatchTable P, const FT* __restrict__ flux, Scalars* sc) {
+__global__ void k_mass_inout(Mesh M, Cases C, Patches P, const FT* __restrict__ flux) {
   __shared__ double sin_[256], sout[256];
-  const size_t FB = (size_t)M.F + M.B;
+  const int c = blockIdx.x;
+  Scalars* sc = C.sc + c;
   double a = 0.0, b = 0.0;
   if (!sc->abort) {
-    for (int k = threadIdx.x; k < M.B; k += blockDim.x) {
+    for (int k = C.bface0[c] + threadIdx.x; k < C.bface0[c + 1]; k += blockDim.x) {
       int p = M.bpatch[k];
       if (P.bc[p] != BC_OUTFLOW) continue;
       double m = double(flux[M.F + k]);
@@ -1339,43 +1409,46 @@ __global__ void k_mass_inout(Mesh M, PatchTable P, const FT* __restrict__ flux, 
     }
     __syncthreads();
   }
-  (void)FB;
   if (threadIdx.x == 0 && !sc->abort) {
     sc->mass_in = sc->mass_in + sin_[0];
     sc->mass_out = sc->mass_out + sout[0];
   }
 }
 
-__global__ void k_fill_scalars(Scalars* sc, double time) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    sc->time = time;
-    sc->dt = 0.0;
-    sc->dt_local_min = 0.0;
-    sc->mass_in = 0.0;
-    sc->mass_out = 0.0;
-    sc->n_invalid = 0;
-    sc->n_negative = 0;
-    sc->n_bounds_fix = 0;
-    sc->n_pos_fix = 0;
-    sc->steps = 0;
-    sc->abort = 0;
-    sc->last_step = 0;
-    sc->finished = 0;
-  }
+__global__ void k_fill_scalars(Scalars* scs, int K, double time,
+                               unsigned long long* dtmin) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= K) return;
+  Scalars* sc = scs + c;
+  sc->time = time;
+  sc->dt = 0.0;
+  sc->dt_local_min = 0.0;
+  sc->mass_in = 0.0;
+  sc->mass_out = 0.0;
+  sc->n_invalid = 0;
+  sc->n_negative = 0;
+  sc->n_bounds_fix = 0;
+  sc->n_pos_fix = 0;
+  sc->steps = 0;
+  sc->abort = 0;
+  sc->last_step = 0;
+  sc->finished = 0;
+  dtmin[c] = 0x7FF0000000000000ULL;
 }
 
-__global__ void k_reset_counters(Scalars* sc) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    sc->mass_in = 0.0;
-    sc->mass_out = 0.0;
-    sc->n_invalid = 0;
-    sc->n_negative = 0;
-    sc->n_bounds_fix = 0;
-    sc->n_pos_fix = 0;
-    sc->abort = 0;
-    sc->last_step = 0;
-    sc->finished = 0;
-  }
+__global__ void k_reset_counters(Scalars* scs, int K) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= K) return;
+  Scalars* sc = scs + c;
+  sc->mass_in = 0.0;
+  sc->mass_out = 0.0;
+  sc->n_invalid = 0;
+  sc->n_negative = 0;
+  sc->n_bounds_fix = 0;
+  sc->n_pos_fix = 0;
+  sc->abort = 0;
+  sc->last_step = 0;
+  sc->finished = 0;
 }
 
 }  // namespace rflu

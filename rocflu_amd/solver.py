@@ -85,6 +85,40 @@ def region_arrays(mesh: Mesh, num: Numerics) -> dict:
     return keep
 
 
+def region_desc(mesh: Mesh, num: Numerics, d: "nat.RegionDesc"):
+    """Fills a descriptor for one region; returns the arrays it points at (keep them
+    alive until the region is created)."""
+    k = region_arrays(mesh, num)
+    i32, f64 = ctypes.c_int32, ctypes.c_double
+    d.index_base = 0
+    d.n_cells = int(mesh.n_cells)
+    d.n_faces = int(mesh.n_faces)
+    d.n_patches = int(mesh.n_patches)
+    d.f2c = nat.ptr(k["f2c"], i32)
+    d.fn = nat.ptr(k["fn"], f64)
+    d.fc = nat.ptr(k["fc"], f64)
+    d.cofg = nat.ptr(k["cofg"], f64)
+    d.vol = nat.ptr(k["vol"], f64)
+    d.bpatch_ptr = nat.ptr(k["bpatch_ptr"], i32)
+    d.bf2c = nat.ptr(k["bf2c"], i32)
+    d.bfn = nat.ptr(k["bfn"], f64)
+    d.bfc = nat.ptr(k["bfc"], f64)
+    d.patch_bc = nat.ptr(k["patch_bc"], i32)
+    d.patch_opt = nat.ptr(k["patch_opt"], i32)
+    d.patch_pout = nat.ptr(k["patch_pout"], f64)
+    d.patch_order = nat.ptr(k["patch_order"], i32)
+    if num.order == 2:
+        d.c2cs_ptr = nat.ptr(k["c2cs_ptr"], i32)
+        d.c2cs_idx = nat.ptr(k["c2cs_idx"], i32)
+        d.xyz_moms = nat.ptr(k["xyz_moms"], f64)
+    d.gamma, d.cp = num.gamma, num.cp
+    d.discr, d.order, d.reconst, d.rk, d.precision = (
+        num.discr, num.order, num.reconst, num.rk, num.precision)
+    d.cfl, d.dt_imposed, d.dt_min_limit = num.cfl, num.dt_imposed, num.dt_min_limit
+    d.eps_entr, d.diss_fact = num.eps_entr, num.diss_fact
+    return k
+
+
 class GpuRegion:
     """One region on the GPU behind the C-ABI."""
 
@@ -93,40 +127,20 @@ class GpuRegion:
         self.num = num
         self.n = int(mesh.n_cells)
         L = nat.lib()
-        k = region_arrays(mesh, num)
-        i32, f64 = ctypes.c_int32, ctypes.c_double
         d = nat.RegionDesc()
-        d.index_base = 0
-        d.n_cells = self.n
-        d.n_faces = int(mesh.n_faces)
-        d.n_patches = int(mesh.n_patches)
-        d.f2c = nat.ptr(k["f2c"], i32)
-        d.fn = nat.ptr(k["fn"], f64)
-        d.fc = nat.ptr(k["fc"], f64)
-        d.cofg = nat.ptr(k["cofg"], f64)
-        d.vol = nat.ptr(k["vol"], f64)
-        d.bpatch_ptr = nat.ptr(k["bpatch_ptr"], i32)
-        d.bf2c = nat.ptr(k["bf2c"], i32)
-        d.bfn = nat.ptr(k["bfn"], f64)
-        d.bfc = nat.ptr(k["bfc"], f64)
-        d.patch_bc = nat.ptr(k["patch_bc"], i32)
-        d.patch_opt = nat.ptr(k["patch_opt"], i32)
-        d.patch_pout = nat.ptr(k["patch_pout"], f64)
-        d.patch_order = nat.ptr(k["patch_order"], i32)
-        if num.order == 2:
-            d.c2cs_ptr = nat.ptr(k["c2cs_ptr"], i32)
-            d.c2cs_idx = nat.ptr(k["c2cs_idx"], i32)
-            d.xyz_moms = nat.ptr(k["xyz_moms"], f64)
-        d.gamma, d.cp = num.gamma, num.cp
-        d.discr, d.order, d.reconst, d.rk, d.precision = (
-            num.discr, num.order, num.reconst, num.rk, num.precision)
-        d.cfl, d.dt_imposed, d.dt_min_limit = num.cfl, num.dt_imposed, num.dt_min_limit
-        d.eps_entr, d.diss_fact = num.eps_entr, num.diss_fact
+        keep = region_desc(mesh, num, d)
         h = ctypes.c_int32(0)
         nat.check(L.rflu_gpu_create_region(ctypes.byref(d), ctypes.byref(h)),
                   "rflu_gpu_create_region")
+        del keep
         self.handle = h.value
         self._lib = L
+
+    def signature(self) -> str:
+        m, num = self.mesh, self.num
+        s = int(m.c2cs_idx.size) if (num.order == 2 and m.has_stencils) else 0
+        return (f"N{m.n_cells}-F{m.n_faces}-B{m.n_bfaces}-S{s}"
+                f"-d{num.discr}-o{num.order}-r{num.reconst}-k{num.rk}-p{num.precision}")
 
     # -- state ---------------------------------------------------------------
     def upload(self, cv: np.ndarray, dv: Optional[np.ndarray] = None, time: float = 0.0):
@@ -267,3 +281,57 @@ class GpuRegion:
             self.close()
         except Exception:
             pass
+
+
+class GpuBatch(GpuRegion):
+    """Independent cases (meshes with their own gas constants, CFL, dt controls and
+    patches; one scheme) in one handle: every launch of the stage covers all cases,
+    and each case keeps its own time, time step, end time and checks
+    (``rflu_gpu_create_batch``).  State arrays are lists, one (n_cells, 5) array per
+    case."""
+
+    def __init__(self, cases: Sequence[tuple]):
+        if not cases:
+            raise ValueError("a batch needs at least one case")
+        self.meshes = [m for m, _ in cases]
+        self.nums = [n for _, n in cases]
+        self.mesh, self.num = self.meshes[0], self.nums[0]
+        self.sizes = [int(m.n_cells) for m in self.meshes]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.n = int(self.offsets[-1])
+        self.k = len(cases)
+        L = nat.lib()
+        descs = (nat.RegionDesc * self.k)()
+        keep = [region_desc(m, num, descs[i]) for i, (m, num) in enumerate(cases)]
+        h = ctypes.c_int32(0)
+        nat.check(L.rflu_gpu_create_batch(self.k, descs, ctypes.byref(h)), "rflu_gpu_create_batch")
+        del keep
+        self.handle = h.value
+        self._lib = L
+
+    def signature(self) -> str:
+        return f"batch{self.k}-N{self.n}-" + super().signature()
+
+    def upload(self, cvs, dv=None, time: float = 0.0):
+        if len(cvs) != self.k:
+            raise ValueError("one state per case")
+        cv = np.concatenate([np.asarray(c, dtype=np.float64).reshape(-1, 5) for c in cvs])
+        super().upload(cv, None, time)
+
+    def download(self):
+        cv, dv = super().download()
+        o = self.offsets
+        return ([cv[o[i]:o[i + 1]] for i in range(self.k)],
+                [dv[o[i]:o[i + 1]] for i in range(self.k)])
+
+    def case_stats(self, icase: int) -> dict:
+        s = nat.GpuStats()
+        nat.check(self._lib.rflu_gpu_get_case_stats(self.handle, int(icase), ctypes.byref(s)),
+                  "rflu_gpu_get_case_stats")
+        return {name: getattr(s, name) for name, _ in nat.GpuStats._fields_}
+
+    def time_step(self, current_time=None):
+        raise NotImplementedError("a batch advances through time_steps")
+
+    def runge_kutta(self, end_time=None):
+        raise NotImplementedError("a batch advances through time_steps")

@@ -1,0 +1,100 @@
+"""Batched cases (rflu_gpu_create_batch): every case of a batch must give the bits of
+the same case run as a region of its own, and the oracle's result to the FP64
+tolerance, while cases differ in mesh, gamma, Mach number, CFL and case type."""
+import numpy as np
+import pytest
+
+from rocflu_amd import cases
+from rocflu_amd.solver import GpuBatch, GpuRegion, Numerics
+from rocflu_amd import _native as nat
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-12
+
+
+def rel_l2(a, b):
+    return np.array([np.linalg.norm(a[:, k] - b[:, k]) / max(np.linalg.norm(b[:, k]), 1e-300)
+                     for k in range(5)])
+
+
+def small_ensemble(k=5, precision=0, reconst=11, rk=4):
+    ens = cases.ensemble(n_cases=k, min_cells=700, max_cells=4000, seed=11, precision=precision,
+                         reconst=reconst, rk=rk, dt_imposed=1e-3)
+    # vary the CFL too
+    for i, (_, num, _) in enumerate(ens):
+        num.cfl = 0.3 + 0.1 * (i % 3)
+    return ens
+
+
+def single(m, num, cv0, nsteps, end_time=None):
+    g = GpuRegion(m, num)
+    try:
+        g.upload(cv0)
+        if end_time is not None:
+            g.set_end_time(end_time)
+        g.time_steps(nsteps, graph=True)
+        return g.download()[0], g.stats()
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("precision,reconst", [(0, 11), (1, 11), (0, 2)])
+def test_batch_matches_single_regions(gpu, precision, reconst):
+    ens = small_ensemble(precision=precision, reconst=reconst)
+    print("case sizes", [m.n_cells for m, _, _ in ens], "gammas", [n.gamma for _, n, _ in ens],
+          flush=True)
+    nsteps = 12
+    b = GpuBatch([(m, num) for m, num, _ in ens])
+    try:
+        b.upload([cv0 for _, _, cv0 in ens])
+        b.time_steps(nsteps, graph=True)
+        outs, _ = b.download()
+        stats = [b.case_stats(i) for i in range(len(ens))]
+    finally:
+        b.close()
+    for i, (m, num, cv0) in enumerate(ens):
+        ref, st = single(m, num, cv0, nsteps)
+        assert np.array_equal(outs[i], ref), f"case {i}"
+        assert stats[i]["time"] == st["time"] and stats[i]["steps"] == nsteps
+        if precision == 0:
+            o = Oracle(m, num)
+            cvo, _, ost = o.run(cv0, nsteps)
+            err = rel_l2(outs[i], cvo)
+            print(f"case {i}: rel L2 vs oracle {err}", flush=True)
+            assert np.all(err <= TOL_FP64), (i, err)
+    # cases have their own time steps
+    assert len({s["time"] for s in stats}) > 1
+
+
+def test_batch_end_times_and_a_failing_case(gpu):
+    """Cases reach the common end time after different step counts and stop there;
+    a case that fails its positivity check stops alone."""
+    ens = small_ensemble(k=4)
+    bad = 2
+    cvs = [cv0.copy() for _, _, cv0 in ens]
+    cvs[bad][0, 0] = -1.0        # negative density in one cell of one case
+    end = 0.5
+    b = GpuBatch([(m, num) for m, num, _ in ens])
+    try:
+        b.upload(cvs)
+        b.set_end_time(end)
+        with pytest.raises(nat.RfluError) as ei:
+            b.time_steps(400, graph=True)
+        assert ei.value.code in (nat.RFLU_ERR_NEGATIVE_POSDEF, nat.RFLU_ERR_INVALID_VALUE)
+        outs, _ = b.download()
+        stats = [b.case_stats(i) for i in range(len(ens))]
+    finally:
+        b.close()
+    # the failed case stopped inside its first step; its clock never advanced
+    assert stats[bad]["n_negative"] + stats[bad]["n_invalid"] > 0
+    assert stats[bad]["steps"] == 0 and stats[bad]["time"] == 0.0
+    for i, (m, num, cv0) in enumerate(ens):
+        if i == bad:
+            continue
+        ref, st = single(m, num, cv0, 400, end_time=end)
+        assert st["finished"] == 1 and st["time"] == end
+        assert stats[i]["time"] == end and stats[i]["finished"] == 1
+        assert stats[i]["steps"] == st["steps"]
+        assert np.array_equal(outs[i], ref), f"case {i}"
