@@ -157,6 +157,107 @@ def run(args):
     return m, results, finals
 
 
+def batch_leg(args):
+    """The batched ensemble: independent shock-tube and blast cases of 20k-500k tets
+    each, Mach 1.2-3.0, gamma 1.3-1.67, mixed precision HLLC at second order with
+    the Venkatakrishnan limiter, SSP-RK3; every kernel launch covers all cases and one
+    step of all of them is one captured graph.  Also timed: the same cases advanced
+    one after the other as regions of their own (each with its own graph)."""
+    from rocflu_amd import cases, autotune
+    from rocflu_amd.solver import GpuBatch, GpuRegion
+    t0 = time.time()
+    ens = cases.ensemble(n_cases=args.batch_cases, seed=5)
+    n_tot = sum(int(m.n_cells) for m, _, _ in ens)
+    print(f"# batch: {len(ens)} cases, {n_tot} tets, built in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    b = GpuBatch([(m, num) for m, num, _ in ens])
+    cvs = [cv0 for _, _, cv0 in ens]
+    b.upload(cvs)
+    tuning = autotune.tune(b)
+    b.upload(cvs)
+    b.time_steps(args.warmup, graph=True)
+    b.synchronize()
+    t1 = time.perf_counter()
+    b.time_steps(args.batch_steps, graph=True)
+    b.synchronize()
+    el = time.perf_counter() - t1
+    st = [b.case_stats(i) for i in range(len(ens))]
+    b.close()
+    # the same cases one region at a time
+    t_seq = 0.0
+    for m, num, cv0 in ens:
+        g = GpuRegion(m, num)
+        g.upload(cv0)
+        autotune.apply(g, tuning)
+        g.time_steps(args.warmup, graph=True)
+        g.synchronize()
+        t2 = time.perf_counter()
+        g.time_steps(args.batch_steps, graph=True)
+        g.synchronize()
+        t_seq += time.perf_counter() - t2
+        g.close()
+    n_upd = n_tot * 3 * args.batch_steps
+    out = {"value": n_upd / el, "unit": "cell-updates/s", "cases": len(ens), "cells": n_tot,
+           "steps": args.batch_steps, "ms_per_step": 1e3 * el / args.batch_steps,
+           "sequential_value": n_upd / t_seq, "speedup_vs_sequential": t_seq / el,
+           "dtype": "f32/f64", "failed_cases": sum(1 for x in st if x["n_invalid"] or x["n_negative"]),
+           "tuning": {k: v for k, v in tuning.items() if k != "timings_ms"},
+           "workload": "64 Kuhn-tet boxes, cell counts U(20k, 500k), shock Mach U(1.2, 3.0), "
+                       "gamma U(1.3, 1.67), planar shocks and blasts, HLLC order 2 + "
+                       "Venkatakrishnan, SSP-RK3, mixed precision, slip walls"}
+    print(f"# batch: {out['value']:.4g} cell-upd/s batched, {out['sequential_value']:.4g} one "
+          f"region at a time", file=sys.stderr, flush=True)
+    return out
+
+
+def large_leg(args):
+    """The large single-GPU mesh: 160^3 x 6 Kuhn tets (24.6M cells), the 80^3 case's
+    scheme, FP64 and mixed precision; per-variable error of the mixed run against the
+    FP64 run after the same number of steps."""
+    from rocflu_amd import cases, autotune
+    from rocflu_amd.solver import GpuRegion
+    t0 = time.time()
+    n = args.large_n
+    m = cases.tet_box(n=n, jitter=0.08, seed=2)
+    print(f"# large mesh {n}^3x6: {m.n_cells} tets in {time.time() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    out = {"workload": f"{n}^3 x 6 Kuhn tets ({m.n_cells} cells), jitter 8%, Mach-2 shock + 1% "
+                       f"density noise, HLLC order 2, WLSQ + Venkatakrishnan, SSP-RK3",
+           "cells": int(m.n_cells)}
+    finals = {}
+    for prec in (0, 1):
+        num = build_case(args, prec)
+        cv0 = cases.planar_shock(m, num, 0.3 * n, mach=2.0, noise=0.01, seed=7)
+        g = GpuRegion(m, num)
+        g.upload(cv0)
+        tuning = autotune.tune(g)
+        g.upload(cv0)
+        g.time_steps(args.warmup, graph=True)
+        g.synchronize()
+        t1 = time.perf_counter()
+        g.time_steps(args.large_steps, graph=True)
+        g.synchronize()
+        el = time.perf_counter() - t1
+        g.upload(cv0)
+        g.time_steps(args.large_err_steps, graph=True)
+        finals[prec] = g.download()[0]
+        g.close()
+        key = "fp64" if prec == 0 else "mixed"
+        out[key] = {"value": m.n_cells * 3 * args.large_steps / el,
+                    "ms_per_step": 1e3 * el / args.large_steps,
+                    "tuning": {k: v for k, v in tuning.items() if k != "timings_ms"}}
+        print(f"# large prec {prec}: {out[key]['value']:.4g} cell-upd/s", file=sys.stderr,
+              flush=True)
+    a, b = finals[1], finals[0]
+    out["err_steps"] = args.large_err_steps
+    out["mixed_rel_l2_per_variable"] = [
+        float(np.linalg.norm(a[:, k] - b[:, k]) / np.linalg.norm(b[:, k])) for k in range(5)]
+    mom = np.linalg.norm(b[:, 1:4])
+    out["mixed_rel_l2_momentum_vs_momentum_norm"] = [
+        float(np.linalg.norm(a[:, k] - b[:, k]) / mom) for k in (1, 2, 3)]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +271,13 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-cases", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--large-n", type=int, default=160)
+    ap.add_argument("--large-steps", type=int, default=5)
+    ap.add_argument("--large-err-steps", type=int, default=50)
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain device copy of 512 MiB three times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -245,6 +353,17 @@ def main():
                         "err_steps": args.err_steps,
                         "rel_l2_per_variable": errs,
                         "rel_l2_momentum_vs_momentum_norm": errs_mom}
+    # secondary workloads: a failure is reported in the line instead of losing it
+    if not args.no_batch:
+        try:
+            out["batch"] = batch_leg(args)
+        except Exception as e:  # noqa: BLE001
+            out["batch"] = {"error": repr(e)}
+    if not args.no_large:
+        try:
+            out["large"] = large_leg(args)
+        except Exception as e:  # noqa: BLE001
+            out["large"] = {"error": repr(e)}
     if not args.no_cpu:
         cb = cpu_baseline(args)
         out["cpu_baseline"] = cb

@@ -888,7 +888,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       (rc = dalloc(&R->dt_part, R->n_dt_parts)) || (rc = dalloc(&R->sc, K)) ||
       (rc = dalloc(&R->srad, FB)) || (rc = dalloc(&R->gas_tab, K)) || (rc = dalloc(&R->par, K)) ||
       (rc = dalloc(&R->cell0, K + 1)) || (rc = dalloc(&R->bface0, K + 1)) ||
-      (rc = dalloc(&R->dtmin, K)) || (rc = dalloc(&R->pbc, P)) || (rc = dalloc(&R->popt, P)) ||
+      (rc = dalloc(&R->dtmin, (size_t)K * DTMIN_STRIDE)) || (rc = dalloc(&R->pbc, P)) || (rc = dalloc(&R->popt, P)) ||
       (rc = dalloc(&R->porder2, P)) || (rc = dalloc(&R->ppout, P)) ||
       (K > 1 && (rc = dalloc(&R->cid, N)))) {
     free_region(R);

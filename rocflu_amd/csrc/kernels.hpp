@@ -62,6 +62,9 @@ struct CaseParams {
   double cfl, dt_imposed, dt_min_limit, end_time;
 };
 
+// Slots of the per-case minimum time step are this many 8-byte words apart.
+constexpr int DTMIN_STRIDE = 32;
+
 // Cases of a batched region: K independent cases whose cells, faces and boundary
 // faces occupy consecutive ranges.  With cid == nullptr the region is one case and
 // the kernels use the by-value Gas and the scalars they are given.
@@ -73,7 +76,8 @@ struct Cases {
   const int* cell0;            // [K+1] first cell of each case
   const int* bface0;           // [K+1] first boundary face of each case
   const CaseParams* par;       // [K]
-  unsigned long long* dtmin;   // [K] minimum local dt of the step (bits of a positive double)
+  unsigned long long* dtmin;   // [K * DTMIN_STRIDE] minimum local dt of the step (bits of a
+                               // positive double) at c * DTMIN_STRIDE
 };
 
 struct Mesh {
@@ -1357,28 +1361,65 @@ __global__ void k_dt_finalize(const double* __restrict__ part, int nparts, Scala
 }
 
 // Batched cases: per-case minimum of vol/sum(srad) by an order-independent atomic
-// minimum on the bits of the (positive) local time steps.
+// minimum on the bits of the (positive) local time steps.  Wavefronts whose lanes all
+// belong to one case (all but the few straddling a case boundary) reduce first, then
+// the block merges its wavefronts, so a block issues one atomic per case it covers.
+// Each case's slot sits on its own 256-byte line (DTMIN_STRIDE doubles apart).
 __global__ void k_dt_gather_cases(Mesh M, Cases C, const double* __restrict__ srad) {
+  __shared__ int wc[16];
+  __shared__ double wm[16];
   const int N = M.N;
   int i = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const int c = C.cid[i];
-  if (halted(C.sc + c)) return;
-  double sum = 0.0;
-  int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
-  for (int e = e0; e < e1; ++e) {
-    int sgn = M.c2f_idx[e];
-    sum = sum + srad[(sgn > 0 ? sgn : -sgn) - 1];
+  int c = -1;
+  double dtc = HUGE_VAL;
+  if (i < N) {
+    c = C.cid[i];
+    if (!halted(C.sc + c)) {
+      double sum = 0.0;
+      int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
+      for (int e = e0; e < e1; ++e) {
+        int sgn = M.c2f_idx[e];
+        sum = sum + srad[(sgn > 0 ? sgn : -sgn) - 1];
+      }
+      dtc = M.cell[i].w / sum;
+    }
   }
-  double dtc = M.cell[i].w / sum;
-  atomicMin(C.dtmin + c, (unsigned long long)__double_as_longlong(dtc));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int c0 = __shfl(c, 0, 64);
+  if (__all(c == c0)) {
+    dtc = wave_min(dtc);
+    if (lane == 0) {
+      wc[wv] = c0;
+      wm[wv] = dtc;
+    }
+  } else {
+    if (c >= 0 && dtc < HUGE_VAL)
+      atomicMin(C.dtmin + (size_t)c * DTMIN_STRIDE, (unsigned long long)__double_as_longlong(dtc));
+    if (lane == 0) wc[wv] = -1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (int)(blockDim.x >> 6);
+    int cc = -1;
+    double m = HUGE_VAL;
+    for (int w = 0; w <= nw; ++w) {
+      const int cw = w < nw ? wc[w] : -1;
+      if (cw != cc) {
+        if (cc >= 0 && m < HUGE_VAL)
+          atomicMin(C.dtmin + (size_t)cc * DTMIN_STRIDE, (unsigned long long)__double_as_longlong(m));
+        cc = cw;
+        m = HUGE_VAL;
+      }
+      if (cw >= 0) m = fmin(m, wm[w]);
+    }
+  }
 }
 
 __global__ void k_dt_finalize_cases(Cases C) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C.K) return;
-  unsigned long long bits = C.dtmin[c];
-  C.dtmin[c] = 0x7FF0000000000000ULL;   // +inf for the next step
+  unsigned long long bits = C.dtmin[(size_t)c * DTMIN_STRIDE];
+  C.dtmin[(size_t)c * DTMIN_STRIDE] = 0x7FF0000000000000ULL;   // +inf for the next step
   finalize_dt(C.sc + c, __longlong_as_double((long long)bits), C.par[c], -1.0);
 }
 
@@ -1433,7 +1474,7 @@ __global__ void k_fill_scalars(Scalars* scs, int K, double time,
   sc->abort = 0;
   sc->last_step = 0;
   sc->finished = 0;
-  dtmin[c] = 0x7FF0000000000000ULL;
+  dtmin[(size_t)c * DTMIN_STRIDE] = 0x7FF0000000000000ULL;
 }
 
 __global__ void k_reset_counters(Scalars* scs, int K) {
