@@ -82,8 +82,10 @@ typedef struct {
   const int32_t* bf2c;       /* (nBFaces)                                            */
   const double* bfn;         /* (4, nBFaces)                                         */
   const double* bfc;         /* (3, nBFaces)                                         */
-  const int32_t* patch_bc;   /* (n_patches) 20 outflow, 60 slip wall                 */
-  const int32_t* patch_opt;  /* (n_patches) outflow option 0 sup., 1 sub., 2 mixed   */
+  const int32_t* patch_bc;   /* (n_patches) 11 inflow (BC_INFLOW_VELTEMP), 20 outflow,
+                                60 slip wall (modfloflu/ModParameters.F90:397-400)   */
+  const int32_t* patch_opt;  /* (n_patches) BCSWI_INFLOW_TYPE / BCSWI_OUTFLOW_TYPE:
+                                0 supersonic, 1 subsonic, 2 mixed                    */
   const double* patch_pout;  /* (n_patches) outflow static pressure                  */
   const int32_t* patch_order;/* (n_patches) 1 or 2; NULL = all 1 (reference default) */
   const int32_t* c2cs_ptr;   /* (n_cells+1), order 2 only                            */
@@ -92,6 +94,9 @@ typedef struct {
   double gamma, cp;          /* global%refGamma, global%refCp                        */
   int32_t discr, order, reconst, rk, precision;
   double cfl, dt_imposed, dt_min_limit, eps_entr, diss_fact;
+  const double* patch_vals;  /* (5, n_patches) inflow BCDAT_INFLOW_U, _V, _W, _T, _P
+                                (P used by the non-subsonic options); NULL when no
+                                patch is an inflow                                   */
 } rflu_region_desc;
 
 /* Per-region counters of the after-update checks, accumulated since the last reset. */

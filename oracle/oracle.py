@@ -32,6 +32,7 @@ class OrcRegion(ctypes.Structure):
         ("cfl", ctypes.c_double), ("dt_imposed", ctypes.c_double),
         ("dt_min_limit", ctypes.c_double), ("eps_entr", ctypes.c_double),
         ("diss_fact", ctypes.c_double),
+        ("patch_vals", _f64p),
     ]
 
 
@@ -102,7 +103,7 @@ class Oracle:
         for name in ("f2c", "bpatch_ptr", "bf2c", "patch_bc", "patch_opt", "patch_order",
                      "c2f_ptr", "c2f_idx"):
             setattr(r, name, _p(k[name], i32))
-        for name in ("fn", "fc", "cofg", "vol", "bfn", "bfc", "patch_pout"):
+        for name in ("fn", "fc", "cofg", "vol", "bfn", "bfc", "patch_pout", "patch_vals"):
             setattr(r, name, _p(k[name], f64))
         if num.order == 2:
             r.c2cs_ptr = _p(k["c2cs_ptr"], i32)

@@ -16,6 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#define BC_INFLOW_VELTEMP 11
 #define BC_OUTFLOW 20
 #define BC_SLIPWALL 60
 #define BCOPT_SUPERSONIC 0
@@ -243,6 +244,27 @@ static void bcond_outflow_perf(int bcOpt, double pout, double sxn, double syn, d
   }
 }
 
+/* Inflow boundary state with prescribed velocity and temperature (BC_INFLOW_VELTEMP,
+ * rocflu/RFLU_CentralFirstPatch.F90:375-414, RFLU_CentralSecondPatch.F90:473-495):
+ * subsonic takes the interior density, otherwise the prescribed pressure fixes it
+ * (MixtPerf_D_PRT, MixtPerf_P_DRT, MixtPerf_Eo_DGPUVW). */
+static void inflow_veltemp(int bcOpt, const double* vals, double rl, double rgas, double g,
+                           double* rr, double* rur, double* rvr, double* rwr, double* rer,
+                           double* pr) {
+  double ur = vals[0], vr = vals[1], wr = vals[2], tr = vals[3];
+  if (bcOpt != BCOPT_SUBSONIC) {
+    *pr = vals[4];
+    *rr = *pr / (rgas * tr);
+  } else {
+    *rr = rl;
+    *pr = *rr * rgas * tr;
+  }
+  *rer = *rr * (*pr / (*rr * (g - 1.0)) + 0.5 * (ur * ur + vr * vr + wr * wr));
+  *rur = *rr * ur;
+  *rvr = *rr * vr;
+  *rwr = *rr * wr;
+}
+
 /* Central flux of an outflow face from the interior and boundary states,
  * rocflu/RFLU_CentralFirstPatch.F90:676-686 (same form in CentralSecondPatch). */
 static void outflow_flux(double nx, double ny, double nz, double nm, double fs, double rl,
@@ -377,8 +399,16 @@ static void boundary_flux(const orc_region* R, const double* cv, const double* d
       return;
     }
     int opt = R->patch_opt[patch];
-    double pr = opt != BCOPT_SUPERSONIC ? R->patch_pout[patch] : pl;
     double rr, rur, rvr, rwr, rer;
+    if (bc == BC_INFLOW_VELTEMP) {
+      double pr;
+      inflow_veltemp(opt, R->patch_vals + 5 * (size_t)patch, rl, rgas, gam, &rr, &rur, &rvr,
+                     &rwr, &rer, &pr);
+      outflow_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, rr, rur, rvr, rwr, rer, pr, flx);
+      *has_mass = 1;
+      return;
+    }
+    double pr = opt != BCOPT_SUPERSONIC ? R->patch_pout[patch] : pl;
     bcond_outflow_perf(opt, pr, nx, ny, nz, R->cp, mm, rl, rul, rvl, rwl, rel, pl, &rr, &rur,
                        &rvr, &rwr, &rer);
     outflow_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, rr, rur, rvr, rwr, rer, pr, flx);
@@ -408,9 +438,17 @@ static void boundary_flux(const orc_region* R, const double* cv, const double* d
     return;
   }
   int opt = R->patch_opt[patch];
-  double pr = opt != BCOPT_SUPERSONIC ? R->patch_pout[patch] : pl;
   double rel = rl * (pl / (rl * (gam - 1.0)) + 0.5 * (ul * ul + vl * vl + wl * wl));
   double rr, rur, rvr, rwr, rer;
+  if (bc == BC_INFLOW_VELTEMP) {
+    double pr;
+    inflow_veltemp(opt, R->patch_vals + 5 * (size_t)patch, rl, rgas, gam, &rr, &rur, &rvr, &rwr,
+                   &rer, &pr);
+    outflow_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, rr, rur, rvr, rwr, rer, pr, flx);
+    *has_mass = 1;
+    return;
+  }
+  double pr = opt != BCOPT_SUPERSONIC ? R->patch_pout[patch] : pl;
   bcond_outflow_perf(opt, pr, nx, ny, nz, R->cp, mm, rl, rul, rvl, rwl, rel, pl, &rr, &rur, &rvr,
                      &rwr, &rer);
   outflow_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, rr, rur, rvr, rwr, rer, pr, flx);

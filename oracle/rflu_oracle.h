@@ -28,8 +28,8 @@ typedef struct {
   const int32_t* bf2c;       /* (B) */
   const double* bfn;         /* (4,B) */
   const double* bfc;         /* (3,B) */
-  const int32_t* patch_bc;   /* (P) 20 outflow, 60 slip wall */
-  const int32_t* patch_opt;  /* (P) outflow option 0/1/2 */
+  const int32_t* patch_bc;   /* (P) 11 inflow (velocity + temperature), 20 outflow, 60 slip wall */
+  const int32_t* patch_opt;  /* (P) inflow/outflow option 0 supersonic, 1 subsonic, 2 mixed */
   const double* patch_pout;  /* (P) outflow static pressure */
   const int32_t* patch_order;/* (P) 1 or 2 (boundary-face order); NULL = all first order */
   const int32_t* c2cs_ptr;   /* (N+1) stencil CSR, order 2 */
@@ -44,6 +44,7 @@ typedef struct {
   int32_t reconst;           /* 0 none, 2 WENO-XYZ + clip, 11 Venkatakrishnan */
   int32_t rk;                /* 1 RK4, 2 Wray RK3, 3 Euler, 4 SSP-RK3 */
   double cfl, dt_imposed, dt_min_limit, eps_entr, diss_fact;
+  const double* patch_vals;  /* (5,P) inflow u, v, w, T, p (p: supersonic option only) */
 } orc_region;
 
 typedef struct {

@@ -43,6 +43,7 @@ class RegionDesc(ctypes.Structure):
         ("cfl", ctypes.c_double), ("dt_imposed", ctypes.c_double),
         ("dt_min_limit", ctypes.c_double), ("eps_entr", ctypes.c_double),
         ("diss_fact", ctypes.c_double),
+        ("patch_vals", _f64p),
     ]
 
 

@@ -38,7 +38,7 @@ struct Region {
   double cfl = 0.5, dt_imposed = 0, dt_min_limit = 0;
   double end_time = HUGE_VAL;  // global%maxTime of the fused loop
   Gas gas{};                   // case 0 (the only case of a single region)
-  bool has_outflow = false;
+  bool has_flowthrough = false;
   // batched cases (K == 1: a single region; per-case tables still hold its values)
   int K = 1;
   std::vector<int> cell0_h, bface0_h;
@@ -49,7 +49,7 @@ struct Region {
   int *cell0 = nullptr, *bface0 = nullptr;
   unsigned long long* dtmin = nullptr;
   int *pbc = nullptr, *popt = nullptr, *porder2 = nullptr;
-  double* ppout = nullptr;
+  double *ppout = nullptr, *pvals = nullptr;
   hipStream_t stream = nullptr;
   // device buffers
   int2* f2c = nullptr;
@@ -114,7 +114,7 @@ struct Region {
   }
   Patches patches() const {
     Patches p;
-    p.bc = pbc; p.opt = popt; p.order2 = porder2; p.pout = ppout;
+    p.bc = pbc; p.opt = popt; p.order2 = porder2; p.pout = ppout; p.vals = pvals;
     return p;
   }
   // scalars that gate a whole launch: the single case's, none for a batch
@@ -159,7 +159,7 @@ void free_region(Region* R) {
                   R->rhs_sum, R->dv, R->prim, R->grad, R->grad2, R->flux, R->rhs, R->dt_part,
                   R->sc, R->tile_uptr, R->tile_uidx, R->tile_lptr, R->tile_loc, R->tile_vloc,
                   R->vnb, R->vgeo, R->srad, R->gw, R->cid, R->gas_tab, R->par, R->cell0,
-                  R->bface0, R->dtmin, R->pbc, R->popt, R->porder2, R->ppout};
+                  R->bface0, R->dtmin, R->pbc, R->popt, R->porder2, R->ppout, R->pvals};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (R->stream) (void)hipStreamDestroy(R->stream);
@@ -560,7 +560,7 @@ int launch_fluxes_p(Region* R) {
 int launch_fluxes(Region* R, int stage, bool srad = false) {
   int rc = srad ? launch_fluxes_p<true>(R) : launch_fluxes_p<false>(R);
   if (rc) return rc;
-  if (stage == 1 && R->has_outflow) {
+  if (stage == 1 && R->has_flowthrough) {
     Mesh M = R->mesh();
     if (!flux32(R))
       hipLaunchKernelGGL((k_mass_inout<double>), dim3(R->K), dim3(256), 0, R->stream, M, R->cases(),
@@ -697,9 +697,17 @@ int check_desc(const rflu_region_desc* d, const rflu_region_desc* ref) {
   const int B = P > 0 ? d->bpatch_ptr[P] : 0;
   if (B > 0 && (!d->bf2c || !d->bfn || !d->bfc)) return fail(RFLU_ERR_ARG, "missing boundary arrays");
   if (d->bpatch_ptr && d->bpatch_ptr[0] != 0) return fail(RFLU_ERR_ARG, "bpatch_ptr[0] must be 0");
-  for (int p = 0; p < P; ++p)
-    if (d->patch_bc[p] != BC_OUTFLOW && d->patch_bc[p] != BC_SLIPWALL)
-      return fail(RFLU_ERR_ARG, "only slip-wall (60) and outflow (20) patches are supported");
+  for (int p = 0; p < P; ++p) {
+    if (d->patch_bc[p] != BC_OUTFLOW && d->patch_bc[p] != BC_SLIPWALL &&
+        d->patch_bc[p] != BC_INFLOW_VELTEMP)
+      return fail(RFLU_ERR_ARG, "only inflow (11), outflow (20) and slip-wall (60) patches are supported");
+    if (d->patch_bc[p] == BC_INFLOW_VELTEMP) {
+      if (!d->patch_vals) return fail(RFLU_ERR_ARG, "inflow patches need patch_vals");
+      const double* v = d->patch_vals + 5 * (size_t)p;
+      if (!(v[3] > 0.0) || (d->patch_opt && d->patch_opt[p] != 1 && !(v[4] > 0.0)))
+        return fail(RFLU_ERR_ARG, "inflow temperature (and non-subsonic pressure) must be > 0");
+    }
+  }
   if (ref && (d->order != ref->order || d->discr != ref->discr || d->rk != ref->rk ||
               d->precision != ref->precision || (d->order == 2 && d->reconst != ref->reconst)))
     return fail(RFLU_ERR_ARG, "every case of a batch needs the same discr, order, reconst, rk and precision");
@@ -754,7 +762,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
   R->cell0_h.assign(co.begin(), co.end());
   R->bface0_h.assign(bo.begin(), bo.end());
   std::vector<int> hpbc(P), hpopt(P), hporder2(P);
-  std::vector<double> hppout(P);
+  std::vector<double> hppout(P), hpvals((size_t)5 * P, 0.0);
   for (int c = 0; c < K; ++c) {
     const rflu_region_desc* d = descs + c;
     hgas[c] = gas_of(d);
@@ -767,7 +775,9 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       hpopt[q] = d->patch_opt ? d->patch_opt[p] : 0;
       hporder2[q] = d->patch_order ? (d->patch_order[p] == 2) : 0;
       hppout[q] = d->patch_pout ? d->patch_pout[p] : 0.0;
-      if (d->patch_bc[p] == BC_OUTFLOW) R->has_outflow = true;
+      if (d->patch_vals)
+        for (int k = 0; k < 5; ++k) hpvals[5 * q + k] = d->patch_vals[5 * (size_t)p + k];
+      if (d->patch_bc[p] != BC_SLIPWALL) R->has_flowthrough = true;   // mass crosses the patch
     }
   }
 
@@ -890,6 +900,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       (rc = dalloc(&R->cell0, K + 1)) || (rc = dalloc(&R->bface0, K + 1)) ||
       (rc = dalloc(&R->dtmin, (size_t)K * DTMIN_STRIDE)) || (rc = dalloc(&R->pbc, P)) || (rc = dalloc(&R->popt, P)) ||
       (rc = dalloc(&R->porder2, P)) || (rc = dalloc(&R->ppout, P)) ||
+      (rc = dalloc(&R->pvals, (size_t)5 * P)) ||
       (K > 1 && (rc = dalloc(&R->cid, N)))) {
     free_region(R);
     return rc;
@@ -926,6 +937,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       (rc = dput(R->cell0, hcell0, R->stream)) || (rc = dput(R->bface0, hbface0, R->stream)) ||
       (rc = dput(R->pbc, hpbc, R->stream)) || (rc = dput(R->popt, hpopt, R->stream)) ||
       (rc = dput(R->porder2, hporder2, R->stream)) || (rc = dput(R->ppout, hppout, R->stream)) ||
+      (rc = dput(R->pvals, hpvals, R->stream)) ||
       (K > 1 && (rc = dput(R->cid, hcid, R->stream)))) {
     free_region(R);
     return rc;

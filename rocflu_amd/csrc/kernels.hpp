@@ -19,6 +19,7 @@
 
 namespace rflu {
 
+constexpr int BC_INFLOW_VELTEMP = 11;
 constexpr int BC_OUTFLOW = 20;
 constexpr int BC_SLIPWALL = 60;
 constexpr int BCOPT_SUPERSONIC = 0;
@@ -31,7 +32,33 @@ struct Patches {
   const int* opt;
   const int* order2;
   const double* pout;
+  const double* vals;   // (5, P) inflow u, v, w, T, p
 };
+
+// Inflow boundary state with prescribed velocity and temperature (BC_INFLOW_VELTEMP,
+// rocflu/RFLU_CentralFirstPatch.F90:375-414, RFLU_CentralSecondPatch.F90:473-495):
+// subsonic takes the interior density, otherwise the prescribed pressure fixes it
+// (MixtPerf_D_PRT, MixtPerf_P_DRT, MixtPerf_Eo_DGPUVW).  b = (rho, rho u, rho v,
+// rho w, rho E) of the boundary state; returns its pressure.
+template <typename T, typename V>
+__device__ __forceinline__ T inflow_veltemp(int bcOpt, const V* vals, T rl, double rgas, double g,
+                                            T* b) {
+  const T ur = T(vals[0]), vr = T(vals[1]), wr = T(vals[2]), tr = T(vals[3]);
+  T pr, rr;
+  if (bcOpt != 1) {
+    pr = T(vals[4]);
+    rr = pr / (T(rgas) * tr);
+  } else {
+    rr = rl;
+    pr = rr * T(rgas) * tr;
+  }
+  b[0] = rr;
+  b[1] = rr * ur;
+  b[2] = rr * vr;
+  b[3] = rr * wr;
+  b[4] = rr * (pr / (rr * (T(g) - T(1.0))) + T(0.5) * (ur * ur + vr * vr + wr * wr));
+  return pr;
+}
 
 struct Gas {
   double g, cp, gc;            // refGamma, refCp, R = cp - cp/g (flux functions)
@@ -536,9 +563,13 @@ __global__ void k_bface_flux(Mesh M, State S, Gas G, Cases C, Patches P,
     flx[4] = 0.0;
   } else {
     int opt = P.opt[p];
-    double pr = opt != BCOPT_SUPERSONIC ? P.pout[p] : pl;
-    double bs[5];
-    bcond_outflow(opt, pr, nx, ny, nz, G.gp, rl, rul, rvl, rwl, rel, pl, bs);
+    double pr, bs[5];
+    if (bc == BC_INFLOW_VELTEMP) {
+      pr = inflow_veltemp<double>(opt, P.vals + 5 * (size_t)p, rl, G.rgasp, G.gp, bs);
+    } else {
+      pr = opt != BCOPT_SUPERSONIC ? P.pout[p] : pl;
+      bcond_outflow(opt, pr, nx, ny, nz, G.gp, rl, rul, rvl, rwl, rel, pl, bs);
+    }
     double ql = (rul * nx + rvl * ny + rwl * nz) / rl;
     double qr = (bs[1] * nx + bs[2] * ny + bs[3] * nz) / bs[0];
     flx[0] = 0.5 * (ql * rl + qr * bs[0]) * nm;
@@ -1434,7 +1465,7 @@ __global__ void k_mass_inout(Mesh M, Cases C, Patches P, const FT* __restrict__ 
   if (!sc->abort) {
     for (int k = C.bface0[c] + threadIdx.x; k < C.bface0[c + 1]; k += blockDim.x) {
       int p = M.bpatch[k];
-      if (P.bc[p] != BC_OUTFLOW) continue;
+      if (P.bc[p] == BC_SLIPWALL) continue;   // inflow and outflow faces carry mass
       double m = double(flux[M.F + k]);
       a = a - fmin(m, 0.0);
       b = b + fmax(m, 0.0);

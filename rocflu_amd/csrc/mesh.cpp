@@ -526,8 +526,9 @@ int build_box(rflu_mesh& m, int kind, int nx, int ny, int nz, double lx, double 
   m.patch_opt.assign(bc_opt, bc_opt + 6);
   m.patch_pout.assign(bc_pout, bc_pout + 6);
   for (int s = 0; s < 6; ++s)
-    if (bc_type[s] != RFLU_BC_OUTFLOW && bc_type[s] != RFLU_BC_SLIPWALL) {
-      g_err = "only slip-wall (60) and outflow (20) patches are supported";
+    if (bc_type[s] != RFLU_BC_OUTFLOW && bc_type[s] != RFLU_BC_SLIPWALL &&
+        bc_type[s] != RFLU_BC_INFLOW_VELTEMP) {
+      g_err = "only inflow (11), outflow (20) and slip-wall (60) patches are supported";
       return 4;
     }
 

@@ -105,6 +105,7 @@ PROGRAM rflu_driver
   desc%rk = 2; desc%precision = 0
   desc%cfl = 0.5_C_DOUBLE; desc%dt_imposed = 1.0E-3_C_DOUBLE; desc%dt_min_limit = 0.0_C_DOUBLE
   desc%eps_entr = 0.05_C_DOUBLE; desc%diss_fact = 1.0_C_DOUBLE
+  desc%patch_vals = C_NULL_PTR
   CALL rflu_gpu_check(rflu_gpu_create_region(desc, handle), 'rflu_gpu_create_region')
 
 ! ------------------------------------------------------------------------------

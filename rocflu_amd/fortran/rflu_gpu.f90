@@ -35,6 +35,7 @@ MODULE rflu_gpu
     REAL(C_DOUBLE) :: gamma, cp
     INTEGER(C_INT32_T) :: discr, order, reconst, rk, precision
     REAL(C_DOUBLE) :: cfl, dt_imposed, dt_min_limit, eps_entr, diss_fact
+    TYPE(C_PTR) :: patch_vals
   END TYPE rflu_region_desc
 
   TYPE, BIND(C) :: rflu_gpu_stats

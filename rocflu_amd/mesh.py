@@ -19,6 +19,7 @@ _LIB = None
 
 HEX = 0
 TET6 = 1
+BC_INFLOW_VELTEMP = 11
 BC_OUTFLOW = 20
 BC_SLIPWALL = 60
 BCOPT_SUPERSONIC = 0

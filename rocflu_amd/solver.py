@@ -46,6 +46,8 @@ class Numerics:
     gamma: float = 1.4
     cp: float = 1004.5
     patch_order: Optional[Sequence[int]] = None
+    # per patch (u, v, w, T, p) of inflow patches (BCDAT_INFLOW_U.._P); zeros elsewhere
+    patch_vals: Optional[Sequence[Sequence[float]]] = None
 
     def gas_prime(self):
         """(M, R', gamma') of the dependent-variable path (libflu/RFLU_SetGasVars.F90:172-178)."""
@@ -76,6 +78,10 @@ def region_arrays(mesh: Mesh, num: Numerics) -> dict:
     }
     po = num.patch_order if num.patch_order is not None else [1] * int(mesh.n_patches)
     keep["patch_order"] = np.ascontiguousarray(po, dtype=np.int32)
+    pv = np.zeros((int(mesh.n_patches), 5))
+    if num.patch_vals is not None:
+        pv[:] = np.asarray(num.patch_vals, dtype=np.float64).reshape(-1, 5)
+    keep["patch_vals"] = np.ascontiguousarray(pv.reshape(-1))
     if num.order == 2:
         if not mesh.has_stencils:
             raise ValueError("second-order runs need a mesh built with stencils=True")
@@ -116,6 +122,7 @@ def region_desc(mesh: Mesh, num: Numerics, d: "nat.RegionDesc"):
         num.discr, num.order, num.reconst, num.rk, num.precision)
     d.cfl, d.dt_imposed, d.dt_min_limit = num.cfl, num.dt_imposed, num.dt_min_limit
     d.eps_entr, d.diss_fact = num.eps_entr, num.diss_fact
+    d.patch_vals = nat.ptr(k["patch_vals"], f64)
     return k
 
 

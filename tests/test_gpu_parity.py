@@ -254,3 +254,26 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
             assert np.all(err <= TOL_FP64), err
         else:
             assert np.array_equal(cvg, ref), v
+
+
+@pytest.mark.parametrize("order,opt,discr", [(1, 1, 3), (1, 0, 1), (2, 1, 3), (2, 0, 4)])
+def test_inflow_patch(gpu, order, opt, discr):
+    """Velocity + temperature inflow (BC_INFLOW_VELTEMP) at x-, subsonic outflow at x+,
+    slip walls elsewhere; a blast inside the box."""
+    m = rmesh.box(rmesh.TET6, 6, 5, 5, lengths=(6.0, 5.0, 5.0), jitter=0.08, seed=4,
+                  bc_type=(rmesh.BC_INFLOW_VELTEMP, rmesh.BC_OUTFLOW) + (rmesh.BC_SLIPWALL,) * 4,
+                  bc_opt=(opt, 1, 0, 0, 0, 0), bc_pout=(0.0, 1.0, 0, 0, 0, 0),
+                  stencils=order == 2)
+    base = Numerics()
+    rg = base.gas_prime()[1]
+    vals = [[0.4, 0.05, 0.0, 1.1 / rg, 1.2]] + [[0.0] * 5] * 5
+    num = Numerics(discr=discr, order=order, reconst=11 if order == 2 else 0, rk=4, cfl=0.5,
+                   dt_imposed=1e-3, patch_vals=vals,
+                   patch_order=[2, 2, 1, 1, 1, 1] if order == 2 else None)
+    cv0 = cases.blast(m, num, (3.0, 2.5, 2.5), 1.0, p_in=5.0)
+    cvo, _, st, cvg, _, gst = run_both(m, num, cv0, 30)
+    err = rel_l2(cvg, cvo)
+    print("rel L2 per variable:", err, "mass in/out", gst["mass_in"], gst["mass_out"], flush=True)
+    assert np.all(err <= TOL_FP64), err
+    assert abs(gst["mass_in"] - st["mass_in"]) <= 1e-12 * max(st["mass_in"], 1e-30)
+    assert st["mass_in"] > 0.0

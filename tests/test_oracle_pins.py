@@ -265,3 +265,69 @@ def test_closed_box_conserves_mass_and_energy():
         assert st["rc"] == 0
         for k in (0, 4):
             assert abs((cv[:, k] * m.vol).sum() / (cv0[:, k] * m.vol).sum() - 1) < 1e-13
+
+
+def inflow_box(order=1, stencils=False):
+    """4x3x3 jittered hexes: inflow (velocity + temperature) at x-, supersonic outflow
+    at x+, slip walls elsewhere.  Boundary nodes stay on the box planes."""
+    return rm.box(rm.HEX, 4, 3, 3, lengths=(4.0, 3.0, 3.0), jitter=0.1, seed=3,
+                  bc_type=(rm.BC_INFLOW_VELTEMP, rm.BC_OUTFLOW) + (rm.BC_SLIPWALL,) * 4,
+                  bc_opt=(0,) * 6, stencils=stencils)
+
+
+def normal_flux(rho, u, p, gp, n):
+    """Physical Euler flux F(U).n per unit area for velocity vector u."""
+    q = float(np.dot(u, n))
+    E = p / (gp - 1.0) + 0.5 * rho * float(np.dot(u, u))
+    return np.array([rho * q, rho * u[0] * q + p * n[0], rho * u[1] * q + p * n[1],
+                     rho * u[2] * q + p * n[2], (E + p) * q])
+
+
+@pytest.mark.parametrize("order,opt", [(1, 1), (1, 0), (2, 1), (2, 0)])
+def test_inflow_preserves_free_stream(order, opt):
+    """A uniform stream whose temperature, velocity (and pressure) the inflow patch
+    prescribes has zero residual: the boundary state equals the interior state.  (At
+    order 2 every patch runs at order 2: a first-order outflow patch in a second-order
+    run reads p in the energy slot, as the reference does, which no uniform stream
+    survives.)"""
+    m = inflow_box(order, stencils=order == 2)
+    rho, u, p = 1.2, 0.3, 1.0
+    base = Numerics(discr=3, order=order, reconst=0)
+    mm, rg, gp = base.gas_prime()
+    T = p / (rho * rg)
+    po = [2] * 6 if order == 2 else None
+    num = Numerics(discr=3, order=order, reconst=0, patch_order=po,
+                   patch_vals=[[u, 0.0, 0.0, T, p]] + [[0.0] * 5] * 5)
+    opts = np.array(m.a["patch_opt"]); opts[0] = opt; m.a["patch_opt"] = opts
+    cv = np.tile(cases.prim_to_cons(rho, u, 0.0, 0.0, p, gp), (m.n_cells, 1))
+    r = Oracle(m, num).residual(cv)
+    assert np.abs(r).max() <= 1e-13 * p, np.abs(r).max()
+
+
+@pytest.mark.parametrize("opt", [1, 0])
+def test_inflow_boundary_state_closed_form(opt):
+    """Uniform interior state, different inflow values: the residual of each cell on
+    the inflow patch is half the jump of the physical flux between the boundary state
+    (subsonic: interior density, p = rho R T_in; otherwise p_in and rho = p_in/(R
+    T_in)) and the interior state, times the face area; elsewhere it vanishes."""
+    m = inflow_box()
+    rho, u0, p = 1.2, 0.3, 1.0
+    num0 = Numerics(discr=3)
+    mm, rg, gp = num0.gas_prime()
+    T_in, u_in, p_in = 1.3 * p / (rho * rg), np.array([0.5, 0.1, -0.05]), 2.0
+    num = Numerics(discr=3, patch_vals=[[*u_in, T_in, p_in]] + [[0.0] * 5] * 5)
+    opts = np.array(m.a["patch_opt"]); opts[0] = opt; m.a["patch_opt"] = opts
+    cv = np.tile(cases.prim_to_cons(rho, u0, 0.0, 0.0, p, gp), (m.n_cells, 1))
+    r = Oracle(m, num).residual(cv)
+    if opt == 1:
+        rb, pb = rho, rho * rg * T_in
+    else:
+        rb, pb = p_in / (rg * T_in), p_in
+    expect = np.zeros_like(r)
+    bp = m.a["bpatch_ptr"]
+    for b in range(bp[0], bp[1]):
+        c = m.a["bf2c"][b]
+        n, area = m.a["bfn"][b, :3], m.a["bfn"][b, 3]
+        expect[c] += 0.5 * (normal_flux(rb, u_in, pb, gp, n) -
+                            normal_flux(rho, np.array([u0, 0.0, 0.0]), p, gp, n)) * area
+    assert np.abs(r - expect).max() <= 1e-13 * np.abs(expect).max(), np.abs(r - expect).max()
