@@ -184,6 +184,34 @@ int rflu_gpu_reset_stats(int32_t handle);
 /* Waits for the region's stream. */
 int rflu_gpu_synchronize(int32_t handle);
 
+/*
+ * Differentiation of the residual (verification of the discrete adjoint; no
+ * counterpart in the reference).  States live on the device as [5][n_cells] in the
+ * region's internal cell order, which rflu_gpu_cell_order maps to the caller's cells
+ * (new2old[i] = caller cell of internal cell i).  Device pointers are plain device
+ * addresses on the region's GPU; every call waits for its work.  Tangent and adjoint
+ * are of the first-order FP64 residual R(cv) of a single region (rflu_gpu_residual's
+ * R), with dv = dv(cv) and, for inflow patches, the patch values (u, v, w, T, p)
+ * as inputs.
+ */
+int rflu_gpu_cell_order(int32_t handle, int32_t* new2old);
+/* vol in the internal cell order (host array of n_cells). */
+int rflu_gpu_cell_volumes(int32_t handle, double* vol);
+/* Internal cell and |A| of each boundary face of one patch, in patch face order. */
+int rflu_gpu_patch_faces(int32_t handle, int32_t patch, int32_t* cells, double* areas);
+/* Replaces the 5 values (u, v, w, T, p) of one patch (inflow patches read them). */
+int rflu_gpu_set_patch_values(int32_t handle, int32_t patch, const double* vals);
+/* cv <- cv_dev, then dv (and the primitive copy) from it. */
+int rflu_gpu_set_state_dev(int32_t handle, const double* cv_dev);
+int rflu_gpu_get_state_dev(int32_t handle, double* cv_dev);
+/* R of the current state into rhs_dev. */
+int rflu_gpu_residual_dev(int32_t handle, double* rhs_dev);
+/* drhs_dev = dR/dcv . dcv_dev + dR/dvals . dvals (dvals host (5, n_patches) or NULL). */
+int rflu_gpu_residual_jvp(int32_t handle, const double* dcv_dev, const double* dvals,
+                          double* drhs_dev);
+/* lam_dev = (dR/dcv)^T mu_dev; dvals (host (5, n_patches), or NULL) = (dR/dvals)^T mu. */
+int rflu_gpu_residual_vjp(int32_t handle, const double* mu_dev, double* lam_dev, double* dvals);
+
 /* Launch tuning: threads per block for the face, cell and gradient kernels (64..1024,
  * multiple of 64; 0 keeps the current value). */
 int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,

@@ -78,6 +78,15 @@ SIGNATURES = {
     "rflu_gpu_time_steps": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32]),
     "rflu_gpu_get_stats": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(GpuStats)]),
     "rflu_gpu_get_case_stats": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(GpuStats)]),
+    "rflu_gpu_cell_order": (ctypes.c_int, [ctypes.c_int32, _i32p]),
+    "rflu_gpu_cell_volumes": (ctypes.c_int, [ctypes.c_int32, _f64p]),
+    "rflu_gpu_patch_faces": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _i32p, _f64p]),
+    "rflu_gpu_set_patch_values": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _f64p]),
+    "rflu_gpu_set_state_dev": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p]),
+    "rflu_gpu_get_state_dev": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p]),
+    "rflu_gpu_residual_dev": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p]),
+    "rflu_gpu_residual_jvp": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, _f64p, ctypes.c_void_p]),
+    "rflu_gpu_residual_vjp": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, _f64p]),
     "rflu_gpu_reset_stats": (ctypes.c_int, [ctypes.c_int32]),
     "rflu_gpu_synchronize": (ctypes.c_int, [ctypes.c_int32]),
     "rflu_gpu_set_launch": (ctypes.c_int, [ctypes.c_int32] * 4),

@@ -3,6 +3,7 @@
 // sequences of RungeKuttaMP's callees.
 #include "rocflu_amd.h"
 #include "kernels.hpp"
+#include "adjoint.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -50,6 +51,12 @@ struct Region {
   unsigned long long* dtmin = nullptr;
   int *pbc = nullptr, *popt = nullptr, *porder2 = nullptr;
   double *ppout = nullptr, *pvals = nullptr;
+  std::vector<double> pvals_h;
+  std::vector<int> bptr_h;     // first boundary face of every patch (all cases), P+1
+  int* bptr = nullptr;
+  // derivative work buffers (allocated by the first tangent/adjoint call)
+  double *adj_g = nullptr, *adj_gb = nullptr, *adj_gv = nullptr, *adj_pv = nullptr;
+  double *adj_dflux = nullptr, *adj_dvals = nullptr;
   hipStream_t stream = nullptr;
   // device buffers
   int2* f2c = nullptr;
@@ -159,7 +166,9 @@ void free_region(Region* R) {
                   R->rhs_sum, R->dv, R->prim, R->grad, R->grad2, R->flux, R->rhs, R->dt_part,
                   R->sc, R->tile_uptr, R->tile_uidx, R->tile_lptr, R->tile_loc, R->tile_vloc,
                   R->vnb, R->vgeo, R->srad, R->gw, R->cid, R->gas_tab, R->par, R->cell0,
-                  R->bface0, R->dtmin, R->pbc, R->popt, R->porder2, R->ppout, R->pvals};
+                  R->bface0, R->dtmin, R->pbc, R->popt, R->porder2, R->ppout, R->pvals,
+                  R->bptr, R->adj_g, R->adj_gb, R->adj_gv, R->adj_pv, R->adj_dflux,
+                  R->adj_dvals};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (R->stream) (void)hipStreamDestroy(R->stream);
@@ -763,6 +772,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
   R->bface0_h.assign(bo.begin(), bo.end());
   std::vector<int> hpbc(P), hpopt(P), hporder2(P);
   std::vector<double> hppout(P), hpvals((size_t)5 * P, 0.0);
+  R->bptr_h.assign(P + 1, 0);
   for (int c = 0; c < K; ++c) {
     const rflu_region_desc* d = descs + c;
     hgas[c] = gas_of(d);
@@ -778,6 +788,8 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       if (d->patch_vals)
         for (int k = 0; k < 5; ++k) hpvals[5 * q + k] = d->patch_vals[5 * (size_t)p + k];
       if (d->patch_bc[p] != BC_SLIPWALL) R->has_flowthrough = true;   // mass crosses the patch
+      R->bptr_h[q] = (int)bo[c] + d->bpatch_ptr[p];
+      R->bptr_h[q + 1] = (int)bo[c] + d->bpatch_ptr[p + 1];
     }
   }
 
@@ -889,6 +901,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
   const size_t gsz = grad32(R) ? sizeof(float) : sizeof(double);
   R->n_dt_parts = (int)nblk(N, 256);
   std::vector<int> hcell0(co.begin(), co.end()), hbface0(bo.begin(), bo.end());
+  R->pvals_h = hpvals;
   if ((rc = dalloc(&R->f2c, F)) || (rc = dalloc(&R->fn, F)) || (rc = dalloc(&R->fc, F)) ||
       (rc = dalloc(&R->cell, N)) || (rc = dalloc(&R->bfn, B)) || (rc = dalloc(&R->bfc, B)) ||
       (rc = dalloc(&R->bf2c, B)) || (rc = dalloc(&R->bpatch, B)) ||
@@ -900,7 +913,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       (rc = dalloc(&R->cell0, K + 1)) || (rc = dalloc(&R->bface0, K + 1)) ||
       (rc = dalloc(&R->dtmin, (size_t)K * DTMIN_STRIDE)) || (rc = dalloc(&R->pbc, P)) || (rc = dalloc(&R->popt, P)) ||
       (rc = dalloc(&R->porder2, P)) || (rc = dalloc(&R->ppout, P)) ||
-      (rc = dalloc(&R->pvals, (size_t)5 * P)) ||
+      (rc = dalloc(&R->pvals, (size_t)5 * P)) || (rc = dalloc(&R->bptr, P + 1)) ||
       (K > 1 && (rc = dalloc(&R->cid, N)))) {
     free_region(R);
     return rc;
@@ -937,7 +950,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       (rc = dput(R->cell0, hcell0, R->stream)) || (rc = dput(R->bface0, hbface0, R->stream)) ||
       (rc = dput(R->pbc, hpbc, R->stream)) || (rc = dput(R->popt, hpopt, R->stream)) ||
       (rc = dput(R->porder2, hporder2, R->stream)) || (rc = dput(R->ppout, hppout, R->stream)) ||
-      (rc = dput(R->pvals, hpvals, R->stream)) ||
+      (rc = dput(R->pvals, hpvals, R->stream)) || (rc = dput(R->bptr, R->bptr_h, R->stream)) ||
       (K > 1 && (rc = dput(R->cid, hcid, R->stream)))) {
     free_region(R);
     return rc;
@@ -1385,6 +1398,162 @@ int rflu_gpu_get_option(int32_t h, int32_t option, int32_t* value) {
 
 // Per-kernel timing of the launches made by rflu_gpu_time_steps (HIP events on the
 // region's stream around each launch; graphs are not used while on).
+// ---- differentiation path: device-resident states in the internal cell order ------
+
+int rflu_gpu_cell_order(int32_t h, int32_t* new2old) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!new2old) return fail(RFLU_ERR_ARG, "null array");
+  for (int i = 0; i < R->N; ++i) new2old[i] = R->new2old[i];
+  return RFLU_OK;
+}
+
+int rflu_gpu_cell_volumes(int32_t h, double* vol) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!vol) return fail(RFLU_ERR_ARG, "null array");
+  std::vector<double4> c(R->N);
+  HIP_TRY(hipMemcpyAsync(c.data(), R->cell, c.size() * sizeof(double4), hipMemcpyDeviceToHost, R->stream));
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  for (int i = 0; i < R->N; ++i) vol[i] = c[i].w;
+  return RFLU_OK;
+}
+
+int rflu_gpu_patch_faces(int32_t h, int32_t patch, int32_t* cells, double* areas) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (patch < 0 || patch >= R->P || !cells || !areas) return fail(RFLU_ERR_ARG, "bad arguments");
+  const int b0 = R->bptr_h[patch], nb = R->bptr_h[patch + 1] - b0;
+  if (nb == 0) return RFLU_OK;
+  std::vector<double4> n(nb);
+  HIP_TRY(hipMemcpyAsync(cells, R->bf2c + b0, nb * sizeof(int32_t), hipMemcpyDeviceToHost, R->stream));
+  HIP_TRY(hipMemcpyAsync(n.data(), R->bfn + b0, nb * sizeof(double4), hipMemcpyDeviceToHost, R->stream));
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  for (int k = 0; k < nb; ++k) areas[k] = n[k].w;
+  return RFLU_OK;
+}
+
+int rflu_gpu_set_patch_values(int32_t h, int32_t patch, const double* vals) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (patch < 0 || patch >= R->P || !vals) return fail(RFLU_ERR_ARG, "bad arguments");
+  for (int k = 0; k < 5; ++k) R->pvals_h[5 * (size_t)patch + k] = vals[k];
+  HIP_TRY(hipMemcpyAsync(R->pvals + 5 * (size_t)patch, R->pvals_h.data() + 5 * (size_t)patch,
+                         5 * sizeof(double), hipMemcpyHostToDevice, R->stream));
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  return RFLU_OK;
+}
+
+int rflu_gpu_set_state_dev(int32_t h, const double* cv) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!cv) return fail(RFLU_ERR_ARG, "null state");
+  HIP_TRY(hipMemcpyAsync(R->cv, cv, (size_t)5 * R->N * sizeof(double), hipMemcpyDeviceToDevice, R->stream));
+  Mesh M = R->mesh();
+  State S = R->state();
+  hipLaunchKernelGGL(k_set_dv, dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S, R->gas, R->cases());
+  if (R->order == 2)
+    hipLaunchKernelGGL(k_set_prim, dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  return RFLU_OK;
+}
+
+int rflu_gpu_get_state_dev(int32_t h, double* cv) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!cv) return fail(RFLU_ERR_ARG, "null state");
+  HIP_TRY(hipMemcpyAsync(cv, R->cv, (size_t)5 * R->N * sizeof(double), hipMemcpyDeviceToDevice, R->stream));
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  return RFLU_OK;
+}
+
+int rflu_gpu_residual_dev(int32_t h, double* rhs) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!rhs) return fail(RFLU_ERR_ARG, "null rhs");
+  int rc;
+  if ((rc = launch_gradients(R)) || (rc = launch_fluxes(R, 0))) return rc;
+  Mesh M = R->mesh();
+  if (!flux32(R))
+    hipLaunchKernelGGL((k_gather_rhs<double>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, (const double*)R->flux, rhs);
+  else
+    hipLaunchKernelGGL((k_gather_rhs<float>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, (const float*)R->flux, rhs);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  return RFLU_OK;
+}
+
+namespace {
+int derivative_ready(Region* R) {
+  if (R->K != 1) return fail(RFLU_ERR_ARG, "derivatives are for a single region");
+  if (R->order != 1) return fail(RFLU_ERR_ARG, "derivatives of the residual are first order");
+  if (R->prec != RFLU_PREC_FP64) return fail(RFLU_ERR_ARG, "derivatives are FP64");
+  int rc;
+  const size_t F = R->F, B = R->B, P = R->P;
+  if (!R->adj_g && ((rc = dalloc(&R->adj_g, 10 * F)) || (rc = dalloc(&R->adj_gb, 5 * B)) ||
+                    (rc = dalloc(&R->adj_gv, 5 * B)) || (rc = dalloc(&R->adj_pv, 5 * P)) ||
+                    (rc = dalloc(&R->adj_dflux, 5 * (F + B))) || (rc = dalloc(&R->adj_dvals, 5 * P))))
+    return rc;
+  return 0;
+}
+}  // namespace
+
+int rflu_gpu_residual_jvp(int32_t h, const double* dcv, const double* dvals, double* drhs) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!dcv || !drhs) return fail(RFLU_ERR_ARG, "null array");
+  int rc = derivative_ready(R);
+  if (rc) return rc;
+  if (dvals && R->P > 0)
+    HIP_TRY(hipMemcpyAsync(R->adj_dvals, dvals, 5 * (size_t)R->P * sizeof(double), hipMemcpyHostToDevice, R->stream));
+  Mesh M = R->mesh();
+  State S = R->state();
+  if (R->F > 0) {
+    dim3 g(nblk(R->F, 256)), b(256);
+    if (R->discr == 1) hipLaunchKernelGGL(k_face_jvp<1>, g, b, 0, R->stream, M, S, R->gas, dcv, R->adj_dflux);
+    else if (R->discr == 3) hipLaunchKernelGGL(k_face_jvp<3>, g, b, 0, R->stream, M, S, R->gas, dcv, R->adj_dflux);
+    else hipLaunchKernelGGL(k_face_jvp<4>, g, b, 0, R->stream, M, S, R->gas, dcv, R->adj_dflux);
+  }
+  if (R->B > 0)
+    hipLaunchKernelGGL(k_bface_jvp, dim3(nblk(R->B, 256)), dim3(256), 0, R->stream, M, S, R->gas,
+                       R->patches(), dcv, (const double*)(dvals ? R->adj_dvals : nullptr), R->adj_dflux);
+  hipLaunchKernelGGL((k_gather_rhs<double>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M,
+                     (const double*)R->adj_dflux, drhs);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  return RFLU_OK;
+}
+
+int rflu_gpu_residual_vjp(int32_t h, const double* mu, double* lam, double* dvals) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!mu || !lam) return fail(RFLU_ERR_ARG, "null array");
+  int rc = derivative_ready(R);
+  if (rc) return rc;
+  Mesh M = R->mesh();
+  State S = R->state();
+  if (R->F > 0) {
+    dim3 g(nblk(R->F, 256)), b(256);
+    if (R->discr == 1) hipLaunchKernelGGL(k_face_vjp<1>, g, b, 0, R->stream, M, S, R->gas, mu, R->adj_g);
+    else if (R->discr == 3) hipLaunchKernelGGL(k_face_vjp<3>, g, b, 0, R->stream, M, S, R->gas, mu, R->adj_g);
+    else hipLaunchKernelGGL(k_face_vjp<4>, g, b, 0, R->stream, M, S, R->gas, mu, R->adj_g);
+  }
+  if (R->B > 0)
+    hipLaunchKernelGGL(k_bface_vjp, dim3(nblk(R->B, 256)), dim3(256), 0, R->stream, M, S, R->gas,
+                       R->patches(), mu, R->adj_gb, R->adj_gv);
+  hipLaunchKernelGGL(k_cell_vjp, dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M,
+                     (const double*)R->adj_g, (const double*)R->adj_gb, lam);
+  if (dvals && R->P > 0) {
+    hipLaunchKernelGGL(k_patch_vals_sum, dim3(R->P), dim3(256), 0, R->stream, M, (const int*)R->bptr,
+                       (const double*)R->adj_gv, R->adj_pv);
+    HIP_TRY(hipMemcpyAsync(dvals, R->adj_pv, 5 * (size_t)R->P * sizeof(double), hipMemcpyDeviceToHost, R->stream));
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  return RFLU_OK;
+}
+
 int rflu_gpu_set_profiling(int32_t h, int32_t on) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");

@@ -1,0 +1,143 @@
+"""Tangent and adjoint of the residual and of multi-step SSP-RK3 runs.
+
+Checks: the adjoint kernel is the transpose of the tangent kernel (dot-product test to
+round-off); the tangent kernel matches central finite differences of the residual;
+the torch-composed forward reproduces the solver's own steps; the gradient of a
+time-integrated wall pressure by reverse mode (adjoint kernels, checkpointed) equals
+the forward-mode tangent along seeded directions and central finite differences."""
+import numpy as np
+import pytest
+import torch
+
+from rocflu_amd import cases, mesh as rmesh
+from rocflu_amd.solver import GpuRegion, Numerics
+from rocflu_amd import adjoint as adj
+
+pytestmark = pytest.mark.gpu
+
+
+def blast_box(discr=3, n=(6, 5, 5), subsonic_in=True):
+    m = rmesh.box(rmesh.TET6, *n, lengths=tuple(float(x) for x in n), jitter=0.08, seed=4,
+                  bc_type=(rmesh.BC_INFLOW_VELTEMP, rmesh.BC_OUTFLOW) + (rmesh.BC_SLIPWALL,) * 4,
+                  bc_opt=(1 if subsonic_in else 0, 1, 0, 0, 0, 0), bc_pout=(0.0, 1.0, 0, 0, 0, 0),
+                  stencils=False)
+    rg = Numerics().gas_prime()[1]
+    vals = [[0.4, 0.05, -0.02, 1.1 / rg, 1.2]] + [[0.0] * 5] * 5
+    num = Numerics(discr=discr, order=1, rk=4, cfl=0.5, dt_imposed=1e-3, patch_vals=vals)
+    centre = tuple(0.5 * x for x in n)
+    cv0 = cases.blast(m, num, centre, 0.2 * n[0], p_in=10.0)
+    # a slow stream with seeded noise on every variable: exact ties and zero speeds
+    # (the kinks of min/max/|.| in the flux functions, where a derivative has no
+    # value and a central difference averages the one-sided slopes) are avoided
+    rng = np.random.default_rng(9)
+    gp = num.gas_prime()[2]
+    rho = cv0[:, 0] * (1.0 + 0.01 * rng.standard_normal(m.n_cells))
+    p = (gp - 1.0) * cv0[:, 4] * (1.0 + 0.01 * rng.standard_normal(m.n_cells))
+    vel = np.array([0.3, 0.05, -0.02]) + 0.02 * rng.standard_normal((m.n_cells, 3))
+    cv0 = cases.prim_to_cons(rho, vel[:, 0], vel[:, 1], vel[:, 2], p, gp)
+    return m, num, cv0
+
+
+def rand_like(t, seed):
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    return torch.randn(t.shape, generator=g, dtype=torch.float64).to(t.device)
+
+
+@pytest.mark.parametrize("discr", [1, 3, 4])
+def test_residual_tangent_and_adjoint(gpu, discr):
+    m, num, cv0 = blast_box(discr)
+    g = GpuRegion(m, num)
+    try:
+        dr = adj.DiffRegion(g)
+        u = dr.to_internal(cv0)
+        vals = torch.tensor(num.patch_vals, dtype=torch.float64, device="cuda")
+        du = rand_like(u, 1) * u
+        dvals = rand_like(vals, 2) * vals
+        mu = rand_like(u, 3)
+        jv = dr.jvp_raw(u, vals, du, dvals)
+        lam, lvals = dr.vjp_raw(u, vals, mu)
+        lhs = float((mu * jv).sum())
+        rhs = float((lam * du).sum()) + float((lvals.to(dvals) * dvals).sum())
+        print(f"discr {discr}: <mu, J v> = {lhs:.17g}, <J^T mu, v> = {rhs:.17g}", flush=True)
+        assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+        eps = 1e-6
+        fd = (dr.residual_raw(u + eps * du, vals + eps * dvals) -
+              dr.residual_raw(u - eps * du, vals - eps * dvals)) / (2 * eps)
+        err = float((fd - jv).norm() / jv.norm())
+        print(f"discr {discr}: tangent vs central difference rel {err:.3e}", flush=True)
+        assert err <= 1e-6
+    finally:
+        g.close()
+
+
+def test_torch_forward_is_the_solver_step(gpu):
+    m, num, cv0 = blast_box()
+    g = GpuRegion(m, num)
+    try:
+        dts, cv_solver = adj.record_time_steps(g, cv0, 25)
+        dr = adj.DiffRegion(g)
+        vals = torch.tensor(num.patch_vals, dtype=torch.float64, device="cuda")
+        with torch.no_grad():
+            u = dr.to_internal(cv0)
+            for dt in dts:
+                u = adj.ssp_rk3_step(dr, u, vals, dt)
+        cv_torch = dr.to_caller(u)
+        err = np.abs(cv_torch - cv_solver).max() / np.abs(cv_solver).max()
+        print("torch forward vs solver, max rel", err, flush=True)
+        assert err <= 1e-14
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("discr", [3, 4])
+def test_multistep_adjoint_vs_tangent_and_fd(gpu, discr):
+    m, num, cv0 = blast_box(discr)
+    g = GpuRegion(m, num)
+    try:
+        nsteps = 30
+        dts, _ = adj.record_time_steps(g, cv0, nsteps)
+        dr = adj.DiffRegion(g)
+        gp = num.gas_prime()[2]
+        wall = 2                               # the y- slip wall
+        u0 = dr.to_internal(cv0)
+        vals0 = torch.tensor(num.patch_vals, dtype=torch.float64, device="cuda")
+        mask = torch.zeros_like(vals0)
+        mask[0] = 1.0                          # only the inflow patch's values act
+
+        def J(u, v, ce=10):
+            return adj.time_integrated_wall_pressure(dr, u, v, dts, wall, gp, ce)[0]
+
+        u = u0.clone().requires_grad_(True)
+        v = vals0.clone().requires_grad_(True)
+        Jv = J(u, v)
+        Jv.backward()
+        gu, gv = u.grad.clone(), v.grad.clone()
+        # checkpointing changes nothing
+        u2 = u0.clone().requires_grad_(True)
+        v2 = vals0.clone().requires_grad_(True)
+        J(u2, v2, None).backward()
+        assert torch.equal(u2.grad, gu) and torch.equal(v2.grad, gv)
+        import torch.autograd.forward_ad as fwAD
+        worst_fd = worst_tan = 0.0
+        for k in range(4):
+            du = rand_like(u0, 10 + k) * u0
+            dv = rand_like(vals0, 20 + k) * vals0 * mask
+            ad = float((gu * du).sum() + (gv * dv).sum())
+            with fwAD.dual_level(), torch.no_grad():
+                jt = fwAD.unpack_dual(J(fwAD.make_dual(u0, du), fwAD.make_dual(vals0, dv), None)).tangent
+            tan = float(jt)
+            eps = 1e-6
+            with torch.no_grad():
+                fd = (float(J(u0 + eps * du, vals0 + eps * dv, None)) -
+                      float(J(u0 - eps * du, vals0 - eps * dv, None))) / (2 * eps)
+            e_tan = abs(ad - tan) / abs(tan)
+            e_fd = abs(ad - fd) / abs(fd)
+            print(f"dir {k}: adjoint {ad:.15e} tangent {tan:.15e} fd {fd:.15e} "
+                  f"rel(tan) {e_tan:.2e} rel(fd) {e_fd:.2e}", flush=True)
+            worst_tan = max(worst_tan, e_tan)
+            worst_fd = max(worst_fd, e_fd)
+        assert worst_tan <= 1e-10
+        assert worst_fd <= 1e-5
+        assert float(gv[0].abs().sum()) > 0.0
+    finally:
+        g.close()
