@@ -70,11 +70,14 @@ int rflu_mesh_sizes(const rflu_mesh* m, int64_t out[9]);
  * 1 for double; *count is the number of elements.  Names:
  *   int32:  "f2c"(2,F) "f2v"(4,F) "bf2c"(B) "bf2v"(4,B) "bpatch_ptr"(P+1)
  *           "patch_bc"(P) "patch_opt"(P) "c2v"(nv,N) "c2cs_ptr"(N+1) "c2cs_idx"(S)
- *           "c2f_ptr"(N+1) "c2f_idx"(k*N)
+ *           "c2cs_bptr"(N+1) "c2cs_bface"(2,Sb) "c2f_ptr"(N+1) "c2f_idx"(k*N)
  *   double: "xyz"(3,V) "fn"(4,F) "fc"(3,F) "bfn"(4,B) "bfc"(3,B) "patch_pout"(P)
  *           "vol"(N) "cofg"(3,N) "xyz_moms"(10,N)
  * c2f lists, per cell, the faces of the cell as signed ids: f+1 for interior face f
  * with the cell as c1, -(f+1) as c2, and F+b+1 for boundary face b.
+ * c2cs_bface holds, per cell (CSR c2cs_bptr), the wall faces of its stencil as
+ * (patch, face within the patch), 0-based, nearest first, at most 9
+ * (c2cs(icg)%bFaceMembs).
  */
 int rflu_mesh_array(const rflu_mesh* m, const char* name,
                     const void** ptr, int64_t* count, int* dtype);

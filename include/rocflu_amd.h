@@ -97,6 +97,15 @@ typedef struct {
   const double* patch_vals;  /* (5, n_patches) inflow BCDAT_INFLOW_U, _V, _W, _T, _P
                                 (P used by the non-subsonic options); NULL when no
                                 patch is an inflow                                   */
+  /* Constrained gradients at walls (order 2, mixtInput%cReconstCells = CRECONSTC):
+   * c2cs(icg)%bFaceMembs flattened to a CSR (c2cs_bptr, n_cells+1 0-based offsets)
+   * of (patch, face within the patch) pairs (index_base applies to both), and
+   * mixtInput%cReconstCellsWeight. */
+  const int32_t* c2cs_bptr;
+  const int32_t* c2cs_bface;
+  double creconst_weight;
+  int32_t creconst;          /* 0 off (reference default), 1 on                      */
+  int32_t pad_;
 } rflu_region_desc;
 
 /* Per-region counters of the after-update checks, accumulated since the last reset. */

@@ -33,6 +33,9 @@ class OrcRegion(ctypes.Structure):
         ("dt_min_limit", ctypes.c_double), ("eps_entr", ctypes.c_double),
         ("diss_fact", ctypes.c_double),
         ("patch_vals", _f64p),
+        ("c2cs_bptr", _i32p), ("c2cs_bface", _i32p),
+        ("creconst_weight", ctypes.c_double), ("creconst", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
     ]
 
 
@@ -68,6 +71,7 @@ def lib():
         L.orc_run_until.restype = ctypes.c_int
         L.orc_set_dependent_vars.argtypes = [R, _f64p, _f64p, ctypes.c_int, ctypes.c_int]
         L.orc_gradients.argtypes = [R, _f64p, _f64p]
+        L.orc_gradients_constr.argtypes = [R, _f64p, _f64p]
         L.orc_venkat.argtypes = [R, _f64p, _f64p]
         L.orc_weno_xyz.argtypes = [R, _f64p]
         L.orc_limit_simple.argtypes = [R, _f64p, _f64p]
@@ -105,6 +109,10 @@ class Oracle:
             setattr(r, name, _p(k[name], i32))
         for name in ("fn", "fc", "cofg", "vol", "bfn", "bfc", "patch_pout", "patch_vals"):
             setattr(r, name, _p(k[name], f64))
+        if num.order == 2 and num.creconst:
+            r.c2cs_bptr = _p(k["c2cs_bptr"], i32)
+            r.c2cs_bface = _p(k["c2cs_bface"], i32)
+            r.creconst, r.creconst_weight = 1, num.creconst_weight
         if num.order == 2:
             r.c2cs_ptr = _p(k["c2cs_ptr"], i32)
             r.c2cs_idx = _p(k["c2cs_idx"], i32)
@@ -159,6 +167,14 @@ class Oracle:
         g = np.zeros((self.n, 5, 3))
         self.L.orc_gradients(ctypes.byref(self.r), _p(var, ctypes.c_double),
                              _p(g, ctypes.c_double))
+        return g
+
+    def gradients_constr(self, var, grad):
+        """Constrained refit (CRECONSTC) of the gradients of cells next to walls."""
+        var = np.ascontiguousarray(var, dtype=np.float64)
+        g = np.array(grad, dtype=np.float64, copy=True, order="C")
+        self.L.orc_gradients_constr(ctypes.byref(self.r), _p(var, ctypes.c_double),
+                                    _p(g, ctypes.c_double))
         return g
 
     def venkat(self, var, grad):

@@ -530,6 +530,137 @@ void orc_gradients(const orc_region* R, const double* var, double* grad) {
   }
 }
 
+/* Least-squares pseudo-inverse of an m x 3 matrix a (column-major) by Householder QR:
+ * ainv (3 x m, column-major) = R^-1 Q^T.  The reference inverts with LAPACK's SVD
+ * (libflu/RFLU_InvertMatrixSVD.F90, which drops singular values below 1% of the
+ * previous); for the full-rank matrices of the constrained gradient both give the
+ * same pseudo-inverse to round-off.  Returns 1 if a is rank deficient. */
+static int pinv_qr3(int m, const double* a, double* ainv) {
+  double* q = (double*)malloc((size_t)m * 3 * sizeof(double));
+  double* v = (double*)malloc((size_t)m * 3 * sizeof(double));
+  double beta[3], r[3][3];
+  memcpy(q, a, (size_t)m * 3 * sizeof(double));
+  for (int k = 0; k < 3; ++k) {
+    double nrm = 0.0;
+    for (int i = k; i < m; ++i) nrm += q[k * m + i] * q[k * m + i];
+    nrm = sqrt(nrm);
+    double alpha = q[k * m + k] > 0 ? -nrm : nrm;
+    for (int i = 0; i < m; ++i) v[k * m + i] = i < k ? 0.0 : q[k * m + i];
+    v[k * m + k] -= alpha;
+    double vv = 0.0;
+    for (int i = k; i < m; ++i) vv += v[k * m + i] * v[k * m + i];
+    beta[k] = vv > 0.0 ? 2.0 / vv : 0.0;
+    for (int j = k; j < 3; ++j) {
+      double d = 0.0;
+      for (int i = k; i < m; ++i) d += v[k * m + i] * q[j * m + i];
+      for (int i = k; i < m; ++i) q[j * m + i] -= beta[k] * d * v[k * m + i];
+    }
+  }
+  int bad = 0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r[i][j] = j >= i ? q[j * m + i] : 0.0;
+  for (int i = 0; i < 3; ++i)
+    if (fabs(r[i][i]) < 1e-12 * fabs(r[0][0])) bad = 1;
+  /* columns of Q^T: apply the reflectors to unit vectors e_c, keep the first 3 rows,
+   * then back-substitute R x = (Q^T e_c)[0..2] */
+  double* e = (double*)malloc((size_t)m * sizeof(double));
+  for (int c = 0; c < m; ++c) {
+    for (int i = 0; i < m; ++i) e[i] = i == c ? 1.0 : 0.0;
+    for (int k = 0; k < 3; ++k) {
+      double d = 0.0;
+      for (int i = k; i < m; ++i) d += v[k * m + i] * e[i];
+      for (int i = k; i < m; ++i) e[i] -= beta[k] * d * v[k * m + i];
+    }
+    double x[3];
+    for (int i = 2; i >= 0; --i) {
+      double t = e[i];
+      for (int j = i + 1; j < 3; ++j) t -= r[i][j] * x[j];
+      x[i] = t / r[i][i];
+    }
+    for (int i = 0; i < 3; ++i) ainv[c * 3 + i] = x[i];
+  }
+  free(e); free(v); free(q);
+  return bad;
+}
+
+/* FloatEqual (modfloflu/ModTools.F90:129-148). */
+static int float_equal(double a, double b, double tol) {
+  return fabs(a - b) <= (1.0 + 0.5 * fabs(a + b)) * tol;
+}
+
+/* RFLU_ComputeGradCellsConstr, modflu/RFLU_ModDifferentiationCells.F90:1180-1554, for
+ * the patches of this restatement: slip walls constrain the velocity component along
+ * an axis-aligned wall normal to zero (RFLU_GetConstrType/Value,
+ * modflu/RFLU_ModConstraintUtils.F90:122-334, 366-745).  Cells with wall faces in
+ * their stencil get, per constrained variable, the weighted least-squares gradient
+ * of the members plus the constraint rows (weight cReconstCellsWeight), solved with
+ * column scaling and a pseudo-inverse, accumulated members first.  var (5,N) is
+ * (rho,u,v,w,p); only u, v, w can carry constraints here. */
+void orc_gradients_constr(const orc_region* R, const double* var, double* grad) {
+  if (!R->creconst || !R->c2cs_bptr) return;
+  const double cwt = R->creconst_weight;
+  for (int icg = 0; icg < R->n_cells; ++icg) {
+    int b0 = R->c2cs_bptr[icg], nb = R->c2cs_bptr[icg + 1] - b0;
+    if (nb == 0) continue;
+    const double* xc = R->cofg + 3 * (size_t)icg;
+    int s0 = R->c2cs_ptr[icg], ns = R->c2cs_ptr[icg + 1] - s0;
+    for (int iv = 1; iv <= 3; ++iv) {
+      int axis = iv - 1;
+      int nc = 0;
+      int is_c[9];
+      for (int k = 0; k < nb; ++k) {
+        int p = R->c2cs_bface[2 * (size_t)(b0 + k)], f = R->c2cs_bface[2 * (size_t)(b0 + k) + 1];
+        int b = R->bpatch_ptr[p] + f;
+        is_c[k] = R->patch_bc[p] == BC_SLIPWALL &&
+                  float_equal(fabs(R->bfn[4 * (size_t)b + axis]), 1.0, 1.0E-6);
+        nc += is_c[k];
+      }
+      if (nc == 0) continue;
+      int m = ns + nc;
+      double* a = (double*)malloc((size_t)m * 3 * sizeof(double));
+      double* ai = (double*)malloc((size_t)m * 3 * sizeof(double));
+      double* term = (double*)malloc((size_t)m * sizeof(double));
+      double* dvar = (double*)malloc((size_t)m * sizeof(double));
+      for (int s = 0; s < ns; ++s) {
+        int j = R->c2cs_idx[s0 + s];
+        double dx = R->cofg[3 * (size_t)j] - xc[0];
+        double dy = R->cofg[3 * (size_t)j + 1] - xc[1];
+        double dz = R->cofg[3 * (size_t)j + 2] - xc[2];
+        term[s] = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+        a[0 * m + s] = term[s] * dx;
+        a[1 * m + s] = term[s] * dy;
+        a[2 * m + s] = term[s] * dz;
+        dvar[s] = var[5 * (size_t)j + iv] - var[5 * (size_t)icg + iv];
+      }
+      int row = ns;
+      for (int k = 0; k < nb; ++k) {
+        if (!is_c[k]) continue;
+        int p = R->c2cs_bface[2 * (size_t)(b0 + k)], f = R->c2cs_bface[2 * (size_t)(b0 + k) + 1];
+        const double* xf = R->bfc + 3 * (size_t)(R->bpatch_ptr[p] + f);
+        double dx = xf[0] - xc[0], dy = xf[1] - xc[1], dz = xf[2] - xc[2];
+        term[row] = cwt / sqrt(dx * dx + dy * dy + dz * dz);
+        a[0 * m + row] = term[row] * dx;
+        a[1 * m + row] = term[row] * dy;
+        a[2 * m + row] = term[row] * dz;
+        dvar[row] = 0.0 - var[5 * (size_t)icg + iv];   /* slip wall: u.n = 0 */
+        ++row;
+      }
+      double colmax[3];
+      for (int c = 0; c < 3; ++c) {
+        colmax[c] = -HUGE_VAL;
+        for (int i = 0; i < m; ++i) colmax[c] = dmax(colmax[c], fabs(a[c * m + i]));
+        for (int i = 0; i < m; ++i) a[c * m + i] = a[c * m + i] / colmax[c];
+      }
+      pinv_qr3(m, a, ai);
+      double g[3] = {0.0, 0.0, 0.0};
+      for (int i = 0; i < m; ++i)
+        for (int c = 0; c < 3; ++c) g[c] = g[c] + term[i] * (ai[3 * i + c] / colmax[c]) * dvar[i];
+      for (int c = 0; c < 3; ++c) grad[15 * (size_t)icg + 3 * iv + c] = g[c];
+      free(a); free(ai); free(term); free(dvar);
+    }
+  }
+}
+
 /* RFLU_WENOGradCellsXYZFast_3D, modflu/RFLU_ModWENO.F90:1437-1508: per component, blend
  * the own and member gradients with weights 1/(g^2 + 1e-15), own value first. */
 void orc_weno_xyz(const orc_region* R, double* grad) {
@@ -652,6 +783,7 @@ void orc_cell_gradients(const orc_region* R, double* cv, const double* dv, doubl
     cv[5 * i + 4] = dv[3 * i];
   }
   orc_gradients(R, cv, grad);
+  orc_gradients_constr(R, cv, grad);
   if (R->reconst == 2) {
     orc_weno_xyz(R, grad);
     orc_limit_simple(R, cv, grad);

@@ -45,6 +45,13 @@ typedef struct {
   int32_t rk;                /* 1 RK4, 2 Wray RK3, 3 Euler, 4 SSP-RK3 */
   double cfl, dt_imposed, dt_min_limit, eps_entr, diss_fact;
   const double* patch_vals;  /* (5,P) inflow u, v, w, T, p (p: supersonic option only) */
+  /* constrained gradients (CRECONSTC): wall faces of each stencil, (patch, face in
+   * patch) pairs in CSR c2cs_bptr, and cReconstCellsWeight */
+  const int32_t* c2cs_bptr;
+  const int32_t* c2cs_bface;
+  double creconst_weight;
+  int32_t creconst;          /* 0 off, 1 on */
+  int32_t pad_;
 } orc_region;
 
 typedef struct {
@@ -66,6 +73,7 @@ void orc_set_dependent_vars(const orc_region* r, const double* cv, double* dv, i
 void orc_fluxes(const orc_region* r, const double* cv, const double* dv, const double* grad,
                 double* rhs, orc_stats* st, int stage);
 void orc_gradients(const orc_region* r, const double* var, double* grad);
+void orc_gradients_constr(const orc_region* r, const double* var, double* grad);
 void orc_weno_xyz(const orc_region* r, double* grad);
 void orc_limit_simple(const orc_region* r, const double* var, double* grad);
 void orc_venkat(const orc_region* r, const double* var, double* grad);

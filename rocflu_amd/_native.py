@@ -44,6 +44,9 @@ class RegionDesc(ctypes.Structure):
         ("dt_min_limit", ctypes.c_double), ("eps_entr", ctypes.c_double),
         ("diss_fact", ctypes.c_double),
         ("patch_vals", _f64p),
+        ("c2cs_bptr", _i32p), ("c2cs_bface", _i32p),
+        ("creconst_weight", ctypes.c_double), ("creconst", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
     ]
 
 

@@ -54,6 +54,10 @@ struct Region {
   std::vector<double> pvals_h;
   std::vector<int> bptr_h;     // first boundary face of every patch (all cases), P+1
   int* bptr = nullptr;
+  // constrained gradients (CRECONSTC): (cell, variable) entries and their rows
+  int con_n = 0, con_R = 0;
+  int *con_ecell = nullptr, *con_evar = nullptr, *con_rptr = nullptr, *con_rcell = nullptr;
+  double *con_rval = nullptr, *con_rw = nullptr;
   // derivative work buffers (allocated by the first tangent/adjoint call)
   double *adj_g = nullptr, *adj_gb = nullptr, *adj_gv = nullptr, *adj_pv = nullptr;
   double *adj_dflux = nullptr, *adj_dvals = nullptr;
@@ -126,6 +130,12 @@ struct Region {
   }
   // scalars that gate a whole launch: the single case's, none for a batch
   const Scalars* gate_sc() const { return K == 1 ? sc : nullptr; }
+  Constr constr() const {
+    Constr c;
+    c.n = con_n; c.ecell = con_ecell; c.evar = con_evar; c.rptr = con_rptr;
+    c.rcell = con_rcell; c.rval = con_rval; c.rw = con_rw; c.R = con_R;
+    return c;
+  }
   CaseParams case0() const {
     CaseParams c;
     c.cfl = cfl; c.dt_imposed = dt_imposed; c.dt_min_limit = dt_min_limit; c.end_time = end_time;
@@ -168,7 +178,8 @@ void free_region(Region* R) {
                   R->vnb, R->vgeo, R->srad, R->gw, R->cid, R->gas_tab, R->par, R->cell0,
                   R->bface0, R->dtmin, R->pbc, R->popt, R->porder2, R->ppout, R->pvals,
                   R->bptr, R->adj_g, R->adj_gb, R->adj_gv, R->adj_pv, R->adj_dflux,
-                  R->adj_dvals};
+                  R->adj_dvals, R->con_ecell, R->con_evar, R->con_rptr, R->con_rcell,
+                  R->con_rval, R->con_rw};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (R->stream) (void)hipStreamDestroy(R->stream);
@@ -451,7 +462,8 @@ template <typename GT>
 int launch_gradients_t(Region* R) {
   Mesh M = R->mesh();
   State S = R->state();
-  bool v = R->reconst == RFLU_RECONST_LIM_VENKAT;
+  // with constrained cells the limiter runs after them, as a pass of its own
+  bool v = R->reconst == RFLU_RECONST_LIM_VENKAT && R->con_n == 0;
   switch (R->grad_unroll) {
     case 1: v ? launch_gradient_kernel<GT, true, 1>(R, M, S) : launch_gradient_kernel<GT, false, 1>(R, M, S); break;
     case 2: v ? launch_gradient_kernel<GT, true, 2>(R, M, S) : launch_gradient_kernel<GT, false, 2>(R, M, S); break;
@@ -532,6 +544,20 @@ int launch_gradients(Region* R) {
     rc = grad32(R) ? launch_gradients_t<float>(R) : launch_gradients_t<double>(R);
   }
   if (rc) return rc;
+  if (R->con_n > 0) {
+    Timed t(R, KID_GRAD);
+    Mesh M = R->mesh();
+    State S = R->state();
+    Constr C = R->constr();
+    dim3 g(nblk(R->con_n, 256)), b(256);
+    if (grad32(R)) hipLaunchKernelGGL((k_grad_constr<float>), g, b, 0, R->stream, M, S, C, (float*)R->grad, R->gate_sc());
+    else hipLaunchKernelGGL((k_grad_constr<double>), g, b, 0, R->stream, M, S, C, (double*)R->grad, R->gate_sc());
+    if (R->reconst == RFLU_RECONST_LIM_VENKAT) {
+      dim3 gc(nblk(R->N, R->grad_block)), bc(R->grad_block);
+      if (grad32(R)) hipLaunchKernelGGL((k_venkat_apply<float>), gc, bc, 0, R->stream, M, S, (float*)R->grad, R->gate_sc());
+      else hipLaunchKernelGGL((k_venkat_apply<double>), gc, bc, 0, R->stream, M, S, (double*)R->grad, R->gate_sc());
+    }
+  }
   if (R->reconst == RFLU_RECONST_WENO_XYZ) {
     Timed t(R, KID_BLEND);
     rc = grad32(R) ? launch_blend_t<float>(R) : launch_blend_t<double>(R);
@@ -681,6 +707,163 @@ int status_from(const Scalars& s) {
   if (s.n_invalid) return fail(RFLU_ERR_INVALID_VALUE, "invalid (NaN) values detected");
   if (s.n_negative) return fail(RFLU_ERR_NEGATIVE_POSDEF, "negative positive-definite variables detected");
   return RFLU_OK;
+}
+
+// Pseudo-inverse of an m x 3 matrix a (column-major) by one-sided Jacobi SVD, with the
+// reference's truncation: singular values are sorted descending and the first one
+// below 1% of its predecessor (or s1 <= 100 eps) drops with all later ones
+// (libflu/RFLU_InvertMatrixSVD.F90).  ainv is 3 x m, column-major.  Returns the
+// number of dropped singular values.
+int pinv_svd3(int m, const double* a, double* ainv) {
+  std::vector<double> u(a, a + 3 * (size_t)m);
+  double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        double al = 0, be = 0, ga = 0;
+        for (int i = 0; i < m; ++i) {
+          al += u[p * m + i] * u[p * m + i];
+          be += u[q * m + i] * u[q * m + i];
+          ga += u[p * m + i] * u[q * m + i];
+        }
+        if (ga == 0.0) continue;
+        off = std::max(off, std::fabs(ga) / std::sqrt(al * be));
+        double zeta = (be - al) / (2.0 * ga);
+        double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+        double c = 1.0 / std::sqrt(1.0 + t * t), s = c * t;
+        for (int i = 0; i < m; ++i) {
+          double up = u[p * m + i], uq = u[q * m + i];
+          u[p * m + i] = c * up - s * uq;
+          u[q * m + i] = s * up + c * uq;
+        }
+        for (int i = 0; i < 3; ++i) {
+          double vp = v[i][p], vq = v[i][q];
+          v[i][p] = c * vp - s * vq;
+          v[i][q] = s * vp + c * vq;
+        }
+      }
+    if (off < 1e-15) break;
+  }
+  double sv[3];
+  int ord[3] = {0, 1, 2};
+  for (int j = 0; j < 3; ++j) {
+    double n = 0;
+    for (int i = 0; i < m; ++i) n += u[j * m + i] * u[j * m + i];
+    sv[j] = std::sqrt(n);
+  }
+  std::sort(ord, ord + 3, [&](int x, int y) { return sv[x] > sv[y]; });
+  double sinv[3] = {0, 0, 0};
+  int dropped = 0;
+  if (sv[ord[0]] > 100.0 * 2.220446049250313e-16) sinv[0] = 1.0 / sv[ord[0]];
+  else dropped = 1;
+  for (int k = 1; k < 3 && !dropped; ++k) {
+    if (sv[ord[k]] > 0.01 * sv[ord[k - 1]]) sinv[k] = 1.0 / sv[ord[k]];
+    else dropped = 3 - k;
+  }
+  // A+ = sum_k v_k (1/s_k) u_k^T with u_k = U[:, k] / s_k
+  for (int c = 0; c < m; ++c)
+    for (int i = 0; i < 3; ++i) {
+      double acc = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        int j = ord[k];
+        if (sinv[k] == 0.0) continue;
+        acc += v[i][j] * sinv[k] * sinv[k] * u[j * m + c];
+      }
+      ainv[c * 3 + i] = acc;
+    }
+  return dropped;
+}
+
+// FloatEqual (modfloflu/ModTools.F90:129-148).
+bool float_equal(double a, double b, double tol) {
+  return std::fabs(a - b) <= (1.0 + 0.5 * std::fabs(a + b)) * tol;
+}
+
+// Rows of the constrained gradients of one region (RFLU_ComputeGradCellsConstr,
+// modflu/RFLU_ModDifferentiationCells.F90:1180-1554): for every cell with wall faces in
+// its stencil and every velocity component with constraint rows there (slip walls
+// whose normal is that axis, RFLU_GetConstrType, RFLU_ModConstraintUtils.F90:187-262,
+// value 0, :470-478), the column-scaled least-squares matrix of the members and the
+// constraint rows (weight cReconstCellsWeight), its pseudo-inverse, and the per-row
+// weights term*aInv(:, row).  Cell numbers are the concatenated caller numbering.
+struct ConstrHost {
+  std::vector<int> ecell, evar, rptr{0}, rcell;
+  std::vector<double> rval, rw[3];
+};
+
+int build_constrained(const rflu_region_desc* d, int c0, int p0, const std::vector<int>& bptr,
+                      const std::vector<int>& sten_ptr, const std::vector<int>& sten_idx,
+                      const std::vector<double4>& cell, const std::vector<double4>& bfn,
+                      const std::vector<double4>& bfc, const std::vector<int>& pbc,
+                      ConstrHost& H) {
+  const int n = d->n_cells, base = d->index_base;
+  const double cwt = d->creconst_weight;
+  std::vector<double> a, ai, term;
+  std::vector<int> rc;
+  if (d->c2cs_bptr[0] != 0) return fail(RFLU_ERR_ARG, "c2cs_bptr[0] must be 0");
+  for (int lc = 0; lc < n; ++lc) {
+    const int b0 = d->c2cs_bptr[lc], nb = d->c2cs_bptr[lc + 1] - b0;
+    if (nb < 0) return fail(RFLU_ERR_ARG, "c2cs_bptr not monotone");
+    if (nb == 0) continue;
+    const int icg = c0 + lc;
+    const double4 xc = cell[icg];
+    const int s0 = sten_ptr[icg], ns = sten_ptr[icg + 1] - s0;
+    for (int iv = 1; iv <= 3; ++iv) {
+      std::vector<int> rows;   // global boundary faces of the constraint rows
+      for (int k = 0; k < nb; ++k) {
+        const int p = p0 + d->c2cs_bface[2 * (size_t)(b0 + k)] - base;
+        const int f = d->c2cs_bface[2 * (size_t)(b0 + k) + 1] - base;
+        if (p < p0 || p >= p0 + d->n_patches) return fail(RFLU_ERR_ARG, "c2cs_bface patch out of range");
+        const int b = bptr[p] + f;
+        if (f < 0 || b >= bptr[p + 1]) return fail(RFLU_ERR_ARG, "c2cs_bface face out of range");
+        const double nn = iv == 1 ? bfn[b].x : iv == 2 ? bfn[b].y : bfn[b].z;
+        if (pbc[p] == BC_SLIPWALL && float_equal(std::fabs(nn), 1.0, 1.0E-6)) rows.push_back(b);
+      }
+      if (rows.empty()) continue;
+      const int m = ns + (int)rows.size();
+      a.assign(3 * (size_t)m, 0.0);
+      ai.assign(3 * (size_t)m, 0.0);
+      term.assign(m, 0.0);
+      rc.assign(m, -1);
+      for (int s = 0; s < ns; ++s) {
+        const int j = sten_idx[s0 + s];
+        const double dx = cell[j].x - xc.x, dy = cell[j].y - xc.y, dz = cell[j].z - xc.z;
+        term[s] = 1.0 / std::sqrt(dx * dx + dy * dy + dz * dz);
+        a[s] = term[s] * dx;
+        a[m + s] = term[s] * dy;
+        a[2 * (size_t)m + s] = term[s] * dz;
+        rc[s] = j;
+      }
+      for (size_t k = 0; k < rows.size(); ++k) {
+        const int r = ns + (int)k;
+        const double4 xf = bfc[rows[k]];
+        const double dx = xf.x - xc.x, dy = xf.y - xc.y, dz = xf.z - xc.z;
+        term[r] = cwt / std::sqrt(dx * dx + dy * dy + dz * dz);
+        a[r] = term[r] * dx;
+        a[m + r] = term[r] * dy;
+        a[2 * (size_t)m + r] = term[r] * dz;
+      }
+      double colmax[3];
+      for (int c = 0; c < 3; ++c) {
+        colmax[c] = -HUGE_VAL;
+        for (int i = 0; i < m; ++i) colmax[c] = std::max(colmax[c], std::fabs(a[c * (size_t)m + i]));
+        for (int i = 0; i < m; ++i) a[c * (size_t)m + i] = a[c * (size_t)m + i] / colmax[c];
+      }
+      // the reference stops on a singular matrix (RFLU_ModDifferentiationCells.F90:1432-1436)
+      if (pinv_svd3(m, a.data(), ai.data()) != 0)
+        return fail(RFLU_ERR_ARG, "singular constrained-gradient matrix at cell " + std::to_string(lc));
+      H.ecell.push_back(icg);
+      H.evar.push_back(iv);
+      for (int i = 0; i < m; ++i) {
+        H.rcell.push_back(rc[i]);
+        H.rval.push_back(0.0);   // slip wall: the wall-normal velocity is zero
+        for (int c = 0; c < 3; ++c) H.rw[c].push_back(term[i] * (ai[3 * (size_t)i + c] / colmax[c]));
+      }
+      H.rptr.push_back((int)H.rcell.size());
+    }
+  }
+  return 0;
 }
 
 // Validates one descriptor (one case); ref, when given, is the batch's first case,
@@ -873,11 +1056,31 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
     for (int b = 0; b < B; ++b) hc2f_idx[fill[hbf2c[b]]++] = F + b + 1;
   }
 
+  // ---- constrained-gradient rows (before renumbering; cell numbers remapped below) ----
+  ConstrHost hcon;
+  if (R->order == 2)
+    for (int k = 0; k < K; ++k) {
+      const rflu_region_desc* d = descs + k;
+      if (!d->creconst) continue;
+      if (!d->c2cs_bptr || !d->c2cs_bface) { free_region(R); return fail(RFLU_ERR_ARG, "creconst needs c2cs_bptr and c2cs_bface"); }
+      if ((rc = build_constrained(d, (int)co[k], (int)po[k], R->bptr_h, hsten_ptr, hsten_idx, hcell,
+                                  hbfn, hbfc, hpbc, hcon))) {
+        free_region(R);
+        return rc;
+      }
+    }
+
   // ---- internal renumbering (Morton order of cell centroids within each case) ----
   // Storage order only: every per-cell face list keeps the reference's summation
   // order (original face order) and every stencil keeps its member order, so the
   // arithmetic is the same as in the original numbering.
   renumber(R, hcell, hf2c, hfn, hfc, hbf2c, hc2f_ptr, hc2f_idx, hsten_ptr, hsten_idx, hmoms, hepsq);
+  if (!hcon.ecell.empty()) {
+    std::vector<int> o2n(N);
+    for (int i = 0; i < N; ++i) o2n[R->new2old[i]] = i;
+    for (int& c : hcon.ecell) c = o2n[c];
+    for (int& c : hcon.rcell) if (c >= 0) c = o2n[c];
+  }
   std::vector<unsigned short> hcid;
   if (K > 1) {
     hcid.resize(N);
@@ -978,6 +1181,22 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
         return rc;
       }
       R->tile_loc_len = htile_loc.size();
+    }
+  }
+  if (!hcon.ecell.empty()) {
+    R->con_n = (int)hcon.ecell.size();
+    R->con_R = (int)hcon.rcell.size();
+    std::vector<double> rw;
+    rw.reserve(3 * (size_t)R->con_R);
+    for (int c = 0; c < 3; ++c) rw.insert(rw.end(), hcon.rw[c].begin(), hcon.rw[c].end());
+    if ((rc = dalloc(&R->con_ecell, R->con_n)) || (rc = dalloc(&R->con_evar, R->con_n)) ||
+        (rc = dalloc(&R->con_rptr, R->con_n + 1)) || (rc = dalloc(&R->con_rcell, R->con_R)) ||
+        (rc = dalloc(&R->con_rval, R->con_R)) || (rc = dalloc(&R->con_rw, 3 * (size_t)R->con_R)) ||
+        (rc = dput(R->con_ecell, hcon.ecell, R->stream)) || (rc = dput(R->con_evar, hcon.evar, R->stream)) ||
+        (rc = dput(R->con_rptr, hcon.rptr, R->stream)) || (rc = dput(R->con_rcell, hcon.rcell, R->stream)) ||
+        (rc = dput(R->con_rval, hcon.rval, R->stream)) || (rc = dput(R->con_rw, rw, R->stream))) {
+      free_region(R);
+      return rc;
     }
   }
   hipLaunchKernelGGL(k_fill_scalars, dim3(nblk(K, 64)), dim3(64), 0, R->stream, R->sc, K, 0.0, R->dtmin);

@@ -1246,6 +1246,72 @@ __global__ __launch_bounds__(128) void k_gradient_ldsw(Mesh M, State S, Tiles T,
   for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
 }
 
+// Constrained gradients at walls (RFLU_ComputeGradCellsConstr,
+// modflu/RFLU_ModDifferentiationCells.F90:1180-1554): entry e is one (cell, variable)
+// with constraint rows; its rows [rptr[e], rptr[e+1]) hold the stencil members (cell
+// index) and then the constraint rows (-1, wall value), each with the 3 weights
+// term*aInv(:,row) formed at region creation from the column-scaled pseudo-inverse.
+// The result replaces that variable's gradient, accumulated rows in order, before the
+// WENO blend or the limiter.
+struct Constr {
+  int n;
+  const int* ecell;
+  const int* evar;
+  const int* rptr;
+  const int* rcell;
+  const double* rval;
+  const double* rw;   // [3][R]
+  int R;
+};
+
+template <typename GT>
+__global__ void k_grad_constr(Mesh M, State S, Constr C, GT* __restrict__ grad,
+                              const Scalars* __restrict__ sc) {
+  if (gate(sc)) return;
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= C.n) return;
+  const int N = M.N;
+  const int i = C.ecell[e], v = C.evar[e];
+  const double vi = S.prim[(size_t)v * N + i];
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  for (int r = C.rptr[e]; r < C.rptr[e + 1]; ++r) {
+    const int j = C.rcell[r];
+    const double dVar = (j >= 0 ? S.prim[(size_t)v * N + j] : C.rval[r]) - vi;
+    gx = gx + C.rw[r] * dVar;
+    gy = gy + C.rw[(size_t)C.R + r] * dVar;
+    gz = gz + C.rw[2 * (size_t)C.R + r] * dVar;
+  }
+  grad[(size_t)(3 * v) * N + i] = GT(gx);
+  grad[(size_t)(3 * v + 1) * N + i] = GT(gy);
+  grad[(size_t)(3 * v + 2) * N + i] = GT(gz);
+}
+
+// The Venkatakrishnan limiter as a pass of its own (when constrained gradients sit
+// between the least-squares pass and the limiter).
+template <typename GT>
+__global__ void k_venkat_apply(Mesh M, State S, GT* __restrict__ grad,
+                               const Scalars* __restrict__ sc) {
+  if (gate(sc)) return;
+  const int N = M.N;
+  int i = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double vi[5];
+  GT g[15];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) vi[k] = S.prim[(size_t)k * N + i];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) g[k] = grad[(size_t)k * N + i];
+  venkat_limit<GT>(M, i, vi, g, [&](int q, double* vo) {
+    int o = M.vnb[(size_t)q * N + i];
+    if (o < 0) return false;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) vo[k] = S.prim[(size_t)k * N + o];
+    return true;
+  });
+#pragma unroll
+  for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
+}
+
 // WENO-XYZ blend (modflu/RFLU_ModWENO.F90:1437-1508) followed by the positivity clip
 // of rho and p (modflu/RFLU_ModLimiters.F90:1199-1267).
 template <typename GT>

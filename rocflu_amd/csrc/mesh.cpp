@@ -46,6 +46,7 @@ struct rflu_mesh {
   std::vector<double> patch_pout;
   std::vector<double> vol, cofg;
   std::vector<int32_t> c2cs_ptr, c2cs_idx;
+  std::vector<int32_t> c2cs_bptr, c2cs_bface;   // wall faces of each stencil: (patch, face)
   std::vector<double> xyz_moms;
   std::vector<int32_t> c2f_ptr, c2f_idx;
   int stencil_min = 0, stencil_max = 0;
@@ -367,6 +368,71 @@ void build_stencils(rflu_mesh& m) {
   }
 }
 
+// Wall faces of each cell's stencil (the constrained-gradient rows), as the
+// reference builds c2cs(icg)%bFaceMembs: RFLU_AddBFaces
+// (modflu/RFLU_ModStencilsUtils.F90:134-298) walks the cell itself and then its
+// stencil members; for every wall patch in patch order, the member's face on that
+// patch (found through the patch's cell-sorted face list) joins, up to 2*9 faces;
+// RFLU_SortBFaces (:1501-1585) ranks them by squared distance from the cell centroid
+// with the same partition-exchange sort as the stencil ranking, and
+// RFLU_ModStencilsCells.F90:1059-1079 keeps the first nBFaceMembsMax = 9 (:2866).
+// Slip walls are the wall type of this builder.
+void build_bface_stencils(rflu_mesh& m) {
+  const int N = m.n_cells, P = m.n_patches;
+  const int nmax = 9, nmax_tmp = 2 * nmax;
+  // per wall patch: cell -> face within the patch (-1: none)
+  std::vector<std::vector<int32_t>> face_of(P);
+  for (int p = 0; p < P; ++p) {
+    if (m.patch_bc[p] != RFLU_BC_SLIPWALL) continue;
+    face_of[p].assign(N, -1);
+    for (int b = m.bpatch_ptr[p]; b < m.bpatch_ptr[p + 1]; ++b)
+      face_of[p][m.bf2c[b]] = b - m.bpatch_ptr[p];
+  }
+  m.c2cs_bptr.assign(N + 1, 0);
+  m.c2cs_bface.clear();
+  std::vector<std::vector<int32_t>> lists(N);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int icg = 0; icg < N; ++icg) {
+    std::vector<int32_t> pf;   // (patch, face) pairs
+    auto visit = [&](int isg) {
+      for (int p = 0; p < P; ++p) {
+        if (face_of[p].empty()) continue;
+        int f = face_of[p][isg];
+        if (f < 0) continue;
+        if ((int)pf.size() / 2 >= nmax_tmp) return false;
+        pf.push_back(p);
+        pf.push_back(f);
+      }
+      return true;
+    };
+    bool more = visit(icg);
+    for (int s = m.c2cs_ptr[icg]; more && s < m.c2cs_ptr[icg + 1]; ++s) more = visit(m.c2cs_idx[s]);
+    const int nb = (int)pf.size() / 2;
+    if (nb == 0) continue;
+    std::vector<double> dist(nb);
+    std::vector<int32_t> key(nb);
+    const double* xc = &m.cofg[3 * (size_t)icg];
+    for (int k = 0; k < nb; ++k) {
+      const double* fcp = &m.bfc[3 * (size_t)(m.bpatch_ptr[pf[2 * k]] + pf[2 * k + 1])];
+      double dx = fcp[0] - xc[0], dy = fcp[1] - xc[1], dz = fcp[2] - xc[2];
+      dist[k] = dx * dx + dy * dy + dz * dz;
+      key[k] = k;
+    }
+    rank_sort(dist, key);
+    const int keep = std::min(nb, nmax);
+    lists[icg].resize(2 * (size_t)keep);
+    for (int k = 0; k < keep; ++k) {
+      lists[icg][2 * k] = pf[2 * key[k]];
+      lists[icg][2 * k + 1] = pf[2 * key[k] + 1];
+    }
+  }
+  for (int icg = 0; icg < N; ++icg)
+    m.c2cs_bptr[icg + 1] = m.c2cs_bptr[icg] + (int32_t)lists[icg].size() / 2;
+  m.c2cs_bface.reserve(2 * (size_t)m.c2cs_bptr[N]);
+  for (int icg = 0; icg < N; ++icg)
+    m.c2cs_bface.insert(m.c2cs_bface.end(), lists[icg].begin(), lists[icg].end());
+}
+
 int build_box(rflu_mesh& m, int kind, int nx, int ny, int nz, double lx, double ly, double lz,
               double x0, double y0, double z0, double jitter, uint64_t seed,
               const int32_t* bc_type, const int32_t* bc_opt, const double* bc_pout,
@@ -613,7 +679,10 @@ int build_box(rflu_mesh& m, int kind, int nx, int ny, int nz, double lx, double 
     for (int b = 0; b < B; ++b) m.c2f_idx[fill[m.bf2c[b]]++] = F + b + 1;
   }
 
-  if (want_stencils) build_stencils(m);
+  if (want_stencils) {
+    build_stencils(m);
+    build_bface_stencils(m);
+  }
   return 0;
 }
 
@@ -668,6 +737,8 @@ int rflu_mesh_array(const rflu_mesh* m, const char* name, const void** ptr, int6
   if (n == "c2v") return I(m->c2v);
   if (n == "c2cs_ptr") return I(m->c2cs_ptr);
   if (n == "c2cs_idx") return I(m->c2cs_idx);
+  if (n == "c2cs_bptr") return I(m->c2cs_bptr);
+  if (n == "c2cs_bface") return I(m->c2cs_bface);
   if (n == "c2f_ptr") return I(m->c2f_ptr);
   if (n == "c2f_idx") return I(m->c2f_idx);
   if (n == "xyz") return D(m->xyz);

@@ -106,6 +106,8 @@ PROGRAM rflu_driver
   desc%cfl = 0.5_C_DOUBLE; desc%dt_imposed = 1.0E-3_C_DOUBLE; desc%dt_min_limit = 0.0_C_DOUBLE
   desc%eps_entr = 0.05_C_DOUBLE; desc%diss_fact = 1.0_C_DOUBLE
   desc%patch_vals = C_NULL_PTR
+  desc%c2cs_bptr = C_NULL_PTR; desc%c2cs_bface = C_NULL_PTR
+  desc%creconst_weight = 1.0_C_DOUBLE; desc%creconst = 0; desc%pad_ = 0
   CALL rflu_gpu_check(rflu_gpu_create_region(desc, handle), 'rflu_gpu_create_region')
 
 ! ------------------------------------------------------------------------------

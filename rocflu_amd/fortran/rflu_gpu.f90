@@ -36,6 +36,9 @@ MODULE rflu_gpu
     INTEGER(C_INT32_T) :: discr, order, reconst, rk, precision
     REAL(C_DOUBLE) :: cfl, dt_imposed, dt_min_limit, eps_entr, diss_fact
     TYPE(C_PTR) :: patch_vals
+    TYPE(C_PTR) :: c2cs_bptr, c2cs_bface
+    REAL(C_DOUBLE) :: creconst_weight
+    INTEGER(C_INT32_T) :: creconst, pad_
   END TYPE rflu_region_desc
 
   TYPE, BIND(C) :: rflu_gpu_stats

@@ -56,7 +56,8 @@ def _lib():
 
 _ARRAYS = {
     "f2c": 2, "f2v": 4, "bf2c": 1, "bf2v": 4, "bpatch_ptr": 1, "patch_bc": 1,
-    "patch_opt": 1, "c2v": None, "c2cs_ptr": 1, "c2cs_idx": 1, "c2f_ptr": 1,
+    "patch_opt": 1, "c2v": None, "c2cs_ptr": 1, "c2cs_idx": 1, "c2cs_bptr": 1,
+    "c2cs_bface": 2, "c2f_ptr": 1,
     "c2f_idx": 1, "xyz": 3, "fn": 4, "fc": 3, "bfn": 4, "bfc": 3,
     "patch_pout": 1, "vol": 1, "cofg": 3, "xyz_moms": 10,
 }

@@ -48,6 +48,10 @@ class Numerics:
     patch_order: Optional[Sequence[int]] = None
     # per patch (u, v, w, T, p) of inflow patches (BCDAT_INFLOW_U.._P); zeros elsewhere
     patch_vals: Optional[Sequence[Sequence[float]]] = None
+    # constrained gradients at walls (CRECONSTC, CRECONSTCW; RFLU_InitInputValues.F90:
+    # 188-189 defaults 0 and 1.0)
+    creconst: int = 0
+    creconst_weight: float = 1.0
 
     def gas_prime(self):
         """(M, R', gamma') of the dependent-variable path (libflu/RFLU_SetGasVars.F90:172-178)."""
@@ -88,6 +92,9 @@ def region_arrays(mesh: Mesh, num: Numerics) -> dict:
         keep["c2cs_ptr"] = np.ascontiguousarray(a["c2cs_ptr"], dtype=np.int32)
         keep["c2cs_idx"] = np.ascontiguousarray(a["c2cs_idx"], dtype=np.int32)
         keep["xyz_moms"] = np.ascontiguousarray(a["xyz_moms"], dtype=np.float64).reshape(-1)
+        if num.creconst:
+            keep["c2cs_bptr"] = np.ascontiguousarray(a["c2cs_bptr"], dtype=np.int32)
+            keep["c2cs_bface"] = np.ascontiguousarray(a["c2cs_bface"], dtype=np.int32).reshape(-1)
     return keep
 
 
@@ -123,6 +130,11 @@ def region_desc(mesh: Mesh, num: Numerics, d: "nat.RegionDesc"):
     d.cfl, d.dt_imposed, d.dt_min_limit = num.cfl, num.dt_imposed, num.dt_min_limit
     d.eps_entr, d.diss_fact = num.eps_entr, num.diss_fact
     d.patch_vals = nat.ptr(k["patch_vals"], f64)
+    if num.order == 2 and num.creconst:
+        d.c2cs_bptr = nat.ptr(k["c2cs_bptr"], i32)
+        d.c2cs_bface = nat.ptr(k["c2cs_bface"], i32)
+    d.creconst_weight = num.creconst_weight
+    d.creconst = num.creconst if num.order == 2 else 0
     return k
 
 

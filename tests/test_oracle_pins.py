@@ -331,3 +331,39 @@ def test_inflow_boundary_state_closed_form(opt):
         expect[c] += 0.5 * (normal_flux(rb, u_in, pb, gp, n) -
                             normal_flux(rho, np.array([u0, 0.0, 0.0]), p, gp, n)) * area
     assert np.abs(r - expect).max() <= 1e-13 * np.abs(expect).max(), np.abs(r - expect).max()
+
+
+@pytest.mark.parametrize("kind", [rm.HEX, rm.TET6])
+def test_constrained_gradient_exact_on_admissible_linear_fields(kind):
+    """CRECONSTC 1: a cell's velocity gradient is refitted with the slip-wall rows
+    (u.n = 0 on walls normal to the axis).  For u = a x (zero on the x = 0 wall) the
+    fit of every cell in the lower half of the box is exact, as it is for v = b y and
+    w = c z; density and pressure keep their least-squares gradients."""
+    n = 6
+    m = rm.box(kind, n, n, n, lengths=(float(n),) * 3, jitter=0.1, seed=2,
+               bc_type=(rm.BC_SLIPWALL,) * 6)
+    num = Numerics(discr=3, order=2, reconst=0, creconst=1)
+    x = m.cofg
+    var = np.empty((m.n_cells, 5))
+    var[:, 0] = 1.0 + 0.1 * x[:, 0] - 0.2 * x[:, 1] + 0.05 * x[:, 2]
+    var[:, 1] = 0.3 * x[:, 0]
+    var[:, 2] = -0.2 * x[:, 1]
+    var[:, 3] = 0.7 * x[:, 2]
+    var[:, 4] = 2.0 - 0.1 * x[:, 2]
+    o = Oracle(m, num)
+    g = o.gradients(var)
+    gc = o.gradients_constr(var, g)
+    nb = np.diff(m.a["c2cs_bptr"])
+    lower = (x < 0.5 * n).all(axis=1)
+    assert (nb[lower] > 0).sum() > 10
+    exact = np.array([[0.1, -0.2, 0.05], [0.3, 0, 0], [0, -0.2, 0], [0, 0, 0.7], [0, 0, -0.1]])
+    err = np.abs(gc[lower] - exact[None]).max()
+    assert err <= 1e-12, err
+    # rho and p untouched; velocity rows changed somewhere (the walls act)
+    assert np.array_equal(gc[:, [0, 4]], g[:, [0, 4]])
+    u = var.copy()
+    u[:, 1] = 1.0                                   # violates u = 0 at x walls
+    g1 = o.gradients(u)
+    g2 = o.gradients_constr(u, g1)
+    near = nb > 0
+    assert np.abs(g1[near, 1]).max() <= 1e-12 and np.abs(g2[near, 1, 0]).max() > 1e-3
