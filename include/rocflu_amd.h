@@ -35,6 +35,11 @@ extern "C" {
 #define RFLU_ERR_HIP 4
 #define RFLU_ERR_NOMEM 5
 #define RFLU_ERR_HANDLE 6
+/* Fluid dumps: a file could not be opened, read, written or renamed (the reference's
+ * ERR_FILE_OPEN/_READ/_CLOSE), or it is not a dump this region can read
+ * (ERR_INVALID_MARKER, ERR_INVALID_NCELLS, ERR_ILLEGAL_VALUE). */
+#define RFLU_ERR_FILE 7
+#define RFLU_ERR_DUMP 8
 
 /* Flux schemes (DISCR, modfloflu/ModParameters.F90:662-667). */
 #define RFLU_DISCR_ROE 1
@@ -253,6 +258,40 @@ int rflu_gpu_time_kernel(int32_t handle, int32_t which, int32_t reps, double* ms
  * step, 5 boundary flux).  Turning it on or off clears the totals. */
 int rflu_gpu_set_profiling(int32_t handle, int32_t on);
 int rflu_gpu_get_kernel_times(int32_t handle, double* ms, int64_t* counts, int32_t n);
+
+/*
+ * Fluid dumps (restart files), the reference's RFLUDUMP stream layout, version 5
+ * (modflu/RFLU_ModRestartData.F90:7-33): "RFLUDUMP", int32 version, int64 step,
+ * double time, int32 region, nCells, nCv, form, withPicl, then cv(iCv,1:nCells) one
+ * variable after the other, then int32 number of NSCBC patches (with their face
+ * states).  Gas-only regions write the pure form (1), no PICL fields and no NSCBC
+ * patches; the reader accepts versions 4 and 5 with the checks of
+ * RFLU_ReadRestartData (:693-872): sizes must match the region, the form must be the
+ * pure one, PICL fields are read past, and a dump with NSCBC face states is refused.
+ */
+/* restart_data/fluid_<step:8 digits>_<region:5 digits>.dat under out_dir (which ends
+ * in '/' like global%outDir, or is empty), as FluidDumpName (:258-266).  buf gets at
+ * most buflen bytes including the terminating NUL. */
+int rflu_dump_name(char* buf, int32_t buflen, const char* out_dir, int64_t step, int32_t region);
+/* Host arrays: writes cv(n_cv, n_cells) (component fastest) as a dump, through
+ * <path>.tmp and a rename (RFLU_WriteRestartData :136-215, RFLU_FinishRestartFile
+ * :904-918). */
+int rflu_dump_write(const char* path, int64_t step, double time, int32_t region,
+                    int32_t n_cells, int32_t n_cv, const double* cv);
+/* Host arrays: reads a dump of n_cells cells and n_cv variables into cv(n_cv, n_cells);
+ * step, time and region receive the header fields (any may be NULL). */
+int rflu_dump_read(const char* path, int32_t n_cells, int32_t n_cv, int64_t* step,
+                   double* time, int32_t* region, double* cv);
+/* A case of a region handle (0 for a single region; case i of a batch): writes its
+ * current state as the dump of `region` at (step, time), the host's
+ * global%currentStep/currentTime.  Waits for the stream. */
+int rflu_gpu_write_dump(int32_t handle, int32_t icase, const char* path, int64_t step,
+                        double time, int32_t region);
+/* Reads a dump into case icase: its state, dependent variables and clock (the dump's
+ * time; counters and flags of the case reset as after an upload).  step, time and
+ * region receive the header fields (any may be NULL).  Waits for the stream. */
+int rflu_gpu_read_dump(int32_t handle, int32_t icase, const char* path, int64_t* step,
+                       double* time, int32_t* region);
 
 const char* rflu_gpu_last_error(void);
 

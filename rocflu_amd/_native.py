@@ -17,10 +17,12 @@ RFLU_ERR_ARG = 3
 RFLU_ERR_HIP = 4
 RFLU_ERR_NOMEM = 5
 RFLU_ERR_HANDLE = 6
+RFLU_ERR_FILE = 7
+RFLU_ERR_DUMP = 8
 
 ERROR_NAMES = {
     1: "ERR_INVALID_VALUE", 2: "ERR_NEGATIVE_POSDEF", 3: "ERR_ARG", 4: "ERR_HIP",
-    5: "ERR_NOMEM", 6: "ERR_HANDLE",
+    5: "ERR_NOMEM", 6: "ERR_HANDLE", 7: "ERR_FILE", 8: "ERR_DUMP",
 }
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -99,6 +101,16 @@ SIGNATURES = {
                                                  ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
     "rflu_gpu_set_option": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "rflu_gpu_get_option": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _i32p]),
+    "rflu_dump_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_char_p,
+                                      ctypes.c_int64, ctypes.c_int32]),
+    "rflu_dump_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_double,
+                                       ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _f64p]),
+    "rflu_dump_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.POINTER(ctypes.c_int64), _f64p, _i32p, _f64p]),
+    "rflu_gpu_write_dump": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p,
+                                           ctypes.c_int64, ctypes.c_double, ctypes.c_int32]),
+    "rflu_gpu_read_dump": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p,
+                                          ctypes.POINTER(ctypes.c_int64), _f64p, _i32p]),
     "rflu_gpu_last_error": (ctypes.c_char_p, []),
 }
 

@@ -4,6 +4,7 @@
 #include "rocflu_amd.h"
 #include "kernels.hpp"
 #include "adjoint.hpp"
+#include "dump.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -1199,7 +1200,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       return rc;
     }
   }
-  hipLaunchKernelGGL(k_fill_scalars, dim3(nblk(K, 64)), dim3(64), 0, R->stream, R->sc, K, 0.0, R->dtmin);
+  hipLaunchKernelGGL(k_fill_scalars, dim3(nblk(K, 64)), dim3(64), 0, R->stream, R->sc, 0, K, 0.0, R->dtmin);
   if (hipMemsetAsync(R->srad, 0, FB * sizeof(double), R->stream) != hipSuccess ||
       hipStreamSynchronize(R->stream) != hipSuccess) {
     free_region(R);
@@ -1267,7 +1268,7 @@ int rflu_gpu_upload_state(int32_t h, const double* cv, const double* dv, double 
     State S = R->state();
     hipLaunchKernelGGL(k_set_prim, dim3(nblk(N, 256)), dim3(256), 0, R->stream, M, S);
   }
-  hipLaunchKernelGGL(k_fill_scalars, dim3(nblk(R->K, 64)), dim3(64), 0, R->stream, R->sc, R->K, time, R->dtmin);
+  hipLaunchKernelGGL(k_fill_scalars, dim3(nblk(R->K, 64)), dim3(64), 0, R->stream, R->sc, 0, R->K, time, R->dtmin);
   HIP_TRY(hipStreamSynchronize(R->stream));
   return RFLU_OK;
 }
@@ -1786,6 +1787,122 @@ int rflu_gpu_get_kernel_times(int32_t h, double* ms, int64_t* counts, int32_t n)
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
   if (!ms || !counts || n < 1) return fail(RFLU_ERR_ARG, "bad arguments");
   for (int k = 0; k < n && k < KID_N; ++k) { ms[k] = R->kern_ms[k]; counts[k] = R->kern_n[k]; }
+  return RFLU_OK;
+}
+
+int rflu_dump_name(char* buf, int32_t buflen, const char* out_dir, int64_t step, int32_t region) {
+  if (!buf || buflen <= 0) return fail(RFLU_ERR_ARG, "null or empty name buffer");
+  if (step < 0 || step > 99999999 || region < 0 || region > 99999)
+    return fail(RFLU_ERR_ARG, "step or region does not fit the dump name's fields");
+  std::string s = rflu_dump::name(out_dir ? out_dir : "", step, region);
+  if ((int64_t)s.size() + 1 > buflen) return fail(RFLU_ERR_ARG, "name buffer too small");
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  return RFLU_OK;
+}
+
+namespace {
+
+int dump_status(rflu_dump::Status s, const std::string& msg) {
+  if (s == rflu_dump::OK) return RFLU_OK;
+  return fail(s == rflu_dump::FILE_ERR ? RFLU_ERR_FILE : RFLU_ERR_DUMP, msg);
+}
+
+}  // namespace
+
+int rflu_dump_write(const char* path, int64_t step, double time, int32_t region,
+                    int32_t n_cells, int32_t n_cv, const double* cv) {
+  if (!path || (!cv && n_cells > 0) || n_cells < 0 || n_cv < 1)
+    return fail(RFLU_ERR_ARG, "bad dump arguments");
+  std::vector<std::vector<double>> var(n_cv, std::vector<double>((size_t)n_cells));
+  std::vector<const double*> ptrs(n_cv);
+  for (int iv = 0; iv < n_cv; ++iv) {
+    for (int c = 0; c < n_cells; ++c) var[iv][c] = cv[(size_t)n_cv * c + iv];
+    ptrs[iv] = var[iv].data();
+  }
+  rflu_dump::Header h;
+  h.step = step; h.time = time; h.region = region; h.n_cells = n_cells; h.n_cv = n_cv;
+  std::string msg;
+  return dump_status(rflu_dump::write(path, h, ptrs.data(), msg), msg);
+}
+
+int rflu_dump_read(const char* path, int32_t n_cells, int32_t n_cv, int64_t* step,
+                   double* time, int32_t* region, double* cv) {
+  if (!path || !cv || n_cells < 0 || n_cv < 1) return fail(RFLU_ERR_ARG, "bad dump arguments");
+  std::vector<std::vector<double>> var(n_cv, std::vector<double>((size_t)n_cells));
+  std::vector<double*> ptrs(n_cv);
+  for (int iv = 0; iv < n_cv; ++iv) ptrs[iv] = var[iv].data();
+  rflu_dump::Header h;
+  std::string msg;
+  int rc = dump_status(rflu_dump::read(path, n_cells, n_cv, h, ptrs.data(), msg), msg);
+  if (rc) return rc;
+  for (int c = 0; c < n_cells; ++c)
+    for (int iv = 0; iv < n_cv; ++iv) cv[(size_t)n_cv * c + iv] = var[iv][c];
+  if (step) *step = h.step;
+  if (time) *time = h.time;
+  if (region) *region = h.region;
+  return RFLU_OK;
+}
+
+int rflu_gpu_write_dump(int32_t h, int32_t icase, const char* path, int64_t step, double time,
+                        int32_t region) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!path || icase < 0 || icase >= R->K) return fail(RFLU_ERR_ARG, "bad case or null path");
+  const int N = R->N, c0 = R->cell0_h[icase], n = R->cell0_h[icase + 1] - c0;
+  // device [5][N] in the region's cell order -> the case's cells in the caller's order
+  std::vector<double> dev((size_t)5 * n);
+  for (int iv = 0; iv < 5; ++iv)
+    HIP_TRY(hipMemcpyAsync(dev.data() + (size_t)iv * n, R->cv + (size_t)iv * N + c0,
+                           n * sizeof(double), hipMemcpyDeviceToHost, R->stream));
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  std::vector<double> var((size_t)5 * n);
+  for (int c = 0; c < n; ++c) {
+    int o = R->new2old[c0 + c] - c0;
+    for (int iv = 0; iv < 5; ++iv) var[(size_t)iv * n + o] = dev[(size_t)iv * n + c];
+  }
+  const double* ptrs[5];
+  for (int iv = 0; iv < 5; ++iv) ptrs[iv] = var.data() + (size_t)iv * n;
+  rflu_dump::Header hd;
+  hd.step = step; hd.time = time; hd.region = region; hd.n_cells = n; hd.n_cv = 5;
+  std::string msg;
+  return dump_status(rflu_dump::write(path, hd, ptrs, msg), msg);
+}
+
+int rflu_gpu_read_dump(int32_t h, int32_t icase, const char* path, int64_t* step, double* time,
+                       int32_t* region) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!path || icase < 0 || icase >= R->K) return fail(RFLU_ERR_ARG, "bad case or null path");
+  const int N = R->N, c0 = R->cell0_h[icase], n = R->cell0_h[icase + 1] - c0;
+  std::vector<double> var((size_t)5 * n);
+  double* ptrs[5];
+  for (int iv = 0; iv < 5; ++iv) ptrs[iv] = var.data() + (size_t)iv * n;
+  rflu_dump::Header hd;
+  std::string msg;
+  int rc = dump_status(rflu_dump::read(path, n, 5, hd, ptrs, msg), msg);
+  if (rc) return rc;
+  std::vector<double> dev((size_t)5 * n);
+  for (int c = 0; c < n; ++c) {
+    int o = R->new2old[c0 + c] - c0;
+    for (int iv = 0; iv < 5; ++iv) dev[(size_t)iv * n + c] = var[(size_t)iv * n + o];
+  }
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  for (int iv = 0; iv < 5; ++iv)
+    HIP_TRY(hipMemcpyAsync(R->cv + (size_t)iv * N + c0, dev.data() + (size_t)iv * n,
+                           n * sizeof(double), hipMemcpyHostToDevice, R->stream));
+  // dv = f(cv) holds in every cell after an update, so recomputing it over the whole
+  // region leaves the other cases unchanged
+  Mesh M = R->mesh();
+  State S = R->state();
+  hipLaunchKernelGGL(k_set_dv, dim3(nblk(N, 256)), dim3(256), 0, R->stream, M, S, R->gas, R->cases());
+  if (R->order == 2) hipLaunchKernelGGL(k_set_prim, dim3(nblk(N, 256)), dim3(256), 0, R->stream, M, S);
+  hipLaunchKernelGGL(k_fill_scalars, dim3(1), dim3(64), 0, R->stream, R->sc, icase, icase + 1,
+                     hd.time, R->dtmin);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  if (step) *step = hd.step;
+  if (time) *time = hd.time;
+  if (region) *region = hd.region;
   return RFLU_OK;
 }
 

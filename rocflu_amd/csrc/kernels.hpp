@@ -1553,10 +1553,11 @@ __global__ void k_mass_inout(Mesh M, Cases C, Patches P, const FT* __restrict__ 
   }
 }
 
-__global__ void k_fill_scalars(Scalars* scs, int K, double time,
+// Clock and counters of cases [c0, c1) as after an upload at `time`.
+__global__ void k_fill_scalars(Scalars* scs, int c0, int c1, double time,
                                unsigned long long* dtmin) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= K) return;
+  int c = c0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= c1) return;
   Scalars* sc = scs + c;
   sc->time = time;
   sc->dt = 0.0;

@@ -8,7 +8,8 @@
 ! Usage: rflu_driver nx ny nz nsteps discr order out.bin
 !   Builds the Sod box (nx x ny x nz unit hexahedra, nodes jittered 10%, seed 1,
 !   supersonic outflow at both x ends, slip walls elsewhere), runs nsteps Wray RK3
-!   steps and writes cv(5,nCells) as raw doubles to out.bin.
+!   steps and writes cv(5,nCells) as raw doubles to out.bin, and the region's fluid
+!   dump (step nsteps, region 0) to out.bin.dump.
 !
 !******************************************************************************
 PROGRAM rflu_driver
@@ -148,6 +149,8 @@ PROGRAM rflu_driver
     time = time + dt
   END DO
 
+  CALL rflu_gpu_check(rflu_gpu_write_dump(handle, 0, TRIM(outfile)//'.dump'//C_NULL_CHAR, &
+                                          INT(nsteps, C_INT64_T), time, 0), 'write_dump')
   CALL rflu_gpu_check(rflu_gpu_download_state(handle, cv, dv), 'download')
   CALL rflu_gpu_check(rflu_gpu_destroy_region(handle), 'destroy')
   OPEN(NEWUNIT=u, FILE=TRIM(outfile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')

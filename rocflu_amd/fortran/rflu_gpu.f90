@@ -24,7 +24,8 @@ MODULE rflu_gpu
   INTEGER(C_INT), PARAMETER :: RFLU_OK = 0, RFLU_ERR_INVALID_VALUE = 1, &
                                RFLU_ERR_NEGATIVE_POSDEF = 2, RFLU_ERR_ARG = 3, &
                                RFLU_ERR_HIP = 4, RFLU_ERR_NOMEM = 5, &
-                               RFLU_ERR_HANDLE = 6
+                               RFLU_ERR_HANDLE = 6, RFLU_ERR_FILE = 7, &
+                               RFLU_ERR_DUMP = 8
 
   TYPE, BIND(C) :: rflu_region_desc
     INTEGER(C_INT32_T) :: index_base, n_cells, n_faces, n_patches
@@ -151,6 +152,36 @@ MODULE rflu_gpu
       IMPORT :: C_INT, C_INT32_T, rflu_gpu_stats
       INTEGER(C_INT32_T), VALUE :: handle
       TYPE(rflu_gpu_stats), INTENT(OUT) :: stats
+    END FUNCTION
+
+    ! fluid dumps: in place of RFLU_WriteRestartData / RFLU_ReadRestartData
+    ! (modflu/RFLU_ModRestartData.F90:136-215,693-872); paths end in C_NULL_CHAR
+    INTEGER(C_INT) FUNCTION rflu_gpu_write_dump(handle, icase, path, step, time, region) &
+        BIND(C, NAME='rflu_gpu_write_dump')
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_CHAR
+      INTEGER(C_INT32_T), VALUE :: handle, icase, region
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+      INTEGER(C_INT64_T), VALUE :: step
+      REAL(C_DOUBLE), VALUE :: time
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_read_dump(handle, icase, path, step, time, region) &
+        BIND(C, NAME='rflu_gpu_read_dump')
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_CHAR
+      INTEGER(C_INT32_T), VALUE :: handle, icase
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+      INTEGER(C_INT64_T), INTENT(OUT) :: step
+      REAL(C_DOUBLE), INTENT(OUT) :: time
+      INTEGER(C_INT32_T), INTENT(OUT) :: region
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_dump_name(buf, buflen, out_dir, step, region) &
+        BIND(C, NAME='rflu_dump_name')
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_CHAR
+      CHARACTER(KIND=C_CHAR), INTENT(OUT) :: buf(*)
+      INTEGER(C_INT32_T), VALUE :: buflen, region
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: out_dir(*)
+      INTEGER(C_INT64_T), VALUE :: step
     END FUNCTION
 
     FUNCTION rflu_gpu_last_error() BIND(C, NAME='rflu_gpu_last_error')

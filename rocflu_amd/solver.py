@@ -11,6 +11,7 @@ ERR_NEGATIVE_POSDEF).
 from __future__ import annotations
 
 import ctypes
+import os
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -154,6 +155,7 @@ class GpuRegion:
         del keep
         self.handle = h.value
         self._lib = L
+        self.k = 1
 
     def signature(self) -> str:
         m, num = self.mesh, self.num
@@ -179,6 +181,26 @@ class GpuRegion:
                                                     nat.ptr(dv, ctypes.c_double)),
                   "rflu_gpu_download_state")
         return cv, dv
+
+    # -- fluid dumps (modflu/RFLU_ModRestartData.F90) --------------------------
+    def write_dump(self, path: str, step: int, time: float, region: int = 0, icase: int = 0):
+        """Writes the current state of case icase as the RFLUDUMP file of `region` at
+        (step, time) (RFLU_WriteRestartData)."""
+        nat.check(self._lib.rflu_gpu_write_dump(self.handle, int(icase), os.fsencode(path),
+                                                int(step), float(time), int(region)),
+                  "rflu_gpu_write_dump")
+
+    def read_dump(self, path: str, icase: int = 0):
+        """Reads an RFLUDUMP file into case icase (state, dependent variables and the
+        dump's time, RFLU_ReadRestartData); returns (step, time, region)."""
+        step, time, region = ctypes.c_int64(), ctypes.c_double(), ctypes.c_int32()
+        nat.check(self._lib.rflu_gpu_read_dump(self.handle, int(icase), os.fsencode(path),
+                                               ctypes.byref(step), ctypes.byref(time),
+                                               ctypes.byref(region)),
+                  "rflu_gpu_read_dump")
+        if self.k == 1:
+            self.time = time.value
+        return step.value, time.value, region.value
 
     # -- RungeKuttaMP's callees (libfloflu/RungeKuttaMP.F90:216-437) -----------
     def time_step(self, current_time: Optional[float] = None) -> float:

@@ -8,6 +8,7 @@ import tempfile
 import numpy as np
 import pytest
 
+from oracle import dump as odump
 from rocflu_amd import cases
 from rocflu_amd.solver import GpuRegion, Numerics
 
@@ -28,6 +29,8 @@ def test_fortran_driver_matches_python_host(gpu, order):
         print(p.stdout, flush=True)
         assert p.returncode == 0
         cvf = np.fromfile(out, dtype=np.float64).reshape(-1, 5)
+        d = odump.decode(open(out + ".dump", "rb").read())
+        assert d["step"] == nsteps and d["region"] == 0 and d["cv"].tobytes() == cvf.tobytes()
     m = cases.sod_box(nx=nx, ny=ny, nz=nz, stencils=(order == 2))
     num = Numerics(discr=3, order=order, reconst=2 if order == 2 else 0, rk=2, cfl=0.5,
                    dt_imposed=1e-3)
