@@ -81,25 +81,46 @@ def build_case(args, precision):
     return num
 
 
+def host_threads():
+    """Threads the host grants this process (OMP_NUM_THREADS, else the affinity mask)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
 def cpu_baseline(args):
-    """The oracle (a single-threaded C restatement of the reference loops) on a bounded
-    sample of the same workload: a 20^3 x 6-tet mesh, same scheme, a few steps."""
+    """The oracle (the C restatement of the reference loops) on a bounded sample of the
+    same workload (a cpu_n^3 x 6-tet mesh, same scheme, cpu_steps steps), once with one
+    thread and once with every thread the host grants (OpenMP over cells, gathers in
+    place of the face scatters: the same bits).  value is the all-threads rate."""
     from rocflu_amd import cases
+    from oracle import oracle as orc
     from oracle.oracle import Oracle
     n = args.cpu_n
     m = cases.tet_box(n=n, jitter=0.08, seed=2)
     num = build_case(args, 0)
     cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
     o = Oracle(m, num)
-    o.run(cv0, 1)  # warm-up
     steps = args.cpu_steps
-    t0 = time.perf_counter()
-    _, _, st = o.run(cv0, steps)
-    el = time.perf_counter() - t0
-    return {"value": m.n_cells * 3 * steps / el, "unit": "cell-updates/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{n}^3x6 = {m.n_cells} tets, {steps} SSP-RK3 steps of the same scheme, "
-                      f"{el:.1f} s, single thread"}
+    rates, times = {}, {}
+    finals = {}
+    for th in (1, host_threads()):
+        used = orc.set_threads(th)
+        o.run(cv0, 1)  # warm-up
+        t0 = time.perf_counter()
+        finals[used], _, _ = o.run(cv0, steps)
+        times[used] = time.perf_counter() - t0
+        rates[used] = m.n_cells * 3 * steps / times[used]
+    orc.set_threads(1)
+    tmax = max(rates)
+    same = all(np.array_equal(finals[1], f) for f in finals.values())
+    return {"value": rates[tmax], "unit": "cell-updates/s", "cores": tmax, "kind": "port",
+            "value_1thread": rates[1], "thread_speedup": rates[tmax] / rates[1],
+            "threads_same_bits": bool(same),
+            "sample": f"{n}^3x6 = {m.n_cells} tets, {steps} SSP-RK3 steps of the same scheme: "
+                      f"{times[1]:.1f} s on 1 thread, {times[tmax]:.1f} s on {tmax} threads"}
 
 
 def run(args):
@@ -340,7 +361,7 @@ def main():
     ap.add_argument("--reconst", type=int, default=11)
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--err-steps", type=int, default=200)
-    ap.add_argument("--cpu-n", type=int, default=20)
+    ap.add_argument("--cpu-n", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-cases", type=int, default=64)

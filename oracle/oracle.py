@@ -84,8 +84,18 @@ def lib():
         L.orc_after_update.restype = ctypes.c_int
         L.orc_fluxes.argtypes = [R, _f64p, _f64p, _f64p, _f64p, ctypes.POINTER(OrcStats),
                                  ctypes.c_int]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
         _L = L
     return _L
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's loops (1: the sequential restatement); returns the count
+    in effect.  Any count gives the same bits."""
+    L = lib()
+    L.orc_set_threads(int(n))
+    return L.orc_get_threads()
 
 
 def _p(a, t):

@@ -9,12 +9,37 @@
  * face by face, interior faces first and then patch by patch, which fixes the
  * per-cell summation order the GPU gather reproduces.  Expressions keep the
  * reference's operand order and parenthesisation.
+ *
+ * Threads (orc_set_threads, for timing the CPU path on several cores): per-cell loops
+ * run under OpenMP, and the three face scatters (residual, Venkatakrishnan min/max and
+ * limiter, spectral-radius sums) become per-cell gathers over each cell's faces in
+ * ascending face order (interior faces, then boundary faces in patch order), which adds
+ * the same terms in the same order as the scatter; min and max do not depend on order.
+ * With one thread (the default) every loop is the sequential restatement.
  */
 #include "rflu_oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static int g_threads = 1;
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  g_threads = n < 1 ? 1 : n;
+#else
+  (void)n;
+  g_threads = 1;
+#endif
+}
+
+int orc_get_threads(void) { return g_threads; }
+
+#define ORC_PAR _Pragma("omp parallel for schedule(static) num_threads(g_threads) if(g_threads > 1)")
 
 #define BC_INFLOW_VELTEMP 11
 #define BC_OUTFLOW 20
@@ -42,6 +67,7 @@ static void gas_prime(const orc_region* r, double* mm, double* rgas, double* gam
 void orc_set_dependent_vars(const orc_region* r, const double* cv, double* dv, int beg, int end) {
   double mm, gc, g;
   gas_prime(r, &mm, &gc, &g);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if(g_threads > 1 && end - beg > 4096)
   for (int icg = beg; icg < end; ++icg) {
     const double* c = cv + 5 * (size_t)icg;
     double rr = c[0];
@@ -455,11 +481,85 @@ static void boundary_flux(const orc_region* R, const double* cv, const double* d
   *has_mass = 1;
 }
 
+/* Faces of cell icg in the scatter's order: keys f (interior face f) and F + b
+ * (boundary face b), ascending; sign[k] = +1 where the cell is c1 of an interior face,
+ * -1 where it is c2, 0 for a boundary face.  Returns the count (at most 16). */
+static int cell_faces_sorted(const orc_region* R, int icg, int* key, int* sign) {
+  int n = 0;
+  const int F = R->n_faces;
+  for (int k = R->c2f_ptr[icg]; k < R->c2f_ptr[icg + 1] && n < 16; ++k) {
+    int e = R->c2f_idx[k], kk, sg;
+    if (e > F) { kk = e - 1; sg = 0; }
+    else if (e > 0) { kk = e - 1; sg = 1; }
+    else { kk = -e - 1; sg = -1; }
+    int j = n++;
+    while (j > 0 && key[j - 1] > kk) { key[j] = key[j - 1]; sign[j] = sign[j - 1]; --j; }
+    key[j] = kk;
+    sign[j] = sg;
+  }
+  return n;
+}
+
+static int patch_of(const orc_region* R, int b) {
+  int p = 0;
+  while (b >= R->bpatch_ptr[p + 1]) ++p;
+  return p;
+}
+
+/* orc_fluxes with threads: face fluxes into buffers, then each cell adds its faces'
+ * terms in the scatter's order; mass in/out summed over boundary faces in order. */
+static void orc_fluxes_gather(const orc_region* R, const double* cv, const double* dv,
+                              const double* grad, double* rhs, orc_stats* st, int stage) {
+  const int F = R->n_faces, B = R->bpatch_ptr[R->n_patches], N = R->n_cells;
+  double* ff = (double*)malloc(5 * ((size_t)F + (size_t)B) * sizeof(double));
+  double* bf = ff + 5 * (size_t)F;
+  ORC_PAR
+  for (int f = 0; f < F; ++f) interior_flux(R, cv, dv, grad, f, ff + 5 * (size_t)f);
+  ORC_PAR
+  for (int b = 0; b < B; ++b) {
+    int p = patch_of(R, b), has_mass;
+    int o2 = R->order == 2 && R->patch_order && R->patch_order[p] == 2;
+    boundary_flux(R, cv, dv, grad, p, b, o2, bf + 5 * (size_t)b, &has_mass);
+  }
+  ORC_PAR
+  for (int c = 0; c < N; ++c) {
+    int key[16], sign[16];
+    int n = cell_faces_sorted(R, c, key, sign);
+    double* r = rhs + 5 * (size_t)c;
+    for (int k = 0; k < 5; ++k) r[k] = -0.0;
+    for (int i = 0; i < n; ++i) {
+      if (sign[i] > 0) {
+        for (int k = 0; k < 5; ++k) r[k] = r[k] + ff[5 * (size_t)key[i] + k];
+      } else if (sign[i] < 0) {
+        for (int k = 0; k < 5; ++k) r[k] = r[k] - ff[5 * (size_t)key[i] + k];
+      } else {
+        int b = key[i] - F;
+        const double* x = bf + 5 * (size_t)b;
+        int k0 = R->patch_bc[patch_of(R, b)] == BC_SLIPWALL ? 1 : 0;
+        for (int k = k0; k < 5; ++k) r[k] = r[k] + x[k];
+      }
+    }
+  }
+  if (st && stage == 1)
+    for (int p = 0; p < R->n_patches; ++p) {
+      if (R->patch_bc[p] == BC_SLIPWALL) continue;
+      for (int b = R->bpatch_ptr[p]; b < R->bpatch_ptr[p + 1]; ++b) {
+        st->mass_in = st->mass_in - dmin(bf[5 * (size_t)b], 0.0);
+        st->mass_out = st->mass_out + dmax(bf[5 * (size_t)b], 0.0);
+      }
+    }
+  free(ff);
+}
+
 /* ConvectiveFluxesMP -> RFLU_ComputeFluxInv: rhs = -diss (= 0 here), then the interior
  * face scatter in ascending face order and the patch scatter
  * (libfloflu/ConvectiveFluxes.F90:133-159, modflu/RFLU_ModHLLCFlux.F90:2207-2425). */
 void orc_fluxes(const orc_region* R, const double* cv, const double* dv, const double* grad,
                 double* rhs, orc_stats* st, int stage) {
+  if (g_threads > 1) {
+    orc_fluxes_gather(R, cv, dv, grad, rhs, st, stage);
+    return;
+  }
   size_t N = (size_t)R->n_cells;
   for (size_t i = 0; i < 5 * N; ++i) rhs[i] = -0.0;
   double flx[5];
@@ -491,6 +591,7 @@ void orc_fluxes(const orc_region* R, const double* cv, const double* dv, const d
 /* RFLU_ComputeGradCellsFast_3D, modflu/RFLU_ModDifferentiationCells.F90:931-1144.
  * var (5,N) holds (rho,u,v,w,p); grad (3,5,N). */
 void orc_gradients(const orc_region* R, const double* var, double* grad) {
+  ORC_PAR
   for (int icg = 0; icg < R->n_cells; ++icg) {
     const double* mo = R->xyz_moms + 10 * (size_t)icg;
     double r11 = mo[0], r12 = mo[1], r22 = mo[2], r13 = mo[3], r23 = mo[4], r33 = mo[5];
@@ -599,6 +700,7 @@ static int float_equal(double a, double b, double tol) {
 void orc_gradients_constr(const orc_region* R, const double* var, double* grad) {
   if (!R->creconst || !R->c2cs_bptr) return;
   const double cwt = R->creconst_weight;
+  ORC_PAR
   for (int icg = 0; icg < R->n_cells; ++icg) {
     int b0 = R->c2cs_bptr[icg], nb = R->c2cs_bptr[icg + 1] - b0;
     if (nb == 0) continue;
@@ -666,6 +768,7 @@ void orc_gradients_constr(const orc_region* R, const double* var, double* grad) 
 void orc_weno_xyz(const orc_region* R, double* grad) {
   size_t N = (size_t)R->n_cells;
   double* eno = (double*)malloc(15 * N * sizeof(double));
+  ORC_PAR
   for (size_t icg = 0; icg < N; ++icg) {
     for (int k = 0; k < 15; ++k) {
       double term = grad[15 * icg + k];
@@ -688,6 +791,7 @@ void orc_weno_xyz(const orc_region* R, double* grad) {
  * a positive variable (rho, p) when its extrapolation to any face centroid is <= 0. */
 void orc_limit_simple(const orc_region* R, const double* var, double* grad) {
   static const int pos_vars[2] = {0, 4};
+  ORC_PAR
   for (int icg = 0; icg < R->n_cells; ++icg) {
     double xc = R->cofg[3 * (size_t)icg], yc = R->cofg[3 * (size_t)icg + 1],
            zc = R->cofg[3 * (size_t)icg + 2];
@@ -711,6 +815,70 @@ void orc_limit_simple(const orc_region* R, const double* var, double* grad) {
   }
 }
 
+/* One cell's side of the Venkatakrishnan limiter at face centroid xf (the loop body of
+ * RFLU_ComputeLimiterVenkat for that side). */
+static void venkat_side(const orc_region* R, int ci, const double* xf, const double* var,
+                        const double* grad, const double* vmin, const double* vmax, double* lim) {
+  const double THRD = 1.0 / 3.0, TINY = 1.0E-12, K = 1.0;
+  double dx = R->cofg[3 * (size_t)ci] - xf[0];
+  double dy = R->cofg[3 * (size_t)ci + 1] - xf[1];
+  double dz = R->cofg[3 * (size_t)ci + 2] - xf[2];
+  double ds = pow(R->vol[ci], THRD);
+  double epsq = (K * ds) * (K * ds) * (K * ds);
+  for (int iv = 0; iv < 5; ++iv) {
+    double vi = var[5 * (size_t)ci + iv];
+    double dmn = vmin[5 * (size_t)ci + iv] - vi;
+    double dmx = vmax[5 * (size_t)ci + iv] - vi;
+    const double* g = grad + 15 * (size_t)ci + 3 * iv;
+    double d2 = g[0] * dx + g[1] * dy + g[2] * dz;
+    double* l = lim + 5 * (size_t)ci + iv;
+    if (d2 > 0.0) {
+      double numer = (dmx * dmx + epsq) * d2 + 2 * d2 * d2 * dmx;
+      double denom = d2 * (dmx * dmx + 2 * d2 * d2 + dmx * d2 + epsq);
+      double term = numer / (denom + TINY);
+      *l = dmin(term, *l);
+    } else if (d2 < 0.0) {
+      double numer = (dmn * dmn + epsq) * d2 + 2 * d2 * d2 * dmn;
+      double denom = d2 * (dmn * dmn + 2 * d2 * d2 + dmn * d2 + epsq);
+      double term = numer / (denom + TINY);
+      *l = dmin(term, *l);
+    }
+  }
+}
+
+/* orc_venkat's face loops with threads: each cell takes min and max over its interior
+ * faces' other cells and the limiter minimum over its own sides. */
+static void venkat_gather(const orc_region* R, const double* var, double* grad, double* vmin,
+                          double* vmax, double* lim) {
+  const int F = R->n_faces;
+  ORC_PAR
+  for (int c = 0; c < R->n_cells; ++c) {
+    for (int k = R->c2f_ptr[c]; k < R->c2f_ptr[c + 1]; ++k) {
+      int e = R->c2f_idx[k];
+      if (e > F || e < -F) continue;
+      int f = (e > 0 ? e : -e) - 1;
+      int o = e > 0 ? R->f2c[2 * (size_t)f + 1] : R->f2c[2 * (size_t)f];
+      for (int iv = 0; iv < 5; ++iv) {
+        double vo = var[5 * (size_t)o + iv];
+        vmin[5 * (size_t)c + iv] = dmin(vmin[5 * (size_t)c + iv], vo);
+        vmax[5 * (size_t)c + iv] = dmax(vmax[5 * (size_t)c + iv], vo);
+      }
+    }
+  }
+  ORC_PAR
+  for (int c = 0; c < R->n_cells; ++c) {
+    for (int k = R->c2f_ptr[c]; k < R->c2f_ptr[c + 1]; ++k) {
+      int e = R->c2f_idx[k];
+      if (e > F || e < -F) continue;
+      int f = (e > 0 ? e : -e) - 1;
+      venkat_side(R, c, R->fc + 3 * (size_t)f, var, grad, vmin, vmax, lim);
+    }
+    for (int iv = 0; iv < 5; ++iv)
+      for (int d = 0; d < 3; ++d)
+        grad[15 * (size_t)c + 3 * iv + d] = lim[5 * (size_t)c + iv] * grad[15 * (size_t)c + 3 * iv + d];
+  }
+}
+
 /* RFLU_ComputeLimiterVenkat + RFLU_LimitGradCells,
  * modflu/RFLU_ModLimiters.F90:407-522,1100-1106.  Note dx1 = cofg(c1) - fc (the
  * reference's sign) and eps^2 = (K*vol^(1/3))^3 with K = 1. */
@@ -721,6 +889,11 @@ void orc_venkat(const orc_region* R, const double* var, double* grad) {
   double* vmax = (double*)malloc(5 * N * sizeof(double));
   double* lim = (double*)malloc(5 * N * sizeof(double));
   for (size_t i = 0; i < 5 * N; ++i) { vmin[i] = var[i]; vmax[i] = var[i]; lim[i] = 1.0; }
+  if (g_threads > 1) {
+    venkat_gather(R, var, grad, vmin, vmax, lim);
+    free(vmin); free(vmax); free(lim);
+    return;
+  }
   for (int f = 0; f < R->n_faces; ++f) {
     int c1 = R->f2c[2 * (size_t)f], c2 = R->f2c[2 * (size_t)f + 1];
     for (int iv = 0; iv < 5; ++iv) {
@@ -775,6 +948,7 @@ void orc_venkat(const orc_region* R, const double* var, double* grad) {
  * (modflu/RFLU_ModConvertCv.F90:560-574), which this restatement keeps. */
 void orc_cell_gradients(const orc_region* R, double* cv, const double* dv, double* grad) {
   size_t N = (size_t)R->n_cells;
+  ORC_PAR
   for (size_t i = 0; i < N; ++i) {
     double ir = 1.0 / cv[5 * i];
     cv[5 * i + 1] = ir * cv[5 * i + 1];
@@ -790,6 +964,7 @@ void orc_cell_gradients(const orc_region* R, double* cv, const double* dv, doubl
   } else if (R->reconst == 11) {
     orc_venkat(R, cv, grad);
   }
+  ORC_PAR
   for (size_t i = 0; i < N; ++i) {
     double r = cv[5 * i];
     cv[5 * i + 1] = r * cv[5 * i + 1];
@@ -814,6 +989,7 @@ void orc_rk_update(const orc_region* R, int stage, double dt, double* cv, const 
   int s = stage - 1;
   if (R->rk == 1) {
     double fac = ark4[s] * dt;
+    ORC_PAR
     for (size_t ic = 0; ic < N; ++ic) {
       double adtv = fac / R->vol[ic];
       for (int iv = 0; iv < 5; ++iv) {
@@ -831,6 +1007,7 @@ void orc_rk_update(const orc_region* R, int stage, double dt, double* cv, const 
     }
   } else if (R->rk == 2) {
     double fac = ark3[s] * dt;
+    ORC_PAR
     for (size_t ic = 0; ic < N; ++ic) {
       double adtv = fac / R->vol[ic];
       for (int iv = 0; iv < 5; ++iv) {
@@ -848,6 +1025,7 @@ void orc_rk_update(const orc_region* R, int stage, double dt, double* cv, const 
     }
   } else if (R->rk == 3) {
     double fac = 1.0 * dt;
+    ORC_PAR
     for (size_t ic = 0; ic < N; ++ic) {
       double adtv = fac / R->vol[ic];
       for (int iv = 0; iv < 5; ++iv) {
@@ -856,6 +1034,7 @@ void orc_rk_update(const orc_region* R, int stage, double dt, double* cv, const 
       }
     }
   } else {
+    ORC_PAR
     for (size_t ic = 0; ic < N; ++ic) {
       double adtv = dt / R->vol[ic];
       for (int iv = 0; iv < 5; ++iv) {
@@ -877,14 +1056,18 @@ int orc_after_update(const orc_region* R, double* cv, double* dv, orc_stats* st)
   size_t N = (size_t)R->n_cells;
   double mm, rgas, gamma;
   gas_prime(R, &mm, &rgas, &gamma);
+  int any_nan = 0;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if(g_threads > 1) reduction(|:any_nan)
   for (size_t i = 0; i < N; ++i) {
     double rho = cv[5 * i], rrho = 1.0 / rho;
     double u = rrho * cv[5 * i + 1], v = rrho * cv[5 * i + 2], w = rrho * cv[5 * i + 3];
     double p = dv[3 * i], t = dv[3 * i + 1];
-    if (isnan(rho) || isnan(u) || isnan(v) || isnan(w) || isnan(p) || isnan(t))
-      return ORC_ERR_INVALID_VALUE;
+    if (isnan(rho) || isnan(u) || isnan(v) || isnan(w) || isnan(p) || isnan(t)) any_nan = 1;
   }
+  if (any_nan) return ORC_ERR_INVALID_VALUE;
   const double pTol = 1.0E-12;
+  int64_t nfix = 0;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if(g_threads > 1) reduction(+:nfix)
   for (size_t i = 0; i < N; ++i) {
     double r = cv[5 * i], ir = 1.0 / r;
     double u = ir * cv[5 * i + 1], v = ir * cv[5 * i + 2], w = ir * cv[5 * i + 3];
@@ -892,11 +1075,12 @@ int orc_after_update(const orc_region* R, double* cv, double* dv, orc_stats* st)
     double Vm2 = u * u + v * v + w * w;
     double p = (gamma - 1.0) * r * (Eo - 0.5 * Vm2);
     if (p <= 0.0 && fabs(p) <= pTol) {
-      if (st) st->n_bounds_fix++;
+      nfix++;
       p = pTol;
       cv[5 * i + 4] = r * (p / (r * (gamma - 1.0)) + 0.5 * (u * u + v * v + w * w));
     }
   }
+  if (st) st->n_bounds_fix += nfix;
   for (size_t i = 0; i < N; ++i) {
     double rho = cv[5 * i], rrho = 1.0 / rho;
     double u = rrho * cv[5 * i + 1], v = rrho * cv[5 * i + 2], w = rrho * cv[5 * i + 3];
@@ -916,7 +1100,55 @@ int orc_after_update(const orc_region* R, double* cv, double* dv, orc_stats* st)
 
 /* RFLU_TimeStepInviscid, rocflu/RFLU_TimeStepInviscid.F90:153-334: spectral-radius
  * sums over interior faces and boundary faces, dt = vol/sum, minimum over cells. */
+static double face_srad(const orc_region* R, const double* cv, const double* dv, int f) {
+  const double fsu = 0.0;
+  int c1 = R->f2c[2 * (size_t)f], c2 = R->f2c[2 * (size_t)f + 1];
+  const double* n = R->fn + 4 * (size_t)f;
+  double ir1 = 1.0 / cv[5 * (size_t)c1], ir2 = 1.0 / cv[5 * (size_t)c2];
+  double u1 = cv[5 * (size_t)c1 + 1] * ir1, v1 = cv[5 * (size_t)c1 + 2] * ir1,
+         w1 = cv[5 * (size_t)c1 + 3] * ir1, a1 = dv[3 * (size_t)c1 + 2];
+  double u2 = cv[5 * (size_t)c2 + 1] * ir2, v2 = cv[5 * (size_t)c2 + 2] * ir2,
+         w2 = cv[5 * (size_t)c2 + 3] * ir2, a2 = dv[3 * (size_t)c2 + 2];
+  double vc = 0.5 * ((u1 + u2) * n[0] + (v1 + v2) * n[1] + (w1 + w2) * n[2]);
+  double as = 0.5 * (a1 + a2);
+  return (fabs(vc - fsu) + as) * n[3];
+}
+
+static double bface_srad(const orc_region* R, const double* cv, const double* dv, int b) {
+  const double fsu = 0.0;
+  int c1 = R->bf2c[b];
+  const double* n = R->bfn + 4 * (size_t)b;
+  double ir1 = 1.0 / cv[5 * (size_t)c1];
+  double u1 = cv[5 * (size_t)c1 + 1] * ir1, v1 = cv[5 * (size_t)c1 + 2] * ir1,
+         w1 = cv[5 * (size_t)c1 + 3] * ir1, a1 = dv[3 * (size_t)c1 + 2];
+  double vc = u1 * n[0] + v1 * n[1] + w1 * n[2];
+  return (fabs(vc - fsu) + a1) * n[3];
+}
+
+/* orc_local_dt with threads: per-face spectral radii, then each cell sums its faces'
+ * in the scatter's order. */
+static double local_dt_gather(const orc_region* R, const double* cv, const double* dv) {
+  const int F = R->n_faces, N = R->n_cells;
+  double* sr = (double*)malloc((size_t)F * sizeof(double));
+  ORC_PAR
+  for (int f = 0; f < F; ++f) sr[f] = face_srad(R, cv, dv, f);
+  double mn = HUGE_VAL;
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(min:mn)
+  for (int c = 0; c < N; ++c) {
+    int key[16], sign[16];
+    int n = cell_faces_sorted(R, c, key, sign);
+    double s = 0.0;
+    for (int i = 0; i < n; ++i)
+      s = s + (key[i] < F ? sr[key[i]] : bface_srad(R, cv, dv, key[i] - F));
+    double d = R->vol[c] / s;
+    if (d < mn) mn = d;
+  }
+  free(sr);
+  return mn;
+}
+
 double orc_local_dt(const orc_region* R, const double* cv, const double* dv) {
+  if (g_threads > 1) return local_dt_gather(R, cv, dv);
   size_t N = (size_t)R->n_cells;
   double* dt = (double*)malloc(N * sizeof(double));
   for (size_t i = 0; i < N; ++i) dt[i] = 0.0;

@@ -84,6 +84,10 @@ int orc_after_update(const orc_region* r, double* cv, double* dv, orc_stats* st)
 double orc_local_dt(const orc_region* r, const double* cv, const double* dv);
 double orc_min_timestep(const orc_region* r, double dt_local_min, double current_time);
 int orc_nstages(int rk);
+/* Threads of the CPU path (OpenMP; 1 = the sequential loops, the default).  Results
+ * are the same bits for any count. */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 /* Full time loop: nsteps of (dt, stages).  cv (5,N) and dv (3,N) are updated in place. */
 int orc_run(const orc_region* r, double* cv, double* dv, int64_t nsteps, orc_stats* st);
 /* Time loop to end_time with RFLU_TimeStepping's end-time clamp (at most max_steps). */
