@@ -591,20 +591,26 @@ int launch_fluxes_p(Region* R) {
   return rc;
 }
 
+// Stage 1's mass in/out over flow-through patches from the boundary mass fluxes
+// stored at flux[F + b].
+void launch_mass_inout(Region* R) {
+  if (!R->has_flowthrough) return;
+  Mesh M = R->mesh();
+  if (!flux32(R))
+    hipLaunchKernelGGL((k_mass_inout<double>), dim3(R->K), dim3(256), 0, R->stream, M, R->cases(),
+                       R->patches(), (const double*)R->flux);
+  else
+    hipLaunchKernelGGL((k_mass_inout<float>), dim3(R->K), dim3(256), 0, R->stream, M, R->cases(),
+                       R->patches(), (const float*)R->flux);
+}
+
 // srad: also store the face spectral radii of the time step (first stage of a fused
-// step, whose face pass sees the step-start state).
+// step, whose face pass sees the step-start state; the fused step counts the mass
+// after its time step, which may end a case).
 int launch_fluxes(Region* R, int stage, bool srad = false) {
   int rc = srad ? launch_fluxes_p<true>(R) : launch_fluxes_p<false>(R);
   if (rc) return rc;
-  if (stage == 1 && R->has_flowthrough) {
-    Mesh M = R->mesh();
-    if (!flux32(R))
-      hipLaunchKernelGGL((k_mass_inout<double>), dim3(R->K), dim3(256), 0, R->stream, M, R->cases(),
-                         R->patches(), (const double*)R->flux);
-    else
-      hipLaunchKernelGGL((k_mass_inout<float>), dim3(R->K), dim3(256), 0, R->stream, M, R->cases(),
-                         R->patches(), (const float*)R->flux);
-  }
+  if (stage == 1 && !srad) launch_mass_inout(R);
   return 0;
 }
 
@@ -663,8 +669,11 @@ int launch_step(Region* R) {
     if ((rc = launch_gradients(R))) return rc;
     if ((rc = launch_fluxes(R, s, s == 1))) return rc;
     if (s == 1) {
-      Timed t(R, KID_DT);
-      if ((rc = launch_dt_reduce(R, -1.0, R->end_time))) return rc;
+      {
+        Timed t(R, KID_DT);
+        if ((rc = launch_dt_reduce(R, -1.0, R->end_time))) return rc;
+      }
+      launch_mass_inout(R);
     }
     if ((rc = launch_update(R, s, true))) return rc;
   }

@@ -497,21 +497,16 @@ __device__ __forceinline__ void bcond_outflow(int bcOpt, double pout, double sxn
 // Boundary faces (rocflu/RFLU_CentralFirstPatch.F90:571-906,
 // rocflu/RFLU_CentralSecondPatch.F90:634-1024).  Slip walls store a zero mass flux.
 // Boundary fluxes stay FP64 in every precision mode (they are few).
-template <int ORDER, typename GT, typename FT, bool SRAD>
-__global__ void k_bface_flux(Mesh M, State S, Gas G, Cases C, Patches P,
-                             const GT* __restrict__ grad, FT* __restrict__ flux,
-                             double* __restrict__ srad, const Scalars* __restrict__ sc) {
-  if (gate(sc)) return;
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= M.B) return;
-  if (SRAD) srad[M.F + b] = face_srad<ORDER>(M, S, M.F + b);
+// Flux of boundary face b (patch fluxes, FP64).
+template <int ORDER, typename GT>
+__device__ __forceinline__ void boundary_face_flux(const Mesh& M, const State& S, const Gas& G,
+                                                   const Patches& P, const GT* grad, int b,
+                                                   double* flx) {
   const int N = M.N;
   int c1 = M.bf2c[b];
-  if (C.cid) G = C.gas[C.cid[c1]];
   int p = M.bpatch[b];
   double4 n = M.bfn[b];
   double nx = n.x, ny = n.y, nz = n.z, nm = n.w;
-  double flx[5];
   int bc = P.bc[p];
   double rl, rul, rvl, rwl, rel, pl;
   bool o2 = ORDER == 2 && P.order2[p];
@@ -578,6 +573,19 @@ __global__ void k_bface_flux(Mesh M, State S, Gas G, Cases C, Patches P,
     flx[3] = 0.5 * (ql * rwl + pl * nz + qr * bs[3] + pr * nz) * nm;
     flx[4] = 0.5 * (ql * (rel + pl) + qr * (bs[4] + pr)) * nm;
   }
+}
+
+template <int ORDER, typename GT, typename FT, bool SRAD>
+__global__ void k_bface_flux(Mesh M, State S, Gas G, Cases C, Patches P,
+                             const GT* __restrict__ grad, FT* __restrict__ flux,
+                             double* __restrict__ srad, const Scalars* __restrict__ sc) {
+  if (gate(sc)) return;
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= M.B) return;
+  if (SRAD) srad[M.F + b] = face_srad<ORDER>(M, S, M.F + b);
+  if (C.cid) G = C.gas[C.cid[M.bf2c[b]]];
+  double flx[5];
+  boundary_face_flux<ORDER, GT>(M, S, G, P, grad, b, flx);
   const size_t FB = (size_t)M.F + M.B;
 #pragma unroll
   for (int k = 0; k < 5; ++k) flux[k * FB + M.F + b] = FT(flx[k]);
@@ -1522,13 +1530,16 @@ __global__ void k_dt_finalize_cases(Cases C) {
 
 // Outflow mass bookkeeping of stage 1 (rocflu/RFLU_CentralFirstPatch.F90:697-700):
 // block c sums case c's outflow faces' mass fluxes in face order per thread stride.
+// The fused step launches it after its time step is set, so a case that has finished
+// (or failed) counts no mass for a step it does not take.
 template <typename FT>
 __global__ void k_mass_inout(Mesh M, Cases C, Patches P, const FT* __restrict__ flux) {
   __shared__ double sin_[256], sout[256];
   const int c = blockIdx.x;
   Scalars* sc = C.sc + c;
   double a = 0.0, b = 0.0;
-  if (!sc->abort) {
+  const bool live = !halted(sc);
+  if (live) {
     for (int k = C.bface0[c] + threadIdx.x; k < C.bface0[c + 1]; k += blockDim.x) {
       int p = M.bpatch[k];
       if (P.bc[p] == BC_SLIPWALL) continue;   // inflow and outflow faces carry mass
@@ -1547,7 +1558,7 @@ __global__ void k_mass_inout(Mesh M, Cases C, Patches P, const FT* __restrict__ 
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0 && !sc->abort) {
+  if (threadIdx.x == 0 && live) {
     sc->mass_in = sc->mass_in + sin_[0];
     sc->mass_out = sc->mass_out + sout[0];
   }

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from rocflu_amd import cases, mesh as rmesh
-from rocflu_amd.solver import GpuRegion, Numerics
+from rocflu_amd.solver import GpuBatch, GpuRegion, Numerics
 from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -277,6 +277,54 @@ def test_inflow_patch(gpu, order, opt, discr):
     assert np.all(err <= TOL_FP64), err
     assert abs(gst["mass_in"] - st["mass_in"]) <= 1e-12 * max(st["mass_in"], 1e-30)
     assert st["mass_in"] > 0.0
+
+
+@pytest.mark.parametrize("batch", [False, True])
+def test_mass_bookkeeping_stops_at_the_end_time(gpu, batch):
+    """Mass in/out count only the steps a run takes: asking the fused loop for more steps
+    than reach MAXTIME leaves them at the oracle's values for the run to MAXTIME
+    (regression: steps after the end once added the last stored boundary fluxes)."""
+    m = rmesh.box(rmesh.TET6, 6, 5, 5, lengths=(6.0, 5.0, 5.0), jitter=0.08, seed=4,
+                  bc_type=(rmesh.BC_INFLOW_VELTEMP, rmesh.BC_OUTFLOW) + (rmesh.BC_SLIPWALL,) * 4,
+                  bc_opt=(1, 1, 0, 0, 0, 0), bc_pout=(0.0, 1.0, 0, 0, 0, 0))
+    rg = Numerics().gas_prime()[1]
+    vals = [[0.4, 0.05, 0.0, 1.1 / rg, 1.2]] + [[0.0] * 5] * 5
+    num = Numerics(discr=3, order=2, reconst=11, rk=4, cfl=0.5, dt_imposed=1e-3, patch_vals=vals)
+    cv0 = cases.blast(m, num, (3.0, 2.5, 2.5), 1.0, p_in=5.0)
+    end = 0.06
+    cvo, _, st = Oracle(m, num).run_until(cv0, end)
+    assert st["rc"] == 0 and st["mass_in"] > 0.0 and st["mass_out"] > 0.0
+    nsteps = 3 * st["steps"] + 2
+    if batch:
+        # a second case with smaller steps keeps the batch stepping after the first one
+        # finishes
+        num2 = Numerics(discr=3, order=2, reconst=11, rk=4, cfl=0.2, dt_imposed=1e-3,
+                        patch_vals=vals)
+        g = GpuBatch([(m, num), (m, num2)])
+        try:
+            g.upload([cv0, cv0])
+            g.set_end_time(end)
+            g.time_steps(nsteps, graph=True)
+            gst = g.case_stats(0)
+            cvg = g.download()[0][0]
+        finally:
+            g.close()
+    else:
+        g = GpuRegion(m, num)
+        try:
+            g.upload(cv0)
+            g.set_end_time(end)
+            g.time_steps(nsteps, graph=True)
+            gst = g.stats()
+            cvg = g.download()[0]
+        finally:
+            g.close()
+    print("oracle", st["steps"], st["mass_in"], st["mass_out"], "gpu", gst["steps"],
+          gst["mass_in"], gst["mass_out"], flush=True)
+    assert gst["finished"] and gst["steps"] == st["steps"]
+    assert np.all(rel_l2(cvg, cvo) <= TOL_FP64)
+    for k in ("mass_in", "mass_out"):
+        assert abs(gst[k] - st[k]) <= 1e-12 * st[k], k
 
 
 @pytest.mark.parametrize("kind,reconst,discr", [(rmesh.HEX, 2, 3), (rmesh.TET6, 11, 3),
