@@ -30,6 +30,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32
 
 KERNELS = ["face_flux", "update", "gradient", "blend", "time_step", "bface_flux"]
 
@@ -71,6 +73,26 @@ def algorithmic_bytes(m, num, kernel: str, tuning: dict) -> float:
     if kernel == "time_step":
         return float(4 * (N + 1) + 4 * K + 8 * (F + B) + 8 * N)
     return 0.0
+
+
+def grad_variant(t: dict) -> str:
+    return "grad_t{}_lds{}_u{}_w{}".format(t.get("grad_tile", 128), t["grad_lds"],
+                                            t["grad_unroll"], t["grad_weights"])
+
+
+def gradient_flops(m, num, tuning: dict) -> float:
+    """Floating-point operations of one gradient launch (sqrt and divide counted as
+    one): per stencil member 31 to form its weights (when not stored) and 35 to
+    accumulate 5 variables; per cell 30 for the R-factor coefficients and applying the
+    limiter; per interior face side 115 for the limiter's min/max and 5 Venkatakrishnan
+    terms."""
+    N, F = int(m.n_cells), int(m.n_faces)
+    S = int(m.c2cs_idx.size)
+    per_member = 35 + (0 if tuning.get("grad_weights") else 31)
+    fl = S * per_member + 30 * N
+    if num.reconst == 11:
+        fl += 2 * F * 115
+    return float(fl)
 
 
 def build_case(args, precision):
@@ -412,7 +434,7 @@ def main():
         with open(tfile) as f:
             tj = json.load(f)
         # measured for the same kernel variant only
-        if dom != "gradient" or tj.get("variant") == "grad_lds{grad_lds}_u{grad_unroll}_w{grad_weights}".format(**r["tuning"]):
+        if dom != "gradient" or tj.get("variant") == grad_variant(r["tuning"]):
             traffic = tj.get("bytes_per_launch")
     out = {
         "metric": "cell-updates/s per RK stage, FP64 vs mixed at 1e-6 rel L2; % of HBM peak",
@@ -440,6 +462,13 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": ab, "ms_per_launch": ms_launch},
     }
+    if dom == "gradient":
+        # the other side of the kernel: its floating-point work against the vector peak
+        fl = gradient_flops(m, r["num"], r["tuning"])
+        peak = FP64_PEAK_TFLOPS if main_prec == 0 else FP32_PEAK_TFLOPS
+        tf = fl / (ms_launch * 1e-3) / 1e12
+        out["roofline"]["compute"] = {"flops_per_launch": fl, "achieved": tf, "peak": peak,
+                                      "unit": "TFLOP/s", "frac": tf / peak}
     if 1 in res and 0 in res:
         a, b = finals[1], finals[0]
         errs = [float(np.linalg.norm(a[:, k] - b[:, k]) / np.linalg.norm(b[:, k])) for k in range(5)]

@@ -232,18 +232,20 @@ int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
                         int32_t grad_block);
 
 /* Tuning options (set by the autotuner, rocflu_amd/autotune.py): whether the gradient
- * kernel stages each 128-cell tile's stencil members in LDS (1) or reads them from
- * global memory (0), how many members it handles together (1, 2 or 4), (read only)
- * the LDS records per tile, over how many XCDs the cell and face kernels spread
- * contiguous block ranges (1 = hardware order; 8 on MI355X), and whether the
- * stencil weights are formed once and streamed from HBM (1; 3 weights per stencil
- * entry, 24 B in FP64, 12 B with FP32 gradients; needs LDS tiles) or re-formed in
- * every gradient pass (0).  None changes results. */
+ * kernel stages each tile's stencil members in LDS (1) or reads them from global
+ * memory (0), how many members it handles together (1, 2 or 4), (read only) the LDS
+ * records per tile, over how many XCDs the cell and face kernels spread contiguous
+ * block ranges (1 = hardware order; 8 on MI355X), whether the stencil weights are
+ * formed once and streamed from HBM (1; 3 weights per stencil entry, 24 B in FP64,
+ * 12 B with FP32 gradients; needs LDS tiles) or re-formed in every gradient pass (0),
+ * and how many cells a tile holds (128, 256 or 512, one thread each; the default 256;
+ * setting it rebuilds the tiles on the host).  None changes results. */
 #define RFLU_OPT_GRAD_LDS 1
 #define RFLU_OPT_GRAD_UNROLL 2
 #define RFLU_OPT_TILE_CAP 3
 #define RFLU_OPT_XCD 4
 #define RFLU_OPT_GRAD_WEIGHTS 5
+#define RFLU_OPT_GRAD_TILE 6
 int rflu_gpu_set_option(int32_t handle, int32_t option, int32_t value);
 int rflu_gpu_get_option(int32_t handle, int32_t option, int32_t* value);
 

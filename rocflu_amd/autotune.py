@@ -51,9 +51,24 @@ def apply(g: GpuRegion, choice: dict):
     g.set_option(GpuRegion.OPT_XCD, choice["xcd"])
     g.set_launch(face_block=choice["face_block"], cell_block=choice["cell_block"])
     if g.num.order == 2:
+        # choices cached before tile sizes were tuned were made with 128-cell tiles
+        _set_tile(g, choice.get("grad_tile", 128))
         g.set_option(GpuRegion.OPT_GRAD_LDS, choice["grad_lds"])
         g.set_option(GpuRegion.OPT_GRAD_UNROLL, choice["grad_unroll"])
         g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, choice["grad_weights"])
+
+
+def _set_tile(g: GpuRegion, ts: int) -> bool:
+    """Rebuilds the gradient tiles with ts cells (False if they do not fit LDS)."""
+    if g.get_option(GpuRegion.OPT_GRAD_TILE) == ts:
+        return True
+    try:
+        g.set_option(GpuRegion.OPT_GRAD_TILE, ts)
+    except nat.RfluError as e:
+        if e.code in (nat.RFLU_ERR_ARG, nat.RFLU_ERR_NOMEM):
+            return False
+        raise
+    return True
 
 
 def _time(g: GpuRegion, which: int, reps: int) -> float:
@@ -61,15 +76,22 @@ def _time(g: GpuRegion, which: int, reps: int) -> float:
     return min(g.time_kernel(which, reps) for _ in range(2))
 
 
+TILE_SIZES = (256, 128, 512)
+
+
 def gradient_candidates(g: GpuRegion, allow_weights: bool = True) -> List[dict]:
+    """Gradient variants, grouped by tile size (a size change rebuilds the tiles) and,
+    within a size, formed weights before stored ones."""
     c = []
     if g.get_option(GpuRegion.OPT_TILE_CAP) > 0:
-        for u in (2, 4):
-            c.append(dict(grad_lds=1, grad_unroll=u, grad_weights=0))
-        if allow_weights:
-            for u in (1, 2):
-                c.append(dict(grad_lds=1, grad_unroll=u, grad_weights=1))
-    c.append(dict(grad_lds=0, grad_unroll=4, grad_weights=0))
+        for ts in TILE_SIZES:
+            for u in (2, 4):
+                c.append(dict(grad_tile=ts, grad_lds=1, grad_unroll=u, grad_weights=0))
+            if allow_weights:
+                for u in (1, 2):
+                    c.append(dict(grad_tile=ts, grad_lds=1, grad_unroll=u, grad_weights=1))
+    c.append(dict(grad_tile=g.get_option(GpuRegion.OPT_GRAD_TILE), grad_lds=0, grad_unroll=4,
+                  grad_weights=0))
     return c
 
 
@@ -84,8 +106,8 @@ def tune(g: GpuRegion, reps: int = 10, allow_weights: bool = True, use_cache: bo
         if sig in _cache:
             apply(g, _cache[sig])
             return dict(_cache[sig], cached=True)
-    choice = dict(xcd=8, face_block=256, cell_block=256, grad_lds=0, grad_unroll=4,
-                  grad_weights=0)
+    choice = dict(xcd=8, face_block=256, cell_block=256, grad_tile=256, grad_lds=0,
+                  grad_unroll=4, grad_weights=0)
     timings = {}
     # XCD-aware block order and block sizes of the face and cell kernels
     best = None
@@ -103,6 +125,10 @@ def tune(g: GpuRegion, reps: int = 10, allow_weights: bool = True, use_cache: bo
     if g.num.order == 2:
         best = None
         for cand in gradient_candidates(g, allow_weights):
+            if g.get_option(GpuRegion.OPT_GRAD_TILE) != cand["grad_tile"]:
+                g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, 0)   # no weights rebuilt in vain
+                if not _set_tile(g, cand["grad_tile"]):
+                    continue
             try:
                 g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, cand["grad_weights"])
             except nat.RfluError as e:
@@ -112,13 +138,14 @@ def tune(g: GpuRegion, reps: int = 10, allow_weights: bool = True, use_cache: bo
             g.set_option(GpuRegion.OPT_GRAD_LDS, cand["grad_lds"])
             g.set_option(GpuRegion.OPT_GRAD_UNROLL, cand["grad_unroll"])
             t = _time(g, K_GRAD, reps)
-            key = "grad_lds{grad_lds}_u{grad_unroll}_w{grad_weights}".format(**cand)
+            key = "grad_t{grad_tile}_lds{grad_lds}_u{grad_unroll}_w{grad_weights}".format(**cand)
             timings[key] = t
             if verbose:
                 print(f"autotune {sig}: {key} {t:.3f} ms", flush=True)
             if best is None or t < best[0]:
                 best = (t, cand)
         choice.update(best[1])
+        g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, 0)
         apply(g, choice)
     _cache[sig] = dict(choice)
     _store_file(sig, choice)

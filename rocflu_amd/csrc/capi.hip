@@ -12,6 +12,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace rflu;
@@ -87,6 +88,7 @@ struct Region {
   size_t tile_loc_len = 0;    // entries of the [member][lane] slot table
   void* gw = nullptr;         // precomputed stencil weights [3][tile_loc_len] (GRAD_WEIGHTS)
   int n_tiles = 0, tile_cap = 0;
+  int grad_tile = 256;        // tuning: cells per LDS tile (128, 256 or 512)
   bool use_lds = false;       // tiles were built (every tile fits in LDS)
   bool grad_lds = true;       // tuning: stage stencils in LDS when tiles exist
   int grad_unroll = 4;        // tuning: stencil members whose weights are formed together
@@ -293,64 +295,100 @@ void renumber(Region* R, std::vector<double4>& hcell, std::vector<int2>& hf2c,
   R->new2old.swap(n2o);
 }
 
-// LDS tiles of the gradient kernel: tile t covers cells [t*TS, t*TS+TS); its union of
-// stencil members (and, with the limiter, face neighbours) is staged in LDS;
-// loc[lptr[t] + s*TS + lane] is the LDS slot of the lane's s-th stencil member (member
-// order unchanged) and vloc[(t*vq + q)*TS + lane] that of its q-th limiter neighbour.
-constexpr int GRAD_TILE = 128;
-constexpr int GRAD_TILE_CAP = 1024;   // records of 8 doubles: 64 KiB of LDS
+// LDS tiles of the gradient kernel: tile t covers cells [t*TS, t*TS+TS) (TS =
+// R->grad_tile, one thread per cell); its union of stencil members (and, with the
+// limiter, face neighbours) is staged in LDS; loc[lptr[t] + s*TS + lane] is the LDS
+// slot of the lane's s-th stencil member (member order unchanged) and
+// vloc[(t*vq + q)*TS + lane] that of its q-th limiter neighbour.  A tile's records
+// (8 doubles each when the weights are formed in the kernel) must fit the 160 KiB of
+// LDS of a gfx950 CU.
+constexpr int GRAD_TILE_CAP = 2560;
 
 void build_tiles(Region* R, const std::vector<int>& sp, const std::vector<int>& si,
                  const std::vector<int>& vnb, bool venkat, std::vector<int>& uptr,
                  std::vector<int>& uidx, std::vector<int>& lptr, std::vector<unsigned short>& loc,
                  std::vector<unsigned short>& vloc) {
-  const int N = R->N, TS = GRAD_TILE, VQ = R->vq;
+  const int N = R->N, TS = R->grad_tile, VQ = R->vq;
   const int nt = (N + TS - 1) / TS;
   auto give_up = [&]() {
     R->use_lds = false;
     uptr.clear(); uidx.clear(); lptr.clear(); loc.clear(); vloc.clear();
   };
+  // tiles are independent: unions and slot tables in parallel, then concatenated
+  std::vector<std::vector<int>> tu(nt);
+  std::vector<std::vector<unsigned short>> tl(nt);
+  vloc.assign(venkat ? (size_t)nt * VQ * TS : 1, (unsigned short)0xFFFF);
+  std::vector<char> too_big(nt, 0);
+  auto work = [&](int t0, int t1) {
+    std::vector<int> u;
+    for (int t = t0; t < t1; ++t) {
+      int c0 = t * TS, c1 = std::min(N, c0 + TS);
+      u.clear();
+      int smax = 0;
+      for (int i = c0; i < c1; ++i) {
+        smax = std::max(smax, sp[i + 1] - sp[i]);
+        for (int e = sp[i]; e < sp[i + 1]; ++e) u.push_back(si[e]);
+        if (venkat)
+          for (int q = 0; q < VQ && vnb[(size_t)q * N + i] >= 0; ++q) u.push_back(vnb[(size_t)q * N + i]);
+      }
+      std::sort(u.begin(), u.end());
+      u.erase(std::unique(u.begin(), u.end()), u.end());
+      if ((int)u.size() > GRAD_TILE_CAP) {   // does not fit: gradient kernel reads global memory
+        too_big[t] = 1;
+        continue;
+      }
+      auto slot = [&](int j) { return (unsigned short)(std::lower_bound(u.begin(), u.end(), j) - u.begin()); };
+      std::vector<unsigned short>& L = tl[t];
+      L.assign((size_t)smax * TS, (unsigned short)0);
+      for (int i = c0; i < c1; ++i) {
+        int lane = i - c0;
+        for (int e = sp[i], s = 0; e < sp[i + 1]; ++e, ++s) L[(size_t)s * TS + lane] = slot(si[e]);
+        if (venkat)
+          for (int q = 0; q < VQ && vnb[(size_t)q * N + i] >= 0; ++q)
+            vloc[((size_t)t * VQ + q) * TS + lane] = slot(vnb[(size_t)q * N + i]);
+      }
+      tu[t] = u;
+    }
+  };
+  int nth = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+  nth = std::max(1, std::min(nth, nt / 64));
+  if (nth <= 1) {
+    work(0, nt);
+  } else {
+    std::vector<std::thread> pool;
+    for (int k = 0; k < nth; ++k)
+      pool.emplace_back(work, (int)((long long)nt * k / nth), (int)((long long)nt * (k + 1) / nth));
+    for (auto& th : pool) th.join();
+  }
+  for (int t = 0; t < nt; ++t)
+    if (too_big[t]) {
+      give_up();
+      return;
+    }
   uptr.assign(nt + 1, 0);
   lptr.assign(nt + 1, 0);
-  uidx.clear();
-  loc.clear();
-  vloc.assign(venkat ? (size_t)nt * VQ * TS : 1, (unsigned short)0xFFFF);
+  size_t nu = 0, nl = 0;
   int cap = 0;
-  std::vector<int> u;
   for (int t = 0; t < nt; ++t) {
-    int c0 = t * TS, c1 = std::min(N, c0 + TS);
-    u.clear();
-    int smax = 0;
-    for (int i = c0; i < c1; ++i) {
-      smax = std::max(smax, sp[i + 1] - sp[i]);
-      for (int e = sp[i]; e < sp[i + 1]; ++e) u.push_back(si[e]);
-      if (venkat)
-        for (int q = 0; q < VQ && vnb[(size_t)q * N + i] >= 0; ++q) u.push_back(vnb[(size_t)q * N + i]);
-    }
-    std::sort(u.begin(), u.end());
-    u.erase(std::unique(u.begin(), u.end()), u.end());
-    if ((int)u.size() > GRAD_TILE_CAP) {   // does not fit: gradient kernel reads global memory
-      give_up();
-      return;
-    }
-    auto slot = [&](int j) { return (unsigned short)(std::lower_bound(u.begin(), u.end(), j) - u.begin()); };
-    cap = std::max(cap, (int)u.size());
-    uidx.insert(uidx.end(), u.begin(), u.end());
+    nu += tu[t].size();
+    nl += tl[t].size();
+    cap = std::max(cap, (int)tu[t].size());
+  }
+  if (nu > (size_t)0x7fffffff || nl > (size_t)0x7fffffff) {
+    give_up();
+    return;
+  }
+  uidx.clear();
+  uidx.reserve(nu);
+  loc.clear();
+  loc.reserve(nl);
+  for (int t = 0; t < nt; ++t) {
+    uidx.insert(uidx.end(), tu[t].begin(), tu[t].end());
+    loc.insert(loc.end(), tl[t].begin(), tl[t].end());
     uptr[t + 1] = (int)uidx.size();
-    size_t base = loc.size();
-    loc.resize(base + (size_t)smax * TS, (unsigned short)0);
-    for (int i = c0; i < c1; ++i) {
-      int lane = i - c0;
-      for (int e = sp[i], s = 0; e < sp[i + 1]; ++e, ++s) loc[base + (size_t)s * TS + lane] = slot(si[e]);
-      if (venkat)
-        for (int q = 0; q < VQ && vnb[(size_t)q * N + i] >= 0; ++q)
-          vloc[((size_t)t * VQ + q) * TS + lane] = slot(vnb[(size_t)q * N + i]);
-    }
-    if (loc.size() > (size_t)0x7fffffff) {
-      give_up();
-      return;
-    }
     lptr[t + 1] = (int)loc.size();
+    std::vector<int>().swap(tu[t]);
+    std::vector<unsigned short>().swap(tl[t]);
   }
   R->n_tiles = nt;
   R->tile_cap = cap;
@@ -437,22 +475,32 @@ Tiles region_tiles(const Region* R) {
   Tiles T;
   T.uptr = R->tile_uptr; T.uidx = R->tile_uidx; T.lptr = R->tile_lptr; T.loc = R->tile_loc;
   T.vloc = R->tile_vloc;
-  T.ts = GRAD_TILE; T.cap = R->tile_cap;
+  T.ts = R->grad_tile; T.cap = R->tile_cap;
   return T;
+}
+
+template <typename GT, bool V, int U, int TS>
+void launch_gradient_tiles(Region* R, const Mesh& M, const State& S) {
+  Tiles T = region_tiles(R);
+  if (R->gw) {
+    size_t shm = (size_t)5 * R->tile_cap * sizeof(double);
+    hipLaunchKernelGGL((k_gradient_ldsw<GT, V, 4 * U, TS>), dim3(R->n_tiles), dim3(TS), shm,
+                       R->stream, M, S, T, R->tile_loc_len, (const GT*)R->gw, (GT*)R->grad, R->gate_sc());
+  } else {
+    size_t shm = (size_t)8 * R->tile_cap * sizeof(double);
+    hipLaunchKernelGGL((k_gradient_lds<GT, V, U, TS>), dim3(R->n_tiles), dim3(TS), shm,
+                       R->stream, M, S, T, (GT*)R->grad, R->gate_sc());
+  }
 }
 
 template <typename GT, bool V, int U>
 void launch_gradient_kernel(Region* R, const Mesh& M, const State& S) {
-  if (R->use_lds && R->grad_lds && R->gw) {
-    Tiles T = region_tiles(R);
-    size_t shm = (size_t)5 * R->tile_cap * sizeof(double);
-    hipLaunchKernelGGL((k_gradient_ldsw<GT, V, 4 * U>), dim3(R->n_tiles), dim3(GRAD_TILE), shm,
-                       R->stream, M, S, T, R->tile_loc_len, (const GT*)R->gw, (GT*)R->grad, R->gate_sc());
-  } else if (R->use_lds && R->grad_lds) {
-    Tiles T = region_tiles(R);
-    size_t shm = (size_t)8 * R->tile_cap * sizeof(double);
-    hipLaunchKernelGGL((k_gradient_lds<GT, V, U>), dim3(R->n_tiles), dim3(GRAD_TILE), shm,
-                       R->stream, M, S, T, (GT*)R->grad, R->gate_sc());
+  if (R->use_lds && R->grad_lds) {
+    switch (R->grad_tile) {
+      case 128: launch_gradient_tiles<GT, V, U, 128>(R, M, S); break;
+      case 512: launch_gradient_tiles<GT, V, U, 512>(R, M, S); break;
+      default: launch_gradient_tiles<GT, V, U, 256>(R, M, S); break;
+    }
   } else {
     hipLaunchKernelGGL((k_gradient<GT, V, U>), dim3(nblk(R->N, R->grad_block)), dim3(R->grad_block),
                        0, R->stream, M, S, (GT*)R->grad, R->gate_sc());
@@ -1555,6 +1603,92 @@ int rflu_gpu_time_kernel(int32_t h, int32_t which, int32_t reps, double* ms) {
   return RFLU_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Stencil weights of the tiles' [member][lane] layout, formed on the device.
+int form_weights(Region* R) {
+  const size_t gsz = grad32(R) ? sizeof(float) : sizeof(double);
+  const size_t L = R->tile_loc_len;
+  hipError_t e = hipMalloc(&R->gw, 3 * L * gsz + 16);
+  if (e != hipSuccess) {
+    R->gw = nullptr;
+    return fail(RFLU_ERR_NOMEM, std::string("hipMalloc stencil weights: ") + hipGetErrorString(e));
+  }
+  Mesh M = R->mesh();
+  Tiles T = region_tiles(R);
+  if (grad32(R))
+    hipLaunchKernelGGL((k_grad_weights<float>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream,
+                       M, T, L, (float*)R->gw);
+  else
+    hipLaunchKernelGGL((k_grad_weights<double>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream,
+                       M, T, L, (double*)R->gw);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  return 0;
+}
+
+// Tiles of another size: the stencils and limiter neighbours come back from the
+// device, the tables are rebuilt on the host and replace the old ones (and the stored
+// weights, whose layout follows the tiles).  Tiles that would not fit LDS leave the
+// region as it was.
+int rebuild_tiles(Region* R, int ts) {
+  const int N = R->N;
+  const bool venkat = R->reconst == RFLU_RECONST_LIM_VENKAT;
+  std::vector<int> sp(N + 1), si, vnb;
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  HIP_TRY(hipMemcpy(sp.data(), R->sten_ptr, (N + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  si.resize(sp[N]);
+  if (!si.empty())
+    HIP_TRY(hipMemcpy(si.data(), R->sten_idx, si.size() * sizeof(int), hipMemcpyDeviceToHost));
+  if (venkat) {
+    vnb.resize((size_t)R->vq * N);
+    HIP_TRY(hipMemcpy(vnb.data(), R->vnb, vnb.size() * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  const int old_ts = R->grad_tile, old_nt = R->n_tiles, old_cap = R->tile_cap;
+  const bool old_lds = R->use_lds;
+  std::vector<int> uptr, uidx, lptr;
+  std::vector<unsigned short> loc, vloc;
+  R->grad_tile = ts;
+  build_tiles(R, sp, si, vnb, venkat, uptr, uidx, lptr, loc, vloc);
+  if (!R->use_lds) {
+    R->grad_tile = old_ts; R->n_tiles = old_nt; R->tile_cap = old_cap; R->use_lds = old_lds;
+    return fail(RFLU_ERR_ARG, "stencil tiles of that size do not fit LDS");
+  }
+  int *d_uptr = nullptr, *d_uidx = nullptr, *d_lptr = nullptr;
+  unsigned short *d_loc = nullptr, *d_vloc = nullptr;
+  int rc;
+  if ((rc = dalloc(&d_uptr, uptr.size())) || (rc = dalloc(&d_uidx, uidx.size())) ||
+      (rc = dalloc(&d_lptr, lptr.size())) || (rc = dalloc(&d_loc, loc.size())) ||
+      (rc = dalloc(&d_vloc, vloc.size())) || (rc = dput(d_uptr, uptr, R->stream)) ||
+      (rc = dput(d_uidx, uidx, R->stream)) || (rc = dput(d_lptr, lptr, R->stream)) ||
+      (rc = dput(d_loc, loc, R->stream)) || (rc = dput(d_vloc, vloc, R->stream)) ||
+      hipStreamSynchronize(R->stream) != hipSuccess) {
+    void* bufs[] = {d_uptr, d_uidx, d_lptr, d_loc, d_vloc};
+    for (void* b : bufs)
+      if (b) (void)hipFree(b);
+    R->grad_tile = old_ts; R->n_tiles = old_nt; R->tile_cap = old_cap; R->use_lds = old_lds;
+    return rc ? rc : fail(RFLU_ERR_HIP, "tile upload failed");
+  }
+  void* old[] = {R->tile_uptr, R->tile_uidx, R->tile_lptr, R->tile_loc, R->tile_vloc};
+  for (void* b : old)
+    if (b) (void)hipFree(b);
+  R->tile_uptr = d_uptr; R->tile_uidx = d_uidx; R->tile_lptr = d_lptr;
+  R->tile_loc = d_loc; R->tile_vloc = d_vloc;
+  R->tile_loc_len = loc.size();
+  if (R->gw) {
+    HIP_TRY(hipFree(R->gw));
+    R->gw = nullptr;
+    if ((rc = form_weights(R))) return rc;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
@@ -1584,23 +1718,17 @@ int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
       if (R->gw) break;
       if (R->order != 2 || !R->use_lds)
         return fail(RFLU_ERR_ARG, "GRAD_WEIGHTS needs a second-order region with LDS tiles");
-      const size_t gsz = grad32(R) ? sizeof(float) : sizeof(double);
-      const size_t L = R->tile_loc_len;
-      hipError_t e = hipMalloc(&R->gw, 3 * L * gsz + 16);
-      if (e != hipSuccess) {
-        R->gw = nullptr;
-        return fail(RFLU_ERR_NOMEM, std::string("hipMalloc stencil weights: ") + hipGetErrorString(e));
-      }
-      Mesh M = R->mesh();
-      Tiles T = region_tiles(R);
-      if (grad32(R))
-        hipLaunchKernelGGL((k_grad_weights<float>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream,
-                           M, T, L, (float*)R->gw);
-      else
-        hipLaunchKernelGGL((k_grad_weights<double>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream,
-                           M, T, L, (double*)R->gw);
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(hipStreamSynchronize(R->stream));
+      int rc = form_weights(R);
+      if (rc) return rc;
+      break;
+    }
+    case RFLU_OPT_GRAD_TILE: {
+      if (value != 128 && value != 256 && value != 512)
+        return fail(RFLU_ERR_ARG, "GRAD_TILE is 128, 256 or 512");
+      if (R->order != 2) return fail(RFLU_ERR_ARG, "GRAD_TILE needs a second-order region");
+      if (value == R->grad_tile) break;
+      int rc = rebuild_tiles(R, value);
+      if (rc) return rc;
       break;
     }
     default:
@@ -1620,6 +1748,7 @@ int rflu_gpu_get_option(int32_t h, int32_t option, int32_t* value) {
     case RFLU_OPT_TILE_CAP: *value = R->tile_cap; break;
     case RFLU_OPT_XCD: *value = R->nxcd; break;
     case RFLU_OPT_GRAD_WEIGHTS: *value = R->gw ? 1 : 0; break;
+    case RFLU_OPT_GRAD_TILE: *value = R->grad_tile; break;
     default: return fail(RFLU_ERR_ARG, "unknown option");
   }
   return RFLU_OK;

@@ -1013,8 +1013,8 @@ __global__ void k_gradient(Mesh M, State S, GT* __restrict__ grad, const Scalars
   for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
 }
 
-// Stencil tiles: the union of the stencil members and face neighbours of TS
-// consecutive cells is staged in LDS once per tile (centroid and primitive state, 8
+// Stencil tiles: the union of the stencil members and face neighbours of TS (128, 256
+// or 512) consecutive cells is staged in LDS once per tile (centroid and primitive state, 8
 // doubles per member), and each lane then walks its own stencil, in the reference's
 // member order, reading LDS slots through a coalesced [member][lane] table of 16-bit
 // slot numbers; vloc[(q*TS) + lane] (per tile) holds the slot of the lane's q-th
@@ -1068,8 +1068,8 @@ __device__ __forceinline__ void stage_tile(const Mesh& M, const State& S, const 
   }
 }
 
-template <typename GT, bool VENKAT, int U>
-__global__ __launch_bounds__(128) void k_gradient_lds(Mesh M, State S, Tiles T,
+template <typename GT, bool VENKAT, int U, int TS>
+__global__ __launch_bounds__(TS) void k_gradient_lds(Mesh M, State S, Tiles T,
                                                       GT* __restrict__ grad,
                                                       const Scalars* __restrict__ sc) {
   extern __shared__ double lds[];
@@ -1176,10 +1176,11 @@ __global__ void k_grad_weights(Mesh M, Tiles T, size_t L, GT* __restrict__ w) {
 
 // Gradient with precomputed weights: only the members' primitive state is staged
 // in LDS (5 doubles per record), weights stream from HBM in [member][lane] order
-// through a ring of P members loaded ahead of their use, and the accumulation runs
-// in stencil order as in the member loop above.
-template <typename GT, bool VENKAT, int P>
-__global__ __launch_bounds__(128) void k_gradient_ldsw(Mesh M, State S, Tiles T, size_t L,
+// through a ring of P members loaded ahead of their use (nontemporal loads: read once
+// per pass, they should not evict the state the tiles gather), and the accumulation
+// runs in stencil order as in the member loop above.
+template <typename GT, bool VENKAT, int P, int TS>
+__global__ __launch_bounds__(TS) void k_gradient_ldsw(Mesh M, State S, Tiles T, size_t L,
                                                        const GT* __restrict__ w,
                                                        GT* __restrict__ grad,
                                                        const Scalars* __restrict__ sc) {
@@ -1205,9 +1206,9 @@ __global__ __launch_bounds__(128) void k_gradient_ldsw(Mesh M, State S, Tiles T,
     bool ok = sn < cnt;
     size_t o = (size_t)(ok ? sn : 0) * T.ts;
     rk[m] = ok ? (int)Lp[o] : 0;
-    rw[m][0] = ok ? W[o] : GT(0);
-    rw[m][1] = ok ? W[L + o] : GT(0);
-    rw[m][2] = ok ? W[2 * L + o] : GT(0);
+    rw[m][0] = ok ? __builtin_nontemporal_load(W + o) : GT(0);
+    rw[m][1] = ok ? __builtin_nontemporal_load(W + L + o) : GT(0);
+    rw[m][2] = ok ? __builtin_nontemporal_load(W + 2 * L + o) : GT(0);
   };
 #pragma unroll
   for (int m = 0; m < P; ++m) fetch(m, m);

@@ -229,7 +229,12 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
                 dict(lds=1, unroll=1, xcd=1), dict(lds=1, unroll=2, xcd=3),
                 dict(lds=0, unroll=2, xcd=1, blocks=(64, 512, 128)),
                 dict(lds=1, unroll=4, xcd=8, weights=1), dict(lds=1, unroll=1, xcd=1, weights=1),
-                dict(lds=1, unroll=2, xcd=8, weights=1)]
+                dict(lds=1, unroll=2, xcd=8, weights=1),
+                dict(lds=1, unroll=4, xcd=8, tile=128), dict(lds=1, unroll=2, xcd=8, tile=512),
+                dict(lds=1, unroll=1, xcd=8, tile=128, weights=1),
+                dict(lds=1, unroll=2, xcd=1, tile=512, weights=1),
+                # weights formed for 256-cell tiles, then the tiles rebuilt at 128
+                dict(lds=1, unroll=2, xcd=8, weights=1, tile_after=128)]
     for v in variants:
         print("variant", v, flush=True)
         g = GpuRegion(m, num)
@@ -239,8 +244,14 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
             g.set_option(GpuRegion.OPT_XCD, v["xcd"])
             if "blocks" in v:
                 g.set_launch(*v["blocks"])
+            if "tile" in v:
+                g.set_option(GpuRegion.OPT_GRAD_TILE, v["tile"])
             if v.get("weights"):
                 g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, 1)
+                assert g.get_option(GpuRegion.OPT_GRAD_WEIGHTS) == 1
+            if "tile_after" in v:
+                g.set_option(GpuRegion.OPT_GRAD_TILE, v["tile_after"])
+                assert g.get_option(GpuRegion.OPT_GRAD_TILE) == v["tile_after"]
                 assert g.get_option(GpuRegion.OPT_GRAD_WEIGHTS) == 1
             g.upload(cv0)
             g.time_steps(8, graph=True)
@@ -249,6 +260,13 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
             g.close()
         if ref is None:
             ref = cvg
+            g2 = GpuRegion(m, num)
+            try:
+                with pytest.raises(Exception):
+                    g2.set_option(GpuRegion.OPT_GRAD_TILE, 100)
+                assert g2.get_option(GpuRegion.OPT_GRAD_TILE) == 256
+            finally:
+                g2.close()
             err = rel_l2(cvg, cvo)
             print("rel L2 vs oracle:", err, flush=True)
             assert np.all(err <= TOL_FP64), err
