@@ -204,9 +204,10 @@ int rflu_gpu_synchronize(int32_t handle);
  * region's internal cell order, which rflu_gpu_cell_order maps to the caller's cells
  * (new2old[i] = caller cell of internal cell i).  Device pointers are plain device
  * addresses on the region's GPU; every call waits for its work.  Tangent and adjoint
- * are of the first-order FP64 residual R(cv) of a single region (rflu_gpu_residual's
- * R), with dv = dv(cv) and, for inflow patches, the patch values (u, v, w, T, p)
- * as inputs.
+ * are of the FP64 residual R(cv) of a single region (rflu_gpu_residual's R), first
+ * order, or second order with no gradient modifier or the Venkatakrishnan limiter
+ * (not WENO-XYZ, not constrained gradients), with dv = dv(cv) and, for inflow patches,
+ * the patch values (u, v, w, T, p) as inputs.
  */
 int rflu_gpu_cell_order(int32_t handle, int32_t* new2old);
 /* vol in the internal cell order (host array of n_cells). */

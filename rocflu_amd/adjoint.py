@@ -1,7 +1,8 @@
 """Discrete adjoint of the explicit path, for verification.
 
-The residual R(cv, vals) of a region (first order, FP64; vals are the patches'
-(u, v, w, T, p), read by inflow patches) is a ``torch.autograd.Function`` whose
+The residual R(cv, vals) of a region (FP64, first order or second order with no
+gradient modifier or the Venkatakrishnan limiter; vals are the patches' (u, v, w, T, p),
+read by inflow patches) is a ``torch.autograd.Function`` whose
 backward is the library's hand-written adjoint kernel (``rflu_gpu_residual_vjp``) and
 whose forward-mode rule is the tangent kernel (``rflu_gpu_residual_jvp``).  SSP-RK3
 steps at given time steps are composed from it in torch with the same arithmetic as
@@ -26,7 +27,7 @@ from .solver import GpuRegion
 
 
 class DiffRegion:
-    """A first-order FP64 region seen as a differentiable map of device tensors."""
+    """An FP64 region seen as a differentiable map of device tensors."""
 
     def __init__(self, region: GpuRegion):
         self.g = region

@@ -63,6 +63,11 @@ struct Region {
   // derivative work buffers (allocated by the first tangent/adjoint call)
   double *adj_g = nullptr, *adj_gb = nullptr, *adj_gv = nullptr, *adj_pv = nullptr;
   double *adj_dflux = nullptr, *adj_dvals = nullptr;
+  // second order: primitive-state tangent/adjoint [5][N], gradient values/tangents and
+  // adjoints [15][N], limiter selections [5][N], the transposed stencil (CSR)
+  double *adj_p = nullptr, *adj_ga = nullptr, *adj_gb2 = nullptr, *adj_gc = nullptr, *adj_gr = nullptr;
+  double *adj_selv = nullptr;
+  int *adj_seln = nullptr, *adj_tptr = nullptr, *adj_tidx = nullptr;
   hipStream_t stream = nullptr;
   // device buffers
   int2* f2c = nullptr;
@@ -181,7 +186,9 @@ void free_region(Region* R) {
                   R->vnb, R->vgeo, R->srad, R->gw, R->cid, R->gas_tab, R->par, R->cell0,
                   R->bface0, R->dtmin, R->pbc, R->popt, R->porder2, R->ppout, R->pvals,
                   R->bptr, R->adj_g, R->adj_gb, R->adj_gv, R->adj_pv, R->adj_dflux,
-                  R->adj_dvals, R->con_ecell, R->con_evar, R->con_rptr, R->con_rcell,
+                  R->adj_dvals, R->adj_p, R->adj_ga, R->adj_gb2, R->adj_gc, R->adj_gr, R->adj_selv,
+                  R->adj_seln, R->adj_tptr, R->adj_tidx,
+                  R->con_ecell, R->con_evar, R->con_rptr, R->con_rcell,
                   R->con_rval, R->con_rw};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -1845,14 +1852,43 @@ int rflu_gpu_residual_dev(int32_t h, double* rhs) {
 namespace {
 int derivative_ready(Region* R) {
   if (R->K != 1) return fail(RFLU_ERR_ARG, "derivatives are for a single region");
-  if (R->order != 1) return fail(RFLU_ERR_ARG, "derivatives of the residual are first order");
   if (R->prec != RFLU_PREC_FP64) return fail(RFLU_ERR_ARG, "derivatives are FP64");
+  if (R->order == 2 && R->reconst != RFLU_RECONST_NONE && R->reconst != RFLU_RECONST_LIM_VENKAT)
+    return fail(RFLU_ERR_ARG, "second-order derivatives cover no modifier or the Venkatakrishnan limiter");
+  if (R->order == 2 && R->con_n > 0)
+    return fail(RFLU_ERR_ARG, "second-order derivatives do not cover constrained gradients");
   int rc;
-  const size_t F = R->F, B = R->B, P = R->P;
-  if (!R->adj_g && ((rc = dalloc(&R->adj_g, 10 * F)) || (rc = dalloc(&R->adj_gb, 5 * B)) ||
+  const size_t F = R->F, B = R->B, P = R->P, N = R->N;
+  const bool o2 = R->order == 2;
+  if (!R->adj_g && ((rc = dalloc(&R->adj_g, (o2 ? 20 : 10) * F)) ||
+                    (rc = dalloc(&R->adj_gb, (o2 ? 10 : 5) * B)) ||
                     (rc = dalloc(&R->adj_gv, 5 * B)) || (rc = dalloc(&R->adj_pv, 5 * P)) ||
                     (rc = dalloc(&R->adj_dflux, 5 * (F + B))) || (rc = dalloc(&R->adj_dvals, 5 * P))))
     return rc;
+  if (o2 && !R->adj_p) {
+    if ((rc = dalloc(&R->adj_p, 5 * N)) || (rc = dalloc(&R->adj_ga, 15 * N)) ||
+        (rc = dalloc(&R->adj_gb2, 15 * N)) || (rc = dalloc(&R->adj_gc, 15 * N)) ||
+        (rc = dalloc(&R->adj_gr, 15 * N)) ||
+        (rc = dalloc(&R->adj_selv, 5 * N)) || (rc = dalloc(&R->adj_seln, 5 * N)))
+      return rc;
+    // transposed stencil: for every cell j, the cells whose stencil holds j, ascending
+    std::vector<int> sp(N + 1), si;
+    HIP_TRY(hipStreamSynchronize(R->stream));
+    HIP_TRY(hipMemcpy(sp.data(), R->sten_ptr, (N + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    si.resize(sp[N]);
+    if (!si.empty())
+      HIP_TRY(hipMemcpy(si.data(), R->sten_idx, si.size() * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> tptr(N + 1, 0), tidx(si.size());
+    for (int j : si) tptr[j + 1]++;
+    for (size_t j = 0; j < N; ++j) tptr[j + 1] += tptr[j];
+    std::vector<int> fill(tptr.begin(), tptr.end() - 1);
+    for (size_t i = 0; i < N; ++i)
+      for (int s = sp[i]; s < sp[i + 1]; ++s) tidx[fill[si[s]]++] = (int)i;
+    if ((rc = dalloc(&R->adj_tptr, tptr.size())) || (rc = dalloc(&R->adj_tidx, tidx.size())) ||
+        (rc = dput(R->adj_tptr, tptr, R->stream)) || (rc = dput(R->adj_tidx, tidx, R->stream)))
+      return rc;
+    HIP_TRY(hipStreamSynchronize(R->stream));
+  }
   return 0;
 }
 }  // namespace
@@ -1867,6 +1903,30 @@ int rflu_gpu_residual_jvp(int32_t h, const double* dcv, const double* dvals, dou
     HIP_TRY(hipMemcpyAsync(R->adj_dvals, dvals, 5 * (size_t)R->P * sizeof(double), hipMemcpyHostToDevice, R->stream));
   Mesh M = R->mesh();
   State S = R->state();
+  if (R->order == 2) {
+    // dprim, then the limited gradient's values and tangents, then the faces
+    const dim3 gc(nblk(R->N, 256)), bs(256);
+    hipLaunchKernelGGL(k_prim_jvp, gc, bs, 0, R->stream, M, S, R->gas, dcv, R->adj_p);
+    if (R->reconst == RFLU_RECONST_LIM_VENKAT)
+      hipLaunchKernelGGL(k_grad2_jvp<true>, gc, bs, 0, R->stream, M, S, (const double*)R->adj_p, R->adj_ga, R->adj_gb2);
+    else
+      hipLaunchKernelGGL(k_grad2_jvp<false>, gc, bs, 0, R->stream, M, S, (const double*)R->adj_p, R->adj_ga, R->adj_gb2);
+    if (R->F > 0) {
+      dim3 g(nblk(R->F, 256));
+      const double *dp = R->adj_p, *gv = R->adj_ga, *gd = R->adj_gb2;
+      if (R->discr == 1) hipLaunchKernelGGL(k_face2_jvp<1>, g, bs, 0, R->stream, M, S, R->gas, dp, gv, gd, R->adj_dflux);
+      else if (R->discr == 3) hipLaunchKernelGGL(k_face2_jvp<3>, g, bs, 0, R->stream, M, S, R->gas, dp, gv, gd, R->adj_dflux);
+      else hipLaunchKernelGGL(k_face2_jvp<4>, g, bs, 0, R->stream, M, S, R->gas, dp, gv, gd, R->adj_dflux);
+    }
+    if (R->B > 0)
+      hipLaunchKernelGGL(k_bface2_jvp, dim3(nblk(R->B, 256)), bs, 0, R->stream, M, S, R->gas, R->patches(),
+                         (const double*)R->adj_p, (const double*)R->adj_ga, (const double*)R->adj_gb2,
+                         (const double*)(dvals ? R->adj_dvals : nullptr), R->adj_dflux);
+    hipLaunchKernelGGL((k_gather_rhs<double>), gc, bs, 0, R->stream, M, (const double*)R->adj_dflux, drhs);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(R->stream));
+    return RFLU_OK;
+  }
   if (R->F > 0) {
     dim3 g(nblk(R->F, 256)), b(256);
     if (R->discr == 1) hipLaunchKernelGGL(k_face_jvp<1>, g, b, 0, R->stream, M, S, R->gas, dcv, R->adj_dflux);
@@ -1891,6 +1951,44 @@ int rflu_gpu_residual_vjp(int32_t h, const double* mu, double* lam, double* dval
   if (rc) return rc;
   Mesh M = R->mesh();
   State S = R->state();
+  if (R->order == 2) {
+    const dim3 gc(nblk(R->N, 256)), bs(256);
+    const bool venkat = R->reconst == RFLU_RECONST_LIM_VENKAT;
+    double *graw = R->adj_gb2, *glim = R->adj_ga, *aG = R->adj_gc, *aR = R->adj_gr;
+    // forward sweep: raw and limited gradients
+    if (venkat) hipLaunchKernelGGL(k_grad2_fwd<true>, gc, bs, 0, R->stream, M, S, graw, glim);
+    else hipLaunchKernelGGL(k_grad2_fwd<false>, gc, bs, 0, R->stream, M, S, graw, glim);
+    if (R->F > 0) {
+      dim3 g(nblk(R->F, 256));
+      if (R->discr == 1) hipLaunchKernelGGL(k_face2_vjp<1>, g, bs, 0, R->stream, M, S, R->gas, (const double*)glim, mu, R->adj_g);
+      else if (R->discr == 3) hipLaunchKernelGGL(k_face2_vjp<3>, g, bs, 0, R->stream, M, S, R->gas, (const double*)glim, mu, R->adj_g);
+      else hipLaunchKernelGGL(k_face2_vjp<4>, g, bs, 0, R->stream, M, S, R->gas, (const double*)glim, mu, R->adj_g);
+    }
+    if (R->B > 0)
+      hipLaunchKernelGGL(k_bface2_vjp, dim3(nblk(R->B, 256)), bs, 0, R->stream, M, S, R->gas, R->patches(),
+                         (const double*)glim, mu, R->adj_gb, R->adj_gv);
+    hipLaunchKernelGGL(k_cell2_vjp, gc, bs, 0, R->stream, M, (const double*)R->adj_g, (const double*)R->adj_gb,
+                       R->adj_p, aG);
+    if (venkat) {
+      hipLaunchKernelGGL(k_venkat_vjp, gc, bs, 0, R->stream, M, S, (const double*)graw, (const double*)aG,
+                         aR, R->adj_p, R->adj_seln, R->adj_selv);
+      hipLaunchKernelGGL(k_venkat_nb_gather, gc, bs, 0, R->stream, M, (const int*)R->adj_seln,
+                         (const double*)R->adj_selv, R->adj_p);
+    } else {
+      aR = aG;
+    }
+    hipLaunchKernelGGL(k_grad_vjp, gc, bs, 0, R->stream, M, (const int*)R->adj_tptr, (const int*)R->adj_tidx,
+                       (const double*)aR, R->adj_p);
+    hipLaunchKernelGGL(k_prim_vjp, gc, bs, 0, R->stream, M, S, R->gas, (const double*)R->adj_p, lam);
+    if (dvals && R->P > 0) {
+      hipLaunchKernelGGL(k_patch_vals_sum, dim3(R->P), dim3(256), 0, R->stream, M, (const int*)R->bptr,
+                         (const double*)R->adj_gv, R->adj_pv);
+      HIP_TRY(hipMemcpyAsync(dvals, R->adj_pv, 5 * (size_t)R->P * sizeof(double), hipMemcpyDeviceToHost, R->stream));
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(R->stream));
+    return RFLU_OK;
+  }
   if (R->F > 0) {
     dim3 g(nblk(R->F, 256)), b(256);
     if (R->discr == 1) hipLaunchKernelGGL(k_face_vjp<1>, g, b, 0, R->stream, M, S, R->gas, mu, R->adj_g);

@@ -16,14 +16,15 @@ from rocflu_amd import adjoint as adj
 pytestmark = pytest.mark.gpu
 
 
-def blast_box(discr=3, n=(6, 5, 5), subsonic_in=True):
+def blast_box(discr=3, n=(6, 5, 5), subsonic_in=True, order=1, reconst=0, patch_order=None):
     m = rmesh.box(rmesh.TET6, *n, lengths=tuple(float(x) for x in n), jitter=0.08, seed=4,
                   bc_type=(rmesh.BC_INFLOW_VELTEMP, rmesh.BC_OUTFLOW) + (rmesh.BC_SLIPWALL,) * 4,
                   bc_opt=(1 if subsonic_in else 0, 1, 0, 0, 0, 0), bc_pout=(0.0, 1.0, 0, 0, 0, 0),
-                  stencils=False)
+                  stencils=order == 2)
     rg = Numerics().gas_prime()[1]
     vals = [[0.4, 0.05, -0.02, 1.1 / rg, 1.2]] + [[0.0] * 5] * 5
-    num = Numerics(discr=discr, order=1, rk=4, cfl=0.5, dt_imposed=1e-3, patch_vals=vals)
+    num = Numerics(discr=discr, order=order, reconst=reconst if order == 2 else 0, rk=4, cfl=0.5,
+                   dt_imposed=1e-3, patch_vals=vals, patch_order=patch_order)
     centre = tuple(0.5 * x for x in n)
     cv0 = cases.blast(m, num, centre, 0.2 * n[0], p_in=10.0)
     # a slow stream with seeded noise on every variable: exact ties and zero speeds
@@ -43,9 +44,17 @@ def rand_like(t, seed):
     return torch.randn(t.shape, generator=g, dtype=torch.float64).to(t.device)
 
 
-@pytest.mark.parametrize("discr", [1, 3, 4])
-def test_residual_tangent_and_adjoint(gpu, discr):
-    m, num, cv0 = blast_box(discr)
+@pytest.mark.parametrize("discr,order,reconst,porder", [
+    (1, 1, 0, None), (3, 1, 0, None), (4, 1, 0, None),
+    (3, 2, 11, [2, 2, 1, 1, 1, 1]), (1, 2, 11, [2] * 6), (4, 2, 11, [1] * 6), (3, 2, 0, [2] * 6)])
+def test_residual_tangent_and_adjoint(gpu, discr, order, reconst, porder):
+    """Order 2: WLSQ gradient, optional Venkatakrishnan limiter, first- and second-order
+    patch fluxes (a first-order patch reads the primitive copy with p in the energy
+    slot).  With the limiter the residual is only piecewise smooth (its min/max picks
+    and the flux functions' branch switches move with the reconstructed states), so the
+    central difference is compared on the cells where two step sizes agree, i.e. where
+    no kink lies within the step; the dot-product test holds everywhere."""
+    m, num, cv0 = blast_box(discr, order=order, reconst=reconst, patch_order=porder)
     g = GpuRegion(m, num)
     try:
         dr = adj.DiffRegion(g)
@@ -58,13 +67,25 @@ def test_residual_tangent_and_adjoint(gpu, discr):
         lam, lvals = dr.vjp_raw(u, vals, mu)
         lhs = float((mu * jv).sum())
         rhs = float((lam * du).sum()) + float((lvals.to(dvals) * dvals).sum())
-        print(f"discr {discr}: <mu, J v> = {lhs:.17g}, <J^T mu, v> = {rhs:.17g}", flush=True)
+        print(f"case {discr} {order} {reconst}: <mu, J v> = {lhs:.17g}, "
+              f"<J^T mu, v> = {rhs:.17g}", flush=True)
         assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
-        eps = 1e-6
-        fd = (dr.residual_raw(u + eps * du, vals + eps * dvals) -
-              dr.residual_raw(u - eps * du, vals - eps * dvals)) / (2 * eps)
-        err = float((fd - jv).norm() / jv.norm())
-        print(f"discr {discr}: tangent vs central difference rel {err:.3e}", flush=True)
+        def fd_at(eps):
+            return (dr.residual_raw(u + eps * du, vals + eps * dvals) -
+                    dr.residual_raw(u - eps * du, vals - eps * dvals)) / (2 * eps)
+        fd = fd_at(1e-6)
+        if reconst == 11:
+            scale = float(jv.abs().max())
+            smooth = ((fd - fd_at(1e-7)).abs().amax(dim=0) <= 1e-6 * scale)
+            frac = float(smooth.double().mean())
+            err = float((fd - jv)[:, smooth].norm() / jv[:, smooth].norm())
+            print(f"case {discr} {order} {reconst}: {frac:.4f} of cells smooth at both steps; "
+                  f"tangent vs central difference there rel {err:.3e}", flush=True)
+            assert frac >= 0.97
+        else:
+            err = float((fd - jv).norm() / jv.norm())
+            print(f"case {discr} {order} {reconst}: tangent vs central difference rel {err:.3e}",
+                  flush=True)
         assert err <= 1e-6
     finally:
         g.close()
@@ -89,12 +110,13 @@ def test_torch_forward_is_the_solver_step(gpu):
         g.close()
 
 
-@pytest.mark.parametrize("discr", [3, 4])
-def test_multistep_adjoint_vs_tangent_and_fd(gpu, discr):
-    m, num, cv0 = blast_box(discr)
+@pytest.mark.parametrize("discr,order,reconst", [(3, 1, 0), (4, 1, 0), (3, 2, 11)])
+def test_multistep_adjoint_vs_tangent_and_fd(gpu, discr, order, reconst):
+    m, num, cv0 = blast_box(discr, order=order, reconst=reconst,
+                            patch_order=[2, 2, 1, 1, 1, 1] if order == 2 else None)
     g = GpuRegion(m, num)
     try:
-        nsteps = 30
+        nsteps = 30 if order == 1 else 12
         dts, _ = adj.record_time_steps(g, cv0, nsteps)
         dr = adj.DiffRegion(g)
         gp = num.gas_prime()[2]
@@ -118,7 +140,8 @@ def test_multistep_adjoint_vs_tangent_and_fd(gpu, discr):
         J(u2, v2, None).backward()
         assert torch.equal(u2.grad, gu) and torch.equal(v2.grad, gv)
         import torch.autograd.forward_ad as fwAD
-        worst_fd = worst_tan = 0.0
+        worst_tan = 0.0
+        e_fds = []
         for k in range(4):
             du = rand_like(u0, 10 + k) * u0
             dv = rand_like(vals0, 20 + k) * vals0 * mask
@@ -135,9 +158,12 @@ def test_multistep_adjoint_vs_tangent_and_fd(gpu, discr):
             print(f"dir {k}: adjoint {ad:.15e} tangent {tan:.15e} fd {fd:.15e} "
                   f"rel(tan) {e_tan:.2e} rel(fd) {e_fd:.2e}", flush=True)
             worst_tan = max(worst_tan, e_tan)
-            worst_fd = max(worst_fd, e_fd)
+            e_fds.append(e_fd)
         assert worst_tan <= 1e-10
-        assert worst_fd <= 1e-5
+        # first order: smooth along these directions; second order with the limiter:
+        # piecewise smooth (kinks crossed over 12 steps leave ~1e-5 in a central
+        # difference of J), so the bound is looser there
+        assert max(e_fds) <= (1e-5 if order == 1 else 2e-4)
         assert float(gv[0].abs().sum()) > 0.0
     finally:
         g.close()
