@@ -301,23 +301,27 @@ def large_leg(args):
     return out
 
 
-def adjoint_leg(args):
-    """The discrete adjoint: a spherical blast (p = 100 inside 0.1 L) in a 55^3 x 6-tet
-    box (~1M cells) with a velocity/temperature inflow, first-order HLLC, SSP-RK3 at the
-    solver's own time steps; J = time-integrated pressure on a wall patch over 200
-    steps; gradient with respect to the initial state and the inflow values by reverse
-    mode through the adjoint kernels, checkpointed every 20 steps, compared with central
-    differences (eps = 1e-6) along 32 seeded directions and with the forward-mode
-    tangent along 2 of them."""
+def adjoint_leg(args, order=1, p_in=100.0, dirs=32, steps=None, checkpoint=None,
+                extra_eps=()):
+    """The discrete adjoint: a spherical blast (p = p_in inside 0.1 L) in a 55^3 x 6-tet
+    box (~1M cells) with a velocity/temperature inflow, HLLC of the given order (2: WLSQ
+    gradients + Venkatakrishnan), SSP-RK3 at the solver's own time steps; J =
+    time-integrated pressure on a wall patch over 200 steps; gradient with respect to the
+    initial state and the inflow values by reverse mode through the adjoint kernels,
+    checkpointed every 20 steps, compared with central differences (eps = 1e-6) along
+    `dirs` seeded directions (and, for each of extra_eps, a second difference along the
+    same directions) and with the forward-mode tangent along 2 of them."""
+    steps = args.adj_steps if steps is None else steps
+    checkpoint = args.adj_checkpoint if checkpoint is None else checkpoint
     import torch
     import torch.autograd.forward_ad as fwAD
     from rocflu_amd import cases, adjoint as adj
     from rocflu_amd.solver import GpuRegion
-    m, num, cv0 = cases.adjoint_case(n=args.adj_n)
+    m, num, cv0 = cases.adjoint_case(n=args.adj_n, order=order, p_in=p_in)
     g = GpuRegion(m, num)
     try:
         t0 = time.perf_counter()
-        dts, _ = adj.record_time_steps(g, cv0, args.adj_steps)
+        dts, _ = adj.record_time_steps(g, cv0, steps)
         t_solver = time.perf_counter() - t0
         dr = adj.DiffRegion(g)
         gp = num.gas_prime()[2]
@@ -326,7 +330,7 @@ def adjoint_leg(args):
         mask = torch.zeros_like(vals0)
         mask[0] = 1.0
 
-        def J(u, v, ce=args.adj_checkpoint):
+        def J(u, v, ce=checkpoint):
             return adj.time_integrated_wall_pressure(dr, u, v, dts, 2, gp, ce)[0]
 
         u = u0.clone().requires_grad_(True)
@@ -341,8 +345,9 @@ def adjoint_leg(args):
         gen = torch.Generator(device="cpu").manual_seed(4)
         eps = 1e-6
         rel_fd, rel_tan = [], []
+        rel_extra = {e: [] for e in extra_eps}
         t2 = time.perf_counter()
-        for k in range(args.adj_dirs):
+        for k in range(dirs):
             du = torch.randn(u0.shape, generator=gen, dtype=torch.float64).to(u0.device) * u0
             dv = torch.randn(vals0.shape, generator=gen, dtype=torch.float64).to(u0.device) * vals0 * mask
             ad = float((gu * du).sum() + (gv * dv).sum())
@@ -350,6 +355,11 @@ def adjoint_leg(args):
                 fd = (float(J(u0 + eps * du, vals0 + eps * dv, None)) -
                       float(J(u0 - eps * du, vals0 - eps * dv, None))) / (2 * eps)
             rel_fd.append(abs(ad - fd) / abs(fd))
+            for e in extra_eps:
+                with torch.no_grad():
+                    fe = (float(J(u0 + e * du, vals0 + e * dv, None)) -
+                          float(J(u0 - e * du, vals0 - e * dv, None))) / (2 * e)
+                rel_extra[e].append(abs(ad - fe) / abs(fe))
             if k < 2:
                 with fwAD.dual_level(), torch.no_grad():
                     jt = fwAD.unpack_dual(J(fwAD.make_dual(u0, du), fwAD.make_dual(vals0, dv),
@@ -358,13 +368,17 @@ def adjoint_leg(args):
         t_fd = time.perf_counter() - t2
     finally:
         g.close()
-    out = {"workload": f"{args.adj_n}^3 x 6 Kuhn tets ({m.n_cells} cells), blast p=100 in 0.1 L, "
-                       f"inflow u=0.3 at x-, HLLC order 1, SSP-RK3, 1% seeded noise",
-           "cells": int(m.n_cells), "steps": args.adj_steps,
-           "checkpoint_every": args.adj_checkpoint, "J": float(Jv),
+    out = {"workload": f"{args.adj_n}^3 x 6 Kuhn tets ({m.n_cells} cells), blast p={p_in:g} "
+                       f"in 0.1 L, inflow u=0.3 at x-, HLLC order {order}"
+                       f"{' + Venkatakrishnan' if order == 2 else ''}, SSP-RK3, "
+                       f"1% seeded noise",
+           "cells": int(m.n_cells), "steps": steps,
+           "checkpoint_every": checkpoint, "J": float(Jv),
            "grad_inflow_values": [float(x) for x in gv[0]],
-           "directions": args.adj_dirs, "fd_eps": eps,
+           "directions": dirs, "fd_eps": eps,
            "rel_err_vs_fd_max": max(rel_fd), "rel_err_vs_fd_median": float(np.median(rel_fd)),
+           "other_fd_eps": {f"{e:g}": {"max": max(v), "median": float(np.median(v))}
+                            for e, v in rel_extra.items()},
            "rel_err_vs_tangent_max": max(rel_tan) if rel_tan else None,
            "solver_s": t_solver, "gradient_s": t_grad, "fd_s": t_fd}
     print(f"# adjoint: J={float(Jv):.6e} grad in {t_grad:.1f}s, vs FD max {max(rel_fd):.2e} "
@@ -397,6 +411,11 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-checkpoint", type=int, default=20)
     ap.add_argument("--adj-dirs", type=int, default=32)
+    ap.add_argument("--adj2-dirs", type=int, default=8,
+                    help="directions of the second-order adjoint leg (blast p = 5)")
+    ap.add_argument("--adj2-steps", type=int, default=3)
+    ap.add_argument("--adj2-checkpoint", type=int, default=1)
+    ap.add_argument("--no-adjoint2", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain device copy of 512 MiB three times (calibrates the "
@@ -493,9 +512,22 @@ def main():
             out["large"] = {"error": repr(e)}
     if not args.no_adjoint:
         try:
-            out["adjoint"] = adjoint_leg(args)
+            out["adjoint"] = adjoint_leg(args, order=1, p_in=100.0, dirs=args.adj_dirs)
         except Exception as e:  # noqa: BLE001
             out["adjoint"] = {"error": repr(e)}
+    if not args.no_adjoint2:
+        # second order: the reference's limiter does not keep a p = 100 blast's
+        # reconstructed states positive (NaN within the first step), p = 5 runs.  The
+        # limited residual is piecewise smooth with many kinks per cell at 1M cells, so a
+        # central difference at eps = 1e-6 crosses some of them; it is reported with one at
+        # eps = 1e-9 beside it, on a short horizon
+        try:
+            out["adjoint_order2"] = adjoint_leg(args, order=2, p_in=5.0, dirs=args.adj2_dirs,
+                                                steps=args.adj2_steps,
+                                                checkpoint=args.adj2_checkpoint,
+                                                extra_eps=(1e-9,))
+        except Exception as e:  # noqa: BLE001
+            out["adjoint_order2"] = {"error": repr(e)}
     if not args.no_cpu:
         cb = cpu_baseline(args)
         out["cpu_baseline"] = cb

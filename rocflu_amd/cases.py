@@ -129,23 +129,27 @@ def ensemble(n_cases=64, min_cells=20_000, max_cells=500_000, mach=(1.2, 3.0),
     return out
 
 
-def adjoint_case(n=55, discr=3, p_in=100.0, stream=0.3, noise=0.01, seed=13, jitter=0.08):
+def adjoint_case(n=55, discr=3, p_in=100.0, stream=0.3, noise=0.01, seed=13, jitter=0.08,
+                 order=1, reconst=11):
     """Spherical blast (p = p_in, rho = 1 inside radius 0.1 L; p = 1 outside) in an
     n^3 x 6-tet box with a velocity/temperature inflow at x- carrying a slow stream
     (u = stream, T of rho = 1, p = 1), subsonic outflow (p = 1) at x+ and slip walls
-    elsewhere; first order, SSP-RK3.  Seeded relative noise of amplitude `noise` on
-    density, pressure and velocity keeps states off the kinks of the flux functions
-    (ties in min/max, zero speeds), where derivatives have no value.  Returns (mesh,
-    numerics, cv0); patch 2 (y-) is the wall of the objective."""
+    elsewhere; SSP-RK3, first order or second order (`reconst`: 11 Venkatakrishnan or 0;
+    inflow and outflow patches second order, walls first order).  Seeded relative noise
+    of amplitude `noise` on density, pressure and velocity keeps states off the kinks of
+    the flux functions (ties in min/max, zero speeds), where derivatives have no value.
+    Returns (mesh, numerics, cv0); patch 2 (y-) is the wall of the objective."""
     from .mesh import BC_INFLOW_VELTEMP
     m = box(TET6, n, n, n, lengths=(float(n),) * 3, jitter=jitter, seed=seed,
             bc_type=(BC_INFLOW_VELTEMP, BC_OUTFLOW) + (BC_SLIPWALL,) * 4,
             bc_opt=(BCOPT_SUBSONIC, BCOPT_SUBSONIC, 0, 0, 0, 0), bc_pout=(0.0, 1.0, 0, 0, 0, 0),
-            stencils=False)
+            stencils=order == 2)
     base = Numerics()
     rg = base.gas_prime()[1]
     vals = [[stream, 0.0, 0.0, 1.0 / rg, 1.0]] + [[0.0] * 5] * 5
-    num = Numerics(discr=discr, order=1, rk=4, cfl=0.5, dt_imposed=1e-3, patch_vals=vals)
+    num = Numerics(discr=discr, order=order, reconst=reconst if order == 2 else 0, rk=4, cfl=0.5,
+                   dt_imposed=1e-3, patch_vals=vals,
+                   patch_order=[2, 2, 1, 1, 1, 1] if order == 2 else None)
     gp = num.gas_prime()[2]
     rng = np.random.default_rng(seed)
     d = m.cofg - 0.5 * n
