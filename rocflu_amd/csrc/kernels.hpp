@@ -379,12 +379,11 @@ __device__ __forceinline__ double face_srad(const Mesh& M, const State& S, int f
   return srad_boundary(u1, v1, w1, S.dv[2 * (size_t)N + c], M.bfn[b]);
 }
 
-// Flux of interior face f (computed with f2c(1,f) as the left state).
+// Flux of interior face f (computed with f2c(1,f) = cc.x as the left state).
 template <int DISCR, int ORDER, typename T, typename GT>
 __device__ __forceinline__ void interior_face_flux(const Mesh& M, const State& S, const Gas& G,
-                                                   const GT* grad, int f, T* flx) {
+                                                   const GT* grad, int f, int2 cc, T* flx) {
   const int N = M.N;
-  int2 cc = M.f2c[f];
   int c1 = cc.x, c2 = cc.y;
   double4 n = M.fn[f];
   T nx = T(n.x), ny = T(n.y), nz = T(n.z), nm = T(n.w);
@@ -454,10 +453,12 @@ __global__ void k_face_flux(Mesh M, State S, Gas G, Cases C, const GT* __restric
   if (gate(sc)) return;
   int f = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
   if (f >= M.F) return;
+  // both cells in one 8-byte load, before anything waits on either
+  const int2 cc = M.f2c[f];
   if (SRAD) srad[f] = face_srad<ORDER>(M, S, f);
   T flx[5];
-  if (C.cid) G = C.gas[C.cid[M.f2c[f].x]];
-  interior_face_flux<DISCR, ORDER, T, GT>(M, S, G, grad, f, flx);
+  if (C.cid) G = C.gas[C.cid[cc.x]];
+  interior_face_flux<DISCR, ORDER, T, GT>(M, S, G, grad, f, cc, flx);
   const size_t FB = (size_t)M.F + M.B;
 #pragma unroll
   for (int k = 0; k < 5; ++k) flux[k * FB + f] = FT(flx[k]);
@@ -602,7 +603,10 @@ __device__ __forceinline__ void dep_vars(const Gas& G, double r, double mx, doub
   a = sqrt(G.gp * G.rgasp * t);
 }
 
-// Gathers the residual of cell i over its cell-to-face list.
+// Gathers the residual of cell i over its cell-to-face list.  Lists of up to 8 faces
+// (tetrahedra 4, prisms and pyramids 5, hexahedra 6) load every face id and then every
+// flux value before the first addition, so a cell waits for memory twice instead of
+// twice per face; the additions keep the list order.
 template <typename FT>
 __device__ __forceinline__ void gather_residual(const Mesh& M, const FT* __restrict__ flux, int i,
                                                 double* acc) {
@@ -610,6 +614,31 @@ __device__ __forceinline__ void gather_residual(const Mesh& M, const FT* __restr
 #pragma unroll
   for (int k = 0; k < 5; ++k) acc[k] = -0.0;
   int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
+  const int cnt = e1 - e0;
+  if (cnt <= 8) {
+    int sg[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) sg[q] = q < cnt ? M.c2f_idx[e0 + q] : 1;
+    FT v[8][5];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const size_t f = (size_t)((sg[q] > 0 ? sg[q] : -sg[q]) - 1);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) v[q][k] = q < cnt ? flux[k * FB + f] : FT(0);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (q >= cnt) break;
+      if (sg[q] > 0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc[k] = acc[k] + double(v[q][k]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc[k] = acc[k] - double(v[q][k]);
+      }
+    }
+    return;
+  }
   for (int e = e0; e < e1; ++e) {
     int s = M.c2f_idx[e];
     if (s > 0) {
@@ -1393,6 +1422,32 @@ __global__ void k_face_srad(Mesh M, State S, double* __restrict__ srad,
   srad[f] = face_srad<1>(M, S, f);
 }
 
+// Sum of the spectral radii of cell i's faces in cell-to-face order (all loads of a
+// list of up to 8 faces issued before the first addition).
+__device__ __forceinline__ double cell_srad_sum(const Mesh& M, const double* __restrict__ srad,
+                                                int i) {
+  const int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
+  const int cnt = e1 - e0;
+  double sum = 0.0;
+  if (cnt <= 8) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      int sgn = q < cnt ? M.c2f_idx[e0 + q] : 1;
+      v[q] = q < cnt ? srad[(sgn > 0 ? sgn : -sgn) - 1] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q < cnt) sum = sum + v[q];
+    return sum;
+  }
+  for (int e = e0; e < e1; ++e) {
+    int sgn = M.c2f_idx[e];
+    sum = sum + srad[(sgn > 0 ? sgn : -sgn) - 1];
+  }
+  return sum;
+}
+
 // Local time step vol/sum(srad) per cell, summed over the cell-to-face list in the
 // reference's scatter order, block minimum into part[].
 __global__ void k_dt_gather(Mesh M, const double* __restrict__ srad, double* __restrict__ part,
@@ -1402,15 +1457,7 @@ __global__ void k_dt_gather(Mesh M, const double* __restrict__ srad, double* __r
   const int blk = xcd_block(M.nxcd);
   int i = blk * blockDim.x + threadIdx.x;
   double dtc = HUGE_VAL;
-  if (i < N && !halted(sc)) {
-    double sum = 0.0;
-    int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
-    for (int e = e0; e < e1; ++e) {
-      int sgn = M.c2f_idx[e];
-      sum = sum + srad[(sgn > 0 ? sgn : -sgn) - 1];
-    }
-    dtc = M.cell[i].w / sum;
-  }
+  if (i < N && !halted(sc)) dtc = M.cell[i].w / cell_srad_sum(M, srad, i);
   dtc = wave_min(dtc);
   int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) red[wv] = dtc;
@@ -1480,15 +1527,7 @@ __global__ void k_dt_gather_cases(Mesh M, Cases C, const double* __restrict__ sr
   double dtc = HUGE_VAL;
   if (i < N) {
     c = C.cid[i];
-    if (!halted(C.sc + c)) {
-      double sum = 0.0;
-      int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
-      for (int e = e0; e < e1; ++e) {
-        int sgn = M.c2f_idx[e];
-        sum = sum + srad[(sgn > 0 ? sgn : -sgn) - 1];
-      }
-      dtc = M.cell[i].w / sum;
-    }
+    if (!halted(C.sc + c)) dtc = M.cell[i].w / cell_srad_sum(M, srad, i);
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int c0 = __shfl(c, 0, 64);
