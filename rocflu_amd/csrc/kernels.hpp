@@ -946,43 +946,93 @@ __device__ __forceinline__ void grad_member(const GradCoef<T>& c, double ddx, do
 
 // Venkatakrishnan limiter of cell i in FP64: varMin/varMax over the interior-face
 // neighbours, then the face-wise limiter with dx = cofg - fc (the reference's sign).
-// nbr(q, v) loads the q-th neighbour's (rho,u,v,w,p) and returns false past the last
-// one.  MIN/MAX reductions do not depend on the face order, so the precomputed
-// neighbour slots give the reference's values exactly.
-template <typename T, typename Fetch>
-__device__ __forceinline__ void venkat_limit(const Mesh& M, int i, const double* vi, T* g,
-                                             Fetch nbr) {
-  const int N = M.N;
+// idx(q) gives the handle of the q-th neighbour (negative past the last one) and
+// vals(h, v) loads that neighbour's (rho,u,v,w,p).  MIN/MAX reductions do not depend
+// on the face order, so the precomputed neighbour slots give the reference's values
+// exactly.  With FP64 gradients and at most 8 neighbours (every element type) all
+// handles and face offsets are requested before the first use (measured faster); with
+// FP32 gradients the extra registers cost more than the waits they save.
+template <typename T>
+__device__ __forceinline__ void venkat_face(double d2, double dm, double epsq, double& lim) {
   const double TINY = 1.0E-12;
+  double numer = (dm * dm + epsq) * d2 + 2 * d2 * d2 * dm;
+  double denom = d2 * (dm * dm + 2 * d2 * d2 + dm * d2 + epsq);
+  double term = numer / (denom + TINY);
+  if (d2 != 0.0) lim = term < lim ? term : lim;
+}
+
+template <typename T, typename Idx, typename Vals>
+__device__ __forceinline__ void venkat_limit(const Mesh& M, int i, const double* vi, T* g,
+                                             Idx idx, Vals vals) {
+  const int N = M.N;
   double vmn[5], vmx[5], lim[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) { vmn[k] = vi[k]; vmx[k] = vi[k]; lim[k] = 1.0; }
-  int nv = 0;
-  for (int q = 0; q < M.vq; ++q) {
-    double vo[5];
-    if (!nbr(q, vo)) break;
-    ++nv;
+  if (sizeof(T) == 8 && M.vq <= 8) {
+    int h[8];
+    double gx[8], gy[8], gz[8];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      vmn[k] = vmn[k] < vo[k] ? vmn[k] : vo[k];
-      vmx[k] = vmx[k] > vo[k] ? vmx[k] : vo[k];
+    for (int q = 0; q < 8; ++q) h[q] = q < M.vq ? idx(q) : -1;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const bool on = q < M.vq;
+      gx[q] = on ? M.vgeo[(size_t)(3 * q) * N + i] : 0.0;
+      gy[q] = on ? M.vgeo[(size_t)(3 * q + 1) * N + i] : 0.0;
+      gz[q] = on ? M.vgeo[(size_t)(3 * q + 2) * N + i] : 0.0;
     }
-  }
-  double epsq = M.epsq[i];
-  for (int q = 0; q < nv; ++q) {
-    double dx = M.vgeo[(size_t)(3 * q) * N + i];
-    double dy = M.vgeo[(size_t)(3 * q + 1) * N + i];
-    double dz = M.vgeo[(size_t)(3 * q + 2) * N + i];
+    const double epsq = M.epsq[i];
+    int nv = 8;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      double d2 = double(g[3 * k]) * dx + double(g[3 * k + 1]) * dy + double(g[3 * k + 2]) * dz;
-      // the reference's two branches (d2 > 0 with varMax, d2 < 0 with varMin) share one
-      // expression; selecting the operand keeps the lanes of a wavefront together
-      double dm = d2 > 0.0 ? vmx[k] - vi[k] : vmn[k] - vi[k];
-      double numer = (dm * dm + epsq) * d2 + 2 * d2 * d2 * dm;
-      double denom = d2 * (dm * dm + 2 * d2 * d2 + dm * d2 + epsq);
-      double term = numer / (denom + TINY);
-      if (d2 != 0.0) lim[k] = term < lim[k] ? term : lim[k];
+    for (int q = 0; q < 8; ++q)
+      if (h[q] < 0 && q < nv) nv = q;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (q >= nv) break;
+      double vo[5];
+      vals(h[q], vo);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        vmn[k] = vmn[k] < vo[k] ? vmn[k] : vo[k];
+        vmx[k] = vmx[k] > vo[k] ? vmx[k] : vo[k];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (q >= nv) break;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        double d2 = double(g[3 * k]) * gx[q] + double(g[3 * k + 1]) * gy[q] + double(g[3 * k + 2]) * gz[q];
+        // the reference's two branches (d2 > 0 with varMax, d2 < 0 with varMin) share one
+        // expression; selecting the operand keeps the lanes of a wavefront together
+        double dm = d2 > 0.0 ? vmx[k] - vi[k] : vmn[k] - vi[k];
+        venkat_face<T>(d2, dm, epsq, lim[k]);
+      }
+    }
+  } else {
+    int nv = 0;
+    for (int q = 0; q < M.vq; ++q) {
+      int hq = idx(q);
+      if (hq < 0) break;
+      ++nv;
+      double vo[5];
+      vals(hq, vo);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        vmn[k] = vmn[k] < vo[k] ? vmn[k] : vo[k];
+        vmx[k] = vmx[k] > vo[k] ? vmx[k] : vo[k];
+      }
+    }
+    const double epsq = M.epsq[i];
+    for (int q = 0; q < nv; ++q) {
+      double dx = M.vgeo[(size_t)(3 * q) * N + i];
+      double dy = M.vgeo[(size_t)(3 * q + 1) * N + i];
+      double dz = M.vgeo[(size_t)(3 * q + 2) * N + i];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        double d2 = double(g[3 * k]) * dx + double(g[3 * k + 1]) * dy + double(g[3 * k + 2]) * dz;
+        double dm = d2 > 0.0 ? vmx[k] - vi[k] : vmn[k] - vi[k];
+        venkat_face<T>(d2, dm, epsq, lim[k]);
+      }
     }
   }
 #pragma unroll
@@ -991,7 +1041,6 @@ __device__ __forceinline__ void venkat_limit(const Mesh& M, int i, const double*
     for (int d = 0; d < 3; ++d) g[3 * k + d] = T(lim[k] * double(g[3 * k + d]));
 }
 
-// One thread per cell, members read from global memory (L2/MALL).
 template <typename GT, bool VENKAT, int U>
 __global__ void k_gradient(Mesh M, State S, GT* __restrict__ grad, const Scalars* __restrict__ sc) {
   if (gate(sc)) return;
@@ -1031,13 +1080,11 @@ __global__ void k_gradient(Mesh M, State S, GT* __restrict__ grad, const Scalars
     grad_member<GT>(c, cj.x - ci.x, cj.y - ci.y, cj.z - ci.z, dvar, g);
   }
   if (VENKAT)
-    venkat_limit<GT>(M, i, vi, g, [&](int q, double* vo) {
-      int o = M.vnb[(size_t)q * N + i];
-      if (o < 0) return false;
+    venkat_limit<GT>(M, i, vi, g, [&](int q) { return M.vnb[(size_t)q * N + i]; },
+                     [&](int o, double* vo) {
 #pragma unroll
-      for (int k = 0; k < 5; ++k) vo[k] = S.prim[(size_t)k * N + o];
-      return true;
-    });
+                       for (int k = 0; k < 5; ++k) vo[k] = S.prim[(size_t)k * N + o];
+                     });
 #pragma unroll
   for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
 }
@@ -1167,13 +1214,15 @@ __global__ __launch_bounds__(TS) void k_gradient_lds(Mesh M, State S, Tiles T,
   }
   if (VENKAT) {
     const unsigned short* V = T.vloc + (size_t)t * M.vq * T.ts + threadIdx.x;
-    venkat_limit<GT>(M, i, vi, g, [&](int q, double* vo) {
-      int k = V[(size_t)q * T.ts];
-      if (k == 0xFFFF) return false;
+    venkat_limit<GT>(M, i, vi, g,
+                     [&](int q) {
+                       int k = V[(size_t)q * T.ts];
+                       return k == 0xFFFF ? -1 : k;
+                     },
+                     [&](int k, double* vo) {
 #pragma unroll
-      for (int r = 0; r < 5; ++r) vo[r] = lv[r * cap + k];
-      return true;
-    });
+                       for (int r = 0; r < 5; ++r) vo[r] = lv[r * cap + k];
+                     });
   }
 #pragma unroll
   for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
@@ -1272,13 +1321,15 @@ __global__ __launch_bounds__(TS) void k_gradient_ldsw(Mesh M, State S, Tiles T, 
   }
   if (VENKAT) {
     const unsigned short* V = T.vloc + (size_t)t * M.vq * T.ts + threadIdx.x;
-    venkat_limit<GT>(M, i, vi, g, [&](int q, double* vo) {
-      int k = V[(size_t)q * T.ts];
-      if (k == 0xFFFF) return false;
+    venkat_limit<GT>(M, i, vi, g,
+                     [&](int q) {
+                       int k = V[(size_t)q * T.ts];
+                       return k == 0xFFFF ? -1 : k;
+                     },
+                     [&](int k, double* vo) {
 #pragma unroll
-      for (int r = 0; r < 5; ++r) vo[r] = lv[r * cap + k];
-      return true;
-    });
+                       for (int r = 0; r < 5; ++r) vo[r] = lv[r * cap + k];
+                     });
   }
 #pragma unroll
   for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
@@ -1339,13 +1390,11 @@ __global__ void k_venkat_apply(Mesh M, State S, GT* __restrict__ grad,
   for (int k = 0; k < 5; ++k) vi[k] = S.prim[(size_t)k * N + i];
 #pragma unroll
   for (int k = 0; k < 15; ++k) g[k] = grad[(size_t)k * N + i];
-  venkat_limit<GT>(M, i, vi, g, [&](int q, double* vo) {
-    int o = M.vnb[(size_t)q * N + i];
-    if (o < 0) return false;
+  venkat_limit<GT>(M, i, vi, g, [&](int q) { return M.vnb[(size_t)q * N + i]; },
+                   [&](int o, double* vo) {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) vo[k] = S.prim[(size_t)k * N + o];
-    return true;
-  });
+                     for (int k = 0; k < 5; ++k) vo[k] = S.prim[(size_t)k * N + o];
+                   });
 #pragma unroll
   for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
 }
