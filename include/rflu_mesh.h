@@ -30,9 +30,12 @@ typedef struct rflu_mesh rflu_mesh;
 #define RFLU_MESH_TET6 1   /* six tetrahedra per box cell (Kuhn split on 0-7 axis) */
 
 /* Boundary types (modfloflu/ModParameters.F90:397-438). */
+#define RFLU_BC_INFLOW_TOTANG  10
 #define RFLU_BC_INFLOW_VELTEMP 11
 #define RFLU_BC_OUTFLOW  20
 #define RFLU_BC_SLIPWALL 60
+#define RFLU_BC_FARFIELD 80
+#define RFLU_BC_INJECTION 90
 
 /* Outflow options (modfloflu/ModParameters.F90:501-503). */
 #define RFLU_BCOPT_SUPERSONIC 0

@@ -87,8 +87,10 @@ typedef struct {
   const int32_t* bf2c;       /* (nBFaces)                                            */
   const double* bfn;         /* (4, nBFaces)                                         */
   const double* bfc;         /* (3, nBFaces)                                         */
-  const int32_t* patch_bc;   /* (n_patches) 11 inflow (BC_INFLOW_VELTEMP), 20 outflow,
-                                60 slip wall (modfloflu/ModParameters.F90:397-400)   */
+  const int32_t* patch_bc;   /* (n_patches) 10 inflow (BC_INFLOW_TOTANG), 11 inflow
+                                (BC_INFLOW_VELTEMP), 20 outflow, 60 slip wall,
+                                80 farfield, 90 injection
+                                (modfloflu/ModParameters.F90:397-438)                */
   const int32_t* patch_opt;  /* (n_patches) BCSWI_INFLOW_TYPE / BCSWI_OUTFLOW_TYPE:
                                 0 supersonic, 1 subsonic, 2 mixed                    */
   const double* patch_pout;  /* (n_patches) outflow static pressure                  */
@@ -99,9 +101,15 @@ typedef struct {
   double gamma, cp;          /* global%refGamma, global%refCp                        */
   int32_t discr, order, reconst, rk, precision;
   double cfl, dt_imposed, dt_min_limit, eps_entr, diss_fact;
-  const double* patch_vals;  /* (5, n_patches) inflow BCDAT_INFLOW_U, _V, _W, _T, _P
-                                (P used by the non-subsonic options); NULL when no
-                                patch is an inflow                                   */
+  const double* patch_vals;  /* (5, n_patches) pPatch%mixt%vals(1:5) (constant per
+                                patch; modfloflu/ModParameters.F90:452-494):
+                                11: BCDAT_INFLOW_U, _V, _W, _T, _P (P used by the
+                                    non-subsonic options)
+                                10: BCDAT_INFLOW_PTOT, _TTOT, _BETAH, _BETAV, _MACH
+                                    (MACH used by the non-subsonic options)
+                                80: BCDAT_FARF_MACH, _ATTACK, _SLIP, _PRESS, _TEMP
+                                90: BCDAT_INJECT_MFRATE, _TEMP (rest unused)
+                                NULL when no patch reads values                      */
   /* Constrained gradients at walls (order 2, mixtInput%cReconstCells = CRECONSTC):
    * c2cs(icg)%bFaceMembs flattened to a CSR (c2cs_bptr, n_cells+1 0-based offsets)
    * of (patch, face within the patch) pairs (index_base applies to both), and
@@ -111,6 +119,10 @@ typedef struct {
   double creconst_weight;
   int32_t creconst;          /* 0 off (reference default), 1 on                      */
   int32_t pad_;
+  const int32_t* patch_opt2; /* (n_patches) second switch: BCSWI_INFLOW_FIXED of
+                                inflow 10 patches (0 or 1), BCSWI_FARF_CORR of
+                                farfield patches (0 only: no vortex correction);
+                                NULL = all 0                                         */
 } rflu_region_desc;
 
 /* Per-region counters of the after-update checks, accumulated since the last reset. */

@@ -36,6 +36,7 @@ class OrcRegion(ctypes.Structure):
         ("c2cs_bptr", _i32p), ("c2cs_bface", _i32p),
         ("creconst_weight", ctypes.c_double), ("creconst", ctypes.c_int32),
         ("pad_", ctypes.c_int32),
+        ("patch_opt2", _i32p),
     ]
 
 
@@ -84,6 +85,8 @@ def lib():
         L.orc_after_update.restype = ctypes.c_int
         L.orc_fluxes.argtypes = [R, _f64p, _f64p, _f64p, _f64p, ctypes.POINTER(OrcStats),
                                  ctypes.c_int]
+        L.orc_rind_state.argtypes = [R, ctypes.c_int, _f64p, _f64p, ctypes.c_double, _f64p]
+        L.orc_patch_flux.argtypes = [R, ctypes.c_int, _f64p, _f64p, ctypes.c_double, _f64p]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _L = L
@@ -115,7 +118,7 @@ class Oracle:
         i32, f64 = ctypes.c_int32, ctypes.c_double
         r.n_cells, r.n_faces, r.n_patches = self.n, int(mesh.n_faces), int(mesh.n_patches)
         for name in ("f2c", "bpatch_ptr", "bf2c", "patch_bc", "patch_opt", "patch_order",
-                     "c2f_ptr", "c2f_idx"):
+                     "c2f_ptr", "c2f_idx", "patch_opt2"):
             setattr(r, name, _p(k[name], i32))
         for name in ("fn", "fc", "cofg", "vol", "bfn", "bfc", "patch_pout", "patch_vals"):
             setattr(r, name, _p(k[name], f64))
@@ -133,6 +136,24 @@ class Oracle:
         r.eps_entr, r.diss_fact = num.eps_entr, num.diss_fact
         self.r = r
         self.L = lib()
+
+    def rind_state(self, patch, n, q, pl):
+        """Boundary state of one face of `patch` (see orc_rind_state)."""
+        n = np.ascontiguousarray(n, dtype=np.float64)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.empty(6)
+        self.L.orc_rind_state(ctypes.byref(self.r), int(patch), _p(n, ctypes.c_double),
+                              _p(q, ctypes.c_double), float(pl), _p(out, ctypes.c_double))
+        return out
+
+    def patch_flux(self, patch, n, q, pl):
+        """Flux of one face of `patch` from interior state q and pressure pl."""
+        n = np.ascontiguousarray(n, dtype=np.float64)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.empty(5)
+        self.L.orc_patch_flux(ctypes.byref(self.r), int(patch), _p(n, ctypes.c_double),
+                              _p(q, ctypes.c_double), float(pl), _p(out, ctypes.c_double))
+        return out
 
     def dv(self, cv):
         cv = np.ascontiguousarray(cv, dtype=np.float64)

@@ -41,9 +41,12 @@ int orc_get_threads(void) { return g_threads; }
 
 #define ORC_PAR _Pragma("omp parallel for schedule(static) num_threads(g_threads) if(g_threads > 1)")
 
+#define BC_INFLOW_TOTANG 10
 #define BC_INFLOW_VELTEMP 11
 #define BC_OUTFLOW 20
 #define BC_SLIPWALL 60
+#define BC_FARFIELD 80
+#define BC_INJECTION 90
 #define BCOPT_SUPERSONIC 0
 #define BCOPT_SUBSONIC 1
 #define BCOPT_MIXED 2
@@ -291,6 +294,173 @@ static void inflow_veltemp(int bcOpt, const double* vals, double rl, double rgas
   *rwr = *rr * wr;
 }
 
+/* Inflow boundary state from total pressure, total temperature and flow angles
+ * (BC_INFLOW_TOTANG), libfloflu/BcondInflowPerf.F90:95-223: subsonic faces keep the
+ * interior Riemann invariant Rm = a - (g-1)/2 q.n and solve the total-enthalpy
+ * quadratic for the boundary sound speed; supersonic (and mixed faces with
+ * |q.n| >= a) take the state of the given Mach number.  A subsonic-option face with
+ * |q.n| >= a leaves the reference's outputs unset; here they are NaN, which the
+ * validity check then reports. */
+static void bcond_inflow_perf(int bcOptType, int bcOptFixed, double ptot, double ttot,
+                              double betah, double betav, double mach, double sxn, double syn,
+                              double szn, double cpgas, double mm, double rl, double rul,
+                              double rvl, double rwl, double* rr, double* rur, double* rvr,
+                              double* rwr, double* rer, double* pr) {
+  double rgas = 8314.3 / mm;                   /* MixtPerf_R_M */
+  double g = cpgas / (cpgas - rgas);           /* MixtPerf_G_CpR */
+  double tr, ar, ur, vr, wr;
+  if (bcOptType == BCOPT_SUBSONIC || bcOptType == BCOPT_MIXED) {
+    double gm1 = g - 1.0;
+    double ul = rul / rl, vl = rvl / rl, wl = rwl / rl;
+    double a02 = g * rgas * ttot;                                    /* MixtPerf_C2_GRT */
+    double al = sqrt(a02 - 0.5 * (g - 1.0) * (ul * ul + vl * vl + wl * wl)); /* C_Co2GUVW */
+    double ql = ul * sxn + vl * syn + wl * szn;
+    if (fabs(ql) < al) {
+      double sl = sqrt(ul * ul + vl * vl + wl * wl);
+      double eta;
+      if (bcOptFixed == 0)
+        eta = sl > 1.0E-6 ? ql / sl : -1.0;
+      else
+        eta = -1.0;
+      double Rm = al - 0.5 * gm1 * ql;
+      double disc = 0.5 * gm1 * (eta * eta) *
+                    (a02 / (Rm * Rm) * (1.0 + 0.5 * gm1 * (eta * eta)) - 1.0);
+      double sr;
+      if (disc < 0.0) {
+        ar = sqrt(a02);
+        tr = ttot;
+        *pr = ptot;
+        sr = 0.0;
+      } else {
+        ar = Rm / (1.0 + 0.5 * gm1 * eta * eta) * (1.0 + sqrt(disc));
+        tr = ar * ar / (g * rgas);                                   /* MixtPerf_T_CGR */
+        *pr = ptot * pow(tr / ttot, g / (g - 1.0));                  /* MixtPerf_P_GPoTTo */
+        double c2 = ar * ar;                                         /* MixtPerf_Vm_C2Co2G */
+        sr = a02 > c2 ? sqrt(2.0 / (g - 1.0) * (a02 - c2)) : 0.0;
+      }
+      *rr = *pr / (rgas * tr);                                       /* MixtPerf_D_PRT */
+      ur = sr * cos(betah) * cos(betav);
+      vr = sr * sin(betah);
+      wr = sr * cos(betah) * sin(betav);
+      *rer = *rr * (*pr / (*rr * (g - 1.0)) + 0.5 * sr * sr);        /* MixtPerf_Eo_DGPVm */
+      *rur = *rr * ur;
+      *rvr = *rr * vr;
+      *rwr = *rr * wr;
+      return;
+    }
+    if (bcOptType != BCOPT_MIXED) {
+      *rr = *rur = *rvr = *rwr = *rer = *pr = NAN;
+      return;
+    }
+  }
+  *pr = ptot / pow(1.0 + 0.5 * (g - 1.0) * mach * mach, g / (g - 1.0));  /* MixtPerf_P_GMaPo */
+  tr = ttot / (1.0 + 0.5 * (g - 1.0) * mach * mach);                    /* MixtPerf_T_GMaTo */
+  *rr = *pr / (rgas * tr);
+  ar = sqrt(g * rgas * tr);                                             /* MixtPerf_C_GRT */
+  ur = mach * ar * cos(betah) * cos(betav);
+  vr = mach * ar * sin(betah);
+  wr = mach * ar * cos(betah) * sin(betav);
+  *rer = *rr * (*pr / (*rr * (g - 1.0)) + 0.5 * (ur * ur + vr * vr + wr * wr));
+  *rur = *rr * ur;
+  *rvr = *rr * vr;
+  *rwr = *rr * wr;
+}
+
+/* Farfield boundary state (BC_FARFIELD), modflu/RFLU_ModRindStates.F90:248-420 without
+ * the point-vortex correction (BCOPT_CORR_NO): free stream from Mach number, angles of
+ * attack and sideslip, pressure and temperature; subsonic faces blend it with the
+ * interior state along the acoustic characteristics, supersonic faces take one or the
+ * other by the sign of the interior normal velocity.  The interior pressure is formed
+ * from rho E (MixtPerf_P_DEoGVm2). */
+static void rind_farfield_perf(double cpgas, double mm, double nx, double ny, double nz,
+                               double machInf, double pInf, double tInf, double alphaInf,
+                               double betaInf, double rl, double rul, double rvl, double rwl,
+                               double rel, double* rr, double* rur, double* rvr, double* rwr,
+                               double* rer, double* pr) {
+  double rgas = 8314.3 / mm;
+  double g = cpgas / (cpgas - rgas);
+  double ul = rul / rl, vl = rvl / rl, wl = rwl / rl;
+  double ql = ul * nx + vl * ny + wl * nz;
+  double el = rel / rl;
+  double sl2 = ul * ul + vl * vl + wl * wl;
+  double pl = (g - 1.0) * rl * (el - 0.5 * sl2);
+  double ri = pInf / (rgas * tInf);
+  double pi = pInf;
+  double qi = machInf * sqrt(g * rgas * tInf);
+  double ui = qi * cos(alphaInf) * cos(betaInf);
+  double vi = qi * sin(alphaInf) * cos(betaInf);
+  double wi = qi * sin(betaInf);
+  if (machInf < 1.0) {
+    double al = sqrt(g * pl / rl);
+    double pb, rb, ub, vb, wb;
+    if (ql < 0.0) {
+      pb = 0.5 * (pi + pl - rl * al * ((ui - ul) * nx + (vi - vl) * ny + (wi - wl) * nz));
+      rb = ri - (pi - pb) / (al * al);
+      ub = ui - nx * (pi - pb) / (rl * al);
+      vb = vi - ny * (pi - pb) / (rl * al);
+      wb = wi - nz * (pi - pb) / (rl * al);
+    } else {
+      pb = pi;
+      rb = rl - (pl - pi) / (al * al);
+      ub = ul + nx * (pl - pi) / (rl * al);
+      vb = vl + ny * (pl - pi) / (rl * al);
+      wb = wl + nz * (pl - pi) / (rl * al);
+    }
+    *rr = rb;
+    *rur = rb * ub;
+    *rvr = rb * vb;
+    *rwr = rb * wb;
+    *rer = rb * (pb / (rb * (g - 1.0)) + 0.5 * (ub * ub + vb * vb + wb * wb));
+    *pr = pb;
+  } else if (ql < 0.0) {
+    *rr = ri;
+    *rur = ri * ui;
+    *rvr = ri * vi;
+    *rwr = ri * wi;
+    *rer = ri * (pi / (ri * (g - 1.0)) + 0.5 * qi * qi);
+    *pr = pi;
+  } else {
+    *rr = rl; *rur = rul; *rvr = rvl; *rwr = rwl; *rer = rel;
+    *pr = pl;
+  }
+}
+
+/* Injected gas at a burning surface (BC_INJECTION, mass flux minj > 0),
+ * modflu/RFLU_ModRindStates.F90:459-501: density from the interior pressure and the
+ * injection temperature, velocity -minj/rho along the normal, total enthalpy. */
+static void rind_inject_perf(double cpgas, double mm, double nx, double ny, double nz,
+                             double minj, double tinj, double pl, double fs, double* rl,
+                             double* ul, double* vl, double* wl, double* Hl) {
+  double rgas = 8314.3 / mm;
+  *rl = pl / (rgas * tinj);
+  double ql = -minj / *rl + fs;
+  *ul = ql * nx;
+  *vl = ql * ny;
+  *wl = ql * nz;
+  *Hl = cpgas * tinj + 0.5 * (*ul * *ul + *vl * *vl + *wl * *wl);   /* MixtPerf_Ho_CpTUVW */
+}
+
+/* Wall pressure of a non-burning injection face, modflu/RFLU_ModRindStates.F90:535-584:
+ * isentropic expansion for a receding normal velocity, the piston-shock relation for
+ * an approaching one. */
+static void rind_slipwall_perf(double cpgas, double mm, double nx, double ny, double nz,
+                               double rl, double rul, double rvl, double rwl, double fs,
+                               double* pl) {
+  double rgas = 8314.3 / mm;
+  double g = cpgas / (cpgas - rgas);
+  double irl = 1.0 / rl;
+  double ul = irl * rul, vl = irl * rvl, wl = irl * rwl;
+  double ql = ul * nx + vl * ny + wl * nz - fs;
+  double al = sqrt(g * *pl / rl);
+  if (ql < 0.0) {
+    double term = 1.0 + 0.5 * (g - 1.0) * ql / al;
+    *pl = *pl * pow(term, 2.0 * g / (g - 1.0));
+  } else {
+    double term = (g + 1.0) / 4.0;
+    *pl = *pl + term * rl * ql * (ql + sqrt(al * al + term * term * ql * ql) / term);
+  }
+}
+
 /* Central flux of an outflow face from the interior and boundary states,
  * rocflu/RFLU_CentralFirstPatch.F90:676-686 (same form in CentralSecondPatch). */
 static void outflow_flux(double nx, double ny, double nz, double nm, double fs, double rl,
@@ -393,54 +563,142 @@ static void interior_flux(const orc_region* R, const double* cv, const double* d
   }
 }
 
-/* Boundary-face flux of patch face b (rocflu/RFLU_CentralFirstPatch.F90:571-906,
- * rocflu/RFLU_CentralSecondPatch.F90:634-1024).  Boundary patches run at first order
- * unless the patch asks for second order; this restatement uses the region's
- * patch's order (patch_order[p] == 2 selects second order; the reference default,
- * spaceOrderBFaces = 1, is at libflu/RFLU_InitInputValues.F90:195 and the per-patch
- * ORDER key at modflu/RFLU_ModReadBcInputFile.F90:301-309).  A first-order patch
- * in a second-order run reads cv after CellGradientsMP's conversion, whose slot 5
- * holds p (see orc_cell_gradients), exactly as the reference's outflow patch does. */
-static void boundary_flux(const orc_region* R, const double* cv, const double* dv,
-                          const double* grad, int patch, int b, int patch_order2, double* flx,
-                          int* has_mass) {
-  int c1 = R->bf2c[b];
-  const double* n = R->bfn + 4 * (size_t)b;
+/* Flux of a boundary face from its interior state (rl, rul, rvl, rwl, rel, pl) by
+ * patch type; the same in both orders (rocflu/RFLU_CentralFirstPatch.F90,
+ * RFLU_CentralSecondPatch.F90, case by case):
+ *   slip wall   (0, p n |A|, 0)                                   First :785-906
+ *   inflow 10   BcondInflowPerf state, central average            First :228-374
+ *   inflow 11   prescribed velocity/temperature state, central    First :375-570
+ *   outflow 20  BcondOutflowPerf state, central average           First :571-784
+ *   farfield 80 RFLU_SetRindStateFarfieldPerf state, central      First :948-1042
+ *   injection 90  minj > 0: injected mass, momentum and enthalpy with the interior
+ *               pressure; minj <= 0: a wall with the corrected pressure  First :1043-1127
+ * Returns 1 where the face carries mass (every type but the slip wall). */
+static int patch_flux(const orc_region* R, int patch, const double* n, double rl, double rul,
+                      double rvl, double rwl, double rel, double pl, double* flx) {
   double nx = n[0], ny = n[1], nz = n[2], nm = n[3];
   const double fs = 0.0;
   double mm, rgas, gam;
   gas_prime(R, &mm, &rgas, &gam);
-  const double* q = cv + 5 * (size_t)c1;
   int bc = R->patch_bc[patch];
-  *has_mass = 0;
-  if (!patch_order2) {
-    double rl = q[0], rul = q[1], rvl = q[2], rwl = q[3], rel = q[4];
-    double pl = dv[3 * (size_t)c1];
-    if (bc == BC_SLIPWALL) {
+  int opt = R->patch_opt[patch];
+  int opt2 = R->patch_opt2 ? R->patch_opt2[patch] : 0;
+  const double* vals = R->patch_vals ? R->patch_vals + 5 * (size_t)patch : NULL;
+  double rr, rur, rvr, rwr, rer, pr;
+  switch (bc) {
+    case BC_SLIPWALL:
       flx[0] = 0.0;
       flx[1] = pl * nx * nm;
       flx[2] = pl * ny * nm;
       flx[3] = pl * nz * nm;
       flx[4] = pl * fs * nm;
-      return;
+      return 0;
+    case BC_INJECTION: {
+      double minj = vals[0];
+      if (minj > 0.0) {
+        double ri, ui, vi, wi, Hi;
+        rind_inject_perf(R->cp, mm, nx, ny, nz, minj, vals[1], pl, fs, &ri, &ui, &vi, &wi, &Hi);
+        flx[0] = -minj * nm;
+        flx[1] = (-minj * ui + pl * nx) * nm;
+        flx[2] = (-minj * vi + pl * ny) * nm;
+        flx[3] = (-minj * wi + pl * nz) * nm;
+        flx[4] = (-minj * Hi + pl * fs) * nm;
+      } else {
+        rind_slipwall_perf(R->cp, mm, nx, ny, nz, rl, rul, rvl, rwl, fs, &pl);
+        flx[0] = 0.0;
+        flx[1] = pl * nx * nm;
+        flx[2] = pl * ny * nm;
+        flx[3] = pl * nz * nm;
+        flx[4] = pl * fs * nm;
+      }
+      return 1;
     }
-    int opt = R->patch_opt[patch];
-    double rr, rur, rvr, rwr, rer;
-    if (bc == BC_INFLOW_VELTEMP) {
-      double pr;
-      inflow_veltemp(opt, R->patch_vals + 5 * (size_t)patch, rl, rgas, gam, &rr, &rur, &rvr,
-                     &rwr, &rer, &pr);
-      outflow_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, rr, rur, rvr, rwr, rer, pr, flx);
-      *has_mass = 1;
-      return;
-    }
-    double pr = opt != BCOPT_SUPERSONIC ? R->patch_pout[patch] : pl;
-    bcond_outflow_perf(opt, pr, nx, ny, nz, R->cp, mm, rl, rul, rvl, rwl, rel, pl, &rr, &rur,
-                       &rvr, &rwr, &rer);
-    outflow_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, rr, rur, rvr, rwr, rer, pr, flx);
-    *has_mass = 1;
+    case BC_INFLOW_VELTEMP:
+      inflow_veltemp(opt, vals, rl, rgas, gam, &rr, &rur, &rvr, &rwr, &rer, &pr);
+      break;
+    case BC_INFLOW_TOTANG:
+      bcond_inflow_perf(opt, opt2, vals[0], vals[1], vals[2], vals[3],
+                        opt != BCOPT_SUBSONIC ? vals[4] : 0.0, nx, ny, nz, R->cp, mm, rl, rul,
+                        rvl, rwl, &rr, &rur, &rvr, &rwr, &rer, &pr);
+      break;
+    case BC_FARFIELD:
+      rind_farfield_perf(R->cp, mm, nx, ny, nz, vals[0], vals[3], vals[4], vals[1], vals[2], rl,
+                         rul, rvl, rwl, rel, &rr, &rur, &rvr, &rwr, &rer, &pr);
+      break;
+    default:  /* BC_OUTFLOW */
+      pr = opt != BCOPT_SUPERSONIC ? R->patch_pout[patch] : pl;
+      bcond_outflow_perf(opt, pr, nx, ny, nz, R->cp, mm, rl, rul, rvl, rwl, rel, pl, &rr, &rur,
+                         &rvr, &rwr, &rer);
+      break;
+  }
+  outflow_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, rr, rur, rvr, rwr, rer, pr, flx);
+  return 1;
+}
+
+/* Test entry points: the boundary state of patch `patch` for interior state
+ * q = (rho, rho u, rho v, rho w, rho E) and pressure pl at a face with n = (nx, ny, nz,
+ * |A|), and that face's flux.  out = (rho, rho u, rho v, rho w, rho E, p) of the
+ * boundary state; injection faces give (rho, u, v, w, H, -) of the injected gas when
+ * burning and (-, -, -, -, -, p_wall) otherwise. */
+void orc_rind_state(const orc_region* R, int patch, const double* n, const double* q, double pl,
+                    double* out) {
+  double mm, rgas, gam;
+  gas_prime(R, &mm, &rgas, &gam);
+  int bc = R->patch_bc[patch], opt = R->patch_opt[patch];
+  int opt2 = R->patch_opt2 ? R->patch_opt2[patch] : 0;
+  const double* v = R->patch_vals + 5 * (size_t)patch;
+  for (int k = 0; k < 6; ++k) out[k] = NAN;
+  switch (bc) {
+    case BC_INFLOW_TOTANG:
+      bcond_inflow_perf(opt, opt2, v[0], v[1], v[2], v[3], opt != BCOPT_SUBSONIC ? v[4] : 0.0,
+                        n[0], n[1], n[2], R->cp, mm, q[0], q[1], q[2], q[3], &out[0], &out[1],
+                        &out[2], &out[3], &out[4], &out[5]);
+      break;
+    case BC_FARFIELD:
+      rind_farfield_perf(R->cp, mm, n[0], n[1], n[2], v[0], v[3], v[4], v[1], v[2], q[0], q[1],
+                         q[2], q[3], q[4], &out[0], &out[1], &out[2], &out[3], &out[4], &out[5]);
+      break;
+    case BC_INJECTION:
+      if (v[0] > 0.0) {
+        rind_inject_perf(R->cp, mm, n[0], n[1], n[2], v[0], v[1], pl, 0.0, &out[0], &out[1],
+                         &out[2], &out[3], &out[4]);
+      } else {
+        out[5] = pl;
+        rind_slipwall_perf(R->cp, mm, n[0], n[1], n[2], q[0], q[1], q[2], q[3], 0.0, &out[5]);
+      }
+      break;
+    default:
+      break;
+  }
+}
+
+void orc_patch_flux(const orc_region* R, int patch, const double* n, const double* q, double pl,
+                    double* flx) {
+  (void)patch_flux(R, patch, n, q[0], q[1], q[2], q[3], q[4], pl, flx);
+}
+
+/* Boundary-face flux of patch face b (rocflu/RFLU_CentralFirstPatch.F90,
+ * rocflu/RFLU_CentralSecondPatch.F90).  Boundary patches run at first order
+ * unless the patch asks for second order; this restatement uses the region's
+ * patch's order (patch_order[p] == 2 selects second order; the reference default,
+ * spaceOrderBFaces = 1, is at libflu/RFLU_InitInputValues.F90:195 and the per-patch
+ * ORDER key at modflu/RFLU_ModReadBcInputFile.F90:301-309).  A first-order patch
+ * in a second-order run reads cv after CellGradientsMP's conversion, whose slot 5
+ * holds p (see orc_cell_gradients), exactly as the reference's patches do.  Second
+ * order reconstructs rho, u, v, w, p at the face centroid and forms rho E from them
+ * (RFLU_CentralSecondPatch.F90:258-296). */
+static void boundary_flux(const orc_region* R, const double* cv, const double* dv,
+                          const double* grad, int patch, int b, int patch_order2, double* flx,
+                          int* has_mass) {
+  int c1 = R->bf2c[b];
+  const double* n = R->bfn + 4 * (size_t)b;
+  const double* q = cv + 5 * (size_t)c1;
+  if (!patch_order2) {
+    *has_mass = patch_flux(R, patch, n, q[0], q[1], q[2], q[3], q[4], dv[3 * (size_t)c1], flx);
     return;
   }
+  double mm, rgas, gam;
+  gas_prime(R, &mm, &rgas, &gam);
   double rl = q[0], irl = 1.0 / rl;
   const double* xf = R->bfc + 3 * (size_t)b;
   double dx = xf[0] - R->cofg[3 * (size_t)c1];
@@ -454,31 +712,8 @@ static void boundary_flux(const orc_region* R, const double* cv, const double* d
   vl = vl + gr[6] * dx + gr[7] * dy + gr[8] * dz;
   wl = wl + gr[9] * dx + gr[10] * dy + gr[11] * dz;
   pl = pl + gr[12] * dx + gr[13] * dy + gr[14] * dz;
-  double rul = rl * ul, rvl = rl * vl, rwl = rl * wl;
-  if (bc == BC_SLIPWALL) {
-    flx[0] = 0.0;
-    flx[1] = pl * nx * nm;
-    flx[2] = pl * ny * nm;
-    flx[3] = pl * nz * nm;
-    flx[4] = pl * fs * nm;
-    return;
-  }
-  int opt = R->patch_opt[patch];
   double rel = rl * (pl / (rl * (gam - 1.0)) + 0.5 * (ul * ul + vl * vl + wl * wl));
-  double rr, rur, rvr, rwr, rer;
-  if (bc == BC_INFLOW_VELTEMP) {
-    double pr;
-    inflow_veltemp(opt, R->patch_vals + 5 * (size_t)patch, rl, rgas, gam, &rr, &rur, &rvr, &rwr,
-                   &rer, &pr);
-    outflow_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, rr, rur, rvr, rwr, rer, pr, flx);
-    *has_mass = 1;
-    return;
-  }
-  double pr = opt != BCOPT_SUPERSONIC ? R->patch_pout[patch] : pl;
-  bcond_outflow_perf(opt, pr, nx, ny, nz, R->cp, mm, rl, rul, rvl, rwl, rel, pl, &rr, &rur, &rvr,
-                     &rwr, &rer);
-  outflow_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, rr, rur, rvr, rwr, rer, pr, flx);
-  *has_mass = 1;
+  *has_mass = patch_flux(R, patch, n, rl, rl * ul, rl * vl, rl * wl, rel, pl, flx);
 }
 
 /* Faces of cell icg in the scatter's order: keys f (interior face f) and F + b
@@ -1118,6 +1353,26 @@ static double bface_srad(const orc_region* R, const double* cv, const double* dv
   const double fsu = 0.0;
   int c1 = R->bf2c[b];
   const double* n = R->bfn + 4 * (size_t)b;
+  int p = patch_of(R, b);
+  if (R->patch_bc[p] == BC_INJECTION) {
+    /* rocflu/RFLU_TimeStepInviscid.F90:214-260: the injected state's normal velocity
+     * and the sound speed at the injection temperature; a non-burning face counts its
+     * sound speed only. */
+    const double* vals = R->patch_vals + 5 * (size_t)p;
+    double minj = vals[0], vc, a1;
+    if (minj > 0.0) {
+      double mm, rgas, gam, rl, ul, vl, wl, Hl;
+      gas_prime(R, &mm, &rgas, &gam);
+      rind_inject_perf(R->cp, mm, n[0], n[1], n[2], minj, vals[1], dv[3 * (size_t)c1], fsu, &rl,
+                       &ul, &vl, &wl, &Hl);
+      vc = ul * n[0] + vl * n[1] + wl * n[2];
+      a1 = sqrt(gam * rgas * vals[1]);
+    } else {
+      vc = fsu;
+      a1 = dv[3 * (size_t)c1 + 2];
+    }
+    return (fabs(vc - fsu) + a1) * n[3];
+  }
   double ir1 = 1.0 / cv[5 * (size_t)c1];
   double u1 = cv[5 * (size_t)c1 + 1] * ir1, v1 = cv[5 * (size_t)c1 + 2] * ir1,
          w1 = cv[5 * (size_t)c1 + 3] * ir1, a1 = dv[3 * (size_t)c1 + 2];
@@ -1170,12 +1425,7 @@ double orc_local_dt(const orc_region* R, const double* cv, const double* dv) {
   for (int p = 0; p < R->n_patches; ++p)
     for (int b = R->bpatch_ptr[p]; b < R->bpatch_ptr[p + 1]; ++b) {
       int c1 = R->bf2c[b];
-      const double* n = R->bfn + 4 * (size_t)b;
-      double ir1 = 1.0 / cv[5 * (size_t)c1];
-      double u1 = cv[5 * (size_t)c1 + 1] * ir1, v1 = cv[5 * (size_t)c1 + 2] * ir1,
-             w1 = cv[5 * (size_t)c1 + 3] * ir1, a1 = dv[3 * (size_t)c1 + 2];
-      double vc = u1 * n[0] + v1 * n[1] + w1 * n[2];
-      dt[c1] = dt[c1] + (fabs(vc - fsu) + a1) * n[3];
+      dt[c1] = dt[c1] + bface_srad(R, cv, dv, b);
     }
   double mn = HUGE_VAL;
   for (size_t i = 0; i < N; ++i) {

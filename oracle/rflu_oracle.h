@@ -28,7 +28,9 @@ typedef struct {
   const int32_t* bf2c;       /* (B) */
   const double* bfn;         /* (4,B) */
   const double* bfc;         /* (3,B) */
-  const int32_t* patch_bc;   /* (P) 11 inflow (velocity + temperature), 20 outflow, 60 slip wall */
+  const int32_t* patch_bc;   /* (P) 10 inflow (total conditions and angles), 11 inflow
+                                (velocity + temperature), 20 outflow, 60 slip wall,
+                                80 farfield, 90 injection */
   const int32_t* patch_opt;  /* (P) inflow/outflow option 0 supersonic, 1 subsonic, 2 mixed */
   const double* patch_pout;  /* (P) outflow static pressure */
   const int32_t* patch_order;/* (P) 1 or 2 (boundary-face order); NULL = all first order */
@@ -44,7 +46,10 @@ typedef struct {
   int32_t reconst;           /* 0 none, 2 WENO-XYZ + clip, 11 Venkatakrishnan */
   int32_t rk;                /* 1 RK4, 2 Wray RK3, 3 Euler, 4 SSP-RK3 */
   double cfl, dt_imposed, dt_min_limit, eps_entr, diss_fact;
-  const double* patch_vals;  /* (5,P) inflow u, v, w, T, p (p: supersonic option only) */
+  const double* patch_vals;  /* (5,P) BCDAT 1..5 of the patch: inflow 11 u, v, w, T, p
+                                (p: non-subsonic options); inflow 10 ptot, ttot, betah,
+                                betav, Mach; farfield Mach, attack, sideslip, p, T;
+                                injection mass flux, T */
   /* constrained gradients (CRECONSTC): wall faces of each stencil, (patch, face in
    * patch) pairs in CSR c2cs_bptr, and cReconstCellsWeight */
   const int32_t* c2cs_bptr;
@@ -52,6 +57,7 @@ typedef struct {
   double creconst_weight;
   int32_t creconst;          /* 0 off, 1 on */
   int32_t pad_;
+  const int32_t* patch_opt2; /* (P) BCSWI_INFLOW_FIXED of inflow 10 patches; NULL = all 0 */
 } orc_region;
 
 typedef struct {
@@ -77,6 +83,11 @@ void orc_gradients_constr(const orc_region* r, const double* var, double* grad);
 void orc_weno_xyz(const orc_region* r, double* grad);
 void orc_limit_simple(const orc_region* r, const double* var, double* grad);
 void orc_venkat(const orc_region* r, const double* var, double* grad);
+/* Boundary state and flux of one face of a patch (test entry points). */
+void orc_rind_state(const orc_region* r, int patch, const double* n, const double* q, double pl,
+                    double* out);
+void orc_patch_flux(const orc_region* r, int patch, const double* n, const double* q, double pl,
+                    double* flx);
 void orc_cell_gradients(const orc_region* r, double* cv, const double* dv, double* grad);
 void orc_rk_update(const orc_region* r, int stage, double dt, double* cv, const double* cvOld,
                    const double* rhs, double* rhsSum);

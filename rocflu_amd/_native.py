@@ -49,6 +49,7 @@ class RegionDesc(ctypes.Structure):
         ("c2cs_bptr", _i32p), ("c2cs_bface", _i32p),
         ("creconst_weight", ctypes.c_double), ("creconst", ctypes.c_int32),
         ("pad_", ctypes.c_int32),
+        ("patch_opt2", _i32p),
     ]
 
 

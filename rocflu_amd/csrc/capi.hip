@@ -42,6 +42,7 @@ struct Region {
   double end_time = HUGE_VAL;  // global%maxTime of the fused loop
   Gas gas{};                   // case 0 (the only case of a single region)
   bool has_flowthrough = false;
+  bool has_rind_patch = false;   // a total-condition inflow, farfield or injection patch
   // batched cases (K == 1: a single region; per-case tables still hold its values)
   int K = 1;
   std::vector<int> cell0_h, bface0_h;
@@ -51,7 +52,7 @@ struct Region {
   CaseParams* par = nullptr;
   int *cell0 = nullptr, *bface0 = nullptr;
   unsigned long long* dtmin = nullptr;
-  int *pbc = nullptr, *popt = nullptr, *porder2 = nullptr;
+  int *pbc = nullptr, *popt = nullptr, *popt2 = nullptr, *porder2 = nullptr;
   double *ppout = nullptr, *pvals = nullptr;
   std::vector<double> pvals_h;
   std::vector<int> bptr_h;     // first boundary face of every patch (all cases), P+1
@@ -133,7 +134,7 @@ struct Region {
   }
   Patches patches() const {
     Patches p;
-    p.bc = pbc; p.opt = popt; p.order2 = porder2; p.pout = ppout; p.vals = pvals;
+    p.bc = pbc; p.opt = popt; p.opt2 = popt2; p.order2 = porder2; p.pout = ppout; p.vals = pvals;
     return p;
   }
   // scalars that gate a whole launch: the single case's, none for a batch
@@ -184,7 +185,7 @@ void free_region(Region* R) {
                   R->rhs_sum, R->dv, R->prim, R->grad, R->grad2, R->flux, R->rhs, R->dt_part,
                   R->sc, R->tile_uptr, R->tile_uidx, R->tile_lptr, R->tile_loc, R->tile_vloc,
                   R->vnb, R->vgeo, R->srad, R->gw, R->cid, R->gas_tab, R->par, R->cell0,
-                  R->bface0, R->dtmin, R->pbc, R->popt, R->porder2, R->ppout, R->pvals,
+                  R->bface0, R->dtmin, R->pbc, R->popt, R->popt2, R->porder2, R->ppout, R->pvals,
                   R->bptr, R->adj_g, R->adj_gb, R->adj_gv, R->adj_pv, R->adj_dflux,
                   R->adj_dvals, R->adj_p, R->adj_ga, R->adj_gb2, R->adj_gc, R->adj_gr, R->adj_selv,
                   R->adj_seln, R->adj_tptr, R->adj_tidx,
@@ -709,7 +710,8 @@ int launch_time_step(Region* R, double time_in = -1.0, double end_time = HUGE_VA
   State S = R->state();
   const int nf = R->F + R->B;
   if (nf > 0)
-    hipLaunchKernelGGL(k_face_srad, dim3(nblk(nf, 256)), dim3(256), 0, R->stream, M, S, R->srad, R->gate_sc());
+    hipLaunchKernelGGL(k_face_srad, dim3(nblk(nf, 256)), dim3(256), 0, R->stream, M, S, R->gas,
+                       R->cases(), R->patches(), R->srad, R->gate_sc());
   return launch_dt_reduce(R, time_in, end_time);
 }
 
@@ -955,15 +957,34 @@ int check_desc(const rflu_region_desc* d, const rflu_region_desc* ref) {
   if (B > 0 && (!d->bf2c || !d->bfn || !d->bfc)) return fail(RFLU_ERR_ARG, "missing boundary arrays");
   if (d->bpatch_ptr && d->bpatch_ptr[0] != 0) return fail(RFLU_ERR_ARG, "bpatch_ptr[0] must be 0");
   for (int p = 0; p < P; ++p) {
-    if (d->patch_bc[p] != BC_OUTFLOW && d->patch_bc[p] != BC_SLIPWALL &&
-        d->patch_bc[p] != BC_INFLOW_VELTEMP)
-      return fail(RFLU_ERR_ARG, "only inflow (11), outflow (20) and slip-wall (60) patches are supported");
-    if (d->patch_bc[p] == BC_INFLOW_VELTEMP) {
-      if (!d->patch_vals) return fail(RFLU_ERR_ARG, "inflow patches need patch_vals");
-      const double* v = d->patch_vals + 5 * (size_t)p;
-      if (!(v[3] > 0.0) || (d->patch_opt && d->patch_opt[p] != 1 && !(v[4] > 0.0)))
-        return fail(RFLU_ERR_ARG, "inflow temperature (and non-subsonic pressure) must be > 0");
-    }
+    const int bc = d->patch_bc[p];
+    const int opt = d->patch_opt ? d->patch_opt[p] : 0;
+    const int opt2 = d->patch_opt2 ? d->patch_opt2[p] : 0;
+    if (bc != BC_OUTFLOW && bc != BC_SLIPWALL && bc != BC_INFLOW_VELTEMP &&
+        bc != BC_INFLOW_TOTANG && bc != BC_FARFIELD && bc != BC_INJECTION)
+      return fail(RFLU_ERR_ARG, "patch types: inflow (10, 11), outflow (20), slip wall (60), "
+                                "farfield (80), injection (90)");
+    if (bc == BC_SLIPWALL) continue;
+    if (bc != BC_INJECTION && bc != BC_FARFIELD && (opt < 0 || opt > 2))
+      return fail(RFLU_ERR_ARG, "inflow/outflow option must be 0, 1 or 2");
+    if (bc == BC_OUTFLOW) continue;
+    if (!d->patch_vals) return fail(RFLU_ERR_ARG, "inflow, farfield and injection patches need patch_vals");
+    const double* v = d->patch_vals + 5 * (size_t)p;
+    if (bc == BC_INFLOW_VELTEMP &&
+        (!(v[3] > 0.0) || (opt != 1 && !(v[4] > 0.0))))
+      return fail(RFLU_ERR_ARG, "inflow temperature (and non-subsonic pressure) must be > 0");
+    if (bc == BC_INFLOW_TOTANG &&
+        (!(v[0] > 0.0) || !(v[1] > 0.0) || (opt != 1 && !(v[4] > 0.0)) || opt2 < 0 || opt2 > 1))
+      return fail(RFLU_ERR_ARG, "total-condition inflow needs ptot, ttot (and a non-subsonic "
+                                "Mach number) > 0 and a fixed-angle switch of 0 or 1");
+    if (bc == BC_FARFIELD && (!(v[0] >= 0.0) || !(v[3] > 0.0) || !(v[4] > 0.0)))
+      return fail(RFLU_ERR_ARG, "farfield Mach number must be >= 0, pressure and temperature > 0");
+    if (bc == BC_FARFIELD && opt2 != 0)
+      return fail(RFLU_ERR_ARG, "the farfield vortex correction (BCOPT_CORR_YES) is not supported");
+    if (bc == BC_INJECTION && (!(v[0] == v[0]) || (v[0] > 0.0 && !(v[1] > 0.0))))
+      return fail(RFLU_ERR_ARG, "injection temperature must be > 0 where the mass flux is");
+    if (bc == BC_INJECTION && d->order == 2 && d->creconst)
+      return fail(RFLU_ERR_ARG, "constrained gradients at injection patches are not supported");
   }
   if (ref && (d->order != ref->order || d->discr != ref->discr || d->rk != ref->rk ||
               d->precision != ref->precision || (d->order == 2 && d->reconst != ref->reconst)))
@@ -1018,7 +1039,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
   R->par_h.resize(K);
   R->cell0_h.assign(co.begin(), co.end());
   R->bface0_h.assign(bo.begin(), bo.end());
-  std::vector<int> hpbc(P), hpopt(P), hporder2(P);
+  std::vector<int> hpbc(P), hpopt(P), hpopt2(P), hporder2(P);
   std::vector<double> hppout(P), hpvals((size_t)5 * P, 0.0);
   R->bptr_h.assign(P + 1, 0);
   for (int c = 0; c < K; ++c) {
@@ -1031,11 +1052,15 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       const size_t q = (size_t)po[c] + p;
       hpbc[q] = d->patch_bc[p];
       hpopt[q] = d->patch_opt ? d->patch_opt[p] : 0;
+      hpopt2[q] = d->patch_opt2 ? d->patch_opt2[p] : 0;
       hporder2[q] = d->patch_order ? (d->patch_order[p] == 2) : 0;
       hppout[q] = d->patch_pout ? d->patch_pout[p] : 0.0;
       if (d->patch_vals)
         for (int k = 0; k < 5; ++k) hpvals[5 * q + k] = d->patch_vals[5 * (size_t)p + k];
       if (d->patch_bc[p] != BC_SLIPWALL) R->has_flowthrough = true;   // mass crosses the patch
+      if (d->patch_bc[p] == BC_INFLOW_TOTANG || d->patch_bc[p] == BC_FARFIELD ||
+          d->patch_bc[p] == BC_INJECTION)
+        R->has_rind_patch = true;
       R->bptr_h[q] = (int)bo[c] + d->bpatch_ptr[p];
       R->bptr_h[q + 1] = (int)bo[c] + d->bpatch_ptr[p + 1];
     }
@@ -1180,7 +1205,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       (rc = dalloc(&R->srad, FB)) || (rc = dalloc(&R->gas_tab, K)) || (rc = dalloc(&R->par, K)) ||
       (rc = dalloc(&R->cell0, K + 1)) || (rc = dalloc(&R->bface0, K + 1)) ||
       (rc = dalloc(&R->dtmin, (size_t)K * DTMIN_STRIDE)) || (rc = dalloc(&R->pbc, P)) || (rc = dalloc(&R->popt, P)) ||
-      (rc = dalloc(&R->porder2, P)) || (rc = dalloc(&R->ppout, P)) ||
+      (rc = dalloc(&R->porder2, P)) || (rc = dalloc(&R->ppout, P)) || (rc = dalloc(&R->popt2, P)) ||
       (rc = dalloc(&R->pvals, (size_t)5 * P)) || (rc = dalloc(&R->bptr, P + 1)) ||
       (K > 1 && (rc = dalloc(&R->cid, N)))) {
     free_region(R);
@@ -1218,6 +1243,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       (rc = dput(R->cell0, hcell0, R->stream)) || (rc = dput(R->bface0, hbface0, R->stream)) ||
       (rc = dput(R->pbc, hpbc, R->stream)) || (rc = dput(R->popt, hpopt, R->stream)) ||
       (rc = dput(R->porder2, hporder2, R->stream)) || (rc = dput(R->ppout, hppout, R->stream)) ||
+      (rc = dput(R->popt2, hpopt2, R->stream)) ||
       (rc = dput(R->pvals, hpvals, R->stream)) || (rc = dput(R->bptr, R->bptr_h, R->stream)) ||
       (K > 1 && (rc = dput(R->cid, hcid, R->stream)))) {
     free_region(R);
@@ -1857,6 +1883,9 @@ int derivative_ready(Region* R) {
     return fail(RFLU_ERR_ARG, "second-order derivatives cover no modifier or the Venkatakrishnan limiter");
   if (R->order == 2 && R->con_n > 0)
     return fail(RFLU_ERR_ARG, "second-order derivatives do not cover constrained gradients");
+  if (R->has_rind_patch)
+    return fail(RFLU_ERR_ARG, "derivatives cover slip-wall, outflow and velocity/temperature "
+                              "inflow patches");
   int rc;
   const size_t F = R->F, B = R->B, P = R->P, N = R->N;
   const bool o2 = R->order == 2;

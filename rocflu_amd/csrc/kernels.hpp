@@ -19,9 +19,12 @@
 
 namespace rflu {
 
+constexpr int BC_INFLOW_TOTANG = 10;
 constexpr int BC_INFLOW_VELTEMP = 11;
 constexpr int BC_OUTFLOW = 20;
 constexpr int BC_SLIPWALL = 60;
+constexpr int BC_FARFIELD = 80;
+constexpr int BC_INJECTION = 90;
 constexpr int BCOPT_SUPERSONIC = 0;
 constexpr int BCOPT_SUBSONIC = 1;
 constexpr int BCOPT_MIXED = 2;
@@ -29,10 +32,11 @@ constexpr int BCOPT_MIXED = 2;
 // Boundary patches of a region (all cases), indexed by bpatch[b].
 struct Patches {
   const int* bc;
-  const int* opt;
+  const int* opt;       // BCSWI_INFLOW_TYPE / BCSWI_OUTFLOW_TYPE
+  const int* opt2;      // BCSWI_INFLOW_FIXED of total-condition inflow patches
   const int* order2;
   const double* pout;
-  const double* vals;   // (5, P) inflow u, v, w, T, p
+  const double* vals;   // (5, P) the patch's BCDAT values 1..5
 };
 
 // Inflow boundary state with prescribed velocity and temperature (BC_INFLOW_VELTEMP,
@@ -495,6 +499,256 @@ __device__ __forceinline__ void bcond_outflow(int bcOpt, double pout, double sxn
   }
 }
 
+// Total-condition inflow state (BC_INFLOW_TOTANG, libfloflu/BcondInflowPerf.F90:95-223):
+// vals = (ptot, ttot, betah, betav, Mach).  Subsonic faces keep the interior Riemann
+// invariant a - (g-1)/2 q.n and solve the total-enthalpy quadratic for the boundary
+// sound speed; supersonic faces, and mixed-option faces with |q.n| >= a, take the
+// state of the given Mach number.  A subsonic-option face with |q.n| >= a has no
+// state in the reference (its outputs stay unset); it is NaN here, which the
+// validity check reports.  b = (rho, rho u, rho v, rho w, rho E); returns p.
+__device__ __forceinline__ double bcond_inflow_totang(int bcOptType, int bcOptFixed,
+                                                      const double* vals, double sxn, double syn,
+                                                      double szn, const Gas& G, double rl,
+                                                      double rul, double rvl, double rwl,
+                                                      double* b) {
+  const double rgas = G.rgasp, g = G.gp;
+  const double ptot = vals[0], ttot = vals[1], betah = vals[2], betav = vals[3];
+  const double mach = bcOptType != BCOPT_SUBSONIC ? vals[4] : 0.0;
+  double pr, tr, ar;
+  if (bcOptType == BCOPT_SUBSONIC || bcOptType == BCOPT_MIXED) {
+    const double gm1 = g - 1.0;
+    const double ul = rul / rl, vl = rvl / rl, wl = rwl / rl;
+    const double a02 = g * rgas * ttot;
+    const double al = sqrt(a02 - 0.5 * (g - 1.0) * (ul * ul + vl * vl + wl * wl));
+    const double ql = ul * sxn + vl * syn + wl * szn;
+    if (fabs(ql) < al) {
+      const double sl = sqrt(ul * ul + vl * vl + wl * wl);
+      const double eta = (bcOptFixed == 0 && sl > 1.0E-6) ? ql / sl : -1.0;
+      const double Rm = al - 0.5 * gm1 * ql;
+      const double disc = 0.5 * gm1 * (eta * eta) *
+                          (a02 / (Rm * Rm) * (1.0 + 0.5 * gm1 * (eta * eta)) - 1.0);
+      double sr;
+      if (disc < 0.0) {
+        ar = sqrt(a02);
+        tr = ttot;
+        pr = ptot;
+        sr = 0.0;
+      } else {
+        ar = Rm / (1.0 + 0.5 * gm1 * eta * eta) * (1.0 + sqrt(disc));
+        tr = ar * ar / (g * rgas);
+        pr = ptot * pow(tr / ttot, g / (g - 1.0));
+        const double c2 = ar * ar;
+        sr = a02 > c2 ? sqrt(2.0 / (g - 1.0) * (a02 - c2)) : 0.0;
+      }
+      const double rr = pr / (rgas * tr);
+      const double ur = sr * cos(betah) * cos(betav);
+      const double vr = sr * sin(betah);
+      const double wr = sr * cos(betah) * sin(betav);
+      b[0] = rr;
+      b[1] = rr * ur;
+      b[2] = rr * vr;
+      b[3] = rr * wr;
+      b[4] = rr * (pr / (rr * (g - 1.0)) + 0.5 * sr * sr);
+      return pr;
+    }
+    if (bcOptType != BCOPT_MIXED) {
+      const double nan = __builtin_nan("");
+      b[0] = b[1] = b[2] = b[3] = b[4] = nan;
+      return nan;
+    }
+  }
+  pr = ptot / pow(1.0 + 0.5 * (g - 1.0) * mach * mach, g / (g - 1.0));
+  tr = ttot / (1.0 + 0.5 * (g - 1.0) * mach * mach);
+  const double rr = pr / (rgas * tr);
+  ar = sqrt(g * rgas * tr);
+  const double ur = mach * ar * cos(betah) * cos(betav);
+  const double vr = mach * ar * sin(betah);
+  const double wr = mach * ar * cos(betah) * sin(betav);
+  b[0] = rr;
+  b[1] = rr * ur;
+  b[2] = rr * vr;
+  b[3] = rr * wr;
+  b[4] = rr * (pr / (rr * (g - 1.0)) + 0.5 * (ur * ur + vr * vr + wr * wr));
+  return pr;
+}
+
+// Farfield state (BC_FARFIELD, modflu/RFLU_ModRindStates.F90:248-420, no vortex
+// correction): vals = (Mach, attack, sideslip, p, T).  The interior pressure is
+// formed from rho E.  Returns the boundary pressure.
+__device__ __forceinline__ double rind_farfield(const Gas& G, const double* vals, double nx,
+                                                double ny, double nz, double rl, double rul,
+                                                double rvl, double rwl, double rel, double* b) {
+  const double rgas = G.rgasp, g = G.gp;
+  const double machInf = vals[0], alphaInf = vals[1], betaInf = vals[2], pInf = vals[3],
+               tInf = vals[4];
+  const double ul = rul / rl, vl = rvl / rl, wl = rwl / rl;
+  const double ql = ul * nx + vl * ny + wl * nz;
+  const double el = rel / rl;
+  const double sl2 = ul * ul + vl * vl + wl * wl;
+  const double pl = (g - 1.0) * rl * (el - 0.5 * sl2);
+  const double ri = pInf / (rgas * tInf);
+  const double pi = pInf;
+  const double qi = machInf * sqrt(g * rgas * tInf);
+  const double ui = qi * cos(alphaInf) * cos(betaInf);
+  const double vi = qi * sin(alphaInf) * cos(betaInf);
+  const double wi = qi * sin(betaInf);
+  if (machInf < 1.0) {
+    const double al = sqrt(g * pl / rl);
+    double pb, rb, ub, vb, wb;
+    if (ql < 0.0) {
+      pb = 0.5 * (pi + pl - rl * al * ((ui - ul) * nx + (vi - vl) * ny + (wi - wl) * nz));
+      rb = ri - (pi - pb) / (al * al);
+      ub = ui - nx * (pi - pb) / (rl * al);
+      vb = vi - ny * (pi - pb) / (rl * al);
+      wb = wi - nz * (pi - pb) / (rl * al);
+    } else {
+      pb = pi;
+      rb = rl - (pl - pi) / (al * al);
+      ub = ul + nx * (pl - pi) / (rl * al);
+      vb = vl + ny * (pl - pi) / (rl * al);
+      wb = wl + nz * (pl - pi) / (rl * al);
+    }
+    b[0] = rb;
+    b[1] = rb * ub;
+    b[2] = rb * vb;
+    b[3] = rb * wb;
+    b[4] = rb * (pb / (rb * (g - 1.0)) + 0.5 * (ub * ub + vb * vb + wb * wb));
+    return pb;
+  }
+  if (ql < 0.0) {
+    b[0] = ri;
+    b[1] = ri * ui;
+    b[2] = ri * vi;
+    b[3] = ri * wi;
+    b[4] = ri * (pi / (ri * (g - 1.0)) + 0.5 * qi * qi);
+    return pi;
+  }
+  b[0] = rl; b[1] = rul; b[2] = rvl; b[3] = rwl; b[4] = rel;
+  return pl;
+}
+
+// Injected gas at a burning face (modflu/RFLU_ModRindStates.F90:459-501): density from
+// the interior pressure at the injection temperature, velocity -minj/rho along n.
+__device__ __forceinline__ void rind_inject(const Gas& G, double nx, double ny, double nz,
+                                            double minj, double tinj, double pl, double fs,
+                                            double& rl, double& ul, double& vl, double& wl,
+                                            double& Hl) {
+  rl = pl / (G.rgasp * tinj);
+  const double ql = -minj / rl + fs;
+  ul = ql * nx;
+  vl = ql * ny;
+  wl = ql * nz;
+  Hl = G.cp * tinj + 0.5 * (ul * ul + vl * vl + wl * wl);
+}
+
+// Wall pressure of a non-burning injection face (modflu/RFLU_ModRindStates.F90:535-584):
+// isentropic expansion when the gas recedes from the wall, the piston-shock relation
+// when it approaches.
+__device__ __forceinline__ double rind_slipwall(const Gas& G, double nx, double ny, double nz,
+                                                double rl, double rul, double rvl, double rwl,
+                                                double fs, double pl) {
+  const double g = G.gp;
+  const double irl = 1.0 / rl;
+  const double ul = irl * rul, vl = irl * rvl, wl = irl * rwl;
+  const double ql = ul * nx + vl * ny + wl * nz - fs;
+  const double al = sqrt(g * pl / rl);
+  if (ql < 0.0) {
+    const double term = 1.0 + 0.5 * (g - 1.0) * ql / al;
+    return pl * pow(term, 2.0 * g / (g - 1.0));
+  }
+  const double term = (g + 1.0) / 4.0;
+  return pl + term * rl * ql * (ql + sqrt(al * al + term * term * ql * ql) / term);
+}
+
+// Flux of a boundary face of patch p from its interior state, by patch type (the
+// cases of rocflu/RFLU_CentralFirstPatch.F90 and RFLU_CentralSecondPatch.F90): slip
+// wall (0, p n |A|, 0); injection: injected mass, momentum and enthalpy with the
+// interior pressure, or a wall with the corrected pressure when minj <= 0; the
+// others the central average of the interior and boundary states.
+__device__ __forceinline__ void patch_face_flux(const Gas& G, const Patches& P, int p, double nx,
+                                                double ny, double nz, double nm, double rl,
+                                                double rul, double rvl, double rwl, double rel,
+                                                double pl, double* flx) {
+  const double fs = 0.0;
+  const int bc = P.bc[p];
+  if (bc == BC_SLIPWALL) {
+    flx[0] = 0.0;
+    flx[1] = pl * nx * nm;
+    flx[2] = pl * ny * nm;
+    flx[3] = pl * nz * nm;
+    flx[4] = 0.0;
+    return;
+  }
+  const double* v = P.vals + 5 * (size_t)p;
+  if (bc == BC_INJECTION) {
+    const double minj = v[0];
+    if (minj > 0.0) {
+      double ri, ui, vi, wi, Hi;
+      rind_inject(G, nx, ny, nz, minj, v[1], pl, fs, ri, ui, vi, wi, Hi);
+      flx[0] = -minj * nm;
+      flx[1] = (-minj * ui + pl * nx) * nm;
+      flx[2] = (-minj * vi + pl * ny) * nm;
+      flx[3] = (-minj * wi + pl * nz) * nm;
+      flx[4] = (-minj * Hi + pl * fs) * nm;
+    } else {
+      const double pw = rind_slipwall(G, nx, ny, nz, rl, rul, rvl, rwl, fs, pl);
+      flx[0] = 0.0;
+      flx[1] = pw * nx * nm;
+      flx[2] = pw * ny * nm;
+      flx[3] = pw * nz * nm;
+      flx[4] = pw * fs * nm;
+    }
+    return;
+  }
+  const int opt = P.opt[p];
+  double pr, bs[5];
+  if (bc == BC_INFLOW_VELTEMP) {
+    pr = inflow_veltemp<double>(opt, v, rl, G.rgasp, G.gp, bs);
+  } else if (bc == BC_INFLOW_TOTANG) {
+    pr = bcond_inflow_totang(opt, P.opt2[p], v, nx, ny, nz, G, rl, rul, rvl, rwl, bs);
+  } else if (bc == BC_FARFIELD) {
+    pr = rind_farfield(G, v, nx, ny, nz, rl, rul, rvl, rwl, rel, bs);
+  } else {
+    pr = opt != BCOPT_SUPERSONIC ? P.pout[p] : pl;
+    bcond_outflow(opt, pr, nx, ny, nz, G.gp, rl, rul, rvl, rwl, rel, pl, bs);
+  }
+  const double ql = (rul * nx + rvl * ny + rwl * nz) / rl;
+  const double qr = (bs[1] * nx + bs[2] * ny + bs[3] * nz) / bs[0];
+  flx[0] = 0.5 * (ql * rl + qr * bs[0]) * nm;
+  flx[1] = 0.5 * (ql * rul + pl * nx + qr * bs[1] + pr * nx) * nm;
+  flx[2] = 0.5 * (ql * rvl + pl * ny + qr * bs[2] + pr * ny) * nm;
+  flx[3] = 0.5 * (ql * rwl + pl * nz + qr * bs[3] + pr * nz) * nm;
+  flx[4] = 0.5 * (ql * (rel + pl) + qr * (bs[4] + pr)) * nm;
+}
+
+// Spectral radius of boundary face b.  Injection faces
+// (rocflu/RFLU_TimeStepInviscid.F90:214-260) count the injected gas's normal velocity
+// and its sound speed at the injection temperature when burning, and the cell's sound
+// speed alone otherwise; every other patch the one-sided cell term.
+template <int ORDER>
+__device__ __forceinline__ double bface_srad(const Mesh& M, const State& S, const Gas& G,
+                                             const Patches& P, int b) {
+  const int p = M.bpatch[b];
+  if (P.bc[p] == BC_INJECTION) {
+    const double* v = P.vals + 5 * (size_t)p;
+    const double4 n = M.bfn[b];
+    const int c = M.bf2c[b];
+    const double minj = v[0];
+    double vc, a1;
+    if (minj > 0.0) {
+      double rl, ul, vl, wl, Hl;
+      rind_inject(G, n.x, n.y, n.z, minj, v[1], S.dv[c], 0.0, rl, ul, vl, wl, Hl);
+      vc = ul * n.x + vl * n.y + wl * n.z;
+      a1 = sqrt(G.gp * G.rgasp * v[1]);
+    } else {
+      vc = 0.0;
+      a1 = S.dv[2 * (size_t)M.N + c];
+    }
+    return (fabs(vc) + a1) * n.w;
+  }
+  return face_srad<ORDER>(M, S, M.F + b);
+}
+
 // Boundary faces (rocflu/RFLU_CentralFirstPatch.F90:571-906,
 // rocflu/RFLU_CentralSecondPatch.F90:634-1024).  Slip walls store a zero mass flux.
 // Boundary fluxes stay FP64 in every precision mode (they are few).
@@ -508,7 +762,6 @@ __device__ __forceinline__ void boundary_face_flux(const Mesh& M, const State& S
   int p = M.bpatch[b];
   double4 n = M.bfn[b];
   double nx = n.x, ny = n.y, nz = n.z, nm = n.w;
-  int bc = P.bc[p];
   double rl, rul, rvl, rwl, rel, pl;
   bool o2 = ORDER == 2 && P.order2[p];
   if (!o2) {
@@ -551,29 +804,7 @@ __device__ __forceinline__ void boundary_face_flux(const Mesh& M, const State& S
     rwl = rl * wl;
     rel = rl * (pl / (rl * (G.gp - 1.0)) + 0.5 * (ul * ul + vl * vl + wl * wl));
   }
-  if (bc == BC_SLIPWALL) {
-    flx[0] = 0.0;
-    flx[1] = pl * nx * nm;
-    flx[2] = pl * ny * nm;
-    flx[3] = pl * nz * nm;
-    flx[4] = 0.0;
-  } else {
-    int opt = P.opt[p];
-    double pr, bs[5];
-    if (bc == BC_INFLOW_VELTEMP) {
-      pr = inflow_veltemp<double>(opt, P.vals + 5 * (size_t)p, rl, G.rgasp, G.gp, bs);
-    } else {
-      pr = opt != BCOPT_SUPERSONIC ? P.pout[p] : pl;
-      bcond_outflow(opt, pr, nx, ny, nz, G.gp, rl, rul, rvl, rwl, rel, pl, bs);
-    }
-    double ql = (rul * nx + rvl * ny + rwl * nz) / rl;
-    double qr = (bs[1] * nx + bs[2] * ny + bs[3] * nz) / bs[0];
-    flx[0] = 0.5 * (ql * rl + qr * bs[0]) * nm;
-    flx[1] = 0.5 * (ql * rul + pl * nx + qr * bs[1] + pr * nx) * nm;
-    flx[2] = 0.5 * (ql * rvl + pl * ny + qr * bs[2] + pr * ny) * nm;
-    flx[3] = 0.5 * (ql * rwl + pl * nz + qr * bs[3] + pr * nz) * nm;
-    flx[4] = 0.5 * (ql * (rel + pl) + qr * (bs[4] + pr)) * nm;
-  }
+  patch_face_flux(G, P, p, nx, ny, nz, nm, rl, rul, rvl, rwl, rel, pl, flx);
 }
 
 template <int ORDER, typename GT, typename FT, bool SRAD>
@@ -583,8 +814,8 @@ __global__ void k_bface_flux(Mesh M, State S, Gas G, Cases C, Patches P,
   if (gate(sc)) return;
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= M.B) return;
-  if (SRAD) srad[M.F + b] = face_srad<ORDER>(M, S, M.F + b);
   if (C.cid) G = C.gas[C.cid[M.bf2c[b]]];
+  if (SRAD) srad[M.F + b] = bface_srad<ORDER>(M, S, G, P, b);
   double flx[5];
   boundary_face_flux<ORDER, GT>(M, S, G, P, grad, b, flx);
   const size_t FB = (size_t)M.F + M.B;
@@ -1463,12 +1694,17 @@ __device__ __forceinline__ double wave_min(double v) {
 }
 
 // Face spectral radii as a pass of their own (the unfused time-step entry point).
-__global__ void k_face_srad(Mesh M, State S, double* __restrict__ srad,
-                            const Scalars* __restrict__ sc) {
+__global__ void k_face_srad(Mesh M, State S, Gas G, Cases C, Patches P,
+                            double* __restrict__ srad, const Scalars* __restrict__ sc) {
   if (gate(sc)) return;
   int f = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
   if (f >= M.F + M.B) return;
-  srad[f] = face_srad<1>(M, S, f);
+  if (f < M.F) {
+    srad[f] = face_srad<1>(M, S, f);
+    return;
+  }
+  if (C.cid) G = C.gas[C.cid[M.bf2c[f - M.F]]];
+  srad[f] = bface_srad<1>(M, S, G, P, f - M.F);
 }
 
 // Sum of the spectral radii of cell i's faces in cell-to-face order (all loads of a

@@ -593,8 +593,10 @@ int build_box(rflu_mesh& m, int kind, int nx, int ny, int nz, double lx, double 
   m.patch_pout.assign(bc_pout, bc_pout + 6);
   for (int s = 0; s < 6; ++s)
     if (bc_type[s] != RFLU_BC_OUTFLOW && bc_type[s] != RFLU_BC_SLIPWALL &&
-        bc_type[s] != RFLU_BC_INFLOW_VELTEMP) {
-      g_err = "only inflow (11), outflow (20) and slip-wall (60) patches are supported";
+        bc_type[s] != RFLU_BC_INFLOW_VELTEMP && bc_type[s] != RFLU_BC_INFLOW_TOTANG &&
+        bc_type[s] != RFLU_BC_FARFIELD && bc_type[s] != RFLU_BC_INJECTION) {
+      g_err = "patch types: inflow (10, 11), outflow (20), slip wall (60), farfield (80), "
+              "injection (90)";
       return 4;
     }
 

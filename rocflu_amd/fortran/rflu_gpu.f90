@@ -40,6 +40,7 @@ MODULE rflu_gpu
     TYPE(C_PTR) :: c2cs_bptr, c2cs_bface
     REAL(C_DOUBLE) :: creconst_weight
     INTEGER(C_INT32_T) :: creconst, pad_
+    TYPE(C_PTR) :: patch_opt2
   END TYPE rflu_region_desc
 
   TYPE, BIND(C) :: rflu_gpu_stats

@@ -19,9 +19,12 @@ _LIB = None
 
 HEX = 0
 TET6 = 1
+BC_INFLOW_TOTANG = 10
 BC_INFLOW_VELTEMP = 11
 BC_OUTFLOW = 20
 BC_SLIPWALL = 60
+BC_FARFIELD = 80
+BC_INJECTION = 90
 BCOPT_SUPERSONIC = 0
 BCOPT_SUBSONIC = 1
 BCOPT_MIXED = 2
@@ -93,8 +96,9 @@ def box(kind: int, nx: int, ny: int, nz: int, lengths=(1.0, 1.0, 1.0),
     """Build a jittered box region of hexahedra (``HEX``) or Kuhn tetrahedra (``TET6``).
 
     Patches are x-, x+, y-, y+, z-, z+; ``bc_type`` uses the reference codes
-    (20 outflow, 60 slip wall), ``bc_opt`` the outflow options (0 supersonic,
-    1 subsonic, 2 mixed) and ``bc_pout`` the outflow static pressure.
+    (10 and 11 inflow, 20 outflow, 60 slip wall, 80 farfield, 90 injection),
+    ``bc_opt`` the inflow/outflow options (0 supersonic, 1 subsonic, 2 mixed) and
+    ``bc_pout`` the outflow static pressure.
     """
     lib = _lib()
     t = (ctypes.c_int32 * 6)(*[int(x) for x in bc_type])

@@ -47,8 +47,13 @@ class Numerics:
     gamma: float = 1.4
     cp: float = 1004.5
     patch_order: Optional[Sequence[int]] = None
-    # per patch (u, v, w, T, p) of inflow patches (BCDAT_INFLOW_U.._P); zeros elsewhere
+    # per patch the five BCDAT values of its type: inflow 11 (u, v, w, T, p), inflow 10
+    # (ptot, ttot, betah, betav, Mach), farfield (Mach, attack, sideslip, p, T),
+    # injection (mass flux, T, -, -, -); zeros elsewhere
     patch_vals: Optional[Sequence[Sequence[float]]] = None
+    # per patch BCSWI_INFLOW_FIXED of inflow 10 patches (0 or 1), or BCSWI_FARF_CORR of
+    # farfield patches (0 only); None = all 0
+    patch_opt2: Optional[Sequence[int]] = None
     # constrained gradients at walls (CRECONSTC, CRECONSTCW; RFLU_InitInputValues.F90:
     # 188-189 defaults 0 and 1.0)
     creconst: int = 0
@@ -87,6 +92,8 @@ def region_arrays(mesh: Mesh, num: Numerics) -> dict:
     if num.patch_vals is not None:
         pv[:] = np.asarray(num.patch_vals, dtype=np.float64).reshape(-1, 5)
     keep["patch_vals"] = np.ascontiguousarray(pv.reshape(-1))
+    o2 = num.patch_opt2 if num.patch_opt2 is not None else [0] * int(mesh.n_patches)
+    keep["patch_opt2"] = np.ascontiguousarray(o2, dtype=np.int32)
     if num.order == 2:
         if not mesh.has_stencils:
             raise ValueError("second-order runs need a mesh built with stencils=True")
@@ -136,6 +143,7 @@ def region_desc(mesh: Mesh, num: Numerics, d: "nat.RegionDesc"):
         d.c2cs_bface = nat.ptr(k["c2cs_bface"], i32)
     d.creconst_weight = num.creconst_weight
     d.creconst = num.creconst if num.order == 2 else 0
+    d.patch_opt2 = nat.ptr(k["patch_opt2"], i32)
     return k
 
 

@@ -47,3 +47,40 @@ def test_create_region_rejects_bad_descriptor():
     rc = lib.rflu_gpu_create_region(ctypes.byref(d), ctypes.byref(h))
     assert rc == _native.RFLU_ERR_ARG
     assert lib.rflu_gpu_destroy_region(12345) == _native.RFLU_ERR_HANDLE
+
+
+def test_create_region_checks_patch_arguments():
+    """The descriptor checks of the total-condition inflow, farfield and injection
+    patches run before any device call, so they report here without a GPU."""
+    import pytest
+    from rocflu_amd import _native, mesh as rmesh
+    from rocflu_amd.solver import GpuRegion, Numerics
+    S = rmesh.BC_SLIPWALL
+    rg = Numerics().gas_prime()[1]
+
+    def code(bc, vals, opt=0, opt2=0, order=1, creconst=0):
+        m = rmesh.box(rmesh.HEX, 3, 3, 3, bc_type=(bc,) + (S,) * 5,
+                      bc_opt=(opt, 0, 0, 0, 0, 0), stencils=order == 2)
+        num = Numerics(order=order, reconst=0, patch_vals=[vals] + [[0.0] * 5] * 5,
+                       patch_opt2=[opt2, 0, 0, 0, 0, 0], creconst=creconst)
+        with pytest.raises(_native.RfluError) as e:
+            GpuRegion(m, num)
+        return e.value.code
+
+    ARG = _native.RFLU_ERR_ARG
+    # farfield: vortex correction, non-positive pressure or temperature
+    assert code(rmesh.BC_FARFIELD, [0.5, 0, 0, 1.0, 1.0 / rg], opt2=1) == ARG
+    assert code(rmesh.BC_FARFIELD, [0.5, 0, 0, 0.0, 1.0 / rg]) == ARG
+    # total-condition inflow: supersonic option without a Mach number, bad switches
+    assert code(rmesh.BC_INFLOW_TOTANG, [2.0, 1.0, 0, 0, 0.0], opt=0) == ARG
+    assert code(rmesh.BC_INFLOW_TOTANG, [2.0, 1.0, 0, 0, 0.0], opt=1, opt2=2) == ARG
+    assert code(rmesh.BC_INFLOW_TOTANG, [2.0, 1.0, 0, 0, 0.0], opt=3) == ARG
+    # injection: a burning patch needs a temperature; no constrained gradients
+    assert code(rmesh.BC_INJECTION, [1.0, 0.0, 0, 0, 0]) == ARG
+    assert code(rmesh.BC_INJECTION, [0.0, 0.0, 0, 0, 0], order=2, creconst=1) == ARG
+    # valid descriptors get past the checks and stop only at the missing device
+    if _native.lib().rflu_gpu_device_count() == 0:
+        assert code(rmesh.BC_FARFIELD, [0.5, 0, 0, 1.0, 1.0 / rg]) == _native.RFLU_ERR_HIP
+        assert code(rmesh.BC_INFLOW_TOTANG, [2.0, 1.0, 0, 0, 0.0], opt=1, opt2=1) == \
+            _native.RFLU_ERR_HIP
+        assert code(rmesh.BC_INJECTION, [0.0, 0.0, 0, 0, 0], order=2) == _native.RFLU_ERR_HIP
