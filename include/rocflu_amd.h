@@ -251,14 +251,19 @@ int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
  * block ranges (1 = hardware order; 8 on MI355X), whether the stencil weights are
  * formed once and streamed from HBM (1; 3 weights per stencil entry, 24 B in FP64,
  * 12 B with FP32 gradients; needs LDS tiles) or re-formed in every gradient pass (0),
- * and how many cells a tile holds (128, 256 or 512, one thread each; the default 256;
- * setting it rebuilds the tiles on the host).  None changes results. */
+ * how many cells a tile holds (128, 256 or 512, one thread each; the default 256;
+ * setting it rebuilds the tiles on the host), and whether the second-order interior
+ * flux stages tiles of 64, 128 or 256 cells (with the cells their faces reach) in LDS
+ * and evaluates each tile's faces from there (FACE_TILE; 0, the default, is one
+ * thread per face gathering its two cells from global memory).  None changes
+ * results. */
 #define RFLU_OPT_GRAD_LDS 1
 #define RFLU_OPT_GRAD_UNROLL 2
 #define RFLU_OPT_TILE_CAP 3
 #define RFLU_OPT_XCD 4
 #define RFLU_OPT_GRAD_WEIGHTS 5
 #define RFLU_OPT_GRAD_TILE 6
+#define RFLU_OPT_FACE_TILE 7
 int rflu_gpu_set_option(int32_t handle, int32_t option, int32_t value);
 int rflu_gpu_get_option(int32_t handle, int32_t option, int32_t* value);
 

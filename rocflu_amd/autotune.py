@@ -56,6 +56,23 @@ def apply(g: GpuRegion, choice: dict):
         g.set_option(GpuRegion.OPT_GRAD_LDS, choice["grad_lds"])
         g.set_option(GpuRegion.OPT_GRAD_UNROLL, choice["grad_unroll"])
         g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, choice["grad_weights"])
+        _set_face_tile(g, choice.get("face_tile", 0))
+
+
+def _set_face_tile(g: GpuRegion, ts: int) -> bool:
+    """Sets the face tiles of the second-order flux (False if they do not fit LDS)."""
+    if g.get_option(GpuRegion.OPT_FACE_TILE) == ts:
+        return True
+    try:
+        g.set_option(GpuRegion.OPT_FACE_TILE, ts)
+    except nat.RfluError as e:
+        if e.code in (nat.RFLU_ERR_ARG, nat.RFLU_ERR_NOMEM):
+            return False
+        raise
+    return True
+
+
+FACE_TILES = (0, 128, 64, 256)
 
 
 def _set_tile(g: GpuRegion, ts: int) -> bool:
@@ -107,7 +124,7 @@ def tune(g: GpuRegion, reps: int = 10, allow_weights: bool = True, use_cache: bo
             apply(g, _cache[sig])
             return dict(_cache[sig], cached=True)
     choice = dict(xcd=8, face_block=256, cell_block=256, grad_tile=256, grad_lds=0,
-                  grad_unroll=4, grad_weights=0)
+                  grad_unroll=4, grad_weights=0, face_tile=0)
     timings = {}
     # XCD-aware block order and block sizes of the face and cell kernels
     best = None
@@ -123,6 +140,17 @@ def tune(g: GpuRegion, reps: int = 10, allow_weights: bool = True, use_cache: bo
     g.set_option(GpuRegion.OPT_XCD, choice["xcd"])
     g.set_launch(face_block=choice["face_block"], cell_block=choice["cell_block"])
     if g.num.order == 2:
+        # interior flux: one thread per face, or face tiles staged in LDS
+        best = None
+        for ft in FACE_TILES:
+            if not _set_face_tile(g, ft):
+                continue
+            t = _time(g, K_FACE, reps)
+            timings[f"face_tile{ft}"] = t
+            if best is None or t < best[0]:
+                best = (t, ft)
+        choice["face_tile"] = best[1]
+        _set_face_tile(g, best[1])
         best = None
         for cand in gradient_candidates(g, allow_weights):
             if g.get_option(GpuRegion.OPT_GRAD_TILE) != cand["grad_tile"]:

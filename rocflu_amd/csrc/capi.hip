@@ -95,6 +95,10 @@ struct Region {
   void* gw = nullptr;         // precomputed stencil weights [3][tile_loc_len] (GRAD_WEIGHTS)
   int n_tiles = 0, tile_cap = 0;
   int grad_tile = 256;        // tuning: cells per LDS tile (128, 256 or 512)
+  // face tiles of the second-order interior flux (FACE_TILE; 0 = one thread per face)
+  int face_tile = 0, n_face_tiles = 0, face_cap = 0;
+  int *ft_fptr = nullptr, *ft_eptr = nullptr, *ft_eidx = nullptr;
+  unsigned* ft_fslot = nullptr;
   bool use_lds = false;       // tiles were built (every tile fits in LDS)
   bool grad_lds = true;       // tuning: stage stencils in LDS when tiles exist
   int grad_unroll = 4;        // tuning: stencil members whose weights are formed together
@@ -184,6 +188,7 @@ void free_region(Region* R) {
                   R->c2f_idx, R->sten_ptr, R->sten_idx, R->moms, R->epsq, R->cv, R->cv_old,
                   R->rhs_sum, R->dv, R->prim, R->grad, R->grad2, R->flux, R->rhs, R->dt_part,
                   R->sc, R->tile_uptr, R->tile_uidx, R->tile_lptr, R->tile_loc, R->tile_vloc,
+                  R->ft_fptr, R->ft_eptr, R->ft_eidx, R->ft_fslot,
                   R->vnb, R->vgeo, R->srad, R->gw, R->cid, R->gas_tab, R->par, R->cell0,
                   R->bface0, R->dtmin, R->pbc, R->popt, R->popt2, R->porder2, R->ppout, R->pvals,
                   R->bptr, R->adj_g, R->adj_gb, R->adj_gv, R->adj_pv, R->adj_dflux,
@@ -439,6 +444,12 @@ void build_limiter_geometry(Region* R, const std::vector<int>& c2f_ptr,
 
 // ---- kernel dispatch ------------------------------------------------------------
 
+// LDS bytes of a face tile's records (cofg, primitive state, [sound speed,] gradient).
+template <typename GT>
+size_t face_tile_lds(int cap, bool srad) {
+  return (size_t)cap * ((srad ? 9 : 8) * sizeof(double) + 15 * sizeof(GT));
+}
+
 template <typename T, typename GT, typename FT, bool SR>
 int launch_fluxes_t(Region* R) {
   Mesh M = R->mesh();
@@ -446,6 +457,20 @@ int launch_fluxes_t(Region* R) {
   const GT* g = (const GT*)R->grad_final();
   FT* fl = (FT*)R->flux;
   dim3 fb(R->face_block);
+  if (R->F > 0 && R->order == 2 && R->face_tile > 0) {
+    FaceTiles FTl;
+    FTl.fptr = R->ft_fptr; FTl.eptr = R->ft_eptr; FTl.eidx = R->ft_eidx; FTl.fslot = R->ft_fslot;
+    FTl.ts = R->face_tile; FTl.cap = R->face_cap;
+    const size_t shm = face_tile_lds<GT>(R->face_cap, SR);
+#define RFLU_FACE_TILE(D)                                                                          \
+  hipLaunchKernelGGL((k_face_flux_tile<D, T, GT, FT, SR>), dim3(R->n_face_tiles), dim3(256), shm, \
+                     R->stream, M, S, R->gas, R->cases(), FTl, g, fl, R->srad, R->gate_sc())
+    if (R->discr == 1) RFLU_FACE_TILE(1);
+    else if (R->discr == 3) RFLU_FACE_TILE(3);
+    else RFLU_FACE_TILE(4);
+#undef RFLU_FACE_TILE
+    return 0;
+  }
   if (R->F > 0) {
 #define RFLU_FACE(D, O)                                                                            \
   hipLaunchKernelGGL((k_face_flux<D, O, T, GT, FT, SR>), dim3(nblk(R->F, R->face_block)), fb, 0, \
@@ -1666,6 +1691,81 @@ int form_weights(Region* R) {
 // device, the tables are rebuilt on the host and replace the old ones (and the stored
 // weights, whose layout follows the tiles).  Tiles that would not fit LDS leave the
 // region as it was.
+// Face tiles of ts cells (0: none).  Needs the faces ordered by their lower cell, as
+// the renumbering leaves them; a tile's records must fit 64 KiB of LDS so that
+// several blocks share a CU.
+void free_face_tiles(Region* R) {
+  void* old[] = {R->ft_fptr, R->ft_eptr, R->ft_eidx, R->ft_fslot};
+  for (void* b : old)
+    if (b) (void)hipFree(b);
+  R->ft_fptr = R->ft_eptr = R->ft_eidx = nullptr;
+  R->ft_fslot = nullptr;
+  R->face_tile = R->n_face_tiles = R->face_cap = 0;
+}
+
+int set_face_tiles(Region* R, int ts) {
+  if (ts == 0) {
+    free_face_tiles(R);
+    return 0;
+  }
+  const int N = R->N, F = R->F;
+  std::vector<int2> f2c(F);
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  if (F > 0) HIP_TRY(hipMemcpy(f2c.data(), R->f2c, F * sizeof(int2), hipMemcpyDeviceToHost));
+  const int nt = (N + ts - 1) / ts;
+  std::vector<int> fptr(nt + 1, 0), eptr(nt + 1, 0), eidx;
+  std::vector<unsigned> fslot(F);
+  int prev = 0;
+  for (int f = 0; f < F; ++f) {
+    const int lo = std::min(f2c[f].x, f2c[f].y) / ts;
+    if (lo < prev) return fail(RFLU_ERR_ARG, "faces are not ordered by their lower cell");
+    prev = lo;
+    fptr[lo + 1]++;
+  }
+  for (int t = 0; t < nt; ++t) fptr[t + 1] += fptr[t];
+  int cap = 0;
+  std::vector<int> ext;
+  for (int t = 0; t < nt; ++t) {
+    const int c0 = t * ts, nown = std::min(ts, N - c0);
+    ext.clear();
+    for (int f = fptr[t]; f < fptr[t + 1]; ++f) {
+      const int hi = std::max(f2c[f].x, f2c[f].y);
+      if (hi >= c0 + nown) ext.push_back(hi);
+    }
+    std::sort(ext.begin(), ext.end());
+    ext.erase(std::unique(ext.begin(), ext.end()), ext.end());
+    auto slot = [&](int c) {
+      if (c >= c0 && c < c0 + nown) return c - c0;
+      return nown + (int)(std::lower_bound(ext.begin(), ext.end(), c) - ext.begin());
+    };
+    for (int f = fptr[t]; f < fptr[t + 1]; ++f)
+      fslot[f] = (unsigned)slot(f2c[f].x) | ((unsigned)slot(f2c[f].y) << 16);
+    eidx.insert(eidx.end(), ext.begin(), ext.end());
+    eptr[t + 1] = (int)eidx.size();
+    cap = std::max(cap, nown + (int)ext.size());
+  }
+  const bool fp32 = R->prec == RFLU_PREC_MIXED || R->prec == RFLU_PREC_FP32_GRAD;
+  const size_t bytes = fp32 ? face_tile_lds<float>(cap, true) : face_tile_lds<double>(cap, true);
+  if (cap > 65535 || bytes > 64 * 1024)
+    return fail(RFLU_ERR_ARG, "face tiles of that size do not fit LDS");
+  free_face_tiles(R);
+  int rc;
+  if ((rc = dalloc(&R->ft_fptr, fptr.size())) || (rc = dalloc(&R->ft_eptr, eptr.size())) ||
+      (rc = dalloc(&R->ft_eidx, std::max<size_t>(eidx.size(), 1))) ||
+      (rc = dalloc(&R->ft_fslot, std::max<size_t>(fslot.size(), 1))) ||
+      (rc = dput(R->ft_fptr, fptr, R->stream)) || (rc = dput(R->ft_eptr, eptr, R->stream)) ||
+      (!eidx.empty() && (rc = dput(R->ft_eidx, eidx, R->stream))) ||
+      (!fslot.empty() && (rc = dput(R->ft_fslot, fslot, R->stream)))) {
+    free_face_tiles(R);
+    return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  R->face_tile = ts;
+  R->n_face_tiles = nt;
+  R->face_cap = cap;
+  return 0;
+}
+
 int rebuild_tiles(Region* R, int ts) {
   const int N = R->N;
   const bool venkat = R->reconst == RFLU_RECONST_LIM_VENKAT;
@@ -1755,6 +1855,15 @@ int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
       if (rc) return rc;
       break;
     }
+    case RFLU_OPT_FACE_TILE: {
+      if (value != 0 && value != 64 && value != 128 && value != 256)
+        return fail(RFLU_ERR_ARG, "FACE_TILE is 0, 64, 128 or 256");
+      if (value != 0 && R->order != 2) return fail(RFLU_ERR_ARG, "FACE_TILE needs a second-order region");
+      if (value == R->face_tile) break;
+      int rc = set_face_tiles(R, value);
+      if (rc) return rc;
+      break;
+    }
     case RFLU_OPT_GRAD_TILE: {
       if (value != 128 && value != 256 && value != 512)
         return fail(RFLU_ERR_ARG, "GRAD_TILE is 128, 256 or 512");
@@ -1782,6 +1891,7 @@ int rflu_gpu_get_option(int32_t h, int32_t option, int32_t* value) {
     case RFLU_OPT_XCD: *value = R->nxcd; break;
     case RFLU_OPT_GRAD_WEIGHTS: *value = R->gw ? 1 : 0; break;
     case RFLU_OPT_GRAD_TILE: *value = R->grad_tile; break;
+    case RFLU_OPT_FACE_TILE: *value = R->face_tile; break;
     default: return fail(RFLU_ERR_ARG, "unknown option");
   }
   return RFLU_OK;

@@ -316,26 +316,34 @@ __device__ __forceinline__ void ausm(T nx, T ny, T nz, T nm, T rl, T ul, T vl, T
 // Face state of cell c for a second-order flux: the flux loop reads cv after
 // CellGradientsMP's cons->prim->cons round trip, so momentum is rho*(u) with
 // u = (1/rho)*m, and velocities are that momentum times 1/rho again.
+template <typename T, typename GV>
+__device__ __forceinline__ void recon_values(double rd, double pu, double pv, double pw, double pd,
+                                             double cx, double cy, double cz, GV gv, double xf,
+                                             double yf, double zf, T& r, T& u, T& v, T& w, T& p,
+                                             T& ir) {
+  double ird = 1.0 / rd;
+  double ud = (rd * pu) * ird;
+  double vd = (rd * pv) * ird;
+  double wd = (rd * pw) * ird;
+  T dx = T(xf - cx), dy = T(yf - cy), dz = T(zf - cz);
+  r = T(rd) + T(gv(0)) * dx + T(gv(1)) * dy + T(gv(2)) * dz;
+  u = T(ud) + T(gv(3)) * dx + T(gv(4)) * dy + T(gv(5)) * dz;
+  v = T(vd) + T(gv(6)) * dx + T(gv(7)) * dy + T(gv(8)) * dz;
+  w = T(wd) + T(gv(9)) * dx + T(gv(10)) * dy + T(gv(11)) * dz;
+  p = T(pd) + T(gv(12)) * dx + T(gv(13)) * dy + T(gv(14)) * dz;
+  ir = T(ird);
+}
+
 template <typename T, typename GT>
 __device__ __forceinline__ void recon_state(const State& S, const Mesh& M, const GT* grad, int c,
                                             double xf, double yf, double zf, T& r, T& u, T& v,
                                             T& w, T& p, T& ir) {
   const int N = M.N;
-  double rd = S.prim[c];
-  double ird = 1.0 / rd;
-  double ud = (rd * S.prim[N + c]) * ird;
-  double vd = (rd * S.prim[2 * N + c]) * ird;
-  double wd = (rd * S.prim[3 * N + c]) * ird;
-  double pd = S.prim[4 * N + c];
-  double4 cc = M.cell[c];
-  T dx = T(xf - cc.x), dy = T(yf - cc.y), dz = T(zf - cc.z);
+  const double4 cc = M.cell[c];
   const GT* g = grad + c;
-  r = T(rd) + T(g[0]) * dx + T(g[(size_t)N]) * dy + T(g[2 * (size_t)N]) * dz;
-  u = T(ud) + T(g[3 * (size_t)N]) * dx + T(g[4 * (size_t)N]) * dy + T(g[5 * (size_t)N]) * dz;
-  v = T(vd) + T(g[6 * (size_t)N]) * dx + T(g[7 * (size_t)N]) * dy + T(g[8 * (size_t)N]) * dz;
-  w = T(wd) + T(g[9 * (size_t)N]) * dx + T(g[10 * (size_t)N]) * dy + T(g[11 * (size_t)N]) * dz;
-  p = T(pd) + T(g[12 * (size_t)N]) * dx + T(g[13 * (size_t)N]) * dy + T(g[14 * (size_t)N]) * dz;
-  ir = T(ird);
+  recon_values<T>(S.prim[c], S.prim[N + c], S.prim[2 * N + c], S.prim[3 * N + c],
+                  S.prim[4 * N + c], cc.x, cc.y, cc.z,
+                  [&](int k) { return g[(size_t)k * N]; }, xf, yf, zf, r, u, v, w, p, ir);
 }
 
 // Spectral radius (|v.n| + c)|A| of every face, interior faces first, then boundary
@@ -383,6 +391,38 @@ __device__ __forceinline__ double face_srad(const Mesh& M, const State& S, int f
   return srad_boundary(u1, v1, w1, S.dv[2 * (size_t)N + c], M.bfn[b]);
 }
 
+// Second-order interior flux from the two reconstructed face states (the DISCR
+// dispatch of rocflu/RFLU_ComputeFluxInv.F90:123-140).
+template <int DISCR, typename T>
+__device__ __forceinline__ void face_flux2(const Gas& G, T nx, T ny, T nz, T nm, T rl, T ul, T vl,
+                                           T wl, T pl, T irl, T rr, T ur, T vr, T wr, T pr,
+                                           T irr, T* flx) {
+  const T g = T(G.g), cp = T(G.cp), gc = T(G.gc), one = T(1), half = T(0.5);
+  if (DISCR == 3) {
+    // RFLU_ModHLLCFlux.F90:2262-2267: E of the face state, H with the cell 1/rho.
+    T el = pl / (rl * (g - one)) + half * (ul * ul + vl * vl + wl * wl);
+    T al = tsqrt(g * pl / rl);
+    T Hl = el + irl * pl;
+    T er = pr / (rr * (g - one)) + half * (ur * ur + vr * vr + wr * wr);
+    T ar = tsqrt(g * pr / rr);
+    T Hr = er + irr * pr;
+    hllc<T>(g, nx, ny, nz, nm, rl, ul, vl, wl, pl, al, el, Hl, rr, ur, vr, wr, pr, ar, er, Hr,
+            flx);
+  } else if (DISCR == 1) {
+    T tl = pl / (rl * gc), tr = pr / (rr * gc);
+    T Hl = cp * tl + half * (ul * ul + vl * vl + wl * wl);
+    T Hr = cp * tr + half * (ur * ur + vr * vr + wr * wr);
+    roe<T>(g, T(G.eps_entr), T(G.diss_fact), nx, ny, nz, nm, rl, ul, vl, wl, pl, Hl, rr, ur, vr,
+           wr, pr, Hr, flx);
+  } else {
+    T tl = pl / (rl * gc), tr = pr / (rr * gc);
+    T al = tsqrt(g * gc * tl), ar = tsqrt(g * gc * tr);
+    T Hl = cp * tl + half * (ul * ul + vl * vl + wl * wl);
+    T Hr = cp * tr + half * (ur * ur + vr * vr + wr * wr);
+    ausm<T>(nx, ny, nz, nm, rl, ul, vl, wl, pl, Hl, al, rr, ur, vr, wr, pr, Hr, ar, flx);
+  }
+}
+
 // Flux of interior face f (computed with f2c(1,f) = cc.x as the left state).
 template <int DISCR, int ORDER, typename T, typename GT>
 __device__ __forceinline__ void interior_face_flux(const Mesh& M, const State& S, const Gas& G,
@@ -422,29 +462,7 @@ __device__ __forceinline__ void interior_face_flux(const Mesh& M, const State& S
     T rl, ul, vl, wl, pl, irl, rr, ur, vr, wr, pr, irr;
     recon_state<T, GT>(S, M, grad, c1, xf.x, xf.y, xf.z, rl, ul, vl, wl, pl, irl);
     recon_state<T, GT>(S, M, grad, c2, xf.x, xf.y, xf.z, rr, ur, vr, wr, pr, irr);
-    if (DISCR == 3) {
-      // RFLU_ModHLLCFlux.F90:2262-2267: E of the face state, H with the cell 1/rho.
-      T el = pl / (rl * (g - one)) + half * (ul * ul + vl * vl + wl * wl);
-      T al = tsqrt(g * pl / rl);
-      T Hl = el + irl * pl;
-      T er = pr / (rr * (g - one)) + half * (ur * ur + vr * vr + wr * wr);
-      T ar = tsqrt(g * pr / rr);
-      T Hr = er + irr * pr;
-      hllc<T>(g, nx, ny, nz, nm, rl, ul, vl, wl, pl, al, el, Hl, rr, ur, vr, wr, pr, ar, er, Hr,
-              flx);
-    } else if (DISCR == 1) {
-      T tl = pl / (rl * gc), tr = pr / (rr * gc);
-      T Hl = cp * tl + half * (ul * ul + vl * vl + wl * wl);
-      T Hr = cp * tr + half * (ur * ur + vr * vr + wr * wr);
-      roe<T>(g, T(G.eps_entr), T(G.diss_fact), nx, ny, nz, nm, rl, ul, vl, wl, pl, Hl, rr, ur, vr,
-             wr, pr, Hr, flx);
-    } else {
-      T tl = pl / (rl * gc), tr = pr / (rr * gc);
-      T al = tsqrt(g * gc * tl), ar = tsqrt(g * gc * tr);
-      T Hl = cp * tl + half * (ul * ul + vl * vl + wl * wl);
-      T Hr = cp * tr + half * (ur * ur + vr * vr + wr * wr);
-      ausm<T>(nx, ny, nz, nm, rl, ul, vl, wl, pl, Hl, al, rr, ur, vr, wr, pr, Hr, ar, flx);
-    }
+    face_flux2<DISCR, T>(G, nx, ny, nz, nm, rl, ul, vl, wl, pl, irl, rr, ur, vr, wr, pr, irr, flx);
   }
 }
 
@@ -466,6 +484,93 @@ __global__ void k_face_flux(Mesh M, State S, Gas G, Cases C, const GT* __restric
   const size_t FB = (size_t)M.F + M.B;
 #pragma unroll
   for (int k = 0; k < 5; ++k) flux[k * FB + f] = FT(flx[k]);
+}
+
+// Face tiles of the second-order interior flux.  Faces are ordered by their lower cell,
+// so the faces whose lower cell lies in tile t = [t*ts, t*ts + ts) are the contiguous
+// range [fptr[t], fptr[t+1]).  A tile's records are its own cells (slots 0..ts-1,
+// loaded with consecutive addresses) followed by the higher cells outside the tile
+// that its faces reach (eidx[eptr[t]..eptr[t+1]), ascending); fslot[f] holds the
+// slots of f2c(1,f) and f2c(2,f) in its low and high 16 bits.
+struct FaceTiles {
+  const int* fptr;
+  const int* eptr;
+  const int* eidx;
+  const unsigned* fslot;
+  int ts;
+  int cap;
+};
+
+// Second-order interior fluxes of one face tile per block: the tile's records (cofg,
+// primitive state, gradient; with SRAD also the sound speed) are staged in LDS with
+// coalesced loads, then each thread evaluates faces of the tile from LDS with the
+// arithmetic of k_face_flux, so the results are the same bits.  The scattered
+// per-face gathers of k_face_flux are what bounds it (its address and data paths are
+// busy nearly all the time); here only the records of cells outside the tile are
+// gathered.
+template <int DISCR, typename T, typename GT, typename FT, bool SRAD>
+__global__ __launch_bounds__(256) void k_face_flux_tile(Mesh M, State S, Gas G, Cases C,
+                                                        FaceTiles FTl, const GT* __restrict__ grad,
+                                                        FT* __restrict__ flux,
+                                                        double* __restrict__ srad,
+                                                        const Scalars* __restrict__ sc) {
+  extern __shared__ double lds[];
+  if (gate(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
+  const int N = M.N, cap = FTl.cap;
+  const int t = xcd_block(M.nxcd);
+  const int c0 = t * FTl.ts;
+  const int nown = min(FTl.ts, N - c0);
+  const int e0 = FTl.eptr[t], nrec = nown + (FTl.eptr[t + 1] - e0);
+  constexpr int ND = SRAD ? 9 : 8;   // x y z rho u v w p (c)
+  GT* lg = (GT*)(lds + (size_t)ND * cap);
+  for (int k = threadIdx.x; k < nrec; k += blockDim.x) {
+    const int c = k < nown ? c0 + k : FTl.eidx[e0 + k - nown];
+    const double4 cc = M.cell[c];
+    double pv[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) pv[q] = S.prim[(size_t)q * N + c];
+    GT gv[15];
+#pragma unroll
+    for (int q = 0; q < 15; ++q) gv[q] = grad[(size_t)q * N + c];
+    double a = SRAD ? S.dv[2 * (size_t)N + c] : 0.0;
+    lds[k] = cc.x;
+    lds[cap + k] = cc.y;
+    lds[2 * cap + k] = cc.z;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) lds[(size_t)(3 + q) * cap + k] = pv[q];
+    if (SRAD) lds[(size_t)8 * cap + k] = a;
+#pragma unroll
+    for (int q = 0; q < 15; ++q) lg[(size_t)q * cap + k] = gv[q];
+  }
+  __syncthreads();
+  const size_t FB = (size_t)M.F + M.B;
+  const int f1 = FTl.fptr[t + 1];
+  for (int f = FTl.fptr[t] + (int)threadIdx.x; f < f1; f += blockDim.x) {
+    const unsigned sl = FTl.fslot[f];
+    const int s1 = (int)(sl & 0xFFFFu), s2 = (int)(sl >> 16);
+    const double4 n = M.fn[f];
+    const double4 xf = M.fc[f];
+    Gas Gf = G;
+    if (C.cid) Gf = C.gas[C.cid[M.f2c[f].x]];
+    if (SRAD)
+      srad[f] = srad_interior(lds[4 * cap + s1], lds[5 * cap + s1], lds[6 * cap + s1],
+                              lds[8 * cap + s1], lds[4 * cap + s2], lds[5 * cap + s2],
+                              lds[6 * cap + s2], lds[8 * cap + s2], n);
+    T rl, ul, vl, wl, pl, irl, rr, ur, vr, wr, pr, irr;
+    recon_values<T>(lds[3 * cap + s1], lds[4 * cap + s1], lds[5 * cap + s1], lds[6 * cap + s1],
+                    lds[7 * cap + s1], lds[s1], lds[cap + s1], lds[2 * cap + s1],
+                    [&](int q) { return lg[(size_t)q * cap + s1]; }, xf.x, xf.y, xf.z, rl, ul,
+                    vl, wl, pl, irl);
+    recon_values<T>(lds[3 * cap + s2], lds[4 * cap + s2], lds[5 * cap + s2], lds[6 * cap + s2],
+                    lds[7 * cap + s2], lds[s2], lds[cap + s2], lds[2 * cap + s2],
+                    [&](int q) { return lg[(size_t)q * cap + s2]; }, xf.x, xf.y, xf.z, rr, ur,
+                    vr, wr, pr, irr);
+    T flx[5];
+    face_flux2<DISCR, T>(Gf, T(n.x), T(n.y), T(n.z), T(n.w), rl, ul, vl, wl, pl, irl, rr, ur, vr,
+                         wr, pr, irr, flx);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) flux[k * FB + f] = FT(flx[k]);
+  }
 }
 
 // Outflow boundary state (libfloflu/BcondOutflowPerf.F90:111-157), FP64.

@@ -234,7 +234,10 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
                 dict(lds=1, unroll=1, xcd=8, tile=128, weights=1),
                 dict(lds=1, unroll=2, xcd=1, tile=512, weights=1),
                 # weights formed for 256-cell tiles, then the tiles rebuilt at 128
-                dict(lds=1, unroll=2, xcd=8, weights=1, tile_after=128)]
+                dict(lds=1, unroll=2, xcd=8, weights=1, tile_after=128),
+                # interior fluxes from LDS face tiles
+                dict(lds=1, unroll=4, xcd=8, face_tile=128), dict(lds=1, unroll=4, xcd=1, face_tile=64),
+                dict(lds=0, unroll=2, xcd=8, face_tile=128, tile=128)]
     for v in variants:
         print("variant", v, flush=True)
         g = GpuRegion(m, num)
@@ -249,6 +252,12 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
             if v.get("weights"):
                 g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, 1)
                 assert g.get_option(GpuRegion.OPT_GRAD_WEIGHTS) == 1
+            if "face_tile" in v:
+                with pytest.raises(Exception):   # 256-cell FP64 face tiles exceed 64 KiB here
+                    g.set_option(GpuRegion.OPT_FACE_TILE, 256)
+                assert g.get_option(GpuRegion.OPT_FACE_TILE) == 0
+                g.set_option(GpuRegion.OPT_FACE_TILE, v["face_tile"])
+                assert g.get_option(GpuRegion.OPT_FACE_TILE) == v["face_tile"]
             if "tile_after" in v:
                 g.set_option(GpuRegion.OPT_GRAD_TILE, v["tile_after"])
                 assert g.get_option(GpuRegion.OPT_GRAD_TILE) == v["tile_after"]
@@ -272,6 +281,52 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
             assert np.all(err <= TOL_FP64), err
         else:
             assert np.array_equal(cvg, ref), v
+
+
+@pytest.mark.parametrize("discr,prec", [(1, 0), (4, 0), (3, 1), (4, 1)])
+def test_face_tiles_give_the_same_bits(gpu, discr, prec):
+    """The LDS face-tile flux (FACE_TILE) against the one-thread-per-face kernel, fused
+    and per-callee steps (the fused first stage also stores the face spectral radii),
+    FP64 and mixed precision, Roe and AUSM+ as well as HLLC."""
+    m = small_tet(n=9, bc=rmesh.BC_OUTFLOW)
+    num = Numerics(discr=discr, order=2, reconst=11, rk=4, cfl=0.5, dt_imposed=1e-3,
+                   precision=prec)
+    cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=5)
+    outs = {}
+    for ft in (0, 128, 64):
+        for mode in ("fused", "callee"):
+            print("face_tile", ft, mode, flush=True)
+            g = GpuRegion(m, num)
+            try:
+                g.set_option(GpuRegion.OPT_FACE_TILE, ft)
+                g.upload(cv0)
+                if mode == "fused":
+                    g.time_steps(6)
+                else:
+                    for _ in range(6):
+                        g.runge_kutta()
+                outs[(ft, mode)] = g.download()[0]
+            finally:
+                g.close()
+    for key, cv in outs.items():
+        assert np.array_equal(cv, outs[(0, key[1])]), key
+
+
+def test_face_tiles_in_a_batch(gpu):
+    """Face tiles that straddle the cases of a batch."""
+    ens = cases.ensemble(n_cases=4, min_cells=600, max_cells=2500, seed=21, dt_imposed=1e-3)
+    outs = []
+    for ft in (0, 128):
+        b = GpuBatch([(m, num) for m, num, _ in ens])
+        try:
+            b.set_option(GpuRegion.OPT_FACE_TILE, ft)
+            b.upload([cv0 for _, _, cv0 in ens])
+            b.time_steps(5, graph=True)
+            outs.append(b.download()[0])
+        finally:
+            b.close()
+    for a, c in zip(outs[0], outs[1]):
+        assert np.array_equal(a, c)
 
 
 @pytest.mark.parametrize("order,opt,discr", [(1, 1, 3), (1, 0, 1), (2, 1, 3), (2, 0, 4)])
