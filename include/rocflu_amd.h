@@ -255,7 +255,9 @@ int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
  * setting it rebuilds the tiles on the host), and whether the second-order interior
  * flux stages tiles of 64, 128 or 256 cells (with the cells their faces reach) in LDS
  * and evaluates each tile's faces from there (FACE_TILE; 0, the default, is one
- * thread per face gathering its two cells from global memory).  None changes
+ * thread per face gathering its two cells from global memory), and whether the
+ * update then reads the fluxes of each tile's own faces from an LDS copy
+ * (UPDATE_TILE 1; only while face tiles exist; reads as 0 otherwise).  None changes
  * results. */
 #define RFLU_OPT_GRAD_LDS 1
 #define RFLU_OPT_GRAD_UNROLL 2
@@ -264,6 +266,7 @@ int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
 #define RFLU_OPT_GRAD_WEIGHTS 5
 #define RFLU_OPT_GRAD_TILE 6
 #define RFLU_OPT_FACE_TILE 7
+#define RFLU_OPT_UPDATE_TILE 8
 int rflu_gpu_set_option(int32_t handle, int32_t option, int32_t value);
 int rflu_gpu_get_option(int32_t handle, int32_t option, int32_t* value);
 

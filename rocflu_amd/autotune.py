@@ -57,6 +57,7 @@ def apply(g: GpuRegion, choice: dict):
         g.set_option(GpuRegion.OPT_GRAD_UNROLL, choice["grad_unroll"])
         g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, choice["grad_weights"])
         _set_face_tile(g, choice.get("face_tile", 0))
+        g.set_option(GpuRegion.OPT_UPDATE_TILE, choice.get("update_tile", 0))
 
 
 def _set_face_tile(g: GpuRegion, ts: int) -> bool:
@@ -124,7 +125,7 @@ def tune(g: GpuRegion, reps: int = 10, allow_weights: bool = True, use_cache: bo
             apply(g, _cache[sig])
             return dict(_cache[sig], cached=True)
     choice = dict(xcd=8, face_block=256, cell_block=256, grad_tile=256, grad_lds=0,
-                  grad_unroll=4, grad_weights=0, face_tile=0)
+                  grad_unroll=4, grad_weights=0, face_tile=0, update_tile=0)
     timings = {}
     # XCD-aware block order and block sizes of the face and cell kernels
     best = None
@@ -151,6 +152,17 @@ def tune(g: GpuRegion, reps: int = 10, allow_weights: bool = True, use_cache: bo
                 best = (t, ft)
         choice["face_tile"] = best[1]
         _set_face_tile(g, best[1])
+        if best[1] > 0:
+            # the update over the same tiles, its own faces' fluxes from LDS
+            best = None
+            for ut in (0, 1):
+                g.set_option(GpuRegion.OPT_UPDATE_TILE, ut)
+                t = _time(g, K_UPDATE, reps)
+                timings[f"update_tile{ut}"] = t
+                if best is None or t < best[0]:
+                    best = (t, ut)
+            choice["update_tile"] = best[1]
+            g.set_option(GpuRegion.OPT_UPDATE_TILE, best[1])
         best = None
         for cand in gradient_candidates(g, allow_weights):
             if g.get_option(GpuRegion.OPT_GRAD_TILE) != cand["grad_tile"]:

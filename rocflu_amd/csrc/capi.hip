@@ -96,7 +96,8 @@ struct Region {
   int n_tiles = 0, tile_cap = 0;
   int grad_tile = 256;        // tuning: cells per LDS tile (128, 256 or 512)
   // face tiles of the second-order interior flux (FACE_TILE; 0 = one thread per face)
-  int face_tile = 0, n_face_tiles = 0, face_cap = 0;
+  int face_tile = 0, n_face_tiles = 0, face_cap = 0, face_fcap = 0;
+  bool update_tile = false;   // tuning: the update reads its tile's face fluxes from LDS
   int *ft_fptr = nullptr, *ft_eptr = nullptr, *ft_eidx = nullptr;
   unsigned* ft_fslot = nullptr;
   bool use_lds = false;       // tiles were built (every tile fits in LDS)
@@ -460,7 +461,7 @@ int launch_fluxes_t(Region* R) {
   if (R->F > 0 && R->order == 2 && R->face_tile > 0) {
     FaceTiles FTl;
     FTl.fptr = R->ft_fptr; FTl.eptr = R->ft_eptr; FTl.eidx = R->ft_eidx; FTl.fslot = R->ft_fslot;
-    FTl.ts = R->face_tile; FTl.cap = R->face_cap;
+    FTl.ts = R->face_tile; FTl.cap = R->face_cap; FTl.fcap = R->face_fcap;
     const size_t shm = face_tile_lds<GT>(R->face_cap, SR);
 #define RFLU_FACE_TILE(D)                                                                          \
   hipLaunchKernelGGL((k_face_flux_tile<D, T, GT, FT, SR>), dim3(R->n_face_tiles), dim3(256), shm, \
@@ -571,6 +572,20 @@ int launch_update_rk(Region* R, int stage, int last, double dt_in) {
   dim3 cb(R->cell_block), cg(nblk(R->N, R->cell_block));
   const FT* fl = (const FT*)R->flux;
   const Cases C = R->cases();
+  if (R->update_tile && R->face_tile > 0) {
+    FaceTiles FTl;
+    FTl.fptr = R->ft_fptr; FTl.eptr = R->ft_eptr; FTl.eidx = R->ft_eidx; FTl.fslot = R->ft_fslot;
+    FTl.ts = R->face_tile; FTl.cap = R->face_cap; FTl.fcap = R->face_fcap;
+    const size_t shm = (size_t)5 * R->face_fcap * sizeof(FT);
+    dim3 tb(R->face_tile), tg(R->n_face_tiles);
+    switch (R->rk) {
+      case 1: hipLaunchKernelGGL((k_update_tile<1, FT, FUSED, PRIM>), tg, tb, shm, R->stream, M, S, R->gas, C, FTl, fl, R->sc, stage, last, dt_in); break;
+      case 2: hipLaunchKernelGGL((k_update_tile<2, FT, FUSED, PRIM>), tg, tb, shm, R->stream, M, S, R->gas, C, FTl, fl, R->sc, stage, last, dt_in); break;
+      case 3: hipLaunchKernelGGL((k_update_tile<3, FT, FUSED, PRIM>), tg, tb, shm, R->stream, M, S, R->gas, C, FTl, fl, R->sc, stage, last, dt_in); break;
+      default: hipLaunchKernelGGL((k_update_tile<4, FT, FUSED, PRIM>), tg, tb, shm, R->stream, M, S, R->gas, C, FTl, fl, R->sc, stage, last, dt_in); break;
+    }
+    return 0;
+  }
   switch (R->rk) {
     case 1: hipLaunchKernelGGL((k_update<1, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, C, fl, R->sc, stage, last, dt_in); break;
     case 2: hipLaunchKernelGGL((k_update<2, FT, FUSED, PRIM>), cg, cb, 0, R->stream, M, S, R->gas, C, fl, R->sc, stage, last, dt_in); break;
@@ -1700,7 +1715,7 @@ void free_face_tiles(Region* R) {
     if (b) (void)hipFree(b);
   R->ft_fptr = R->ft_eptr = R->ft_eidx = nullptr;
   R->ft_fslot = nullptr;
-  R->face_tile = R->n_face_tiles = R->face_cap = 0;
+  R->face_tile = R->n_face_tiles = R->face_cap = R->face_fcap = 0;
 }
 
 int set_face_tiles(Region* R, int ts) {
@@ -1723,9 +1738,10 @@ int set_face_tiles(Region* R, int ts) {
     fptr[lo + 1]++;
   }
   for (int t = 0; t < nt; ++t) fptr[t + 1] += fptr[t];
-  int cap = 0;
+  int cap = 0, fcap = 0;
   std::vector<int> ext;
   for (int t = 0; t < nt; ++t) {
+    fcap = std::max(fcap, fptr[t + 1] - fptr[t]);
     const int c0 = t * ts, nown = std::min(ts, N - c0);
     ext.clear();
     for (int f = fptr[t]; f < fptr[t + 1]; ++f) {
@@ -1763,6 +1779,7 @@ int set_face_tiles(Region* R, int ts) {
   R->face_tile = ts;
   R->n_face_tiles = nt;
   R->face_cap = cap;
+  R->face_fcap = fcap;
   return 0;
 }
 
@@ -1855,6 +1872,10 @@ int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
       if (rc) return rc;
       break;
     }
+    case RFLU_OPT_UPDATE_TILE:
+      if (value != 0 && value != 1) return fail(RFLU_ERR_ARG, "UPDATE_TILE is 0 or 1");
+      R->update_tile = value == 1;
+      break;
     case RFLU_OPT_FACE_TILE: {
       if (value != 0 && value != 64 && value != 128 && value != 256)
         return fail(RFLU_ERR_ARG, "FACE_TILE is 0, 64, 128 or 256");
@@ -1892,6 +1913,7 @@ int rflu_gpu_get_option(int32_t h, int32_t option, int32_t* value) {
     case RFLU_OPT_GRAD_WEIGHTS: *value = R->gw ? 1 : 0; break;
     case RFLU_OPT_GRAD_TILE: *value = R->grad_tile; break;
     case RFLU_OPT_FACE_TILE: *value = R->face_tile; break;
+    case RFLU_OPT_UPDATE_TILE: *value = (R->update_tile && R->face_tile > 0) ? 1 : 0; break;
     default: return fail(RFLU_ERR_ARG, "unknown option");
   }
   return RFLU_OK;

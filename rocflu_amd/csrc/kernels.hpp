@@ -498,7 +498,8 @@ struct FaceTiles {
   const int* eidx;
   const unsigned* fslot;
   int ts;
-  int cap;
+  int cap;    // records of the largest tile
+  int fcap;   // faces of the largest tile
 };
 
 // Second-order interior fluxes of one face tile per block: the tile's records (cofg,
@@ -989,6 +990,49 @@ __device__ __forceinline__ void gather_residual(const Mesh& M, const FT* __restr
   }
 }
 
+// gather_residual for a cell of face tile t: the fluxes of the tile's own faces
+// [f0, f0+nf) come from their LDS copy lf ([5][fcap]), the others from memory; the
+// values and the order of the additions are those of gather_residual.
+template <typename FT>
+__device__ __forceinline__ void gather_residual_tile(const Mesh& M, const FT* __restrict__ flux,
+                                                     int i, int f0, int nf, const FT* lf,
+                                                     int fcap, double* acc) {
+  const size_t FB = (size_t)M.F + M.B;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) acc[k] = -0.0;
+  int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
+  for (int e = e0; e < e1; e += 8) {
+    const int cnt = min(8, e1 - e);
+    int sg[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) sg[q] = q < cnt ? M.c2f_idx[e + q] : 1;
+    FT v[8][5];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int f = (sg[q] > 0 ? sg[q] : -sg[q]) - 1;
+      const unsigned l = (unsigned)(f - f0);
+      if (q < cnt && l < (unsigned)nf) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) v[q][k] = lf[k * fcap + (int)l];
+      } else {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) v[q][k] = q < cnt ? flux[k * FB + (size_t)f] : FT(0);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (q >= cnt) break;
+      if (sg[q] > 0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc[k] = acc[k] + double(v[q][k]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc[k] = acc[k] - double(v[q][k]);
+      }
+    }
+  }
+}
+
 // Wavefront-level OR/ADD helpers for the rare after-update events.
 __device__ __forceinline__ void count_event(unsigned long long* ctr) { atomicAdd(ctr, 1ull); }
 
@@ -1138,6 +1182,70 @@ __global__ void k_update(Mesh M, State S, Gas G, Cases C, const FT* __restrict__
     after_update_cell(G, q, p, t, a, sc);
     S.dv[i] = p;
     S.dv[(size_t)N + i] = t;
+    S.dv[2 * (size_t)N + i] = a;
+    if (WRITE_PRIM) {
+      double ir = 1.0 / q[0];
+      S.prim[i] = q[0];
+      S.prim[(size_t)N + i] = ir * q[1];
+      S.prim[2 * (size_t)N + i] = ir * q[2];
+      S.prim[3 * (size_t)N + i] = ir * q[3];
+      S.prim[4 * (size_t)N + i] = p;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) S.cv[(size_t)k * N + i] = q[k];
+}
+
+// k_update over face tiles: a block of ts threads updates the cells of tile t and
+// first copies the fluxes of the tile's own faces (one contiguous range) into LDS with
+// consecutive loads, so that only the faces a cell shares with an earlier tile, and
+// its boundary faces, are gathered from memory.  Same arithmetic, same bits.
+template <int RK, typename FT, bool FUSED, bool WRITE_PRIM>
+__global__ void k_update_tile(Mesh M, State S, Gas G, Cases C, FaceTiles FTl,
+                              const FT* __restrict__ flux, Scalars* sc, int stage,
+                              int last_stage, double dt_in) {
+  extern __shared__ double lds[];
+  if (!C.cid && halted(sc)) return;  // uniform over the grid
+  const int N = M.N;
+  const int t = xcd_block(M.nxcd);
+  const int f0 = FTl.fptr[t], nf = FTl.fptr[t + 1] - f0, fcap = FTl.fcap;
+  FT* lf = (FT*)lds;
+  const size_t FB = (size_t)M.F + M.B;
+  for (int k = threadIdx.x; k < nf; k += blockDim.x) {
+    FT v[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) v[q] = flux[q * FB + (size_t)(f0 + k)];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) lf[q * fcap + k] = v[q];
+  }
+  __syncthreads();
+  const int i = t * FTl.ts + (int)threadIdx.x;
+  if (i >= N) return;
+  int first = 0;
+  if (C.cid) {
+    const int c = C.cid[i];
+    sc = C.sc + c;
+    if (halted(sc)) return;
+    G = C.gas[c];
+    first = C.cell0[c];
+  }
+  double dt = dt_in > 0.0 ? dt_in : sc->dt;
+  if (last_stage && FUSED && i == first) {
+    sc->time = sc->time + dt;
+    sc->steps = sc->steps + 1;
+  }
+  double acc[5];
+  gather_residual_tile<FT>(M, flux, i, f0, nf, lf, fcap, acc);
+  double q[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) q[k] = S.cv[(size_t)k * N + i];
+  rk_update_cell<RK>(S, N, i, stage, dt, M.cell[i].w, acc, q);
+  if (FUSED) {
+    double p, tt, a;
+    dep_vars(G, q[0], q[1], q[2], q[3], q[4], p, tt, a);
+    after_update_cell(G, q, p, tt, a, sc);
+    S.dv[i] = p;
+    S.dv[(size_t)N + i] = tt;
     S.dv[2 * (size_t)N + i] = a;
     if (WRITE_PRIM) {
       double ir = 1.0 / q[0];

@@ -299,6 +299,7 @@ class GpuRegion:
     OPT_GRAD_LDS, OPT_GRAD_UNROLL, OPT_TILE_CAP, OPT_XCD, OPT_GRAD_WEIGHTS = 1, 2, 3, 4, 5
     OPT_GRAD_TILE = 6
     OPT_FACE_TILE = 7
+    OPT_UPDATE_TILE = 8
 
     def set_option(self, option: int, value: int):
         nat.check(self._lib.rflu_gpu_set_option(self.handle, int(option), int(value)),

@@ -285,41 +285,45 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
 
 @pytest.mark.parametrize("discr,prec", [(1, 0), (4, 0), (3, 1), (4, 1)])
 def test_face_tiles_give_the_same_bits(gpu, discr, prec):
-    """The LDS face-tile flux (FACE_TILE) against the one-thread-per-face kernel, fused
-    and per-callee steps (the fused first stage also stores the face spectral radii),
-    FP64 and mixed precision, Roe and AUSM+ as well as HLLC."""
+    """The LDS face-tile flux (FACE_TILE) and the tiled update (UPDATE_TILE) against the
+    one-thread-per-face and one-thread-per-cell kernels, fused and per-callee steps (the
+    fused first stage also stores the face spectral radii), FP64 and mixed precision,
+    Roe and AUSM+ as well as HLLC."""
     m = small_tet(n=9, bc=rmesh.BC_OUTFLOW)
     num = Numerics(discr=discr, order=2, reconst=11, rk=4, cfl=0.5, dt_imposed=1e-3,
                    precision=prec)
     cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=5)
     outs = {}
-    for ft in (0, 128, 64):
+    for ft, ut in ((0, 0), (128, 0), (64, 1), (128, 1)):
         for mode in ("fused", "callee"):
-            print("face_tile", ft, mode, flush=True)
+            print("face_tile", ft, "update_tile", ut, mode, flush=True)
             g = GpuRegion(m, num)
             try:
                 g.set_option(GpuRegion.OPT_FACE_TILE, ft)
+                g.set_option(GpuRegion.OPT_UPDATE_TILE, ut)
+                assert g.get_option(GpuRegion.OPT_UPDATE_TILE) == (1 if ft and ut else 0)
                 g.upload(cv0)
                 if mode == "fused":
                     g.time_steps(6)
                 else:
                     for _ in range(6):
                         g.runge_kutta()
-                outs[(ft, mode)] = g.download()[0]
+                outs[(ft, ut, mode)] = g.download()[0]
             finally:
                 g.close()
     for key, cv in outs.items():
-        assert np.array_equal(cv, outs[(0, key[1])]), key
+        assert np.array_equal(cv, outs[(0, 0, key[2])]), key
 
 
 def test_face_tiles_in_a_batch(gpu):
-    """Face tiles that straddle the cases of a batch."""
+    """Face tiles (and the tiled update) that straddle the cases of a batch."""
     ens = cases.ensemble(n_cases=4, min_cells=600, max_cells=2500, seed=21, dt_imposed=1e-3)
     outs = []
     for ft in (0, 128):
         b = GpuBatch([(m, num) for m, num, _ in ens])
         try:
             b.set_option(GpuRegion.OPT_FACE_TILE, ft)
+            b.set_option(GpuRegion.OPT_UPDATE_TILE, 1)
             b.upload([cv0 for _, _, cv0 in ens])
             b.time_steps(5, graph=True)
             outs.append(b.download()[0])
