@@ -73,9 +73,23 @@ def build_fortran():
     return out
 
 
+def build_checks():
+    """GPU self-checks run by the tests (tests/native)."""
+    hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
+    src = os.path.join(ROOT, "tests", "native", "inv_sqrt_check.hip")
+    if not os.path.exists(src):
+        return None
+    out = src[:-len(".hip")]
+    flags = [f for f in HIPCC_FLAGS if f not in ("-fPIC", "-shared")]
+    _run([hipcc] + flags + ["-I" + INC, "-I" + os.path.join(HERE, "csrc"), src, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(fortran: bool = True):
     build_mesh()
     build_hip()
+    build_checks()
     if fortran:
         build_fortran()
 
