@@ -460,8 +460,9 @@ __device__ __forceinline__ void interior_face_flux(const Mesh& M, const State& S
   int c1 = cc.x, c2 = cc.y;
   double4 n = M.fn[f];
   T nx = T(n.x), ny = T(n.y), nz = T(n.z), nm = T(n.w);
-  const T g = T(G.g), cp = T(G.cp), gc = T(G.gc), one = T(1), half = T(0.5);
   if (ORDER == 1) {
+    const T g = T(G.g);
+    (void)g;
     double rl = S.cv[c1], rr = S.cv[c2];
     double irl = 1.0 / rl, irr = 1.0 / rr;
     double ul = S.cv[N + c1] * irl, vl = S.cv[2 * N + c1] * irl, wl = S.cv[3 * N + c1] * irl;
