@@ -1,8 +1,9 @@
 """Discrete adjoint of the explicit path, for verification.
 
 The residual R(cv, vals) of a region (FP64, first order or second order with no
-gradient modifier or the Venkatakrishnan limiter; vals are the patches' (u, v, w, T, p),
-read by inflow patches) is a ``torch.autograd.Function`` whose
+gradient modifier or the Venkatakrishnan limiter; vals are the patches' five BCDAT
+values, read by inflow, farfield and injection patches) is a ``torch.autograd.Function``
+whose
 backward is the library's hand-written adjoint kernel (``rflu_gpu_residual_vjp``) and
 whose forward-mode rule is the tangent kernel (``rflu_gpu_residual_jvp``).  SSP-RK3
 steps at given time steps are composed from it in torch with the same arithmetic as

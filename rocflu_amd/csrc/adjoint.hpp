@@ -41,6 +41,35 @@ __device__ __forceinline__ D1 tsqrt(D1 a) {
   return D1(s, a.d / (2.0 * s));
 }
 __device__ __forceinline__ D1 tabs(D1 a) { return a.v < 0.0 ? -a : a; }
+__device__ __forceinline__ D1 tcos(D1 a) { return D1(cos(a.v), -sin(a.v) * a.d); }
+__device__ __forceinline__ D1 tsin(D1 a) { return D1(sin(a.v), cos(a.v) * a.d); }
+__device__ __forceinline__ D1 tpow(D1 a, double e) {
+  const double p = pow(a.v, e);
+  return D1(p, e * pow(a.v, e - 1.0) * a.d);
+}
+
+// Boundary state and pressure of the patches whose state comes from the patch values
+// (total-condition inflow, farfield), or false for the other types.
+template <typename T>
+__device__ __forceinline__ bool rind_state_t(const Gas& G, int bc, int opt, int opt2, const T* vals,
+                                             const T* q, double nx, double ny, double nz, T* bs,
+                                             T& prb) {
+  if (bc == BC_INFLOW_TOTANG) {
+    prb = bcond_inflow_totang<T>(opt, opt2, vals, nx, ny, nz, G, q[0], q[1], q[2], q[3], bs);
+    return true;
+  }
+  if (bc == BC_FARFIELD) {
+    prb = rind_farfield<T>(G, vals, nx, ny, nz, q[0], q[1], q[2], q[3], q[4], bs);
+    return true;
+  }
+  return false;
+}
+
+// Patches whose flux depends on their values (the value adjoints are formed there).
+__device__ __forceinline__ bool has_value_inputs(int bc) {
+  return bc == BC_INFLOW_VELTEMP || bc == BC_INFLOW_TOTANG || bc == BC_FARFIELD ||
+         bc == BC_INJECTION;
+}
 
 // Dependent variables p, T, c of a conservative state (RFLU_SetDependentVars).
 template <typename T>
@@ -121,8 +150,8 @@ __device__ __forceinline__ void bcond_outflow_t(int bcOpt, T pout, double sxn, d
 // First-order boundary-face flux (k_bface_flux at order 1) from the cell's
 // conservative state and the patch values.
 template <typename T>
-__device__ __forceinline__ void bflux1_t(const Gas& G, int bc, int opt, double pout, const T* vals,
-                                         const T* q, double4 n, T* flx) {
+__device__ __forceinline__ void bflux1_t(const Gas& G, int bc, int opt, int opt2, double pout,
+                                         const T* vals, const T* q, double4 n, T* flx) {
   T pl, tl, al;
   dep_vars_t<T>(G, q, pl, tl, al);
   const double nx = n.x, ny = n.y, nz = n.z, nm = n.w;
@@ -134,8 +163,13 @@ __device__ __forceinline__ void bflux1_t(const Gas& G, int bc, int opt, double p
     flx[4] = T(0.0);
     return;
   }
+  if (bc == BC_INJECTION) {
+    injection_flux<T>(G, vals, nx, ny, nz, nm, q[0], q[1], q[2], q[3], pl, flx);
+    return;
+  }
   T bs[5], pr;
-  if (bc == BC_INFLOW_VELTEMP) {
+  if (rind_state_t<T>(G, bc, opt, opt2, vals, q, nx, ny, nz, bs, pr)) {
+  } else if (bc == BC_INFLOW_VELTEMP) {
     pr = inflow_veltemp<T, T>(opt, vals, q[0], G.rgasp, G.gp, bs);
   } else {
     pr = opt != BCOPT_SUPERSONIC ? T(pout) : pl;
@@ -192,7 +226,7 @@ __global__ void k_bface_jvp(Mesh M, State S, Gas G, Patches P, const double* __r
     q[k] = D1(q0[k], dcv[(size_t)k * N + c]);
     vals[k] = D1(P.vals[5 * (size_t)p + k], dvals ? dvals[5 * (size_t)p + k] : 0.0);
   }
-  bflux1_t<D1>(G, P.bc[p], P.opt[p], P.pout[p], vals, q, M.bfn[b], flx);
+  bflux1_t<D1>(G, P.bc[p], P.opt[p], P.opt2[p], P.pout[p], vals, q, M.bfn[b], flx);
   const size_t FB = (size_t)M.F + M.B;
 #pragma unroll
   for (int k = 0; k < 5; ++k) dflux[k * FB + M.F + b] = flx[k].d;
@@ -245,7 +279,7 @@ __global__ void k_bface_vjp(Mesh M, State S, Gas G, Patches P, const double* __r
 #pragma unroll
   for (int k = 0; k < 5; ++k) w[k] = mu[(size_t)k * N + c];
   const double4 n = M.bfn[b];
-  const int nseed = bc == BC_INFLOW_VELTEMP ? 10 : 5;
+  const int nseed = has_value_inputs(bc) ? 10 : 5;
   for (int j = 0; j < 10; ++j) {
     if (j >= nseed) {
       gv[(size_t)(j - 5) * B + b] = 0.0;
@@ -257,7 +291,7 @@ __global__ void k_bface_vjp(Mesh M, State S, Gas G, Patches P, const double* __r
       q[k] = D1(q0[k], j == k ? 1.0 : 0.0);
       vals[k] = D1(P.vals[5 * (size_t)p + k], j == k + 5 ? 1.0 : 0.0);
     }
-    bflux1_t<D1>(G, bc, P.opt[p], P.pout[p], vals, q, n, flx);
+    bflux1_t<D1>(G, bc, P.opt[p], P.opt2[p], P.pout[p], vals, q, n, flx);
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < 5; ++k) s = s + w[k] * flx[k].d;
@@ -399,9 +433,9 @@ __device__ __forceinline__ void flux2_t(const Gas& G, const T* L, T irl, const T
 // (recon_t's values; E from the face state), a first-order patch reads the primitive
 // copy with p in the energy slot.
 template <typename T>
-__device__ __forceinline__ void bflux2_t(const Gas& G, int bc, int opt, int order2, double pout,
-                                         const T* vals, const T* pr, const T* g, double dxd,
-                                         double dyd, double dzd, double4 n, T* flx) {
+__device__ __forceinline__ void bflux2_t(const Gas& G, int bc, int opt, int opt2, int order2,
+                                         double pout, const T* vals, const T* pr, const T* g,
+                                         double dxd, double dyd, double dzd, double4 n, T* flx) {
   const double nx = n.x, ny = n.y, nz = n.z, nm = n.w;
   T q[5], pl;
   if (!order2) {
@@ -430,8 +464,13 @@ __device__ __forceinline__ void bflux2_t(const Gas& G, int bc, int opt, int orde
     flx[4] = T(0.0);
     return;
   }
+  if (bc == BC_INJECTION) {
+    injection_flux<T>(G, vals, nx, ny, nz, nm, q[0], q[1], q[2], q[3], pl, flx);
+    return;
+  }
   T bs[5], prb;
-  if (bc == BC_INFLOW_VELTEMP) {
+  if (rind_state_t<T>(G, bc, opt, opt2, vals, q, nx, ny, nz, bs, prb)) {
+  } else if (bc == BC_INFLOW_VELTEMP) {
     prb = inflow_veltemp<T, T>(opt, vals, q[0], G.rgasp, G.gp, bs);
   } else {
     prb = opt != BCOPT_SUPERSONIC ? T(pout) : pl;
@@ -602,7 +641,7 @@ __global__ void k_bface2_jvp(Mesh M, State S, Gas G, Patches P, const double* __
 #pragma unroll
   for (int k = 0; k < 5; ++k)
     vals[k] = D1(P.vals[5 * (size_t)p + k], dvals ? dvals[5 * (size_t)p + k] : 0.0);
-  bflux2_t<D1>(G, P.bc[p], P.opt[p], P.order2[p], P.pout[p], vals, pr, g, xf.x - cc.x,
+  bflux2_t<D1>(G, P.bc[p], P.opt[p], P.opt2[p], P.order2[p], P.pout[p], vals, pr, g, xf.x - cc.x,
                xf.y - cc.y, xf.z - cc.z, M.bfn[b], flx);
   const size_t FB = (size_t)M.F + M.B;
 #pragma unroll
@@ -694,7 +733,7 @@ __global__ void k_bface2_vjp(Mesh M, State S, Gas G, Patches P, const double* __
   for (int k = 0; k < 5; ++k) w[k] = mu[(size_t)k * N + c];
   for (int j = 0; j < 15; ++j) {
     const bool vseed = j >= 10;
-    if ((j >= 5 && j < 10 && !o2) || (vseed && bc != BC_INFLOW_VELTEMP)) {
+    if ((j >= 5 && j < 10 && !o2) || (vseed && !has_value_inputs(bc))) {
       if (j < 10) gB[(size_t)j * B + b] = 0.0;
       else gV[(size_t)(j - 10) * B + b] = 0.0;
       continue;
@@ -716,7 +755,7 @@ __global__ void k_bface2_vjp(Mesh M, State S, Gas G, Patches P, const double* __
 #pragma unroll
       for (int a = 0; a < 3; ++a) g[3 * v + a].d = d[a] / dd;
     }
-    bflux2_t<D1>(G, bc, P.opt[p], o2, P.pout[p], vals, pr, g, xf.x - cc.x, xf.y - cc.y,
+    bflux2_t<D1>(G, bc, P.opt[p], P.opt2[p], o2, P.pout[p], vals, pr, g, xf.x - cc.x, xf.y - cc.y,
                  xf.z - cc.z, n, flx);
     double s = 0.0;
 #pragma unroll

@@ -42,7 +42,6 @@ struct Region {
   double end_time = HUGE_VAL;  // global%maxTime of the fused loop
   Gas gas{};                   // case 0 (the only case of a single region)
   bool has_flowthrough = false;
-  bool has_rind_patch = false;   // a total-condition inflow, farfield or injection patch
   // batched cases (K == 1: a single region; per-case tables still hold its values)
   int K = 1;
   std::vector<int> cell0_h, bface0_h;
@@ -1098,9 +1097,6 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       if (d->patch_vals)
         for (int k = 0; k < 5; ++k) hpvals[5 * q + k] = d->patch_vals[5 * (size_t)p + k];
       if (d->patch_bc[p] != BC_SLIPWALL) R->has_flowthrough = true;   // mass crosses the patch
-      if (d->patch_bc[p] == BC_INFLOW_TOTANG || d->patch_bc[p] == BC_FARFIELD ||
-          d->patch_bc[p] == BC_INJECTION)
-        R->has_rind_patch = true;
       R->bptr_h[q] = (int)bo[c] + d->bpatch_ptr[p];
       R->bptr_h[q + 1] = (int)bo[c] + d->bpatch_ptr[p + 1];
     }
@@ -2015,9 +2011,6 @@ int derivative_ready(Region* R) {
     return fail(RFLU_ERR_ARG, "second-order derivatives cover no modifier or the Venkatakrishnan limiter");
   if (R->order == 2 && R->con_n > 0)
     return fail(RFLU_ERR_ARG, "second-order derivatives do not cover constrained gradients");
-  if (R->has_rind_patch)
-    return fail(RFLU_ERR_ARG, "derivatives cover slip-wall, outflow and velocity/temperature "
-                              "inflow patches");
   int rc;
   const size_t F = R->F, B = R->B, P = R->P, N = R->N;
   const bool o2 = R->order == 2;

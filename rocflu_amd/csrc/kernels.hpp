@@ -635,6 +635,12 @@ __device__ __forceinline__ void bcond_outflow(int bcOpt, double pout, double sxn
   }
 }
 
+// Elementary functions of the boundary states for any arithmetic T (double here; the
+// derivative kernels add dual-number overloads).
+__device__ __forceinline__ double tcos(double x) { return cos(x); }
+__device__ __forceinline__ double tsin(double x) { return sin(x); }
+__device__ __forceinline__ double tpow(double x, double e) { return pow(x, e); }
+
 // Total-condition inflow state (BC_INFLOW_TOTANG, libfloflu/BcondInflowPerf.F90:95-223):
 // vals = (ptot, ttot, betah, betav, Mach).  Subsonic faces keep the interior Riemann
 // invariant a - (g-1)/2 q.n and solve the total-enthalpy quadratic for the boundary
@@ -642,121 +648,120 @@ __device__ __forceinline__ void bcond_outflow(int bcOpt, double pout, double sxn
 // state of the given Mach number.  A subsonic-option face with |q.n| >= a has no
 // state in the reference (its outputs stay unset); it is NaN here, which the
 // validity check reports.  b = (rho, rho u, rho v, rho w, rho E); returns p.
-__device__ __forceinline__ double bcond_inflow_totang(int bcOptType, int bcOptFixed,
-                                                      const double* vals, double sxn, double syn,
-                                                      double szn, const Gas& G, double rl,
-                                                      double rul, double rvl, double rwl,
-                                                      double* b) {
+template <typename T>
+__device__ __forceinline__ T bcond_inflow_totang(int bcOptType, int bcOptFixed, const T* vals,
+                                                 double sxn, double syn, double szn, const Gas& G,
+                                                 T rl, T rul, T rvl, T rwl, T* b) {
   const double rgas = G.rgasp, g = G.gp;
-  const double ptot = vals[0], ttot = vals[1], betah = vals[2], betav = vals[3];
-  const double mach = bcOptType != BCOPT_SUBSONIC ? vals[4] : 0.0;
-  double pr, tr, ar;
+  const T ptot = vals[0], ttot = vals[1], betah = vals[2], betav = vals[3];
+  const T mach = bcOptType != BCOPT_SUBSONIC ? vals[4] : T(0.0);
+  T pr, tr, ar;
   if (bcOptType == BCOPT_SUBSONIC || bcOptType == BCOPT_MIXED) {
     const double gm1 = g - 1.0;
-    const double ul = rul / rl, vl = rvl / rl, wl = rwl / rl;
-    const double a02 = g * rgas * ttot;
-    const double al = sqrt(a02 - 0.5 * (g - 1.0) * (ul * ul + vl * vl + wl * wl));
-    const double ql = ul * sxn + vl * syn + wl * szn;
-    if (fabs(ql) < al) {
-      const double sl = sqrt(ul * ul + vl * vl + wl * wl);
-      const double eta = (bcOptFixed == 0 && sl > 1.0E-6) ? ql / sl : -1.0;
-      const double Rm = al - 0.5 * gm1 * ql;
-      const double disc = 0.5 * gm1 * (eta * eta) *
-                          (a02 / (Rm * Rm) * (1.0 + 0.5 * gm1 * (eta * eta)) - 1.0);
-      double sr;
-      if (disc < 0.0) {
-        ar = sqrt(a02);
+    const T ul = rul / rl, vl = rvl / rl, wl = rwl / rl;
+    const T a02 = T(g * rgas) * ttot;
+    const T al = tsqrt(a02 - T(0.5 * (g - 1.0)) * (ul * ul + vl * vl + wl * wl));
+    const T ql = ul * T(sxn) + vl * T(syn) + wl * T(szn);
+    if (tabs(ql) < al) {
+      const T sl = tsqrt(ul * ul + vl * vl + wl * wl);
+      const T eta = (bcOptFixed == 0 && sl > T(1.0E-6)) ? ql / sl : T(-1.0);
+      const T Rm = al - T(0.5 * gm1) * ql;
+      const T disc = T(0.5 * gm1) * (eta * eta) *
+                     (a02 / (Rm * Rm) * (T(1.0) + T(0.5 * gm1) * (eta * eta)) - T(1.0));
+      T sr;
+      if (disc < T(0.0)) {
+        ar = tsqrt(a02);
         tr = ttot;
         pr = ptot;
-        sr = 0.0;
+        sr = T(0.0);
       } else {
-        ar = Rm / (1.0 + 0.5 * gm1 * eta * eta) * (1.0 + sqrt(disc));
-        tr = ar * ar / (g * rgas);
-        pr = ptot * pow(tr / ttot, g / (g - 1.0));
-        const double c2 = ar * ar;
-        sr = a02 > c2 ? sqrt(2.0 / (g - 1.0) * (a02 - c2)) : 0.0;
+        ar = Rm / (T(1.0) + T(0.5 * gm1) * eta * eta) * (T(1.0) + tsqrt(disc));
+        tr = ar * ar / T(g * rgas);
+        pr = ptot * tpow(tr / ttot, g / (g - 1.0));
+        const T c2 = ar * ar;
+        sr = a02 > c2 ? tsqrt(T(2.0 / (g - 1.0)) * (a02 - c2)) : T(0.0);
       }
-      const double rr = pr / (rgas * tr);
-      const double ur = sr * cos(betah) * cos(betav);
-      const double vr = sr * sin(betah);
-      const double wr = sr * cos(betah) * sin(betav);
+      const T rr = pr / (T(rgas) * tr);
+      const T ur = sr * tcos(betah) * tcos(betav);
+      const T vr = sr * tsin(betah);
+      const T wr = sr * tcos(betah) * tsin(betav);
       b[0] = rr;
       b[1] = rr * ur;
       b[2] = rr * vr;
       b[3] = rr * wr;
-      b[4] = rr * (pr / (rr * (g - 1.0)) + 0.5 * sr * sr);
+      b[4] = rr * (pr / (rr * T(g - 1.0)) + T(0.5) * sr * sr);
       return pr;
     }
     if (bcOptType != BCOPT_MIXED) {
-      const double nan = __builtin_nan("");
+      const T nan = T(__builtin_nan(""));
       b[0] = b[1] = b[2] = b[3] = b[4] = nan;
       return nan;
     }
   }
-  pr = ptot / pow(1.0 + 0.5 * (g - 1.0) * mach * mach, g / (g - 1.0));
-  tr = ttot / (1.0 + 0.5 * (g - 1.0) * mach * mach);
-  const double rr = pr / (rgas * tr);
-  ar = sqrt(g * rgas * tr);
-  const double ur = mach * ar * cos(betah) * cos(betav);
-  const double vr = mach * ar * sin(betah);
-  const double wr = mach * ar * cos(betah) * sin(betav);
+  pr = ptot / tpow(T(1.0) + T(0.5 * (g - 1.0)) * mach * mach, g / (g - 1.0));
+  tr = ttot / (T(1.0) + T(0.5 * (g - 1.0)) * mach * mach);
+  const T rr = pr / (T(rgas) * tr);
+  ar = tsqrt(T(g * rgas) * tr);
+  const T ur = mach * ar * tcos(betah) * tcos(betav);
+  const T vr = mach * ar * tsin(betah);
+  const T wr = mach * ar * tcos(betah) * tsin(betav);
   b[0] = rr;
   b[1] = rr * ur;
   b[2] = rr * vr;
   b[3] = rr * wr;
-  b[4] = rr * (pr / (rr * (g - 1.0)) + 0.5 * (ur * ur + vr * vr + wr * wr));
+  b[4] = rr * (pr / (rr * T(g - 1.0)) + T(0.5) * (ur * ur + vr * vr + wr * wr));
   return pr;
 }
 
 // Farfield state (BC_FARFIELD, modflu/RFLU_ModRindStates.F90:248-420, no vortex
 // correction): vals = (Mach, attack, sideslip, p, T).  The interior pressure is
 // formed from rho E.  Returns the boundary pressure.
-__device__ __forceinline__ double rind_farfield(const Gas& G, const double* vals, double nx,
-                                                double ny, double nz, double rl, double rul,
-                                                double rvl, double rwl, double rel, double* b) {
+template <typename T>
+__device__ __forceinline__ T rind_farfield(const Gas& G, const T* vals, double nx, double ny,
+                                           double nz, T rl, T rul, T rvl, T rwl, T rel, T* b) {
   const double rgas = G.rgasp, g = G.gp;
-  const double machInf = vals[0], alphaInf = vals[1], betaInf = vals[2], pInf = vals[3],
-               tInf = vals[4];
-  const double ul = rul / rl, vl = rvl / rl, wl = rwl / rl;
-  const double ql = ul * nx + vl * ny + wl * nz;
-  const double el = rel / rl;
-  const double sl2 = ul * ul + vl * vl + wl * wl;
-  const double pl = (g - 1.0) * rl * (el - 0.5 * sl2);
-  const double ri = pInf / (rgas * tInf);
-  const double pi = pInf;
-  const double qi = machInf * sqrt(g * rgas * tInf);
-  const double ui = qi * cos(alphaInf) * cos(betaInf);
-  const double vi = qi * sin(alphaInf) * cos(betaInf);
-  const double wi = qi * sin(betaInf);
-  if (machInf < 1.0) {
-    const double al = sqrt(g * pl / rl);
-    double pb, rb, ub, vb, wb;
-    if (ql < 0.0) {
-      pb = 0.5 * (pi + pl - rl * al * ((ui - ul) * nx + (vi - vl) * ny + (wi - wl) * nz));
+  const T machInf = vals[0], alphaInf = vals[1], betaInf = vals[2], pInf = vals[3],
+          tInf = vals[4];
+  const T ul = rul / rl, vl = rvl / rl, wl = rwl / rl;
+  const T ql = ul * T(nx) + vl * T(ny) + wl * T(nz);
+  const T el = rel / rl;
+  const T sl2 = ul * ul + vl * vl + wl * wl;
+  const T pl = T(g - 1.0) * rl * (el - T(0.5) * sl2);
+  const T ri = pInf / (T(rgas) * tInf);
+  const T pi = pInf;
+  const T qi = machInf * tsqrt(T(g * rgas) * tInf);
+  const T ui = qi * tcos(alphaInf) * tcos(betaInf);
+  const T vi = qi * tsin(alphaInf) * tcos(betaInf);
+  const T wi = qi * tsin(betaInf);
+  if (machInf < T(1.0)) {
+    const T al = tsqrt(T(g) * pl / rl);
+    T pb, rb, ub, vb, wb;
+    if (ql < T(0.0)) {
+      pb = T(0.5) * (pi + pl - rl * al * ((ui - ul) * T(nx) + (vi - vl) * T(ny) + (wi - wl) * T(nz)));
       rb = ri - (pi - pb) / (al * al);
-      ub = ui - nx * (pi - pb) / (rl * al);
-      vb = vi - ny * (pi - pb) / (rl * al);
-      wb = wi - nz * (pi - pb) / (rl * al);
+      ub = ui - T(nx) * (pi - pb) / (rl * al);
+      vb = vi - T(ny) * (pi - pb) / (rl * al);
+      wb = wi - T(nz) * (pi - pb) / (rl * al);
     } else {
       pb = pi;
       rb = rl - (pl - pi) / (al * al);
-      ub = ul + nx * (pl - pi) / (rl * al);
-      vb = vl + ny * (pl - pi) / (rl * al);
-      wb = wl + nz * (pl - pi) / (rl * al);
+      ub = ul + T(nx) * (pl - pi) / (rl * al);
+      vb = vl + T(ny) * (pl - pi) / (rl * al);
+      wb = wl + T(nz) * (pl - pi) / (rl * al);
     }
     b[0] = rb;
     b[1] = rb * ub;
     b[2] = rb * vb;
     b[3] = rb * wb;
-    b[4] = rb * (pb / (rb * (g - 1.0)) + 0.5 * (ub * ub + vb * vb + wb * wb));
+    b[4] = rb * (pb / (rb * T(g - 1.0)) + T(0.5) * (ub * ub + vb * vb + wb * wb));
     return pb;
   }
-  if (ql < 0.0) {
+  if (ql < T(0.0)) {
     b[0] = ri;
     b[1] = ri * ui;
     b[2] = ri * vi;
     b[3] = ri * wi;
-    b[4] = ri * (pi / (ri * (g - 1.0)) + 0.5 * qi * qi);
+    b[4] = ri * (pi / (ri * T(g - 1.0)) + T(0.5) * qi * qi);
     return pi;
   }
   b[0] = rl; b[1] = rul; b[2] = rvl; b[3] = rwl; b[4] = rel;
@@ -765,35 +770,62 @@ __device__ __forceinline__ double rind_farfield(const Gas& G, const double* vals
 
 // Injected gas at a burning face (modflu/RFLU_ModRindStates.F90:459-501): density from
 // the interior pressure at the injection temperature, velocity -minj/rho along n.
-__device__ __forceinline__ void rind_inject(const Gas& G, double nx, double ny, double nz,
-                                            double minj, double tinj, double pl, double fs,
-                                            double& rl, double& ul, double& vl, double& wl,
-                                            double& Hl) {
-  rl = pl / (G.rgasp * tinj);
-  const double ql = -minj / rl + fs;
-  ul = ql * nx;
-  vl = ql * ny;
-  wl = ql * nz;
-  Hl = G.cp * tinj + 0.5 * (ul * ul + vl * vl + wl * wl);
+template <typename T>
+__device__ __forceinline__ void rind_inject(const Gas& G, double nx, double ny, double nz, T minj,
+                                            T tinj, T pl, double fs, T& rl, T& ul, T& vl, T& wl,
+                                            T& Hl) {
+  rl = pl / (T(G.rgasp) * tinj);
+  const T ql = -minj / rl + T(fs);
+  ul = ql * T(nx);
+  vl = ql * T(ny);
+  wl = ql * T(nz);
+  Hl = T(G.cp) * tinj + T(0.5) * (ul * ul + vl * vl + wl * wl);
 }
 
 // Wall pressure of a non-burning injection face (modflu/RFLU_ModRindStates.F90:535-584):
 // isentropic expansion when the gas recedes from the wall, the piston-shock relation
 // when it approaches.
-__device__ __forceinline__ double rind_slipwall(const Gas& G, double nx, double ny, double nz,
-                                                double rl, double rul, double rvl, double rwl,
-                                                double fs, double pl) {
+template <typename T>
+__device__ __forceinline__ T rind_slipwall(const Gas& G, double nx, double ny, double nz, T rl,
+                                           T rul, T rvl, T rwl, double fs, T pl) {
   const double g = G.gp;
-  const double irl = 1.0 / rl;
-  const double ul = irl * rul, vl = irl * rvl, wl = irl * rwl;
-  const double ql = ul * nx + vl * ny + wl * nz - fs;
-  const double al = sqrt(g * pl / rl);
-  if (ql < 0.0) {
-    const double term = 1.0 + 0.5 * (g - 1.0) * ql / al;
-    return pl * pow(term, 2.0 * g / (g - 1.0));
+  const T irl = T(1.0) / rl;
+  const T ul = irl * rul, vl = irl * rvl, wl = irl * rwl;
+  const T ql = ul * T(nx) + vl * T(ny) + wl * T(nz) - T(fs);
+  const T al = tsqrt(T(g) * pl / rl);
+  if (ql < T(0.0)) {
+    const T term = T(1.0) + T(0.5 * (g - 1.0)) * ql / al;
+    return pl * tpow(term, 2.0 * g / (g - 1.0));
   }
   const double term = (g + 1.0) / 4.0;
-  return pl + term * rl * ql * (ql + sqrt(al * al + term * term * ql * ql) / term);
+  return pl + T(term) * rl * ql * (ql + tsqrt(al * al + T(term * term) * ql * ql) / T(term));
+}
+
+// Flux of an injection face (the reference's form for both orders): the injected
+// mass, momentum and enthalpy with the interior pressure when burning (minj > 0), a
+// wall with the corrected pressure otherwise.
+template <typename T>
+__device__ __forceinline__ void injection_flux(const Gas& G, const T* v, double nx, double ny,
+                                               double nz, double nm, T rl, T rul, T rvl, T rwl,
+                                               T pl, T* flx) {
+  const double fs = 0.0;
+  const T minj = v[0];
+  if (minj > T(0.0)) {
+    T ri, ui, vi, wi, Hi;
+    rind_inject<T>(G, nx, ny, nz, minj, v[1], pl, fs, ri, ui, vi, wi, Hi);
+    flx[0] = -minj * T(nm);
+    flx[1] = (-minj * ui + pl * T(nx)) * T(nm);
+    flx[2] = (-minj * vi + pl * T(ny)) * T(nm);
+    flx[3] = (-minj * wi + pl * T(nz)) * T(nm);
+    flx[4] = (-minj * Hi + pl * T(fs)) * T(nm);
+  } else {
+    const T pw = rind_slipwall<T>(G, nx, ny, nz, rl, rul, rvl, rwl, fs, pl);
+    flx[0] = T(0.0);
+    flx[1] = pw * T(nx) * T(nm);
+    flx[2] = pw * T(ny) * T(nm);
+    flx[3] = pw * T(nz) * T(nm);
+    flx[4] = pw * T(fs) * T(nm);
+  }
 }
 
 // Flux of a boundary face of patch p from its interior state, by patch type (the
@@ -805,7 +837,6 @@ __device__ __forceinline__ void patch_face_flux(const Gas& G, const Patches& P, 
                                                 double ny, double nz, double nm, double rl,
                                                 double rul, double rvl, double rwl, double rel,
                                                 double pl, double* flx) {
-  const double fs = 0.0;
   const int bc = P.bc[p];
   if (bc == BC_SLIPWALL) {
     flx[0] = 0.0;
@@ -817,23 +848,7 @@ __device__ __forceinline__ void patch_face_flux(const Gas& G, const Patches& P, 
   }
   const double* v = P.vals + 5 * (size_t)p;
   if (bc == BC_INJECTION) {
-    const double minj = v[0];
-    if (minj > 0.0) {
-      double ri, ui, vi, wi, Hi;
-      rind_inject(G, nx, ny, nz, minj, v[1], pl, fs, ri, ui, vi, wi, Hi);
-      flx[0] = -minj * nm;
-      flx[1] = (-minj * ui + pl * nx) * nm;
-      flx[2] = (-minj * vi + pl * ny) * nm;
-      flx[3] = (-minj * wi + pl * nz) * nm;
-      flx[4] = (-minj * Hi + pl * fs) * nm;
-    } else {
-      const double pw = rind_slipwall(G, nx, ny, nz, rl, rul, rvl, rwl, fs, pl);
-      flx[0] = 0.0;
-      flx[1] = pw * nx * nm;
-      flx[2] = pw * ny * nm;
-      flx[3] = pw * nz * nm;
-      flx[4] = pw * fs * nm;
-    }
+    injection_flux<double>(G, v, nx, ny, nz, nm, rl, rul, rvl, rwl, pl, flx);
     return;
   }
   const int opt = P.opt[p];
@@ -841,9 +856,9 @@ __device__ __forceinline__ void patch_face_flux(const Gas& G, const Patches& P, 
   if (bc == BC_INFLOW_VELTEMP) {
     pr = inflow_veltemp<double>(opt, v, rl, G.rgasp, G.gp, bs);
   } else if (bc == BC_INFLOW_TOTANG) {
-    pr = bcond_inflow_totang(opt, P.opt2[p], v, nx, ny, nz, G, rl, rul, rvl, rwl, bs);
+    pr = bcond_inflow_totang<double>(opt, P.opt2[p], v, nx, ny, nz, G, rl, rul, rvl, rwl, bs);
   } else if (bc == BC_FARFIELD) {
-    pr = rind_farfield(G, v, nx, ny, nz, rl, rul, rvl, rwl, rel, bs);
+    pr = rind_farfield<double>(G, v, nx, ny, nz, rl, rul, rvl, rwl, rel, bs);
   } else {
     pr = opt != BCOPT_SUPERSONIC ? P.pout[p] : pl;
     bcond_outflow(opt, pr, nx, ny, nz, G.gp, rl, rul, rvl, rwl, rel, pl, bs);
@@ -873,7 +888,7 @@ __device__ __forceinline__ double bface_srad(const Mesh& M, const State& S, cons
     double vc, a1;
     if (minj > 0.0) {
       double rl, ul, vl, wl, Hl;
-      rind_inject(G, n.x, n.y, n.z, minj, v[1], S.dv[c], 0.0, rl, ul, vl, wl, Hl);
+      rind_inject<double>(G, n.x, n.y, n.z, minj, v[1], S.dv[c], 0.0, rl, ul, vl, wl, Hl);
       vc = ul * n.x + vl * n.y + wl * n.z;
       a1 = sqrt(G.gp * G.rgasp * v[1]);
     } else {

@@ -91,6 +91,81 @@ def test_residual_tangent_and_adjoint(gpu, discr, order, reconst, porder):
         g.close()
 
 
+def patch_box(discr=3, order=1, reconst=0, patch_order=None):
+    """A slow +x stream through a box with a total-condition inflow (x-, subsonic), a
+    farfield (x+, Mach 0.5), a burning and a non-burning injection patch (y-, y+) and
+    slip walls (z-, z+); seeded noise keeps states off the flux functions' kinks."""
+    n = (6, 5, 5)
+    I = rmesh.BC_INJECTION
+    m = rmesh.box(rmesh.TET6, *n, lengths=tuple(float(x) for x in n), jitter=0.08, seed=4,
+                  bc_type=(rmesh.BC_INFLOW_TOTANG, rmesh.BC_FARFIELD, I, I, rmesh.BC_SLIPWALL,
+                           rmesh.BC_SLIPWALL),
+                  bc_opt=(1, 0, 0, 0, 0, 0), stencils=order == 2)
+    rg = Numerics().gas_prime()[1]
+    vals = [[1.3, 1.15 / rg, 0.05, -0.03, 0.0], [0.5, 0.1, 0.02, 1.0, 1.0 / rg],
+            [0.3, 2.0 / rg, 0.0, 0.0, 0.0], [0.0] * 5, [0.0] * 5, [0.0] * 5]
+    num = Numerics(discr=discr, order=order, reconst=reconst if order == 2 else 0, rk=4, cfl=0.5,
+                   dt_imposed=1e-3, patch_vals=vals, patch_opt2=[1, 0, 0, 0, 0, 0],
+                   patch_order=patch_order)
+    rng = np.random.default_rng(17)
+    gp = num.gas_prime()[2]
+    rho = 1.0 + 0.01 * rng.standard_normal(m.n_cells)
+    p = 1.0 + 0.01 * rng.standard_normal(m.n_cells)
+    vel = np.array([0.3, 0.05, -0.02]) + 0.02 * rng.standard_normal((m.n_cells, 3))
+    cv0 = cases.prim_to_cons(rho, vel[:, 0], vel[:, 1], vel[:, 2], p, gp)
+    return m, num, cv0
+
+
+@pytest.mark.parametrize("discr,order,reconst,porder", [
+    (3, 1, 0, None), (1, 1, 0, None), (4, 1, 0, None),
+    (3, 2, 0, [2] * 6), (3, 2, 11, [2, 1, 2, 1, 1, 1])])
+def test_patch_types_tangent_and_adjoint(gpu, discr, order, reconst, porder):
+    """Total-condition inflow, farfield and injection patches (values included as
+    inputs): dot-product test of the adjoint against the tangent, and the tangent
+    against central differences (on the smooth cells where the limiter is on)."""
+    m, num, cv0 = patch_box(discr, order, reconst, porder)
+    g = GpuRegion(m, num)
+    try:
+        dr = adj.DiffRegion(g)
+        u = dr.to_internal(cv0)
+        vals = torch.tensor(num.patch_vals, dtype=torch.float64, device="cuda")
+        du = rand_like(u, 11) * u
+        dvals = rand_like(vals, 12) * vals
+        mu = rand_like(u, 13)
+        jv = dr.jvp_raw(u, vals, du, dvals)
+        lam, lvals = dr.vjp_raw(u, vals, mu)
+        lhs = float((mu * jv).sum())
+        rhs = float((lam * du).sum()) + float((lvals.to(dvals) * dvals).sum())
+        print(f"patches {discr} {order} {reconst}: <mu, J v> = {lhs:.17g}, "
+              f"<J^T mu, v> = {rhs:.17g}", flush=True)
+        assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+        # the values that enter the fluxes have nonzero adjoints: the inflow's totals and
+        # angles (not its Mach number, unused by the subsonic option), the injection's
+        # mass flux and temperature, and of the farfield only the free-stream pressure,
+        # since every farfield face here is a subsonic outflow face
+        lv = lvals.cpu().numpy().reshape(-1, 5)
+        assert np.all(lv[0, :4] != 0.0) and lv[0, 4] == 0.0
+        assert lv[1, 3] != 0.0 and np.all(lv[1, [0, 1, 2, 4]] == 0.0)
+        assert np.all(lv[2, :2] != 0.0) and np.all(lv[3:] == 0.0)
+
+        def fd_at(eps):
+            return (dr.residual_raw(u + eps * du, vals + eps * dvals) -
+                    dr.residual_raw(u - eps * du, vals - eps * dvals)) / (2 * eps)
+        fd = fd_at(1e-6)
+        if reconst == 11:
+            scale = float(jv.abs().max())
+            smooth = ((fd - fd_at(1e-7)).abs().amax(dim=0) <= 1e-6 * scale)
+            assert float(smooth.double().mean()) >= 0.97
+            err = float((fd - jv)[:, smooth].norm() / jv[:, smooth].norm())
+        else:
+            err = float((fd - jv).norm() / jv.norm())
+        print(f"patches {discr} {order} {reconst}: tangent vs central difference rel {err:.3e}",
+              flush=True)
+        assert err <= 1e-6
+    finally:
+        g.close()
+
+
 def test_torch_forward_is_the_solver_step(gpu):
     m, num, cv0 = blast_box()
     g = GpuRegion(m, num)
