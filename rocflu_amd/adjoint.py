@@ -1,7 +1,7 @@
 """Discrete adjoint of the explicit path, for verification.
 
 The residual R(cv, vals) of a region (FP64, first order or second order with no
-gradient modifier or the Venkatakrishnan limiter; vals are the patches' five BCDAT
+gradient modifier, the Venkatakrishnan limiter or WENO-XYZ; vals are the patches' five BCDAT
 values, read by inflow, farfield and injection patches) is a ``torch.autograd.Function``
 whose
 backward is the library's hand-written adjoint kernel (``rflu_gpu_residual_vjp``) and

@@ -41,6 +41,8 @@ __device__ __forceinline__ D1 tsqrt(D1 a) {
   return D1(s, a.d / (2.0 * s));
 }
 __device__ __forceinline__ D1 tabs(D1 a) { return a.v < 0.0 ? -a : a; }
+__device__ __forceinline__ double dval(double a) { return a; }
+__device__ __forceinline__ double dval(D1 a) { return a.v; }
 __device__ __forceinline__ D1 tcos(D1 a) { return D1(cos(a.v), -sin(a.v) * a.d); }
 __device__ __forceinline__ D1 tsin(D1 a) { return D1(sin(a.v), cos(a.v) * a.d); }
 __device__ __forceinline__ D1 tpow(D1 a, double e) {
@@ -661,6 +663,146 @@ __global__ void k_grad2_fwd(Mesh M, State S, double* __restrict__ graw, double* 
   for (int k = 0; k < 15; ++k) {
     graw[(size_t)k * N + i] = gr[k];
     glim[(size_t)k * N + i] = gl[k];
+  }
+}
+
+// ---- WENO-XYZ and the positivity clip (RECONST 2) ------------------------------------
+// Per component, g'_i = L_i / S_i with L_i = sum w_j g_j, S_i = sum w_j over the cell
+// itself and its stencil members, w = 1/(g^2 + 1e-15) (k_weno_clip's order), then the
+// clip zeroes the three components of rho or p when an extrapolation to one of the
+// cell's faces is not positive.  The clip is a selection: its derivative keeps or
+// zeroes those components as the values decide.  The adjoint of the blend is a gather
+// over the cell itself and then the transposed stencil, with
+// dg'_i/dg_j = (w_j + (g_j - g'_i) dw_j) / S_i, dw_j = -2 g_j w_j^2.
+
+// Clip decisions of cell i (keep[0] rho, keep[1] p) from its blended gradient g.
+template <typename T>
+__device__ __forceinline__ void weno_clip_keep(const Mesh& M, const State& S, int i, const T* g,
+                                               bool* keep) {
+  const int N = M.N;
+  const double4 ci = M.cell[i];
+  const int pos[2] = {0, 4};
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int iv = pos[q];
+    const double varc = S.prim[(size_t)iv * N + i];
+    keep[q] = true;
+    for (int e = M.c2f_ptr[i]; e < M.c2f_ptr[i + 1]; ++e) {
+      const int sgn = M.c2f_idx[e];
+      const int f = (sgn > 0 ? sgn : -sgn) - 1;
+      const double4 xf = f < M.F ? M.fc[f] : M.bfc[f - M.F];
+      const double dx = xf.x - ci.x, dy = xf.y - ci.y, dz = xf.z - ci.z;
+      const double varf = varc + dval(g[3 * iv]) * dx + dval(g[3 * iv + 1]) * dy +
+                          dval(g[3 * iv + 2]) * dz;
+      if (varf <= 0.0) {
+        keep[q] = false;
+        break;
+      }
+    }
+  }
+}
+
+// Tangent: blended and clipped gradient values gv and tangents gd from the raw
+// gradient's values graw and tangents draw.
+__global__ void k_weno_jvp(Mesh M, State S, const double* __restrict__ graw,
+                           const double* __restrict__ draw, double* __restrict__ gv,
+                           double* __restrict__ gd) {
+  const int N = M.N;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const D1 eps(1.0E-15);
+  const int s0 = M.sten_ptr[i], s1 = M.sten_ptr[i + 1];
+  D1 g[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) {
+    const D1 term(graw[(size_t)k * N + i], draw[(size_t)k * N + i]);
+    D1 sumw = D1(1.0) / (term * term + eps);
+    D1 loc = sumw * term;
+    for (int s = s0; s < s1; ++s) {
+      const int j = M.sten_idx[s];
+      const D1 t(graw[(size_t)k * N + j], draw[(size_t)k * N + j]);
+      const D1 w = D1(1.0) / (t * t + eps);
+      sumw = sumw + w;
+      loc = loc + w * t;
+    }
+    g[k] = loc / sumw;
+  }
+  bool keep[2];
+  weno_clip_keep<D1>(M, S, i, g, keep);
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+    if (!keep[q])
+      for (int d = 0; d < 3; ++d) g[3 * (q ? 4 : 0) + d] = D1(0.0);
+#pragma unroll
+  for (int k = 0; k < 15; ++k) {
+    gv[(size_t)k * N + i] = g[k].v;
+    gd[(size_t)k * N + i] = g[k].d;
+  }
+}
+
+// Adjoint forward sweep: the clipped blend gfin (what the fluxes read), the blend before
+// the clip gpre, the normalisers sw and the clip decisions keep[q*N + i].
+__global__ void k_weno_fwd(Mesh M, State S, const double* __restrict__ graw,
+                           double* __restrict__ gfin, double* __restrict__ gpre,
+                           double* __restrict__ sw, int* __restrict__ keepo) {
+  const int N = M.N;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double eps = 1.0E-15;
+  const int s0 = M.sten_ptr[i], s1 = M.sten_ptr[i + 1];
+  double g[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) {
+    const double term = graw[(size_t)k * N + i];
+    double sumw = 1.0 / (term * term + eps);
+    double loc = sumw * term;
+    for (int s = s0; s < s1; ++s) {
+      const double t = graw[(size_t)k * N + M.sten_idx[s]];
+      const double w = 1.0 / (t * t + eps);
+      sumw = sumw + w;
+      loc = loc + w * t;
+    }
+    g[k] = loc / sumw;
+    gpre[(size_t)k * N + i] = g[k];
+    sw[(size_t)k * N + i] = sumw;
+  }
+  bool keep[2];
+  weno_clip_keep<double>(M, S, i, g, keep);
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    keepo[(size_t)q * N + i] = keep[q] ? 1 : 0;
+    if (!keep[q])
+      for (int d = 0; d < 3; ++d) g[3 * (q ? 4 : 0) + d] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 15; ++k) gfin[(size_t)k * N + i] = g[k];
+}
+
+// Adjoint: aR_j = sum over i in {j} + T(j) of dg'_i/dg_j keep_i aG_i, per component.
+__global__ void k_weno_vjp(Mesh M, const int* __restrict__ tptr, const int* __restrict__ tidx,
+                           const double* __restrict__ graw, const double* __restrict__ gpre,
+                           const double* __restrict__ sw, const int* __restrict__ keep,
+                           const double* __restrict__ aG, double* __restrict__ aR) {
+  const int N = M.N;
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const double eps = 1.0E-15;
+  const int t0 = tptr[j], t1 = tptr[j + 1];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) {
+    const int v = k / 3;
+    const int q = v == 0 ? 0 : v == 4 ? 1 : -1;
+    const double g = graw[(size_t)k * N + j];
+    const double w = 1.0 / (g * g + eps);
+    const double dw = -2.0 * g * w * w;
+    double acc = 0.0;
+    for (int t = t0 - 1; t < t1; ++t) {
+      const int i = t < t0 ? j : tidx[t];
+      if (q >= 0 && !keep[(size_t)q * N + i]) continue;
+      acc = acc + (w + (g - gpre[(size_t)k * N + i]) * dw) / sw[(size_t)k * N + i] *
+                      aG[(size_t)k * N + i];
+    }
+    aR[(size_t)k * N + j] = acc;
   }
 }
 

@@ -68,6 +68,7 @@ struct Region {
   double *adj_p = nullptr, *adj_ga = nullptr, *adj_gb2 = nullptr, *adj_gc = nullptr, *adj_gr = nullptr;
   double *adj_selv = nullptr;
   int *adj_seln = nullptr, *adj_tptr = nullptr, *adj_tidx = nullptr;
+  double *adj_wpre = nullptr, *adj_wsum = nullptr;   // WENO-XYZ blend and normalisers
   hipStream_t stream = nullptr;
   // device buffers
   int2* f2c = nullptr;
@@ -193,7 +194,7 @@ void free_region(Region* R) {
                   R->bface0, R->dtmin, R->pbc, R->popt, R->popt2, R->porder2, R->ppout, R->pvals,
                   R->bptr, R->adj_g, R->adj_gb, R->adj_gv, R->adj_pv, R->adj_dflux,
                   R->adj_dvals, R->adj_p, R->adj_ga, R->adj_gb2, R->adj_gc, R->adj_gr, R->adj_selv,
-                  R->adj_seln, R->adj_tptr, R->adj_tidx,
+                  R->adj_seln, R->adj_tptr, R->adj_tidx, R->adj_wpre, R->adj_wsum,
                   R->con_ecell, R->con_evar, R->con_rptr, R->con_rcell,
                   R->con_rval, R->con_rw};
   for (void* b : bufs)
@@ -2007,8 +2008,10 @@ namespace {
 int derivative_ready(Region* R) {
   if (R->K != 1) return fail(RFLU_ERR_ARG, "derivatives are for a single region");
   if (R->prec != RFLU_PREC_FP64) return fail(RFLU_ERR_ARG, "derivatives are FP64");
-  if (R->order == 2 && R->reconst != RFLU_RECONST_NONE && R->reconst != RFLU_RECONST_LIM_VENKAT)
-    return fail(RFLU_ERR_ARG, "second-order derivatives cover no modifier or the Venkatakrishnan limiter");
+  if (R->order == 2 && R->reconst != RFLU_RECONST_NONE && R->reconst != RFLU_RECONST_LIM_VENKAT &&
+      R->reconst != RFLU_RECONST_WENO_XYZ)
+    return fail(RFLU_ERR_ARG, "second-order derivatives cover no modifier, WENO-XYZ or the "
+                              "Venkatakrishnan limiter");
   if (R->order == 2 && R->con_n > 0)
     return fail(RFLU_ERR_ARG, "second-order derivatives do not cover constrained gradients");
   int rc;
@@ -2043,6 +2046,9 @@ int derivative_ready(Region* R) {
       return rc;
     HIP_TRY(hipStreamSynchronize(R->stream));
   }
+  if (o2 && R->reconst == RFLU_RECONST_WENO_XYZ && !R->adj_wpre) {
+    if ((rc = dalloc(&R->adj_wpre, 15 * N)) || (rc = dalloc(&R->adj_wsum, 15 * N))) return rc;
+  }
   return 0;
 }
 }  // namespace
@@ -2065,16 +2071,24 @@ int rflu_gpu_residual_jvp(int32_t h, const double* dcv, const double* dvals, dou
       hipLaunchKernelGGL(k_grad2_jvp<true>, gc, bs, 0, R->stream, M, S, (const double*)R->adj_p, R->adj_ga, R->adj_gb2);
     else
       hipLaunchKernelGGL(k_grad2_jvp<false>, gc, bs, 0, R->stream, M, S, (const double*)R->adj_p, R->adj_ga, R->adj_gb2);
+    const double *gv = R->adj_ga, *gd = R->adj_gb2;
+    if (R->reconst == RFLU_RECONST_WENO_XYZ) {
+      // blend and clip the raw gradient's values and tangents
+      hipLaunchKernelGGL(k_weno_jvp, gc, bs, 0, R->stream, M, S, (const double*)R->adj_ga,
+                         (const double*)R->adj_gb2, R->adj_gc, R->adj_gr);
+      gv = R->adj_gc;
+      gd = R->adj_gr;
+    }
     if (R->F > 0) {
       dim3 g(nblk(R->F, 256));
-      const double *dp = R->adj_p, *gv = R->adj_ga, *gd = R->adj_gb2;
+      const double *dp = R->adj_p;
       if (R->discr == 1) hipLaunchKernelGGL(k_face2_jvp<1>, g, bs, 0, R->stream, M, S, R->gas, dp, gv, gd, R->adj_dflux);
       else if (R->discr == 3) hipLaunchKernelGGL(k_face2_jvp<3>, g, bs, 0, R->stream, M, S, R->gas, dp, gv, gd, R->adj_dflux);
       else hipLaunchKernelGGL(k_face2_jvp<4>, g, bs, 0, R->stream, M, S, R->gas, dp, gv, gd, R->adj_dflux);
     }
     if (R->B > 0)
       hipLaunchKernelGGL(k_bface2_jvp, dim3(nblk(R->B, 256)), bs, 0, R->stream, M, S, R->gas, R->patches(),
-                         (const double*)R->adj_p, (const double*)R->adj_ga, (const double*)R->adj_gb2,
+                         (const double*)R->adj_p, gv, gd,
                          (const double*)(dvals ? R->adj_dvals : nullptr), R->adj_dflux);
     hipLaunchKernelGGL((k_gather_rhs<double>), gc, bs, 0, R->stream, M, (const double*)R->adj_dflux, drhs);
     HIP_TRY(hipGetLastError());
@@ -2109,9 +2123,13 @@ int rflu_gpu_residual_vjp(int32_t h, const double* mu, double* lam, double* dval
     const dim3 gc(nblk(R->N, 256)), bs(256);
     const bool venkat = R->reconst == RFLU_RECONST_LIM_VENKAT;
     double *graw = R->adj_gb2, *glim = R->adj_ga, *aG = R->adj_gc, *aR = R->adj_gr;
-    // forward sweep: raw and limited gradients
+    const bool weno = R->reconst == RFLU_RECONST_WENO_XYZ;
+    // forward sweep: raw and limited (or blended and clipped) gradients
     if (venkat) hipLaunchKernelGGL(k_grad2_fwd<true>, gc, bs, 0, R->stream, M, S, graw, glim);
     else hipLaunchKernelGGL(k_grad2_fwd<false>, gc, bs, 0, R->stream, M, S, graw, glim);
+    if (weno)
+      hipLaunchKernelGGL(k_weno_fwd, gc, bs, 0, R->stream, M, S, (const double*)graw, glim,
+                         R->adj_wpre, R->adj_wsum, R->adj_seln);
     if (R->F > 0) {
       dim3 g(nblk(R->F, 256));
       if (R->discr == 1) hipLaunchKernelGGL(k_face2_vjp<1>, g, bs, 0, R->stream, M, S, R->gas, (const double*)glim, mu, R->adj_g);
@@ -2128,6 +2146,10 @@ int rflu_gpu_residual_vjp(int32_t h, const double* mu, double* lam, double* dval
                          aR, R->adj_p, R->adj_seln, R->adj_selv);
       hipLaunchKernelGGL(k_venkat_nb_gather, gc, bs, 0, R->stream, M, (const int*)R->adj_seln,
                          (const double*)R->adj_selv, R->adj_p);
+    } else if (weno) {
+      hipLaunchKernelGGL(k_weno_vjp, gc, bs, 0, R->stream, M, (const int*)R->adj_tptr,
+                         (const int*)R->adj_tidx, (const double*)graw, (const double*)R->adj_wpre,
+                         (const double*)R->adj_wsum, (const int*)R->adj_seln, (const double*)aG, aR);
     } else {
       aR = aG;
     }

@@ -46,11 +46,12 @@ def rand_like(t, seed):
 
 @pytest.mark.parametrize("discr,order,reconst,porder", [
     (1, 1, 0, None), (3, 1, 0, None), (4, 1, 0, None),
-    (3, 2, 11, [2, 2, 1, 1, 1, 1]), (1, 2, 11, [2] * 6), (4, 2, 11, [1] * 6), (3, 2, 0, [2] * 6)])
+    (3, 2, 11, [2, 2, 1, 1, 1, 1]), (1, 2, 11, [2] * 6), (4, 2, 11, [1] * 6), (3, 2, 0, [2] * 6),
+    (3, 2, 2, [2] * 6), (1, 2, 2, [2, 2, 1, 1, 1, 1]), (4, 2, 2, [1] * 6)])
 def test_residual_tangent_and_adjoint(gpu, discr, order, reconst, porder):
-    """Order 2: WLSQ gradient, optional Venkatakrishnan limiter, first- and second-order
-    patch fluxes (a first-order patch reads the primitive copy with p in the energy
-    slot).  With the limiter the residual is only piecewise smooth (its min/max picks
+    """Order 2: WLSQ gradient, optional Venkatakrishnan limiter or WENO-XYZ blend with the
+    positivity clip, first- and second-order patch fluxes (a first-order patch reads the
+    primitive copy with p in the energy slot).  With the limiter the residual is only piecewise smooth (its min/max picks
     and the flux functions' branch switches move with the reconstructed states), so the
     central difference is compared on the cells where two step sizes agree, i.e. where
     no kink lies within the step; the dot-product test holds everywhere."""
@@ -82,6 +83,17 @@ def test_residual_tangent_and_adjoint(gpu, discr, order, reconst, porder):
             print(f"case {discr} {order} {reconst}: {frac:.4f} of cells smooth at both steps; "
                   f"tangent vs central difference there rel {err:.3e}", flush=True)
             assert frac >= 0.97
+        elif reconst == 2:
+            # the WENO weights 1/(g^2 + 1e-15) make the residual smooth but strongly
+            # curved where a gradient component is small: the central difference's
+            # error falls as eps^2 (4e-4, 4e-6, 4e-8 at 1e-5, 1e-6, 1e-7), so it is
+            # compared at 1e-7, with the eps^2 fall checked
+            fd7 = fd_at(1e-7)
+            err6 = float((fd - jv).norm() / jv.norm())
+            err = float((fd7 - jv).norm() / jv.norm())
+            print(f"case {discr} {order} {reconst}: tangent vs central difference rel "
+                  f"{err6:.3e} (eps 1e-6), {err:.3e} (eps 1e-7)", flush=True)
+            assert err < 0.1 * err6 or err6 <= 1e-7
         else:
             err = float((fd - jv).norm() / jv.norm())
             print(f"case {discr} {order} {reconst}: tangent vs central difference rel {err:.3e}",
@@ -185,7 +197,7 @@ def test_torch_forward_is_the_solver_step(gpu):
         g.close()
 
 
-@pytest.mark.parametrize("discr,order,reconst", [(3, 1, 0), (4, 1, 0), (3, 2, 11)])
+@pytest.mark.parametrize("discr,order,reconst", [(3, 1, 0), (4, 1, 0), (3, 2, 11), (3, 2, 2)])
 def test_multistep_adjoint_vs_tangent_and_fd(gpu, discr, order, reconst):
     m, num, cv0 = blast_box(discr, order=order, reconst=reconst,
                             patch_order=[2, 2, 1, 1, 1, 1] if order == 2 else None)
