@@ -218,8 +218,9 @@ int rflu_gpu_synchronize(int32_t handle);
  * addresses on the region's GPU; every call waits for its work.  Tangent and adjoint
  * are of the FP64 residual R(cv) of a single region (rflu_gpu_residual's R), first
  * order, or second order with no gradient modifier, the Venkatakrishnan limiter or the
- * WENO-XYZ blend with its positivity clip (not constrained gradients), with dv = dv(cv) and the patch values of
- * inflow, farfield and injection patches (their five BCDAT values) as inputs.
+ * WENO-XYZ blend with its positivity clip, with or without wall-constrained gradients,
+ * with dv = dv(cv) and the patch values of inflow, farfield and injection patches
+ * (their five BCDAT values) as inputs.
  */
 int rflu_gpu_cell_order(int32_t handle, int32_t* new2old);
 /* vol in the internal cell order (host array of n_cells). */

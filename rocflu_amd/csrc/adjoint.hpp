@@ -496,11 +496,56 @@ __device__ __forceinline__ T venkat_phi(T d2, T dm, double epsq) {
   return numer / (denom + T(1.0E-12));
 }
 
+// Wall-constrained velocity gradients (CRECONSTC 1) in the derivative kernels: the
+// entries and rows that k_grad_constr applies, the entries of every cell (cptr/cent,
+// ascending entry) and, for every cell j, the rows that read j's value (trptr over
+// trow/tent: row and its entry, ascending row).  cptr is null without constraints.
+struct ConstrD {
+  Constr C;
+  const int* cptr;
+  const int* cent;
+  const int* trptr;
+  const int* trow;
+  const int* tent;
+};
+
+// Overwrites the raw gradient of cell i with its constrained rows (k_grad_constr's
+// arithmetic and order: g_d = sum_r rw_d(r) (value(r) - v_i), value(r) the row cell's
+// primitive or the row's wall value).
+template <typename T, typename Load>
+__device__ __forceinline__ void constrain_t(const ConstrD& CD, int i, Load pr, const T* vi,
+                                            T* graw) {
+  const Constr& C = CD.C;
+  for (int q = CD.cptr[i]; q < CD.cptr[i + 1]; ++q) {
+    const int e = CD.cent[q], v = C.evar[e];
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+      if (k != v) continue;
+      T g0 = T(0.0), g1 = T(0.0), g2 = T(0.0);
+      for (int r = C.rptr[e]; r < C.rptr[e + 1]; ++r) {
+        const int j = C.rcell[r];
+        T vj[5];
+        if (j >= 0) pr(j, vj);
+        else vj[k] = T(C.rval[r]);
+        const T dVar = vj[k] - vi[k];
+        g0 = g0 + T(C.rw[r]) * dVar;
+        g1 = g1 + T(C.rw[(size_t)C.R + r]) * dVar;
+        g2 = g2 + T(C.rw[2 * (size_t)C.R + r]) * dVar;
+      }
+      graw[3 * k] = g0;
+      graw[3 * k + 1] = g1;
+      graw[3 * k + 2] = g2;
+    }
+  }
+}
+
 // Gradient and limited gradient of cell i in arithmetic T from the primitive states
 // (pr(j) loads cell j's 5 values), venkat_limit's selections: min/max keep the earlier
-// value on ties, the limiter the smaller term, faces with d2 = 0 skipped.
+// value on ties, the limiter the smaller term, faces with d2 = 0 skipped.  Constrained
+// entries replace their components between the least-squares pass and the limiter.
 template <typename T, bool VENKAT, typename Load>
-__device__ __forceinline__ void grad_limited_t(const Mesh& M, int i, Load pr, T* graw, T* glim) {
+__device__ __forceinline__ void grad_limited_t(const Mesh& M, const ConstrD& CD, int i, Load pr,
+                                               T* graw, T* glim) {
   const int N = M.N;
   GradCoef<double> c = grad_coef<double>(M, i);
   const double4 ci = M.cell[i];
@@ -523,6 +568,7 @@ __device__ __forceinline__ void grad_limited_t(const Mesh& M, int i, Load pr, T*
       graw[3 * k + 2] = graw[3 * k + 2] + T(w[2]) * dVar;
     }
   }
+  if (CD.cptr) constrain_t<T>(CD, i, pr, vi, graw);
 #pragma unroll
   for (int k = 0; k < 15; ++k) glim[k] = graw[k];
   if (!VENKAT) return;
@@ -585,13 +631,13 @@ __global__ void k_prim_jvp(Mesh M, State S, Gas G, const double* __restrict__ dc
 
 // Limited gradient (value and tangent) of every cell: gv, gd [15][N].
 template <bool VENKAT>
-__global__ void k_grad2_jvp(Mesh M, State S, const double* __restrict__ dprim,
+__global__ void k_grad2_jvp(Mesh M, ConstrD CD, State S, const double* __restrict__ dprim,
                             double* __restrict__ gv, double* __restrict__ gd) {
   const int N = M.N;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   D1 graw[15], glim[15];
-  grad_limited_t<D1, VENKAT>(M, i, [&](int c, D1* v) {
+  grad_limited_t<D1, VENKAT>(M, CD, i, [&](int c, D1* v) {
 #pragma unroll
     for (int k = 0; k < 5; ++k) v[k] = D1(S.prim[(size_t)k * N + c], dprim[(size_t)k * N + c]);
   }, graw, glim);
@@ -653,12 +699,13 @@ __global__ void k_bface2_jvp(Mesh M, State S, Gas G, Patches P, const double* __
 // ---- second-order adjoint ------------------------------------------------------------
 // Values of the limited gradient and of the raw one (the adjoint's forward sweep).
 template <bool VENKAT>
-__global__ void k_grad2_fwd(Mesh M, State S, double* __restrict__ graw, double* __restrict__ glim) {
+__global__ void k_grad2_fwd(Mesh M, ConstrD CD, State S, double* __restrict__ graw,
+                            double* __restrict__ glim) {
   const int N = M.N;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   double gr[15], gl[15];
-  grad_limited_t<double, VENKAT>(M, i, [&](int c, double* v) { load_prim(S, N, c, v); }, gr, gl);
+  grad_limited_t<double, VENKAT>(M, CD, i, [&](int c, double* v) { load_prim(S, N, c, v); }, gr, gl);
 #pragma unroll
   for (int k = 0; k < 15; ++k) {
     graw[(size_t)k * N + i] = gr[k];
@@ -1049,6 +1096,52 @@ __global__ void k_venkat_nb_gather(Mesh M, const int* __restrict__ selN,
   }
 #pragma unroll
   for (int k = 0; k < 5; ++k) aP[(size_t)k * N + j] = aP[(size_t)k * N + j] + acc[k];
+}
+
+// Constrained-gradient adjoint, before k_grad_vjp: aP(v, j) += sum over the rows r
+// that read cell j (entry (i, v)) of rw(:, r) . aR(3v..3v+2, i), minus, for j's own
+// entries, sum over all their rows of rw(:, r) . aR(3v..3v+2, j) (each row subtracts
+// the cell's own value).  k_constr_clear then zeroes the constrained components of aR,
+// whose least-squares values the constraints replaced.
+__global__ void k_constr_vjp(Mesh M, ConstrD CD, const double* __restrict__ aR,
+                             double* __restrict__ aP) {
+  const int N = M.N;
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const Constr& C = CD.C;
+  const size_t R = C.R;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int q = CD.trptr[j]; q < CD.trptr[j + 1]; ++q) {
+    const int r = CD.trow[q], e = CD.tent[q];
+    const int i = C.ecell[e], v = C.evar[e];
+    const double s = C.rw[r] * aR[(size_t)(3 * v) * N + i] +
+                     C.rw[R + r] * aR[(size_t)(3 * v + 1) * N + i] +
+                     C.rw[2 * R + r] * aR[(size_t)(3 * v + 2) * N + i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (k == v - 1) acc[k] = acc[k] + s;
+  }
+  for (int q = CD.cptr[j]; q < CD.cptr[j + 1]; ++q) {
+    const int e = CD.cent[q], v = C.evar[e];
+    const double a0 = aR[(size_t)(3 * v) * N + j], a1 = aR[(size_t)(3 * v + 1) * N + j],
+                 a2 = aR[(size_t)(3 * v + 2) * N + j];
+    double s = 0.0;
+    for (int r = C.rptr[e]; r < C.rptr[e + 1]; ++r)
+      s = s + (C.rw[r] * a0 + C.rw[R + r] * a1 + C.rw[2 * R + r] * a2);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (k == v - 1) acc[k] = acc[k] - s;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) aP[(size_t)(k + 1) * N + j] = aP[(size_t)(k + 1) * N + j] + acc[k];
+}
+
+__global__ void k_constr_clear(Mesh M, Constr C, double* __restrict__ aR) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= C.n) return;
+  const int N = M.N, i = C.ecell[e], v = C.evar[e];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) aR[(size_t)(3 * v + d) * N + i] = 0.0;
 }
 
 // Gradient adjoint: aP(j) += sum over the cells i whose stencil holds j (transposed

@@ -69,6 +69,9 @@ struct Region {
   double *adj_selv = nullptr;
   int *adj_seln = nullptr, *adj_tptr = nullptr, *adj_tidx = nullptr;
   double *adj_wpre = nullptr, *adj_wsum = nullptr;   // WENO-XYZ blend and normalisers
+  // constrained gradients: entries of every cell and the rows that read every cell (CSR)
+  int *adj_cptr = nullptr, *adj_cent = nullptr, *adj_ctptr = nullptr, *adj_ctrow = nullptr,
+      *adj_ctent = nullptr;
   hipStream_t stream = nullptr;
   // device buffers
   int2* f2c = nullptr;
@@ -150,6 +153,13 @@ struct Region {
     c.rcell = con_rcell; c.rval = con_rval; c.rw = con_rw; c.R = con_R;
     return c;
   }
+  ConstrD constr_d() const {
+    ConstrD d;
+    d.C = constr();
+    d.cptr = con_n > 0 ? adj_cptr : nullptr;
+    d.cent = adj_cent; d.trptr = adj_ctptr; d.trow = adj_ctrow; d.tent = adj_ctent;
+    return d;
+  }
   CaseParams case0() const {
     CaseParams c;
     c.cfl = cfl; c.dt_imposed = dt_imposed; c.dt_min_limit = dt_min_limit; c.end_time = end_time;
@@ -195,6 +205,7 @@ void free_region(Region* R) {
                   R->bptr, R->adj_g, R->adj_gb, R->adj_gv, R->adj_pv, R->adj_dflux,
                   R->adj_dvals, R->adj_p, R->adj_ga, R->adj_gb2, R->adj_gc, R->adj_gr, R->adj_selv,
                   R->adj_seln, R->adj_tptr, R->adj_tidx, R->adj_wpre, R->adj_wsum,
+                  R->adj_cptr, R->adj_cent, R->adj_ctptr, R->adj_ctrow, R->adj_ctent,
                   R->con_ecell, R->con_evar, R->con_rptr, R->con_rcell,
                   R->con_rval, R->con_rw};
   for (void* b : bufs)
@@ -2012,8 +2023,6 @@ int derivative_ready(Region* R) {
       R->reconst != RFLU_RECONST_WENO_XYZ)
     return fail(RFLU_ERR_ARG, "second-order derivatives cover no modifier, WENO-XYZ or the "
                               "Venkatakrishnan limiter");
-  if (R->order == 2 && R->con_n > 0)
-    return fail(RFLU_ERR_ARG, "second-order derivatives do not cover constrained gradients");
   int rc;
   const size_t F = R->F, B = R->B, P = R->P, N = R->N;
   const bool o2 = R->order == 2;
@@ -2049,6 +2058,41 @@ int derivative_ready(Region* R) {
   if (o2 && R->reconst == RFLU_RECONST_WENO_XYZ && !R->adj_wpre) {
     if ((rc = dalloc(&R->adj_wpre, 15 * N)) || (rc = dalloc(&R->adj_wsum, 15 * N))) return rc;
   }
+  if (o2 && R->con_n > 0 && !R->adj_cptr) {
+    // the entries of every cell, and for every cell j the rows whose cell is j
+    const int E = R->con_n, RR = R->con_R;
+    std::vector<int> ec(E), rp(E + 1), rcl(RR);
+    HIP_TRY(hipStreamSynchronize(R->stream));
+    HIP_TRY(hipMemcpy(ec.data(), R->con_ecell, E * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(rp.data(), R->con_rptr, (E + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(rcl.data(), R->con_rcell, RR * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> cptr(N + 1, 0), cent(E), tptr(N + 1, 0);
+    for (int e = 0; e < E; ++e) cptr[ec[e] + 1]++;
+    for (int r = 0; r < RR; ++r)
+      if (rcl[r] >= 0) tptr[rcl[r] + 1]++;
+    for (size_t j = 0; j < N; ++j) {
+      cptr[j + 1] += cptr[j];
+      tptr[j + 1] += tptr[j];
+    }
+    std::vector<int> trow(tptr[N]), tent(tptr[N]);
+    std::vector<int> cf(cptr.begin(), cptr.end() - 1), tf(tptr.begin(), tptr.end() - 1);
+    for (int e = 0; e < E; ++e) {
+      cent[cf[ec[e]]++] = e;
+      for (int r = rp[e]; r < rp[e + 1]; ++r)
+        if (rcl[r] >= 0) {
+          const int q = tf[rcl[r]]++;
+          trow[q] = r;
+          tent[q] = e;
+        }
+    }
+    if ((rc = dalloc(&R->adj_cptr, cptr.size())) || (rc = dalloc(&R->adj_cent, cent.size())) ||
+        (rc = dalloc(&R->adj_ctptr, tptr.size())) || (rc = dalloc(&R->adj_ctrow, trow.size())) ||
+        (rc = dalloc(&R->adj_ctent, tent.size())) || (rc = dput(R->adj_cptr, cptr, R->stream)) ||
+        (rc = dput(R->adj_cent, cent, R->stream)) || (rc = dput(R->adj_ctptr, tptr, R->stream)) ||
+        (rc = dput(R->adj_ctrow, trow, R->stream)) || (rc = dput(R->adj_ctent, tent, R->stream)))
+      return rc;
+    HIP_TRY(hipStreamSynchronize(R->stream));
+  }
   return 0;
 }
 }  // namespace
@@ -2068,9 +2112,9 @@ int rflu_gpu_residual_jvp(int32_t h, const double* dcv, const double* dvals, dou
     const dim3 gc(nblk(R->N, 256)), bs(256);
     hipLaunchKernelGGL(k_prim_jvp, gc, bs, 0, R->stream, M, S, R->gas, dcv, R->adj_p);
     if (R->reconst == RFLU_RECONST_LIM_VENKAT)
-      hipLaunchKernelGGL(k_grad2_jvp<true>, gc, bs, 0, R->stream, M, S, (const double*)R->adj_p, R->adj_ga, R->adj_gb2);
+      hipLaunchKernelGGL(k_grad2_jvp<true>, gc, bs, 0, R->stream, M, R->constr_d(), S, (const double*)R->adj_p, R->adj_ga, R->adj_gb2);
     else
-      hipLaunchKernelGGL(k_grad2_jvp<false>, gc, bs, 0, R->stream, M, S, (const double*)R->adj_p, R->adj_ga, R->adj_gb2);
+      hipLaunchKernelGGL(k_grad2_jvp<false>, gc, bs, 0, R->stream, M, R->constr_d(), S, (const double*)R->adj_p, R->adj_ga, R->adj_gb2);
     const double *gv = R->adj_ga, *gd = R->adj_gb2;
     if (R->reconst == RFLU_RECONST_WENO_XYZ) {
       // blend and clip the raw gradient's values and tangents
@@ -2125,8 +2169,8 @@ int rflu_gpu_residual_vjp(int32_t h, const double* mu, double* lam, double* dval
     double *graw = R->adj_gb2, *glim = R->adj_ga, *aG = R->adj_gc, *aR = R->adj_gr;
     const bool weno = R->reconst == RFLU_RECONST_WENO_XYZ;
     // forward sweep: raw and limited (or blended and clipped) gradients
-    if (venkat) hipLaunchKernelGGL(k_grad2_fwd<true>, gc, bs, 0, R->stream, M, S, graw, glim);
-    else hipLaunchKernelGGL(k_grad2_fwd<false>, gc, bs, 0, R->stream, M, S, graw, glim);
+    if (venkat) hipLaunchKernelGGL(k_grad2_fwd<true>, gc, bs, 0, R->stream, M, R->constr_d(), S, graw, glim);
+    else hipLaunchKernelGGL(k_grad2_fwd<false>, gc, bs, 0, R->stream, M, R->constr_d(), S, graw, glim);
     if (weno)
       hipLaunchKernelGGL(k_weno_fwd, gc, bs, 0, R->stream, M, S, (const double*)graw, glim,
                          R->adj_wpre, R->adj_wsum, R->adj_seln);
@@ -2152,6 +2196,10 @@ int rflu_gpu_residual_vjp(int32_t h, const double* mu, double* lam, double* dval
                          (const double*)R->adj_wsum, (const int*)R->adj_seln, (const double*)aG, aR);
     } else {
       aR = aG;
+    }
+    if (R->con_n > 0) {
+      hipLaunchKernelGGL(k_constr_vjp, gc, bs, 0, R->stream, M, R->constr_d(), (const double*)aR, R->adj_p);
+      hipLaunchKernelGGL(k_constr_clear, dim3(nblk(R->con_n, 256)), bs, 0, R->stream, M, R->constr(), aR);
     }
     hipLaunchKernelGGL(k_grad_vjp, gc, bs, 0, R->stream, M, (const int*)R->adj_tptr, (const int*)R->adj_tidx,
                        (const double*)aR, R->adj_p);

@@ -5,6 +5,8 @@ round-off); the tangent kernel matches central finite differences of the residua
 the torch-composed forward reproduces the solver's own steps; the gradient of a
 time-integrated wall pressure by reverse mode (adjoint kernels, checkpointed) equals
 the forward-mode tangent along seeded directions and central finite differences."""
+import dataclasses
+
 import numpy as np
 import pytest
 import torch
@@ -16,7 +18,8 @@ from rocflu_amd import adjoint as adj
 pytestmark = pytest.mark.gpu
 
 
-def blast_box(discr=3, n=(6, 5, 5), subsonic_in=True, order=1, reconst=0, patch_order=None):
+def blast_box(discr=3, n=(6, 5, 5), subsonic_in=True, order=1, reconst=0, patch_order=None,
+              creconst=0):
     m = rmesh.box(rmesh.TET6, *n, lengths=tuple(float(x) for x in n), jitter=0.08, seed=4,
                   bc_type=(rmesh.BC_INFLOW_VELTEMP, rmesh.BC_OUTFLOW) + (rmesh.BC_SLIPWALL,) * 4,
                   bc_opt=(1 if subsonic_in else 0, 1, 0, 0, 0, 0), bc_pout=(0.0, 1.0, 0, 0, 0, 0),
@@ -24,7 +27,8 @@ def blast_box(discr=3, n=(6, 5, 5), subsonic_in=True, order=1, reconst=0, patch_
     rg = Numerics().gas_prime()[1]
     vals = [[0.4, 0.05, -0.02, 1.1 / rg, 1.2]] + [[0.0] * 5] * 5
     num = Numerics(discr=discr, order=order, reconst=reconst if order == 2 else 0, rk=4, cfl=0.5,
-                   dt_imposed=1e-3, patch_vals=vals, patch_order=patch_order)
+                   dt_imposed=1e-3, patch_vals=vals, patch_order=patch_order,
+                   creconst=creconst if order == 2 else 0)
     centre = tuple(0.5 * x for x in n)
     cv0 = cases.blast(m, num, centre, 0.2 * n[0], p_in=10.0)
     # a slow stream with seeded noise on every variable: exact ties and zero speeds
@@ -44,18 +48,22 @@ def rand_like(t, seed):
     return torch.randn(t.shape, generator=g, dtype=torch.float64).to(t.device)
 
 
-@pytest.mark.parametrize("discr,order,reconst,porder", [
-    (1, 1, 0, None), (3, 1, 0, None), (4, 1, 0, None),
-    (3, 2, 11, [2, 2, 1, 1, 1, 1]), (1, 2, 11, [2] * 6), (4, 2, 11, [1] * 6), (3, 2, 0, [2] * 6),
-    (3, 2, 2, [2] * 6), (1, 2, 2, [2, 2, 1, 1, 1, 1]), (4, 2, 2, [1] * 6)])
-def test_residual_tangent_and_adjoint(gpu, discr, order, reconst, porder):
-    """Order 2: WLSQ gradient, optional Venkatakrishnan limiter or WENO-XYZ blend with the
+@pytest.mark.parametrize("discr,order,reconst,porder,creconst", [
+    (1, 1, 0, None, 0), (3, 1, 0, None, 0), (4, 1, 0, None, 0),
+    (3, 2, 11, [2, 2, 1, 1, 1, 1], 0), (1, 2, 11, [2] * 6, 0), (4, 2, 11, [1] * 6, 0),
+    (3, 2, 0, [2] * 6, 0), (3, 2, 2, [2] * 6, 0), (1, 2, 2, [2, 2, 1, 1, 1, 1], 0),
+    (4, 2, 2, [1] * 6, 0),
+    (3, 2, 0, [2] * 6, 1), (3, 2, 11, [2, 2, 1, 1, 1, 1], 1), (1, 2, 2, [2] * 6, 1)])
+def test_residual_tangent_and_adjoint(gpu, discr, order, reconst, porder, creconst):
+    """Order 2: WLSQ gradient, optional wall-constrained velocity gradients on the four
+    slip walls (CRECONSTC 1), optional Venkatakrishnan limiter or WENO-XYZ blend with the
     positivity clip, first- and second-order patch fluxes (a first-order patch reads the
     primitive copy with p in the energy slot).  With the limiter the residual is only piecewise smooth (its min/max picks
     and the flux functions' branch switches move with the reconstructed states), so the
     central difference is compared on the cells where two step sizes agree, i.e. where
     no kink lies within the step; the dot-product test holds everywhere."""
-    m, num, cv0 = blast_box(discr, order=order, reconst=reconst, patch_order=porder)
+    m, num, cv0 = blast_box(discr, order=order, reconst=reconst, patch_order=porder,
+                            creconst=creconst)
     g = GpuRegion(m, num)
     try:
         dr = adj.DiffRegion(g)
@@ -68,9 +76,21 @@ def test_residual_tangent_and_adjoint(gpu, discr, order, reconst, porder):
         lam, lvals = dr.vjp_raw(u, vals, mu)
         lhs = float((mu * jv).sum())
         rhs = float((lam * du).sum()) + float((lvals.to(dvals) * dvals).sum())
-        print(f"case {discr} {order} {reconst}: <mu, J v> = {lhs:.17g}, "
+        print(f"case {discr} {order} {reconst} {creconst}: <mu, J v> = {lhs:.17g}, "
               f"<J^T mu, v> = {rhs:.17g}", flush=True)
         assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+        if creconst:
+            # the constraints act on the tangent: the unconstrained one differs
+            g0 = GpuRegion(m, dataclasses.replace(num, creconst=0))
+            try:
+                dr0 = adj.DiffRegion(g0)
+                assert torch.equal(dr0.new2old, dr.new2old)
+                jv0 = dr0.jvp_raw(u, vals, du, dvals)
+            finally:
+                g0.close()
+            diff = float((jv - jv0).norm() / jv.norm())
+            print(f"constrained vs free tangent rel {diff:.3e}", flush=True)
+            assert diff > 1e-6
         def fd_at(eps):
             return (dr.residual_raw(u + eps * du, vals + eps * dvals) -
                     dr.residual_raw(u - eps * du, vals - eps * dvals)) / (2 * eps)
@@ -197,10 +217,12 @@ def test_torch_forward_is_the_solver_step(gpu):
         g.close()
 
 
-@pytest.mark.parametrize("discr,order,reconst", [(3, 1, 0), (4, 1, 0), (3, 2, 11), (3, 2, 2)])
-def test_multistep_adjoint_vs_tangent_and_fd(gpu, discr, order, reconst):
+@pytest.mark.parametrize("discr,order,reconst,creconst", [(3, 1, 0, 0), (4, 1, 0, 0), (3, 2, 11, 0),
+                                                   (3, 2, 2, 0), (3, 2, 11, 1)])
+def test_multistep_adjoint_vs_tangent_and_fd(gpu, discr, order, reconst, creconst):
     m, num, cv0 = blast_box(discr, order=order, reconst=reconst,
-                            patch_order=[2, 2, 1, 1, 1, 1] if order == 2 else None)
+                            patch_order=[2, 2, 1, 1, 1, 1] if order == 2 else None,
+                            creconst=creconst)
     g = GpuRegion(m, num)
     try:
         nsteps = 30 if order == 1 else 12
