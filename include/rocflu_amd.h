@@ -75,7 +75,8 @@ extern "C" {
  */
 typedef struct {
   int32_t index_base;        /* 0 or 1: base of f2c, bf2c and c2cs_idx values      */
-  int32_t n_cells;           /* nCells (no virtual cells on a single region)         */
+  int32_t n_cells;           /* nCells: actual cells (n_virtual virtual cells follow
+                                them in every per-cell array)                        */
   int32_t n_faces;           /* nFaces                                               */
   int32_t n_patches;
   const int32_t* f2c;        /* (2, n_faces)                                         */
@@ -123,6 +124,13 @@ typedef struct {
                                 inflow 10 patches (0 or 1), BCSWI_FARF_CORR of
                                 farfield patches (0 only: no vortex correction);
                                 NULL = all 0                                         */
+  int32_t n_virtual;         /* nCellsTot - nCells: virtual cells, numbered after the
+                                actual ones; they take part in faces (actual-virtual
+                                faces) and stencils, while update, checks and time
+                                step cover the actual cells; their state comes from
+                                rflu_gpu_exchange.  0 for a region of its own and for
+                                every case of a batch                                */
+  int32_t pad2_;
 } rflu_region_desc;
 
 /* Per-region counters of the after-update checks, accumulated since the last reset. */
@@ -188,6 +196,37 @@ int rflu_gpu_after_update(int32_t handle, int32_t istage);
 /* Residual R(cv) of the current state (gradients included for order 2), gathered
  * into rhs (5,n_cells); waits.  For inspection and tests. */
 int rflu_gpu_residual(int32_t handle, double* rhs);
+
+/*
+ * Regions of one problem on one device, coupled through virtual cells (the
+ * reference's multi-region runs).  rflu_gpu_set_borders gives a region its borders
+ * (t_border, modflu/RFLU_ModCommLists.F90 builds them): border b pairs with border
+ * nbr_border[b] (index_base applies) of region nbr_handle[b]; it sends the actual cells
+ * send_cells[send_ptr[b] .. send_ptr[b+1]) and receives, in the same order as the
+ * neighbour's send list for this region, into the virtual cells
+ * recv_cells[recv_ptr[b] .. recv_ptr[b+1]) (cell numbers in the descriptor's numbering
+ * and index base).  Replaces earlier borders.
+ *
+ * rflu_gpu_exchange copies the sending cells' values to the receiving virtual cells
+ * of every border of the listed regions (each border's neighbour must be listed):
+ * RFLU_XCHG_STATE copies cv and then sets dv (and the primitive copy) of the virtual
+ * cells -- RFLU_MPI_CopyWrapper + RFLU_SetVarsContWrapper(nCells+1, nCellsTot)
+ * (modflu/RFLU_ModMPI.F90:564-660, libfloflu/RungeKuttaMP.F90:446-453);
+ * RFLU_XCHG_GRAD copies the limited (or blended) cell gradients --
+ * RFLU_MPI_ExchangeGradCells (modflu/RFLU_ModMPI.F90:1382-1511,
+ * RungeKuttaMP.F90:229-236).  The call waits for the listed regions' queued work
+ * first.  A step of regions: dt = the minimum of rflu_gpu_time_step over the regions;
+ * per stage, rflu_gpu_cell_gradients of each, RFLU_XCHG_GRAD (order 2), then
+ * rflu_gpu_convective_fluxes, rflu_gpu_rk_update, rflu_gpu_set_dependent_vars and
+ * rflu_gpu_after_update of each, then RFLU_XCHG_STATE.
+ */
+#define RFLU_XCHG_STATE 1
+#define RFLU_XCHG_GRAD 2
+int rflu_gpu_set_borders(int32_t handle, int32_t n_borders, const int32_t* nbr_handle,
+                         const int32_t* nbr_border, const int32_t* send_ptr,
+                         const int32_t* send_cells, const int32_t* recv_ptr,
+                         const int32_t* recv_cells);
+int rflu_gpu_exchange(int32_t n_regions, const int32_t* handles, int32_t what);
 
 /* RFLU_TimeStepping's loop body (rocflu/RFLU_TimeStepping.F90:353-640) nsteps times:
  * time step, all RungeKuttaMP stages with the update, dependent variables and checks

@@ -50,6 +50,8 @@ class RegionDesc(ctypes.Structure):
         ("creconst_weight", ctypes.c_double), ("creconst", ctypes.c_int32),
         ("pad_", ctypes.c_int32),
         ("patch_opt2", _i32p),
+        ("n_virtual", ctypes.c_int32),
+        ("pad2_", ctypes.c_int32),
     ]
 
 
@@ -82,6 +84,9 @@ SIGNATURES = {
     "rflu_gpu_after_update": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32]),
     "rflu_gpu_residual": (ctypes.c_int, [ctypes.c_int32, _f64p]),
     "rflu_gpu_time_steps": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32]),
+    "rflu_gpu_set_borders": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _i32p, _i32p, _i32p,
+                                            _i32p, _i32p, _i32p]),
+    "rflu_gpu_exchange": (ctypes.c_int, [ctypes.c_int32, _i32p, ctypes.c_int32]),
     "rflu_gpu_get_stats": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(GpuStats)]),
     "rflu_gpu_get_case_stats": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(GpuStats)]),
     "rflu_gpu_cell_order": (ctypes.c_int, [ctypes.c_int32, _i32p]),

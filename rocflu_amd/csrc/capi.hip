@@ -35,8 +35,20 @@ int fail(int code, const std::string& msg) {
 
 enum KernelId { KID_FACE = 0, KID_UPDATE = 1, KID_GRAD = 2, KID_BLEND = 3, KID_DT = 4, KID_BFACE = 5, KID_N = 6 };
 
+// One border of a region (t_border): the actual cells it sends to a neighbouring
+// region and the virtual cells it receives from it, as internal cell numbers.
+struct Border {
+  int nbr = 0;          // neighbour's handle
+  int nbr_border = 0;   // the neighbour's border that matches this one
+  int nsend = 0, nrecv = 0;
+  int *send = nullptr, *recv = nullptr;
+};
+
 struct Region {
   int N = 0, F = 0, B = 0, P = 0;
+  int nact = 0;                // actual cells; [nact, N) are virtual cells
+  int base = 0;                // the descriptor's index base (border cell lists)
+  std::vector<Border> borders;
   int order = 1, discr = 3, reconst = 0, rk = 2, prec = 0;
   double cfl = 0.5, dt_imposed = 0, dt_min_limit = 0;
   double end_time = HUGE_VAL;  // global%maxTime of the fused loop
@@ -120,7 +132,7 @@ struct Region {
 
   Mesh mesh() const {
     Mesh m;
-    m.N = N; m.F = F; m.B = B;
+    m.N = N; m.F = F; m.B = B; m.nact = nact;
     m.f2c = f2c; m.fn = fn; m.fc = fc; m.cell = cell;
     m.bf2c = bf2c; m.bfn = bfn; m.bfc = bfc; m.bpatch = bpatch;
     m.c2f_ptr = c2f_ptr; m.c2f_idx = c2f_idx;
@@ -210,6 +222,10 @@ void free_region(Region* R) {
                   R->con_rval, R->con_rw};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  for (Border& b : R->borders) {
+    if (b.send) (void)hipFree(b.send);
+    if (b.recv) (void)hipFree(b.recv);
+  }
   if (R->stream) (void)hipStreamDestroy(R->stream);
   delete R;
 }
@@ -261,7 +277,13 @@ void renumber(Region* R, std::vector<double4>& hcell, std::vector<int2>& hf2c,
               std::vector<int>& hsten_idx, std::vector<double>& hmoms, std::vector<double>& hepsq) {
   const int N = R->N, F = R->F;
   std::vector<int> n2o(N), o2n(N);
-  for (int c = 0; c < R->K; ++c) morton_order(hcell, R->cell0_h[c], R->cell0_h[c + 1], n2o);
+  if (R->nact < N) {
+    // virtual cells stay after the actual ones
+    morton_order(hcell, 0, R->nact, n2o);
+    morton_order(hcell, R->nact, N, n2o);
+  } else {
+    for (int c = 0; c < R->K; ++c) morton_order(hcell, R->cell0_h[c], R->cell0_h[c + 1], n2o);
+  }
   for (int i = 0; i < N; ++i) o2n[n2o[i]] = i;
   {
     std::vector<double4> c2(N);
@@ -989,7 +1011,10 @@ int build_constrained(const rflu_region_desc* d, int c0, int p0, const std::vect
 int check_desc(const rflu_region_desc* d, const rflu_region_desc* ref) {
   const int N = d->n_cells, F = d->n_faces, P = d->n_patches;
   const int base = d->index_base;
-  if (N <= 0 || F < 0 || P < 0) return fail(RFLU_ERR_ARG, "bad region sizes");
+  if (N <= 0 || F < 0 || P < 0 || d->n_virtual < 0) return fail(RFLU_ERR_ARG, "bad region sizes");
+  if ((long long)N + d->n_virtual > 0x7fffffff) return fail(RFLU_ERR_ARG, "cell count must fit int32");
+  if (d->n_virtual > 0 && d->order == 2 && d->creconst)
+    return fail(RFLU_ERR_ARG, "regions with virtual cells do not take constrained gradients");
   if (base != 0 && base != 1) return fail(RFLU_ERR_ARG, "index_base must be 0 or 1");
   if (!d->f2c || !d->fn || !d->cofg || !d->vol || (P > 0 && (!d->bpatch_ptr || !d->patch_bc)))
     return fail(RFLU_ERR_ARG, "missing grid arrays");
@@ -1037,6 +1062,8 @@ int check_desc(const rflu_region_desc* d, const rflu_region_desc* ref) {
     if (bc == BC_INJECTION && d->order == 2 && d->creconst)
       return fail(RFLU_ERR_ARG, "constrained gradients at injection patches are not supported");
   }
+  if (ref && (d->n_virtual > 0 || ref->n_virtual > 0))
+    return fail(RFLU_ERR_ARG, "a batch of cases takes no virtual cells");
   if (ref && (d->order != ref->order || d->discr != ref->discr || d->rk != ref->rk ||
               d->precision != ref->precision || (d->order == 2 && d->reconst != ref->reconst)))
     return fail(RFLU_ERR_ARG, "every case of a batch needs the same discr, order, reconst, rk and precision");
@@ -1066,11 +1093,11 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
   for (int c = 0; c < K; ++c) {
     const rflu_region_desc* d = descs + c;
     if ((rc = check_desc(d, c ? descs : nullptr))) return rc;
-    co[c + 1] = co[c] + d->n_cells;
+    co[c + 1] = co[c] + d->n_cells + d->n_virtual;
     fo[c + 1] = fo[c] + d->n_faces;
     bo[c + 1] = bo[c] + (d->n_patches > 0 ? d->bpatch_ptr[d->n_patches] : 0);
     po[c + 1] = po[c] + d->n_patches;
-    so[c + 1] = so[c] + (d->order == 2 ? d->c2cs_ptr[d->n_cells] : 0);
+    so[c + 1] = so[c] + (d->order == 2 ? d->c2cs_ptr[d->n_cells + d->n_virtual] : 0);
   }
   if (co[K] > 0x7fffffff || fo[K] + bo[K] > 0x7fffffff || so[K] > 0x7fffffff)
     return fail(RFLU_ERR_ARG, "cell, face and stencil counts must fit int32");
@@ -1082,6 +1109,8 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
 
   Region* R = new Region();
   R->N = N; R->F = F; R->B = B; R->P = P; R->K = K;
+  R->nact = K == 1 ? d0->n_cells : N;
+  R->base = d0->index_base;
   R->order = d0->order; R->discr = d0->discr; R->reconst = d0->order == 2 ? d0->reconst : 0;
   R->rk = d0->rk; R->prec = d0->precision;
   R->cfl = d0->cfl; R->dt_imposed = d0->dt_imposed; R->dt_min_limit = d0->dt_min_limit;
@@ -1128,7 +1157,9 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
   }
   for (int k = 0; k < K; ++k) {
     const rflu_region_desc* d = descs + k;
-    const int n = d->n_cells, nf = d->n_faces, np = d->n_patches, base = d->index_base;
+    // n counts actual and virtual cells; boundary faces belong to actual cells
+    const int n = d->n_cells + d->n_virtual, na = d->n_cells, nf = d->n_faces;
+    const int np = d->n_patches, base = d->index_base;
     const int c0 = (int)co[k], f0 = (int)fo[k], b0 = (int)bo[k], p0 = (int)po[k];
     const int nb = (int)(bo[k + 1] - bo[k]);
     for (int f = 0; f < nf; ++f) {
@@ -1150,7 +1181,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       for (int b = d->bpatch_ptr[p]; b < d->bpatch_ptr[p + 1]; ++b) {
         if (b < 0 || b >= nb) { free_region(R); return fail(RFLU_ERR_ARG, "bad bpatch_ptr"); }
         int c1 = d->bf2c[b] - base;
-        if (c1 < 0 || c1 >= n) { free_region(R); return fail(RFLU_ERR_ARG, "bf2c out of range"); }
+        if (c1 < 0 || c1 >= na) { free_region(R); return fail(RFLU_ERR_ARG, "bf2c out of range"); }
         hbf2c[b0 + b] = c0 + c1;
         hbpatch[b0 + b] = p0 + p;
         hbfn[b0 + b] = make_double4(d->bfn[4 * (size_t)b], d->bfn[4 * (size_t)b + 1], d->bfn[4 * (size_t)b + 2], d->bfn[4 * (size_t)b + 3]);
@@ -1514,6 +1545,123 @@ int rflu_gpu_after_update(int32_t h, int32_t istage) {
   return status_from(s);
 }
 
+int rflu_gpu_set_borders(int32_t h, int32_t n_borders, const int32_t* nbr_handle,
+                         const int32_t* nbr_border, const int32_t* send_ptr,
+                         const int32_t* send_cells, const int32_t* recv_ptr,
+                         const int32_t* recv_cells) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (R->K != 1) return fail(RFLU_ERR_ARG, "borders belong to single regions");
+  if (n_borders < 0 || (n_borders > 0 && (!nbr_handle || !nbr_border || !send_ptr || !recv_ptr)))
+    return fail(RFLU_ERR_ARG, "bad border arguments");
+  if (n_borders > 0 && (send_ptr[0] != 0 || recv_ptr[0] != 0))
+    return fail(RFLU_ERR_ARG, "send_ptr[0] and recv_ptr[0] must be 0");
+  std::vector<int> o2n(R->N);
+  for (int i = 0; i < R->N; ++i) o2n[R->new2old[i]] = i;
+  std::vector<Border> nb(n_borders);
+  auto drop = [&nb]() {
+    for (Border& b : nb) {
+      if (b.send) (void)hipFree(b.send);
+      if (b.recv) (void)hipFree(b.recv);
+    }
+  };
+  for (int b = 0; b < n_borders; ++b) {
+    Border& B = nb[b];
+    B.nbr = nbr_handle[b];
+    B.nbr_border = nbr_border[b] - R->base;
+    B.nsend = send_ptr[b + 1] - send_ptr[b];
+    B.nrecv = recv_ptr[b + 1] - recv_ptr[b];
+    if (B.nbr == h || !get(B.nbr) || B.nbr_border < 0 || B.nsend < 0 || B.nrecv < 0 ||
+        (B.nsend > 0 && !send_cells) || (B.nrecv > 0 && !recv_cells)) {
+      drop();
+      return fail(RFLU_ERR_ARG, "border " + std::to_string(b) + ": bad neighbour or lists");
+    }
+    std::vector<int> hs(B.nsend), hr(B.nrecv);
+    for (int e = 0; e < B.nsend; ++e) {
+      const int c = send_cells[send_ptr[b] + e] - R->base;
+      if (c < 0 || c >= R->nact) { drop(); return fail(RFLU_ERR_ARG, "send cells must be actual cells"); }
+      hs[e] = o2n[c];
+    }
+    for (int e = 0; e < B.nrecv; ++e) {
+      const int c = recv_cells[recv_ptr[b] + e] - R->base;
+      if (c < R->nact || c >= R->N) { drop(); return fail(RFLU_ERR_ARG, "receive cells must be virtual cells"); }
+      hr[e] = o2n[c];
+    }
+    int rc;
+    if ((rc = dalloc(&B.send, hs.size())) || (rc = dalloc(&B.recv, hr.size())) ||
+        (rc = dput(B.send, hs, R->stream)) || (rc = dput(B.recv, hr, R->stream))) {
+      drop();
+      return rc;
+    }
+  }
+  if (hipStreamSynchronize(R->stream) != hipSuccess) { drop(); return fail(RFLU_ERR_HIP, "border upload failed"); }
+  for (Border& b : R->borders) {
+    if (b.send) (void)hipFree(b.send);
+    if (b.recv) (void)hipFree(b.recv);
+  }
+  R->borders.swap(nb);
+  return RFLU_OK;
+}
+
+int rflu_gpu_exchange(int32_t n_regions, const int32_t* handles, int32_t what) {
+  if (n_regions < 1 || !handles) return fail(RFLU_ERR_ARG, "bad arguments");
+  if (what != RFLU_XCHG_STATE && what != RFLU_XCHG_GRAD) return fail(RFLU_ERR_ARG, "what must be RFLU_XCHG_STATE or RFLU_XCHG_GRAD");
+  std::vector<Region*> rs(n_regions);
+  for (int i = 0; i < n_regions; ++i) {
+    rs[i] = get(handles[i]);
+    if (!rs[i]) return fail(RFLU_ERR_HANDLE, "bad region handle");
+    if (rs[i]->K != 1) return fail(RFLU_ERR_ARG, "borders belong to single regions");
+    if (what == RFLU_XCHG_GRAD && rs[i]->order != 2) return fail(RFLU_ERR_ARG, "gradients exist at order 2 only");
+  }
+  // check every pairing before anything moves
+  for (int i = 0; i < n_regions; ++i)
+    for (const Border& b : rs[i]->borders) {
+      Region* D = get(b.nbr);
+      if (!D || std::find(handles, handles + n_regions, b.nbr) == handles + n_regions)
+        return fail(RFLU_ERR_ARG, "a border's neighbour is not in the exchange");
+      if (b.nbr_border >= (int)D->borders.size() || D->borders[b.nbr_border].nbr != handles[i] ||
+          D->borders[b.nbr_border].nrecv != b.nsend)
+        return fail(RFLU_ERR_ARG, "border lists do not match (ERR_BUFFERDIM_MISMATCH)");
+      if (what == RFLU_XCHG_GRAD && (grad32(D) != grad32(rs[i]) || D->reconst != rs[i]->reconst))
+        return fail(RFLU_ERR_ARG, "neighbouring regions need the same precision and reconstruction");
+    }
+  // a barrier: every region's queued work ends before its cells are read or written
+  for (Region* R : rs) HIP_TRY(hipStreamSynchronize(R->stream));
+  for (int i = 0; i < n_regions; ++i) {
+    Region* R = rs[i];
+    for (const Border& b : R->borders) {
+      if (b.nsend == 0) continue;
+      Region* D = get(b.nbr);
+      const int* didx = D->borders[b.nbr_border].recv;
+      const dim3 g(nblk(b.nsend, 256)), bs(256);
+      if (what == RFLU_XCHG_STATE)
+        hipLaunchKernelGGL((k_halo_copy<double>), g, bs, 0, R->stream, (const double*)R->cv, R->N,
+                           D->cv, D->N, 5, (const int*)b.send, didx, b.nsend);
+      else if (grad32(R))
+        hipLaunchKernelGGL((k_halo_copy<float>), g, bs, 0, R->stream, (const float*)R->grad_final(), R->N,
+                           (float*)D->grad_final(), D->N, 15, (const int*)b.send, didx, b.nsend);
+      else
+        hipLaunchKernelGGL((k_halo_copy<double>), g, bs, 0, R->stream, (const double*)R->grad_final(), R->N,
+                           (double*)D->grad_final(), D->N, 15, (const int*)b.send, didx, b.nsend);
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  for (Region* R : rs) HIP_TRY(hipStreamSynchronize(R->stream));
+  if (what == RFLU_XCHG_STATE)
+    for (Region* R : rs) {
+      if (R->nact == R->N) continue;
+      Mesh M = R->mesh();
+      State S = R->state();
+      const dim3 g(nblk(R->N - R->nact, 256)), bs(256);
+      if (R->order == 2)
+        hipLaunchKernelGGL((k_set_vars_range<true>), g, bs, 0, R->stream, M, S, R->gas, R->nact, R->N);
+      else
+        hipLaunchKernelGGL((k_set_vars_range<false>), g, bs, 0, R->stream, M, S, R->gas, R->nact, R->N);
+    }
+  HIP_TRY(hipGetLastError());
+  return RFLU_OK;
+}
+
 int rflu_gpu_residual(int32_t h, double* rhs) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
@@ -1539,6 +1687,9 @@ int rflu_gpu_time_steps(int32_t h, int64_t nsteps, int32_t use_graph) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
   if (nsteps < 0) return fail(RFLU_ERR_ARG, "nsteps < 0");
+  if (R->nact < R->N)
+    return fail(RFLU_ERR_ARG, "a region with virtual cells advances through the per-callee "
+                              "entry points and rflu_gpu_exchange");
   int rc;
   if (use_graph && !R->profiling) {
     if (R->graph_dirty || !R->graph) {
@@ -2018,6 +2169,7 @@ int rflu_gpu_residual_dev(int32_t h, double* rhs) {
 namespace {
 int derivative_ready(Region* R) {
   if (R->K != 1) return fail(RFLU_ERR_ARG, "derivatives are for a single region");
+  if (R->nact < R->N) return fail(RFLU_ERR_ARG, "derivatives are for regions without virtual cells");
   if (R->prec != RFLU_PREC_FP64) return fail(RFLU_ERR_ARG, "derivatives are FP64");
   if (R->order == 2 && R->reconst != RFLU_RECONST_NONE && R->reconst != RFLU_RECONST_LIM_VENKAT &&
       R->reconst != RFLU_RECONST_WENO_XYZ)

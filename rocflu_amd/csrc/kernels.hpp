@@ -113,6 +113,7 @@ struct Cases {
 
 struct Mesh {
   int N, F, B;
+  int nact;                // actual cells [0, nact); virtual cells [nact, N) follow
   const int2* f2c;
   const double4* fn;
   const double4* fc;
@@ -1201,7 +1202,7 @@ __global__ void k_update(Mesh M, State S, Gas G, Cases C, const FT* __restrict__
   if (!C.cid && halted(sc)) return;
   const int N = M.N;
   int i = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+  if (i >= M.nact) return;   // virtual cells take their state from the halo exchange
   int first = 0;
   if (C.cid) {
     const int c = C.cid[i];
@@ -1265,7 +1266,7 @@ __global__ void k_update_tile(Mesh M, State S, Gas G, Cases C, FaceTiles FTl,
   }
   __syncthreads();
   const int i = t * FTl.ts + (int)threadIdx.x;
-  if (i >= N) return;
+  if (i >= M.nact) return;
   int first = 0;
   if (C.cid) {
     const int c = C.cid[i];
@@ -1335,12 +1336,50 @@ __global__ void k_set_prim(Mesh M, State S) {
   S.prim[4 * (size_t)N + i] = S.dv[i];
 }
 
+// Dependent variables and, at order 2, the primitive copy of the cells [i0, i1) --
+// RFLU_SetVarsContWrapper over the virtual cells after a halo exchange
+// (libfloflu/RungeKuttaMP.F90:451-452), with k_set_dv's and k_set_prim's arithmetic.
+template <bool PRIM>
+__global__ void k_set_vars_range(Mesh M, State S, Gas G, int i0, int i1) {
+  const int N = M.N;
+  int i = i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= i1) return;
+  double p, t, a;
+  double r = S.cv[i];
+  dep_vars(G, r, S.cv[(size_t)N + i], S.cv[2 * (size_t)N + i], S.cv[3 * (size_t)N + i],
+           S.cv[4 * (size_t)N + i], p, t, a);
+  S.dv[i] = p;
+  S.dv[(size_t)N + i] = t;
+  S.dv[2 * (size_t)N + i] = a;
+  if (PRIM) {
+    double ir = 1.0 / r;
+    S.prim[i] = r;
+    S.prim[(size_t)N + i] = ir * S.cv[(size_t)N + i];
+    S.prim[2 * (size_t)N + i] = ir * S.cv[2 * (size_t)N + i];
+    S.prim[3 * (size_t)N + i] = ir * S.cv[3 * (size_t)N + i];
+    S.prim[4 * (size_t)N + i] = p;
+  }
+}
+
+// Halo copy between two regions on one device (RFLU_MPI_CopyCellData,
+// modflu/RFLU_ModMPI.F90:332-390): the ncomp components ([k][ns] layout) of the
+// sending region's cells sidx[e] to the receiving region's cells didx[e] ([k][nd]).
+template <typename T>
+__global__ void k_halo_copy(const T* __restrict__ src, int ns, T* __restrict__ dst, int nd,
+                            int ncomp, const int* __restrict__ sidx, const int* __restrict__ didx,
+                            int n) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int a = sidx[e], b = didx[e];
+  for (int k = 0; k < ncomp; ++k) dst[(size_t)k * nd + b] = src[(size_t)k * ns + a];
+}
+
 // After-update checks as a separate pass (the unfused entry points).
 template <bool WRITE_PRIM>
 __global__ void k_after_update(Mesh M, State S, Gas G, Cases C, Scalars* sc) {
   const int N = M.N;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+  if (i >= M.nact) return;   // the checks cover actual cells (AfterUpdateMP over nCells)
   if (C.cid) {
     G = C.gas[C.cid[i]];
     sc = C.sc + C.cid[i];
@@ -1996,11 +2035,10 @@ __device__ __forceinline__ double cell_srad_sum(const Mesh& M, const double* __r
 __global__ void k_dt_gather(Mesh M, const double* __restrict__ srad, double* __restrict__ part,
                             const Scalars* __restrict__ sc) {
   __shared__ double red[16];
-  const int N = M.N;
   const int blk = xcd_block(M.nxcd);
   int i = blk * blockDim.x + threadIdx.x;
   double dtc = HUGE_VAL;
-  if (i < N && !halted(sc)) dtc = M.cell[i].w / cell_srad_sum(M, srad, i);
+  if (i < M.nact && !halted(sc)) dtc = M.cell[i].w / cell_srad_sum(M, srad, i);
   dtc = wave_min(dtc);
   int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) red[wv] = dtc;

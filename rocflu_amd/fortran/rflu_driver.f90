@@ -110,6 +110,7 @@ PROGRAM rflu_driver
   desc%c2cs_bptr = C_NULL_PTR; desc%c2cs_bface = C_NULL_PTR
   desc%creconst_weight = 1.0_C_DOUBLE; desc%creconst = 0; desc%pad_ = 0
   desc%patch_opt2 = C_NULL_PTR
+  desc%n_virtual = 0; desc%pad2_ = 0
   CALL rflu_gpu_check(rflu_gpu_create_region(desc, handle), 'rflu_gpu_create_region')
 
 ! ------------------------------------------------------------------------------

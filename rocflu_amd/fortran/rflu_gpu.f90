@@ -26,6 +26,7 @@ MODULE rflu_gpu
                                RFLU_ERR_HIP = 4, RFLU_ERR_NOMEM = 5, &
                                RFLU_ERR_HANDLE = 6, RFLU_ERR_FILE = 7, &
                                RFLU_ERR_DUMP = 8
+  INTEGER(C_INT32_T), PARAMETER :: RFLU_XCHG_STATE = 1, RFLU_XCHG_GRAD = 2
 
   TYPE, BIND(C) :: rflu_region_desc
     INTEGER(C_INT32_T) :: index_base, n_cells, n_faces, n_patches
@@ -41,6 +42,7 @@ MODULE rflu_gpu
     REAL(C_DOUBLE) :: creconst_weight
     INTEGER(C_INT32_T) :: creconst, pad_
     TYPE(C_PTR) :: patch_opt2
+    INTEGER(C_INT32_T) :: n_virtual, pad2_
   END TYPE rflu_region_desc
 
   TYPE, BIND(C) :: rflu_gpu_stats
@@ -132,6 +134,23 @@ MODULE rflu_gpu
       IMPORT :: C_INT, C_INT32_T, C_DOUBLE
       INTEGER(C_INT32_T), VALUE :: handle
       REAL(C_DOUBLE), INTENT(OUT) :: rhs(*)
+    END FUNCTION
+
+    ! Regions coupled through virtual cells: borders (t_border) and the halo copy
+    ! (RFLU_MPI_CopyWrapper / RFLU_MPI_ExchangeGradCells for regions on one device).
+    INTEGER(C_INT) FUNCTION rflu_gpu_set_borders(handle, n_borders, nbr_handle, nbr_border, &
+        send_ptr, send_cells, recv_ptr, recv_cells) BIND(C, NAME='rflu_gpu_set_borders')
+      IMPORT :: C_INT, C_INT32_T
+      INTEGER(C_INT32_T), VALUE :: handle, n_borders
+      INTEGER(C_INT32_T), INTENT(IN) :: nbr_handle(*), nbr_border(*), send_ptr(*), &
+                                        send_cells(*), recv_ptr(*), recv_cells(*)
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_gpu_exchange(n_regions, handles, what) &
+        BIND(C, NAME='rflu_gpu_exchange')
+      IMPORT :: C_INT, C_INT32_T
+      INTEGER(C_INT32_T), VALUE :: n_regions, what
+      INTEGER(C_INT32_T), INTENT(IN) :: handles(*)
     END FUNCTION
 
     INTEGER(C_INT) FUNCTION rflu_gpu_time_steps(handle, nsteps, use_graph) &

@@ -77,6 +77,7 @@ class Mesh:
     n_patches: int
     n_verts: int
     a: dict = field(default_factory=dict)
+    n_virtual: int = 0          # virtual cells numbered after the n_cells actual ones
 
     def __getattr__(self, name):
         a = self.__dict__.get("a")

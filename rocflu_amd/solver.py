@@ -113,6 +113,7 @@ def region_desc(mesh: Mesh, num: Numerics, d: "nat.RegionDesc"):
     i32, f64 = ctypes.c_int32, ctypes.c_double
     d.index_base = 0
     d.n_cells = int(mesh.n_cells)
+    d.n_virtual = int(getattr(mesh, "n_virtual", 0))
     d.n_faces = int(mesh.n_faces)
     d.n_patches = int(mesh.n_patches)
     d.f2c = nat.ptr(k["f2c"], i32)
@@ -153,7 +154,8 @@ class GpuRegion:
     def __init__(self, mesh: Mesh, num: Numerics):
         self.mesh = mesh
         self.num = num
-        self.n = int(mesh.n_cells)
+        # actual cells, then virtual cells (a region of a multi-region problem)
+        self.n = int(mesh.n_cells) + int(getattr(mesh, "n_virtual", 0))
         L = nat.lib()
         d = nat.RegionDesc()
         keep = region_desc(mesh, num, d)
