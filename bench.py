@@ -301,6 +301,72 @@ def large_leg(args):
     return out
 
 
+def regions_leg(args):
+    """Regions coupled through virtual cells on one device (--regions K): the n^3 x 6
+    shock case split into K x-slabs, stepped through the per-callee entry points with the
+    halo exchanges, against the single region through the same entry points (states
+    compared bit for bit); the exchanges alone timed, with the bytes they move."""
+    from rocflu_amd import cases, regions as rgs
+    from rocflu_amd.solver import GpuRegion, n_stages
+    n, K, steps = args.regions_n, args.regions, args.regions_steps
+    m = cases.tet_box(n=n, jitter=0.08, seed=2)
+    num = build_case(args, 0)
+    cv0 = cases.planar_shock(m, num, 0.3 * n, mach=2.0, noise=0.01, seed=7)
+    x = m.cofg[:, 0]
+    part = np.minimum((K * (x - x.min()) / np.ptp(x)).astype(int), K - 1)
+    t0 = time.time()
+    parts = rgs.partition(m, part, order=2)
+    print(f"# regions: {K} slabs of {m.n_cells} tets, virtual cells "
+          f"{[p.mesh.n_virtual for p in parts]}, built in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+
+    def timed(stepper, sync):
+        for _ in range(args.warmup):
+            stepper()
+        sync()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            stepper()
+        sync()
+        return time.perf_counter() - t1
+
+    g = GpuRegion(m, num)
+    g.upload(cv0)
+    el1 = timed(g.runge_kutta, g.synchronize)
+    cv1 = g.download()[0]
+    g.close()
+    R = rgs.GpuRegions(parts, num)
+    try:
+        R.upload(cv0)
+        sync = lambda: [r.synchronize() for r in R.regions]  # noqa: E731
+        elK = timed(R.runge_kutta, sync)
+        cvK = R.download()[0]
+        reps = 20
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            R.exchange(rgs.XCHG_STATE)
+        sync()
+        el_state = (time.perf_counter() - t1) / reps
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            R.exchange(rgs.XCHG_GRAD)
+        sync()
+        el_grad = (time.perf_counter() - t1) / reps
+    finally:
+        R.close()
+    nv = int(sum(p.mesh.n_virtual for p in parts))
+    st = n_stages(num.rk)
+    return {"workload": f"{n}^3 x 6 Kuhn tets ({m.n_cells} cells) in {K} x-slab regions, "
+                        f"HLLC order 2 + Venkatakrishnan, SSP-RK3, FP64, per-callee entry points",
+            "regions": K, "virtual_cells": nv, "steps": steps,
+            "identical_to_single_region": bool(np.array_equal(cvK, cv1)),
+            "single_region_ms_per_step": 1e3 * el1 / steps,
+            "regions_ms_per_step": 1e3 * elK / steps,
+            "exchange_state_ms": 1e3 * el_state, "exchange_grad_ms": 1e3 * el_grad,
+            "halo_bytes_per_step": nv * (40 + 120) * st,
+            "value": m.n_cells * st * steps / elK}
+
+
 def adjoint_leg(args, order=1, p_in=100.0, dirs=32, steps=None, checkpoint=None,
                 extra_eps=()):
     """The discrete adjoint: a spherical blast (p = p_in inside 0.1 L) in a 55^3 x 6-tet
@@ -417,6 +483,10 @@ def main():
     ap.add_argument("--adj2-checkpoint", type=int, default=1)
     ap.add_argument("--no-adjoint2", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--regions", type=int, default=0,
+                    help="also run K regions coupled through virtual cells (0: off)")
+    ap.add_argument("--regions-n", type=int, default=40)
+    ap.add_argument("--regions-steps", type=int, default=5)
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain device copy of 512 MiB three times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -540,6 +610,11 @@ def main():
                                                 extra_eps=(1e-9,))
         except Exception as e:  # noqa: BLE001
             out["adjoint_order2"] = {"error": repr(e)}
+    if args.regions > 1:
+        try:
+            out["regions"] = regions_leg(args)
+        except Exception as e:  # noqa: BLE001
+            out["regions"] = {"error": repr(e)}
     if not args.no_cpu:
         cb = cpu_baseline(args)
         out["cpu_baseline"] = cb

@@ -273,10 +273,13 @@ class GpuRegions:
                 g.cell_gradients()
             if self.num.order == 2:
                 self.exchange(XCHG_GRAD)
+            # every region's work is queued on its own stream before the first check
+            # waits for one of them, so the regions run side by side
             for g in self.regions:
                 g.convective_fluxes(s)
                 g.rk_update(s, dt)
                 g.set_dependent_vars()
+            for g in self.regions:
                 g.after_update(s)
             self.exchange(XCHG_STATE)
         self.time = self.time + dt
