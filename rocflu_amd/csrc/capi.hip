@@ -554,9 +554,38 @@ void launch_gradient_tiles(Region* R, const Mesh& M, const State& S) {
     hipLaunchKernelGGL((k_gradient_ldsw<GT, V, 4 * U, TS>), dim3(R->n_tiles), dim3(TS), shm,
                        R->stream, M, S, T, R->tile_loc_len, (const GT*)R->gw, (GT*)R->grad, R->gate_sc());
   } else {
-    size_t shm = (size_t)8 * R->tile_cap * sizeof(double);
-    hipLaunchKernelGGL((k_gradient_lds<GT, V, U, TS>), dim3(R->n_tiles), dim3(TS), shm,
-                       R->stream, M, S, T, (GT*)R->grad, R->gate_sc());
+    // a compile-time LDS stride when it keeps the workgroups per CU (160 KiB of LDS)
+    const int cap = R->tile_cap;
+    auto per_cu = [](int c) { return (160 * 1024) / (8 * c * (int)sizeof(double)); };
+    int capc = 0;
+    for (int c : {1024, 1280, 1536, 2048})
+      if (cap <= c) {
+        if (per_cu(c) == per_cu(cap)) capc = c;
+        break;
+      }
+    const size_t shm = (size_t)8 * (capc ? capc : cap) * sizeof(double);
+    switch (capc) {
+      case 1024:
+        hipLaunchKernelGGL((k_gradient_lds<GT, V, U, TS, 1024>), dim3(R->n_tiles), dim3(TS), shm,
+                           R->stream, M, S, T, (GT*)R->grad, R->gate_sc());
+        break;
+      case 1280:
+        hipLaunchKernelGGL((k_gradient_lds<GT, V, U, TS, 1280>), dim3(R->n_tiles), dim3(TS), shm,
+                           R->stream, M, S, T, (GT*)R->grad, R->gate_sc());
+        break;
+      case 1536:
+        hipLaunchKernelGGL((k_gradient_lds<GT, V, U, TS, 1536>), dim3(R->n_tiles), dim3(TS), shm,
+                           R->stream, M, S, T, (GT*)R->grad, R->gate_sc());
+        break;
+      case 2048:
+        hipLaunchKernelGGL((k_gradient_lds<GT, V, U, TS, 2048>), dim3(R->n_tiles), dim3(TS), shm,
+                           R->stream, M, S, T, (GT*)R->grad, R->gate_sc());
+        break;
+      default:
+        hipLaunchKernelGGL((k_gradient_lds<GT, V, U, TS, 0>), dim3(R->n_tiles), dim3(TS), shm,
+                           R->stream, M, S, T, (GT*)R->grad, R->gate_sc());
+        break;
+    }
   }
 }
 

@@ -1672,14 +1672,16 @@ __device__ __forceinline__ void stage_tile(const Mesh& M, const State& S, const 
   }
 }
 
-template <typename GT, bool VENKAT, int U, int TS>
+// CAPC > 0: the LDS arrays' stride is that compile-time constant (>= T.cap), so the
+// arrays' offsets become immediates of the LDS loads instead of per-member additions.
+template <typename GT, bool VENKAT, int U, int TS, int CAPC>
 __global__ __launch_bounds__(TS) void k_gradient_lds(Mesh M, State S, Tiles T,
                                                       GT* __restrict__ grad,
                                                       const Scalars* __restrict__ sc) {
   extern __shared__ double lds[];
   if (gate(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
   const int N = M.N;
-  const int cap = T.cap;
+  const int cap = CAPC > 0 ? CAPC : T.cap;
   const double* lx = lds;
   const double* ly = lds + cap;
   const double* lz = lds + 2 * cap;
