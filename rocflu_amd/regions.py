@@ -159,6 +159,9 @@ def partition(mesh: Mesh, part, order: int = 2) -> List[RegionPart]:
         else:
             a["c2cs_ptr"] = np.zeros(n_loc + 1, dtype=np.int32)
             a["c2cs_idx"] = np.zeros(0, dtype=np.int32)
+        # no wall-face stencil members: constrained gradients are not taken here
+        a["c2cs_bptr"] = np.zeros(n_loc + 1, dtype=np.int32)
+        a["c2cs_bface"] = np.zeros((0, 2), dtype=np.int32)
         # cell -> face lists in the gather order (interior faces ascending, then
         # boundary faces), for the host side's use; the library builds its own
         F = len(fl)

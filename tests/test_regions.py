@@ -108,3 +108,43 @@ def test_partition_rejects_bad_maps():
     part[:10] = 2                              # region 1 has no cells
     with pytest.raises(ValueError):
         rg.partition(m, part)
+
+
+def test_virtual_cell_arguments_are_checked_without_a_device():
+    """Descriptor and border checks that report before any device call."""
+    import ctypes
+
+    from rocflu_amd import _native as nat
+    from rocflu_amd.solver import GpuRegion, Numerics
+    m = grid()
+    parts = rg.partition(m, slabs(m, 2), order=2)
+    sub = parts[0].mesh
+    vals = [[0.4, 0.0, 0.0, 1.1 / Numerics().gas_prime()[1], 1.2]] + [[0.0] * 5] * 5
+
+    def code(mesh, **kw):
+        with pytest.raises(nat.RfluError) as e:
+            GpuRegion(mesh, Numerics(order=2, reconst=11, patch_vals=vals, **kw))
+        return e.value.code
+
+    # constrained gradients are not taken with virtual cells
+    assert code(sub, creconst=1) == nat.RFLU_ERR_ARG
+    sub_bad = rg.Mesh(**{**sub.__dict__, "n_virtual": -1})
+    assert code(sub_bad) == nat.RFLU_ERR_ARG
+    L = nat.lib()
+    # a batch takes no virtual cells
+    from rocflu_amd.solver import region_desc
+    num = Numerics(order=2, reconst=11, patch_vals=vals)
+    descs = (nat.RegionDesc * 2)()
+    keep = [region_desc(sub, num, descs[0]), region_desc(sub, num, descs[1])]
+    h = ctypes.c_int32(0)
+    assert L.rflu_gpu_create_batch(2, descs, ctypes.byref(h)) == nat.RFLU_ERR_ARG
+    del keep
+    # unknown handles
+    hs = np.asarray([777], dtype=np.int32)
+    assert L.rflu_gpu_exchange(1, nat.ptr(hs, ctypes.c_int32), 1) == nat.RFLU_ERR_HANDLE
+    z = np.zeros(2, dtype=np.int32)
+    p = nat.ptr(z, ctypes.c_int32)
+    assert L.rflu_gpu_set_borders(777, 1, p, p, p, p, p, p) == nat.RFLU_ERR_HANDLE
+    if L.rflu_gpu_device_count() == 0:
+        # a valid region with virtual cells stops only at the missing device
+        assert code(sub) == nat.RFLU_ERR_HIP
