@@ -4,7 +4,8 @@
   FP64 results round like the reference's gfortran build).
 - ``librflu_mesh.so``: host region builder (g++).
 - ``fortran/librflu_gpu_f.so`` + ``fortran/rflu_driver``: the ISO_C_BINDING module and
-  a Fortran driver that calls the C-ABI in RungeKuttaMP's order (amdflang, when present).
+  a Fortran driver that calls the C-ABI in RungeKuttaMP's order (amdflang, when present);
+  ``fortran/rflu_regions_driver``: the same loop over regions coupled through virtual cells.
 """
 from __future__ import annotations
 
@@ -70,6 +71,11 @@ def build_fortran():
     out = os.path.join(fdir, "rflu_driver")
     _run([fc, "-O2", "-J", fdir, mod, drv, "-L" + HERE, "-lrocflu_amd", "-lrflu_mesh",
           "-Wl,-rpath,$ORIGIN/..", "-o", out], cwd=fdir)
+    # the host of several regions coupled through virtual cells
+    drv2 = os.path.join(fdir, "rflu_regions_driver.f90")
+    if os.path.exists(drv2):
+        _run([fc, "-O2", "-J", fdir, mod, drv2, "-L" + HERE, "-lrocflu_amd",
+              "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(fdir, "rflu_regions_driver")], cwd=fdir)
     return out
 
 
