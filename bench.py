@@ -315,7 +315,8 @@ def regions_leg(args):
     x = m.cofg[:, 0]
     part = np.minimum((K * (x - x.min()) / np.ptp(x)).astype(int), K - 1)
     t0 = time.time()
-    parts = rgs.partition(m, part, order=2)
+    # the WENO-XYZ blend reads virtual cells' raw gradients: two layers, else one
+    parts = rgs.partition(m, part, order=2, blend=args.reconst == 2)
     print(f"# regions: {K} slabs of {m.n_cells} tets, virtual cells "
           f"{[p.mesh.n_virtual for p in parts]}, built in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)

@@ -14,7 +14,8 @@ exchanges the gradients of the cells mirrored after the gradient pass
   further cell that a first-layer cell's stencil reaches; each layer in grid order.
   First-layer cells keep their whole stencils (second-layer cells none), so the raw
   gradients they form are those of the cells they mirror -- the WENO blend of an
-  actual cell reads them before the gradient exchange;
+  actual cell reads them before the gradient exchange (without the blend, blend=False
+  keeps the first layer only, without stencils);
 * interior faces: the grid's faces with an actual cell, in grid order and
   orientation; boundary faces: those of actual cells, patch by patch in grid order;
 * borders: one per neighbouring region s (ascending s): the actual cells that are
@@ -78,8 +79,12 @@ def _csr_rows(ptr: np.ndarray, idx: np.ndarray, rows: np.ndarray):
     return ptr2, idx[pos]
 
 
-def partition(mesh: Mesh, part, order: int = 2) -> List[RegionPart]:
-    """Regions of `mesh` for the cell-to-region map `part` (values 0..K-1)."""
+def partition(mesh: Mesh, part, order: int = 2, blend: bool = True) -> List[RegionPart]:
+    """Regions of `mesh` for the cell-to-region map `part` (values 0..K-1).
+
+    blend=False (order 2 without the WENO-XYZ blend): nothing reads a virtual cell's
+    gradient before the gradient exchange replaces it, so the virtual cells are the
+    first layer only, without stencils."""
     part = np.asarray(part, dtype=np.int64).reshape(-1)
     N = int(mesh.n_cells)
     if part.shape != (N,) or part.min() < 0:
@@ -116,7 +121,7 @@ def partition(mesh: Mesh, part, order: int = 2) -> List[RegionPart]:
             near[_csr_rows(sp, si, act)[1]] = True
         near &= ~isact
         v1 = np.flatnonzero(near)
-        if order == 2:
+        if order == 2 and blend:
             far = np.zeros(N, dtype=bool)
             far[_csr_rows(sp, si, v1)[1]] = True
             far &= ~isact & ~near
@@ -149,10 +154,11 @@ def partition(mesh: Mesh, part, order: int = 2) -> List[RegionPart]:
         a["bfc"] = bfc[bsel].copy()
         n_loc = len(cells)
         if order == 2:
-            # actual and first-layer cells keep their stencils, second-layer cells none
-            rows = np.concatenate([act, v1])
+            # actual and (with the blend) first-layer cells keep their stencils, the
+            # other virtual cells none
+            rows = np.concatenate([act, v1]) if blend else act
             p2, i2 = _csr_rows(sp, si, rows)
-            ptr = np.concatenate([p2, np.full(len(v2), p2[-1], dtype=np.int64)])
+            ptr = np.concatenate([p2, np.full(n_loc - len(rows), p2[-1], dtype=np.int64)])
             a["c2cs_ptr"] = ptr.astype(np.int32)
             a["c2cs_idx"] = g2l[i2].astype(np.int32)
             assert np.all(a["c2cs_idx"] >= 0)

@@ -37,14 +37,16 @@ def partition_map(m, kind, k=3):
     return (c[:, 0] + 0.4 * c[:, 1] > 0).astype(int) + 2 * (c[:, 2] - 0.3 * c[:, 0] > 0)
 
 
-@pytest.mark.parametrize("order,discr,reconst,prec,kind", [
-    (1, 1, 0, 0, "slabs"), (1, 3, 0, 0, "blocks"),
-    (2, 3, 11, 0, "slabs"), (2, 3, 2, 0, "blocks"), (2, 4, 0, 0, "slabs"),
-    (2, 1, 11, 0, "blocks"), (2, 3, 11, 1, "slabs"), (2, 3, 2, 1, "blocks")])
-def test_regions_match_the_single_region(gpu, order, discr, reconst, prec, kind):
+@pytest.mark.parametrize("order,discr,reconst,prec,kind,blend", [
+    (1, 1, 0, 0, "slabs", True), (1, 3, 0, 0, "blocks", True),
+    (2, 3, 11, 0, "slabs", True), (2, 3, 2, 0, "blocks", True), (2, 4, 0, 0, "slabs", True),
+    (2, 1, 11, 0, "blocks", True), (2, 3, 11, 1, "slabs", True), (2, 3, 2, 1, "blocks", True),
+    (2, 3, 11, 0, "blocks", False), (2, 4, 0, 1, "slabs", False)])
+def test_regions_match_the_single_region(gpu, order, discr, reconst, prec, kind, blend):
+    """blend=False: one virtual layer without stencils (enough without WENO-XYZ)."""
     m, num, cv0 = setup(order, discr, reconst, prec)
     part = partition_map(m, kind)
-    parts = rg.partition(m, part, order=order)
+    parts = rg.partition(m, part, order=order, blend=blend)
     nsteps = 12
     print(f"case: order={order} discr={discr} reconst={reconst} prec={prec} {kind} "
           f"regions={len(parts)} cells={m.n_cells} virtual={[p.mesh.n_virtual for p in parts]}",

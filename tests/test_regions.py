@@ -28,11 +28,12 @@ def blocks(m, seed=1):
     return ((c[:, 0] + 0.4 * c[:, 1] > 0).astype(int) + 2 * (c[:, 2] - 0.3 * c[:, 0] > 0))
 
 
-@pytest.mark.parametrize("order,kind", [(1, "slabs"), (2, "slabs"), (2, "blocks")])
-def test_partition_invariants(order, kind):
+@pytest.mark.parametrize("order,kind,blend", [(1, "slabs", True), (2, "slabs", True),
+                                              (2, "blocks", True), (2, "blocks", False)])
+def test_partition_invariants(order, kind, blend):
     m = grid(order=order)
     part = slabs(m, 3) if kind == "slabs" else blocks(m)
-    parts = rg.partition(m, part, order=order)
+    parts = rg.partition(m, part, order=order, blend=blend)
     K = len(parts)
     f2c = m.f2c.reshape(-1, 2)
     # every actual cell in exactly one region, in grid order
