@@ -87,6 +87,7 @@ def lib():
                                  ctypes.c_int]
         L.orc_rind_state.argtypes = [R, ctypes.c_int, _f64p, _f64p, ctypes.c_double, _f64p]
         L.orc_patch_flux.argtypes = [R, ctypes.c_int, _f64p, _f64p, ctypes.c_double, _f64p]
+        L.orc_interior_flux.argtypes = [R, _f64p, _f64p, _f64p, ctypes.c_int, _f64p]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _L = L
@@ -153,6 +154,18 @@ class Oracle:
         out = np.empty(5)
         self.L.orc_patch_flux(ctypes.byref(self.r), int(patch), _p(n, ctypes.c_double),
                               _p(q, ctypes.c_double), float(pl), _p(out, ctypes.c_double))
+        return out
+
+    def interior_flux(self, cv, dv, grad, f):
+        """Flux of interior face f alone (cv (N,5), dv (N,3), grad (N,5,3) at order 2)."""
+        cv = np.ascontiguousarray(cv, dtype=np.float64)
+        dv = np.ascontiguousarray(dv, dtype=np.float64)
+        grad = np.ascontiguousarray(np.zeros((self.n, 5, 3)) if grad is None else grad,
+                                    dtype=np.float64)
+        out = np.empty(5)
+        self.L.orc_interior_flux(ctypes.byref(self.r), _p(cv, ctypes.c_double),
+                                 _p(dv, ctypes.c_double), _p(grad, ctypes.c_double), int(f),
+                                 _p(out, ctypes.c_double))
         return out
 
     def dv(self, cv):

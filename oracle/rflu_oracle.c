@@ -639,7 +639,9 @@ static int patch_flux(const orc_region* R, int patch, const double* n, double rl
  * q = (rho, rho u, rho v, rho w, rho E) and pressure pl at a face with n = (nx, ny, nz,
  * |A|), and that face's flux.  out = (rho, rho u, rho v, rho w, rho E, p) of the
  * boundary state; injection faces give (rho, u, v, w, H, -) of the injected gas when
- * burning and (-, -, -, -, -, p_wall) otherwise. */
+ * burning and (-, -, -, -, -, p_wall) otherwise; outflow faces give the pressure the
+ * face flux uses (the prescribed one unless the option is supersonic, also where the
+ * state is copied, rocflu/RFLU_CentralFirstPatch.F90:595-599). */
 void orc_rind_state(const orc_region* R, int patch, const double* n, const double* q, double pl,
                     double* out) {
   double mm, rgas, gam;
@@ -658,6 +660,11 @@ void orc_rind_state(const orc_region* R, int patch, const double* n, const doubl
       rind_farfield_perf(R->cp, mm, n[0], n[1], n[2], v[0], v[3], v[4], v[1], v[2], q[0], q[1],
                          q[2], q[3], q[4], &out[0], &out[1], &out[2], &out[3], &out[4], &out[5]);
       break;
+    case BC_OUTFLOW:
+      out[5] = opt != BCOPT_SUPERSONIC ? R->patch_pout[patch] : pl;
+      bcond_outflow_perf(opt, out[5], n[0], n[1], n[2], R->cp, mm, q[0], q[1], q[2], q[3], q[4],
+                         pl, &out[0], &out[1], &out[2], &out[3], &out[4]);
+      break;
     case BC_INJECTION:
       if (v[0] > 0.0) {
         rind_inject_perf(R->cp, mm, n[0], n[1], n[2], v[0], v[1], pl, 0.0, &out[0], &out[1],
@@ -670,6 +677,13 @@ void orc_rind_state(const orc_region* R, int patch, const double* n, const doubl
     default:
       break;
   }
+}
+
+/* Test entry point: the flux of interior face f alone, from cv, dv and (order 2) the
+ * gradients grad(3,5,N), as ConvectiveFluxesMP evaluates it before the scatter. */
+void orc_interior_flux(const orc_region* R, const double* cv, const double* dv,
+                       const double* grad, int f, double* flx) {
+  interior_flux(R, cv, dv, grad, f, flx);
 }
 
 void orc_patch_flux(const orc_region* R, int patch, const double* n, const double* q, double pl,

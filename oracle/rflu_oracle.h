@@ -86,6 +86,8 @@ void orc_venkat(const orc_region* r, const double* var, double* grad);
 /* Boundary state and flux of one face of a patch (test entry points). */
 void orc_rind_state(const orc_region* r, int patch, const double* n, const double* q, double pl,
                     double* out);
+void orc_interior_flux(const orc_region* r, const double* cv, const double* dv,
+                       const double* grad, int f, double* flx);
 void orc_patch_flux(const orc_region* r, int patch, const double* n, const double* q, double pl,
                     double* flx);
 void orc_cell_gradients(const orc_region* r, double* cv, const double* dv, double* grad);

@@ -41,7 +41,7 @@ import numpy as np
 
 from . import _native as nat
 from .mesh import Mesh
-from .solver import GpuRegion, Numerics, n_stages
+from .solver import RECONST_WENO_XYZ, GpuRegion, Numerics, n_stages
 
 XCHG_STATE = 1
 XCHG_GRAD = 2
@@ -60,6 +60,8 @@ class RegionPart:
     mesh: Mesh               # n_cells actual cells, then mesh.n_virtual virtual cells
     cells: np.ndarray        # grid cell of every local cell (actual, then virtual)
     borders: List[Border] = field(default_factory=list)
+    order: int = 2           # the order the virtual layers were built for
+    blend: bool = True       # whether the virtual cells next to actual cells keep stencils
 
     @property
     def n_act(self) -> int:
@@ -184,7 +186,7 @@ def partition(mesh: Mesh, part, order: int = 2, blend: bool = True) -> List[Regi
         m = Mesh(kind=mesh.kind, dims=mesh.dims, n_cells=len(act), n_faces=F,
                  n_bfaces=int(len(bsel)), n_patches=P, n_verts=0, a=a,
                  n_virtual=int(len(v1) + len(v2)))
-        parts.append(RegionPart(mesh=m, cells=cells))
+        parts.append(RegionPart(mesh=m, cells=cells, order=order, blend=bool(blend)))
 
     # borders
     owner_of = part
@@ -207,11 +209,36 @@ def partition(mesh: Mesh, part, order: int = 2, blend: bool = True) -> List[Regi
     return parts
 
 
+def check_parts(parts: List[RegionPart], num: Numerics) -> None:
+    """Rejects regions whose virtual layers cannot carry `num`'s scheme: at order 2 with
+    the WENO-XYZ blend an actual cell blends the raw gradients of the virtual cells in
+    its stencil before the gradient exchange replaces them, so those virtual cells need
+    stencils of their own (partition(..., blend=True)); a run without them would read
+    gradients that were never formed."""
+    for r, p in enumerate(parts):
+        if num.order > p.order:
+            raise ValueError(f"region {r} was partitioned for order {p.order}, the scheme is "
+                             f"order {num.order}")
+        if num.order != 2 or num.reconst != RECONST_WENO_XYZ:
+            continue
+        m, na = p.mesh, p.n_act
+        ptr = np.asarray(m.c2cs_ptr, dtype=np.int64)
+        idx = np.asarray(m.c2cs_idx, dtype=np.int64)
+        read = np.unique(idx[:ptr[na]])
+        read = read[read >= na]
+        empty = read[ptr[read + 1] == ptr[read]]
+        if empty.size:
+            raise ValueError(f"region {r}: {empty.size} virtual cells in actual cells' stencils "
+                             "have no stencil of their own; WENO-XYZ needs "
+                             "partition(..., blend=True)")
+
+
 class GpuRegions:
     """The regions of one problem on the GPU, advanced as RungeKuttaMP advances its
     regions (libfloflu/RungeKuttaMP.F90:148-508, rocflu/RFLU_TimeStepping.F90:429-640)."""
 
     def __init__(self, parts: List[RegionPart], num: Numerics):
+        check_parts(parts, num)
         self.parts = parts
         self.num = num
         self.n = int(sum(p.n_act for p in parts))

@@ -31,7 +31,8 @@ __global__ void k_check(int set, uint64_t n, uint64_t seed, unsigned long long* 
     } else if (set == 1) {
       x = ldexp(m, (int)(r % 55) - 27);
     } else if (set == 2) {
-      const double k = (double)(1 + (r % 100000000ULL));
+      // k < 2^26 keeps k*k below 2^52, so the product is exact: a perfect square
+      const double k = (double)(1 + (r % ((1ULL << 26) - 1)));
       const uint64_t b = (uint64_t)__double_as_longlong(k * k);
       const int w = (int)(i % 3);
       x = __longlong_as_double((long long)(w == 0 ? b : w == 1 ? b + 1 : b - 1));

@@ -149,3 +149,22 @@ def test_virtual_cell_arguments_are_checked_without_a_device():
     if L.rflu_gpu_device_count() == 0:
         # a valid region with virtual cells stops only at the missing device
         assert code(sub) == nat.RFLU_ERR_HIP
+
+
+def test_weno_needs_blend_partitions():
+    """WENO-XYZ blends raw gradients of the virtual cells in actual cells' stencils
+    before the gradient exchange: a partition built with blend=False gives those cells
+    no stencils and is refused before any device call; blend=True is accepted, and a
+    first-order partition is refused for a second-order scheme."""
+    from rocflu_amd.solver import Numerics
+    m = grid()
+    weno = Numerics(order=2, reconst=2)
+    with pytest.raises(ValueError, match="blend=True"):
+        rg.check_parts(rg.partition(m, slabs(m, 2), order=2, blend=False), weno)
+    rg.check_parts(rg.partition(m, slabs(m, 2), order=2, blend=True), weno)
+    rg.check_parts(rg.partition(m, slabs(m, 2), order=2, blend=False),
+                   Numerics(order=2, reconst=11))
+    with pytest.raises(ValueError, match="order 1"):
+        rg.check_parts(rg.partition(m, slabs(m, 2), order=1), Numerics(order=2, reconst=11))
+    with pytest.raises(ValueError, match="blend=True"):
+        rg.GpuRegions(rg.partition(m, slabs(m, 2), order=2, blend=False), weno)
