@@ -276,6 +276,18 @@ int rflu_gpu_residual_dev(int32_t handle, double* rhs_dev);
 /* drhs_dev = dR/dcv . dcv_dev + dR/dvals . dvals (dvals host (5, n_patches) or NULL). */
 int rflu_gpu_residual_jvp(int32_t handle, const double* dcv_dev, const double* dvals,
                           double* drhs_dev);
+/* rflu_gpu_residual_jvp without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels
+ * and bits as rflu_gpu_residual_jvp, whose first call allocates the tangent's work
+ * arrays; here they live in one static 64 MiB device array per GPU, so a region whose
+ * work arrays exceed it (5(F+B) + 5P doubles at first order; plus 35 N, or 65 N with
+ * WENO-XYZ, at second order, each rounded up to 256 bytes) gets RFLU_ERR_ARG.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (several host
+ * threads or regions on different streams) race and must be serialised by the caller.
+ * Returns after the work is done, like rflu_gpu_residual_jvp. */
+int rflu_gpu_residual_jvp_static(int32_t handle, const double* dcv_dev, const double* dvals,
+                                 double* drhs_dev);
 /* lam_dev = (dR/dcv)^T mu_dev; dvals (host (5, n_patches), or NULL) = (dR/dvals)^T mu. */
 int rflu_gpu_residual_vjp(int32_t handle, const double* mu_dev, double* lam_dev, double* dvals);
 

@@ -97,6 +97,7 @@ SIGNATURES = {
     "rflu_gpu_get_state_dev": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p]),
     "rflu_gpu_residual_dev": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p]),
     "rflu_gpu_residual_jvp": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, _f64p, ctypes.c_void_p]),
+    "rflu_gpu_residual_jvp_static": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, _f64p, ctypes.c_void_p]),
     "rflu_gpu_residual_vjp": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, _f64p]),
     "rflu_gpu_reset_stats": (ctypes.c_int, [ctypes.c_int32]),
     "rflu_gpu_synchronize": (ctypes.c_int, [ctypes.c_int32]),

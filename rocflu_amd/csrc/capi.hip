@@ -2025,6 +2025,34 @@ int rebuild_tiles(Region* R, int ts) {
 
 }  // namespace
 
+// The allocation-free tangent's work arrays live in one static array per GPU (the
+// code object's own data segment, allocated when the library's kernels are loaded on
+// that GPU and held until the process ends).  Host code addresses it only through the
+// device address hipGetSymbolAddress returns on that GPU.
+static constexpr size_t kScratchDoubles = (size_t)8 << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// The current GPU's address of `scratch` (looked up once per GPU, with it current).
+static int scratch_address(double** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(RFLU_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(RFLU_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+
 extern "C" {
 
 int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
@@ -2196,7 +2224,8 @@ int rflu_gpu_residual_dev(int32_t h, double* rhs) {
 }
 
 namespace {
-int derivative_ready(Region* R) {
+// Whether the region's residual has derivative kernels (no allocation).
+static int derivative_check(Region* R) {
   if (R->K != 1) return fail(RFLU_ERR_ARG, "derivatives are for a single region");
   if (R->nact < R->N) return fail(RFLU_ERR_ARG, "derivatives are for regions without virtual cells");
   if (R->prec != RFLU_PREC_FP64) return fail(RFLU_ERR_ARG, "derivatives are FP64");
@@ -2204,7 +2233,13 @@ int derivative_ready(Region* R) {
       R->reconst != RFLU_RECONST_WENO_XYZ)
     return fail(RFLU_ERR_ARG, "second-order derivatives cover no modifier, WENO-XYZ or the "
                               "Venkatakrishnan limiter");
-  int rc;
+  return 0;
+}
+
+// derivative_check, then the region's derivative buffers (allocated on first use).
+static int derivative_ready(Region* R) {
+  int rc = derivative_check(R);
+  if (rc) return rc;
   const size_t F = R->F, B = R->B, P = R->P, N = R->N;
   const bool o2 = R->order == 2;
   if (!R->adj_g && ((rc = dalloc(&R->adj_g, (o2 ? 20 : 10) * F)) ||
@@ -2278,6 +2313,80 @@ int derivative_ready(Region* R) {
 }
 }  // namespace
 
+namespace {
+// Device work arrays of the tangent: per-face tangent fluxes 5(F+B), the patch values'
+// tangents 5P (device copy, or null), and at order 2 the primitive tangents 5N, the
+// limited gradient's values and tangents 15N each and, with WENO-XYZ, the blended ones.
+struct JvpWork {
+  double *dflux, *dp, *ga, *gb, *gc, *gr;
+  const double* dvals;
+};
+
+// drhs = dR/dcv . dcv (+ dR/dvals . dvals) with the kernels of the tangent, on the
+// region's stream; returns after the stream has finished.
+static int jvp_launch(Region* R, const double* dcv, const JvpWork& w, double* drhs) {
+  Mesh M = R->mesh();
+  State S = R->state();
+  if (R->order == 2) {
+    // dprim, then the limited gradient's values and tangents, then the faces
+    const dim3 gc(nblk(R->N, 256)), bs(256);
+    hipLaunchKernelGGL(k_prim_jvp, gc, bs, 0, R->stream, M, S, R->gas, dcv, w.dp);
+    if (R->reconst == RFLU_RECONST_LIM_VENKAT)
+      hipLaunchKernelGGL(k_grad2_jvp<true>, gc, bs, 0, R->stream, M, R->constr_d(), S, (const double*)w.dp, w.ga, w.gb);
+    else
+      hipLaunchKernelGGL(k_grad2_jvp<false>, gc, bs, 0, R->stream, M, R->constr_d(), S, (const double*)w.dp, w.ga, w.gb);
+    const double *gv = w.ga, *gd = w.gb;
+    if (R->reconst == RFLU_RECONST_WENO_XYZ) {
+      // blend and clip the raw gradient's values and tangents
+      hipLaunchKernelGGL(k_weno_jvp, gc, bs, 0, R->stream, M, S, (const double*)w.ga,
+                         (const double*)w.gb, w.gc, w.gr);
+      gv = w.gc;
+      gd = w.gr;
+    }
+    if (R->F > 0) {
+      dim3 g(nblk(R->F, 256));
+      const double* dp = w.dp;
+      if (R->discr == 1) hipLaunchKernelGGL(k_face2_jvp<1>, g, bs, 0, R->stream, M, S, R->gas, dp, gv, gd, w.dflux);
+      else if (R->discr == 3) hipLaunchKernelGGL(k_face2_jvp<3>, g, bs, 0, R->stream, M, S, R->gas, dp, gv, gd, w.dflux);
+      else hipLaunchKernelGGL(k_face2_jvp<4>, g, bs, 0, R->stream, M, S, R->gas, dp, gv, gd, w.dflux);
+    }
+    if (R->B > 0)
+      hipLaunchKernelGGL(k_bface2_jvp, dim3(nblk(R->B, 256)), bs, 0, R->stream, M, S, R->gas, R->patches(),
+                         (const double*)w.dp, gv, gd, w.dvals, w.dflux);
+    hipLaunchKernelGGL((k_gather_rhs<double>), gc, bs, 0, R->stream, M, (const double*)w.dflux, drhs);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(R->stream));
+    return RFLU_OK;
+  }
+  if (R->F > 0) {
+    dim3 g(nblk(R->F, 256)), b(256);
+    if (R->discr == 1) hipLaunchKernelGGL(k_face_jvp<1>, g, b, 0, R->stream, M, S, R->gas, dcv, w.dflux);
+    else if (R->discr == 3) hipLaunchKernelGGL(k_face_jvp<3>, g, b, 0, R->stream, M, S, R->gas, dcv, w.dflux);
+    else hipLaunchKernelGGL(k_face_jvp<4>, g, b, 0, R->stream, M, S, R->gas, dcv, w.dflux);
+  }
+  if (R->B > 0)
+    hipLaunchKernelGGL(k_bface_jvp, dim3(nblk(R->B, 256)), dim3(256), 0, R->stream, M, S, R->gas,
+                       R->patches(), dcv, w.dvals, w.dflux);
+  hipLaunchKernelGGL((k_gather_rhs<double>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M,
+                     (const double*)w.dflux, drhs);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  return RFLU_OK;
+}
+
+// Doubles of the tangent's work arrays for region R, each rounded up to 256 bytes.
+static size_t jvp_work_doubles(const Region* R) {
+  auto up = [](size_t n) { return (n + 31) & ~(size_t)31; };
+  const size_t N = R->N;
+  size_t n = up(5 * ((size_t)R->F + R->B)) + up(5 * (size_t)R->P);
+  if (R->order == 2) {
+    n += up(5 * N) + 2 * up(15 * N);
+    if (R->reconst == RFLU_RECONST_WENO_XYZ) n += 2 * up(15 * N);
+  }
+  return n;
+}
+}  // namespace
+
 int rflu_gpu_residual_jvp(int32_t h, const double* dcv, const double* dvals, double* drhs) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
@@ -2286,54 +2395,46 @@ int rflu_gpu_residual_jvp(int32_t h, const double* dcv, const double* dvals, dou
   if (rc) return rc;
   if (dvals && R->P > 0)
     HIP_TRY(hipMemcpyAsync(R->adj_dvals, dvals, 5 * (size_t)R->P * sizeof(double), hipMemcpyHostToDevice, R->stream));
-  Mesh M = R->mesh();
-  State S = R->state();
+  JvpWork w{R->adj_dflux, R->adj_p, R->adj_ga, R->adj_gb2, R->adj_gc, R->adj_gr,
+            dvals ? (const double*)R->adj_dvals : nullptr};
+  return jvp_launch(R, dcv, w, drhs);
+}
+
+int rflu_gpu_residual_jvp_static(int32_t h, const double* dcv, const double* dvals, double* drhs) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (!dcv || !drhs) return fail(RFLU_ERR_ARG, "null array");
+  int rc = derivative_check(R);
+  if (rc) return rc;
+  const size_t need = jvp_work_doubles(R);
+  if (need > kScratchDoubles)
+    return fail(RFLU_ERR_ARG, "the allocation-free tangent needs " + std::to_string(need * 8) +
+                                  " bytes of work arrays; its static scratch holds " +
+                                  std::to_string(kScratchDoubles * 8) +
+                                  " (use rflu_gpu_residual_jvp)");
+  double* base = nullptr;
+  if ((rc = scratch_address(&base))) return rc;
+  auto up = [](size_t n) { return (n + 31) & ~(size_t)31; };
+  const size_t N = R->N;
+  double* q = base;
+  auto take = [&](size_t n) { double* r = q; q += up(n); return r; };
+  JvpWork w{};
+  w.dflux = take(5 * ((size_t)R->F + R->B));
+  double* dv = take(5 * (size_t)R->P);
   if (R->order == 2) {
-    // dprim, then the limited gradient's values and tangents, then the faces
-    const dim3 gc(nblk(R->N, 256)), bs(256);
-    hipLaunchKernelGGL(k_prim_jvp, gc, bs, 0, R->stream, M, S, R->gas, dcv, R->adj_p);
-    if (R->reconst == RFLU_RECONST_LIM_VENKAT)
-      hipLaunchKernelGGL(k_grad2_jvp<true>, gc, bs, 0, R->stream, M, R->constr_d(), S, (const double*)R->adj_p, R->adj_ga, R->adj_gb2);
-    else
-      hipLaunchKernelGGL(k_grad2_jvp<false>, gc, bs, 0, R->stream, M, R->constr_d(), S, (const double*)R->adj_p, R->adj_ga, R->adj_gb2);
-    const double *gv = R->adj_ga, *gd = R->adj_gb2;
+    w.dp = take(5 * N);
+    w.ga = take(15 * N);
+    w.gb = take(15 * N);
     if (R->reconst == RFLU_RECONST_WENO_XYZ) {
-      // blend and clip the raw gradient's values and tangents
-      hipLaunchKernelGGL(k_weno_jvp, gc, bs, 0, R->stream, M, S, (const double*)R->adj_ga,
-                         (const double*)R->adj_gb2, R->adj_gc, R->adj_gr);
-      gv = R->adj_gc;
-      gd = R->adj_gr;
+      w.gc = take(15 * N);
+      w.gr = take(15 * N);
     }
-    if (R->F > 0) {
-      dim3 g(nblk(R->F, 256));
-      const double *dp = R->adj_p;
-      if (R->discr == 1) hipLaunchKernelGGL(k_face2_jvp<1>, g, bs, 0, R->stream, M, S, R->gas, dp, gv, gd, R->adj_dflux);
-      else if (R->discr == 3) hipLaunchKernelGGL(k_face2_jvp<3>, g, bs, 0, R->stream, M, S, R->gas, dp, gv, gd, R->adj_dflux);
-      else hipLaunchKernelGGL(k_face2_jvp<4>, g, bs, 0, R->stream, M, S, R->gas, dp, gv, gd, R->adj_dflux);
-    }
-    if (R->B > 0)
-      hipLaunchKernelGGL(k_bface2_jvp, dim3(nblk(R->B, 256)), bs, 0, R->stream, M, S, R->gas, R->patches(),
-                         (const double*)R->adj_p, gv, gd,
-                         (const double*)(dvals ? R->adj_dvals : nullptr), R->adj_dflux);
-    hipLaunchKernelGGL((k_gather_rhs<double>), gc, bs, 0, R->stream, M, (const double*)R->adj_dflux, drhs);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(R->stream));
-    return RFLU_OK;
   }
-  if (R->F > 0) {
-    dim3 g(nblk(R->F, 256)), b(256);
-    if (R->discr == 1) hipLaunchKernelGGL(k_face_jvp<1>, g, b, 0, R->stream, M, S, R->gas, dcv, R->adj_dflux);
-    else if (R->discr == 3) hipLaunchKernelGGL(k_face_jvp<3>, g, b, 0, R->stream, M, S, R->gas, dcv, R->adj_dflux);
-    else hipLaunchKernelGGL(k_face_jvp<4>, g, b, 0, R->stream, M, S, R->gas, dcv, R->adj_dflux);
+  if (dvals && R->P > 0) {
+    HIP_TRY(hipMemcpyAsync(dv, dvals, 5 * (size_t)R->P * sizeof(double), hipMemcpyHostToDevice, R->stream));
+    w.dvals = dv;
   }
-  if (R->B > 0)
-    hipLaunchKernelGGL(k_bface_jvp, dim3(nblk(R->B, 256)), dim3(256), 0, R->stream, M, S, R->gas,
-                       R->patches(), dcv, (const double*)(dvals ? R->adj_dvals : nullptr), R->adj_dflux);
-  hipLaunchKernelGGL((k_gather_rhs<double>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M,
-                     (const double*)R->adj_dflux, drhs);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(R->stream));
-  return RFLU_OK;
+  return jvp_launch(R, dcv, w, drhs);
 }
 
 int rflu_gpu_residual_vjp(int32_t h, const double* mu, double* lam, double* dvals) {
