@@ -174,7 +174,8 @@ int rflu_gpu_download_state(int32_t handle, double* cv, double* dv);
 /* RFLU_TimeStepInviscid + RFLU_MinimumTimeStep (rocflu/RFLU_TimeStepInviscid.F90:
  * 153-334, rocflu/RFLU_MinimumTimeStep.F90:126-245) for a single region at the host's
  * global%currentTime: sets the device dtMin (and the device time to current_time).
- * If dt_out is not NULL, waits and returns it. */
+ * If dt_out is not NULL, waits and returns it, and returns the error of an after-update
+ * check that failed since the last read (the state stopped there). */
 int rflu_gpu_time_step(int32_t handle, double current_time, double* dt_out);
 
 /* RKInitMP (libfloflu/RkInitGeneric.F90:128-160). */
@@ -185,13 +186,27 @@ int rflu_gpu_cell_gradients(int32_t handle);
 int rflu_gpu_convective_fluxes(int32_t handle, int32_t istage);
 /* RKUpdateMP (libfloflu/RkUpdateGeneric.F90:255-325) with the residual gathered per
  * cell over the cell-to-face list in the reference's summation order; dt_min is the
- * host's global%dtMin (after its end-time clamp). */
+ * host's global%dtMin (after its end-time clamp).  Queued: launched by the next entry
+ * point (see rflu_gpu_after_update). */
 int rflu_gpu_rk_update(int32_t handle, int32_t istage, double dt_min);
 /* RFLU_SetVarsContWrapper over all cells (libflu/RFLU_SetDependentVars.F90:255-284). */
 int rflu_gpu_set_dependent_vars(int32_t handle);
-/* AfterUpdateMP (libfloflu/AfterUpdateMP.F90:134-136); waits and returns
- * RFLU_ERR_INVALID_VALUE / RFLU_ERR_NEGATIVE_POSDEF like the reference's ErrorStop. */
+/* AfterUpdateMP (libfloflu/AfterUpdateMP.F90:134-136): RFLU_CheckValidity,
+ * RFLU_EnforceBounds, RFLU_CheckPositivity on the device; does not wait.  A failed
+ * check stops the region where the reference's ErrorStop stops it: every later launch
+ * leaves its state alone, and the next call that reads the device (rflu_gpu_time_step
+ * with dt_out, rflu_gpu_check_status, rflu_gpu_time_steps) returns
+ * RFLU_ERR_INVALID_VALUE or RFLU_ERR_NEGATIVE_POSDEF.  The update of
+ * rflu_gpu_rk_update and the dependent variables of rflu_gpu_set_dependent_vars are
+ * launched here with the checks as one pass over the cells when they were the calls
+ * just before (otherwise, and before any other entry point, as separate passes; the
+ * results are the same bits). */
 int rflu_gpu_after_update(int32_t handle, int32_t istage);
+/* Waits for the region's work and returns the status of its after-update checks
+ * (RFLU_OK, RFLU_ERR_INVALID_VALUE or RFLU_ERR_NEGATIVE_POSDEF).  No counterpart in
+ * the reference (whose checks stop the run where they fail): the per-callee host calls
+ * it where it needs to know, e.g. after its last step. */
+int rflu_gpu_check_status(int32_t handle);
 
 /* Residual R(cv) of the current state (gradients included for order 2), gathered
  * into rhs (5,n_cells); waits.  For inspection and tests. */
@@ -309,8 +324,12 @@ int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
  * and evaluates each tile's faces from there (FACE_TILE; 0, the default, is one
  * thread per face gathering its two cells from global memory), and whether the
  * update then reads the fluxes of each tile's own faces from an LDS copy
- * (UPDATE_TILE 1; only while face tiles exist; reads as 0 otherwise).  None changes
- * results. */
+ * (UPDATE_TILE 1; only while face tiles exist; reads as 0 otherwise).  None of these
+ * changes results.  GRAD_FAST 1 replaces the gradient's member loop by its moment form
+ * (per member one reciprocal of |d|^2 and fused multiply-adds, the cell's inverse
+ * moment matrix applied once; LDS tiles only, ahead of stored weights): the same least
+ * squares gradient with other rounding, so results differ from the reference's in the
+ * last bits (not part of the bit-identical set; 0 is the default). */
 #define RFLU_OPT_GRAD_LDS 1
 #define RFLU_OPT_GRAD_UNROLL 2
 #define RFLU_OPT_TILE_CAP 3
@@ -319,6 +338,7 @@ int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
 #define RFLU_OPT_GRAD_TILE 6
 #define RFLU_OPT_FACE_TILE 7
 #define RFLU_OPT_UPDATE_TILE 8
+#define RFLU_OPT_GRAD_FAST 9
 int rflu_gpu_set_option(int32_t handle, int32_t option, int32_t value);
 int rflu_gpu_get_option(int32_t handle, int32_t option, int32_t* value);
 

@@ -1330,6 +1330,9 @@ int orc_after_update(const orc_region* R, double* cv, double* dv, orc_stats* st)
     }
   }
   if (st) st->n_bounds_fix += nfix;
+  /* RFLU_CheckPositivity.F90:147-253: every cell is floored and counted, and the error
+   * stop comes after the loop */
+  int64_t nlocs = 0;
   for (size_t i = 0; i < N; ++i) {
     double rho = cv[5 * i], rrho = 1.0 / rho;
     double u = rrho * cv[5 * i + 1], v = rrho * cv[5 * i + 2], w = rrho * cv[5 * i + 3];
@@ -1342,9 +1345,9 @@ int orc_after_update(const orc_region* R, double* cv, double* dv, orc_stats* st)
       orc_set_dependent_vars(R, cv, dv, (int)i, (int)i + 1);
       p = dv[3 * i];
     }
-    if (rho <= 0.0 || p <= 0.0) return ORC_ERR_NEGATIVE_POSDEF;
+    if (rho <= 0.0 || p <= 0.0) nlocs++;
   }
-  return ORC_OK;
+  return nlocs > 0 ? ORC_ERR_NEGATIVE_POSDEF : ORC_OK;
 }
 
 /* RFLU_TimeStepInviscid, rocflu/RFLU_TimeStepInviscid.F90:153-334: spectral-radius

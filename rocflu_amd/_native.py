@@ -82,6 +82,7 @@ SIGNATURES = {
     "rflu_gpu_set_end_time": (ctypes.c_int, [ctypes.c_int32, ctypes.c_double]),
     "rflu_gpu_set_dependent_vars": (ctypes.c_int, [ctypes.c_int32]),
     "rflu_gpu_after_update": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32]),
+    "rflu_gpu_check_status": (ctypes.c_int, [ctypes.c_int32]),
     "rflu_gpu_residual": (ctypes.c_int, [ctypes.c_int32, _f64p]),
     "rflu_gpu_time_steps": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32]),
     "rflu_gpu_set_borders": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _i32p, _i32p, _i32p,

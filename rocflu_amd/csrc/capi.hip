@@ -118,6 +118,14 @@ struct Region {
   bool use_lds = false;       // tiles were built (every tile fits in LDS)
   bool grad_lds = true;       // tuning: stage stencils in LDS when tiles exist
   int grad_unroll = 4;        // tuning: stencil members whose weights are formed together
+  bool grad_fast = false;     // tuning: moment form of the gradient (LDS tiles; not bit-identical)
+  // per-callee path: the update asked for by rflu_gpu_rk_update (stage, dt) and the
+  // dependent variables of rflu_gpu_set_dependent_vars, not launched yet; the next
+  // rflu_gpu_after_update launches them with its checks as one fused update, any other
+  // entry point launches them separately first (flush_pending): the same bits
+  int pend_stage = 0;
+  double pend_dt = 0.0;
+  bool pend_dv = false;
   int nxcd = 8;               // tuning: XCD-aware block order (MI355X: 8 XCDs); 1 = off
   int n_dt_parts = 0;
   int face_block = 256, cell_block = 256, grad_block = 256;
@@ -549,7 +557,7 @@ Tiles region_tiles(const Region* R) {
 template <typename GT, bool V, int U, int TS>
 void launch_gradient_tiles(Region* R, const Mesh& M, const State& S) {
   Tiles T = region_tiles(R);
-  if (R->gw) {
+  if (R->gw && !R->grad_fast) {
     size_t shm = (size_t)5 * R->tile_cap * sizeof(double);
     hipLaunchKernelGGL((k_gradient_ldsw<GT, V, 4 * U, TS>), dim3(R->n_tiles), dim3(TS), shm,
                        R->stream, M, S, T, R->tile_loc_len, (const GT*)R->gw, (GT*)R->grad, R->gate_sc());
@@ -564,6 +572,20 @@ void launch_gradient_tiles(Region* R, const Mesh& M, const State& S) {
         break;
       }
     const size_t shm = (size_t)8 * (capc ? capc : cap) * sizeof(double);
+    if (R->grad_fast) {
+      switch (capc) {
+#define RFLU_MOM(C)                                                                          \
+  hipLaunchKernelGGL((k_gradient_mom<GT, V, U, TS, C>), dim3(R->n_tiles), dim3(TS), shm, R->stream, \
+                     M, S, T, (GT*)R->grad, R->gate_sc())
+        case 1024: RFLU_MOM(1024); break;
+        case 1280: RFLU_MOM(1280); break;
+        case 1536: RFLU_MOM(1536); break;
+        case 2048: RFLU_MOM(2048); break;
+        default: RFLU_MOM(0); break;
+#undef RFLU_MOM
+      }
+      return;
+    }
     switch (capc) {
       case 1024:
         hipLaunchKernelGGL((k_gradient_lds<GT, V, U, TS, 1024>), dim3(R->n_tiles), dim3(TS), shm,
@@ -772,9 +794,10 @@ int launch_fluxes(Region* R, int stage, bool srad = false) {
   return 0;
 }
 
-int launch_update(Region* R, int stage, bool fused, double dt_in = -1.0) {
+int launch_update(Region* R, int stage, bool fused, double dt_in = -1.0, bool advance = true) {
   Timed t(R, KID_UPDATE);
-  int last = stage == (R->rk == 1 ? 4 : R->rk == 3 ? 1 : 3);
+  // the last stage of a fused step advances the device clock (not for the host's stages)
+  int last = advance && stage == (R->rk == 1 ? 4 : R->rk == 3 ? 1 : 3);
   bool prim = R->order == 2;
   if (!flux32(R)) {
     if (fused) return prim ? launch_update_rk<double, true, true>(R, stage, last, dt_in)
@@ -866,6 +889,7 @@ int read_scalars(Region* R, Scalars* h) {
     h->n_pos_fix += s.n_pos_fix;
     h->steps = std::min(h->steps, s.steps);
     h->abort |= s.abort;
+    h->fail |= s.fail;
     h->last_step &= s.last_step;
     h->finished &= s.finished;
   }
@@ -1412,6 +1436,31 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
 
 }  // namespace
 
+// Dependent variables of every cell (and the primitive copy at order 2).
+int launch_set_dv(Region* R) {
+  Mesh M = R->mesh();
+  State S = R->state();
+  hipLaunchKernelGGL(k_set_dv, dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S, R->gas, R->cases());
+  if (R->order == 2)
+    hipLaunchKernelGGL(k_set_prim, dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S);
+  HIP_TRY(hipGetLastError());
+  return RFLU_OK;
+}
+
+// Launches what rflu_gpu_rk_update and rflu_gpu_set_dependent_vars left pending, as
+// separate kernels (the state they would have produced, before any other use).
+int flush_pending(Region* R) {
+  if (R->pend_stage == 0) return 0;
+  const int s = R->pend_stage;
+  const bool dv = R->pend_dv;
+  R->pend_stage = 0;
+  R->pend_dv = false;
+  int rc = launch_update(R, s, false, R->pend_dt);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());
+  return dv ? launch_set_dv(R) : 0;
+}
+
 extern "C" {
 
 const char* rflu_gpu_last_error(void) { return g_err.c_str(); }
@@ -1444,6 +1493,7 @@ int rflu_gpu_destroy_region(int32_t h) {
 int rflu_gpu_upload_state(int32_t h, const double* cv, const double* dv, double time) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!cv) return fail(RFLU_ERR_ARG, "null cv");
   const int N = R->N;
   std::vector<double> hc((size_t)5 * N);
@@ -1474,6 +1524,7 @@ int rflu_gpu_upload_state(int32_t h, const double* cv, const double* dv, double 
 int rflu_gpu_download_state(int32_t h, double* cv, double* dv) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   const int N = R->N;
   std::vector<double> hc((size_t)5 * N), hd((size_t)3 * N);
   HIP_TRY(hipMemcpyAsync(hc.data(), R->cv, hc.size() * sizeof(double), hipMemcpyDeviceToHost, R->stream));
@@ -1491,15 +1542,19 @@ int rflu_gpu_download_state(int32_t h, double* cv, double* dv) {
 int rflu_gpu_time_step(int32_t h, double current_time, double* dt_out) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (R->K > 1) return fail(RFLU_ERR_ARG, "a batch of cases advances through rflu_gpu_time_steps");
   if (!(current_time >= 0.0)) return fail(RFLU_ERR_ARG, "current_time must be >= 0");
   int rc = launch_time_step(R, current_time);
   if (rc) return rc;
   HIP_TRY(hipGetLastError());
   if (dt_out) {
+    // the step's one read of the device: it also reports a check that failed in an
+    // earlier stage (the region's state stopped there)
     Scalars s;
     if ((rc = read_scalars(R, &s))) return rc;
     *dt_out = s.dt;
+    if ((rc = status_from(s))) return rc;
   }
   return RFLU_OK;
 }
@@ -1510,6 +1565,7 @@ int rflu_gpu_rk_init(int32_t h, int32_t istage) {
   // for inviscid runs, so rhs starts at zero inside the gather.
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (istage < 1 || istage > nstages(R->rk)) return fail(RFLU_ERR_ARG, "bad stage");
   return RFLU_OK;
 }
@@ -1517,6 +1573,7 @@ int rflu_gpu_rk_init(int32_t h, int32_t istage) {
 int rflu_gpu_cell_gradients(int32_t h) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   int rc = launch_gradients(R);
   if (rc) return rc;
   HIP_TRY(hipGetLastError());
@@ -1526,6 +1583,7 @@ int rflu_gpu_cell_gradients(int32_t h) {
 int rflu_gpu_convective_fluxes(int32_t h, int32_t istage) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (istage < 1 || istage > nstages(R->rk)) return fail(RFLU_ERR_ARG, "bad stage");
   int rc = launch_fluxes(R, istage);
   if (rc) return rc;
@@ -1539,28 +1597,42 @@ int rflu_gpu_rk_update(int32_t h, int32_t istage, double dt_min) {
   if (R->K > 1) return fail(RFLU_ERR_ARG, "a batch of cases advances through rflu_gpu_time_steps");
   if (istage < 1 || istage > nstages(R->rk)) return fail(RFLU_ERR_ARG, "bad stage");
   if (!(dt_min > 0.0)) return fail(RFLU_ERR_ARG, "dt_min must be > 0");
-  int rc = launch_update(R, istage, false, dt_min);
+  int rc = flush_pending(R);
   if (rc) return rc;
-  HIP_TRY(hipGetLastError());
+  R->pend_stage = istage;
+  R->pend_dt = dt_min;
+  R->pend_dv = false;
   return RFLU_OK;
 }
 
 int rflu_gpu_set_dependent_vars(int32_t h) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
-  Mesh M = R->mesh();
-  State S = R->state();
-  hipLaunchKernelGGL(k_set_dv, dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S, R->gas, R->cases());
-  if (R->order == 2)
-    hipLaunchKernelGGL(k_set_prim, dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S);
-  HIP_TRY(hipGetLastError());
-  return RFLU_OK;
+  if (R->pend_stage > 0 && !R->pend_dv) {
+    R->pend_dv = true;
+    return RFLU_OK;
+  }
+  int rc = flush_pending(R);
+  if (rc) return rc;
+  return launch_set_dv(R);
 }
 
 int rflu_gpu_after_update(int32_t h, int32_t istage) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
   if (istage < 1 || istage > nstages(R->rk)) return fail(RFLU_ERR_ARG, "bad stage");
+  if (R->pend_stage > 0 && R->pend_dv) {
+    // update, dependent variables, checks and the primitive copy in one pass
+    const int s = R->pend_stage;
+    R->pend_stage = 0;
+    R->pend_dv = false;
+    int rc = launch_update(R, s, true, R->pend_dt, false);
+    if (rc) return rc;
+    HIP_TRY(hipGetLastError());
+    return RFLU_OK;
+  }
+  int rc = flush_pending(R);
+  if (rc) return rc;
   Mesh M = R->mesh();
   State S = R->state();
   if (R->order == 2)
@@ -1568,9 +1640,16 @@ int rflu_gpu_after_update(int32_t h, int32_t istage) {
   else
     hipLaunchKernelGGL((k_after_update<false>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, S, R->gas, R->cases(), R->sc);
   HIP_TRY(hipGetLastError());
-  Scalars s;
-  int rc = read_scalars(R, &s);
+  return RFLU_OK;
+}
+
+int rflu_gpu_check_status(int32_t h) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  int rc = flush_pending(R);
   if (rc) return rc;
+  Scalars s;
+  if ((rc = read_scalars(R, &s))) return rc;
   return status_from(s);
 }
 
@@ -1580,6 +1659,7 @@ int rflu_gpu_set_borders(int32_t h, int32_t n_borders, const int32_t* nbr_handle
                          const int32_t* recv_cells) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (R->K != 1) return fail(RFLU_ERR_ARG, "borders belong to single regions");
   if (n_borders < 0 || (n_borders > 0 && (!nbr_handle || !nbr_border || !send_ptr || !recv_ptr)))
     return fail(RFLU_ERR_ARG, "bad border arguments");
@@ -1639,6 +1719,7 @@ int rflu_gpu_exchange(int32_t n_regions, const int32_t* handles, int32_t what) {
   for (int i = 0; i < n_regions; ++i) {
     rs[i] = get(handles[i]);
     if (!rs[i]) return fail(RFLU_ERR_HANDLE, "bad region handle");
+    if (int frc = flush_pending(rs[i])) return frc;
     if (rs[i]->K != 1) return fail(RFLU_ERR_ARG, "borders belong to single regions");
     if (what == RFLU_XCHG_GRAD && rs[i]->order != 2) return fail(RFLU_ERR_ARG, "gradients exist at order 2 only");
   }
@@ -1694,6 +1775,7 @@ int rflu_gpu_exchange(int32_t n_regions, const int32_t* handles, int32_t what) {
 int rflu_gpu_residual(int32_t h, double* rhs) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!rhs) return fail(RFLU_ERR_ARG, "null rhs");
   int rc;
   if ((rc = launch_gradients(R)) || (rc = launch_fluxes(R, 0))) return rc;
@@ -1715,6 +1797,7 @@ int rflu_gpu_residual(int32_t h, double* rhs) {
 int rflu_gpu_time_steps(int32_t h, int64_t nsteps, int32_t use_graph) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (nsteps < 0) return fail(RFLU_ERR_ARG, "nsteps < 0");
   if (R->nact < R->N)
     return fail(RFLU_ERR_ARG, "a region with virtual cells advances through the per-callee "
@@ -1749,6 +1832,7 @@ int rflu_gpu_time_steps(int32_t h, int64_t nsteps, int32_t use_graph) {
 int rflu_gpu_set_end_time(int32_t h, double end_time) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!(end_time > 0.0)) return fail(RFLU_ERR_ARG, "end_time must be > 0");
   R->end_time = end_time;
   for (CaseParams& cp : R->par_h) cp.end_time = end_time;
@@ -1762,6 +1846,7 @@ int rflu_gpu_set_end_time(int32_t h, double end_time) {
 int rflu_gpu_get_stats(int32_t h, rflu_gpu_stats* st) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!st) return fail(RFLU_ERR_ARG, "null stats");
   Scalars s;
   int rc = read_scalars(R, &s);
@@ -1782,6 +1867,7 @@ int rflu_gpu_get_stats(int32_t h, rflu_gpu_stats* st) {
 int rflu_gpu_get_case_stats(int32_t h, int32_t icase, rflu_gpu_stats* st) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!st || icase < 0 || icase >= R->K) return fail(RFLU_ERR_ARG, "bad case or null stats");
   std::vector<Scalars> v;
   int rc = read_cases(R, v);
@@ -1803,6 +1889,7 @@ int rflu_gpu_get_case_stats(int32_t h, int32_t icase, rflu_gpu_stats* st) {
 int rflu_gpu_reset_stats(int32_t h) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   hipLaunchKernelGGL(k_reset_counters, dim3(nblk(R->K, 64)), dim3(64), 0, R->stream, R->sc, R->K);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(R->stream));
@@ -1812,6 +1899,7 @@ int rflu_gpu_reset_stats(int32_t h) {
 int rflu_gpu_synchronize(int32_t h) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   HIP_TRY(hipStreamSynchronize(R->stream));
   return RFLU_OK;
 }
@@ -1819,6 +1907,7 @@ int rflu_gpu_synchronize(int32_t h) {
 int rflu_gpu_set_launch(int32_t h, int32_t face_block, int32_t cell_block, int32_t grad_block) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   auto ok = [](int b) { return b == 0 || (b >= 64 && b <= 1024 && b % 64 == 0); };
   if (!ok(face_block) || !ok(cell_block) || !ok(grad_block))
     return fail(RFLU_ERR_ARG, "block sizes must be multiples of 64 in [64, 1024]");
@@ -1832,6 +1921,7 @@ int rflu_gpu_set_launch(int32_t h, int32_t face_block, int32_t cell_block, int32
 int rflu_gpu_time_kernel(int32_t h, int32_t which, int32_t reps, double* ms) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!ms || reps < 1 || which < 0 || which > 5) return fail(RFLU_ERR_ARG, "bad arguments");
   hipEvent_t a, b;
   HIP_TRY(hipEventCreate(&a));
@@ -2058,6 +2148,7 @@ extern "C" {
 int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   switch (option) {
     case RFLU_OPT_GRAD_LDS:
       if (value != 0 && value != 1) return fail(RFLU_ERR_ARG, "GRAD_LDS is 0 or 1");
@@ -2088,6 +2179,12 @@ int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
       if (rc) return rc;
       break;
     }
+    case RFLU_OPT_GRAD_FAST:
+      if (value != 0 && value != 1) return fail(RFLU_ERR_ARG, "GRAD_FAST is 0 or 1");
+      if (value == 1 && (R->order != 2 || !R->use_lds))
+        return fail(RFLU_ERR_ARG, "GRAD_FAST needs a second-order region with LDS tiles");
+      R->grad_fast = value == 1;
+      break;
     case RFLU_OPT_UPDATE_TILE:
       if (value != 0 && value != 1) return fail(RFLU_ERR_ARG, "UPDATE_TILE is 0 or 1");
       R->update_tile = value == 1;
@@ -2130,6 +2227,7 @@ int rflu_gpu_get_option(int32_t h, int32_t option, int32_t* value) {
     case RFLU_OPT_GRAD_TILE: *value = R->grad_tile; break;
     case RFLU_OPT_FACE_TILE: *value = R->face_tile; break;
     case RFLU_OPT_UPDATE_TILE: *value = (R->update_tile && R->face_tile > 0) ? 1 : 0; break;
+    case RFLU_OPT_GRAD_FAST: *value = (R->grad_fast && R->use_lds && R->grad_lds) ? 1 : 0; break;
     default: return fail(RFLU_ERR_ARG, "unknown option");
   }
   return RFLU_OK;
@@ -2175,6 +2273,7 @@ int rflu_gpu_patch_faces(int32_t h, int32_t patch, int32_t* cells, double* areas
 int rflu_gpu_set_patch_values(int32_t h, int32_t patch, const double* vals) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (patch < 0 || patch >= R->P || !vals) return fail(RFLU_ERR_ARG, "bad arguments");
   for (int k = 0; k < 5; ++k) R->pvals_h[5 * (size_t)patch + k] = vals[k];
   HIP_TRY(hipMemcpyAsync(R->pvals + 5 * (size_t)patch, R->pvals_h.data() + 5 * (size_t)patch,
@@ -2186,6 +2285,7 @@ int rflu_gpu_set_patch_values(int32_t h, int32_t patch, const double* vals) {
 int rflu_gpu_set_state_dev(int32_t h, const double* cv) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!cv) return fail(RFLU_ERR_ARG, "null state");
   HIP_TRY(hipMemcpyAsync(R->cv, cv, (size_t)5 * R->N * sizeof(double), hipMemcpyDeviceToDevice, R->stream));
   Mesh M = R->mesh();
@@ -2201,6 +2301,7 @@ int rflu_gpu_set_state_dev(int32_t h, const double* cv) {
 int rflu_gpu_get_state_dev(int32_t h, double* cv) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!cv) return fail(RFLU_ERR_ARG, "null state");
   HIP_TRY(hipMemcpyAsync(cv, R->cv, (size_t)5 * R->N * sizeof(double), hipMemcpyDeviceToDevice, R->stream));
   HIP_TRY(hipStreamSynchronize(R->stream));
@@ -2210,6 +2311,7 @@ int rflu_gpu_get_state_dev(int32_t h, double* cv) {
 int rflu_gpu_residual_dev(int32_t h, double* rhs) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!rhs) return fail(RFLU_ERR_ARG, "null rhs");
   int rc;
   if ((rc = launch_gradients(R)) || (rc = launch_fluxes(R, 0))) return rc;
@@ -2390,6 +2492,7 @@ static size_t jvp_work_doubles(const Region* R) {
 int rflu_gpu_residual_jvp(int32_t h, const double* dcv, const double* dvals, double* drhs) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!dcv || !drhs) return fail(RFLU_ERR_ARG, "null array");
   int rc = derivative_ready(R);
   if (rc) return rc;
@@ -2403,6 +2506,7 @@ int rflu_gpu_residual_jvp(int32_t h, const double* dcv, const double* dvals, dou
 int rflu_gpu_residual_jvp_static(int32_t h, const double* dcv, const double* dvals, double* drhs) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!dcv || !drhs) return fail(RFLU_ERR_ARG, "null array");
   int rc = derivative_check(R);
   if (rc) return rc;
@@ -2440,6 +2544,7 @@ int rflu_gpu_residual_jvp_static(int32_t h, const double* dcv, const double* dva
 int rflu_gpu_residual_vjp(int32_t h, const double* mu, double* lam, double* dvals) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!mu || !lam) return fail(RFLU_ERR_ARG, "null array");
   int rc = derivative_ready(R);
   if (rc) return rc;
@@ -2519,6 +2624,7 @@ int rflu_gpu_residual_vjp(int32_t h, const double* mu, double* lam, double* dval
 int rflu_gpu_set_profiling(int32_t h, int32_t on) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   R->profiling = on != 0;
   for (int k = 0; k < KID_N; ++k) { R->kern_ms[k] = 0; R->kern_n[k] = 0; }
   return RFLU_OK;
@@ -2527,6 +2633,7 @@ int rflu_gpu_set_profiling(int32_t h, int32_t on) {
 int rflu_gpu_get_kernel_times(int32_t h, double* ms, int64_t* counts, int32_t n) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!ms || !counts || n < 1) return fail(RFLU_ERR_ARG, "bad arguments");
   for (int k = 0; k < n && k < KID_N; ++k) { ms[k] = R->kern_ms[k]; counts[k] = R->kern_n[k]; }
   return RFLU_OK;
@@ -2589,6 +2696,7 @@ int rflu_gpu_write_dump(int32_t h, int32_t icase, const char* path, int64_t step
                         int32_t region) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!path || icase < 0 || icase >= R->K) return fail(RFLU_ERR_ARG, "bad case or null path");
   const int N = R->N, c0 = R->cell0_h[icase], n = R->cell0_h[icase + 1] - c0;
   // device [5][N] in the region's cell order -> the case's cells in the caller's order
@@ -2614,6 +2722,7 @@ int rflu_gpu_read_dump(int32_t h, int32_t icase, const char* path, int64_t* step
                        int32_t* region) {
   Region* R = get(h);
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  if (int frc = flush_pending(R)) return frc;
   if (!path || icase < 0 || icase >= R->K) return fail(RFLU_ERR_ARG, "bad case or null path");
   const int N = R->N, c0 = R->cell0_h[icase], n = R->cell0_h[icase + 1] - c0;
   std::vector<double> var((size_t)5 * n);

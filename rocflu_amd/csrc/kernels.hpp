@@ -77,16 +77,34 @@ struct Scalars {
   double mass_in, mass_out;
   unsigned long long n_invalid, n_negative, n_bounds_fix, n_pos_fix;
   unsigned long long steps;
-  int abort;      // a failed check: every later launch of the region returns at once
+  int abort;      // a check failed in an earlier launch: every launch returns at once
   int last_step;  // the end-time clamp shortened this step (RFLU_TimeStepping.F90:445-448)
   int finished;   // set by the next step's time-step kernel after the last step
-  int pad;
+  int fail;       // a check failed (set by the launch that checks; promoted to abort by
+                  // the next launch that is not an update, so that the launch that finds
+                  // the failure still updates and checks every cell)
 };
 
+// Whether the update kernels skip the case: a failure of an earlier launch, or done.
+// They must not read `fail`, which their own launch may be setting.
 __device__ __forceinline__ bool halted(const Scalars* sc) { return sc->abort || sc->finished; }
+// Whether any other kernel skips it.
+__device__ __forceinline__ bool stopped(const Scalars* sc) {
+  return sc->abort || sc->fail || sc->finished;
+}
+// fail -> abort, by one thread of a launch that runs between two updates.
+__device__ __forceinline__ void promote(Scalars* sc) {
+  if (sc->fail && !sc->abort) sc->abort = 1;
+}
 // Early exit of a whole launch: the single-case scalars, or none (nullptr) for a batch
-// of cases, whose kernels run every case and leave halted cases to the update.
-__device__ __forceinline__ bool gate(const Scalars* sc) { return sc && halted(sc); }
+// of cases, whose kernels run every case and leave stopped cases to the update.
+// Promotes a failure found by the previous update.
+__device__ __forceinline__ bool gate(const Scalars* sc) {
+  if (!sc) return false;
+  if (sc->fail && !sc->abort && blockIdx.x == 0 && threadIdx.x == 0)
+    promote(const_cast<Scalars*>(sc));
+  return stopped(sc);
+}
 
 // Per-case time-step controls (RFLU_MinimumTimeStep inputs and global%maxTime).
 struct CaseParams {
@@ -966,7 +984,11 @@ __global__ void k_bface_flux(Mesh M, State S, Gas G, Cases C, Patches P,
   if (gate(sc)) return;
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= M.B) return;
-  if (C.cid) G = C.gas[C.cid[M.bf2c[b]]];
+  if (C.cid) {
+    const int c = C.cid[M.bf2c[b]];
+    G = C.gas[c];
+    if (b == C.bface0[c]) promote(C.sc + c);   // once per case and stage
+  }
   if (SRAD) srad[M.F + b] = bface_srad<ORDER>(M, S, G, P, b);
   double flx[5];
   boundary_face_flux<ORDER, GT>(M, S, G, P, grad, b, flx);
@@ -1092,7 +1114,7 @@ __device__ __forceinline__ void after_update_cell(const Gas& G, double* q, doubl
     double u = rrho * q[1], v = rrho * q[2], w = rrho * q[3];
     if (isnan(q[0]) || isnan(u) || isnan(v) || isnan(w) || isnan(p) || isnan(t)) {
       count_event(&sc->n_invalid);
-      sc->abort = 1;
+      sc->fail = 1;
     }
   }
   // RFLU_EnforceBounds.F90:169-231 (p recomputed with its own rounding, dv untouched)
@@ -1121,7 +1143,7 @@ __device__ __forceinline__ void after_update_cell(const Gas& G, double* q, doubl
     }
     if (rho <= 0.0 || p <= 0.0) {
       count_event(&sc->n_negative);
-      sc->abort = 1;
+      sc->fail = 1;
     }
   }
 }
@@ -1378,11 +1400,13 @@ __global__ void k_halo_copy(const T* __restrict__ src, int ns, T* __restrict__ d
 template <bool WRITE_PRIM>
 __global__ void k_after_update(Mesh M, State S, Gas G, Cases C, Scalars* sc) {
   const int N = M.N;
+  if (!C.cid && halted(sc)) return;   // stopped at an earlier stage's failure
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M.nact) return;   // the checks cover actual cells (AfterUpdateMP over nCells)
   if (C.cid) {
     G = C.gas[C.cid[i]];
     sc = C.sc + C.cid[i];
+    if (halted(sc)) return;
   }
   double q[5];
 #pragma unroll
@@ -1758,6 +1782,151 @@ __global__ __launch_bounds__(TS) void k_gradient_lds(Mesh M, State S, Tiles T,
   for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
 }
 
+// Moment form of the same least-squares gradient.  The member loop's weights are
+// w_j = term^2 (A d_j) with term = 1/|d_j|, d_j = x_j - x_i and A the cell's symmetric
+// matrix built from the R-factor coefficients (A = (sum_j d_j d_j^T / |d_j|^2)^-1), so
+// the gradient of variable v is A (sum_j (v_j - v_i) d_j / |d_j|^2).  Per member this
+// is one reciprocal of |d|^2 (hardware estimate and two Newton steps) and 21 fused
+// multiply-adds; A is applied once per cell.  Mathematically equal to the member loop,
+// not bit for bit (other rounding, fused operations), so it is a tuning option outside
+// the bit-identical set.
+template <typename T>
+struct MomMat {
+  T a00, a01, a02, a11, a12, a22;
+};
+
+template <typename T>
+__device__ __forceinline__ MomMat<T> mom_mat(const Mesh& M, int i) {
+  const GradCoef<double> c = grad_coef<double>(M, i);
+  MomMat<T> a;
+  a.a00 = T(c.c1111 + c.c12 * c.c12 * c.c2222 + c.c13 * c.c13 * c.c3333);
+  a.a01 = T(c.c12 * c.c2222 + c.c13 * c.c23 * c.c3333);
+  a.a02 = T(c.c13 * c.c3333);
+  a.a11 = T(c.c2222 + c.c23 * c.c23 * c.c3333);
+  a.a12 = T(c.c23 * c.c3333);
+  a.a22 = T(c.c3333);
+  return a;
+}
+
+__device__ __forceinline__ double recip_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ float recip_nr(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+__device__ __forceinline__ double tfma(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float tfma(float a, float b, float c) { return fmaf(a, b, c); }
+
+template <typename T>
+__device__ __forceinline__ void mom_member(double ddx, double ddy, double ddz, const double* dvar,
+                                           T* acc) {
+  const T dx = T(ddx), dy = T(ddy), dz = T(ddz);
+  const T inv = recip_nr(tfma(dz, dz, tfma(dy, dy, dx * dx)));
+  const T ex = dx * inv, ey = dy * inv, ez = dz * inv;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const T dv = T(dvar[k]);
+    acc[3 * k] = tfma(dv, ex, acc[3 * k]);
+    acc[3 * k + 1] = tfma(dv, ey, acc[3 * k + 1]);
+    acc[3 * k + 2] = tfma(dv, ez, acc[3 * k + 2]);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void mom_apply(const MomMat<T>& a, T* g) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const T x = g[3 * k], y = g[3 * k + 1], z = g[3 * k + 2];
+    g[3 * k] = tfma(a.a02, z, tfma(a.a01, y, a.a00 * x));
+    g[3 * k + 1] = tfma(a.a12, z, tfma(a.a11, y, a.a01 * x));
+    g[3 * k + 2] = tfma(a.a22, z, tfma(a.a12, y, a.a02 * x));
+  }
+}
+
+// The LDS-tile gradient (stage_tile, slot table, Venkatakrishnan limiter as in
+// k_gradient_lds) with the moment form of the member loop; U members per group.
+template <typename GT, bool VENKAT, int U, int TS, int CAPC>
+__global__ __launch_bounds__(TS) void k_gradient_mom(Mesh M, State S, Tiles T,
+                                                      GT* __restrict__ grad,
+                                                      const Scalars* __restrict__ sc) {
+  extern __shared__ double lds[];
+  if (gate(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
+  const int N = M.N;
+  const int cap = CAPC > 0 ? CAPC : T.cap;
+  const double* lx = lds;
+  const double* ly = lds + cap;
+  const double* lz = lds + 2 * cap;
+  const double* lv = lds + 3 * cap;  // 5 arrays: rho, u, v, w, p
+  const int t = xcd_block(M.nxcd);
+  const int i = t * T.ts + (int)threadIdx.x;
+  const bool live = i < N;
+  const int ic = live ? i : N - 1;
+  const double4 ci = M.cell[ic];
+  double vi[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) vi[k] = S.prim[(size_t)k * N + ic];
+  const int cnt = live ? M.sten_ptr[ic + 1] - M.sten_ptr[ic] : 0;
+  const unsigned short* L = T.loc + T.lptr[t] + threadIdx.x;
+  int nk[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) nk[u] = u < cnt ? (int)L[(size_t)u * T.ts] : 0;
+  const int u0 = T.uptr[t];
+  stage_tile<true>(M, S, T.uidx + u0, T.uptr[t + 1] - u0, cap, lds);
+  __syncthreads();
+  if (!live) return;
+  GT g[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) g[k] = GT(0);
+  int s = 0;
+  for (; s + U <= cnt; s += U) {
+    int kk[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      kk[u] = nk[u];
+      int sn = s + U + u;
+      nk[u] = sn < cnt ? (int)L[(size_t)sn * T.ts] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = kk[u];
+      double dvar[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) dvar[q] = lv[q * cap + k] - vi[q];
+      mom_member<GT>(lx[k] - ci.x, ly[k] - ci.y, lz[k] - ci.z, dvar, g);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (s + u < cnt) {
+      const int k = nk[u];
+      double dvar[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) dvar[q] = lv[q * cap + k] - vi[q];
+      mom_member<GT>(lx[k] - ci.x, ly[k] - ci.y, lz[k] - ci.z, dvar, g);
+    }
+  }
+  mom_apply<GT>(mom_mat<GT>(M, i), g);
+  if (VENKAT) {
+    const unsigned short* V = T.vloc + (size_t)t * M.vq * T.ts + threadIdx.x;
+    venkat_limit<GT>(M, i, vi, g,
+                     [&](int q) {
+                       int k = V[(size_t)q * T.ts];
+                       return k == 0xFFFF ? -1 : k;
+                     },
+                     [&](int k, double* vo) {
+#pragma unroll
+                       for (int r = 0; r < 5; ++r) vo[r] = lv[r * cap + k];
+                     });
+  }
+#pragma unroll
+  for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
+}
+
 // Stencil weights of every (cell, member) pair, formed once per region with the same
 // arithmetic as the member loop (the weights depend on the grid only).  Stored in
 // the tiles' [member][lane] order: w[d*L + lptr[t] + s*TS + lane] for component d.
@@ -2040,7 +2209,7 @@ __global__ void k_dt_gather(Mesh M, const double* __restrict__ srad, double* __r
   const int blk = xcd_block(M.nxcd);
   int i = blk * blockDim.x + threadIdx.x;
   double dtc = HUGE_VAL;
-  if (i < M.nact && !halted(sc)) dtc = M.cell[i].w / cell_srad_sum(M, srad, i);
+  if (i < M.nact && !stopped(sc)) dtc = M.cell[i].w / cell_srad_sum(M, srad, i);
   dtc = wave_min(dtc);
   int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) red[wv] = dtc;
@@ -2071,7 +2240,7 @@ __device__ __forceinline__ void finalize_dt(Scalars* sc, double mm, const CasePa
   if (dtMin <= 0.0) dtMin = 6.0E-12;
   dtMin = dtMin > cp.dt_min_limit ? dtMin : cp.dt_min_limit;
   if (sc->last_step) sc->finished = 1;
-  if (!halted(sc)) {
+  if (!stopped(sc)) {
     if (sc->time + dtMin > cp.end_time) {
       dtMin = cp.end_time - sc->time;
       sc->last_step = 1;
@@ -2110,7 +2279,7 @@ __global__ void k_dt_gather_cases(Mesh M, Cases C, const double* __restrict__ sr
   double dtc = HUGE_VAL;
   if (i < N) {
     c = C.cid[i];
-    if (!halted(C.sc + c)) dtc = M.cell[i].w / cell_srad_sum(M, srad, i);
+    if (!stopped(C.sc + c)) dtc = M.cell[i].w / cell_srad_sum(M, srad, i);
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int c0 = __shfl(c, 0, 64);
@@ -2161,7 +2330,7 @@ __global__ void k_mass_inout(Mesh M, Cases C, Patches P, const FT* __restrict__ 
   const int c = blockIdx.x;
   Scalars* sc = C.sc + c;
   double a = 0.0, b = 0.0;
-  const bool live = !halted(sc);
+  const bool live = !stopped(sc);
   if (live) {
     for (int k = C.bface0[c] + threadIdx.x; k < C.bface0[c + 1]; k += blockDim.x) {
       int p = M.bpatch[k];
@@ -2206,6 +2375,7 @@ __global__ void k_fill_scalars(Scalars* scs, int c0, int c1, double time,
   sc->abort = 0;
   sc->last_step = 0;
   sc->finished = 0;
+  sc->fail = 0;
   dtmin[(size_t)c * DTMIN_STRIDE] = 0x7FF0000000000000ULL;
 }
 
@@ -2222,6 +2392,7 @@ __global__ void k_reset_counters(Scalars* scs, int K) {
   sc->abort = 0;
   sc->last_step = 0;
   sc->finished = 0;
+  sc->fail = 0;
 }
 
 }  // namespace rflu

@@ -150,6 +150,8 @@ PROGRAM rflu_driver
     END DO
     time = time + dt
   END DO
+! the after-update checks of the last step (earlier ones surface at rflu_gpu_time_step)
+  CALL rflu_gpu_check(rflu_gpu_check_status(handle), 'AfterUpdateMP')
 
   CALL rflu_gpu_check(rflu_gpu_write_dump(handle, 0, TRIM(outfile)//'.dump'//C_NULL_CHAR, &
                                           INT(nsteps, C_INT64_T), time, 0), 'write_dump')

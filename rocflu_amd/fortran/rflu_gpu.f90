@@ -130,6 +130,11 @@ MODULE rflu_gpu
       INTEGER(C_INT32_T), VALUE :: handle, istage
     END FUNCTION
 
+    INTEGER(C_INT) FUNCTION rflu_gpu_check_status(handle) BIND(C, NAME='rflu_gpu_check_status')
+      IMPORT :: C_INT, C_INT32_T
+      INTEGER(C_INT32_T), VALUE :: handle
+    END FUNCTION
+
     INTEGER(C_INT) FUNCTION rflu_gpu_residual(handle, rhs) BIND(C, NAME='rflu_gpu_residual')
       IMPORT :: C_INT, C_INT32_T, C_DOUBLE
       INTEGER(C_INT32_T), VALUE :: handle

@@ -137,6 +137,10 @@ PROGRAM rflu_regions_driver
     END DO
     time = time + dt
   END DO
+! the after-update checks of the last step (earlier ones surface at rflu_gpu_time_step)
+  DO iReg = 1,nReg
+    CALL rflu_gpu_check(rflu_gpu_check_status(handles(iReg)), 'AfterUpdateMP')
+  END DO
 
   DO iReg = 1,nReg
     ASSOCIATE ( r => reg(iReg) )

@@ -240,7 +240,13 @@ class GpuRegion:
                   "rflu_gpu_set_dependent_vars")
 
     def after_update(self, stage: int):
+        """Queues the checks; a failure is reported by the next time_step() or
+        check_status() (the state stops where it failed)."""
         nat.check(self._lib.rflu_gpu_after_update(self.handle, stage), "rflu_gpu_after_update")
+
+    def check_status(self):
+        """Raises the error of a failed after-update check (waits for the device)."""
+        nat.check(self._lib.rflu_gpu_check_status(self.handle), "rflu_gpu_check_status")
 
     def runge_kutta(self, end_time: Optional[float] = None):
         """One time step through the separate entry points: RFLU_TimeStepping's dt and
@@ -302,6 +308,7 @@ class GpuRegion:
     OPT_GRAD_TILE = 6
     OPT_FACE_TILE = 7
     OPT_UPDATE_TILE = 8
+    OPT_GRAD_FAST = 9
 
     def set_option(self, option: int, value: int):
         nat.check(self._lib.rflu_gpu_set_option(self.handle, int(option), int(value)),

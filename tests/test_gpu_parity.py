@@ -456,3 +456,69 @@ def test_full_size_closed_box_conserves(gpu, prec):
               flush=True)
         assert abs(t - t0) <= 1e-12 * abs(t0)
     assert not np.array_equal(cv, cv0)
+
+
+@pytest.mark.parametrize("discr,reconst", [(3, 0), (4, 2), (1, 0)])
+def test_fast_gradient_within_fp64_bar(gpu, discr, reconst):
+    """The moment form of the gradient (GRAD_FAST: one reciprocal of |d|^2 and fused
+    multiply-adds per stencil member, the inverse moment matrix once per cell) is the
+    same least-squares gradient with other rounding: without a modifier or with the
+    WENO-XYZ blend, a Mach-2 shock with 1% noise on a 16^3 x 6 mesh (SSP-RK3) stays
+    within 1e-12 relative L2 per variable of the oracle after 100 steps, and differs
+    from it (it is not the bit-identical path)."""
+    m = cases.tet_box(n=16, jitter=0.08, seed=2)
+    num = Numerics(discr=discr, order=2, reconst=reconst, rk=4, cfl=0.5, dt_imposed=1e-4)
+    cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
+    print(f"case: fast gradient, cells={m.n_cells} discr={discr} reconst={reconst} steps=100",
+          flush=True)
+    from oracle import oracle as orc
+    orc.set_threads(8)
+    try:
+        cvo, _, st = Oracle(m, num).run(cv0, 100)
+    finally:
+        orc.set_threads(1)
+    assert st["rc"] == 0
+    outs = {}
+    for fast in (0, 1):
+        g = GpuRegion(m, num)
+        try:
+            g.set_option(GpuRegion.OPT_GRAD_FAST, fast)
+            assert g.get_option(GpuRegion.OPT_GRAD_FAST) == fast
+            g.upload(cv0)
+            g.time_steps(100)
+            outs[fast] = g.download()[0]
+        finally:
+            g.close()
+    assert np.array_equal(outs[0], cvo)
+    err = rel_l2(outs[1], cvo)
+    print("fast gradient rel L2 per variable after 100 steps:", err, flush=True)
+    assert np.all(err <= TOL_FP64), err
+    assert np.any(err > 0.0)
+
+
+def test_fast_gradient_with_venkatakrishnan_departs_from_the_bar(gpu):
+    """With the Venkatakrishnan limiter, differences at the rounding level grow: the
+    reference's additive TINY = 1e-12 in the limiter's denominator
+    (RFLU_ModLimiters.F90:505-518) makes phi change by O(1e-4) for rounding-level changes
+    of Delta_2 on faces where Delta_2 is below ~1e-12.  The moment form therefore leaves
+    the 1e-12 bar within 10 steps (so the tuner offers it only where results may differ
+    from the reference's), while the bit-identical path stays on the oracle."""
+    m = cases.tet_box(n=16, jitter=0.08, seed=2)
+    num = Numerics(discr=3, order=2, reconst=11, rk=4, cfl=0.5, dt_imposed=1e-4)
+    cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
+    print(f"case: fast gradient + Venkatakrishnan, cells={m.n_cells}", flush=True)
+    outs = {}
+    for fast, steps in ((0, 1), (1, 1), (0, 10), (1, 10)):
+        g = GpuRegion(m, num)
+        try:
+            g.set_option(GpuRegion.OPT_GRAD_FAST, fast)
+            g.upload(cv0)
+            g.time_steps(steps)
+            outs[fast, steps] = g.download()[0]
+        finally:
+            g.close()
+    e1 = rel_l2(outs[1, 1], outs[0, 1])
+    e10 = rel_l2(outs[1, 10], outs[0, 10])
+    print("one step:", e1, "ten steps:", e10, flush=True)
+    assert np.all(e1 <= 1e-14)
+    assert e10.max() > 1e3 * e1.max()

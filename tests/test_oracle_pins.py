@@ -367,3 +367,21 @@ def test_constrained_gradient_exact_on_admissible_linear_fields(kind):
     g2 = o.gradients_constr(u, g1)
     near = nb > 0
     assert np.abs(g1[near, 1]).max() <= 1e-12 and np.abs(g2[near, 1, 0]).max() > 1e-3
+
+
+def test_positivity_check_floors_every_cell_before_stopping():
+    """RFLU_CheckPositivity (libflu/RFLU_CheckPositivity.F90:147-253) floors the energy of
+    every cell with p <= 0 and counts the cells still negative; the error stop comes
+    after the loop, so a cell after the first failing one is still floored."""
+    m = rm.box(rm.HEX, 3, 1, 1, lengths=(3.0, 1.0, 1.0), jitter=0.0,
+               bc_type=(rm.BC_SLIPWALL,) * 6, stencils=False)
+    num = Numerics()
+    o = Oracle(m, num)
+    gp = num.gas_prime()[2]
+    cv = np.array([cases.prim_to_cons(1.0, 0.5, 0.0, 0.0, 1.0, gp) for _ in range(3)])
+    cv[0, 0] = -0.2                                  # negative density: the error
+    cv[2, 4] = 0.5 * 1.0 * 0.25 - 0.01               # p < 0 after it: floored
+    rc, cv2, dv2, st = o.after_update(cv, o.dv(cv))
+    assert rc == 2                                   # ERR_NEGATIVE_POSDEF
+    assert st["n_pos_fix"] == 1                      # cell 2 only (cell 0 has p > 0)
+    assert np.isclose(cv2[2, 4], 1.0 * (1.0e4 + 0.5 * 0.25)) and dv2[2, 0] > 0
