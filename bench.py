@@ -4,14 +4,16 @@
 Metric: cell-updates/s per RK stage (real cells x stages / wall time of the time-step
 loop: dt + all stages), FP64 and mixed precision, with the mixed run's error against
 the FP64 GPU run per conserved variable, and the dominant kernel's HBM roofline
-fraction.  Workload (default): 80^3 box split into 6 Kuhn tetrahedra per box cell
-(3,072,000 tets), interior nodes jittered +-8%, Mach-2 planar shock into quiescent gas
-with 1% seeded density noise, second-order HLLC with WLSQ gradients and the
-Venkatakrishnan limiter, SSP-RK3 (three stages, as the workload names it; the
-reference's own three-stage scheme, Wray RK3, runs the same kernels per stage), slip
-walls.  A step is one time step (dt + 3 stages) over the whole mesh; inputs are
-resident in HBM before the timed region.  Kernel variants (all bit-identical) are
-chosen by the project's autotuner before timing.
+fraction.  Headline workload (the largest single-GPU configuration): a 160^3 box split
+into 6 Kuhn tetrahedra per box cell (24,576,000 tets), interior nodes jittered +-8%,
+Mach-2 planar shock into quiescent gas with 1% seeded density noise, second-order HLLC
+with WLSQ gradients and the Venkatakrishnan limiter, SSP-RK3 (three stages), slip
+walls; FP64 through the bit-identical kernels.  Beside it: the same mesh in mixed
+precision, the same run through the per-callee (drop-in) entry points, the 80^3 x 6
+configuration (HLLC and AUSM+, FP64 and mixed), the 64-case batch, the adjoint and the
+CPU oracle.  A step is one time step (dt + 3 stages) over the whole mesh; inputs are
+resident in HBM before the timed region.  Kernel variants are chosen by the project's
+autotuner before timing (bit-identical ones only for FP64).
 
 Prints ONE JSON line.
 """
@@ -36,10 +38,14 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32
 KERNELS = ["face_flux", "update", "gradient", "blend", "time_step", "bface_flux"]
 
 
-def algorithmic_bytes(m, num, kernel: str, tuning: dict) -> float:
-    """Compulsory bytes of one launch: every array element the kernel needs, once, at
-    its stored width (double4 geometry counted at 32 B; re-reads of neighbours' data
-    through L2/LDS are not counted)."""
+def algorithmic_bytes(m, num, kernel: str, tuning: dict, compulsory: bool = False) -> float:
+    """Bytes of one launch: every array element the kernel needs, once, at its stored
+    width (double4 geometry counted at 32 B; re-reads of neighbours' data through L2/LDS
+    are not counted).  compulsory=True leaves out stored stencil weights (a design
+    choice that trades HBM bytes for arithmetic) and counts what forming them needs
+    (centroids and R-factor moments) instead."""
+    if compulsory and tuning.get("grad_weights"):
+        tuning = dict(tuning, grad_weights=0)
     N, F, B = int(m.n_cells), int(m.n_faces), int(m.n_bfaces)
     S = int(m.c2cs_idx.size) if num.order == 2 else 0
     K = int(m.c2f_idx.size)               # cell-to-face entries (2F + B)
@@ -76,8 +82,9 @@ def algorithmic_bytes(m, num, kernel: str, tuning: dict) -> float:
 
 
 def grad_variant(t: dict) -> str:
-    return "grad_t{}_lds{}_u{}_w{}".format(t.get("grad_tile", 128), t["grad_lds"],
-                                            t["grad_unroll"], t["grad_weights"])
+    v = "grad_t{}_lds{}_u{}_w{}".format(t.get("grad_tile", 128), t["grad_lds"],
+                                        t["grad_unroll"], t["grad_weights"])
+    return v + ("_fast" if t.get("grad_fast") else "")
 
 
 def gradient_flops(m, num, tuning: dict) -> float:
@@ -89,18 +96,31 @@ def gradient_flops(m, num, tuning: dict) -> float:
     N, F = int(m.n_cells), int(m.n_faces)
     S = int(m.c2cs_idx.size)
     per_member = 35 + (0 if tuning.get("grad_weights") else 31)
+    if tuning.get("grad_fast"):
+        per_member = 28              # moment form: |d|^2, reciprocal, 3 scalings, 15 FMAs
     fl = S * per_member + 30 * N
     if num.reconst == 11:
         fl += 2 * F * 115
     return float(fl)
 
 
-def build_case(args, precision):
-    from rocflu_amd import cases
+def build_case(args, precision, discr=None):
     from rocflu_amd.solver import Numerics
-    num = Numerics(discr=args.discr, order=2, reconst=args.reconst, rk=4, cfl=0.5,
-                   dt_imposed=1e-4, dt_min_limit=0.0, precision=precision)
+    num = Numerics(discr=args.discr if discr is None else discr, order=2,
+                   reconst=args.reconst, rk=4, cfl=0.5, dt_imposed=1e-4, dt_min_limit=0.0,
+                   precision=precision)
     return num
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def host_threads():
@@ -140,64 +160,78 @@ def cpu_baseline(args):
     same = all(np.array_equal(finals[1], f) for f in finals.values())
     return {"value": rates[tmax], "unit": "cell-updates/s", "cores": tmax, "kind": "port",
             "value_1thread": rates[1], "thread_speedup": rates[tmax] / rates[1],
-            "threads_same_bits": bool(same),
+            "threads_same_bits": bool(same), "cpu_model": cpu_model(),
             "sample": f"{n}^3x6 = {m.n_cells} tets, {steps} SSP-RK3 steps of the same scheme: "
                       f"{times[1]:.1f} s on 1 thread, {times[tmax]:.1f} s on {tmax} threads"}
 
 
-def run(args):
-    from rocflu_amd import cases
+def time_region(args, m, num, exact: bool, steps: int, err_steps: int, dropin: bool = False):
+    """One region of mesh m with numerics num: autotuned (exact: bit-identical variants
+    only), then `steps` fused steps timed three ways -- eager (the value), eager with HIP
+    events around every launch (per-kernel milliseconds), and as a replayed graph -- and
+    with dropin the same steps through the per-callee entry points (the reference
+    host's call sequence); finally err_steps graph steps from the initial state for the
+    accuracy comparison.  Every timed run starts from the same initial state."""
+    from rocflu_amd import cases, autotune
     from rocflu_amd.solver import GpuRegion
-    from rocflu_amd import autotune
     import torch  # device memory and streams are owned by the C-ABI library
+    cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
+    g = GpuRegion(m, num)
+    try:
+        g.upload(cv0)
+        tuning = autotune.tune(g, exact=exact)
+        tuning = {k: v for k, v in tuning.items() if k != "timings_ms"}
 
-    t0 = time.time()
-    m = cases.tet_box(n=args.n, jitter=0.08, seed=2)
-    print(f"# mesh {args.n}^3x6: {m.n_cells} tets, {m.n_faces} faces, "
-          f"{m.c2cs_idx.size} stencil entries, built in {time.time() - t0:.1f}s",
-          file=sys.stderr, flush=True)
-    results = {}
-    finals = {}
-    for prec in ([0, 1] if args.precision == "both" else [0 if args.precision == "fp64" else 1]):
-        num = build_case(args, prec)
-        cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
-        g = GpuRegion(m, num)
+        def timed(fn, n):
+            fn(args.warmup)
+            g.synchronize()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            fn(n)
+            g.synchronize()
+            torch.cuda.synchronize()
+            return time.perf_counter() - t1
+
         g.upload(cv0)
-        tuning = autotune.tune(g)
-        print(f"# prec {prec}: tuning {tuning}", file=sys.stderr, flush=True)
+        el = timed(lambda n: g.time_steps(n), steps)
         g.upload(cv0)
-        g.time_steps(args.warmup)
-        g.synchronize()
-        torch.cuda.synchronize()
         g.set_profiling(True)
-        t1 = time.perf_counter()
-        g.time_steps(args.steps)
-        g.synchronize()
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t1
+        el_prof = timed(lambda n: g.time_steps(n), steps)
         kt = g.kernel_times()
         g.set_profiling(False)
-        # graph replay of the same loop (launch overhead removed)
         g.upload(cv0)
-        g.time_steps(args.warmup, graph=True)
-        g.synchronize()
-        t2 = time.perf_counter()
-        g.time_steps(args.steps, graph=True)
-        g.synchronize()
-        elg = time.perf_counter() - t2
-        # accuracy leg: identical step count from the same state for both precisions
-        g.upload(cv0)
-        g.time_steps(args.err_steps, graph=True)
-        finals[prec] = g.download()[0]
-        st = g.stats()
+        elg = timed(lambda n: g.time_steps(n, graph=True), steps)
+        n_upd = m.n_cells * 3 * steps
+        res = {"value": n_upd / el, "ms_per_step": 1e3 * el / steps, "graph_value": n_upd / elg,
+               "graph_ms_per_step": 1e3 * elg / steps,
+               "profiled_ms_per_step": 1e3 * el_prof / steps,
+               "kernels": kt, "num": num, "tuning": tuning}
+        if dropin:
+            def callee(n):
+                for _ in range(n):
+                    g.runge_kutta()
+                g.check_status()
+            g.upload(cv0)
+            eld = timed(callee, steps)
+            res["dropin"] = {"value": n_upd / eld, "ms_per_step": 1e3 * eld / steps,
+                             "vs_fused": el / eld}
+        res["final"] = None
+        if err_steps:
+            g.upload(cv0)
+            g.time_steps(err_steps, graph=True)
+            res["final"] = g.download()[0]
+        res["stats"] = g.stats()
+    finally:
         g.close()
-        n_upd = m.n_cells * 3 * args.steps
-        results[prec] = {"value": n_upd / el, "ms_per_step": 1e3 * el / args.steps,
-                         "graph_value": n_upd / elg, "kernels": kt, "num": num, "stats": st,
-                         "tuning": {k: v for k, v in tuning.items() if k != "timings_ms"}}
-        print(f"# prec {prec}: {n_upd / el:.4g} cell-upd/s ({1e3 * el / args.steps:.3f} ms/step), "
-              f"graph {n_upd / elg:.4g}; kernels {kt}", file=sys.stderr, flush=True)
-    return m, results, finals
+    print(f"# {num.discr}/{num.precision} on {m.n_cells}: {res['value']:.4g} cell-upd/s "
+          f"({res['ms_per_step']:.3f} ms/step), graph {res['graph_value']:.4g}"
+          + (f", per-callee {res['dropin']['ms_per_step']:.3f} ms/step" if dropin else "")
+          + f"; tuning {tuning}; kernels {kt}", file=sys.stderr, flush=True)
+    return res
+
+
+def rel_l2_per_variable(a, b):
+    return [float(np.linalg.norm(a[:, k] - b[:, k]) / np.linalg.norm(b[:, k])) for k in range(5)]
 
 
 def batch_leg(args):
@@ -216,7 +250,7 @@ def batch_leg(args):
     b = GpuBatch([(m, num) for m, num, _ in ens])
     cvs = [cv0 for _, _, cv0 in ens]
     b.upload(cvs)
-    tuning = autotune.tune(b)
+    tuning = autotune.tune(b, exact=False)
     b.upload(cvs)
     b.time_steps(args.warmup, graph=True)
     b.synchronize()
@@ -250,54 +284,6 @@ def batch_leg(args):
                        "Venkatakrishnan, SSP-RK3, mixed precision, slip walls"}
     print(f"# batch: {out['value']:.4g} cell-upd/s batched, {out['sequential_value']:.4g} one "
           f"region at a time", file=sys.stderr, flush=True)
-    return out
-
-
-def large_leg(args):
-    """The large single-GPU mesh: 160^3 x 6 Kuhn tets (24.6M cells), the 80^3 case's
-    scheme, FP64 and mixed precision; per-variable error of the mixed run against the
-    FP64 run after the same number of steps."""
-    from rocflu_amd import cases, autotune
-    from rocflu_amd.solver import GpuRegion
-    t0 = time.time()
-    n = args.large_n
-    m = cases.tet_box(n=n, jitter=0.08, seed=2)
-    print(f"# large mesh {n}^3x6: {m.n_cells} tets in {time.time() - t0:.1f}s", file=sys.stderr,
-          flush=True)
-    out = {"workload": f"{n}^3 x 6 Kuhn tets ({m.n_cells} cells), jitter 8%, Mach-2 shock + 1% "
-                       f"density noise, HLLC order 2, WLSQ + Venkatakrishnan, SSP-RK3",
-           "cells": int(m.n_cells)}
-    finals = {}
-    for prec in (0, 1):
-        num = build_case(args, prec)
-        cv0 = cases.planar_shock(m, num, 0.3 * n, mach=2.0, noise=0.01, seed=7)
-        g = GpuRegion(m, num)
-        g.upload(cv0)
-        tuning = autotune.tune(g)
-        g.upload(cv0)
-        g.time_steps(args.warmup, graph=True)
-        g.synchronize()
-        t1 = time.perf_counter()
-        g.time_steps(args.large_steps, graph=True)
-        g.synchronize()
-        el = time.perf_counter() - t1
-        g.upload(cv0)
-        g.time_steps(args.large_err_steps, graph=True)
-        finals[prec] = g.download()[0]
-        g.close()
-        key = "fp64" if prec == 0 else "mixed"
-        out[key] = {"value": m.n_cells * 3 * args.large_steps / el,
-                    "ms_per_step": 1e3 * el / args.large_steps,
-                    "tuning": {k: v for k, v in tuning.items() if k != "timings_ms"}}
-        print(f"# large prec {prec}: {out[key]['value']:.4g} cell-upd/s", file=sys.stderr,
-              flush=True)
-    a, b = finals[1], finals[0]
-    out["err_steps"] = args.large_err_steps
-    out["mixed_rel_l2_per_variable"] = [
-        float(np.linalg.norm(a[:, k] - b[:, k]) / np.linalg.norm(b[:, k])) for k in range(5)]
-    mom = np.linalg.norm(b[:, 1:4])
-    out["mixed_rel_l2_momentum_vs_momentum_norm"] = [
-        float(np.linalg.norm(a[:, k] - b[:, k]) / mom) for k in (1, 2, 3)]
     return out
 
 
@@ -459,21 +445,23 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=80, help="box cells per side (6 tets each)")
+    ap.add_argument("--n", type=int, default=160,
+                    help="headline box cells per side (6 tets each): the largest single-GPU "
+                         "configuration, 160^3 x 6 = 24.6M tets")
     ap.add_argument("--discr", type=int, default=3, help="1 Roe, 3 HLLC, 4 AUSM+")
     ap.add_argument("--reconst", type=int, default=11)
-    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
-    ap.add_argument("--err-steps", type=int, default=200)
+    ap.add_argument("--err-steps", type=int, default=100,
+                    help="steps of the mixed-vs-FP64 comparison on the headline mesh")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--c2-n", type=int, default=80, help="the 80^3 x 6 configuration")
+    ap.add_argument("--c2-err-steps", type=int, default=200)
+    ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-cases", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--large-n", type=int, default=160)
-    ap.add_argument("--large-steps", type=int, default=5)
-    ap.add_argument("--large-err-steps", type=int, default=50)
-    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--adj-n", type=int, default=55)
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-checkpoint", type=int, default=20)
@@ -510,89 +498,118 @@ def main():
         print(json.dumps({"error": "run with one rank"}))
         sys.exit(2)
 
-    m, res, finals = run(args)
-    main_prec = 0 if 0 in res else 1
-    r = res[main_prec]
-    kt = r["kernels"]
+    from rocflu_amd import cases
+    flux_name = {1: "Roe", 3: "HLLC", 4: "AUSM+"}
+    lim_name = "Venkatakrishnan" if args.reconst == 11 else "WENO-XYZ"
+    t0 = time.time()
+    m = cases.tet_box(n=args.n, jitter=0.08, seed=2)
+    print(f"# mesh {args.n}^3x6: {m.n_cells} tets, {m.n_faces} faces, "
+          f"{m.c2cs_idx.size} stencil entries, built in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    fp = time_region(args, m, build_case(args, 0), exact=True, steps=args.steps,
+                     err_steps=0 if args.no_mixed else args.err_steps, dropin=True)
+    kt = fp["kernels"]
     dom = max(KERNELS, key=lambda k: kt[k][0])
     ms_launch = kt[dom][0] / max(kt[dom][1], 1)
-    ab = algorithmic_bytes(m, r["num"], dom, r["tuning"])
+    ab = algorithmic_bytes(m, fp["num"], dom, fp["tuning"])
     achieved = ab / (ms_launch * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_tet{args.n}_prec{main_prec}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_tet{args.n}_prec0.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             tj = json.load(f)
         # measured for the same kernel variant only
-        if dom != "gradient" or tj.get("variant") == grad_variant(r["tuning"]):
+        if dom != "gradient" or tj.get("variant") == grad_variant(fp["tuning"]):
             traffic = tj.get("bytes_per_launch")
+
+    def step_hbm(mesh, rr):
+        """Compulsory bytes of a whole step (three stages of gradient, fluxes and
+        update, one time step) over the timed step, against the HBM peak."""
+        per_stage = sum(algorithmic_bytes(mesh, rr["num"], k, rr["tuning"], compulsory=True)
+                        for k in ("gradient", "face_flux", "update"))
+        total = 3 * per_stage + algorithmic_bytes(mesh, rr["num"], "time_step", rr["tuning"])
+        gbs = total / (rr["ms_per_step"] * 1e-3) / 1e9
+        return {"compulsory_bytes_per_step": total, "achieved": gbs, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+
     out = {
         "metric": "cell-updates/s per RK stage, FP64 vs mixed at 1e-6 rel L2; % of HBM peak",
-        "value": r["value"],
+        "value": fp["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r["ms_per_step"],
+        "ms_per_step": fp["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if main_prec == 0 else "f32/f64",
+        "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.n}^3 x 6 Kuhn tets ({m.n_cells} cells, {m.n_faces} faces), "
-                               f"jitter 8%, Mach-2 shock + 1% density noise, "
-                               f"{ {1: 'Roe', 3: 'HLLC', 4: 'AUSM+'}[args.discr]} order 2, "
-                               f"WLSQ + {'Venkatakrishnan' if args.reconst == 11 else 'WENO-XYZ'}, "
-                               f"SSP-RK3, slip walls",
+        "config": {"workload": f"{args.n}^3 x 6 Kuhn tets ({m.n_cells} cells, {m.n_faces} "
+                               f"faces), jitter 8%, Mach-2 shock + 1% density noise, "
+                               f"{flux_name[args.discr]} order 2, WLSQ + {lim_name}, SSP-RK3, "
+                               f"slip walls, FP64 (bit-identical kernels)",
                    "cells": int(m.n_cells), "stages_per_step": 3},
-        "graph_value": r["graph_value"],
+        "graph_value": fp["graph_value"],
         "kernel_ms": {k: [v[0], v[1]] for k, v in kt.items()},
-        "tuning": r["tuning"],
+        "tuning": fp["tuning"],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": ab, "ms_per_launch": ms_launch},
+                     "algorithmic_bytes_per_launch": ab, "ms_per_launch": ms_launch,
+                     "variant": grad_variant(fp["tuning"]) if dom == "gradient" else dom},
+        "step_hbm": step_hbm(m, fp),
+        "dropin": fp["dropin"],
     }
-    # the whole stage loop against the same roofline: every kernel's compulsory bytes
-    # per step (three stages of gradient, fluxes and update, one time step) over the
-    # timed step
-    def step_hbm(prec):
-        rr = res[prec]
-        per_stage = sum(algorithmic_bytes(m, rr["num"], k, rr["tuning"])
-                        for k in ("gradient", "face_flux", "update"))
-        total = 3 * per_stage + algorithmic_bytes(m, rr["num"], "time_step", rr["tuning"])
-        gbs = total / (rr["ms_per_step"] * 1e-3) / 1e9
-        return {"algorithmic_bytes_per_step": total, "achieved": gbs, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
-    out["step_hbm"] = step_hbm(main_prec)
     if dom == "gradient":
         # the other side of the kernel: its floating-point work against the vector peak
-        fl = gradient_flops(m, r["num"], r["tuning"])
-        peak = FP64_PEAK_TFLOPS if main_prec == 0 else FP32_PEAK_TFLOPS
+        fl = gradient_flops(m, fp["num"], fp["tuning"])
         tf = fl / (ms_launch * 1e-3) / 1e12
-        out["roofline"]["compute"] = {"flops_per_launch": fl, "achieved": tf, "peak": peak,
-                                      "unit": "TFLOP/s", "frac": tf / peak}
-    if 1 in res and 0 in res:
-        a, b = finals[1], finals[0]
-        errs = [float(np.linalg.norm(a[:, k] - b[:, k]) / np.linalg.norm(b[:, k])) for k in range(5)]
-        mom = np.linalg.norm(b[:, 1:4])
-        errs_mom = [float(np.linalg.norm(a[:, k] - b[:, k]) / mom) for k in (1, 2, 3)]
-        out["mixed"] = {"value": res[1]["value"], "ms_per_step": res[1]["ms_per_step"],
-                        "graph_value": res[1]["graph_value"], "step_hbm": step_hbm(1),
-                        "speedup_vs_fp64": res[1]["value"] / res[0]["value"],
+        out["roofline"]["compute"] = {"flops_per_launch": fl, "achieved": tf,
+                                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                      "frac": tf / FP64_PEAK_TFLOPS}
+    if not args.no_mixed:
+        mx = time_region(args, m, build_case(args, 1), exact=False, steps=args.steps,
+                         err_steps=args.err_steps)
+        out["mixed"] = {"value": mx["value"], "ms_per_step": mx["ms_per_step"],
+                        "graph_value": mx["graph_value"], "step_hbm": step_hbm(m, mx),
+                        "speedup_vs_fp64": mx["value"] / fp["value"],
+                        "dtype": "f32 gradients and interior fluxes, f64 state/residual/limiter",
                         "err_steps": args.err_steps,
-                        "rel_l2_per_variable": errs,
-                        "rel_l2_momentum_vs_momentum_norm": errs_mom}
+                        "rel_l2_per_variable": rel_l2_per_variable(mx["final"], fp["final"]),
+                        "kernel_ms": {k: [v[0], v[1]] for k, v in mx["kernels"].items()},
+                        "tuning": mx["tuning"]}
+    # the 80^3 x 6 configuration: HLLC and AUSM+ in FP64, HLLC mixed
+    if not args.no_c2:
+        try:
+            t0 = time.time()
+            m2 = cases.tet_box(n=args.c2_n, jitter=0.08, seed=2)
+            print(f"# mesh {args.c2_n}^3x6 built in {time.time() - t0:.1f}s", file=sys.stderr,
+                  flush=True)
+            c2 = {"workload": f"{args.c2_n}^3 x 6 Kuhn tets ({m2.n_cells} cells), jitter 8%, "
+                              f"Mach-2 shock + 1% density noise, order 2, WLSQ + {lim_name}, "
+                              f"SSP-RK3, slip walls", "cells": int(m2.n_cells)}
+            finals = {}
+            for label, discr, prec, exact, es in (("hllc_fp64", 3, 0, True, args.c2_err_steps),
+                                                  ("ausm_fp64", 4, 0, True, 0),
+                                                  ("hllc_mixed", 3, 1, False, args.c2_err_steps)):
+                r = time_region(args, m2, build_case(args, prec, discr), exact, args.steps, es)
+                finals[label] = r["final"]
+                c2[label] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
+                             "graph_value": r["graph_value"], "step_hbm": step_hbm(m2, r),
+                             "kernel_ms": {k: [v[0], v[1]] for k, v in r["kernels"].items()},
+                             "tuning": r["tuning"]}
+            c2["hllc_mixed"]["err_steps"] = args.c2_err_steps
+            c2["hllc_mixed"]["rel_l2_per_variable"] = rel_l2_per_variable(
+                finals["hllc_mixed"], finals["hllc_fp64"])
+            out["config_80"] = c2
+        except Exception as e:  # noqa: BLE001
+            out["config_80"] = {"error": repr(e)}
     # secondary workloads: a failure is reported in the line instead of losing it
     if not args.no_batch:
         try:
             out["batch"] = batch_leg(args)
         except Exception as e:  # noqa: BLE001
             out["batch"] = {"error": repr(e)}
-    if not args.no_large:
-        try:
-            out["large"] = large_leg(args)
-        except Exception as e:  # noqa: BLE001
-            out["large"] = {"error": repr(e)}
     if not args.no_adjoint:
         try:
             out["adjoint"] = adjoint_leg(args, order=1, p_in=100.0, dirs=args.adj_dirs)
@@ -619,7 +636,7 @@ def main():
     if not args.no_cpu:
         cb = cpu_baseline(args)
         out["cpu_baseline"] = cb
-        out["gpu_over_cpu"] = r["value"] / cb["value"]
+        out["gpu_over_cpu"] = fp["value"] / cb["value"]
     print(json.dumps(out), flush=True)
 
 

@@ -9,15 +9,19 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_traffic_record_names_a_variant_the_bench_can_match():
+    import glob
     import sys
     sys.path.insert(0, ROOT)
     import bench
-    p = os.path.join(ROOT, "profiles", "traffic_gradient_tet80_prec0.json")
-    d = json.load(open(p))
-    m = re.fullmatch(r"grad_t(\d+)_lds([01])_u(\d)_w([01])", d["variant"])
-    assert m, d["variant"]
-    t = {"grad_tile": int(m.group(1)), "grad_lds": int(m.group(2)),
-         "grad_unroll": int(m.group(3)), "grad_weights": int(m.group(4))}
-    assert bench.grad_variant(t) == d["variant"]
-    assert d["bytes_per_launch"] > 0
-    assert os.path.exists(os.path.join(ROOT, d["source"]))
+    files = sorted(glob.glob(os.path.join(ROOT, "profiles", "traffic_gradient_tet*_prec*.json")))
+    assert any("tet160_prec0" in f for f in files)      # the headline configuration
+    for p in files:
+        d = json.load(open(p))
+        m = re.fullmatch(r"grad_t(\d+)_lds([01])_u(\d)_w([01])(_fast)?", d["variant"])
+        assert m, d["variant"]
+        t = {"grad_tile": int(m.group(1)), "grad_lds": int(m.group(2)),
+             "grad_unroll": int(m.group(3)), "grad_weights": int(m.group(4)),
+             "grad_fast": 1 if m.group(5) else 0}
+        assert bench.grad_variant(t) == d["variant"]
+        assert d["bytes_per_launch"] > 0
+        assert os.path.exists(os.path.join(ROOT, d["source"]))
