@@ -48,6 +48,7 @@ extern "C" {
 /* Gradient modifiers (RECONST, modfloflu/ModParameters.F90:678-683). */
 #define RFLU_RECONST_NONE 0
 #define RFLU_RECONST_WENO_XYZ 2
+#define RFLU_RECONST_LIM_BARTHJESP 10
 #define RFLU_RECONST_LIM_VENKAT 11
 /* Time integrators (RKSCHEME, modfloflu/ModParameters.F90:615-618); 4 is Shu-Osher
  * SSP-RK3, which the reference does not have. */

@@ -88,6 +88,8 @@ def lib():
         L.orc_rind_state.argtypes = [R, ctypes.c_int, _f64p, _f64p, ctypes.c_double, _f64p]
         L.orc_patch_flux.argtypes = [R, ctypes.c_int, _f64p, _f64p, ctypes.c_double, _f64p]
         L.orc_interior_flux.argtypes = [R, _f64p, _f64p, _f64p, ctypes.c_int, _f64p]
+        L.orc_barth_jesp.argtypes = [R, _f64p, _f64p]
+        L.orc_sym_eigen3.argtypes = [_f64p, _f64p, _f64p]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _L = L
@@ -227,6 +229,14 @@ class Oracle:
         self.L.orc_venkat(ctypes.byref(self.r), _p(var, ctypes.c_double), _p(g, ctypes.c_double))
         return g
 
+    def barth_jesp(self, var, grad):
+        """RECONST 10: Barth-Jespersen for rho and p, the velocity gradient limited along
+        the principal axes of its symmetric part; grad (N,5,3) -> limited copy."""
+        var = np.ascontiguousarray(var, dtype=np.float64)
+        g = np.array(grad, dtype=np.float64, copy=True, order="C")
+        self.L.orc_barth_jesp(ctypes.byref(self.r), _p(var, ctypes.c_double), _p(g, ctypes.c_double))
+        return g
+
     def weno_xyz(self, grad):
         g = np.array(grad, dtype=np.float64, copy=True, order="C")
         self.L.orc_weno_xyz(ctypes.byref(self.r), _p(g, ctypes.c_double))
@@ -263,3 +273,13 @@ class Oracle:
         rc = self.L.orc_after_update(ctypes.byref(self.r), _p(cv, ctypes.c_double),
                                      _p(dv, ctypes.c_double), ctypes.byref(st))
         return rc, cv, dv, {name: getattr(st, name) for name, _ in OrcStats._fields_}
+
+
+def sym_eigen3(m):
+    """The limiter's Jacobi eigen solver on a symmetric 3x3 matrix: (values ascending,
+    vectors as columns)."""
+    a = np.asfortranarray(np.asarray(m, dtype=np.float64)).reshape(-1, order="F").copy()
+    vec = np.empty(9)
+    val = np.empty(3)
+    lib().orc_sym_eigen3(_p(a, ctypes.c_double), _p(vec, ctypes.c_double), _p(val, ctypes.c_double))
+    return val, vec.reshape(3, 3, order="F")

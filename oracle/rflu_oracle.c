@@ -1191,6 +1191,202 @@ void orc_venkat(const orc_region* R, const double* var, double* grad) {
   free(vmin); free(vmax); free(lim);
 }
 
+/* RFLU_SymmetricEigen3, modflu/RFLU_ModLimiters.F90:791-886: cyclic Jacobi rotations
+ * of a symmetric 3x3 matrix m (column-major, m[i + 3 j]) until the off-diagonal squares
+ * fall to 1e-24 of the total (at most 50 sweeps); values ascending, vectors as columns
+ * (vec[i + 3 j] is component i of vector j). */
+void orc_sym_eigen3(const double* m, double* vec, double* val) {
+  double r[9];
+  for (int k = 0; k < 9; ++k) r[k] = m[k];
+  for (int k = 0; k < 9; ++k) vec[k] = 0.0;
+  vec[0] = vec[4] = vec[8] = 1.0;
+  double total = 0.0;
+  for (int k = 0; k < 9; ++k) total = total + r[k] * r[k];
+  for (int sweep = 1; sweep <= 50; ++sweep) {
+    double off = r[3] * r[3] + r[6] * r[6] + r[7] * r[7];
+    if (off <= 1.0E-24 * total) break;
+    for (int ip = 0; ip < 2; ++ip) {
+      for (int iq = ip + 1; iq < 3; ++iq) {
+        double apq = r[ip + 3 * iq];
+        if (apq == 0.0) continue;
+        double theta = (r[iq + 3 * iq] - r[ip + 3 * ip]) / (2.0 * apq);
+        double t;
+        if (fabs(theta) > 1.0E+150)
+          t = 0.5 / theta;
+        else
+          t = copysign(1.0, theta) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        double c = 1.0 / sqrt(t * t + 1.0);
+        double sn = t * c;
+        int ir = 3 - ip - iq;
+        double arp = r[ir + 3 * ip], arq = r[ir + 3 * iq];
+        r[ip + 3 * ip] = r[ip + 3 * ip] - t * r[ip + 3 * iq];
+        r[iq + 3 * iq] = r[iq + 3 * iq] + t * r[ip + 3 * iq];
+        r[ip + 3 * iq] = 0.0;
+        r[iq + 3 * ip] = 0.0;
+        r[ir + 3 * ip] = c * arp - sn * arq;
+        r[ip + 3 * ir] = r[ir + 3 * ip];
+        r[ir + 3 * iq] = sn * arp + c * arq;
+        r[iq + 3 * ir] = r[ir + 3 * iq];
+        for (int i = 0; i < 3; ++i) {
+          double vp = vec[i + 3 * ip], vq = vec[i + 3 * iq];
+          vec[i + 3 * ip] = c * vp - sn * vq;
+          vec[i + 3 * iq] = sn * vp + c * vq;
+        }
+      }
+    }
+  }
+  for (int i = 0; i < 3; ++i) val[i] = r[i + 3 * i];
+  for (int i = 0; i < 2; ++i) {
+    int imin = i;
+    for (int k = i + 1; k < 3; ++k)
+      if (val[k] < val[imin]) imin = k;  /* MINLOC: the first of equal minima */
+    if (imin != i) {
+      double h = val[i];
+      val[i] = val[imin];
+      val[imin] = h;
+      for (int k = 0; k < 3; ++k) {
+        double cc = vec[k + 3 * i];
+        vec[k + 3 * i] = vec[k + 3 * imin];
+        vec[k + 3 * imin] = cc;
+      }
+    }
+  }
+}
+
+/* RECONST 10: RFLU_ComputeLimiterBarthJesp (modflu/RFLU_ModLimiters.F90:132-322) for rho
+ * and p (the velocity skipped, iVelVar given: libfloflu/CellGradientsMP.F90:229-238),
+ * then RFLU_LimitGradVelocityBarthJesp (:588-764), which limits the velocity gradient
+ * along the principal axes of its symmetric part, and RFLU_LimitGradCells (:1100-1106).
+ * var (5,N) holds (rho,u,v,w,p), grad (3,5,N).  dx = cofg - fc, the reference's sign;
+ * Fortran MATMUL and DOT_PRODUCT sums run in index order. */
+void orc_barth_jesp(const orc_region* R, const double* var, double* grad) {
+  const size_t N = (size_t)R->n_cells;
+  double* vmin = (double*)malloc(5 * N * sizeof(double));
+  double* vmax = (double*)malloc(5 * N * sizeof(double));
+  double* lim = (double*)malloc(5 * N * sizeof(double));
+  double* axes = (double*)malloc(9 * N * sizeof(double));
+  double* eig = (double*)malloc(3 * N * sizeof(double));
+  double* alpha = (double*)malloc(3 * N * sizeof(double));
+  double* dmn = (double*)malloc(3 * N * sizeof(double));
+  double* dmx = (double*)malloc(3 * N * sizeof(double));
+  static const int lv[2] = {0, 4};
+  for (size_t i = 0; i < 5 * N; ++i) { vmin[i] = var[i]; vmax[i] = var[i]; lim[i] = 1.0; }
+  for (int f = 0; f < R->n_faces; ++f) {
+    int c1 = R->f2c[2 * (size_t)f], c2 = R->f2c[2 * (size_t)f + 1];
+    for (int a = 0; a < 2; ++a) {
+      int iv = lv[a];
+      double v1 = var[5 * (size_t)c1 + iv], v2 = var[5 * (size_t)c2 + iv];
+      vmin[5 * (size_t)c1 + iv] = dmin(vmin[5 * (size_t)c1 + iv], v2);
+      vmin[5 * (size_t)c2 + iv] = dmin(vmin[5 * (size_t)c2 + iv], v1);
+      vmax[5 * (size_t)c1 + iv] = dmax(vmax[5 * (size_t)c1 + iv], v2);
+      vmax[5 * (size_t)c2 + iv] = dmax(vmax[5 * (size_t)c2 + iv], v1);
+    }
+  }
+  for (int f = 0; f < R->n_faces; ++f) {
+    const double* xf = R->fc + 3 * (size_t)f;
+    for (int side = 0; side < 2; ++side) {
+      int ci = R->f2c[2 * (size_t)f + side];
+      double dx = R->cofg[3 * (size_t)ci] - xf[0];
+      double dy = R->cofg[3 * (size_t)ci + 1] - xf[1];
+      double dz = R->cofg[3 * (size_t)ci + 2] - xf[2];
+      for (int a = 0; a < 2; ++a) {
+        int iv = lv[a];
+        const double* g = grad + 15 * (size_t)ci + 3 * iv;
+        double d2 = g[0] * dx + g[1] * dy + g[2] * dz;
+        double vi = var[5 * (size_t)ci + iv];
+        double* l = lim + 5 * (size_t)ci + iv;
+        if (d2 > 0.0) {
+          double term = dmin(1.0, (vmax[5 * (size_t)ci + iv] - vi) / d2);
+          *l = dmin(term, *l);
+        } else if (d2 < 0.0) {
+          double term = dmin(1.0, (vmin[5 * (size_t)ci + iv] - vi) / d2);
+          *l = dmin(term, *l);
+        }
+      }
+    }
+  }
+  /* velocity: principal axes of the symmetric part of the velocity gradient G, with
+   * G(i,k) = d u_i / d x_k = grad(k, 1+i, c) */
+  for (size_t c = 0; c < N; ++c) {
+    const double* G = grad + 15 * c + 3;
+    double S[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) S[i + 3 * j] = 0.5 * (G[3 * i + j] + G[3 * j + i]);
+    orc_sym_eigen3(S, axes + 9 * c, eig + 3 * c);
+    for (int j = 0; j < 3; ++j) { alpha[3 * c + j] = 1.0; dmn[3 * c + j] = 0.0; dmx[3 * c + j] = 0.0; }
+  }
+  for (int f = 0; f < R->n_faces; ++f) {
+    int cc[2] = {R->f2c[2 * (size_t)f], R->f2c[2 * (size_t)f + 1]};
+    double du[3];
+    for (int i = 0; i < 3; ++i) du[i] = var[5 * (size_t)cc[1] + 1 + i] - var[5 * (size_t)cc[0] + 1 + i];
+    for (int side = 0; side < 2; ++side) {
+      size_t c = (size_t)cc[side];
+      const double* A = axes + 9 * c;
+      for (int j = 0; j < 3; ++j) {
+        double proj = A[0 + 3 * j] * du[0] + A[1 + 3 * j] * du[1] + A[2 + 3 * j] * du[2];
+        if (side == 1) proj = -proj;
+        dmn[3 * c + j] = dmin(dmn[3 * c + j], proj);
+        dmx[3 * c + j] = dmax(dmx[3 * c + j], proj);
+      }
+    }
+  }
+  for (int f = 0; f < R->n_faces; ++f) {
+    const double* xf = R->fc + 3 * (size_t)f;
+    for (int side = 0; side < 2; ++side) {
+      size_t c = (size_t)R->f2c[2 * (size_t)f + side];
+      double off[3] = {R->cofg[3 * c] - xf[0], R->cofg[3 * c + 1] - xf[1], R->cofg[3 * c + 2] - xf[2]};
+      const double* G = grad + 15 * c + 3;
+      double sh[3];
+      for (int i = 0; i < 3; ++i) sh[i] = G[3 * i] * off[0] + G[3 * i + 1] * off[1] + G[3 * i + 2] * off[2];
+      const double* A = axes + 9 * c;
+      for (int j = 0; j < 3; ++j) {
+        double proj = A[0 + 3 * j] * sh[0] + A[1 + 3 * j] * sh[1] + A[2 + 3 * j] * sh[2];
+        if (proj > 0.0)
+          alpha[3 * c + j] = dmin(alpha[3 * c + j], dmx[3 * c + j] / proj);
+        else if (proj < 0.0)
+          alpha[3 * c + j] = dmin(alpha[3 * c + j], dmn[3 * c + j] / proj);
+      }
+    }
+  }
+  for (size_t c = 0; c < N; ++c) {
+    double* al = alpha + 3 * c;
+    const double* ev = eig + 3 * c;
+    double mx = fabs(ev[0]);
+    if (fabs(ev[1]) > mx) mx = fabs(ev[1]);
+    if (fabs(ev[2]) > mx) mx = fabs(ev[2]);
+    double scale = 1.0E-3 * mx;
+    int e12 = ev[1] - ev[0] <= scale, e23 = ev[2] - ev[1] <= scale;
+    if (e12 && e23) {
+      double m = dmin(dmin(al[0], al[1]), al[2]);
+      al[0] = al[1] = al[2] = m;
+    } else if (e12) {
+      double m = dmin(al[0], al[1]);
+      al[0] = al[1] = m;
+    } else if (e23) {
+      double m = dmin(al[1], al[2]);
+      al[1] = al[2] = m;
+    }
+    /* rotated = A^T G, rows scaled by alpha, G = A rotated */
+    double* G = grad + 15 * c + 3;
+    const double* A = axes + 9 * c;
+    double rot[9];
+    for (int j = 0; j < 3; ++j)
+      for (int l = 0; l < 3; ++l)
+        rot[j + 3 * l] = A[0 + 3 * j] * G[3 * 0 + l] + A[1 + 3 * j] * G[3 * 1 + l] + A[2 + 3 * j] * G[3 * 2 + l];
+    for (int j = 0; j < 3; ++j)
+      for (int l = 0; l < 3; ++l) rot[j + 3 * l] = al[j] * rot[j + 3 * l];
+    for (int i = 0; i < 3; ++i)
+      for (int l = 0; l < 3; ++l)
+        G[3 * i + l] = A[i + 3 * 0] * rot[0 + 3 * l] + A[i + 3 * 1] * rot[1 + 3 * l] + A[i + 3 * 2] * rot[2 + 3 * l];
+  }
+  /* RFLU_LimitGradCells: rho and p by their limiters (the velocity's are 1) */
+  for (size_t i = 0; i < N; ++i)
+    for (int a = 0; a < 2; ++a)
+      for (int d = 0; d < 3; ++d)
+        grad[15 * i + 3 * lv[a] + d] = lim[5 * i + lv[a]] * grad[15 * i + 3 * lv[a] + d];
+  free(vmin); free(vmax); free(lim); free(axes); free(eig); free(alpha); free(dmn); free(dmx);
+}
+
 /* CellGradientsMP, libfloflu/CellGradientsMP.F90:160-296: cons -> (rho,u,v,w,p) in
  * place, gradients, reconstruction modifier, and back to conservative form.  The
  * back-conversion of the default build leaves p in slot 5
@@ -1212,6 +1408,8 @@ void orc_cell_gradients(const orc_region* R, double* cv, const double* dv, doubl
     orc_limit_simple(R, cv, grad);
   } else if (R->reconst == 11) {
     orc_venkat(R, cv, grad);
+  } else if (R->reconst == 10) {
+    orc_barth_jesp(R, cv, grad);
   }
   ORC_PAR
   for (size_t i = 0; i < N; ++i) {

@@ -43,7 +43,8 @@ typedef struct {
   double gamma, cp;          /* global%refGamma, global%refCp */
   int32_t discr;             /* 1 Roe, 3 HLLC, 4 AUSM+ */
   int32_t order;             /* 1 or 2 */
-  int32_t reconst;           /* 0 none, 2 WENO-XYZ + clip, 11 Venkatakrishnan */
+  int32_t reconst;           /* 0 none, 2 WENO-XYZ + clip, 10 Barth-Jespersen (velocity in
+                                the frame of its symmetric gradient), 11 Venkatakrishnan */
   int32_t rk;                /* 1 RK4, 2 Wray RK3, 3 Euler, 4 SSP-RK3 */
   double cfl, dt_imposed, dt_min_limit, eps_entr, diss_fact;
   const double* patch_vals;  /* (5,P) BCDAT 1..5 of the patch: inflow 11 u, v, w, T, p
@@ -86,6 +87,8 @@ void orc_venkat(const orc_region* r, const double* var, double* grad);
 /* Boundary state and flux of one face of a patch (test entry points). */
 void orc_rind_state(const orc_region* r, int patch, const double* n, const double* q, double pl,
                     double* out);
+void orc_sym_eigen3(const double* m, double* vec, double* val);
+void orc_barth_jesp(const orc_region* r, const double* var, double* grad);
 void orc_interior_flux(const orc_region* r, const double* cv, const double* dv,
                        const double* grad, int f, double* flx);
 void orc_patch_flux(const orc_region* r, int patch, const double* n, const double* q, double pl,

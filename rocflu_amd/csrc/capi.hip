@@ -734,10 +734,20 @@ int launch_gradients(Region* R) {
     if (grad32(R)) hipLaunchKernelGGL((k_grad_constr<float>), g, b, 0, R->stream, M, S, C, (float*)R->grad, R->gate_sc());
     else hipLaunchKernelGGL((k_grad_constr<double>), g, b, 0, R->stream, M, S, C, (double*)R->grad, R->gate_sc());
     if (R->reconst == RFLU_RECONST_LIM_VENKAT) {
+      if (!R->vnb || !R->vgeo) return fail(RFLU_ERR_ARG, "limiter neighbour lists missing");
       dim3 gc(nblk(R->N, R->grad_block)), bc(R->grad_block);
       if (grad32(R)) hipLaunchKernelGGL((k_venkat_apply<float>), gc, bc, 0, R->stream, M, S, (float*)R->grad, R->gate_sc());
       else hipLaunchKernelGGL((k_venkat_apply<double>), gc, bc, 0, R->stream, M, S, (double*)R->grad, R->gate_sc());
     }
+  }
+  if (R->reconst == RFLU_RECONST_LIM_BARTHJESP) {
+    if (!R->vnb || !R->vgeo) return fail(RFLU_ERR_ARG, "limiter neighbour lists missing");
+    Timed t(R, KID_BLEND);
+    Mesh M = R->mesh();
+    State S = R->state();
+    dim3 gc(nblk(R->N, R->grad_block)), bc(R->grad_block);
+    if (grad32(R)) hipLaunchKernelGGL((k_barth_jesp<float>), gc, bc, 0, R->stream, M, S, (float*)R->grad, R->gate_sc());
+    else hipLaunchKernelGGL((k_barth_jesp<double>), gc, bc, 0, R->stream, M, S, (double*)R->grad, R->gate_sc());
   }
   if (R->reconst == RFLU_RECONST_WENO_XYZ) {
     Timed t(R, KID_BLEND);
@@ -1078,8 +1088,8 @@ int check_desc(const rflu_region_desc* d, const rflu_region_desc* ref) {
   if (d->order == 2) {
     if (!d->fc || !d->c2cs_ptr || !d->c2cs_idx || !d->xyz_moms)
       return fail(RFLU_ERR_ARG, "order 2 needs fc, stencils and xyz_moms");
-    if (d->reconst != 0 && d->reconst != 2 && d->reconst != 11)
-      return fail(RFLU_ERR_ARG, "reconst must be 0, 2 or 11");
+    if (d->reconst != 0 && d->reconst != 2 && d->reconst != 10 && d->reconst != 11)
+      return fail(RFLU_ERR_ARG, "reconst must be 0, 2, 10 or 11");
   }
   if (!(d->gamma > 1.0) || !(d->cp > 0.0)) return fail(RFLU_ERR_ARG, "bad gas constants");
   const int B = P > 0 ? d->bpatch_ptr[P] : 0;
@@ -1314,7 +1324,10 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
   std::vector<int> hvnb;
   std::vector<double> hvgeo;
   const bool venkat = R->order == 2 && R->reconst == RFLU_RECONST_LIM_VENKAT;
-  if (venkat) build_limiter_geometry(R, hc2f_ptr, hc2f_idx, hf2c, hfc, hcell, hvnb, hvgeo);
+  // face neighbours and cofg - fc of the limiters (Barth-Jespersen reads them in a pass
+  // of its own, outside the gradient tiles)
+  if (venkat || (R->order == 2 && R->reconst == RFLU_RECONST_LIM_BARTHJESP))
+    build_limiter_geometry(R, hc2f_ptr, hc2f_idx, hf2c, hfc, hcell, hvnb, hvgeo);
   if (R->order == 2)
     build_tiles(R, hsten_ptr, hsten_idx, hvnb, venkat, htile_uptr, htile_uidx, htile_lptr,
                 htile_loc, htile_vloc);
@@ -1387,7 +1400,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       free_region(R);
       return rc;
     }
-    if (venkat) {
+    if (!hvnb.empty()) {   // the limiters' neighbour lists (Venkatakrishnan, Barth-Jespersen)
       if ((rc = dalloc(&R->vnb, hvnb.size())) || (rc = dalloc(&R->vgeo, hvgeo.size())) ||
           (rc = dput(R->vnb, hvnb, R->stream)) || (rc = dput(R->vgeo, hvgeo, R->stream))) {
         free_region(R);

@@ -2098,6 +2098,200 @@ __global__ void k_venkat_apply(Mesh M, State S, GT* __restrict__ grad,
   for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
 }
 
+// Eigen-decomposition of a symmetric 3x3 matrix m (column-major) by cyclic Jacobi
+// rotations, as RFLU_SymmetricEigen3 (modflu/RFLU_ModLimiters.F90:791-886): sweeps
+// until the off-diagonal squares fall to 1e-24 of the total (at most 50), then values
+// ascending with their vectors (columns of vec, vec[i + 3 j]).
+__device__ __forceinline__ void sym_eigen3(const double* m, double* vec, double* val) {
+  double r[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) r[k] = m[k];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) vec[k] = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+  double total = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) total = total + r[k] * r[k];
+  for (int sweep = 0; sweep < 50; ++sweep) {
+    const double off = r[3] * r[3] + r[6] * r[6] + r[7] * r[7];
+    if (off <= 1.0E-24 * total) break;
+#pragma unroll
+    for (int pq = 0; pq < 3; ++pq) {
+      const int ip = pq < 2 ? 0 : 1, iq = pq == 0 ? 1 : 2, ir = 3 - ip - iq;
+      const double apq = r[ip + 3 * iq];
+      if (apq == 0.0) continue;
+      const double theta = (r[iq + 3 * iq] - r[ip + 3 * ip]) / (2.0 * apq);
+      const double t = fabs(theta) > 1.0E+150
+                           ? 0.5 / theta
+                           : copysign(1.0, theta) / (fabs(theta) + sqrt(theta * theta + 1.0));
+      const double c = 1.0 / sqrt(t * t + 1.0);
+      const double sn = t * c;
+      const double arp = r[ir + 3 * ip], arq = r[ir + 3 * iq];
+      r[ip + 3 * ip] = r[ip + 3 * ip] - t * r[ip + 3 * iq];
+      r[iq + 3 * iq] = r[iq + 3 * iq] + t * r[ip + 3 * iq];
+      r[ip + 3 * iq] = 0.0;
+      r[iq + 3 * ip] = 0.0;
+      r[ir + 3 * ip] = c * arp - sn * arq;
+      r[ip + 3 * ir] = r[ir + 3 * ip];
+      r[ir + 3 * iq] = sn * arp + c * arq;
+      r[iq + 3 * ir] = r[ir + 3 * iq];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double vp = vec[i + 3 * ip], vq = vec[i + 3 * iq];
+        vec[i + 3 * ip] = c * vp - sn * vq;
+        vec[i + 3 * iq] = sn * vp + c * vq;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) val[i] = r[i + 3 * i];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    int imin = i;
+    for (int k = i + 1; k < 3; ++k)
+      if (val[k] < val[imin]) imin = k;  // the first of equal minima
+    if (imin != i) {
+      const double h = val[i];
+      val[i] = val[imin];
+      val[imin] = h;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double cc = vec[k + 3 * i];
+        vec[k + 3 * i] = vec[k + 3 * imin];
+        vec[k + 3 * imin] = cc;
+      }
+    }
+  }
+}
+
+// RECONST 10 for one cell, after the gradient pass (and the constrained rows):
+// RFLU_ComputeLimiterBarthJesp (modflu/RFLU_ModLimiters.F90:132-322) for rho and p,
+// RFLU_LimitGradVelocityBarthJesp (:588-764) for the velocity gradient G (limited along
+// the principal axes of its symmetric part, against the neighbours' velocity
+// differences along those axes), and RFLU_LimitGradCells (:1100-1106).  Face
+// neighbours and cofg - fc come from the limiter lists (vnb, vgeo); min/max reductions
+// do not depend on the face order, and the face neighbour's velocity difference
+// u_o - u_i has the bits of the reference's side-2 projection negated.
+template <typename GT>
+__global__ void k_barth_jesp(Mesh M, State S, GT* __restrict__ grad,
+                             const Scalars* __restrict__ sc) {
+  if (gate(sc)) return;
+  const int N = M.N;
+  const int i = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double vi[5], g[15];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) vi[k] = S.prim[(size_t)k * N + i];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) g[k] = double(grad[(size_t)k * N + i]);
+  // rho and p: neighbour extrema, then min over faces of min(1, Delta/Delta_2)
+  double mn0 = vi[0], mx0 = vi[0], mn4 = vi[4], mx4 = vi[4];
+  int nq = 0;
+  for (int q = 0; q < M.vq; ++q) {
+    const int o = M.vnb[(size_t)q * N + i];
+    if (o < 0) break;
+    ++nq;
+    const double r = S.prim[o], p = S.prim[4 * (size_t)N + o];
+    mn0 = mn0 < r ? mn0 : r;
+    mx0 = mx0 > r ? mx0 : r;
+    mn4 = mn4 < p ? mn4 : p;
+    mx4 = mx4 > p ? mx4 : p;
+  }
+  double l0 = 1.0, l4 = 1.0;
+  // velocity: principal axes of the symmetric part of G (G(i,k) = g[3 (1+i) + k])
+  double Sm[9], A[9], ev[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) Sm[a + 3 * b] = 0.5 * (g[3 * (1 + a) + b] + g[3 * (1 + b) + a]);
+  sym_eigen3(Sm, A, ev);
+  double dmn[3] = {0.0, 0.0, 0.0}, dmx[3] = {0.0, 0.0, 0.0}, al[3] = {1.0, 1.0, 1.0};
+  for (int q = 0; q < nq; ++q) {
+    const int o = M.vnb[(size_t)q * N + i];
+    const double dx = M.vgeo[(size_t)(3 * q) * N + i];
+    const double dy = M.vgeo[(size_t)(3 * q + 1) * N + i];
+    const double dz = M.vgeo[(size_t)(3 * q + 2) * N + i];
+    double d2 = g[0] * dx + g[1] * dy + g[2] * dz;
+    if (d2 > 0.0) {
+      const double t = 1.0 < (mx0 - vi[0]) / d2 ? 1.0 : (mx0 - vi[0]) / d2;
+      l0 = t < l0 ? t : l0;
+    } else if (d2 < 0.0) {
+      const double t = 1.0 < (mn0 - vi[0]) / d2 ? 1.0 : (mn0 - vi[0]) / d2;
+      l0 = t < l0 ? t : l0;
+    }
+    d2 = g[12] * dx + g[13] * dy + g[14] * dz;
+    if (d2 > 0.0) {
+      const double t = 1.0 < (mx4 - vi[4]) / d2 ? 1.0 : (mx4 - vi[4]) / d2;
+      l4 = t < l4 ? t : l4;
+    } else if (d2 < 0.0) {
+      const double t = 1.0 < (mn4 - vi[4]) / d2 ? 1.0 : (mn4 - vi[4]) / d2;
+      l4 = t < l4 ? t : l4;
+    }
+    const double du[3] = {S.prim[(size_t)N + o] - vi[1], S.prim[2 * (size_t)N + o] - vi[2],
+                          S.prim[3 * (size_t)N + o] - vi[3]};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double pr = A[0 + 3 * j] * du[0] + A[1 + 3 * j] * du[1] + A[2 + 3 * j] * du[2];
+      dmn[j] = dmn[j] < pr ? dmn[j] : pr;
+      dmx[j] = dmx[j] > pr ? dmx[j] : pr;
+    }
+  }
+  for (int q = 0; q < nq; ++q) {
+    const double off[3] = {M.vgeo[(size_t)(3 * q) * N + i], M.vgeo[(size_t)(3 * q + 1) * N + i],
+                           M.vgeo[(size_t)(3 * q + 2) * N + i]};
+    double sh[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      sh[a] = g[3 * (1 + a)] * off[0] + g[3 * (1 + a) + 1] * off[1] + g[3 * (1 + a) + 2] * off[2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double pr = A[0 + 3 * j] * sh[0] + A[1 + 3 * j] * sh[1] + A[2 + 3 * j] * sh[2];
+      if (pr > 0.0) {
+        const double t = dmx[j] / pr;
+        al[j] = al[j] < t ? al[j] : t;
+      } else if (pr < 0.0) {
+        const double t = dmn[j] / pr;
+        al[j] = al[j] < t ? al[j] : t;
+      }
+    }
+  }
+  double emax = fabs(ev[0]);
+  emax = fabs(ev[1]) > emax ? fabs(ev[1]) : emax;
+  emax = fabs(ev[2]) > emax ? fabs(ev[2]) : emax;
+  const double escale = 1.0E-3 * emax;
+  const bool e12 = ev[1] - ev[0] <= escale, e23 = ev[2] - ev[1] <= escale;
+  if (e12 && e23) {
+    double m = al[0] < al[1] ? al[0] : al[1];
+    m = m < al[2] ? m : al[2];
+    al[0] = al[1] = al[2] = m;
+  } else if (e12) {
+    const double m = al[0] < al[1] ? al[0] : al[1];
+    al[0] = al[1] = m;
+  } else if (e23) {
+    const double m = al[1] < al[2] ? al[1] : al[2];
+    al[1] = al[2] = m;
+  }
+  double rot[9];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+      rot[j + 3 * l] = al[j] * (A[0 + 3 * j] * g[3 + l] + A[1 + 3 * j] * g[6 + l] +
+                                A[2 + 3 * j] * g[9 + l]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+      g[3 * (1 + a) + l] = A[a + 3 * 0] * rot[0 + 3 * l] + A[a + 3 * 1] * rot[1 + 3 * l] +
+                           A[a + 3 * 2] * rot[2 + 3 * l];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    g[d] = l0 * g[d];
+    g[12 + d] = l4 * g[12 + d];
+  }
+#pragma unroll
+  for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = GT(g[k]);
+}
+
 // WENO-XYZ blend (modflu/RFLU_ModWENO.F90:1437-1508) followed by the positivity clip
 // of rho and p (modflu/RFLU_ModLimiters.F90:1199-1267).
 template <typename GT>

@@ -21,7 +21,7 @@ from . import _native as nat
 from .mesh import Mesh
 
 DISCR_ROE, DISCR_HLLC, DISCR_AUSMPLUS = 1, 3, 4
-RECONST_NONE, RECONST_WENO_XYZ, RECONST_LIM_VENKAT = 0, 2, 11
+RECONST_NONE, RECONST_WENO_XYZ, RECONST_LIM_BARTHJESP, RECONST_LIM_VENKAT = 0, 2, 10, 11
 RK4_CLASSICAL, RK3_WRAY, RK1_EULER, RK3_SSP = 1, 2, 3, 4
 PREC_FP64, PREC_MIXED, PREC_FP32_GRAD, PREC_FP32_FLUX = 0, 1, 2, 3
 

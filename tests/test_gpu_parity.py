@@ -109,7 +109,8 @@ def small_tet(n=8, bc=rmesh.BC_SLIPWALL, seed=5):
     return cases.tet_box(n=n, jitter=0.08, seed=seed, bc=bc)
 
 
-@pytest.mark.parametrize("discr,reconst", [(3, 11), (4, 11), (1, 11), (3, 2), (3, 0)])
+@pytest.mark.parametrize("discr,reconst", [(3, 11), (4, 11), (1, 11), (3, 2), (3, 0), (3, 10),
+                                           (1, 10)])
 def test_second_order_tets(gpu, discr, reconst):
     """Config 2's scheme on a reduced 8^3 x 6 Kuhn mesh (3,072 tets), Mach-2 shock."""
     m = small_tet()
@@ -405,7 +406,7 @@ def test_mass_bookkeeping_stops_at_the_end_time(gpu, batch):
 
 
 @pytest.mark.parametrize("kind,reconst,discr", [(rmesh.HEX, 2, 3), (rmesh.TET6, 11, 3),
-                                                (rmesh.TET6, 0, 4)])
+                                                (rmesh.TET6, 0, 4), (rmesh.TET6, 10, 3)])
 def test_constrained_gradients(gpu, kind, reconst, discr):
     """CRECONSTC 1 (wall-constrained velocity gradients): the GPU precomputes the rows
     with a Jacobi SVD and the reference's truncation, the oracle re-solves each cell's

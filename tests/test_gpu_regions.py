@@ -41,7 +41,8 @@ def partition_map(m, kind, k=3):
     (1, 1, 0, 0, "slabs", True), (1, 3, 0, 0, "blocks", True),
     (2, 3, 11, 0, "slabs", True), (2, 3, 2, 0, "blocks", True), (2, 4, 0, 0, "slabs", True),
     (2, 1, 11, 0, "blocks", True), (2, 3, 11, 1, "slabs", True), (2, 3, 2, 1, "blocks", True),
-    (2, 3, 11, 0, "blocks", False), (2, 4, 0, 1, "slabs", False)])
+    (2, 3, 11, 0, "blocks", False), (2, 4, 0, 1, "slabs", False), (2, 3, 10, 0, "blocks", False),
+    (2, 1, 10, 1, "slabs", True)])
 def test_regions_match_the_single_region(gpu, order, discr, reconst, prec, kind, blend):
     """blend=False: one virtual layer without stencils (enough without WENO-XYZ)."""
     m, num, cv0 = setup(order, discr, reconst, prec)
