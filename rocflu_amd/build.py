@@ -2,9 +2,11 @@
 
 - ``librocflu_amd.so``: HIP kernels + C-ABI (hipcc, gfx950, FMA contraction off so
   FP64 results round like the reference's gfortran build).
-- ``librflu_mesh.so``: host region builder (g++).
+- ``librflu_mesh.so``: host region builder, grid and input file readers (g++).
 - ``fortran/librflu_gpu_f.so`` + ``fortran/rflu_driver``: the ISO_C_BINDING module and
   a Fortran driver that calls the C-ABI in RungeKuttaMP's order (amdflang, when present);
+  ``fortran/rflu_case_driver``: the same loop for a case read from the reference's input
+  files;
   ``fortran/rflu_regions_driver``: the same loop over regions coupled through virtual cells.
 """
 from __future__ import annotations
@@ -49,11 +51,11 @@ def build_hip():
 
 
 def build_mesh():
-    src = os.path.join(HERE, "csrc", "mesh.cpp")
+    srcs = [os.path.join(HERE, "csrc", f) for f in ("mesh.cpp", "grid_io.cpp", "inputs.cpp")]
     out = os.path.join(HERE, "librflu_mesh.so")
     tmp = out + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall", "-Wextra",
-          "-I" + INC, src, "-o", tmp])
+          "-I" + INC] + [s for s in srcs if os.path.exists(s)] + ["-o", tmp])
     os.replace(tmp, out)
     return out
 
@@ -71,6 +73,11 @@ def build_fortran():
     out = os.path.join(fdir, "rflu_driver")
     _run([fc, "-O2", "-J", fdir, mod, drv, "-L" + HERE, "-lrocflu_amd", "-lrflu_mesh",
           "-Wl,-rpath,$ORIGIN/..", "-o", out], cwd=fdir)
+    # the host that runs a case from the reference's .inp / .bc / .hyb.asc files
+    drv3 = os.path.join(fdir, "rflu_case_driver.f90")
+    if os.path.exists(drv3):
+        _run([fc, "-O2", "-J", fdir, mod, drv3, "-L" + HERE, "-lrocflu_amd", "-lrflu_mesh",
+              "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(fdir, "rflu_case_driver")], cwd=fdir)
     # the host of several regions coupled through virtual cells
     drv2 = os.path.join(fdir, "rflu_regions_driver.f90")
     if os.path.exists(drv2):

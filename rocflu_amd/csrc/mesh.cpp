@@ -1,9 +1,11 @@
-// Host-side region builder: box meshes of hexahedra or Kuhn tetrahedra, face list,
-// geometry, cell-to-cell stencils and their R-factors.  See include/rflu_mesh.h.
+// Host-side region builder: face list, geometry, cell-to-cell stencils and their
+// R-factors of a grid (read from a grid file or generated as a jittered box of
+// hexahedra or Kuhn tetrahedra).  See include/rflu_mesh.h.
 //
 // Everything here runs once per region on the host, as the reference's
 // RFLU_InitFlowSolver does (rocflu/RFLU_InitFlowSolver.F90:495-946).
 #include "rflu_mesh.h"
+#include "mesh_internal.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -31,26 +33,7 @@ struct SplitMix64 {
 
 }  // namespace
 
-struct rflu_mesh {
-  int kind = 0, nx = 0, ny = 0, nz = 0;
-  int nvpc = 8;  // vertices per cell
-  int nfpc = 6;  // faces per cell
-  int n_verts = 0, n_cells = 0, n_faces = 0, n_bfaces = 0, n_patches = 6;
-  std::vector<double> xyz;
-  std::vector<int32_t> c2v;
-  std::vector<int32_t> f2c, f2v;
-  std::vector<double> fn, fc;
-  std::vector<int32_t> bpatch_ptr, bf2c, bf2v;
-  std::vector<double> bfn, bfc;
-  std::vector<int32_t> patch_bc, patch_opt;
-  std::vector<double> patch_pout;
-  std::vector<double> vol, cofg;
-  std::vector<int32_t> c2cs_ptr, c2cs_idx;
-  std::vector<int32_t> c2cs_bptr, c2cs_bface;   // wall faces of each stencil: (patch, face)
-  std::vector<double> xyz_moms;
-  std::vector<int32_t> c2f_ptr, c2f_idx;
-  int stencil_min = 0, stencil_max = 0;
-};
+void rflu::set_error(const std::string& msg) { g_err = msg; }
 
 namespace {
 
@@ -97,11 +80,17 @@ void face_geom(const rflu_mesh& m, const int32_t* fv, double* vec, double* cen) 
   }
 }
 
-// Local faces of a hexahedron, corner index = bx + 2*by + 4*bz, given as perimeter
-// cycles; orientation is fixed afterwards from the undisplaced geometry.
-const int HEX_FACES[6][4] = {{0, 2, 6, 4}, {1, 3, 7, 5}, {0, 1, 5, 4},
-                             {2, 3, 7, 6}, {0, 1, 3, 2}, {4, 5, 7, 6}};
-const int TET_FACES[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+// Local faces of the four cell kinds, 0-based, in the reference's order and vertex
+// cycles (f2vTet, f2vHex, f2vPri, f2vPyr of modflu/RFLU_ModGrid.F90:92-102): a cycle's
+// right-hand normal (libfloflu/FaceVector.F90) points out of the cell.  -1 ends a
+// triangle.
+const int LF_TET[4][4] = {{0, 1, 2, -1}, {1, 3, 2, -1}, {0, 2, 3, -1}, {0, 3, 1, -1}};
+const int LF_HEX[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5},
+                          {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}};
+const int LF_PRI[5][4] = {{0, 2, 1, -1}, {0, 1, 4, 3}, {1, 2, 5, 4}, {0, 3, 5, 2}, {3, 4, 5, -1}};
+const int LF_PYR[5][4] = {{0, 3, 2, 1}, {0, 1, 4, -1}, {1, 2, 4, -1}, {2, 3, 4, -1}, {0, 4, 3, -1}};
+struct CellKind { int nv, nf; const int (*lf)[4]; };
+const CellKind KINDS[4] = {{4, 4, LF_TET}, {8, 6, LF_HEX}, {6, 5, LF_PRI}, {5, 5, LF_PYR}};
 
 struct FaceKey {
   int32_t v[4];
@@ -225,17 +214,19 @@ void rank_sort(std::vector<double>& a, std::vector<int32_t>& b) {
 }
 
 void build_stencils(rflu_mesh& m) {
-  const int N = m.n_cells, nv = m.nvpc;
+  const int N = m.n_cells;
+  const int32_t* cp = m.c2v_ptr.data();
+  const int32_t* cvx = m.c2v.data();
   // vertex -> cells, ascending cell order
   std::vector<int32_t> v2c_ptr(m.n_verts + 1, 0), v2c;
   for (int c = 0; c < N; ++c)
-    for (int k = 0; k < nv; ++k) v2c_ptr[m.c2v[nv * c + k] + 1]++;
+    for (int k = cp[c]; k < cp[c + 1]; ++k) v2c_ptr[cvx[k] + 1]++;
   for (int v = 0; v < m.n_verts; ++v) v2c_ptr[v + 1] += v2c_ptr[v];
   v2c.resize(v2c_ptr[m.n_verts]);
   {
     std::vector<int32_t> fill(v2c_ptr.begin(), v2c_ptr.end() - 1);
     for (int c = 0; c < N; ++c)
-      for (int k = 0; k < nv; ++k) v2c[fill[m.c2v[nv * c + k]]++] = c;
+      for (int k = cp[c]; k < cp[c + 1]; ++k) v2c[fill[cvx[k]]++] = c;
   }
   // Stencil size rules: RFLU_SetInfoC2CStencil calls the four-argument
   // RFLU_ComputeStencilSize(global, dimens=3, factor=1, order=1)
@@ -259,9 +250,11 @@ void build_stencils(rflu_mesh& m) {
     // Basic stencil: vertex neighbours, ranked by shared-vertex fraction, best
     // stencilSizeMax kept, then sorted by cell index
     // (modflu/RFLU_ModStencilsCells.F90:1207-1416).
+    // the shared-vertex fraction is counted against the cell's own vertex count
+    const int nv = cp[icg + 1] - cp[icg];
     tmp.clear();
     for (int k = 0; k < nv; ++k) {
-      int v = m.c2v[nv * icg + k];
+      int v = cvx[cp[icg] + k];
       for (int j = v2c_ptr[v]; j < v2c_ptr[v + 1]; ++j) tmp.push_back(v2c[j]);
     }
     std::sort(tmp.begin(), tmp.end());
@@ -270,14 +263,15 @@ void build_stencils(rflu_mesh& m) {
     frac.clear();
     fid.clear();
     int vs[8];
-    for (int k = 0; k < nv; ++k) vs[k] = m.c2v[nv * icg + k];
+    for (int k = 0; k < nv; ++k) vs[k] = cvx[cp[icg] + k];
     std::sort(vs, vs + nv);
     for (int c2 : tmp) {
       int vs2[8];
-      for (int k = 0; k < nv; ++k) vs2[k] = m.c2v[nv * c2 + k];
-      std::sort(vs2, vs2 + nv);
+      const int nv2 = cp[c2 + 1] - cp[c2];
+      for (int k = 0; k < nv2; ++k) vs2[k] = cvx[cp[c2] + k];
+      std::sort(vs2, vs2 + nv2);
       int common = 0;
-      for (int a = 0, b = 0; a < nv && b < nv;) {
+      for (int a = 0, b = 0; a < nv && b < nv2;) {
         if (vs[a] == vs2[b]) { ++common; ++a; ++b; }
         else if (vs[a] < vs2[b]) ++a;
         else ++b;
@@ -307,8 +301,8 @@ void build_stencils(rflu_mesh& m) {
       bool full = false;
       for (int isl = lay_beg; isl < lay_end && !full; ++isl) {
         int c = memb[isl];
-        for (int k = 0; k < nv && !full; ++k) {
-          int v = m.c2v[nv * c + k];
+        for (int k = cp[c]; k < cp[c + 1] && !full; ++k) {
+          int v = cvx[k];
           for (int j = v2c_ptr[v]; j < v2c_ptr[v + 1]; ++j) {
             int c2 = v2c[j];
             if (c2 == icg) continue;
@@ -433,25 +427,25 @@ void build_bface_stencils(rflu_mesh& m) {
     m.c2cs_bface.insert(m.c2cs_bface.end(), lists[icg].begin(), lists[icg].end());
 }
 
-int build_box(rflu_mesh& m, int kind, int nx, int ny, int nz, double lx, double ly, double lz,
-              double x0, double y0, double z0, double jitter, uint64_t seed,
-              const int32_t* bc_type, const int32_t* bc_opt, const double* bc_pout,
-              int want_stencils) {
+// A jittered box as a grid: nodes in i-fastest order, interior nodes displaced by
+// U(-jitter, jitter) x spacing per coordinate from a splitmix64 stream; one hexahedron
+// or six Kuhn tetrahedra (axis order a, b, c: corner 0 -> e_a -> e_a+e_b -> 7) per box
+// cell, with their vertices ordered as the reference's local faces expect (the right-
+// hand normal of every local face points out of the cell on the undisplaced nodes);
+// patches x-, x+, y-, y+, z-, z+ holding the cell faces on each side, in cell order.
+int grid_box(rflu_grid& g, int kind, int nx, int ny, int nz, double lx, double ly, double lz,
+             double x0, double y0, double z0, double jitter, uint64_t seed) {
   if (nx < 1 || ny < 1 || nz < 1) { g_err = "box dimensions must be >= 1"; return 1; }
   if (kind != RFLU_MESH_HEX && kind != RFLU_MESH_TET6) { g_err = "unknown cell kind"; return 1; }
   if (!(jitter >= 0.0 && jitter < 0.25)) { g_err = "jitter must be in [0, 0.25)"; return 1; }
   const int64_t ncell64 = (int64_t)nx * ny * nz * (kind == RFLU_MESH_TET6 ? 6 : 1);
   if (ncell64 > 400000000LL) { g_err = "mesh too large"; return 1; }
-  m.kind = kind; m.nx = nx; m.ny = ny; m.nz = nz;
-  m.nvpc = kind == RFLU_MESH_HEX ? 8 : 4;
-  m.nfpc = kind == RFLU_MESH_HEX ? 6 : 4;
   const int NX = nx + 1, NY = ny + 1, NZ = nz + 1;
-  m.n_verts = NX * NY * NZ;
+  g.n_verts = NX * NY * NZ;
   const double dx = lx / nx, dy = ly / ny, dz = lz / nz;
 
-  // Undisplaced node positions (used to orient faces), then displaced ones.
-  std::vector<double> xyz0((size_t)3 * m.n_verts);
-  m.xyz.resize((size_t)3 * m.n_verts);
+  std::vector<double> xyz0((size_t)3 * g.n_verts);
+  g.xyz.resize((size_t)3 * g.n_verts);
   SplitMix64 rng(seed);
   for (int k = 0; k < NZ; ++k)
     for (int j = 0; j < NY; ++j)
@@ -468,15 +462,16 @@ int build_box(rflu_mesh& m, int kind, int nx, int ny, int nz, double lx, double 
           py += jitter * dy * rng.sym();
           pz += jitter * dz * rng.sym();
         }
-        m.xyz[3 * v] = px; m.xyz[3 * v + 1] = py; m.xyz[3 * v + 2] = pz;
+        g.xyz[3 * v] = px; g.xyz[3 * v + 1] = py; g.xyz[3 * v + 2] = pz;
       }
 
-  // Cells
-  const int nsub = kind == RFLU_MESH_HEX ? 1 : 6;
-  m.n_cells = (int)ncell64;
-  m.c2v.resize((size_t)m.nvpc * m.n_cells);
-  // Kuhn split: the tetrahedron of axis order (a,b,c) is 0 -> e_a -> e_a+e_b -> 7.
+  // corner index b = bx + 2*by + 4*bz; the reference's hexahedron is the bottom
+  // quadrilateral counter-clockwise seen from +z, then the top one above it
+  const int HEX_ORDER[8] = {0, 1, 3, 2, 4, 5, 7, 6};
   const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  std::vector<int32_t>& c2v = kind == RFLU_MESH_HEX ? g.hex2v : g.tet2v;
+  const int nv = kind == RFLU_MESH_HEX ? 8 : 4;
+  c2v.resize((size_t)nv * ncell64);
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -487,39 +482,169 @@ int build_box(rflu_mesh& m, int kind, int nx, int ny, int nz, double lx, double 
         }
         size_t box = (size_t)i + (size_t)nx * (j + (size_t)ny * k);
         if (kind == RFLU_MESH_HEX) {
-          for (int b = 0; b < 8; ++b) m.c2v[8 * box + b] = corner[b];
-        } else {
-          for (int t = 0; t < 6; ++t) {
-            int b1 = 1 << perms[t][0];
-            int b2 = b1 | (1 << perms[t][1]);
-            int32_t* tv = &m.c2v[4 * (6 * box + t)];
-            tv[0] = corner[0]; tv[1] = corner[b1]; tv[2] = corner[b2]; tv[3] = corner[7];
+          for (int b = 0; b < 8; ++b) c2v[8 * box + b] = corner[HEX_ORDER[b]];
+          continue;
+        }
+        for (int t = 0; t < 6; ++t) {
+          int b1 = 1 << perms[t][0];
+          int b2 = b1 | (1 << perms[t][1]);
+          int32_t* tv = &c2v[4 * (6 * box + t)];
+          tv[0] = corner[0]; tv[1] = corner[b1]; tv[2] = corner[b2]; tv[3] = corner[7];
+          // local face (0,1,2) must face away from vertex 3
+          const double* p0 = &xyz0[3 * tv[0]];
+          double a[3], c[3], e[3];
+          for (int d = 0; d < 3; ++d) {
+            a[d] = xyz0[3 * tv[1] + d] - p0[d];
+            c[d] = xyz0[3 * tv[2] + d] - p0[d];
+            e[d] = xyz0[3 * tv[3] + d] - p0[d];
           }
+          double det = (a[1] * c[2] - a[2] * c[1]) * e[0] + (a[2] * c[0] - a[0] * c[2]) * e[1] +
+                       (a[0] * c[1] - a[1] * c[0]) * e[2];
+          if (det > 0) std::swap(tv[1], tv[2]);
         }
       }
-  (void)nsub;
 
-  // Face list: faces are created in the order cells and their local faces are
-  // visited; the first visiting cell is c1 and the vertex cycle is oriented so that
-  // the face vector points out of c1.
-  struct Pending { int32_t c1, c2, v[4], patch; };
+  // patch faces: the cell faces whose vertices all lie on one box side
+  auto on_side = [&](int v, int s) {
+    int i = v % NX, j = (v / NX) % NY, k = v / (NX * NY);
+    switch (s) {
+      case 0: return i == 0;
+      case 1: return i == nx;
+      case 2: return j == 0;
+      case 3: return j == ny;
+      case 4: return k == 0;
+      default: return k == nz;
+    }
+  };
+  const CellKind& ck = KINDS[kind == RFLU_MESH_HEX ? 1 : 0];
+  std::vector<std::vector<int32_t>> side_faces(6);
+  for (int64_t c = 0; c < ncell64; ++c) {
+    const int32_t* cv = &c2v[(size_t)nv * c];
+    for (int lf = 0; lf < ck.nf; ++lf) {
+      const int nvf = ck.lf[lf][3] < 0 ? 3 : 4;
+      for (int s = 0; s < 6; ++s) {
+        bool all = true;
+        for (int a = 0; a < nvf && all; ++a) all = on_side(cv[ck.lf[lf][a]], s);
+        if (!all) continue;
+        for (int a = 0; a < nvf; ++a) side_faces[s].push_back(cv[ck.lf[lf][a]]);
+        break;
+      }
+    }
+  }
+  g.patch_type.assign(6, 0);
+  g.patch_ntri.assign(6, 0);
+  g.patch_nquad.assign(6, 0);
+  g.patch_name = {"x-", "x+", "y-", "y+", "z-", "z+"};
+  for (int s = 0; s < 6; ++s) {
+    std::vector<int32_t>& dst = kind == RFLU_MESH_HEX ? g.bquad2v : g.btri2v;
+    dst.insert(dst.end(), side_faces[s].begin(), side_faces[s].end());
+    (kind == RFLU_MESH_HEX ? g.patch_nquad : g.patch_ntri)[s] =
+        (int32_t)side_faces[s].size() / (kind == RFLU_MESH_HEX ? 4 : 3);
+  }
+  g.title = "box";
+  return 0;
+}
+
+}  // namespace
+
+// The region of a grid.  Faces are created in the order cells (tetrahedra, hexahedra,
+// prisms, pyramids, as the reference numbers them globally) and their local faces
+// are visited; the first visiting cell is c1 and the face keeps that cell's local
+// vertex cycle, which points out of c1 (modflu/RFLU_ModFaceList.F90:784-1000).
+// Boundary faces are the patches' faces, patch by patch, triangles then
+// quadrilaterals, with the grid's vertex cycles and the cell found by matching
+// vertex sets (RFLU_ModFaceList.F90:1025-1085).
+int rflu::build_region(rflu_mesh& m, const rflu_grid& g, const int32_t* bc_type,
+                       const int32_t* bc_opt, const double* bc_pout, int want_stencils) {
+  const int V = g.n_verts;
+  if (V < 4 || g.xyz.size() != (size_t)3 * V) { g_err = "grid: bad vertex count"; return 1; }
+  const std::vector<int32_t>* lists[4] = {&g.tet2v, &g.hex2v, &g.pri2v, &g.pyr2v};
+  int64_t ncell64 = 0, nkind[4];
+  int kinds_present = 0, only = -1;
+  for (int t = 0; t < 4; ++t) {
+    if (lists[t]->size() % KINDS[t].nv) { g_err = "grid: ragged cell list"; return 1; }
+    nkind[t] = (int64_t)lists[t]->size() / KINDS[t].nv;
+    ncell64 += nkind[t];
+    if (nkind[t]) { ++kinds_present; only = t; }
+  }
+  if (ncell64 < 1) { g_err = "grid: no cells"; return 1; }
+  if (ncell64 > 400000000LL) { g_err = "mesh too large"; return 1; }
+  const int P = (int)g.patch_type.size();
+  if (P < 1 || (int)g.patch_ntri.size() != P || (int)g.patch_nquad.size() != P) {
+    g_err = "grid: no patches";
+    return 1;
+  }
+  {
+    int64_t nt = 0, nq = 0;
+    for (int p = 0; p < P; ++p) {
+      if (g.patch_ntri[p] < 0 || g.patch_nquad[p] < 0) { g_err = "grid: bad patch"; return 1; }
+      nt += g.patch_ntri[p];
+      nq += g.patch_nquad[p];
+    }
+    if ((size_t)(3 * nt) != g.btri2v.size() || (size_t)(4 * nq) != g.bquad2v.size()) {
+      g_err = "grid: patch face counts do not match the face lists";
+      return 1;
+    }
+  }
+  for (int p = 0; p < P; ++p)
+    if (bc_type[p] != RFLU_BC_OUTFLOW && bc_type[p] != RFLU_BC_SLIPWALL &&
+        bc_type[p] != RFLU_BC_INFLOW_VELTEMP && bc_type[p] != RFLU_BC_INFLOW_TOTANG &&
+        bc_type[p] != RFLU_BC_FARFIELD && bc_type[p] != RFLU_BC_INJECTION) {
+      g_err = "patch " + std::to_string(p + 1) + " has boundary type " +
+              std::to_string(bc_type[p]) +
+              "; built types: inflow (10, 11), outflow (20), slip wall (60), farfield (80), "
+              "injection (90)";
+      return 4;
+    }
+  for (const std::vector<int32_t>* l : lists)
+    for (int32_t v : *l)
+      if (v < 0 || v >= V) { g_err = "grid: cell vertex out of range"; return 1; }
+  for (int32_t v : g.btri2v)
+    if (v < 0 || v >= V) { g_err = "grid: patch vertex out of range"; return 1; }
+  for (int32_t v : g.bquad2v)
+    if (v < 0 || v >= V) { g_err = "grid: patch vertex out of range"; return 1; }
+
+  m.n_verts = V;
+  m.xyz = g.xyz;
+  m.n_cells = (int)ncell64;
+  m.n_tets = (int)nkind[0]; m.n_hexs = (int)nkind[1];
+  m.n_pris = (int)nkind[2]; m.n_pyrs = (int)nkind[3];
+  m.nvpc = kinds_present == 1 ? KINDS[only].nv : 0;
+  m.n_patches = P;
+  m.c2v_ptr.assign((size_t)m.n_cells + 1, 0);
+  m.c2v.clear();
+  m.c2v.reserve(g.tet2v.size() + g.hex2v.size() + g.pri2v.size() + g.pyr2v.size());
+  std::vector<int8_t> ckind(m.n_cells);
+  {
+    int c = 0;
+    for (int t = 0; t < 4; ++t)
+      for (int64_t i = 0; i < nkind[t]; ++i, ++c) {
+        ckind[c] = (int8_t)t;
+        m.c2v.insert(m.c2v.end(), lists[t]->begin() + KINDS[t].nv * i,
+                     lists[t]->begin() + KINDS[t].nv * (i + 1));
+        m.c2v_ptr[c + 1] = m.c2v_ptr[c] + KINDS[t].nv;
+      }
+  }
+
+  struct Pending { int32_t c1, c2, v[4], bface; };
   std::vector<Pending> faces;
-  faces.reserve((size_t)m.n_cells * (kind == RFLU_MESH_HEX ? 3 : 2) + 16);
+  faces.reserve((size_t)m.n_cells * 3 + 16);
   std::unordered_map<FaceKey, int32_t, FaceKeyHash> map;
   map.reserve(faces.capacity() * 2);
-  const int nfl = m.nfpc, nvf = kind == RFLU_MESH_HEX ? 4 : 3;
+  auto make_key = [](const int32_t* fv, int nvf) {
+    FaceKey key;
+    for (int a = 0; a < 4; ++a) key.v[a] = a < nvf ? fv[a] : -1;
+    std::sort(key.v, key.v + nvf);
+    return key;
+  };
   for (int c = 0; c < m.n_cells; ++c) {
-    const int32_t* cv = &m.c2v[(size_t)m.nvpc * c];
-    double cc[3] = {0, 0, 0};
-    for (int a = 0; a < m.nvpc; ++a)
-      for (int d = 0; d < 3; ++d) cc[d] += xyz0[3 * cv[a] + d] / m.nvpc;
-    for (int lf = 0; lf < nfl; ++lf) {
+    const CellKind& ck = KINDS[ckind[c]];
+    const int32_t* cv = &m.c2v[m.c2v_ptr[c]];
+    for (int lf = 0; lf < ck.nf; ++lf) {
+      const int nvf = ck.lf[lf][3] < 0 ? 3 : 4;
       int32_t fv[4] = {-1, -1, -1, -1};
-      for (int a = 0; a < nvf; ++a)
-        fv[a] = kind == RFLU_MESH_HEX ? cv[HEX_FACES[lf][a]] : cv[TET_FACES[lf][a]];
-      FaceKey key;
-      std::memcpy(key.v, fv, sizeof(fv));
-      std::sort(key.v, key.v + nvf);
+      for (int a = 0; a < nvf; ++a) fv[a] = cv[ck.lf[lf][a]];
+      FaceKey key = make_key(fv, nvf);
       auto it = map.find(key);
       if (it != map.end()) {
         Pending& p = faces[it->second];
@@ -528,90 +653,70 @@ int build_box(rflu_mesh& m, int kind, int nx, int ny, int nz, double lx, double 
         map.erase(it);
         continue;
       }
-      // orient outward from c using the undisplaced nodes
-      double vec[3], cen[3];
-      {
-        const double* p1 = &xyz0[3 * fv[0]];
-        const double* p2 = &xyz0[3 * fv[1]];
-        const double* p3 = &xyz0[3 * fv[2]];
-        if (nvf == 3) { face_vector_tria(p1, p2, p3, vec); face_centroid_tria(p1, p2, p3, cen); }
-        else {
-          const double* p4 = &xyz0[3 * fv[3]];
-          face_vector_quad(p1, p2, p3, p4, vec);
-          face_centroid_quad(p1, p2, p3, p4, cen);
-        }
-      }
-      double dot = 0;
-      for (int d = 0; d < 3; ++d) dot += vec[d] * (cen[d] - cc[d]);
-      if (dot < 0) std::reverse(fv, fv + nvf);
       Pending p;
-      p.c1 = c; p.c2 = -1; p.patch = -1;
+      p.c1 = c; p.c2 = -1; p.bface = -1;
       std::memcpy(p.v, fv, sizeof(fv));
       map.emplace(key, (int32_t)faces.size());
       faces.push_back(p);
     }
   }
-  // Split into interior faces and boundary faces per patch.
-  auto vijk = [&](int v, int& i, int& j, int& k) {
-    i = v % NX; j = (v / NX) % NY; k = v / (NX * NY);
-  };
-  std::vector<std::vector<int32_t>> pfaces(6);
-  m.f2c.clear(); m.f2v.clear();
-  for (size_t f = 0; f < faces.size(); ++f) {
-    Pending& p = faces[f];
-    if (p.c2 >= 0) {
-      m.f2c.push_back(p.c1); m.f2c.push_back(p.c2);
-      for (int a = 0; a < 4; ++a) m.f2v.push_back(p.v[a]);
-      continue;
-    }
-    int side = -1;
-    int cnt[6] = {0, 0, 0, 0, 0, 0};
-    for (int a = 0; a < nvf; ++a) {
-      int i, j, k;
-      vijk(p.v[a], i, j, k);
-      cnt[0] += i == 0; cnt[1] += i == nx; cnt[2] += j == 0;
-      cnt[3] += j == ny; cnt[4] += k == 0; cnt[5] += k == nz;
-    }
-    for (int s = 0; s < 6; ++s)
-      if (cnt[s] == nvf) { side = s; break; }
-    if (side < 0) { g_err = "boundary face not on a box side"; return 3; }
-    pfaces[side].push_back((int32_t)f);
-  }
-  m.n_faces = (int)(m.f2c.size() / 2);
-  m.bpatch_ptr.assign(7, 0);
+  // boundary faces, patch by patch
+  m.bpatch_ptr.assign(P + 1, 0);
   m.bf2c.clear(); m.bf2v.clear();
-  for (int s = 0; s < 6; ++s) {
-    for (int32_t f : pfaces[s]) {
-      m.bf2c.push_back(faces[f].c1);
-      for (int a = 0; a < 4; ++a) m.bf2v.push_back(faces[f].v[a]);
+  {
+    size_t it_tri = 0, it_quad = 0;
+    for (int p = 0; p < P; ++p) {
+      for (int pass = 0; pass < 2; ++pass) {
+        const int nvf = pass == 0 ? 3 : 4;
+        const int cnt = pass == 0 ? g.patch_ntri[p] : g.patch_nquad[p];
+        for (int k = 0; k < cnt; ++k) {
+          const int32_t* src = pass == 0 ? &g.btri2v[3 * it_tri++] : &g.bquad2v[4 * it_quad++];
+          int32_t fv[4] = {-1, -1, -1, -1};
+          for (int a = 0; a < nvf; ++a) fv[a] = src[a];
+          auto f = map.find(make_key(fv, nvf));
+          if (f == map.end() || faces[f->second].bface >= 0) {
+            g_err = "patch " + std::to_string(p + 1) + " face " + std::to_string(k + 1) +
+                    (f == map.end() ? " is not a boundary face of any cell"
+                                    : " is on the boundary twice");
+            return 3;
+          }
+          faces[f->second].bface = (int32_t)m.bf2c.size();
+          m.bf2c.push_back(faces[f->second].c1);
+          for (int a = 0; a < 4; ++a) m.bf2v.push_back(fv[a]);
+        }
+      }
+      m.bpatch_ptr[p + 1] = (int32_t)m.bf2c.size();
     }
-    m.bpatch_ptr[s + 1] = (int32_t)m.bf2c.size();
   }
+  if (m.bf2c.size() != map.size()) {
+    g_err = "grid: " + std::to_string(map.size() - m.bf2c.size()) +
+            " cell faces have one cell and lie on no patch";
+    return 3;
+  }
+  m.f2c.clear(); m.f2v.clear();
+  for (const Pending& p : faces) {
+    if (p.c2 < 0) continue;
+    m.f2c.push_back(p.c1); m.f2c.push_back(p.c2);
+    for (int a = 0; a < 4; ++a) m.f2v.push_back(p.v[a]);
+  }
+  std::vector<Pending>().swap(faces);
+  m.n_faces = (int)(m.f2c.size() / 2);
   m.n_bfaces = (int)m.bf2c.size();
-  m.patch_bc.assign(bc_type, bc_type + 6);
-  m.patch_opt.assign(bc_opt, bc_opt + 6);
-  m.patch_pout.assign(bc_pout, bc_pout + 6);
-  for (int s = 0; s < 6; ++s)
-    if (bc_type[s] != RFLU_BC_OUTFLOW && bc_type[s] != RFLU_BC_SLIPWALL &&
-        bc_type[s] != RFLU_BC_INFLOW_VELTEMP && bc_type[s] != RFLU_BC_INFLOW_TOTANG &&
-        bc_type[s] != RFLU_BC_FARFIELD && bc_type[s] != RFLU_BC_INJECTION) {
-      g_err = "patch types: inflow (10, 11), outflow (20), slip wall (60), farfield (80), "
-              "injection (90)";
-      return 4;
-    }
+  m.patch_bc.assign(bc_type, bc_type + P);
+  m.patch_opt.assign(bc_opt, bc_opt + P);
+  m.patch_pout.assign(bc_pout, bc_pout + P);
 
   // Geometry (modflu/RFLU_ModGeometry.F90:307-665): approximate centroids, face
   // vectors/centroids, volumes and centroids from the divergence theorem.
   const int N = m.n_cells, F = m.n_faces, B = m.n_bfaces;
   std::vector<double> app((size_t)3 * N, 0.0);
-  {
-    const double term = 1.0 / m.nvpc;
-    for (int c = 0; c < N; ++c) {
-      double s[3] = {0, 0, 0};
-      for (int a = 0; a < m.nvpc; ++a)
-        for (int d = 0; d < 3; ++d) s[d] += m.xyz[3 * m.c2v[(size_t)m.nvpc * c + a] + d];
-      for (int d = 0; d < 3; ++d) app[3 * c + d] = term * s[d];
-    }
+  for (int c = 0; c < N; ++c) {
+    const int nv = m.c2v_ptr[c + 1] - m.c2v_ptr[c];
+    const double term = 1.0 / nv;
+    double s[3] = {0, 0, 0};
+    for (int a = m.c2v_ptr[c]; a < m.c2v_ptr[c + 1]; ++a)
+      for (int d = 0; d < 3; ++d) s[d] += m.xyz[3 * m.c2v[a] + d];
+    for (int d = 0; d < 3; ++d) app[3 * c + d] = term * s[d];
   }
   m.vol.assign(N, 0.0);
   m.cofg.assign((size_t)3 * N, 0.0);
@@ -650,6 +755,13 @@ int build_box(rflu_mesh& m, int kind, int nx, int ny, int nz, double lx, double 
     int c1 = m.bf2c[b];
     double t = (c[0] - app[3 * c1]) * v[0] + (c[1] - app[3 * c1 + 1]) * v[1] +
                (c[2] - app[3 * c1 + 2]) * v[2];
+    if (!(t > 0.0)) {   // a patch face's vertex cycle must point out of its cell
+      int p = (int)(std::upper_bound(m.bpatch_ptr.begin(), m.bpatch_ptr.end(), b) -
+                    m.bpatch_ptr.begin()) - 1;
+      g_err = "patch " + std::to_string(p + 1) + " face " +
+              std::to_string(b - m.bpatch_ptr[p] + 1) + " points into its cell";
+      return 5;
+    }
     m.vol[c1] = m.vol[c1] + t;
     t = c[0] * v[0] + c[1] * v[1] + c[2] * v[2];
     for (int d = 0; d < 3; ++d) m.cofg[3 * c1 + d] = m.cofg[3 * c1 + d] + t * c[d];
@@ -688,9 +800,48 @@ int build_box(rflu_mesh& m, int kind, int nx, int ny, int nz, double lx, double 
   return 0;
 }
 
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    g_err = "out of host memory";
+    return 6;
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int rflu_grid_box(int kind, int nx, int ny, int nz, double lx, double ly, double lz, double x0,
+                  double y0, double z0, double jitter, uint64_t seed, rflu_grid** out) {
+  if (!out) { g_err = "null argument"; return 1; }
+  *out = nullptr;
+  rflu_grid* g = new (std::nothrow) rflu_grid();
+  if (!g) { g_err = "out of host memory"; return 6; }
+  int rc = guarded([&] { return grid_box(*g, kind, nx, ny, nz, lx, ly, lz, x0, y0, z0, jitter, seed); });
+  if (rc) { delete g; return rc; }
+  *out = g;
+  return 0;
+}
+
+int rflu_mesh_build(const rflu_grid* g, const int32_t* bc_type, const int32_t* bc_opt,
+                    const double* bc_pout, int build_stencils_flag, rflu_mesh** out) {
+  if (!g || !out || !bc_type || !bc_opt || !bc_pout) { g_err = "null argument"; return 1; }
+  *out = nullptr;
+  rflu_mesh* m = new (std::nothrow) rflu_mesh();
+  if (!m) { g_err = "out of host memory"; return 6; }
+  int rc = guarded([&] {
+    return rflu::build_region(*m, *g, bc_type, bc_opt, bc_pout, build_stencils_flag);
+  });
+  if (rc) { delete m; return rc; }
+  m->kind = RFLU_MESH_GRID;
+  *out = m;
+  return 0;
+}
 
 int rflu_mesh_box(int kind, int nx, int ny, int nz, double lx, double ly, double lz, double x0,
                   double y0, double z0, double jitter, uint64_t seed, const int32_t bc_type[6],
@@ -700,15 +851,13 @@ int rflu_mesh_box(int kind, int nx, int ny, int nz, double lx, double ly, double
   *out = nullptr;
   rflu_mesh* m = new (std::nothrow) rflu_mesh();
   if (!m) { g_err = "out of host memory"; return 6; }
-  int rc;
-  try {
-    rc = build_box(*m, kind, nx, ny, nz, lx, ly, lz, x0, y0, z0, jitter, seed, bc_type, bc_opt,
-                   bc_pout, build_stencils_flag);
-  } catch (const std::bad_alloc&) {
-    g_err = "out of host memory";
-    rc = 6;
-  }
+  int rc = guarded([&] {
+    rflu_grid g;
+    int r = grid_box(g, kind, nx, ny, nz, lx, ly, lz, x0, y0, z0, jitter, seed);
+    return r ? r : rflu::build_region(*m, g, bc_type, bc_opt, bc_pout, build_stencils_flag);
+  });
   if (rc) { delete m; return rc; }
+  m->kind = kind; m->nx = nx; m->ny = ny; m->nz = nz;
   *out = m;
   return 0;
 }
@@ -737,6 +886,7 @@ int rflu_mesh_array(const rflu_mesh* m, const char* name, const void** ptr, int6
   if (n == "patch_bc") return I(m->patch_bc);
   if (n == "patch_opt") return I(m->patch_opt);
   if (n == "c2v") return I(m->c2v);
+  if (n == "c2v_ptr") return I(m->c2v_ptr);
   if (n == "c2cs_ptr") return I(m->c2cs_ptr);
   if (n == "c2cs_idx") return I(m->c2cs_idx);
   if (n == "c2cs_bptr") return I(m->c2cs_bptr);

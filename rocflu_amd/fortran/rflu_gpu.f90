@@ -1,7 +1,8 @@
 !******************************************************************************
 !
 ! rflu_gpu -- ISO_C_BINDING interface of the MI355X explicit residual/update path
-! (include/rocflu_amd.h) and of the host region builder (include/rflu_mesh.h).
+! (include/rocflu_amd.h), of the host region builder and grid files
+! (include/rflu_mesh.h) and of the user input files (include/rflu_input.h).
 !
 ! A reference host keeps its t_region and replaces the bodies of RungeKuttaMP's
 ! callees with these calls, one region handle per t_region:
@@ -44,6 +45,37 @@ MODULE rflu_gpu
     TYPE(C_PTR) :: patch_opt2
     INTEGER(C_INT32_T) :: n_virtual, pad2_
   END TYPE rflu_region_desc
+
+  ! rflu_inp and rflu_bc_patch of include/rflu_input.h
+  TYPE, BIND(C) :: rflu_inp
+    INTEGER(C_INT32_T) :: grid_format, solution_format, grid_source
+    INTEGER(C_INT32_T) :: flow_model, move_grid, gas_model
+    REAL(C_DOUBLE) :: cfl
+    INTEGER(C_INT32_T) :: discr, order
+    REAL(C_DOUBLE) :: eps_entr, diss_fact
+    INTEGER(C_INT32_T) :: reconst, dimens, creconst_cells, creconst_faces
+    REAL(C_DOUBLE) :: creconst_cells_weight, creconst_faces_weight, toler_ict
+    INTEGER(C_INT32_T) :: stencil_dimens_cells, stencil_dimens_faces, stencil_dimens_bfaces
+    INTEGER(C_INT32_T) :: order_bfaces, axi_flag
+    REAL(C_DOUBLE) :: sigma, kp, ku, minf
+    INTEGER(C_INT32_T) :: flow_type, solver_type, rk_scheme, start_step, max_iter
+    INTEGER(C_INT32_T) :: write_iter, print_iter
+    REAL(C_DOUBLE) :: dt_imposed, max_time, write_time, print_time, dt_min_limit, res_tol
+    REAL(C_DOUBLE) :: ref_velocity, ref_pressure, ref_density, ref_cp, ref_gamma, ref_length
+    INTEGER(C_INT32_T) :: init_flag
+    REAL(C_DOUBLE) :: init_vel(3), init_press, init_dens, init_temp
+    INTEGER(C_INT32_T) :: init_ival(19)
+    REAL(C_DOUBLE) :: init_rval(25)
+    INTEGER(C_INT32_T) :: sections
+  END TYPE rflu_inp
+
+  TYPE, BIND(C) :: rflu_bc_patch
+    INTEGER(C_INT32_T) :: bc_type, opt, opt2, order, creconst, related, axis
+    REAL(C_DOUBLE) :: angle
+    REAL(C_DOUBLE) :: vals(5)
+    REAL(C_DOUBLE) :: pout
+    CHARACTER(KIND=C_CHAR) :: name(80)
+  END TYPE rflu_bc_patch
 
   TYPE, BIND(C) :: rflu_gpu_stats
     INTEGER(C_INT64_T) :: n_invalid, n_negative, n_bounds_fix, n_pos_fix
@@ -247,6 +279,60 @@ MODULE rflu_gpu
       INTEGER(C_INT64_T), INTENT(OUT) :: count
       INTEGER(C_INT), INTENT(OUT) :: dtype
     END FUNCTION
+
+    FUNCTION rflu_mesh_last_error() BIND(C, NAME='rflu_mesh_last_error')
+      IMPORT :: C_PTR
+      TYPE(C_PTR) :: rflu_mesh_last_error
+    END FUNCTION
+
+    ! grids (include/rflu_mesh.h)
+    INTEGER(C_INT) FUNCTION rflu_grid_read_centaur(path, out) &
+        BIND(C, NAME='rflu_grid_read_centaur')
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+      TYPE(C_PTR), INTENT(OUT) :: out
+    END FUNCTION
+
+    SUBROUTINE rflu_grid_destroy(g) BIND(C, NAME='rflu_grid_destroy')
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: g
+    END SUBROUTINE
+
+    INTEGER(C_INT) FUNCTION rflu_grid_sizes(g, out) BIND(C, NAME='rflu_grid_sizes')
+      IMPORT :: C_INT, C_PTR, C_INT64_T
+      TYPE(C_PTR), VALUE :: g
+      INTEGER(C_INT64_T), INTENT(OUT) :: out(8)
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_mesh_build(g, bc_type, bc_opt, bc_pout, build_stencils, out) &
+        BIND(C, NAME='rflu_mesh_build')
+      IMPORT :: C_INT, C_PTR, C_INT32_T, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g
+      INTEGER(C_INT32_T), INTENT(IN) :: bc_type(*), bc_opt(*)
+      REAL(C_DOUBLE), INTENT(IN) :: bc_pout(*)
+      INTEGER(C_INT), VALUE :: build_stencils
+      TYPE(C_PTR), INTENT(OUT) :: out
+    END FUNCTION
+
+    ! user input files (include/rflu_input.h)
+    INTEGER(C_INT) FUNCTION rflu_read_inp(path, out) BIND(C, NAME='rflu_read_inp')
+      IMPORT :: C_INT, C_CHAR, rflu_inp
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+      TYPE(rflu_inp), INTENT(OUT) :: out
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_read_bc(path, n_patches, order_bfaces, out) &
+        BIND(C, NAME='rflu_read_bc')
+      IMPORT :: C_INT, C_CHAR, rflu_bc_patch
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+      INTEGER(C_INT), VALUE :: n_patches, order_bfaces
+      TYPE(rflu_bc_patch), INTENT(OUT) :: out(*)
+    END FUNCTION
+
+    FUNCTION rflu_input_last_error() BIND(C, NAME='rflu_input_last_error')
+      IMPORT :: C_PTR
+      TYPE(C_PTR) :: rflu_input_last_error
+    END FUNCTION
   END INTERFACE
 
 CONTAINS
@@ -268,5 +354,25 @@ CONTAINS
       STOP 1
     END IF
   END SUBROUTINE rflu_gpu_check
+
+  ! The same for the host libraries: errptr is rflu_mesh_last_error() or
+  ! rflu_input_last_error().
+  SUBROUTINE rflu_host_check(rc, where, errptr)
+    INTEGER(C_INT), INTENT(IN) :: rc
+    CHARACTER(*), INTENT(IN) :: where
+    TYPE(C_PTR), INTENT(IN) :: errptr
+    CHARACTER(KIND=C_CHAR), POINTER :: msg(:)
+    INTEGER :: n
+    IF ( rc /= 0 ) THEN
+      CALL C_F_POINTER(errptr, msg, [512])
+      n = 0
+      DO WHILE ( n < 512 )
+        IF ( msg(n+1) == C_NULL_CHAR ) EXIT
+        n = n + 1
+      END DO
+      WRITE(*,'(A,A,I0,A,512A)') where, ': error ', rc, ': ', msg(1:n)
+      STOP 1
+    END IF
+  END SUBROUTINE rflu_host_check
 
 END MODULE rflu_gpu

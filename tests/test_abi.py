@@ -28,8 +28,11 @@ def test_python_bindings_cover_header():
 
 def test_mesh_library_exports_header_symbols():
     lib = ctypes.CDLL(os.path.join(ROOT, "rocflu_amd", "librflu_mesh.so"))
-    missing = [n for n in declared("rflu_mesh.h") if not hasattr(lib, n)]
-    assert not missing, missing
+    for header in ("rflu_mesh.h", "rflu_input.h"):
+        names = declared(header)
+        assert len(names) >= 3, header
+        missing = [n for n in names if not hasattr(lib, n)]
+        assert not missing, (header, missing)
 
 
 def test_device_count_without_gpu_is_safe():

@@ -39,9 +39,14 @@ def test_oracle_near_reference_golden_gas_jump_constrained():
     ranked (RFLU_AddBFaces collects at most 18 in member order, RFLU_SortBFaces keeps
     9), and that order follows the reference's cell numbering, which comes from its
     shipped mesh generator and cannot be rebuilt here; permuting the member order moves
-    the result by ~1.5e-4.  So the golden pins the constrained run only to that band:
-    per x column, the sorted values of every field agree within 1e-3 of the field's
-    scale (density and energy within 1e-4), and the conserved totals to 1e-10.  The
+    the result by ~1.5e-4, and so does the rounding of the geometry: equidistant wall
+    faces are ranked by squared distances that differ in the last bits, and a box whose
+    hexahedra list their vertices in another order (another start of each face's
+    vertex cycle) ranks them differently (measured worst: rho u 8.7e-4 and 1.0e-3, rho E
+    6.4e-5 and 1.2e-4 for two vertex orders).  So the golden pins the constrained run
+    only to that band: per x column, the sorted values of every field agree within
+    1.5e-3 of the field's scale (density and energy within 2e-4), and the conserved
+    totals to 1e-10.  The
     constrained fit itself is pinned by tests/test_oracle_pins.py
     (test_constrained_gradient_exact_on_admissible_linear_fields)."""
     g = gasjump.load("gas_jump_closed_np2")
@@ -64,7 +69,7 @@ def test_oracle_near_reference_golden_gas_jump_constrained():
                 ours = np.sort(cv[ix == base + col, f])
                 worst[f] = max(worst[f], np.abs(gk - ours).max() / scale[f])
     print("per-field worst value-set difference / scale", worst)
-    assert worst.max() <= 1e-3 and worst[0] <= 1e-4 and worst[4] <= 1e-4, worst
+    assert worst.max() <= 1.5e-3 and worst[0] <= 2e-4 and worst[4] <= 2e-4, worst
     vol = m.vol
     for k, key in ((0, "gas_rho"), (4, "gas_rhoE")):
         tot = float((cv[:, k] * vol).sum())
