@@ -33,13 +33,17 @@ typedef struct rflu_grid rflu_grid;
 #define RFLU_MESH_TET6 1   /* six tetrahedra per box cell (Kuhn split on 0-7 axis) */
 #define RFLU_MESH_GRID 2   /* built by rflu_mesh_build from a grid                */
 
-/* Boundary types (modfloflu/ModParameters.F90:397-438). */
+/* Boundary types (modfloflu/ModParameters.F90:397-441). */
 #define RFLU_BC_INFLOW_TOTANG  10
 #define RFLU_BC_INFLOW_VELTEMP 11
 #define RFLU_BC_OUTFLOW  20
 #define RFLU_BC_SLIPWALL 60
 #define RFLU_BC_FARFIELD 80
 #define RFLU_BC_INJECTION 90
+#define RFLU_BC_SYMMETRY 100
+#define RFLU_BC_PERIODIC 110
+/* The outer faces of symmetry/periodic virtual layers (rflu_mesh_build_sype). */
+#define RFLU_BC_VIRTUAL  130
 
 /* Outflow options (modfloflu/ModParameters.F90:501-503). */
 #define RFLU_BCOPT_SUPERSONIC 0
@@ -106,6 +110,30 @@ int rflu_grid_array(const rflu_grid* g, const char* name,
  */
 int rflu_mesh_build(const rflu_grid* g, const int32_t* bc_type, const int32_t* bc_opt,
                     const double* bc_pout, int build_stencils, rflu_mesh** out);
+
+/*
+ * The region of a grid with symmetry (100) and periodic (110) patches, made as the
+ * reference's partitioner makes it (utilities/part/rflupart.F90:361-372,
+ * modflu/RFLU_ModSymmetryPeriodic.F90:148-1421): for each such patch in patch order,
+ * the cells touching the source patch's vertices (the patch itself for symmetry, the
+ * related patch for periodic) and two further vertex-neighbour layers -- virtual cells
+ * of earlier patches included, those of a periodic patch's own related patch not --
+ * are copied across the patch (mirrored in its plane, or moved by the period along
+ * AXIS) as virtual cells numbered after the actual ones.  related (1-based), axis
+ * (1..3) and angle (radians; 0 only) are the .bc file's RELPATCH, AXIS and ANGLE per
+ * patch.  The symmetry and periodic patches leave the region (their faces join
+ * actual and virtual cells); the outer faces of the virtual layers form a last patch
+ * (patch_map -1, patch_bc 60: its fluxes reach virtual cells only, and it is not a
+ * wall of the stencils' wall-face lists).  The grid must have cells of one kind.
+ * Extra arrays: "virt_src"(Nv) the actual cell each virtual cell mirrors (0-based),
+ * "virt_xf"(Nv) its transform, "virt_patch"(Nv) the grid patch whose copy made it, "xforms"(9,nT) the transforms (row-major 3x3: momentum
+ * and gradients of a virtual cell are those of its actual cell turned by it), and
+ * "patch_map"(P') the grid patch of each region patch.  The virtual cells are the last
+ * Nv of nCells (rflu_mesh_sizes out[0] counts them).
+ */
+int rflu_mesh_build_sype(const rflu_grid* g, const int32_t* bc_type, const int32_t* bc_opt,
+                         const double* bc_pout, const int32_t* related, const int32_t* axis,
+                         const double* angle, int build_stencils, rflu_mesh** out);
 
 /*
  * Build a box region [x0,x0+lx]x[y0,y0+ly]x[z0,z0+lz] with nx*ny*nz box cells.

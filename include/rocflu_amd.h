@@ -235,6 +235,10 @@ int rflu_gpu_residual(int32_t handle, double* rhs);
  * per stage, rflu_gpu_cell_gradients of each, RFLU_XCHG_GRAD (order 2), then
  * rflu_gpu_convective_fluxes, rflu_gpu_rk_update, rflu_gpu_set_dependent_vars and
  * rflu_gpu_after_update of each, then RFLU_XCHG_STATE.
+ *
+ * A border may pair a region with itself (nbr_handle[b] = handle, nbr_border[b] = b):
+ * the virtual cells of symmetry and periodic patches (rflu_mesh_build_sype) take the
+ * values of the actual cells they mirror that way, in the same exchanges.
  */
 #define RFLU_XCHG_STATE 1
 #define RFLU_XCHG_GRAD 2
@@ -243,6 +247,12 @@ int rflu_gpu_set_borders(int32_t handle, int32_t n_borders, const int32_t* nbr_h
                          const int32_t* send_cells, const int32_t* recv_ptr,
                          const int32_t* recv_cells);
 int rflu_gpu_exchange(int32_t n_regions, const int32_t* handles, int32_t what);
+/* Values received through border `border` (index_base applies) are turned by m
+ * (row-major 3x3; NULL: copied as they are): the momentum vector m (rho u), the density
+ * and pressure gradients m g, the velocity gradient m (G m^T) -- the reflection of a
+ * symmetry patch or the turn of a periodic one (RFLU_RELP_TransformWrapper and
+ * RFLU_RELP_TransformGradWrapper, modflu/RFLU_ModRelatedPatches.F90:380-435,673-735). */
+int rflu_gpu_set_border_transform(int32_t handle, int32_t border, const double* m);
 
 /* RFLU_TimeStepping's loop body (rocflu/RFLU_TimeStepping.F90:353-640) nsteps times:
  * time step, all RungeKuttaMP stages with the update, dependent variables and checks

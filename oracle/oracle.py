@@ -37,6 +37,8 @@ class OrcRegion(ctypes.Structure):
         ("creconst_weight", ctypes.c_double), ("creconst", ctypes.c_int32),
         ("pad_", ctypes.c_int32),
         ("patch_opt2", _i32p),
+        ("n_act", ctypes.c_int32), ("pad2_", ctypes.c_int32),
+        ("virt_src", _i32p), ("virt_xf", _i32p), ("xforms", _f64p),
     ]
 
 
@@ -114,7 +116,8 @@ class Oracle:
     def __init__(self, mesh, num):
         from rocflu_amd.solver import region_arrays
         self.mesh, self.num = mesh, num
-        self.n = int(mesh.n_cells)
+        self.n_act = int(mesh.n_cells)
+        self.n = self.n_act + int(getattr(mesh, "n_virtual", 0))
         k = region_arrays(mesh, num)
         self._keep = k
         r = OrcRegion()
@@ -137,6 +140,13 @@ class Oracle:
         r.discr, r.order, r.reconst, r.rk = num.discr, num.order, num.reconst, num.rk
         r.cfl, r.dt_imposed, r.dt_min_limit = num.cfl, num.dt_imposed, num.dt_min_limit
         r.eps_entr, r.diss_fact = num.eps_entr, num.diss_fact
+        if self.n > self.n_act:
+            for name in ("virt_src", "virt_xf"):
+                k[name] = np.ascontiguousarray(mesh.a[name], dtype=np.int32)
+                setattr(r, name, _p(k[name], i32))
+            k["xforms"] = np.ascontiguousarray(mesh.a["xforms"], dtype=np.float64).reshape(-1)
+            r.xforms = _p(k["xforms"], f64)
+            r.n_act = self.n_act
         self.r = r
         self.L = lib()
 
@@ -177,20 +187,35 @@ class Oracle:
                                       _p(dv, ctypes.c_double), 0, self.n)
         return dv
 
+    def _full(self, cv, dv):
+        """State arrays over every cell: the actual cells' rows, then the virtual cells'
+        (filled by the oracle from the cells they mirror)."""
+        cv = np.asarray(cv, dtype=np.float64)
+        full = np.zeros((self.n, 5))
+        full[:cv.shape[0]] = cv
+        if dv is None:
+            d = np.zeros((self.n, 3))
+            self.L.orc_set_dependent_vars(ctypes.byref(self.r), _p(full, ctypes.c_double),
+                                          _p(d, ctypes.c_double), 0, self.n_act)
+        else:
+            d = np.zeros((self.n, 3))
+            d[:np.asarray(dv).shape[0]] = dv
+        return np.ascontiguousarray(full), np.ascontiguousarray(d)
+
     def run(self, cv, nsteps, time=0.0, dv=None):
-        cv = np.array(cv, dtype=np.float64, copy=True, order="C")
-        dv = self.dv(cv) if dv is None else np.array(dv, dtype=np.float64, copy=True, order="C")
+        """nsteps of the time loop from cv (the actual cells; with virtual cells the
+        result keeps the actual cells' rows)."""
+        cv, dv = self._full(cv, dv)
         st = OrcStats()
         st.time = time
         rc = self.L.orc_run(ctypes.byref(self.r), _p(cv, ctypes.c_double),
                             _p(dv, ctypes.c_double), int(nsteps), ctypes.byref(st))
         stats = {name: getattr(st, name) for name, _ in OrcStats._fields_}
         stats["rc"] = rc
-        return cv, dv, stats
+        return cv[:self.n_act], dv[:self.n_act], stats
 
     def run_until(self, cv, end_time, max_steps=10**9, time=0.0, dv=None):
-        cv = np.array(cv, dtype=np.float64, copy=True, order="C")
-        dv = self.dv(cv) if dv is None else np.array(dv, dtype=np.float64, copy=True, order="C")
+        cv, dv = self._full(cv, dv)
         st = OrcStats()
         st.time = time
         rc = self.L.orc_run_until(ctypes.byref(self.r), _p(cv, ctypes.c_double),
@@ -198,7 +223,7 @@ class Oracle:
                                   ctypes.byref(st))
         stats = {name: getattr(st, name) for name, _ in OrcStats._fields_}
         stats["rc"] = rc
-        return cv, dv, stats
+        return cv[:self.n_act], dv[:self.n_act], stats
 
     def residual(self, cv, dv=None):
         cv = np.ascontiguousarray(cv, dtype=np.float64)

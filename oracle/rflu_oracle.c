@@ -1420,6 +1420,49 @@ void orc_cell_gradients(const orc_region* R, double* cv, const double* dv, doubl
   }
 }
 
+static int n_actual(const orc_region* R) { return R->n_act > 0 ? R->n_act : R->n_cells; }
+
+/* Symmetry/periodic virtual cells: the copy of RungeKuttaMP.F90:440-459 (state, then
+ * RFLU_SetVarsContWrapper over the virtual cells) and :229-236 (final gradients), each
+ * turned as modflu/RFLU_ModRelatedPatches.F90:380-435 and 673-735 turn them: momentum
+ * m (rho u), density and pressure gradients m g, velocity gradient m (G m^T). */
+void orc_fill_virtual_state(const orc_region* R, double* cv, double* dv) {
+  const int na = n_actual(R);
+  if (na == R->n_cells) return;
+  for (int c = na; c < R->n_cells; ++c) {
+    const int s = R->virt_src[c - na];
+    const double* m = R->xforms + 9 * (size_t)R->virt_xf[c - na];
+    const double* x = cv + 5 * (size_t)s;
+    double* y = cv + 5 * (size_t)c;
+    y[0] = x[0];
+    for (int i = 0; i < 3; ++i) y[1 + i] = m[3 * i] * x[1] + m[3 * i + 1] * x[2] + m[3 * i + 2] * x[3];
+    y[4] = x[4];
+  }
+  orc_set_dependent_vars(R, cv, dv, na, R->n_cells);
+}
+
+void orc_fill_virtual_grad(const orc_region* R, double* grad) {
+  const int na = n_actual(R);
+  if (na == R->n_cells) return;
+  for (int c = na; c < R->n_cells; ++c) {
+    const int s = R->virt_src[c - na];
+    const double* m = R->xforms + 9 * (size_t)R->virt_xf[c - na];
+    const double* x = grad + 15 * (size_t)s;
+    double* y = grad + 15 * (size_t)c;
+    for (int v = 0; v < 5; v += 4)
+      for (int i = 0; i < 3; ++i)
+        y[3 * v + i] = m[3 * i] * x[3 * v] + m[3 * i + 1] * x[3 * v + 1] + m[3 * i + 2] * x[3 * v + 2];
+    double G[9], K[9];
+    for (int k = 0; k < 9; ++k) G[k] = x[3 + k];
+    for (int i = 0; i < 3; ++i)
+      for (int l = 0; l < 3; ++l)
+        K[3 * i + l] = G[3 * i] * m[3 * l] + G[3 * i + 1] * m[3 * l + 1] + G[3 * i + 2] * m[3 * l + 2];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        y[3 + 3 * i + j] = m[3 * i] * K[j] + m[3 * i + 1] * K[3 + j] + m[3 * i + 2] * K[6 + j];
+  }
+}
+
 int orc_nstages(int rk) { return rk == 1 ? 4 : rk == 2 ? 3 : rk == 3 ? 1 : rk == 4 ? 3 : 0; }
 
 /* RkUpdateGeneric, libfloflu/RkUpdateGeneric.F90:255-325 (static grid); ark/grk from
@@ -1500,7 +1543,7 @@ void orc_rk_update(const orc_region* R, int stage, double dt, double* cv, const 
  * (libflu/RFLU_CheckValidity.F90:141-222), EnforceBounds (libflu/RFLU_EnforceBounds.F90:
  * 169-231) and CheckPositivity (libflu/RFLU_CheckPositivity.F90:147-253). */
 int orc_after_update(const orc_region* R, double* cv, double* dv, orc_stats* st) {
-  size_t N = (size_t)R->n_cells;
+  size_t N = (size_t)n_actual(R);   /* AfterUpdateMP covers the actual cells */
   double mm, rgas, gamma;
   gas_prime(R, &mm, &rgas, &gamma);
   int any_nan = 0;
@@ -1598,13 +1641,14 @@ static double bface_srad(const orc_region* R, const double* cv, const double* dv
 /* orc_local_dt with threads: per-face spectral radii, then each cell sums its faces'
  * in the scatter's order. */
 static double local_dt_gather(const orc_region* R, const double* cv, const double* dv) {
-  const int F = R->n_faces, N = R->n_cells;
+  const int F = R->n_faces;
   double* sr = (double*)malloc((size_t)F * sizeof(double));
   ORC_PAR
   for (int f = 0; f < F; ++f) sr[f] = face_srad(R, cv, dv, f);
   double mn = HUGE_VAL;
+  const int na = n_actual(R);   /* RFLU_MinimumTimeStep: the actual cells */
 #pragma omp parallel for schedule(static) num_threads(g_threads) reduction(min:mn)
-  for (int c = 0; c < N; ++c) {
+  for (int c = 0; c < na; ++c) {
     int key[16], sign[16];
     int n = cell_faces_sorted(R, c, key, sign);
     double s = 0.0;
@@ -1643,7 +1687,7 @@ double orc_local_dt(const orc_region* R, const double* cv, const double* dv) {
       dt[c1] = dt[c1] + bface_srad(R, cv, dv, b);
     }
   double mn = HUGE_VAL;
-  for (size_t i = 0; i < N; ++i) {
+  for (size_t i = 0; i < (size_t)n_actual(R); ++i) {
     dt[i] = R->vol[i] / dt[i];
     if (dt[i] < mn) mn = dt[i];
   }
@@ -1681,14 +1725,20 @@ static int orc_stage(const orc_region* R, int stage, double dt, double* cv, doub
   } else {
     memcpy(cvOld, cv, 5 * N * sizeof(double));
   }
-  if (R->order == 2) orc_cell_gradients(R, cv, dv, grad);
+  if (R->order == 2) {
+    orc_cell_gradients(R, cv, dv, grad);
+    orc_fill_virtual_grad(R, grad);
+  }
   orc_fluxes(R, cv, dv, grad, rhs, st, stage);
   orc_rk_update(R, stage, dt, cv, cvOld, rhs, rhsSum);
   orc_set_dependent_vars(R, cv, dv, 0, R->n_cells);
-  return orc_after_update(R, cv, dv, st);
+  int rc = orc_after_update(R, cv, dv, st);
+  if (rc == ORC_OK) orc_fill_virtual_state(R, cv, dv);
+  return rc;
 }
 
 int orc_run(const orc_region* R, double* cv, double* dv, int64_t nsteps, orc_stats* st) {
+  orc_fill_virtual_state(R, cv, dv);
   size_t N = (size_t)R->n_cells;
   int ns = orc_nstages(R->rk);
   if (ns == 0 || (R->order != 1 && R->order != 2)) return ORC_ERR_ARG;
@@ -1715,6 +1765,7 @@ int orc_run(const orc_region* R, double* cv, double* dv, int64_t nsteps, orc_sta
  * the time advance. */
 int orc_run_until(const orc_region* R, double* cv, double* dv, double end_time,
                   int64_t max_steps, orc_stats* st) {
+  orc_fill_virtual_state(R, cv, dv);
   size_t N = (size_t)R->n_cells;
   int ns = orc_nstages(R->rk);
   if (ns == 0 || (R->order != 1 && R->order != 2)) return ORC_ERR_ARG;

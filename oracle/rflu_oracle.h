@@ -59,6 +59,14 @@ typedef struct {
   int32_t creconst;          /* 0 off, 1 on */
   int32_t pad_;
   const int32_t* patch_opt2; /* (P) BCSWI_INFLOW_FIXED of inflow 10 patches; NULL = all 0 */
+  /* symmetry/periodic virtual cells [n_act, n_cells): each takes the state and the final
+   * gradients of the actual cell virt_src[k] turned by xforms[9*virt_xf[k]] (row-major);
+   * n_act = 0: every cell is actual */
+  int32_t n_act;
+  int32_t pad2_;
+  const int32_t* virt_src;
+  const int32_t* virt_xf;
+  const double* xforms;
 } orc_region;
 
 typedef struct {
@@ -109,6 +117,12 @@ int orc_run(const orc_region* r, double* cv, double* dv, int64_t nsteps, orc_sta
 /* Time loop to end_time with RFLU_TimeStepping's end-time clamp (at most max_steps). */
 int orc_run_until(const orc_region* r, double* cv, double* dv, double end_time,
                   int64_t max_steps, orc_stats* st);
+/* The virtual cells' state (cv, then dv) and final gradients from the actual cells they
+ * mirror (RFLU_MPI_CopyWrapper / RFLU_MPI_ExchangeGradCells with
+ * RFLU_RELP_TransformWrapper / RFLU_RELP_TransformGradWrapper); no-ops without virtual
+ * cells. */
+void orc_fill_virtual_state(const orc_region* r, double* cv, double* dv);
+void orc_fill_virtual_grad(const orc_region* r, double* grad);
 /* One residual evaluation R(cv) for the current state (gradients included for order 2). */
 void orc_residual(const orc_region* r, const double* cv, const double* dv, double* rhs);
 

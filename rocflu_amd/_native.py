@@ -88,6 +88,7 @@ SIGNATURES = {
     "rflu_gpu_set_borders": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _i32p, _i32p, _i32p,
                                             _i32p, _i32p, _i32p]),
     "rflu_gpu_exchange": (ctypes.c_int, [ctypes.c_int32, _i32p, ctypes.c_int32]),
+    "rflu_gpu_set_border_transform": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _f64p]),
     "rflu_gpu_get_stats": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(GpuStats)]),
     "rflu_gpu_get_case_stats": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(GpuStats)]),
     "rflu_gpu_cell_order": (ctypes.c_int, [ctypes.c_int32, _i32p]),

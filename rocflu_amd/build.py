@@ -51,7 +51,7 @@ def build_hip():
 
 
 def build_mesh():
-    srcs = [os.path.join(HERE, "csrc", f) for f in ("mesh.cpp", "grid_io.cpp", "inputs.cpp")]
+    srcs = [os.path.join(HERE, "csrc", f) for f in ("mesh.cpp", "grid_io.cpp", "inputs.cpp", "sype.cpp")]
     out = os.path.join(HERE, "librflu_mesh.so")
     tmp = out + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall", "-Wextra",

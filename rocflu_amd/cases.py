@@ -57,7 +57,7 @@ def planar_shock(mesh: Mesh, num: Numerics, x_shock: float, mach: float = 2.0,
     relative density perturbation of amplitude `noise` everywhere."""
     gp = num.gas_prime()[2]
     rho2, u2, p2 = normal_shock(mach, gp, rho1, p1)
-    x = mesh.cofg[:, 0]
+    x = mesh.cofg[:mesh.n_cells, 0]
     behind = x < x_shock
     rng = np.random.default_rng(seed)
     pert = 1.0 + noise * (2.0 * rng.random(mesh.n_cells) - 1.0)

@@ -38,10 +38,12 @@ enum KernelId { KID_FACE = 0, KID_UPDATE = 1, KID_GRAD = 2, KID_BLEND = 3, KID_D
 // One border of a region (t_border): the actual cells it sends to a neighbouring
 // region and the virtual cells it receives from it, as internal cell numbers.
 struct Border {
-  int nbr = 0;          // neighbour's handle
+  int nbr = 0;          // neighbour's handle (the region itself for symmetry/periodic)
   int nbr_border = 0;   // the neighbour's border that matches this one
   int nsend = 0, nrecv = 0;
   int *send = nullptr, *recv = nullptr;
+  bool turn = false;    // data received through this border is turned by xf
+  Xform xf;
 };
 
 struct Region {
@@ -1076,8 +1078,6 @@ int check_desc(const rflu_region_desc* d, const rflu_region_desc* ref) {
   const int base = d->index_base;
   if (N <= 0 || F < 0 || P < 0 || d->n_virtual < 0) return fail(RFLU_ERR_ARG, "bad region sizes");
   if ((long long)N + d->n_virtual > 0x7fffffff) return fail(RFLU_ERR_ARG, "cell count must fit int32");
-  if (d->n_virtual > 0 && d->order == 2 && d->creconst)
-    return fail(RFLU_ERR_ARG, "regions with virtual cells do not take constrained gradients");
   if (base != 0 && base != 1) return fail(RFLU_ERR_ARG, "index_base must be 0 or 1");
   if (!d->f2c || !d->fn || !d->cofg || !d->vol || (P > 0 && (!d->bpatch_ptr || !d->patch_bc)))
     return fail(RFLU_ERR_ARG, "missing grid arrays");
@@ -1693,7 +1693,7 @@ int rflu_gpu_set_borders(int32_t h, int32_t n_borders, const int32_t* nbr_handle
     B.nbr_border = nbr_border[b] - R->base;
     B.nsend = send_ptr[b + 1] - send_ptr[b];
     B.nrecv = recv_ptr[b + 1] - recv_ptr[b];
-    if (B.nbr == h || !get(B.nbr) || B.nbr_border < 0 || B.nsend < 0 || B.nrecv < 0 ||
+    if (!get(B.nbr) || B.nbr_border < 0 || B.nsend < 0 || B.nrecv < 0 ||
         (B.nsend > 0 && !send_cells) || (B.nrecv > 0 && !recv_cells)) {
       drop();
       return fail(RFLU_ERR_ARG, "border " + std::to_string(b) + ": bad neighbour or lists");
@@ -1722,6 +1722,23 @@ int rflu_gpu_set_borders(int32_t h, int32_t n_borders, const int32_t* nbr_handle
     if (b.recv) (void)hipFree(b.recv);
   }
   R->borders.swap(nb);
+  return RFLU_OK;
+}
+
+int rflu_gpu_set_border_transform(int32_t h, int32_t border, const double* m) {
+  Region* R = get(h);
+  if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
+  const int b = border - R->base;
+  if (b < 0 || b >= (int)R->borders.size()) return fail(RFLU_ERR_ARG, "no such border");
+  Border& B = R->borders[b];
+  if (!m) {
+    B.turn = false;
+    return RFLU_OK;
+  }
+  for (int k = 0; k < 9; ++k)
+    if (!std::isfinite(m[k])) return fail(RFLU_ERR_ARG, "transform entries must be finite");
+  for (int k = 0; k < 9; ++k) B.xf.m[k] = m[k];
+  B.turn = true;
   return RFLU_OK;
 }
 
@@ -1757,7 +1774,20 @@ int rflu_gpu_exchange(int32_t n_regions, const int32_t* handles, int32_t what) {
       Region* D = get(b.nbr);
       const int* didx = D->borders[b.nbr_border].recv;
       const dim3 g(nblk(b.nsend, 256)), bs(256);
-      if (what == RFLU_XCHG_STATE)
+      const Border& rb = D->borders[b.nbr_border];
+      if (rb.turn) {
+        // symmetry and periodic virtual cells: momentum and gradients turned
+        // (RFLU_RELP_TransformWrapper)
+        if (what == RFLU_XCHG_STATE)
+          hipLaunchKernelGGL((k_halo_turn<double>), g, bs, 0, R->stream, (const double*)R->cv, R->N,
+                             D->cv, D->N, 5, (const int*)b.send, didx, b.nsend, rb.xf);
+        else if (grad32(R))
+          hipLaunchKernelGGL((k_halo_turn<float>), g, bs, 0, R->stream, (const float*)R->grad_final(), R->N,
+                             (float*)D->grad_final(), D->N, 15, (const int*)b.send, didx, b.nsend, rb.xf);
+        else
+          hipLaunchKernelGGL((k_halo_turn<double>), g, bs, 0, R->stream, (const double*)R->grad_final(), R->N,
+                             (double*)D->grad_final(), D->N, 15, (const int*)b.send, didx, b.nsend, rb.xf);
+      } else if (what == RFLU_XCHG_STATE)
         hipLaunchKernelGGL((k_halo_copy<double>), g, bs, 0, R->stream, (const double*)R->cv, R->N,
                            D->cv, D->N, 5, (const int*)b.send, didx, b.nsend);
       else if (grad32(R))

@@ -1396,6 +1396,49 @@ __global__ void k_halo_copy(const T* __restrict__ src, int ns, T* __restrict__ d
   for (int k = 0; k < ncomp; ++k) dst[(size_t)k * nd + b] = src[(size_t)k * ns + a];
 }
 
+// A 3x3 transform, row-major (the turn of a symmetry or periodic copy).
+struct Xform {
+  double m[9];
+};
+
+// The halo copy of symmetry/periodic virtual cells: state (ncomp 5: the momentum
+// vector turned, y_i = sum_k m_ik x_k) or gradients (ncomp 15, component 3v+d: the
+// density and pressure gradients turned as vectors, the velocity gradient G_ij =
+// d u_i / d x_j as M (G M^T), sums in index order) (RFLU_RELP_TransformWrapper and
+// RFLU_RELP_TransformGradWrapper, modflu/RFLU_ModRelatedPatches.F90:380-435, 673-735).
+template <typename T>
+__global__ void k_halo_turn(const T* __restrict__ src, int ns, T* __restrict__ dst, int nd,
+                            int ncomp, const int* __restrict__ sidx,
+                            const int* __restrict__ didx, int n, Xform X) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int a = sidx[e], b = didx[e];
+  const double* m = X.m;
+  auto in = [&](int k) { return (double)src[(size_t)k * ns + a]; };
+  auto out = [&](int k, double v) { dst[(size_t)k * nd + b] = (T)v; };
+  if (ncomp == 5) {
+    out(0, in(0));
+    const double x0 = in(1), x1 = in(2), x2 = in(3);
+    for (int i = 0; i < 3; ++i) out(1 + i, m[3 * i] * x0 + m[3 * i + 1] * x1 + m[3 * i + 2] * x2);
+    out(4, in(4));
+    return;
+  }
+  for (int v = 0; v < 5; v += 4) {
+    const double x0 = in(3 * v), x1 = in(3 * v + 1), x2 = in(3 * v + 2);
+    for (int i = 0; i < 3; ++i)
+      out(3 * v + i, m[3 * i] * x0 + m[3 * i + 1] * x1 + m[3 * i + 2] * x2);
+  }
+  double G[9], K[9];
+  for (int k = 0; k < 9; ++k) G[k] = in(3 + k);   // rows u, v, w; columns x, y, z
+  // M (G M^T), the reference's order
+  for (int i = 0; i < 3; ++i)
+    for (int l = 0; l < 3; ++l)
+      K[3 * i + l] = G[3 * i] * m[3 * l] + G[3 * i + 1] * m[3 * l + 1] + G[3 * i + 2] * m[3 * l + 2];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      out(3 + 3 * i + j, m[3 * i] * K[j] + m[3 * i + 1] * K[3 + j] + m[3 * i + 2] * K[6 + j]);
+}
+
 // After-update checks as a separate pass (the unfused entry points).
 template <bool WRITE_PRIM>
 __global__ void k_after_update(Mesh M, State S, Gas G, Cases C, Scalars* sc) {

@@ -80,17 +80,8 @@ void face_geom(const rflu_mesh& m, const int32_t* fv, double* vec, double* cen) 
   }
 }
 
-// Local faces of the four cell kinds, 0-based, in the reference's order and vertex
-// cycles (f2vTet, f2vHex, f2vPri, f2vPyr of modflu/RFLU_ModGrid.F90:92-102): a cycle's
-// right-hand normal (libfloflu/FaceVector.F90) points out of the cell.  -1 ends a
-// triangle.
-const int LF_TET[4][4] = {{0, 1, 2, -1}, {1, 3, 2, -1}, {0, 2, 3, -1}, {0, 3, 1, -1}};
-const int LF_HEX[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5},
-                          {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}};
-const int LF_PRI[5][4] = {{0, 2, 1, -1}, {0, 1, 4, 3}, {1, 2, 5, 4}, {0, 3, 5, 2}, {3, 4, 5, -1}};
-const int LF_PYR[5][4] = {{0, 3, 2, 1}, {0, 1, 4, -1}, {1, 2, 4, -1}, {2, 3, 4, -1}, {0, 4, 3, -1}};
-struct CellKind { int nv, nf; const int (*lf)[4]; };
-const CellKind KINDS[4] = {{4, 4, LF_TET}, {8, 6, LF_HEX}, {6, 5, LF_PRI}, {5, 5, LF_PYR}};
+using rflu::CellKind;
+using rflu::KINDS;
 
 struct FaceKey {
   int32_t v[4];
@@ -589,11 +580,12 @@ int rflu::build_region(rflu_mesh& m, const rflu_grid& g, const int32_t* bc_type,
   for (int p = 0; p < P; ++p)
     if (bc_type[p] != RFLU_BC_OUTFLOW && bc_type[p] != RFLU_BC_SLIPWALL &&
         bc_type[p] != RFLU_BC_INFLOW_VELTEMP && bc_type[p] != RFLU_BC_INFLOW_TOTANG &&
-        bc_type[p] != RFLU_BC_FARFIELD && bc_type[p] != RFLU_BC_INJECTION) {
+        bc_type[p] != RFLU_BC_FARFIELD && bc_type[p] != RFLU_BC_INJECTION &&
+        bc_type[p] != RFLU_BC_VIRTUAL) {
       g_err = "patch " + std::to_string(p + 1) + " has boundary type " +
               std::to_string(bc_type[p]) +
               "; built types: inflow (10, 11), outflow (20), slip wall (60), farfield (80), "
-              "injection (90)";
+              "injection (90); symmetry (100) and periodic (110) through rflu_mesh_build_sype";
       return 4;
     }
   for (const std::vector<int32_t>* l : lists)
@@ -902,6 +894,11 @@ int rflu_mesh_array(const rflu_mesh* m, const char* name, const void** ptr, int6
   if (n == "vol") return D(m->vol);
   if (n == "cofg") return D(m->cofg);
   if (n == "xyz_moms") return D(m->xyz_moms);
+  if (n == "virt_src") return I(m->virt_src);
+  if (n == "virt_xf") return I(m->virt_xf);
+  if (n == "virt_patch") return I(m->virt_patch);
+  if (n == "patch_map") return I(m->patch_map);
+  if (n == "xforms") return D(m->xforms);
   g_err = "unknown array name: " + n;
   return 2;
 }

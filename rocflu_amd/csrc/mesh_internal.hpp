@@ -42,9 +42,30 @@ struct rflu_mesh {
   std::vector<double> xyz_moms;
   std::vector<int32_t> c2f_ptr, c2f_idx;
   int stencil_min = 0, stencil_max = 0;
+  // symmetry/periodic virtual cells [n_cells - virt_src.size(), n_cells): the actual
+  // cell each mirrors, the index of the transform of its data (xforms, 9 per transform,
+  // row-major), the grid patch whose copy made it, and per region patch the grid patch
+  // it came from (-1: the outer faces of the virtual layers)
+  std::vector<int32_t> virt_src, virt_xf, virt_patch, patch_map;
+  std::vector<double> xforms;
 };
 
 namespace rflu {
+
+// Local faces of the four cell kinds, 0-based, in the reference's order and vertex
+// cycles (f2vTet, f2vHex, f2vPri, f2vPyr of modflu/RFLU_ModGrid.F90:92-102): a cycle's
+// right-hand normal (libfloflu/FaceVector.F90) points out of the cell.  -1 ends a
+// triangle.
+inline constexpr int LF_TET[4][4] = {{0, 1, 2, -1}, {1, 3, 2, -1}, {0, 2, 3, -1}, {0, 3, 1, -1}};
+inline constexpr int LF_HEX[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5},
+                                     {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}};
+inline constexpr int LF_PRI[5][4] = {{0, 2, 1, -1}, {0, 1, 4, 3}, {1, 2, 5, 4},
+                                     {0, 3, 5, 2}, {3, 4, 5, -1}};
+inline constexpr int LF_PYR[5][4] = {{0, 3, 2, 1}, {0, 1, 4, -1}, {1, 2, 4, -1},
+                                     {2, 3, 4, -1}, {0, 4, 3, -1}};
+struct CellKind { int nv, nf; const int (*lf)[4]; };
+inline constexpr CellKind KINDS[4] = {{4, 4, LF_TET}, {8, 6, LF_HEX}, {6, 5, LF_PRI},
+                                      {5, 5, LF_PYR}};
 
 // Error text of the calling thread's last failed call.
 void set_error(const std::string& msg);

@@ -13,6 +13,9 @@
 !   writes cv(5,nCells) as raw doubles to out.bin and the fluid dump to out.bin.dump.
 !   The initial state is rfluinit's: INITFLOW FLAG 1 (uniform, DENS or TEMP) or
 !   FLAG 3 for casename rectshktb (utilities/init/RFLU_InitFlowHardCode.F90:839-901).
+!   Symmetry and periodic patches become virtual cells (rflu_mesh_build_sype) that
+!   take their actual cells' state and gradients through borders of the region with
+!   itself, exchanged where RungeKuttaMP exchanges them.
 !
 !******************************************************************************
 PROGRAM rflu_case_driver
@@ -22,7 +25,9 @@ PROGRAM rflu_case_driver
   IMPLICIT NONE
 
   CHARACTER(LEN=512) :: dir, casename, arg, outfile, base
-  INTEGER :: maxsteps, istep, istage, nStages, icg, ip, u, nPatches, nCells, nFaces, nBFaces
+  INTEGER :: maxsteps, istep, istage, nStages, icg, ip, u, nPatches, nCells, nFaces, nBFaces, &
+             nVirt, nTot, nRegPatches, nXf, ib, k, nb
+  LOGICAL :: sype
   INTEGER(C_INT) :: rc
   INTEGER(C_INT32_T) :: handle
   INTEGER(C_INT64_T) :: gsz(8), sizes(9), cnt
@@ -36,9 +41,13 @@ PROGRAM rflu_case_driver
   REAL(C_DOUBLE), POINTER :: dp(:)
   INTEGER(C_INT32_T), ALLOCATABLE, TARGET :: f2c(:,:), bf2c(:), bptr(:), pbc(:), popt(:), &
                                              popt2(:), porder(:), sptr(:), sidx(:), &
-                                             bsptr(:), bsidx(:)
+                                             bsptr(:), bsidx(:), prel(:), paxis(:), &
+                                             vsrc(:), vxf(:), pmap(:), rbc(:), ropt(:), &
+                                             ropt2(:), rorder(:), nbrh(:), nbrb(:), &
+                                             sndp(:), sndc(:), rcvp(:), rcvc(:)
   REAL(C_DOUBLE), ALLOCATABLE, TARGET :: fn(:,:), fc(:,:), cofg(:,:), vol(:), bfn(:,:), &
-                                         bfc(:,:), pout(:), moms(:,:), pvals(:,:)
+                                         bfc(:,:), pout(:), moms(:,:), pvals(:,:), pang(:), &
+                                         xfm(:,:), rpout(:), rvals(:,:)
   REAL(C_DOUBLE), ALLOCATABLE, TARGET :: cv(:,:), dv(:,:)
   REAL(C_DOUBLE) :: time, dt, cp, g, gc, mm, d, uu, vv, ww, pr, x, y, z
 
@@ -64,7 +73,8 @@ PROGRAM rflu_case_driver
                     INT(inp%order_bfaces, C_INT), bcs)
   CALL rflu_host_check(rc, 'RFLU_ReadBcInputFile', rflu_input_last_error())
   ALLOCATE(pbc(nPatches), popt(nPatches), popt2(nPatches), porder(nPatches), &
-           pout(nPatches), pvals(5,nPatches))
+           pout(nPatches), pvals(5,nPatches), prel(nPatches), paxis(nPatches), pang(nPatches))
+  sype = .FALSE.
   DO ip = 1,nPatches
     IF ( bcs(ip)%bc_type == 0 ) THEN
       WRITE(*,'(A,I0,A)') 'patch ', ip, ' has no section in the .bc file'
@@ -76,22 +86,60 @@ PROGRAM rflu_case_driver
     porder(ip) = bcs(ip)%order
     pout(ip) = bcs(ip)%pout
     pvals(:,ip) = bcs(ip)%vals
+    prel(ip) = bcs(ip)%related
+    paxis(ip) = bcs(ip)%axis
+    pang(ip) = bcs(ip)%angle
+    IF ( pbc(ip) == 100 .OR. pbc(ip) == 110 ) sype = .TRUE.
   END DO
 
 ! ------------------------------------------------------------------------------
 ! Region (RFLU_InitFlowSolver): face list, geometry, stencils
 ! ------------------------------------------------------------------------------
-  rc = rflu_mesh_build(grid, pbc, popt, pout, MERGE(1_C_INT, 0_C_INT, inp%order == 2), mesh)
+  IF ( sype ) THEN
+    ! symmetry and periodic patches become virtual cells (rflupart's RFLU_SYPE_* step)
+    rc = rflu_mesh_build_sype(grid, pbc, popt, pout, prel, paxis, pang, &
+                              MERGE(1_C_INT, 0_C_INT, inp%order == 2), mesh)
+  ELSE
+    rc = rflu_mesh_build(grid, pbc, popt, pout, MERGE(1_C_INT, 0_C_INT, inp%order == 2), mesh)
+  END IF
   CALL rflu_host_check(rc, 'RFLU_BuildFaceList', rflu_mesh_last_error())
   CALL rflu_grid_destroy(grid)
   IF ( rflu_mesh_sizes(mesh, sizes) /= 0 ) STOP 3
-  nCells = INT(sizes(1)); nFaces = INT(sizes(2)); nBFaces = INT(sizes(3))
-  ALLOCATE(f2c(2,nFaces), fn(4,nFaces), fc(3,nFaces), cofg(3,nCells), vol(nCells))
-  ALLOCATE(bf2c(nBFaces), bfn(4,nBFaces), bfc(3,nBFaces), bptr(nPatches+1))
+  nTot = INT(sizes(1)); nFaces = INT(sizes(2)); nBFaces = INT(sizes(3))
+  nRegPatches = INT(sizes(4))
+  nVirt = 0
+  IF ( sype ) THEN
+    CALL get_int('virt_src', ip32); nVirt = SIZE(ip32)
+    ALLOCATE(vsrc(nVirt)); vsrc = ip32 + 1
+    CALL get_int('virt_xf', ip32)
+    ALLOCATE(vxf(nVirt)); vxf = ip32
+    CALL get_dbl('xforms', dp); nXf = SIZE(dp)/9
+    ALLOCATE(xfm(9,nXf)); xfm = RESHAPE(dp, (/9,nXf/))
+    CALL get_int('patch_map', ip32)
+    ALLOCATE(pmap(nRegPatches)); pmap = ip32
+  END IF
+  nCells = nTot - nVirt
+  ! the region's patches: the grid's, or the kept ones and the virtual layers' outer faces
+  ALLOCATE(rbc(nRegPatches), ropt(nRegPatches), ropt2(nRegPatches), rorder(nRegPatches), &
+           rpout(nRegPatches), rvals(5,nRegPatches))
+  CALL get_int('patch_bc', ip32);  rbc = ip32
+  CALL get_int('patch_opt', ip32); ropt = ip32
+  CALL get_dbl('patch_pout', dp);  rpout = dp
+  DO ip = 1,nRegPatches
+    k = ip
+    IF ( sype ) k = pmap(ip) + 1
+    IF ( k >= 1 ) THEN
+      ropt2(ip) = popt2(k); rorder(ip) = porder(k); rvals(:,ip) = pvals(:,k)
+    ELSE
+      ropt2(ip) = 0; rorder(ip) = 1; rvals(:,ip) = 0.0_C_DOUBLE
+    END IF
+  END DO
+  ALLOCATE(f2c(2,nFaces), fn(4,nFaces), fc(3,nFaces), cofg(3,nTot), vol(nTot))
+  ALLOCATE(bf2c(nBFaces), bfn(4,nBFaces), bfc(3,nBFaces), bptr(nRegPatches+1))
   CALL get_int('f2c', ip32);   f2c  = RESHAPE(ip32, (/2,nFaces/)) + 1
   CALL get_dbl('fn', dp);      fn   = RESHAPE(dp, (/4,nFaces/))
   CALL get_dbl('fc', dp);      fc   = RESHAPE(dp, (/3,nFaces/))
-  CALL get_dbl('cofg', dp);    cofg = RESHAPE(dp, (/3,nCells/))
+  CALL get_dbl('cofg', dp);    cofg = RESHAPE(dp, (/3,nTot/))
   CALL get_dbl('vol', dp);     vol  = dp
   CALL get_int('bf2c', ip32);  bf2c = ip32 + 1
   CALL get_dbl('bfn', dp);     bfn  = RESHAPE(dp, (/4,nBFaces/))
@@ -99,13 +147,13 @@ PROGRAM rflu_case_driver
   CALL get_int('bpatch_ptr', ip32); bptr = ip32
   IF ( inp%order == 2 ) THEN
     CALL get_int('c2cs_ptr', ip32)
-    ALLOCATE(sptr(nCells+1)); sptr = ip32
+    ALLOCATE(sptr(nTot+1)); sptr = ip32
     CALL get_int('c2cs_idx', ip32)
     ALLOCATE(sidx(SIZE(ip32))); sidx = ip32 + 1
     CALL get_dbl('xyz_moms', dp)
-    ALLOCATE(moms(10,nCells)); moms = RESHAPE(dp, (/10,nCells/))
+    ALLOCATE(moms(10,nTot)); moms = RESHAPE(dp, (/10,nTot/))
     CALL get_int('c2cs_bptr', ip32)
-    ALLOCATE(bsptr(nCells+1)); bsptr = ip32
+    ALLOCATE(bsptr(nTot+1)); bsptr = ip32
     CALL get_int('c2cs_bface', ip32)
     ALLOCATE(bsidx(MAX(1,SIZE(ip32)))); bsidx = 0
     IF ( SIZE(ip32) > 0 ) bsidx(1:SIZE(ip32)) = ip32 + 1
@@ -115,14 +163,14 @@ PROGRAM rflu_case_driver
   cp = inp%ref_cp
   g  = inp%ref_gamma
   desc%index_base = 1
-  desc%n_cells = nCells; desc%n_faces = nFaces; desc%n_patches = nPatches
+  desc%n_cells = nCells; desc%n_faces = nFaces; desc%n_patches = nRegPatches
   desc%f2c = C_LOC(f2c); desc%fn = C_LOC(fn); desc%fc = C_LOC(fc)
   desc%cofg = C_LOC(cofg); desc%vol = C_LOC(vol)
   desc%bpatch_ptr = C_LOC(bptr); desc%bf2c = C_LOC(bf2c)
   desc%bfn = C_LOC(bfn); desc%bfc = C_LOC(bfc)
-  desc%patch_bc = C_LOC(pbc); desc%patch_opt = C_LOC(popt)
-  desc%patch_pout = C_LOC(pout); desc%patch_order = C_LOC(porder)
-  desc%patch_vals = C_LOC(pvals); desc%patch_opt2 = C_LOC(popt2)
+  desc%patch_bc = C_LOC(rbc); desc%patch_opt = C_LOC(ropt)
+  desc%patch_pout = C_LOC(rpout); desc%patch_order = C_LOC(rorder)
+  desc%patch_vals = C_LOC(rvals); desc%patch_opt2 = C_LOC(ropt2)
   IF ( inp%order == 2 ) THEN
     desc%c2cs_ptr = C_LOC(sptr); desc%c2cs_idx = C_LOC(sidx); desc%xyz_moms = C_LOC(moms)
     desc%c2cs_bptr = C_LOC(bsptr); desc%c2cs_bface = C_LOC(bsidx)
@@ -137,8 +185,41 @@ PROGRAM rflu_case_driver
   desc%eps_entr = inp%eps_entr; desc%diss_fact = inp%diss_fact
   desc%creconst_weight = inp%creconst_cells_weight
   desc%creconst = inp%creconst_cells; desc%pad_ = 0
-  desc%n_virtual = 0; desc%pad2_ = 0
+  desc%n_virtual = nVirt; desc%pad2_ = 0
   CALL rflu_gpu_check(rflu_gpu_create_region(desc, handle), 'rflu_gpu_create_region')
+
+  ! symmetry/periodic virtual cells: borders of the region with itself, one per
+  ! transform, the identity (transform 0) a plain copy
+  IF ( nVirt > 0 ) THEN
+    nb = 0
+    DO k = 0,nXf-1
+      IF ( COUNT(vxf == k) > 0 ) nb = nb + 1
+    END DO
+    ALLOCATE(nbrh(nb), nbrb(nb), sndp(nb+1), rcvp(nb+1), sndc(nVirt), rcvc(nVirt))
+    sndp(1) = 0; ib = 0
+    DO k = 0,nXf-1
+      IF ( COUNT(vxf == k) == 0 ) CYCLE
+      ib = ib + 1
+      nbrh(ib) = handle; nbrb(ib) = ib
+      sndp(ib+1) = sndp(ib)
+      DO icg = 1,nVirt
+        IF ( vxf(icg) /= k ) CYCLE
+        sndp(ib+1) = sndp(ib+1) + 1
+        sndc(sndp(ib+1)) = vsrc(icg)
+        rcvc(sndp(ib+1)) = nCells + icg
+      END DO
+    END DO
+    rcvp = sndp
+    CALL rflu_gpu_check(rflu_gpu_set_borders(handle, nb, nbrh, nbrb, sndp, sndc, rcvp, rcvc), &
+                        'RFLU_SYPE borders')
+    ib = 0
+    DO k = 0,nXf-1
+      IF ( COUNT(vxf == k) == 0 ) CYCLE
+      ib = ib + 1
+      IF ( k > 0 ) CALL rflu_gpu_check(rflu_gpu_set_border_transform(handle, ib, xfm(:,k+1)), &
+                                       'RFLU_RELP transform')
+    END DO
+  END IF
 
 ! ------------------------------------------------------------------------------
 ! Initial state (rfluinit)
@@ -147,7 +228,8 @@ PROGRAM rflu_case_driver
   mm = 8314.3_C_DOUBLE/gc
   gc = 8314.3_C_DOUBLE/mm
   g  = cp/(cp - gc)
-  ALLOCATE(cv(5,nCells), dv(3,nCells))
+  ALLOCATE(cv(5,nTot), dv(3,nTot))
+  cv = 0.0_C_DOUBLE
   DO icg = 1,nCells
     x = cofg(1,icg); y = cofg(2,icg); z = cofg(3,icg)
     IF ( inp%init_flag == 1 ) THEN
@@ -177,7 +259,10 @@ PROGRAM rflu_case_driver
     cv(4,icg) = d*ww
     cv(5,icg) = d*(pr/(d*(g - 1.0_C_DOUBLE)) + 0.5_C_DOUBLE*(uu*uu + vv*vv + ww*ww))
   END DO
+  IF ( nVirt > 0 ) cv(:,nCells+1:nTot) = cv(:,vsrc)
   CALL rflu_gpu_check(rflu_gpu_upload_state(handle, cv, C_NULL_PTR, 0.0_C_DOUBLE), 'upload')
+  IF ( nVirt > 0 ) CALL rflu_gpu_check(rflu_gpu_exchange(1, (/handle/), RFLU_XCHG_STATE), &
+                                       'RFLU_MPI_CopyWrapper')
 
 ! ------------------------------------------------------------------------------
 ! Time loop (RFLU_TimeStepping + RungeKuttaMP with the GPU callees)
@@ -200,10 +285,15 @@ PROGRAM rflu_case_driver
     DO istage = 1,nStages
       CALL rflu_gpu_check(rflu_gpu_rk_init(handle, istage), 'RKInitMP')
       CALL rflu_gpu_check(rflu_gpu_cell_gradients(handle), 'CellGradientsMP')
+      IF ( nVirt > 0 .AND. inp%order == 2 ) &
+        CALL rflu_gpu_check(rflu_gpu_exchange(1, (/handle/), RFLU_XCHG_GRAD), &
+                            'RFLU_MPI_ExchangeGradCells')
       CALL rflu_gpu_check(rflu_gpu_convective_fluxes(handle, istage), 'ConvectiveFluxesMP')
       CALL rflu_gpu_check(rflu_gpu_rk_update(handle, istage, dt), 'RKUpdateMP')
       CALL rflu_gpu_check(rflu_gpu_set_dependent_vars(handle), 'RFLU_SetVarsContWrapper')
       CALL rflu_gpu_check(rflu_gpu_after_update(handle, istage), 'AfterUpdateMP')
+      IF ( nVirt > 0 ) CALL rflu_gpu_check(rflu_gpu_exchange(1, (/handle/), RFLU_XCHG_STATE), &
+                                           'RFLU_MPI_CopyWrapper')
     END DO
     time = time + dt
     istep = istep + 1
@@ -216,10 +306,10 @@ PROGRAM rflu_case_driver
   CALL rflu_gpu_check(rflu_gpu_download_state(handle, cv, dv), 'download')
   CALL rflu_gpu_check(rflu_gpu_destroy_region(handle), 'destroy')
   OPEN(NEWUNIT=u, FILE=TRIM(outfile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
-  WRITE(u) cv
+  WRITE(u) cv(:,1:nCells)
   CLOSE(u)
   WRITE(*,'(A,A,A,I0,A,I0,A,ES23.16,A,ES23.16)') 'rflu_case_driver: ', TRIM(casename), ': ', &
-        nCells, ' cells, ', istep, ' steps, time ', time, ', sum(rho) ', SUM(cv(1,:))
+        nCells, ' cells, ', istep, ' steps, time ', time, ', sum(rho) ', SUM(cv(1,1:nCells))
 
 CONTAINS
 

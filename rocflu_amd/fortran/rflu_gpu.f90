@@ -190,6 +190,13 @@ MODULE rflu_gpu
       INTEGER(C_INT32_T), INTENT(IN) :: handles(*)
     END FUNCTION
 
+    INTEGER(C_INT) FUNCTION rflu_gpu_set_border_transform(handle, border, m) &
+        BIND(C, NAME='rflu_gpu_set_border_transform')
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+      INTEGER(C_INT32_T), VALUE :: handle, border
+      REAL(C_DOUBLE), INTENT(IN) :: m(9)
+    END FUNCTION
+
     INTEGER(C_INT) FUNCTION rflu_gpu_time_steps(handle, nsteps, use_graph) &
         BIND(C, NAME='rflu_gpu_time_steps')
       IMPORT :: C_INT, C_INT32_T, C_INT64_T
@@ -302,6 +309,17 @@ MODULE rflu_gpu
       IMPORT :: C_INT, C_PTR, C_INT64_T
       TYPE(C_PTR), VALUE :: g
       INTEGER(C_INT64_T), INTENT(OUT) :: out(8)
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION rflu_mesh_build_sype(g, bc_type, bc_opt, bc_pout, related, axis, &
+                                                 angle, build_stencils, out) &
+        BIND(C, NAME='rflu_mesh_build_sype')
+      IMPORT :: C_INT, C_PTR, C_INT32_T, C_DOUBLE
+      TYPE(C_PTR), VALUE :: g
+      INTEGER(C_INT32_T), INTENT(IN) :: bc_type(*), bc_opt(*), related(*), axis(*)
+      REAL(C_DOUBLE), INTENT(IN) :: bc_pout(*), angle(*)
+      INTEGER(C_INT), VALUE :: build_stencils
+      TYPE(C_PTR), INTENT(OUT) :: out
     END FUNCTION
 
     INTEGER(C_INT) FUNCTION rflu_mesh_build(g, bc_type, bc_opt, bc_pout, build_stencils, out) &

@@ -139,9 +139,32 @@ def load_case(casename: str, directory: str = ".", grid_path: str | None = None,
             raise InputError(f"patch {i + 1} has no section in {base}.bc")
     if stencils is None:
         stencils = inp.order == 2
-    mesh = grid.build([b.bc_type for b in bcs], [b.opt for b in bcs],
-                      [b.pout for b in bcs], stencils=stencils)
-    return Case(mesh=mesh, num=numerics(inp, bcs), inp=inp, bcs=bcs, grid=grid)
+    types = [b.bc_type for b in bcs]
+    if any(t in (BCT_SYMMETRY, BCT_PERIODIC) for t in types):
+        # symmetry/periodic patches become virtual cells; the region's patches are the
+        # kept ones (patch_map) and the virtual layers' outer faces (-1)
+        mesh = grid.build_sype(types, [b.opt for b in bcs], [b.pout for b in bcs],
+                               [b.related for b in bcs], [b.axis for b in bcs],
+                               [b.angle for b in bcs], stencils=stencils)
+        kept = [bcs[p] if p >= 0 else None for p in mesh.a["patch_map"]]
+        num = numerics(inp, bcs)
+        num.patch_order = [b.order if b else 1 for b in kept]
+        num.patch_vals = [[float(v) for v in b.vals] if b else [0.0] * 5 for b in kept]
+        num.patch_opt2 = [int(b.opt2) if b else 0 for b in kept]
+    else:
+        mesh = grid.build(types, [b.opt for b in bcs], [b.pout for b in bcs], stencils=stencils)
+        num = numerics(inp, bcs)
+    return Case(mesh=mesh, num=num, inp=inp, bcs=bcs, grid=grid)
+
+
+def gpu_host(case: Case):
+    """The GPU host of a case: a GpuRegion, or for a region with symmetry/periodic
+    virtual cells the one-region GpuRegions that exchanges them."""
+    from .regions import GpuRegions, sype_part
+    from .solver import GpuRegion
+    if case.mesh.n_virtual:
+        return GpuRegions([sype_part(case.mesh)], case.num)
+    return GpuRegion(case.mesh, case.num)
 
 
 INITFLOW_FROMSCRATCH, INITFLOW_FROMHARDCODE = 1, 3
@@ -158,8 +181,8 @@ def initial_state(case: Case, casename: str):
     gamma (libfloflu/MixtPerf_E.F90:70-81, libflu/RFLU_SetGasVars.F90:172-178)."""
     inp, m = case.inp, case.mesh
     _, rgas, g = case.num.gas_prime()
-    x, y, z = m.cofg[:, 0], m.cofg[:, 1], m.cofg[:, 2]
-    n = m.n_cells
+    n = m.n_cells                      # the actual cells (virtual ones follow them)
+    x, y, z = m.cofg[:n, 0], m.cofg[:n, 1], m.cofg[:n, 2]
     if inp.init_flag == INITFLOW_FROMSCRATCH:
         p = np.full(n, inp.init_press)
         u, v, w = (np.full(n, inp.init_vel[k]) for k in range(3))

@@ -49,6 +49,8 @@ def _lib():
             ctypes.c_double, ctypes.c_uint64, i32p, i32p, f64p, ctypes.c_int,
             ctypes.POINTER(vp)]
         lib.rflu_mesh_build.argtypes = [vp, i32p, i32p, f64p, ctypes.c_int, ctypes.POINTER(vp)]
+        lib.rflu_mesh_build_sype.argtypes = [vp, i32p, i32p, f64p, i32p, i32p, f64p, ctypes.c_int,
+                                             ctypes.POINTER(vp)]
         lib.rflu_grid_box.argtypes = [
             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
             ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -98,6 +100,7 @@ _ARRAYS = {
     "c2cs_bface": 2, "c2f_ptr": 1,
     "c2f_idx": 1, "xyz": 3, "fn": 4, "fc": 3, "bfn": 4, "bfc": 3,
     "patch_pout": 1, "vol": 1, "cofg": 3, "xyz_moms": 10,
+    "virt_src": 1, "virt_xf": 1, "virt_patch": 1, "patch_map": 1, "xforms": 9,
 }
 
 
@@ -126,6 +129,8 @@ class Mesh:
 
 
 def _mesh_from_handle(h, kind, dims) -> Mesh:
+    """The region's arrays; virtual cells (symmetry/periodic) are the last
+    len(virt_src) cells and are not counted in n_cells."""
     lib = _lib()
     try:
         sz = (ctypes.c_int64 * 9)()
@@ -141,8 +146,10 @@ def _mesh_from_handle(h, kind, dims) -> Mesh:
             arrays[name] = arr
     finally:
         lib.rflu_mesh_destroy(h)
-    return Mesh(kind=kind, dims=dims, n_cells=n_cells, n_faces=n_faces,
-                n_bfaces=n_bfaces, n_patches=n_patches, n_verts=n_verts, a=arrays)
+    n_virtual = int(arrays["virt_src"].size)
+    return Mesh(kind=kind, dims=dims, n_cells=n_cells - n_virtual, n_faces=n_faces,
+                n_bfaces=n_bfaces, n_patches=n_patches, n_verts=n_verts, a=arrays,
+                n_virtual=n_virtual)
 
 
 def box(kind: int, nx: int, ny: int, nz: int, lengths=(1.0, 1.0, 1.0),
@@ -218,6 +225,25 @@ class Grid:
         h = ctypes.c_void_p()
         _check(_lib().rflu_mesh_build(self._h, t, o, p, 1 if stencils else 0, ctypes.byref(h)),
                "rflu_mesh_build")
+        return _mesh_from_handle(h, GRID, None)
+
+    def build_sype(self, bc_type, bc_opt, bc_pout, related, axis, angle=None,
+                   stencils: bool = True) -> Mesh:
+        """The region of this grid with symmetry (100) and periodic (110) patches as
+        virtual cells (``rflu_mesh_build_sype``); ``related`` are 1-based patch numbers,
+        ``axis`` 1..3, ``angle`` radians (0)."""
+        P = self.sizes["n_patches"]
+        angle = [0.0] * P if angle is None else angle
+        args = [bc_type, bc_opt, bc_pout, related, axis, angle]
+        if any(len(a) != P for a in args):
+            raise ValueError(f"the grid has {P} patches")
+        i32 = lambda a: (ctypes.c_int32 * P)(*[int(x) for x in a])
+        f64 = lambda a: (ctypes.c_double * P)(*[float(x) for x in a])
+        h = ctypes.c_void_p()
+        _check(_lib().rflu_mesh_build_sype(self._h, i32(bc_type), i32(bc_opt), f64(bc_pout),
+                                           i32(related), i32(axis), f64(angle),
+                                           1 if stencils else 0, ctypes.byref(h)),
+               "rflu_mesh_build_sype")
         return _mesh_from_handle(h, GRID, None)
 
 

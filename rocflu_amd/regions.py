@@ -28,6 +28,9 @@ the virtual cells' stencils from the region's own cells, which can make its WENO
 results near borders depend on the partition; that construction is a preprocessing
 step, outside this path.)
 
+``sype_part`` makes the one region of a grid with symmetry/periodic patches, whose
+virtual cells mirror its own actual cells through borders of the region with itself.
+
 ``GpuRegions`` is the host loop of RungeKuttaMP over the regions, through the
 per-callee entry points and ``rflu_gpu_exchange``.
 """
@@ -62,6 +65,8 @@ class RegionPart:
     borders: List[Border] = field(default_factory=list)
     order: int = 2           # the order the virtual layers were built for
     blend: bool = True       # whether the virtual cells next to actual cells keep stencils
+    # per border, the 3x3 turn (row-major) of what it receives, or None: a plain copy
+    xforms: Optional[List[Optional[np.ndarray]]] = None
 
     @property
     def n_act(self) -> int:
@@ -209,13 +214,38 @@ def partition(mesh: Mesh, part, order: int = 2, blend: bool = True) -> List[Regi
     return parts
 
 
+def sype_part(mesh: Mesh) -> RegionPart:
+    """The one region of a grid with symmetry/periodic patches (``Grid.build_sype``): its
+    virtual cells take the state and final gradients of the actual cells they mirror
+    through borders of the region with itself, one per transform (the first, the
+    identity, a plain copy), as the reference's serial region does through its own
+    comm lists and RFLU_RELP_TransformWrapper."""
+    n = int(mesh.n_cells)
+    src = np.asarray(mesh.a["virt_src"], dtype=np.int32)
+    xf = np.asarray(mesh.a["virt_xf"], dtype=np.int32)
+    mats = np.asarray(mesh.a["xforms"], dtype=np.float64).reshape(-1, 9)
+    borders, xforms = [], []
+    for k in np.unique(xf):
+        sel = np.nonzero(xf == k)[0]
+        borders.append(Border(nbr=0, nbr_border=len(borders), send=src[sel],
+                              recv=(n + sel).astype(np.int32)))
+        xforms.append(None if k == 0 else mats[k].copy())
+    cells = np.concatenate([np.arange(n), src]).astype(np.int64)
+    return RegionPart(mesh=mesh, cells=cells, borders=borders, order=2, blend=True,
+                      xforms=xforms)
+
+
 def check_parts(parts: List[RegionPart], num: Numerics) -> None:
-    """Rejects regions whose virtual layers cannot carry `num`'s scheme: at order 2 with
+    """Rejects regions whose virtual layers cannot carry `num`'s scheme: constrained
+    gradients in partitioned regions (their wall-face lists end at the region); at order 2 with
     the WENO-XYZ blend an actual cell blends the raw gradients of the virtual cells in
     its stencil before the gradient exchange replaces them, so those virtual cells need
     stencils of their own (partition(..., blend=True)); a run without them would read
     gradients that were never formed."""
     for r, p in enumerate(parts):
+        if num.order == 2 and num.creconst and p.xforms is None and p.mesh.n_virtual:
+            raise ValueError(f"region {r}: the partitioned regions' wall-face lists stop at "
+                             "the region; constrained gradients (CRECONSTC) need one region")
         if num.order > p.order:
             raise ValueError(f"region {r} was partitioned for order {p.order}, the scheme is "
                              f"order {num.order}")
@@ -262,6 +292,12 @@ class GpuRegions:
                     g.handle, nb, nat.ptr(nbr, i32), nat.ptr(nbrb, i32), nat.ptr(sptr, i32),
                     nat.ptr(scell, i32), nat.ptr(rptr, i32), nat.ptr(rcell, i32)),
                     "rflu_gpu_set_borders")
+                for b, m in enumerate(p.xforms or []):
+                    if m is not None:
+                        mm = np.ascontiguousarray(m, dtype=np.float64).reshape(9)
+                        nat.check(self._lib.rflu_gpu_set_border_transform(
+                            g.handle, b, nat.ptr(mm, ctypes.c_double)),
+                            "rflu_gpu_set_border_transform")
         except Exception:
             self.close()
             raise
@@ -278,6 +314,9 @@ class GpuRegions:
         for p, g in zip(self.parts, self.regions):
             g.upload(cv[p.cells], time=time)
         self.time = float(time)
+        if any(p.borders for p in self.parts):
+            # the virtual cells as the exchange sets them (turned copies included)
+            self.exchange(XCHG_STATE)
 
     def download(self):
         """(cv, dv) of every grid cell from the regions' actual cells."""

@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CASES = {
     "gas_jump_closed_creconst0_np2": "box/gas-jump/closed-creconst0/np2",
     "gas_jump_closed_np2": "box/gas-jump/closed/np2",
+    "gas_jump_periodic_np1": "box/gas-jump/periodic/np1",
+    "gas_jump_symmetry_np1": "box/gas-jump/symmetry/np1",
 }
 
 

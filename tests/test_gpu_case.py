@@ -5,7 +5,8 @@ velocity/temperature inflow, the shock-tube initial state of INITFLOW FLAG 3) wi
 Fortran host (rocflu_amd/fortran/rflu_case_driver) reads the three files and drives
 the per-callee GPU path; the Python host reads them through rocflu_amd.inputs and
 drives the same entry points: both give the same bits, and the oracle the same state
-to 1e-12.  The template's y and z patches are periodic; here they are slip walls."""
+to 1e-12.  The template's y and z patches are periodic (virtual cells exchanged through
+borders of the region with itself); a variant makes them slip walls."""
 import os
 import re
 import subprocess
@@ -17,7 +18,6 @@ from oracle import dump as odump
 from oracle.oracle import Oracle
 from rocflu_amd import inputs as ri
 from rocflu_amd import mesh as rm
-from rocflu_amd.solver import GpuRegion
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -36,11 +36,11 @@ def slip_walls_for_periodic(bc_text):
     return "".join(out)
 
 
-def make_case(tmp, kind):
+def make_case(tmp, kind, periodic=True):
     with open(os.path.join(TEMPLATES, "rectshktb.inp")) as f:
         inp = f.read()
     with open(os.path.join(TEMPLATES, "rectshktb.bc")) as f:
-        bc = slip_walls_for_periodic(f.read())
+        bc = f.read() if periodic else slip_walls_for_periodic(f.read())
     with open(os.path.join(tmp, "rectshktb.inp"), "w") as f:
         f.write(inp)
     with open(os.path.join(tmp, "rectshktb.bc"), "w") as f:
@@ -51,13 +51,15 @@ def make_case(tmp, kind):
     g.write_centaur(os.path.join(tmp, "rectshktb.hyb.asc"))
 
 
-@pytest.mark.parametrize("kind", [rm.HEX, rm.TET6])
-def test_case_from_input_files(gpu, tmp_path, kind):
+@pytest.mark.parametrize("kind,periodic", [(rm.HEX, True), (rm.TET6, True), (rm.HEX, False),
+                                           (rm.TET6, False)])
+def test_case_from_input_files(gpu, tmp_path, kind, periodic):
     assert os.path.exists(DRIVER), "build the Fortran drivers first (amdflang)"
-    make_case(str(tmp_path), kind)
+    make_case(str(tmp_path), kind, periodic)
     nsteps = 20
     out = os.path.join(tmp_path, "cv.bin")
-    print(f"case: rectshktb from input files, kind={kind}, {nsteps} steps", flush=True)
+    print(f"case: rectshktb from input files, kind={kind}, periodic={periodic}, {nsteps} steps",
+          flush=True)
     p = subprocess.run([DRIVER, str(tmp_path), "rectshktb", str(nsteps), out],
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
     print(p.stdout, flush=True)
@@ -68,15 +70,18 @@ def test_case_from_input_files(gpu, tmp_path, kind):
 
     case = ri.load_case("rectshktb", str(tmp_path))
     assert case.num.creconst == 1 and case.num.patch_order[0] == 2
-    assert [b.bc_type for b in case.bcs] == [11, 20, 60, 60, 60, 60]
+    side = 110 if periodic else 60
+    assert [b.bc_type for b in case.bcs] == [11, 20] + [side] * 4
+    assert (case.mesh.n_virtual > 0) == periodic
     cv0 = ri.initial_state(case, "rectshktb")
     assert (cv0[:, 1] > 0).any() and (cv0[:, 1] == 0).any()     # the shock is in the box
-    g = GpuRegion(case.mesh, case.num)
+    g = ri.gpu_host(case)
     try:
         g.upload(cv0)
         for _ in range(nsteps):
             g.runge_kutta(end_time=case.inp.max_time)
-        g.check_status()
+        for r in getattr(g, "regions", [g]):
+            r.check_status()
         cvp, _ = g.download()
         time = g.time
     finally:

@@ -127,8 +127,9 @@ def test_virtual_cell_arguments_are_checked_without_a_device():
             GpuRegion(mesh, Numerics(order=2, reconst=11, patch_vals=vals, **kw))
         return e.value.code
 
-    # constrained gradients are not taken with virtual cells
-    assert code(sub, creconst=1) == nat.RFLU_ERR_ARG
+    # partitioned regions take no constrained gradients (their wall lists end at the region)
+    with pytest.raises(ValueError, match="CRECONSTC"):
+        rg.check_parts(parts, Numerics(order=2, reconst=11, creconst=1, patch_vals=vals))
     sub_bad = rg.Mesh(**{**sub.__dict__, "n_virtual": -1})
     assert code(sub_bad) == nat.RFLU_ERR_ARG
     L = nat.lib()
