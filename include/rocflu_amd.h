@@ -390,11 +390,13 @@ int rflu_dump_read(const char* path, int32_t n_cells, int32_t n_cv, int64_t* ste
                    double* time, int32_t* region, double* cv);
 /* A case of a region handle (0 for a single region; case i of a batch): writes its
  * current state as the dump of `region` at (step, time), the host's
- * global%currentStep/currentTime.  Waits for the stream. */
+ * global%currentStep/currentTime.  A region with virtual cells writes its actual cells
+ * (cv(:,1:nCells), as the reference does).  Waits for the stream. */
 int rflu_gpu_write_dump(int32_t handle, int32_t icase, const char* path, int64_t step,
                         double time, int32_t region);
 /* Reads a dump into case icase: its state, dependent variables and clock (the dump's
- * time; counters and flags of the case reset as after an upload).  step, time and
+ * time; counters and flags of the case reset as after an upload).  Virtual cells keep
+ * their values until the next rflu_gpu_exchange.  step, time and
  * region receive the header fields (any may be NULL).  Waits for the stream. */
 int rflu_gpu_read_dump(int32_t handle, int32_t icase, const char* path, int64_t* step,
                        double* time, int32_t* region);

@@ -1220,8 +1220,9 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
   }
   for (int k = 0; k < K; ++k) {
     const rflu_region_desc* d = descs + k;
-    // n counts actual and virtual cells; boundary faces belong to actual cells
-    const int n = d->n_cells + d->n_virtual, na = d->n_cells, nf = d->n_faces;
+    // n counts actual and virtual cells; boundary faces belong to actual cells, or (the
+    // outer faces of symmetry/periodic virtual layers) to virtual ones
+    const int n = d->n_cells + d->n_virtual, nf = d->n_faces;
     const int np = d->n_patches, base = d->index_base;
     const int c0 = (int)co[k], f0 = (int)fo[k], b0 = (int)bo[k], p0 = (int)po[k];
     const int nb = (int)(bo[k + 1] - bo[k]);
@@ -1244,7 +1245,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
       for (int b = d->bpatch_ptr[p]; b < d->bpatch_ptr[p + 1]; ++b) {
         if (b < 0 || b >= nb) { free_region(R); return fail(RFLU_ERR_ARG, "bad bpatch_ptr"); }
         int c1 = d->bf2c[b] - base;
-        if (c1 < 0 || c1 >= na) { free_region(R); return fail(RFLU_ERR_ARG, "bf2c out of range"); }
+        if (c1 < 0 || c1 >= n) { free_region(R); return fail(RFLU_ERR_ARG, "bf2c out of range"); }
         hbf2c[b0 + b] = c0 + c1;
         hbpatch[b0 + b] = p0 + p;
         hbfn[b0 + b] = make_double4(d->bfn[4 * (size_t)b], d->bfn[4 * (size_t)b + 1], d->bfn[4 * (size_t)b + 2], d->bfn[4 * (size_t)b + 3]);
@@ -2741,7 +2742,10 @@ int rflu_gpu_write_dump(int32_t h, int32_t icase, const char* path, int64_t step
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
   if (int frc = flush_pending(R)) return frc;
   if (!path || icase < 0 || icase >= R->K) return fail(RFLU_ERR_ARG, "bad case or null path");
-  const int N = R->N, c0 = R->cell0_h[icase], n = R->cell0_h[icase + 1] - c0;
+  // the actual cells, as RFLU_WriteRestartData writes cv(:,1:nCells) (virtual cells
+  // come first-to-last after them in both numberings)
+  const int N = R->N, c0 = R->cell0_h[icase];
+  const int n = R->K == 1 ? R->nact : R->cell0_h[icase + 1] - c0;
   // device [5][N] in the region's cell order -> the case's cells in the caller's order
   std::vector<double> dev((size_t)5 * n);
   for (int iv = 0; iv < 5; ++iv)
@@ -2767,7 +2771,10 @@ int rflu_gpu_read_dump(int32_t h, int32_t icase, const char* path, int64_t* step
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
   if (int frc = flush_pending(R)) return frc;
   if (!path || icase < 0 || icase >= R->K) return fail(RFLU_ERR_ARG, "bad case or null path");
-  const int N = R->N, c0 = R->cell0_h[icase], n = R->cell0_h[icase + 1] - c0;
+  // the actual cells (RFLU_ReadRestartData reads cv(:,1:nCells)); the next exchange
+  // sets the virtual ones
+  const int N = R->N, c0 = R->cell0_h[icase];
+  const int n = R->K == 1 ? R->nact : R->cell0_h[icase + 1] - c0;
   std::vector<double> var((size_t)5 * n);
   double* ptrs[5];
   for (int iv = 0; iv < 5; ++iv) ptrs[iv] = var.data() + (size_t)iv * n;
