@@ -49,7 +49,8 @@ def algorithmic_bytes(m, num, kernel: str, tuning: dict, compulsory: bool = Fals
     N, F, B = int(m.n_cells), int(m.n_faces), int(m.n_bfaces)
     S = int(m.c2cs_idx.size) if num.order == 2 else 0
     K = int(m.c2f_idx.size)               # cell-to-face entries (2F + B)
-    fb = 4 if num.precision in (1, 3) else 8   # flux buffer width
+    fpf = 32 if num.precision == 3 else 40     # flux buffer bytes per face (FP32 mass/energy
+                                               # and FP64 momenta in the FP32-flux split)
     gb = 4 if num.precision in (1, 2) else 8   # gradient width
     if kernel == "gradient":
         b = N * (40 + 15 * gb) + 4 * (N + 1)   # own state in, gradient out, stencil counts
@@ -65,14 +66,14 @@ def algorithmic_bytes(m, num, kernel: str, tuning: dict, compulsory: bool = Fals
             b += 8 * N + 2 * F * (24 + 2)       # eps^2; cofg - fc and slot per face side
         return float(b)
     if kernel == "face_flux":
-        per_face = 8 + 32 + (32 if num.order == 2 else 0) + 5 * fb
+        per_face = 8 + 32 + (32 if num.order == 2 else 0) + fpf
         per_cell = (40 + 32 + 15 * gb) if num.order == 2 else (40 + 24)
         return float(F * per_face + N * per_cell)
     if kernel == "update":
         # c2f, flux (each face value once), cv in/out, RK storage (SSP-RK3: cvOld once;
         # Wray: rhsSum written, read+written, read over 3 stages), vol, dv out, prim out
         rk_store = 40 if num.rk == 4 else 40 * 4 / 3
-        b = 4 * (N + 1) + 4 * K + 5 * fb * (F + B) + N * (40 + 40 + rk_store + 8 + 24)
+        b = 4 * (N + 1) + 4 * K + fpf * (F + B) + N * (40 + 40 + rk_store + 8 + 24)
         if num.order == 2:
             b += 40 * N
         return float(b)
@@ -228,6 +229,45 @@ def time_region(args, m, num, exact: bool, steps: int, err_steps: int, dropin: b
           + (f", per-callee {res['dropin']['ms_per_step']:.3f} ms/step" if dropin else "")
           + f"; tuning {tuning}; kernels {kt}", file=sys.stderr, flush=True)
     return res
+
+
+def final_state(m, num, steps, fast=False):
+    """State after `steps` fused steps from the headline initial state, default kernel
+    variants (fast: the FP64 moment-form gradient, which only sums in another order)."""
+    from rocflu_amd import cases
+    from rocflu_amd.solver import GpuRegion
+    cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
+    g = GpuRegion(m, num)
+    try:
+        if fast:
+            g.set_option(GpuRegion.OPT_GRAD_FAST, 1)
+        g.upload(cv0)
+        g.time_steps(steps, graph=True)
+        return g.download()[0]
+    finally:
+        g.close()
+
+
+def mixed_accuracy(args, m, final64):
+    """The mixed run's error per conserved variable beside what sets it: after one step
+    (the stage's own rounding), after err_steps without a modifier (no amplification),
+    and after err_steps with the limiter next to an FP64 run whose gradient only sums in
+    another order (the moment form) -- the limiter amplifies rounding-level differences
+    of the gradients in the transverse momenta whatever causes them."""
+    from rocflu_amd.solver import Numerics
+    out = {}
+    mk = lambda prec, reconst: Numerics(discr=args.discr, order=2, reconst=reconst, rk=4,
+                                        cfl=0.5, dt_imposed=1e-4, dt_min_limit=0.0,
+                                        precision=prec)
+    a, b = final_state(m, mk(1, args.reconst), 1), final_state(m, mk(0, args.reconst), 1)
+    out["one_step"] = rel_l2_per_variable(a, b)
+    r = final_state(m, mk(0, args.reconst), args.err_steps, fast=True)
+    out["fp64_reorder_drift"] = rel_l2_per_variable(r, final64)
+    a = final_state(m, mk(1, 0), args.err_steps)
+    b = final_state(m, mk(0, 0), args.err_steps)
+    out["no_modifier"] = rel_l2_per_variable(a, b)
+    print(f"# mixed accuracy: {out}", file=sys.stderr, flush=True)
+    return out
 
 
 def rel_l2_per_variable(a, b):
@@ -573,9 +613,11 @@ def main():
         out["mixed"] = {"value": mx["value"], "ms_per_step": mx["ms_per_step"],
                         "graph_value": mx["graph_value"], "step_hbm": step_hbm(m, mx),
                         "speedup_vs_fp64": mx["value"] / fp["value"],
-                        "dtype": "f32 gradients and interior fluxes, f64 state/residual/limiter",
+                        "dtype": "f32 gradients and reconstruction increments, f64 fluxes/"
+                                 "state/residual/limiter/positivity",
                         "err_steps": args.err_steps,
                         "rel_l2_per_variable": rel_l2_per_variable(mx["final"], fp["final"]),
+                        "accuracy": mixed_accuracy(args, m, fp["final"]),
                         "kernel_ms": {k: [v[0], v[1]] for k, v in mx["kernels"].items()},
                         "tuning": mx["tuning"]}
     # the 80^3 x 6 configuration: HLLC and AUSM+ in FP64, HLLC mixed

@@ -56,13 +56,18 @@ extern "C" {
 #define RFLU_RK3_WRAY 2
 #define RFLU_RK1_EULER 3
 #define RFLU_RK3_SSP 4
-/* Arithmetic: FP64 everywhere, or FP32 gradients and fluxes with FP64 residual
- * accumulation, state, limiter and positivity checks. */
+/* Arithmetic: FP64 everywhere, or the mixed mode: FP32 cell gradients (stencil
+ * weights, accumulation, storage) and FP32 reconstruction increments, with FP64 face
+ * fluxes, flux buffer, residual accumulation, state, limiter and positivity checks.
+ * (The face-flux functions stay FP64 because HLLC's star pressure responds to the FP32
+ * rounding of the normal velocities with rho*a*dq, as large as rounding p itself, which
+ * sets the error of the momenta normal to the main flow.) */
 #define RFLU_PREC_FP64 0
 #define RFLU_PREC_MIXED 1
-/* Diagnostic splits of the mixed mode: FP32 only in the gradients, or only in the
- * interior fluxes. */
+/* The mixed mode under its older name (FP32 only in the gradients): the same kernels. */
 #define RFLU_PREC_FP32_GRAD 2
+/* Diagnostic: FP64 gradients, FP32 interior-flux arithmetic with the pressure terms in
+ * FP64 and a flux buffer of FP32 mass/energy and FP64 momenta. */
 #define RFLU_PREC_FP32_FLUX 3
 
 /*

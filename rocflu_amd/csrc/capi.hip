@@ -13,6 +13,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 using namespace rflu;
@@ -37,6 +38,8 @@ enum KernelId { KID_FACE = 0, KID_UPDATE = 1, KID_GRAD = 2, KID_BLEND = 3, KID_D
 
 // One border of a region (t_border): the actual cells it sends to a neighbouring
 // region and the virtual cells it receives from it, as internal cell numbers.
+enum { FBUF_F64 = 0, FBUF_MIX = 1 };
+
 struct Border {
   int nbr = 0;          // neighbour's handle (the region itself for symmetry/periodic)
   int nbr_border = 0;   // the neighbour's border that matches this one
@@ -52,6 +55,7 @@ struct Region {
   int base = 0;                // the descriptor's index base (border cell lists)
   std::vector<Border> borders;
   int order = 1, discr = 3, reconst = 0, rk = 2, prec = 0;
+  int fbuf = 0;                // flux buffer: FBUF_F64 or FBUF_MIX (FluxMix)
   double cfl = 0.5, dt_imposed = 0, dt_min_limit = 0;
   double end_time = HUGE_VAL;  // global%maxTime of the fused loop
   Gas gas{};                   // case 0 (the only case of a single region)
@@ -662,7 +666,7 @@ int launch_update_rk(Region* R, int stage, int last, double dt_in) {
     FaceTiles FTl;
     FTl.fptr = R->ft_fptr; FTl.eptr = R->ft_eptr; FTl.eidx = R->ft_eidx; FTl.fslot = R->ft_fslot;
     FTl.ts = R->face_tile; FTl.cap = R->face_cap; FTl.fcap = R->face_fcap;
-    const size_t shm = (size_t)5 * R->face_fcap * sizeof(FT);
+    const size_t shm = (size_t)5 * R->face_fcap * sizeof(typename FluxIO<FT>::val_t);
     dim3 tb(R->face_tile), tg(R->n_face_tiles);
     switch (R->rk) {
       case 1: hipLaunchKernelGGL((k_update_tile<1, FT, FUSED, PRIM>), tg, tb, shm, R->stream, M, S, R->gas, C, FTl, fl, R->sc, stage, last, dt_in); break;
@@ -714,10 +718,18 @@ void collect_events(Region* R) {
   R->events.clear();
 }
 
-// FP32 gradient storage/arithmetic for the mixed and FP32-gradient modes.
+// FP32 gradient storage/arithmetic (the mixed mode; FP32_GRAD is its older name).
 inline bool grad32(const Region* R) { return R->prec == RFLU_PREC_MIXED || R->prec == RFLU_PREC_FP32_GRAD; }
-// FP32 interior-flux arithmetic and flux buffer for the mixed and FP32-flux modes.
-inline bool flux32(const Region* R) { return R->prec == RFLU_PREC_MIXED || R->prec == RFLU_PREC_FP32_FLUX; }
+// FP32 interior-flux arithmetic (the FP32_FLUX diagnostic split only).
+inline bool flux32(const Region* R) { return R->prec == RFLU_PREC_FP32_FLUX; }
+
+// Calls fn with a null pointer of the region's flux-buffer type (double or FluxMix),
+// so one generic lambda launches the matching kernel instantiation.
+template <typename Fn>
+int with_fbuf(const Region* R, Fn&& fn) {
+  if (R->fbuf == FBUF_MIX) return fn((FluxMix*)nullptr);
+  return fn((double*)nullptr);
+}
 
 int launch_gradients(Region* R) {
   if (R->order != 2) return 0;
@@ -765,9 +777,8 @@ int launch_fluxes_p(Region* R) {
     Timed t(R, KID_FACE);
     switch (R->prec) {
       case RFLU_PREC_FP64: rc = launch_fluxes_t<double, double, double, SR>(R); break;
-      case RFLU_PREC_MIXED: rc = launch_fluxes_t<float, float, float, SR>(R); break;
-      case RFLU_PREC_FP32_GRAD: rc = launch_fluxes_t<double, float, double, SR>(R); break;
-      default: rc = launch_fluxes_t<float, double, float, SR>(R); break;
+      case RFLU_PREC_FP32_FLUX: rc = launch_fluxes_t<float, double, FluxMix, SR>(R); break;
+      default: rc = launch_fluxes_t<double, float, double, SR>(R); break;   // mixed
     }
   }
   if (rc) return rc;
@@ -775,9 +786,8 @@ int launch_fluxes_p(Region* R) {
     Timed t(R, KID_BFACE);
     switch (R->prec) {
       case RFLU_PREC_FP64: rc = launch_bfaces_t<double, double, SR>(R); break;
-      case RFLU_PREC_MIXED: rc = launch_bfaces_t<float, float, SR>(R); break;
-      case RFLU_PREC_FP32_GRAD: rc = launch_bfaces_t<float, double, SR>(R); break;
-      default: rc = launch_bfaces_t<double, float, SR>(R); break;
+      case RFLU_PREC_FP32_FLUX: rc = launch_bfaces_t<double, FluxMix, SR>(R); break;
+      default: rc = launch_bfaces_t<float, double, SR>(R); break;   // mixed
     }
   }
   return rc;
@@ -788,12 +798,12 @@ int launch_fluxes_p(Region* R) {
 void launch_mass_inout(Region* R) {
   if (!R->has_flowthrough) return;
   Mesh M = R->mesh();
-  if (!flux32(R))
-    hipLaunchKernelGGL((k_mass_inout<double>), dim3(R->K), dim3(256), 0, R->stream, M, R->cases(),
-                       R->patches(), (const double*)R->flux);
-  else
-    hipLaunchKernelGGL((k_mass_inout<float>), dim3(R->K), dim3(256), 0, R->stream, M, R->cases(),
-                       R->patches(), (const float*)R->flux);
+  with_fbuf(R, [&](auto* tag) {
+    using FT = std::remove_pointer_t<decltype(tag)>;
+    hipLaunchKernelGGL((k_mass_inout<FT>), dim3(R->K), dim3(256), 0, R->stream, M, R->cases(),
+                       R->patches(), (const FT*)R->flux);
+    return 0;
+  });
 }
 
 // srad: also store the face spectral radii of the time step (first stage of a fused
@@ -811,14 +821,12 @@ int launch_update(Region* R, int stage, bool fused, double dt_in = -1.0, bool ad
   // the last stage of a fused step advances the device clock (not for the host's stages)
   int last = advance && stage == (R->rk == 1 ? 4 : R->rk == 3 ? 1 : 3);
   bool prim = R->order == 2;
-  if (!flux32(R)) {
-    if (fused) return prim ? launch_update_rk<double, true, true>(R, stage, last, dt_in)
-                           : launch_update_rk<double, true, false>(R, stage, last, dt_in);
-    return launch_update_rk<double, false, false>(R, stage, last, dt_in);
-  }
-  if (fused) return prim ? launch_update_rk<float, true, true>(R, stage, last, dt_in)
-                         : launch_update_rk<float, true, false>(R, stage, last, dt_in);
-  return launch_update_rk<float, false, false>(R, stage, last, dt_in);
+  return with_fbuf(R, [&](auto* tag) {
+    using FT = std::remove_pointer_t<decltype(tag)>;
+    if (fused) return prim ? launch_update_rk<FT, true, true>(R, stage, last, dt_in)
+                           : launch_update_rk<FT, true, false>(R, stage, last, dt_in);
+    return launch_update_rk<FT, false, false>(R, stage, last, dt_in);
+  });
 }
 
 // Minimum local time step from the face spectral radii already in R->srad, then
@@ -1336,7 +1344,9 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
   // ---- device allocation ----
   if (hipStreamCreate(&R->stream) != hipSuccess) { free_region(R); return fail(RFLU_ERR_HIP, "hipStreamCreate"); }
   const size_t FB = (size_t)F + B;
-  const size_t fsz = flux32(R) ? sizeof(float) : sizeof(double);
+  // flux buffer: FP64, except with FP32 interior fluxes (mass and energy FP32, momenta FP64)
+  R->fbuf = flux32(R) ? FBUF_MIX : FBUF_F64;
+  const size_t fbytes = R->fbuf == FBUF_MIX ? 32 : 40;
   const size_t gsz = grad32(R) ? sizeof(float) : sizeof(double);
   R->n_dt_parts = (int)nblk(N, 256);
   std::vector<int> hcell0(co.begin(), co.end()), hbface0(bo.begin(), bo.end());
@@ -1361,7 +1371,7 @@ int create_cases(int32_t K, const rflu_region_desc* descs, int32_t* handle) {
     if ((rc = dalloc(&R->cv_old, 5 * (size_t)N))) { free_region(R); return rc; }
   }
   {
-    hipError_t e = hipMalloc(&R->flux, 5 * FB * fsz + 16);
+    hipError_t e = hipMalloc(&R->flux, fbytes * FB + 16);
     if (e != hipSuccess) { free_region(R); return fail(RFLU_ERR_NOMEM, "hipMalloc flux"); }
   }
   if (R->order == 2) {
@@ -1824,10 +1834,12 @@ int rflu_gpu_residual(int32_t h, double* rhs) {
   int rc;
   if ((rc = launch_gradients(R)) || (rc = launch_fluxes(R, 0))) return rc;
   Mesh M = R->mesh();
-  if (!flux32(R))
-    hipLaunchKernelGGL((k_gather_rhs<double>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, (const double*)R->flux, R->rhs);
-  else
-    hipLaunchKernelGGL((k_gather_rhs<float>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, (const float*)R->flux, R->rhs);
+  with_fbuf(R, [&](auto* tag) {
+    using FT = std::remove_pointer_t<decltype(tag)>;
+    hipLaunchKernelGGL((k_gather_rhs<FT>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M,
+                       (const FT*)R->flux, R->rhs);
+    return 0;
+  });
   HIP_TRY(hipGetLastError());
   const int N = R->N;
   std::vector<double> hr((size_t)5 * N);
@@ -2360,10 +2372,12 @@ int rflu_gpu_residual_dev(int32_t h, double* rhs) {
   int rc;
   if ((rc = launch_gradients(R)) || (rc = launch_fluxes(R, 0))) return rc;
   Mesh M = R->mesh();
-  if (!flux32(R))
-    hipLaunchKernelGGL((k_gather_rhs<double>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, (const double*)R->flux, rhs);
-  else
-    hipLaunchKernelGGL((k_gather_rhs<float>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M, (const float*)R->flux, rhs);
+  with_fbuf(R, [&](auto* tag) {
+    using FT = std::remove_pointer_t<decltype(tag)>;
+    hipLaunchKernelGGL((k_gather_rhs<FT>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream, M,
+                       (const FT*)R->flux, rhs);
+    return 0;
+  });
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(R->stream));
   return RFLU_OK;

@@ -174,6 +174,37 @@ struct State {
   double* prim;    // [5][N]
 };
 
+// Face-flux buffer [5][F+B] (interior faces, then boundary faces).  FT = double or
+// float holds the five components at that width; FT = FluxMix (the mixed mode) holds
+// mass and energy in FP32 and the three momentum components in FP64: F+B floats of
+// mass, F+B floats of energy, then [3][F+B] doubles -- 32 bytes per face.  val_t is
+// the register/LDS type of one value; the residual sums in FP64 whatever it is.
+struct FluxMix {};
+template <typename FT>
+struct FluxIO {
+  using val_t = FT;
+  static __device__ __forceinline__ void put(FT* p, size_t FB, int k, size_t f, double v) {
+    p[k * FB + f] = FT(v);
+  }
+  static __device__ __forceinline__ FT get(const FT* p, size_t FB, int k, size_t f) {
+    return p[k * FB + f];
+  }
+};
+template <>
+struct FluxIO<FluxMix> {
+  using val_t = double;
+  static __device__ __forceinline__ void put(FluxMix* p, size_t FB, int k, size_t f, double v) {
+    if (k == 0) ((float*)p)[f] = float(v);
+    else if (k == 4) ((float*)p)[FB + f] = float(v);
+    else ((double*)p)[(size_t)k * FB + f] = v;   // doubles 1..3 of [4][F+B] (floats fill 0)
+  }
+  static __device__ __forceinline__ double get(const FluxMix* p, size_t FB, int k, size_t f) {
+    if (k == 0) return double(((const float*)p)[f]);
+    if (k == 4) return double(((const float*)p)[FB + f]);
+    return ((const double*)p)[(size_t)k * FB + f];
+  }
+};
+
 template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
 template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 __device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
@@ -361,6 +392,169 @@ __device__ __forceinline__ void ausm(T nx, T ny, T nz, T nm, T rl, T ul, T vl, T
   flx[4] = (af * (mfp * rl * Hl + mfm * rr * Hr)) * nm;
 }
 
+// ---------------------------------------------------------------------------------
+// Mixed precision: the same three fluxes in FP32 with their pressure terms in FP64.
+// A face's momentum flux carries p*n*|A| with p of order one, while the momenta
+// normal to the main flow stay orders of magnitude smaller; rounding p*n*|A| to FP32
+// (6e-8 of p|A| per face) would set their error.  So the face pressures PL, PR arrive
+// in FP64 (the cell's FP64 pressure plus its FP32 reconstruction increment) and the
+// pressure parts of the momentum fluxes -- p*n*|A| of the upwind and central forms,
+// HLLC's star pressure, AUSM+'s split pressure with its weights -- are formed in FP64;
+// the mass flux, the convective and dissipative parts and the energy flux stay FP32.
+// Pressure differences enter the FP32 parts as FP64 differences rounded once.  The
+// momentum components leave in FP64.  Mathematically these are hllc/roe/ausm above.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ void hllc_mp(float g, double nxd, double nyd, double nzd, double nmd,
+                                        float rl, float ul, float vl, float wl, double PL, float al,
+                                        float el, float Hl, float rr, float ur, float vr, float wr,
+                                        double PR, float ar, float er, float Hr, double* flx) {
+  const float one = 1.f, half = 0.5f, zero = 0.f;
+  const float nx = float(nxd), ny = float(nyd), nz = float(nzd), nm = float(nmd);
+  const float pl = float(PL), pr = float(PR);
+  float ql = ul * nx + vl * ny + wl * nz;
+  float qr = ur * nx + vr * ny + wr * nz;
+  float wt1 = sqrtf(rr / rl);
+  float wt2 = one / (one + wt1);
+  float us = wt2 * (ul + wt1 * ur);
+  float vs = wt2 * (vl + wt1 * vr);
+  float ws = wt2 * (wl + wt1 * wr);
+  float qs = wt2 * (ql + wt1 * qr);
+  float Hs = wt2 * (Hl + wt1 * Hr);
+  float Vms = us * us + vs * vs + ws * ws;
+  float as = sqrtf((g - one) * (Hs - half * Vms));
+  float sl = tmin(ql - al, qs - as);
+  float sr = tmax(qr + ar, qs + as);
+  if (sl > zero) {
+    flx[0] = double(ql * rl * nm);
+    flx[1] = (double(ql * rl * ul) + PL * nxd) * nmd;
+    flx[2] = (double(ql * rl * vl) + PL * nyd) * nmd;
+    flx[3] = (double(ql * rl * wl) + PL * nzd) * nmd;
+    flx[4] = double((ql * (rl * el + pl)) * nm);
+  } else if (sr < zero) {
+    flx[0] = double(qr * rr * nm);
+    flx[1] = (double(qr * rr * ur) + PR * nxd) * nmd;
+    flx[2] = (double(qr * rr * vr) + PR * nyd) * nmd;
+    flx[3] = (double(qr * rr * wr) + PR * nzd) * nmd;
+    flx[4] = double((qr * (rr * er + pr)) * nm);
+  } else {
+    float sm = (rr * qr * (sr - qr) - rl * ql * (sl - ql) + float(PL - PR)) /
+               (rr * (sr - qr) - rl * (sl - ql));
+    const float dl = rl * (ql - sl) * (ql - sm);   // ps - pl
+    const double PS = PL + double(dl);
+    const float ps = float(PS);
+    float term, rs, rus, rvs, rws, res;
+    if (sm > zero) {
+      term = one / (sl - sm);
+      rs = term * (sl - ql) * rl;
+      rus = term * ((sl - ql) * rl * ul + dl * nx);
+      rvs = term * ((sl - ql) * rl * vl + dl * ny);
+      rws = term * ((sl - ql) * rl * wl + dl * nz);
+      res = term * ((sl - ql) * rl * el + (ps * sm - pl * ql));
+    } else {
+      const float dr = float(PS - PR);             // ps - pr
+      term = one / (sr - sm);
+      rs = term * (sr - qr) * rr;
+      rus = term * ((sr - qr) * rr * ur + dr * nx);
+      rvs = term * ((sr - qr) * rr * vr + dr * ny);
+      rws = term * ((sr - qr) * rr * wr + dr * nz);
+      res = term * ((sr - qr) * rr * er + (ps * sm - pr * qr));
+    }
+    flx[0] = double(sm * rs * nm);
+    flx[1] = (double(sm * rus) + PS * nxd) * nmd;
+    flx[2] = (double(sm * rvs) + PS * nyd) * nmd;
+    flx[3] = (double(sm * rws) + PS * nzd) * nmd;
+    flx[4] = double((sm * (res + ps)) * nm);
+  }
+}
+
+__device__ __forceinline__ void roe_mp(float g, float epsentr, float dissFact, double nxd,
+                                       double nyd, double nzd, double nmd, float rl, float ul,
+                                       float vl, float wl, double PL, float Hl, float rr, float ur,
+                                       float vr, float wr, double PR, float Hr, double* flx) {
+  const float one = 1.f, half = 0.5f, two = 2.f;
+  const float nx = float(nxd), ny = float(nyd), nz = float(nzd), nm = float(nmd);
+  float ql = ul * nx + vl * ny + wl * nz;
+  float qr = ur * nx + vr * ny + wr * nz;
+  float rh = sqrtf(rl * rr);
+  float wt = rl / (rl + rh);
+  float uh = wt * ul + (one - wt) * ur;
+  float vh = wt * vl + (one - wt) * vr;
+  float wh = wt * wl + (one - wt) * wr;
+  float Hh = wt * Hl + (one - wt) * Hr;
+  float qh = uh * nx + vh * ny + wh * nz;
+  float sh = half * (uh * uh + vh * vh + wh * wh);
+  float ah = sqrtf((g - one) * (Hh - half * (two * sh)));
+  float l1 = fabsf(qh - ah);
+  float l2 = fabsf(qh);
+  float l5 = fabsf(qh + ah);
+  float efc = epsentr * l5;
+  l1 = ef_hh(l1, efc);
+  l2 = ef_hh(l2, efc);
+  l5 = ef_hh(l5, efc);
+  float dr = rr - rl, du = ur - ul, de = vr - vl, dw = wr - wl, dp = float(PR - PL);
+  float dq = du * nx + de * ny + dw * nz;
+  float dv1 = (dp - rh * ah * dq) / (two * ah * ah);
+  float dv5 = (dp + rh * ah * dq) / (two * ah * ah);
+  float dv2 = dr - dp / (ah * ah);
+  float t1 = l1 * dv1, t2 = l2 * dv2, t3 = l2 * rh, t5 = l5 * dv5;
+  float hd = half * dissFact;
+  const double PC = 0.5 * (PL + PR);
+  float fc1 = half * (ql * rl + qr * rr) * nm;
+  double fc2 = (double(half * (ql * rl * ul + qr * rr * ur)) + PC * nxd) * nmd;
+  double fc3 = (double(half * (ql * rl * vl + qr * rr * vr)) + PC * nyd) * nmd;
+  double fc4 = (double(half * (ql * rl * wl + qr * rr * wr)) + PC * nzd) * nmd;
+  float fc5 = half * (ql * rl * Hl + qr * rr * Hr) * nm;
+  float fd1 = hd * (t1 + t2 + t5) * nm;
+  float fd2 = hd * (t1 * (uh - nx * ah) + t2 * uh + t3 * (du - nx * dq) + t5 * (uh + nx * ah)) * nm;
+  float fd3 = hd * (t1 * (vh - ny * ah) + t2 * vh + t3 * (de - ny * dq) + t5 * (vh + ny * ah)) * nm;
+  float fd4 = hd * (t1 * (wh - nz * ah) + t2 * wh + t3 * (dw - nz * dq) + t5 * (wh + nz * ah)) * nm;
+  float fd5 = hd * (t1 * (Hh - qh * ah) + t2 * sh + t3 * (uh * du + vh * de + wh * dw - qh * dq) +
+                    t5 * (Hh + qh * ah)) * nm;
+  flx[0] = double(fc1 - fd1);
+  flx[1] = fc2 - double(fd2);
+  flx[2] = fc3 - double(fd3);
+  flx[3] = fc4 - double(fd4);
+  flx[4] = double(fc5 - fd5);
+}
+
+// AUSM+ pressure weights P+(M) and P-(M) of one side (alpha = 3/16), in FP64.
+__device__ __forceinline__ double ausm_pweight(double m, bool sub, double side) {
+  if (sub)
+    return 0.25 * (m + side) * (m + side) * (2.0 - side * m) +
+           side * 0.1875 * m * (m * m - 1.0) * (m * m - 1.0);
+  return 0.5 * (1.0 + side * m / fabs(m));
+}
+
+__device__ __forceinline__ void ausm_mp(double nxd, double nyd, double nzd, double nmd, float rl,
+                                        float ul, float vl, float wl, double PL, float Hl, float al,
+                                        float rr, float ur, float vr, float wr, double PR, float Hr,
+                                        float ar, double* flx) {
+  const float one = 1.f, half = 0.5f, q4 = 0.25f, e8 = 0.125f;
+  const float nx = float(nxd), ny = float(nyd), nz = float(nzd), nm = float(nmd);
+  float ql = ul * nx + vl * ny + wl * nz;
+  float qr = ur * nx + vr * ny + wr * nz;
+  float af = half * (al + ar);
+  float ml = ql / af, mla = fabsf(ml);
+  float mr = qr / af, mra = fabsf(mr);
+  const bool subl = mla <= one, subr = mra <= one;
+  float mlp = subl ? q4 * (ml + one) * (ml + one) + e8 * (ml * ml - one) * (ml * ml - one)
+                   : half * (ml + mla);
+  float mrm = subr ? -q4 * (mr - one) * (mr - one) - e8 * (mr * mr - one) * (mr * mr - one)
+                   : half * (mr - mra);
+  const double wtl = ausm_pweight(double(ml), subl, 1.0);
+  const double wtr = ausm_pweight(double(mr), subr, -1.0);
+  float mf = mlp + mrm;
+  float mfa = fabsf(mf);
+  float mfp = half * (mf + mfa);
+  float mfm = half * (mf - mfa);
+  const double PF = wtl * PL + wtr * PR;
+  flx[0] = double((af * (mfp * rl + mfm * rr)) * nm);
+  flx[1] = (double(af * (mfp * rl * ul + mfm * rr * ur)) + PF * nxd) * nmd;
+  flx[2] = (double(af * (mfp * rl * vl + mfm * rr * vr)) + PF * nyd) * nmd;
+  flx[3] = (double(af * (mfp * rl * wl + mfm * rr * wr)) + PF * nzd) * nmd;
+  flx[4] = double((af * (mfp * rl * Hl + mfm * rr * Hr)) * nm);
+}
+
 // Face state of cell c for a second-order flux: the flux loop reads cv after
 // CellGradientsMP's cons->prim->cons round trip, so momentum is rho*(u) with
 // u = (1/rho)*m, and velocities are that momentum times 1/rho again.
@@ -368,7 +562,7 @@ template <typename T, typename GV>
 __device__ __forceinline__ void recon_values(double rd, double pu, double pv, double pw, double pd,
                                              double cx, double cy, double cz, GV gv, double xf,
                                              double yf, double zf, T& r, T& u, T& v, T& w, T& p,
-                                             T& ir) {
+                                             T& ir, double& P) {
   double ird = 1.0 / rd;
   double ud = (rd * pu) * ird;
   double vd = (rd * pv) * ird;
@@ -378,20 +572,27 @@ __device__ __forceinline__ void recon_values(double rd, double pu, double pv, do
   u = T(ud) + T(gv(3)) * dx + T(gv(4)) * dy + T(gv(5)) * dz;
   v = T(vd) + T(gv(6)) * dx + T(gv(7)) * dy + T(gv(8)) * dz;
   w = T(wd) + T(gv(9)) * dx + T(gv(10)) * dy + T(gv(11)) * dz;
-  p = T(pd) + T(gv(12)) * dx + T(gv(13)) * dy + T(gv(14)) * dz;
+  if constexpr (sizeof(T) == 4) {
+    // mixed precision: the face pressure in FP64, the cell's value plus the FP32 increment
+    P = pd + double(T(gv(12)) * dx + T(gv(13)) * dy + T(gv(14)) * dz);
+    p = T(P);
+  } else {
+    p = T(pd) + T(gv(12)) * dx + T(gv(13)) * dy + T(gv(14)) * dz;
+    P = double(p);
+  }
   ir = T(ird);
 }
 
 template <typename T, typename GT>
 __device__ __forceinline__ void recon_state(const State& S, const Mesh& M, const GT* grad, int c,
                                             double xf, double yf, double zf, T& r, T& u, T& v,
-                                            T& w, T& p, T& ir) {
+                                            T& w, T& p, T& ir, double& P) {
   const int N = M.N;
   const double4 cc = M.cell[c];
   const GT* g = grad + c;
   recon_values<T>(S.prim[c], S.prim[N + c], S.prim[2 * N + c], S.prim[3 * N + c],
                   S.prim[4 * N + c], cc.x, cc.y, cc.z,
-                  [&](int k) { return g[(size_t)k * N]; }, xf, yf, zf, r, u, v, w, p, ir);
+                  [&](int k) { return g[(size_t)k * N]; }, xf, yf, zf, r, u, v, w, p, ir, P);
 }
 
 // Spectral radius (|v.n| + c)|A| of every face, interior faces first, then boundary
@@ -471,14 +672,48 @@ __device__ __forceinline__ void face_flux2(const Gas& G, T nx, T ny, T nz, T nm,
   }
 }
 
+// face_flux2 in mixed precision: FP32 states, FP64 face pressures and geometry.
+template <int DISCR>
+__device__ __forceinline__ void face_flux2_mp(const Gas& G, double4 n, float rl, float ul,
+                                              float vl, float wl, double PL, float irl, float rr,
+                                              float ur, float vr, float wr, double PR, float irr,
+                                              double* flx) {
+  const float g = float(G.g), cp = float(G.cp), gc = float(G.gc), one = 1.f, half = 0.5f;
+  const float pl = float(PL), pr = float(PR);
+  if (DISCR == 3) {
+    float el = pl / (rl * (g - one)) + half * (ul * ul + vl * vl + wl * wl);
+    float al = sqrtf(g * pl / rl);
+    float Hl = el + irl * pl;
+    float er = pr / (rr * (g - one)) + half * (ur * ur + vr * vr + wr * wr);
+    float ar = sqrtf(g * pr / rr);
+    float Hr = er + irr * pr;
+    hllc_mp(g, n.x, n.y, n.z, n.w, rl, ul, vl, wl, PL, al, el, Hl, rr, ur, vr, wr, PR, ar, er, Hr,
+            flx);
+  } else if (DISCR == 1) {
+    float tl = pl / (rl * gc), tr = pr / (rr * gc);
+    float Hl = cp * tl + half * (ul * ul + vl * vl + wl * wl);
+    float Hr = cp * tr + half * (ur * ur + vr * vr + wr * wr);
+    roe_mp(g, float(G.eps_entr), float(G.diss_fact), n.x, n.y, n.z, n.w, rl, ul, vl, wl, PL, Hl,
+           rr, ur, vr, wr, PR, Hr, flx);
+  } else {
+    float tl = pl / (rl * gc), tr = pr / (rr * gc);
+    float al = sqrtf(g * gc * tl), ar = sqrtf(g * gc * tr);
+    float Hl = cp * tl + half * (ul * ul + vl * vl + wl * wl);
+    float Hr = cp * tr + half * (ur * ur + vr * vr + wr * wr);
+    ausm_mp(n.x, n.y, n.z, n.w, rl, ul, vl, wl, PL, Hl, al, rr, ur, vr, wr, PR, Hr, ar, flx);
+  }
+}
+
 // Flux of interior face f (computed with f2c(1,f) = cc.x as the left state).
 template <int DISCR, int ORDER, typename T, typename GT>
 __device__ __forceinline__ void interior_face_flux(const Mesh& M, const State& S, const Gas& G,
-                                                   const GT* grad, int f, int2 cc, T* flx) {
+                                                   const GT* grad, int f, int2 cc,
+                                                   double* flx) {
   const int N = M.N;
   int c1 = cc.x, c2 = cc.y;
   double4 n = M.fn[f];
   T nx = T(n.x), ny = T(n.y), nz = T(n.z), nm = T(n.w);
+  constexpr bool MP = sizeof(T) == 4;   // FP32 arithmetic, FP64 pressure terms
   if (ORDER == 1) {
     const T g = T(G.g);
     (void)g;
@@ -491,27 +726,43 @@ __device__ __forceinline__ void interior_face_flux(const Mesh& M, const State& S
       double al = S.dv[2 * N + c1], ar = S.dv[2 * N + c2];
       double el = S.cv[4 * N + c1] * irl, er = S.cv[4 * N + c2] * irr;
       double Hl = el + irl * pl, Hr = er + irr * pr;
-      hllc<T>(g, nx, ny, nz, nm, T(rl), T(ul), T(vl), T(wl), T(pl), T(al), T(el), T(Hl), T(rr),
-              T(ur), T(vr), T(wr), T(pr), T(ar), T(er), T(Hr), flx);
+      if constexpr (MP)
+        hllc_mp(g, n.x, n.y, n.z, n.w, T(rl), T(ul), T(vl), T(wl), pl, T(al), T(el), T(Hl), T(rr),
+                T(ur), T(vr), T(wr), pr, T(ar), T(er), T(Hr), flx);
+      else
+        hllc<T>(g, nx, ny, nz, nm, T(rl), T(ul), T(vl), T(wl), T(pl), T(al), T(el), T(Hl), T(rr),
+                T(ur), T(vr), T(wr), T(pr), T(ar), T(er), T(Hr), flx);
     } else if (DISCR == 1) {
       double tl = S.dv[N + c1], tr = S.dv[N + c2];
       double Hl = G.cp * tl + 0.5 * (ul * ul + vl * vl + wl * wl);
       double Hr = G.cp * tr + 0.5 * (ur * ur + vr * vr + wr * wr);
-      roe<T>(g, T(G.eps_entr), T(G.diss_fact), nx, ny, nz, nm, T(rl), T(ul), T(vl), T(wl), T(pl),
-             T(Hl), T(rr), T(ur), T(vr), T(wr), T(pr), T(Hr), flx);
+      if constexpr (MP)
+        roe_mp(g, T(G.eps_entr), T(G.diss_fact), n.x, n.y, n.z, n.w, T(rl), T(ul), T(vl), T(wl),
+               pl, T(Hl), T(rr), T(ur), T(vr), T(wr), pr, T(Hr), flx);
+      else
+        roe<T>(g, T(G.eps_entr), T(G.diss_fact), nx, ny, nz, nm, T(rl), T(ul), T(vl), T(wl), T(pl),
+               T(Hl), T(rr), T(ur), T(vr), T(wr), T(pr), T(Hr), flx);
     } else {
       double al = S.dv[2 * N + c1], ar = S.dv[2 * N + c2];
       double Hl = S.cv[4 * N + c1] * irl + pl / rl;
       double Hr = S.cv[4 * N + c2] * irr + pr / rr;
-      ausm<T>(nx, ny, nz, nm, T(rl), T(ul), T(vl), T(wl), T(pl), T(Hl), T(al), T(rr), T(ur),
-              T(vr), T(wr), T(pr), T(Hr), T(ar), flx);
+      if constexpr (MP)
+        ausm_mp(n.x, n.y, n.z, n.w, T(rl), T(ul), T(vl), T(wl), pl, T(Hl), T(al), T(rr), T(ur),
+                T(vr), T(wr), pr, T(Hr), T(ar), flx);
+      else
+        ausm<T>(nx, ny, nz, nm, T(rl), T(ul), T(vl), T(wl), T(pl), T(Hl), T(al), T(rr), T(ur),
+                T(vr), T(wr), T(pr), T(Hr), T(ar), flx);
     }
   } else {
     double4 xf = M.fc[f];
     T rl, ul, vl, wl, pl, irl, rr, ur, vr, wr, pr, irr;
-    recon_state<T, GT>(S, M, grad, c1, xf.x, xf.y, xf.z, rl, ul, vl, wl, pl, irl);
-    recon_state<T, GT>(S, M, grad, c2, xf.x, xf.y, xf.z, rr, ur, vr, wr, pr, irr);
-    face_flux2<DISCR, T>(G, nx, ny, nz, nm, rl, ul, vl, wl, pl, irl, rr, ur, vr, wr, pr, irr, flx);
+    double PL, PR;
+    recon_state<T, GT>(S, M, grad, c1, xf.x, xf.y, xf.z, rl, ul, vl, wl, pl, irl, PL);
+    recon_state<T, GT>(S, M, grad, c2, xf.x, xf.y, xf.z, rr, ur, vr, wr, pr, irr, PR);
+    if constexpr (MP)
+      face_flux2_mp<DISCR>(G, n, rl, ul, vl, wl, PL, irl, rr, ur, vr, wr, PR, irr, flx);
+    else
+      face_flux2<DISCR, T>(G, nx, ny, nz, nm, rl, ul, vl, wl, pl, irl, rr, ur, vr, wr, pr, irr, flx);
   }
 }
 
@@ -527,12 +778,12 @@ __global__ void k_face_flux(Mesh M, State S, Gas G, Cases C, const GT* __restric
   // both cells in one 8-byte load, before anything waits on either
   const int2 cc = M.f2c[f];
   if (SRAD) srad[f] = face_srad<ORDER>(M, S, f);
-  T flx[5];
+  double flx[5];
   if (C.cid) G = C.gas[C.cid[cc.x]];
   interior_face_flux<DISCR, ORDER, T, GT>(M, S, G, grad, f, cc, flx);
   const size_t FB = (size_t)M.F + M.B;
 #pragma unroll
-  for (int k = 0; k < 5; ++k) flux[k * FB + f] = FT(flx[k]);
+  for (int k = 0; k < 5; ++k) FluxIO<FT>::put(flux, FB, k, f, flx[k]);
 }
 
 // Face tiles of the second-order interior flux.  Faces are ordered by their lower cell,
@@ -607,19 +858,23 @@ __global__ __launch_bounds__(256) void k_face_flux_tile(Mesh M, State S, Gas G, 
                               lds[8 * cap + s1], lds[4 * cap + s2], lds[5 * cap + s2],
                               lds[6 * cap + s2], lds[8 * cap + s2], n);
     T rl, ul, vl, wl, pl, irl, rr, ur, vr, wr, pr, irr;
+    double PL, PR;
     recon_values<T>(lds[3 * cap + s1], lds[4 * cap + s1], lds[5 * cap + s1], lds[6 * cap + s1],
                     lds[7 * cap + s1], lds[s1], lds[cap + s1], lds[2 * cap + s1],
                     [&](int q) { return lg[(size_t)q * cap + s1]; }, xf.x, xf.y, xf.z, rl, ul,
-                    vl, wl, pl, irl);
+                    vl, wl, pl, irl, PL);
     recon_values<T>(lds[3 * cap + s2], lds[4 * cap + s2], lds[5 * cap + s2], lds[6 * cap + s2],
                     lds[7 * cap + s2], lds[s2], lds[cap + s2], lds[2 * cap + s2],
                     [&](int q) { return lg[(size_t)q * cap + s2]; }, xf.x, xf.y, xf.z, rr, ur,
-                    vr, wr, pr, irr);
-    T flx[5];
-    face_flux2<DISCR, T>(Gf, T(n.x), T(n.y), T(n.z), T(n.w), rl, ul, vl, wl, pl, irl, rr, ur, vr,
-                         wr, pr, irr, flx);
+                    vr, wr, pr, irr, PR);
+    double flx[5];
+    if constexpr (sizeof(T) == 4)
+      face_flux2_mp<DISCR>(Gf, n, rl, ul, vl, wl, PL, irl, rr, ur, vr, wr, PR, irr, flx);
+    else
+      face_flux2<DISCR, T>(Gf, T(n.x), T(n.y), T(n.z), T(n.w), rl, ul, vl, wl, pl, irl, rr, ur,
+                           vr, wr, pr, irr, flx);
 #pragma unroll
-    for (int k = 0; k < 5; ++k) flux[k * FB + f] = FT(flx[k]);
+    for (int k = 0; k < 5; ++k) FluxIO<FT>::put(flux, FB, k, f, flx[k]);
   }
 }
 
@@ -994,7 +1249,7 @@ __global__ void k_bface_flux(Mesh M, State S, Gas G, Cases C, Patches P,
   boundary_face_flux<ORDER, GT>(M, S, G, P, grad, b, flx);
   const size_t FB = (size_t)M.F + M.B;
 #pragma unroll
-  for (int k = 0; k < 5; ++k) flux[k * FB + M.F + b] = FT(flx[k]);
+  for (int k = 0; k < 5; ++k) FluxIO<FT>::put(flux, FB, k, M.F + b, flx[k]);
 }
 
 // Dependent variables p, T, c (libflu/RFLU_SetDependentVars.F90:255-284).
@@ -1024,12 +1279,12 @@ __device__ __forceinline__ void gather_residual(const Mesh& M, const FT* __restr
     int sg[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) sg[q] = q < cnt ? M.c2f_idx[e0 + q] : 1;
-    FT v[8][5];
+    typename FluxIO<FT>::val_t v[8][5];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const size_t f = (size_t)((sg[q] > 0 ? sg[q] : -sg[q]) - 1);
 #pragma unroll
-      for (int k = 0; k < 5; ++k) v[q][k] = q < cnt ? flux[k * FB + f] : FT(0);
+      for (int k = 0; k < 5; ++k) v[q][k] = q < cnt ? FluxIO<FT>::get(flux, FB, k, f) : 0;
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -1049,11 +1304,11 @@ __device__ __forceinline__ void gather_residual(const Mesh& M, const FT* __restr
     if (s > 0) {
       size_t f = (size_t)(s - 1);
 #pragma unroll
-      for (int k = 0; k < 5; ++k) acc[k] = acc[k] + double(flux[k * FB + f]);
+      for (int k = 0; k < 5; ++k) acc[k] = acc[k] + double(FluxIO<FT>::get(flux, FB, k, f));
     } else {
       size_t f = (size_t)(-s - 1);
 #pragma unroll
-      for (int k = 0; k < 5; ++k) acc[k] = acc[k] - double(flux[k * FB + f]);
+      for (int k = 0; k < 5; ++k) acc[k] = acc[k] - double(FluxIO<FT>::get(flux, FB, k, f));
     }
   }
 }
@@ -1063,7 +1318,8 @@ __device__ __forceinline__ void gather_residual(const Mesh& M, const FT* __restr
 // values and the order of the additions are those of gather_residual.
 template <typename FT>
 __device__ __forceinline__ void gather_residual_tile(const Mesh& M, const FT* __restrict__ flux,
-                                                     int i, int f0, int nf, const FT* lf,
+                                                     int i, int f0, int nf,
+                                                     const typename FluxIO<FT>::val_t* lf,
                                                      int fcap, double* acc) {
   const size_t FB = (size_t)M.F + M.B;
 #pragma unroll
@@ -1074,7 +1330,7 @@ __device__ __forceinline__ void gather_residual_tile(const Mesh& M, const FT* __
     int sg[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) sg[q] = q < cnt ? M.c2f_idx[e + q] : 1;
-    FT v[8][5];
+    typename FluxIO<FT>::val_t v[8][5];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int f = (sg[q] > 0 ? sg[q] : -sg[q]) - 1;
@@ -1084,7 +1340,7 @@ __device__ __forceinline__ void gather_residual_tile(const Mesh& M, const FT* __
         for (int k = 0; k < 5; ++k) v[q][k] = lf[k * fcap + (int)l];
       } else {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) v[q][k] = q < cnt ? flux[k * FB + (size_t)f] : FT(0);
+        for (int k = 0; k < 5; ++k) v[q][k] = q < cnt ? FluxIO<FT>::get(flux, FB, k, (size_t)f) : 0;
       }
     }
 #pragma unroll
@@ -1277,12 +1533,13 @@ __global__ void k_update_tile(Mesh M, State S, Gas G, Cases C, FaceTiles FTl,
   const int N = M.N;
   const int t = xcd_block(M.nxcd);
   const int f0 = FTl.fptr[t], nf = FTl.fptr[t + 1] - f0, fcap = FTl.fcap;
-  FT* lf = (FT*)lds;
+  using VT = typename FluxIO<FT>::val_t;
+  VT* lf = (VT*)lds;
   const size_t FB = (size_t)M.F + M.B;
   for (int k = threadIdx.x; k < nf; k += blockDim.x) {
-    FT v[5];
+    VT v[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) v[q] = flux[q * FB + (size_t)(f0 + k)];
+    for (int q = 0; q < 5; ++q) v[q] = FluxIO<FT>::get(flux, FB, q, (size_t)(f0 + k));
 #pragma unroll
     for (int q = 0; q < 5; ++q) lf[q * fcap + k] = v[q];
   }
@@ -2572,7 +2829,7 @@ __global__ void k_mass_inout(Mesh M, Cases C, Patches P, const FT* __restrict__ 
     for (int k = C.bface0[c] + threadIdx.x; k < C.bface0[c + 1]; k += blockDim.x) {
       int p = M.bpatch[k];
       if (P.bc[p] == BC_SLIPWALL) continue;   // inflow and outflow faces carry mass
-      double m = double(flux[M.F + k]);
+      double m = double(FluxIO<FT>::get(flux, (size_t)M.F + M.B, 0, (size_t)M.F + k));
       a = a - fmin(m, 0.0);
       b = b + fmax(m, 0.0);
     }

@@ -165,22 +165,90 @@ def test_time_step_matches_oracle(gpu):
     g.close()
 
 
-def test_mixed_precision_error_vs_fp64(gpu):
-    """FP32 gradients and fluxes, FP64 accumulation/state: per-variable error vs FP64."""
-    m = small_tet(n=10)
-    res = {}
-    for prec in (0, 1):
-        num = Numerics(discr=3, order=2, reconst=11, rk=2, cfl=0.5, dt_imposed=1e-3,
+def _mixed_vs_fp64(m, reconst, steps, floor=False):
+    """Final states of the FP64 and mixed runs (and, with floor, of an FP64 run whose
+    gradient sums in another order: the moment form) from the same Mach-2 shock with
+    1% density noise, HLLC order 2, SSP-RK3."""
+    out = {}
+    for key, prec, fast in (("fp64", 0, 0), ("mixed", 1, 0), ("reorder", 0, 1)):
+        if key == "reorder" and not floor:
+            continue
+        num = Numerics(discr=3, order=2, reconst=reconst, rk=4, cfl=0.5, dt_imposed=1e-3,
                        precision=prec)
         cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
         g = GpuRegion(m, num)
-        g.upload(cv0)
-        g.time_steps(20)
-        res[prec] = g.download()[0]
-        g.close()
-    err = rel_l2(res[1], res[0])
+        try:
+            if fast:
+                g.set_option(GpuRegion.OPT_GRAD_FAST, 1)
+            g.upload(cv0)
+            g.time_steps(steps)
+            out[key] = g.download()[0]
+        finally:
+            g.close()
+    return out
+
+
+@pytest.mark.parametrize("reconst,steps", [(0, 100), (2, 100), (11, 1)])
+def test_mixed_precision_error_vs_fp64(gpu, reconst, steps):
+    """The mixed mode (FP32 gradients and reconstruction increments, FP64 fluxes and
+    state) within 1e-6 relative L2 of the FP64 run in every conserved variable: after
+    100 steps without a modifier and with WENO-XYZ, after one step with the
+    Venkatakrishnan limiter."""
+    m = cases.tet_box(n=12, jitter=0.08, seed=2)
+    print(f"case: mixed vs FP64, 12^3 x 6, reconst={reconst}, {steps} steps", flush=True)
+    r = _mixed_vs_fp64(m, reconst, steps)
+    err = rel_l2(r["mixed"], r["fp64"])
     print("mixed vs FP64 rel L2 per variable:", err, flush=True)
-    assert np.all(err <= 1e-5), err
+    assert np.all(err <= 1e-6), err
+    assert np.any(err > 0.0)
+
+
+def test_mixed_precision_with_the_limiter_over_steps(gpu):
+    """With the Venkatakrishnan limiter, rounding-level differences in the gradients
+    grow with the steps in the momenta normal to the shock (they start at zero; the
+    limiter's face minimum and its additive TINY switch on differences of that size):
+    after 30 steps the mixed run stays within 1e-6 of the FP64 run in rho, rho u and
+    rho E and within 1e-4 in rho v and rho w.  An FP64 run whose gradient only sums in
+    another order (the moment form) is reported beside it."""
+    m = cases.tet_box(n=12, jitter=0.08, seed=2)
+    print("case: mixed vs FP64 and FP64 reorder, 12^3 x 6, Venkatakrishnan, 30 steps",
+          flush=True)
+    r = _mixed_vs_fp64(m, 11, 30, floor=True)
+    err = rel_l2(r["mixed"], r["fp64"])
+    flo = rel_l2(r["reorder"], r["fp64"])
+    print("mixed vs FP64:", err, "FP64 reorder:", flo, flush=True)
+    assert np.all(err[[0, 1, 4]] <= 1e-6), err
+    assert np.all(err[[2, 3]] <= 1e-4), err
+    assert np.all(flo > 0.0), flo
+
+
+def test_fp32_flux_split(gpu):
+    """The FP32-flux diagnostic split (FP64 gradients, FP32 interior-flux arithmetic with
+    the pressure terms in FP64, FP32 mass and energy fluxes stored beside FP64 momenta):
+    a closed box keeps its totals of rho and rho E to 1e-12 (each stored flux enters
+    both cells of its face), and rho, rho u and rho E stay within 1e-6 of the FP64 run
+    after 50 steps without a modifier."""
+    m = cases.tet_box(n=12, jitter=0.08, seed=2)
+    out = {}
+    for prec in (0, 3):
+        num = Numerics(discr=3, order=2, reconst=0, rk=4, cfl=0.5, dt_imposed=1e-3,
+                       precision=prec)
+        cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
+        print(f"case: 12^3 x 6, precision {prec}, 50 steps", flush=True)
+        g = GpuRegion(m, num)
+        try:
+            g.upload(cv0)
+            g.time_steps(50)
+            out[prec] = g.download()[0]
+        finally:
+            g.close()
+        for k in (0, 4):
+            t0, t = float(np.dot(cv0[:, k], m.vol)), float(np.dot(out[prec][:, k], m.vol))
+            assert abs(t - t0) <= 1e-12 * abs(t0), (prec, k, t0, t)
+    err = rel_l2(out[3], out[0])
+    print("FP32-flux split vs FP64:", err, flush=True)
+    assert np.all(err[[0, 1, 4]] <= 1e-6), err
+    assert np.all(err > 0.0), err
 
 
 def test_gpu_matches_reference_golden_gas_jump(gpu):
@@ -432,9 +500,9 @@ def test_full_size_closed_box_conserves(gpu, prec):
     """Config 2's mesh at full size (80^3 x 6 = 3,072,000 tets, jitter 8%) closed by slip
     walls, its Mach-2 shock with 1% density noise, HLLC order 2 + Venkatakrishnan,
     SSP-RK3: slip walls carry no mass or energy, so the totals of rho and rho E are
-    those of the initial state to round-off after 20 steps (FP64 state in both
-    precision modes; the mixed mode's FP32 face fluxes enter both cells of a face with
-    the same bits, so they cancel in the totals as well)."""
+    those of the initial state to round-off after 20 steps (FP64 state and fluxes in
+    both precision modes; each face flux enters both cells of the face with the same
+    bits, so the fluxes cancel in the totals)."""
     m = cases.tet_box(n=80)
     num = Numerics(discr=3, order=2, reconst=11, rk=4, cfl=0.5, dt_imposed=1e-3,
                    precision=prec)
