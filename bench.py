@@ -58,10 +58,12 @@ def algorithmic_bytes(m, num, kernel: str, tuning: dict, compulsory: bool = Fals
             b += 2 * S                          # 16-bit LDS slot of every member
         else:
             b += 4 * S                          # member indices
-        if tuning.get("grad_weights"):
+        if tuning.get("grad_weights") == 1:
             b += 3 * gb * S                     # stored stencil weights
         else:
             b += N * (32 + 48)                  # centroids, R-factor moments
+            if tuning.get("grad_weights") == 2:
+                b += gb * S                     # stored inverse distances
         if num.reconst == 11:
             b += 8 * N + 2 * F * (24 + 2)       # eps^2; cofg - fc and slot per face side
         return float(b)
@@ -96,7 +98,9 @@ def gradient_flops(m, num, tuning: dict) -> float:
     terms."""
     N, F = int(m.n_cells), int(m.n_faces)
     S = int(m.c2cs_idx.size)
-    per_member = 35 + (0 if tuning.get("grad_weights") else 31)
+    # weights stored: none formed; inverse distances stored: 24 of the 31 (no |d|^2,
+    # square root or quotient)
+    per_member = 35 + {1: 0, 2: 24}.get(tuning.get("grad_weights", 0), 31)
     if tuning.get("grad_fast"):
         per_member = 28              # moment form: |d|^2, reciprocal, 3 scalings, 15 FMAs
     fl = S * per_member + 30 * N
@@ -274,6 +278,38 @@ def rel_l2_per_variable(a, b):
     return [float(np.linalg.norm(a[:, k] - b[:, k]) / np.linalg.norm(b[:, k])) for k in range(5)]
 
 
+def persistent_vs_launches(b, cvs, args, n_tot):
+    """One time step of every case as one persistent launch (PERSISTENT: grid barriers
+    between the phases) against the same step as separate launches, both replayed as a
+    graph with the persistent step's kernel choices (256-cell gradient tiles with the
+    moment form, 128-cell face tiles); the final states compared bit for bit."""
+    from rocflu_amd import autotune
+    from rocflu_amd.solver import GpuRegion
+    autotune._set_tile(b, 256)
+    b.set_option(GpuRegion.OPT_GRAD_WEIGHTS, 0)
+    b.set_option(GpuRegion.OPT_FACE_TILE, 128)
+    b.set_option(GpuRegion.OPT_UPDATE_TILE, 1)
+    b.set_option(GpuRegion.OPT_GRAD_FAST, 1)
+    out, finals = {}, {}
+    for pers in (0, 1):
+        b.set_option(GpuRegion.OPT_PERSISTENT, pers)
+        b.upload(cvs)
+        b.time_steps(args.warmup, graph=True)
+        b.synchronize()
+        t1 = time.perf_counter()
+        b.time_steps(args.batch_steps, graph=True)
+        b.synchronize()
+        el = time.perf_counter() - t1
+        key = "persistent" if pers else "separate_launches"
+        out[key] = {"ms_per_step": 1e3 * el / args.batch_steps,
+                    "value": n_tot * 3 * args.batch_steps / el}
+        finals[pers] = b.download()[0]
+    b.set_option(GpuRegion.OPT_PERSISTENT, 0)
+    out["same_bits"] = bool(all(np.array_equal(x, y) for x, y in zip(finals[0], finals[1])))
+    print(f"# batch persistent: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def batch_leg(args):
     """The batched ensemble: independent shock-tube and blast cases of 20k-500k tets
     each, Mach 1.2-3.0, gamma 1.3-1.67, mixed precision HLLC at second order with
@@ -299,6 +335,7 @@ def batch_leg(args):
     b.synchronize()
     el = time.perf_counter() - t1
     st = [b.case_stats(i) for i in range(len(ens))]
+    pers = persistent_vs_launches(b, cvs, args, n_tot)
     b.close()
     # the same cases one region at a time
     t_seq = 0.0
@@ -319,6 +356,7 @@ def batch_leg(args):
            "sequential_value": n_upd / t_seq, "speedup_vs_sequential": t_seq / el,
            "dtype": "f32/f64", "failed_cases": sum(1 for x in st if x["n_invalid"] or x["n_negative"]),
            "tuning": {k: v for k, v in tuning.items() if k != "timings_ms"},
+           "persistent": pers,
            "workload": "64 Kuhn-tet boxes, cell counts U(20k, 500k), shock Mach U(1.2, 3.0), "
                        "gamma U(1.3, 1.67), planar shocks and blasts, HLLC order 2 + "
                        "Venkatakrishnan, SSP-RK3, mixed precision, slip walls"}

@@ -333,7 +333,9 @@ int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
  * records per tile, over how many XCDs the cell and face kernels spread contiguous
  * block ranges (1 = hardware order; 8 on MI355X), whether the stencil weights are
  * formed once and streamed from HBM (1; 3 weights per stencil entry, 24 B in FP64,
- * 12 B with FP32 gradients; needs LDS tiles) or re-formed in every gradient pass (0),
+ * 12 B with FP32 gradients; needs LDS tiles), only each member's inverse distance is
+ * (2; 8 B per entry in FP64, with 4 members per group; the rest of the weights formed
+ * in the pass) or all are re-formed in every gradient pass (0),
  * how many cells a tile holds (128, 256 or 512, one thread each; the default 256;
  * setting it rebuilds the tiles on the host), and whether the second-order interior
  * flux stages tiles of 64, 128 or 256 cells (with the cells their faces reach) in LDS
@@ -345,7 +347,12 @@ int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
  * (per member one reciprocal of |d|^2 and fused multiply-adds, the cell's inverse
  * moment matrix applied once; LDS tiles only, ahead of stored weights): the same least
  * squares gradient with other rounding, so results differ from the reference's in the
- * last bits (not part of the bit-identical set; 0 is the default). */
+ * last bits (not part of the bit-identical set; 0 is the default).  PERSISTENT 1 (a
+ * batch at order 2 with SSP-RK3, no modifier or the Venkatakrishnan limiter, 256-cell
+ * gradient tiles and face tiles) advances every case by one time step in one
+ * persistent launch whose resident blocks take the gradient tiles, face tiles,
+ * boundary faces, per-case time steps and update tiles of all cases phase by phase
+ * between grid barriers; the same bits as the separate launches (0, the default). */
 #define RFLU_OPT_GRAD_LDS 1
 #define RFLU_OPT_GRAD_UNROLL 2
 #define RFLU_OPT_TILE_CAP 3
@@ -355,6 +362,7 @@ int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
 #define RFLU_OPT_FACE_TILE 7
 #define RFLU_OPT_UPDATE_TILE 8
 #define RFLU_OPT_GRAD_FAST 9
+#define RFLU_OPT_PERSISTENT 10
 int rflu_gpu_set_option(int32_t handle, int32_t option, int32_t value);
 int rflu_gpu_get_option(int32_t handle, int32_t option, int32_t* value);
 

@@ -113,6 +113,8 @@ def gradient_candidates(g: GpuRegion, allow_weights: bool = True,
                 c.append(dict(grad_tile=ts, grad_lds=1, grad_unroll=u, grad_weights=0,
                               grad_fast=0))
             if allow_weights:
+                c.append(dict(grad_tile=ts, grad_lds=1, grad_unroll=4, grad_weights=2,
+                              grad_fast=0))
                 for u in (1, 2):
                     c.append(dict(grad_tile=ts, grad_lds=1, grad_unroll=u, grad_weights=1,
                                   grad_fast=0))

@@ -113,7 +113,9 @@ struct Region {
   unsigned short* tile_loc = nullptr;
   unsigned short* tile_vloc = nullptr;
   size_t tile_loc_len = 0;    // entries of the [member][lane] slot table
-  void* gw = nullptr;         // precomputed stencil weights [3][tile_loc_len] (GRAD_WEIGHTS)
+  void* gw = nullptr;         // GRAD_WEIGHTS 1: stencil weights [3][tile_loc_len];
+                              // 2: inverse distances [tile_loc_len]
+  int gw_kind = 0;
   int n_tiles = 0, tile_cap = 0;
   int grad_tile = 256;        // tuning: cells per LDS tile (128, 256 or 512)
   // face tiles of the second-order interior flux (FACE_TILE; 0 = one thread per face)
@@ -139,6 +141,13 @@ struct Region {
   bool graph_dirty = true;
   // kernel timing
   bool profiling = false;
+  // PERSISTENT: one launch per time step for a batch (k_step_persistent); its grid
+  // barrier words (count, generation, error) and the blocks that fit the device at once
+  bool persistent = false;
+  unsigned* pbar = nullptr;
+  int pers_blocks = 0;
+  size_t pers_shm = 0;
+  int pers_key = -1;
   struct Ev { int kid; hipEvent_t a, b; };
   std::vector<Ev> events;
   double kern_ms[KID_N] = {};
@@ -233,7 +242,7 @@ void free_region(Region* R) {
                   R->adj_seln, R->adj_tptr, R->adj_tidx, R->adj_wpre, R->adj_wsum,
                   R->adj_cptr, R->adj_cent, R->adj_ctptr, R->adj_ctrow, R->adj_ctent,
                   R->con_ecell, R->con_evar, R->con_rptr, R->con_rcell,
-                  R->con_rval, R->con_rw};
+                  R->con_rval, R->con_rw, R->pbar};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (Border& b : R->borders) {
@@ -560,10 +569,28 @@ Tiles region_tiles(const Region* R) {
   return T;
 }
 
+// The formed-weight tile kernel with the LDS stride capc (0: the runtime one).
+template <typename GT, bool V, int U, int TS, bool TW>
+void launch_gradient_lds(Region* R, const Mesh& M, const State& S, const Tiles& T, int capc,
+                         size_t shm) {
+  const GT* tw = TW ? (const GT*)R->gw : nullptr;
+#define RFLU_LDS(C)                                                                           \
+  hipLaunchKernelGGL((k_gradient_lds<GT, V, U, TS, C, TW>), dim3(R->n_tiles), dim3(TS), shm, \
+                     R->stream, M, S, T, tw, (GT*)R->grad, R->gate_sc())
+  switch (capc) {
+    case 1024: RFLU_LDS(1024); break;
+    case 1280: RFLU_LDS(1280); break;
+    case 1536: RFLU_LDS(1536); break;
+    case 2048: RFLU_LDS(2048); break;
+    default: RFLU_LDS(0); break;
+  }
+#undef RFLU_LDS
+}
+
 template <typename GT, bool V, int U, int TS>
 void launch_gradient_tiles(Region* R, const Mesh& M, const State& S) {
   Tiles T = region_tiles(R);
-  if (R->gw && !R->grad_fast) {
+  if (R->gw && R->gw_kind == 1 && !R->grad_fast) {
     size_t shm = (size_t)5 * R->tile_cap * sizeof(double);
     hipLaunchKernelGGL((k_gradient_ldsw<GT, V, 4 * U, TS>), dim3(R->n_tiles), dim3(TS), shm,
                        R->stream, M, S, T, R->tile_loc_len, (const GT*)R->gw, (GT*)R->grad, R->gate_sc());
@@ -592,28 +619,14 @@ void launch_gradient_tiles(Region* R, const Mesh& M, const State& S) {
       }
       return;
     }
-    switch (capc) {
-      case 1024:
-        hipLaunchKernelGGL((k_gradient_lds<GT, V, U, TS, 1024>), dim3(R->n_tiles), dim3(TS), shm,
-                           R->stream, M, S, T, (GT*)R->grad, R->gate_sc());
-        break;
-      case 1280:
-        hipLaunchKernelGGL((k_gradient_lds<GT, V, U, TS, 1280>), dim3(R->n_tiles), dim3(TS), shm,
-                           R->stream, M, S, T, (GT*)R->grad, R->gate_sc());
-        break;
-      case 1536:
-        hipLaunchKernelGGL((k_gradient_lds<GT, V, U, TS, 1536>), dim3(R->n_tiles), dim3(TS), shm,
-                           R->stream, M, S, T, (GT*)R->grad, R->gate_sc());
-        break;
-      case 2048:
-        hipLaunchKernelGGL((k_gradient_lds<GT, V, U, TS, 2048>), dim3(R->n_tiles), dim3(TS), shm,
-                           R->stream, M, S, T, (GT*)R->grad, R->gate_sc());
-        break;
-      default:
-        hipLaunchKernelGGL((k_gradient_lds<GT, V, U, TS, 0>), dim3(R->n_tiles), dim3(TS), shm,
-                           R->stream, M, S, T, (GT*)R->grad, R->gate_sc());
-        break;
+    // stored inverse distances (GRAD_WEIGHTS 2) with groups of 4 members
+    if constexpr (U == 4) {
+      if (R->gw && R->gw_kind == 2) {
+        launch_gradient_lds<GT, V, U, TS, true>(R, M, S, T, capc, shm);
+        return;
+      }
     }
+    launch_gradient_lds<GT, V, U, TS, false>(R, M, S, T, capc, shm);
   }
 }
 
@@ -862,10 +875,100 @@ int launch_time_step(Region* R, double time_in = -1.0, double end_time = HUGE_VA
 
 int nstages(int rk) { return rk == 1 ? 4 : rk == 2 ? 3 : rk == 3 ? 1 : 3; }
 
+// Whether the persistent step can run this region as it is configured (empty if so).
+std::string persistent_unsupported(const Region* R) {
+  if (R->K < 2) return "PERSISTENT needs a batch of cases";
+  if (R->order != 2 || R->rk != RFLU_RK3_SSP) return "PERSISTENT needs order 2 and SSP-RK3";
+  if (R->reconst != RFLU_RECONST_NONE && R->reconst != RFLU_RECONST_LIM_VENKAT)
+    return "PERSISTENT supports no modifier or the Venkatakrishnan limiter";
+  if (R->con_n > 0) return "PERSISTENT does not run constrained gradients";
+  if (R->prec == RFLU_PREC_FP32_FLUX) return "PERSISTENT runs FP64 or mixed precision";
+  if (!R->use_lds || !R->grad_lds || R->grad_tile != 256)
+    return "PERSISTENT needs 256-cell gradient tiles in LDS (GRAD_TILE 256, GRAD_LDS 1)";
+  if (R->face_tile <= 0 || 256 % R->face_tile) return "PERSISTENT needs face tiles (FACE_TILE)";
+  return "";
+}
+
+template <typename GT, int D, bool V, bool FAST>
+int launch_persistent_t(Region* R, const PersistentPlan& pl, const GridBar& gb, bool dry) {
+  auto kern = k_step_persistent<GT, D, V, FAST>;
+  const int key = (int)sizeof(GT) * 1000 + D * 100 + (V ? 10 : 0) + (FAST ? 1 : 0);
+  if (R->pers_key != key || R->pers_blocks <= 0) {
+    int per_cu = 0, dev = 0, ncu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern),
+                                                         256, R->pers_shm));
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (per_cu < 1) return fail(RFLU_ERR_ARG, "PERSISTENT: the step's LDS does not fit a CU");
+    R->pers_blocks = per_cu * ncu;
+    R->pers_key = key;
+  }
+  if (dry) return 0;
+  Mesh M = R->mesh();
+  State S = R->state();
+  FaceTiles FTl;
+  FTl.fptr = R->ft_fptr; FTl.eptr = R->ft_eptr; FTl.eidx = R->ft_eidx; FTl.fslot = R->ft_fslot;
+  FTl.ts = R->face_tile; FTl.cap = R->face_cap; FTl.fcap = R->face_fcap;
+  hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(1), 0, R->stream, gb);
+  hipLaunchKernelGGL(kern, dim3(R->pers_blocks), dim3(256), R->pers_shm, R->stream, M, S, R->gas,
+                     R->cases(), R->patches(), region_tiles(R), FTl, (GT*)R->grad,
+                     (double*)R->flux, R->srad, pl, gb);
+  return 0;
+}
+
+// One time step of a batch as one persistent launch (the phases of launch_step).  dry:
+// only size the grid (before a graph capture, which should see launches only).
+int launch_persistent(Region* R, bool dry = false) {
+  std::string why = persistent_unsupported(R);
+  if (!why.empty()) return fail(RFLU_ERR_ARG, why);
+  PersistentPlan pl;
+  pl.n_grad_tiles = R->n_tiles;
+  pl.n_face_tiles = R->n_face_tiles;
+  pl.upd_per_iter = 256 / R->face_tile;
+  pl.upd_lds_stride = (int)(5 * (size_t)R->face_fcap * sizeof(double));
+  pl.n_cell_blocks = (int)nblk(R->N, 256);
+  pl.flowthrough = R->has_flowthrough ? 1 : 0;
+  const bool g32 = grad32(R);
+  size_t shm = (size_t)8 * R->tile_cap * sizeof(double);
+  shm = std::max(shm, g32 ? face_tile_lds<float>(R->face_cap, true)
+                          : face_tile_lds<double>(R->face_cap, true));
+  shm = std::max(shm, (size_t)pl.upd_per_iter * pl.upd_lds_stride);
+  if (shm != R->pers_shm) {
+    R->pers_shm = shm;
+    R->pers_blocks = 0;
+  }
+  GridBar gb;
+  gb.count = R->pbar;
+  gb.gen = R->pbar + 1;
+  gb.err = (int*)(R->pbar + 2);
+  const bool v = R->reconst == RFLU_RECONST_LIM_VENKAT;
+#define RFLU_PERS(GT, D)                                                             \
+  (v ? (R->grad_fast ? launch_persistent_t<GT, D, true, true>(R, pl, gb, dry)            \
+                     : launch_persistent_t<GT, D, true, false>(R, pl, gb, dry))          \
+     : (R->grad_fast ? launch_persistent_t<GT, D, false, true>(R, pl, gb, dry)           \
+                     : launch_persistent_t<GT, D, false, false>(R, pl, gb, dry)))
+  int rc;
+  if (g32) rc = R->discr == 1 ? RFLU_PERS(float, 1) : R->discr == 3 ? RFLU_PERS(float, 3) : RFLU_PERS(float, 4);
+  else rc = R->discr == 1 ? RFLU_PERS(double, 1) : R->discr == 3 ? RFLU_PERS(double, 3) : RFLU_PERS(double, 4);
+#undef RFLU_PERS
+  return rc;
+}
+
+// After persistent steps: whether a grid barrier timed out (a read of the error word).
+int persistent_check(Region* R) {
+  if (!R->persistent || !R->pbar) return 0;
+  int err = 0;
+  HIP_TRY(hipMemcpyAsync(&err, R->pbar + 2, sizeof(int), hipMemcpyDeviceToHost, R->stream));
+  HIP_TRY(hipStreamSynchronize(R->stream));
+  if (err) return fail(RFLU_ERR_HIP, "persistent step: a grid barrier wait timed out");
+  return 0;
+}
+
 // One fused time step.  The time step only enters at the first update, so it is
 // reduced after the first stage's face pass, which also stores the face spectral
 // radii of the step-start state.
 int launch_step(Region* R) {
+  if (R->persistent) return launch_persistent(R);
   int rc;
   for (int s = 1; s <= nstages(R->rk); ++s) {
     if ((rc = launch_gradients(R))) return rc;
@@ -1859,6 +1962,7 @@ int rflu_gpu_time_steps(int32_t h, int64_t nsteps, int32_t use_graph) {
     return fail(RFLU_ERR_ARG, "a region with virtual cells advances through the per-callee "
                               "entry points and rflu_gpu_exchange");
   int rc;
+  if (R->persistent && (rc = launch_persistent(R, true))) return rc;
   if (use_graph && !R->profiling) {
     if (R->graph_dirty || !R->graph) {
       if (R->graph) { (void)hipGraphExecDestroy(R->graph); R->graph = nullptr; }
@@ -1879,6 +1983,7 @@ int rflu_gpu_time_steps(int32_t h, int64_t nsteps, int32_t use_graph) {
       if ((rc = launch_step(R))) return rc;
   }
   HIP_TRY(hipGetLastError());
+  if ((rc = persistent_check(R))) return rc;
   Scalars sc;
   if ((rc = read_scalars(R, &sc))) return rc;
   if (R->profiling) collect_events(R);
@@ -2015,17 +2120,26 @@ int rflu_gpu_time_kernel(int32_t h, int32_t which, int32_t reps, double* ms) {
 namespace {
 
 // Stencil weights of the tiles' [member][lane] layout, formed on the device.
-int form_weights(Region* R) {
+int form_weights(Region* R, int kind) {
   const size_t gsz = grad32(R) ? sizeof(float) : sizeof(double);
   const size_t L = R->tile_loc_len;
-  hipError_t e = hipMalloc(&R->gw, 3 * L * gsz + 16);
+  hipError_t e = hipMalloc(&R->gw, (kind == 2 ? 1 : 3) * L * gsz + 16);
   if (e != hipSuccess) {
     R->gw = nullptr;
+    R->gw_kind = 0;
     return fail(RFLU_ERR_NOMEM, std::string("hipMalloc stencil weights: ") + hipGetErrorString(e));
   }
+  R->gw_kind = kind;
   Mesh M = R->mesh();
   Tiles T = region_tiles(R);
-  if (grad32(R))
+  if (kind == 2) {
+    if (grad32(R))
+      hipLaunchKernelGGL((k_grad_terms<float>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream,
+                         M, T, (float*)R->gw);
+    else
+      hipLaunchKernelGGL((k_grad_terms<double>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream,
+                         M, T, (double*)R->gw);
+  } else if (grad32(R))
     hipLaunchKernelGGL((k_grad_weights<float>), dim3(nblk(R->N, 256)), dim3(256), 0, R->stream,
                        M, T, L, (float*)R->gw);
   else
@@ -2164,7 +2278,7 @@ int rebuild_tiles(Region* R, int ts) {
   if (R->gw) {
     HIP_TRY(hipFree(R->gw));
     R->gw = nullptr;
-    if ((rc = form_weights(R))) return rc;
+    if ((rc = form_weights(R, R->gw_kind))) return rc;
   }
   return 0;
 }
@@ -2219,19 +2333,18 @@ int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
       R->nxcd = value;
       break;
     case RFLU_OPT_GRAD_WEIGHTS: {
-      if (value != 0 && value != 1) return fail(RFLU_ERR_ARG, "GRAD_WEIGHTS is 0 or 1");
-      if (value == 0) {
-        if (R->gw) {
-          HIP_TRY(hipStreamSynchronize(R->stream));
-          HIP_TRY(hipFree(R->gw));
-          R->gw = nullptr;
-        }
-        break;
-      }
-      if (R->gw) break;
-      if (R->order != 2 || !R->use_lds)
+      if (value < 0 || value > 2) return fail(RFLU_ERR_ARG, "GRAD_WEIGHTS is 0, 1 or 2");
+      if (R->gw && R->gw_kind == value) break;
+      if (value != 0 && (R->order != 2 || !R->use_lds))
         return fail(RFLU_ERR_ARG, "GRAD_WEIGHTS needs a second-order region with LDS tiles");
-      int rc = form_weights(R);
+      if (R->gw) {
+        HIP_TRY(hipStreamSynchronize(R->stream));
+        HIP_TRY(hipFree(R->gw));
+        R->gw = nullptr;
+        R->gw_kind = 0;
+      }
+      if (value == 0) break;
+      int rc = form_weights(R, value);
       if (rc) return rc;
       break;
     }
@@ -2252,6 +2365,19 @@ int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
       if (value == R->face_tile) break;
       int rc = set_face_tiles(R, value);
       if (rc) return rc;
+      break;
+    }
+    case RFLU_OPT_PERSISTENT: {
+      if (value != 0 && value != 1) return fail(RFLU_ERR_ARG, "PERSISTENT is 0 or 1");
+      if (value == 1) {
+        std::string why = persistent_unsupported(R);
+        if (!why.empty()) return fail(RFLU_ERR_ARG, why);
+        if (!R->pbar) {
+          if (int rc = dalloc(&R->pbar, 4)) return rc;
+          HIP_TRY(hipMemset(R->pbar, 0, 4 * sizeof(unsigned)));
+        }
+      }
+      R->persistent = value == 1;
       break;
     }
     case RFLU_OPT_GRAD_TILE: {
@@ -2279,11 +2405,12 @@ int rflu_gpu_get_option(int32_t h, int32_t option, int32_t* value) {
     case RFLU_OPT_GRAD_UNROLL: *value = R->grad_unroll; break;
     case RFLU_OPT_TILE_CAP: *value = R->tile_cap; break;
     case RFLU_OPT_XCD: *value = R->nxcd; break;
-    case RFLU_OPT_GRAD_WEIGHTS: *value = R->gw ? 1 : 0; break;
+    case RFLU_OPT_GRAD_WEIGHTS: *value = R->gw ? R->gw_kind : 0; break;
     case RFLU_OPT_GRAD_TILE: *value = R->grad_tile; break;
     case RFLU_OPT_FACE_TILE: *value = R->face_tile; break;
     case RFLU_OPT_UPDATE_TILE: *value = (R->update_tile && R->face_tile > 0) ? 1 : 0; break;
     case RFLU_OPT_GRAD_FAST: *value = (R->grad_fast && R->use_lds && R->grad_lds) ? 1 : 0; break;
+    case RFLU_OPT_PERSISTENT: *value = R->persistent ? 1 : 0; break;
     default: return fail(RFLU_ERR_ARG, "unknown option");
   }
   return RFLU_OK;

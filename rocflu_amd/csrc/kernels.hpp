@@ -810,15 +810,11 @@ struct FaceTiles {
 // busy nearly all the time); here only the records of cells outside the tile are
 // gathered.
 template <int DISCR, typename T, typename GT, typename FT, bool SRAD>
-__global__ __launch_bounds__(256) void k_face_flux_tile(Mesh M, State S, Gas G, Cases C,
+__device__ __forceinline__ void face_tile_body(Mesh M, State S, Gas G, Cases C,
                                                         FaceTiles FTl, const GT* __restrict__ grad,
                                                         FT* __restrict__ flux,
-                                                        double* __restrict__ srad,
-                                                        const Scalars* __restrict__ sc) {
-  extern __shared__ double lds[];
-  if (gate(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
+                                                        double* __restrict__ srad, int t, double* lds) {
   const int N = M.N, cap = FTl.cap;
-  const int t = xcd_block(M.nxcd);
   const int c0 = t * FTl.ts;
   const int nown = min(FTl.ts, N - c0);
   const int e0 = FTl.eptr[t], nrec = nown + (FTl.eptr[t + 1] - e0);
@@ -877,6 +873,18 @@ __global__ __launch_bounds__(256) void k_face_flux_tile(Mesh M, State S, Gas G, 
     for (int k = 0; k < 5; ++k) FluxIO<FT>::put(flux, FB, k, f, flx[k]);
   }
 }
+
+template <int DISCR, typename T, typename GT, typename FT, bool SRAD>
+__global__ __launch_bounds__(256) void k_face_flux_tile(Mesh M, State S, Gas G, Cases C,
+                                                        FaceTiles FTl, const GT* __restrict__ grad,
+                                                        FT* __restrict__ flux,
+                                                        double* __restrict__ srad,
+                                                        const Scalars* __restrict__ sc) {
+  extern __shared__ double lds[];
+  if (gate(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
+  face_tile_body<DISCR, T, GT, FT, SRAD>(M, S, G, C, FTl, grad, flux, srad, xcd_block(M.nxcd), lds);
+}
+
 
 // Outflow boundary state (libfloflu/BcondOutflowPerf.F90:111-157), FP64.
 __device__ __forceinline__ void bcond_outflow(int bcOpt, double pout, double sxn, double syn,
@@ -1233,12 +1241,10 @@ __device__ __forceinline__ void boundary_face_flux(const Mesh& M, const State& S
 }
 
 template <int ORDER, typename GT, typename FT, bool SRAD>
-__global__ void k_bface_flux(Mesh M, State S, Gas G, Cases C, Patches P,
-                             const GT* __restrict__ grad, FT* __restrict__ flux,
-                             double* __restrict__ srad, const Scalars* __restrict__ sc) {
-  if (gate(sc)) return;
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= M.B) return;
+__device__ __forceinline__ void bface_body(const Mesh& M, const State& S, Gas G, const Cases& C,
+                                           const Patches& P, const GT* __restrict__ grad,
+                                           FT* __restrict__ flux, double* __restrict__ srad,
+                                           int b) {
   if (C.cid) {
     const int c = C.cid[M.bf2c[b]];
     G = C.gas[c];
@@ -1250,6 +1256,16 @@ __global__ void k_bface_flux(Mesh M, State S, Gas G, Cases C, Patches P,
   const size_t FB = (size_t)M.F + M.B;
 #pragma unroll
   for (int k = 0; k < 5; ++k) FluxIO<FT>::put(flux, FB, k, M.F + b, flx[k]);
+}
+
+template <int ORDER, typename GT, typename FT, bool SRAD>
+__global__ void k_bface_flux(Mesh M, State S, Gas G, Cases C, Patches P,
+                             const GT* __restrict__ grad, FT* __restrict__ flux,
+                             double* __restrict__ srad, const Scalars* __restrict__ sc) {
+  if (gate(sc)) return;
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= M.B) return;
+  bface_body<ORDER, GT, FT, SRAD>(M, S, G, C, P, grad, flux, srad, b);
 }
 
 // Dependent variables p, T, c (libflu/RFLU_SetDependentVars.F90:255-284).
@@ -1524,19 +1540,22 @@ __global__ void k_update(Mesh M, State S, Gas G, Cases C, const FT* __restrict__
 // first copies the fluxes of the tile's own faces (one contiguous range) into LDS with
 // consecutive loads, so that only the faces a cell shares with an earlier tile, and
 // its boundary faces, are gathered from memory.  Same arithmetic, same bits.
+// Tile t's part of k_update_tile for the lanes [0, nl) of a block (lane = this thread's):
+// copies the tile's own face fluxes into lf, waits for the whole block (every thread of
+// the block calls it once, valid or not), then updates the tile's cells.
 template <int RK, typename FT, bool FUSED, bool WRITE_PRIM>
-__global__ void k_update_tile(Mesh M, State S, Gas G, Cases C, FaceTiles FTl,
-                              const FT* __restrict__ flux, Scalars* sc, int stage,
-                              int last_stage, double dt_in) {
-  extern __shared__ double lds[];
-  if (!C.cid && halted(sc)) return;  // uniform over the grid
+__device__ __forceinline__ void update_tile_body(const Mesh& M, const State& S, Gas G,
+                                                 const Cases& C, const FaceTiles& FTl,
+                                                 const FT* __restrict__ flux, Scalars* sc,
+                                                 int stage, int last_stage, double dt_in, int t,
+                                                 int lane, int nl, bool valid, void* lds) {
   const int N = M.N;
-  const int t = xcd_block(M.nxcd);
-  const int f0 = FTl.fptr[t], nf = FTl.fptr[t + 1] - f0, fcap = FTl.fcap;
+  const int f0 = valid ? FTl.fptr[t] : 0, nf = valid ? FTl.fptr[t + 1] - f0 : 0;
+  const int fcap = FTl.fcap;
   using VT = typename FluxIO<FT>::val_t;
   VT* lf = (VT*)lds;
   const size_t FB = (size_t)M.F + M.B;
-  for (int k = threadIdx.x; k < nf; k += blockDim.x) {
+  for (int k = lane; k < nf; k += nl) {
     VT v[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) v[q] = FluxIO<FT>::get(flux, FB, q, (size_t)(f0 + k));
@@ -1544,7 +1563,8 @@ __global__ void k_update_tile(Mesh M, State S, Gas G, Cases C, FaceTiles FTl,
     for (int q = 0; q < 5; ++q) lf[q * fcap + k] = v[q];
   }
   __syncthreads();
-  const int i = t * FTl.ts + (int)threadIdx.x;
+  if (!valid) return;
+  const int i = t * FTl.ts + lane;
   if (i >= M.nact) return;
   int first = 0;
   if (C.cid) {
@@ -1583,6 +1603,17 @@ __global__ void k_update_tile(Mesh M, State S, Gas G, Cases C, FaceTiles FTl,
   }
 #pragma unroll
   for (int k = 0; k < 5; ++k) S.cv[(size_t)k * N + i] = q[k];
+}
+
+template <int RK, typename FT, bool FUSED, bool WRITE_PRIM>
+__global__ void k_update_tile(Mesh M, State S, Gas G, Cases C, FaceTiles FTl,
+                              const FT* __restrict__ flux, Scalars* sc, int stage,
+                              int last_stage, double dt_in) {
+  extern __shared__ double lds[];
+  if (!C.cid && halted(sc)) return;  // uniform over the grid
+  update_tile_body<RK, FT, FUSED, WRITE_PRIM>(M, S, G, C, FTl, flux, sc, stage, last_stage, dt_in,
+                                              xcd_block(M.nxcd), (int)threadIdx.x,
+                                              (int)blockDim.x, true, lds);
 }
 
 // Dependent variables (and the primitive copy used by second-order kernels).
@@ -1760,12 +1791,18 @@ __device__ __forceinline__ GradCoef<T> grad_coef(const Mesh& M, int i) {
   return c;
 }
 
-// Weights of one stencil member (xj - xi) and its contribution (var_j - var_i).
+// Inverse distance of one stencil member, the reference's 1/SQRT(dx^2 + dy^2 + dz^2).
 template <typename T>
-__device__ __forceinline__ void grad_weights(const GradCoef<T>& c, double ddx, double ddy,
-                                             double ddz, T* w) {
+__device__ __forceinline__ T grad_term(double ddx, double ddy, double ddz) {
   T dx = T(ddx), dy = T(ddy), dz = T(ddz);
-  T term = inv_sqrt_cr(dx * dx + dy * dy + dz * dz);
+  return inv_sqrt_cr(dx * dx + dy * dy + dz * dz);
+}
+
+// Weights of one stencil member (xj - xi) with its inverse distance term.
+template <typename T>
+__device__ __forceinline__ void grad_weights_t(const GradCoef<T>& c, T term, double ddx,
+                                               double ddy, double ddz, T* w) {
+  T dx = T(ddx), dy = T(ddy), dz = T(ddz);
   dx = term * dx;
   dy = term * dy;
   dz = term * dz;
@@ -1775,6 +1812,13 @@ __device__ __forceinline__ void grad_weights(const GradCoef<T>& c, double ddx, d
   w[0] = term * (term1 + c.c12 * term2 + c.c13 * term3);
   w[1] = term * (term2 + c.c23 * term3);
   w[2] = term * (term3);
+}
+
+// Weights of one stencil member (xj - xi) and its contribution (var_j - var_i).
+template <typename T>
+__device__ __forceinline__ void grad_weights(const GradCoef<T>& c, double ddx, double ddy,
+                                             double ddz, T* w) {
+  grad_weights_t<T>(c, grad_term<T>(ddx, ddy, ddz), ddx, ddy, ddz, w);
 }
 
 template <typename T>
@@ -1957,9 +2001,33 @@ struct Tiles {
   int cap;
 };
 
+// LDS records of the formed-weight gradient tiles (centroid and primitive state): four
+// arrays of 16-byte pairs (x,y) (z,rho) (u,v) (w,p), each read with one 128-bit LDS
+// load (against eight 64-bit arrays: same FP64 time, 17% less with FP32 gradients,
+// whose member loop waits on LDS more).
+__device__ __forceinline__ void rec_store(double* lds, int cap, int k, double4 c, const double* v) {
+  double2* lp = (double2*)lds;
+  lp[k] = make_double2(c.x, c.y);
+  lp[cap + k] = make_double2(c.z, v[0]);
+  lp[2 * cap + k] = make_double2(v[1], v[2]);
+  lp[3 * cap + k] = make_double2(v[3], v[4]);
+}
+__device__ __forceinline__ void rec_load(const double* lds, int cap, int k, double& x, double& y,
+                                         double& z, double* v) {
+  const double2* lp = (const double2*)lds;
+  const double2 a = lp[k], b = lp[cap + k], c = lp[2 * cap + k], d = lp[3 * cap + k];
+  x = a.x; y = a.y; z = b.x;
+  v[0] = b.y; v[1] = c.x; v[2] = c.y; v[3] = d.x; v[4] = d.y;
+}
+__device__ __forceinline__ void rec_vals(const double* lds, int cap, int k, double* v) {
+  const double2* lp = (const double2*)lds;
+  const double2 b = lp[cap + k], c = lp[2 * cap + k], d = lp[3 * cap + k];
+  v[0] = b.y; v[1] = c.x; v[2] = c.y; v[3] = d.x; v[4] = d.y;
+}
+
 // Stages a tile's union records in LDS: each thread issues the loads of up to four
-// records before writing any of them, so the gathers overlap.  Layout: x, y, z (with
-// COORDS) then rho, u, v, w, p, each an array of cap doubles.
+// records before writing any of them, so the gathers overlap.  Layout: with COORDS the
+// pair arrays of rec_store, otherwise rho, u, v, w, p, each an array of cap doubles.
 template <bool COORDS>
 __device__ __forceinline__ void stage_tile(const Mesh& M, const State& S, const int* uidx, int nu,
                                            int cap, double* lds) {
@@ -1986,31 +2054,26 @@ __device__ __forceinline__ void stage_tile(const Mesh& M, const State& S, const 
       if (jj[r] < 0) continue;
       const int k = k0 + r * (int)blockDim.x;
       if (COORDS) {
-        lds[k] = cc[r].x;
-        lds[cap + k] = cc[r].y;
-        lds[2 * cap + k] = cc[r].z;
-      }
+        rec_store(lds, cap, k, cc[r], pp[r]);
+      } else {
 #pragma unroll
-      for (int q = 0; q < 5; ++q) lv[q * cap + k] = pp[r][q];
+        for (int q = 0; q < 5; ++q) lv[q * cap + k] = pp[r][q];
+      }
     }
   }
 }
 
 // CAPC > 0: the LDS arrays' stride is that compile-time constant (>= T.cap), so the
 // arrays' offsets become immediates of the LDS loads instead of per-member additions.
-template <typename GT, bool VENKAT, int U, int TS, int CAPC>
-__global__ __launch_bounds__(TS) void k_gradient_lds(Mesh M, State S, Tiles T,
-                                                      GT* __restrict__ grad,
-                                                      const Scalars* __restrict__ sc) {
-  extern __shared__ double lds[];
-  if (gate(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
+// TW: each member's inverse distance comes from tw ([member][lane], formed once per
+// region by k_grad_terms with the same arithmetic) instead of its square root and
+// quotient, loaded one group ahead like the slot numbers; same bits.
+template <typename GT, bool VENKAT, int U, int TS, int CAPC, bool TW>
+__device__ __forceinline__ void grad_lds_body(Mesh M, State S, Tiles T,
+                                                      const GT* __restrict__ tw,
+                                                      GT* __restrict__ grad, int t, double* lds) {
   const int N = M.N;
   const int cap = CAPC > 0 ? CAPC : T.cap;
-  const double* lx = lds;
-  const double* ly = lds + cap;
-  const double* lz = lds + 2 * cap;
-  const double* lv = lds + 3 * cap;  // 5 arrays: rho, u, v, w, p
-  const int t = xcd_block(M.nxcd);
   // the cell's own data and first slot numbers are requested before the staging
   const int i = t * T.ts + (int)threadIdx.x;
   const bool live = i < N;
@@ -2021,11 +2084,17 @@ __global__ __launch_bounds__(TS) void k_gradient_lds(Mesh M, State S, Tiles T,
 #pragma unroll
   for (int k = 0; k < 5; ++k) vi[k] = S.prim[(size_t)k * N + ic];
   const int cnt = live ? M.sten_ptr[ic + 1] - M.sten_ptr[ic] : 0;
-  const unsigned short* L = T.loc + T.lptr[t] + threadIdx.x;
-  // Slot numbers are loaded one group of U members ahead of their use.
+  const size_t lbase = (size_t)T.lptr[t] + threadIdx.x;
+  const unsigned short* L = T.loc + lbase;
+  const GT* TWp = TW ? tw + lbase : nullptr;
+  // Slot numbers (and stored terms) are loaded one group of U members ahead of their use.
   int nk[U];
+  GT nt[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u) nk[u] = u < cnt ? (int)L[(size_t)u * T.ts] : 0;
+  for (int u = 0; u < U; ++u) {
+    nk[u] = u < cnt ? (int)L[(size_t)u * T.ts] : 0;
+    if (TW) nt[u] = u < cnt ? TWp[(size_t)u * T.ts] : GT(1);
+  }
   const int u0 = T.uptr[t];
   stage_tile<true>(M, S, T.uidx + u0, T.uptr[t + 1] - u0, cap, lds);
   __syncthreads();
@@ -2038,20 +2107,29 @@ __global__ __launch_bounds__(TS) void k_gradient_lds(Mesh M, State S, Tiles T,
   // accumulated one member after the other in stencil order.
   for (; s + U <= cnt; s += U) {
     int kk[U];
+    GT tt[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       kk[u] = nk[u];
       int sn = s + U + u;
       nk[u] = sn < cnt ? (int)L[(size_t)sn * T.ts] : 0;
+      if (TW) {
+        tt[u] = nt[u];
+        nt[u] = sn < cnt ? TWp[(size_t)sn * T.ts] : GT(1);
+      }
     }
     GT w[U][3];
     double dvar[U][5];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       int k = kk[u];
-      grad_weights<GT>(c, lx[k] - ci.x, ly[k] - ci.y, lz[k] - ci.z, w[u]);
+      double xj, yj, zj, vj[5];
+      rec_load(lds, cap, k, xj, yj, zj, vj);
+      const double dx = xj - ci.x, dy = yj - ci.y, dz = zj - ci.z;
+      if (TW) grad_weights_t<GT>(c, tt[u], dx, dy, dz, w[u]);
+      else grad_weights<GT>(c, dx, dy, dz, w[u]);
 #pragma unroll
-      for (int q = 0; q < 5; ++q) dvar[u][q] = lv[q * cap + k] - vi[q];
+      for (int q = 0; q < 5; ++q) dvar[u][q] = vj[q] - vi[q];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) grad_accum<GT>(w[u], dvar[u], g);
@@ -2060,10 +2138,15 @@ __global__ __launch_bounds__(TS) void k_gradient_lds(Mesh M, State S, Tiles T,
   for (int u = 0; u < U; ++u) {
     if (s + u < cnt) {
       int k = nk[u];
-      double dvar[5];
+      double dvar[5], xj, yj, zj, vj[5];
+      rec_load(lds, cap, k, xj, yj, zj, vj);
 #pragma unroll
-      for (int q = 0; q < 5; ++q) dvar[q] = lv[q * cap + k] - vi[q];
-      grad_member<GT>(c, lx[k] - ci.x, ly[k] - ci.y, lz[k] - ci.z, dvar, g);
+      for (int q = 0; q < 5; ++q) dvar[q] = vj[q] - vi[q];
+      const double dx = xj - ci.x, dy = yj - ci.y, dz = zj - ci.z;
+      GT w[3];
+      if (TW) grad_weights_t<GT>(c, nt[u], dx, dy, dz, w);
+      else grad_weights<GT>(c, dx, dy, dz, w);
+      grad_accum<GT>(w, dvar, g);
     }
   }
   if (VENKAT) {
@@ -2073,14 +2156,22 @@ __global__ __launch_bounds__(TS) void k_gradient_lds(Mesh M, State S, Tiles T,
                        int k = V[(size_t)q * T.ts];
                        return k == 0xFFFF ? -1 : k;
                      },
-                     [&](int k, double* vo) {
-#pragma unroll
-                       for (int r = 0; r < 5; ++r) vo[r] = lv[r * cap + k];
-                     });
+                     [&](int k, double* vo) { rec_vals(lds, cap, k, vo); });
   }
 #pragma unroll
   for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
 }
+
+template <typename GT, bool VENKAT, int U, int TS, int CAPC, bool TW>
+__global__ __launch_bounds__(TS) void k_gradient_lds(Mesh M, State S, Tiles T,
+                                                      const GT* __restrict__ tw,
+                                                      GT* __restrict__ grad,
+                                                      const Scalars* __restrict__ sc) {
+  extern __shared__ double lds[];
+  if (gate(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
+  grad_lds_body<GT, VENKAT, U, TS, CAPC, TW>(M, S, T, tw, grad, xcd_block(M.nxcd), lds);
+}
+
 
 // Moment form of the same least-squares gradient.  The member loop's weights are
 // w_j = term^2 (A d_j) with term = 1/|d_j|, d_j = x_j - x_i and A the cell's symmetric
@@ -2151,18 +2242,10 @@ __device__ __forceinline__ void mom_apply(const MomMat<T>& a, T* g) {
 // The LDS-tile gradient (stage_tile, slot table, Venkatakrishnan limiter as in
 // k_gradient_lds) with the moment form of the member loop; U members per group.
 template <typename GT, bool VENKAT, int U, int TS, int CAPC>
-__global__ __launch_bounds__(TS) void k_gradient_mom(Mesh M, State S, Tiles T,
-                                                      GT* __restrict__ grad,
-                                                      const Scalars* __restrict__ sc) {
-  extern __shared__ double lds[];
-  if (gate(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
+__device__ __forceinline__ void grad_mom_body(Mesh M, State S, Tiles T,
+                                                      GT* __restrict__ grad, int t, double* lds) {
   const int N = M.N;
   const int cap = CAPC > 0 ? CAPC : T.cap;
-  const double* lx = lds;
-  const double* ly = lds + cap;
-  const double* lz = lds + 2 * cap;
-  const double* lv = lds + 3 * cap;  // 5 arrays: rho, u, v, w, p
-  const int t = xcd_block(M.nxcd);
   const int i = t * T.ts + (int)threadIdx.x;
   const bool live = i < N;
   const int ic = live ? i : N - 1;
@@ -2195,9 +2278,13 @@ __global__ __launch_bounds__(TS) void k_gradient_mom(Mesh M, State S, Tiles T,
     for (int u = 0; u < U; ++u) {
       const int k = kk[u];
       double dvar[5];
+      {
+        double xj, yj, zj, vj[5];
+        rec_load(lds, cap, k, xj, yj, zj, vj);
 #pragma unroll
-      for (int q = 0; q < 5; ++q) dvar[q] = lv[q * cap + k] - vi[q];
-      mom_member<GT>(lx[k] - ci.x, ly[k] - ci.y, lz[k] - ci.z, dvar, g);
+        for (int q = 0; q < 5; ++q) dvar[q] = vj[q] - vi[q];
+        mom_member<GT>(xj - ci.x, yj - ci.y, zj - ci.z, dvar, g);
+      }
     }
   }
 #pragma unroll
@@ -2205,9 +2292,13 @@ __global__ __launch_bounds__(TS) void k_gradient_mom(Mesh M, State S, Tiles T,
     if (s + u < cnt) {
       const int k = nk[u];
       double dvar[5];
+      {
+        double xj, yj, zj, vj[5];
+        rec_load(lds, cap, k, xj, yj, zj, vj);
 #pragma unroll
-      for (int q = 0; q < 5; ++q) dvar[q] = lv[q * cap + k] - vi[q];
-      mom_member<GT>(lx[k] - ci.x, ly[k] - ci.y, lz[k] - ci.z, dvar, g);
+        for (int q = 0; q < 5; ++q) dvar[q] = vj[q] - vi[q];
+        mom_member<GT>(xj - ci.x, yj - ci.y, zj - ci.z, dvar, g);
+      }
     }
   }
   mom_apply<GT>(mom_mat<GT>(M, i), g);
@@ -2218,14 +2309,21 @@ __global__ __launch_bounds__(TS) void k_gradient_mom(Mesh M, State S, Tiles T,
                        int k = V[(size_t)q * T.ts];
                        return k == 0xFFFF ? -1 : k;
                      },
-                     [&](int k, double* vo) {
-#pragma unroll
-                       for (int r = 0; r < 5; ++r) vo[r] = lv[r * cap + k];
-                     });
+                     [&](int k, double* vo) { rec_vals(lds, cap, k, vo); });
   }
 #pragma unroll
   for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
 }
+
+template <typename GT, bool VENKAT, int U, int TS, int CAPC>
+__global__ __launch_bounds__(TS) void k_gradient_mom(Mesh M, State S, Tiles T,
+                                                      GT* __restrict__ grad,
+                                                      const Scalars* __restrict__ sc) {
+  extern __shared__ double lds[];
+  if (gate(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
+  grad_mom_body<GT, VENKAT, U, TS, CAPC>(M, S, T, grad, xcd_block(M.nxcd), lds);
+}
+
 
 // Stencil weights of every (cell, member) pair, formed once per region with the same
 // arithmetic as the member loop (the weights depend on the grid only).  Stored in
@@ -2248,6 +2346,23 @@ __global__ void k_grad_weights(Mesh M, Tiles T, size_t L, GT* __restrict__ w) {
     w[o] = ww[0];
     w[L + o] = ww[1];
     w[2 * L + o] = ww[2];
+  }
+}
+
+// Inverse distances of every (cell, member) pair in [member][lane] order of the tiles
+// (the stored terms of k_gradient_lds<..., TW = true>).
+template <typename GT>
+__global__ void k_grad_terms(Mesh M, Tiles T, GT* __restrict__ tw) {
+  const int N = M.N;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int t = i / T.ts, lane = i - t * T.ts;
+  double4 ci = M.cell[i];
+  const size_t base = (size_t)T.lptr[t] + lane;
+  const int s0 = M.sten_ptr[i], s1 = M.sten_ptr[i + 1];
+  for (int s = s0; s < s1; ++s) {
+    double4 cj = M.cell[M.sten_idx[s]];
+    tw[base + (size_t)(s - s0) * T.ts] = grad_term<GT>(cj.x - ci.x, cj.y - ci.y, cj.z - ci.z);
   }
 }
 
@@ -2604,20 +2719,31 @@ __global__ void k_weno_clip(Mesh M, State S, const GT* __restrict__ gin, GT* __r
   typedef GT T;
   const T eps = T(1.0E-15);
   int s0 = M.sten_ptr[i], s1 = M.sten_ptr[i + 1];
-  T g[15];
+  // members outermost: each member's index is loaded once and its 15 values are
+  // requested together; every component still sums its members in stencil order
+  T sumw[15], loc[15], g[15];
 #pragma unroll
   for (int k = 0; k < 15; ++k) {
     T term = gin[(size_t)k * N + i];
-    T sumw = T(1) / (term * term + eps);
-    T loc = sumw * term;
-    for (int s = s0; s < s1; ++s) {
-      T t = gin[(size_t)k * N + M.sten_idx[s]];
-      T w = T(1) / (t * t + eps);
-      sumw = sumw + w;
-      loc = loc + w * t;
-    }
-    g[k] = loc / sumw;
+    sumw[k] = T(1) / (term * term + eps);
+    loc[k] = sumw[k] * term;
   }
+  int jn = s0 < s1 ? M.sten_idx[s0] : 0;
+  for (int s = s0; s < s1; ++s) {
+    const int j = jn;
+    if (s + 1 < s1) jn = M.sten_idx[s + 1];
+    T t[15];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) t[k] = gin[(size_t)k * N + j];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) {
+      T w = T(1) / (t[k] * t[k] + eps);
+      sumw[k] = sumw[k] + w;
+      loc[k] = loc[k] + w * t[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 15; ++k) g[k] = loc[k] / sumw[k];
   double4 ci = M.cell[i];
   const int pos[2] = {0, 4};
 #pragma unroll
@@ -2764,11 +2890,12 @@ __global__ void k_dt_finalize(const double* __restrict__ part, int nparts, Scala
 // belong to one case (all but the few straddling a case boundary) reduce first, then
 // the block merges its wavefronts, so a block issues one atomic per case it covers.
 // Each case's slot sits on its own 256-byte line (DTMIN_STRIDE doubles apart).
-__global__ void k_dt_gather_cases(Mesh M, Cases C, const double* __restrict__ srad) {
+__device__ __forceinline__ void dt_gather_cases_body(const Mesh& M, const Cases& C,
+                                                     const double* __restrict__ srad, int blk) {
   __shared__ int wc[16];
   __shared__ double wm[16];
   const int N = M.N;
-  int i = xcd_block(M.nxcd) * blockDim.x + threadIdx.x;
+  int i = blk * blockDim.x + threadIdx.x;
   int c = -1;
   double dtc = HUGE_VAL;
   if (i < N) {
@@ -2806,12 +2933,20 @@ __global__ void k_dt_gather_cases(Mesh M, Cases C, const double* __restrict__ sr
   }
 }
 
-__global__ void k_dt_finalize_cases(Cases C) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C.K) return;
+__global__ void k_dt_gather_cases(Mesh M, Cases C, const double* __restrict__ srad) {
+  dt_gather_cases_body(M, C, srad, xcd_block(M.nxcd));
+}
+
+__device__ __forceinline__ void dt_finalize_case(const Cases& C, int c) {
   unsigned long long bits = C.dtmin[(size_t)c * DTMIN_STRIDE];
   C.dtmin[(size_t)c * DTMIN_STRIDE] = 0x7FF0000000000000ULL;   // +inf for the next step
   finalize_dt(C.sc + c, __longlong_as_double((long long)bits), C.par[c], -1.0);
+}
+
+__global__ void k_dt_finalize_cases(Cases C) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C.K) return;
+  dt_finalize_case(C, c);
 }
 
 // Outflow mass bookkeeping of stage 1 (rocflu/RFLU_CentralFirstPatch.F90:697-700):
@@ -2819,9 +2954,9 @@ __global__ void k_dt_finalize_cases(Cases C) {
 // The fused step launches it after its time step is set, so a case that has finished
 // (or failed) counts no mass for a step it does not take.
 template <typename FT>
-__global__ void k_mass_inout(Mesh M, Cases C, Patches P, const FT* __restrict__ flux) {
+__device__ __forceinline__ void mass_inout_body(const Mesh& M, const Cases& C, const Patches& P,
+                                                const FT* __restrict__ flux, int c) {
   __shared__ double sin_[256], sout[256];
-  const int c = blockIdx.x;
   Scalars* sc = C.sc + c;
   double a = 0.0, b = 0.0;
   const bool live = !stopped(sc);
@@ -2848,6 +2983,11 @@ __global__ void k_mass_inout(Mesh M, Cases C, Patches P, const FT* __restrict__ 
     sc->mass_in = sc->mass_in + sin_[0];
     sc->mass_out = sc->mass_out + sout[0];
   }
+}
+
+template <typename FT>
+__global__ void k_mass_inout(Mesh M, Cases C, Patches P, const FT* __restrict__ flux) {
+  mass_inout_body<FT>(M, C, P, flux, (int)blockIdx.x);
 }
 
 // Clock and counters of cases [c0, c1) as after an upload at `time`.
@@ -2887,6 +3027,123 @@ __global__ void k_reset_counters(Scalars* scs, int K) {
   sc->last_step = 0;
   sc->finished = 0;
   sc->fail = 0;
+}
+
+// ---------------------------------------------------------------------------------
+// Persistent step of a batch: one launch advances every case by one time step (all RK
+// stages).  Its blocks stay resident and take work items of every case in turn --
+// gradient tiles, face tiles and boundary faces, the per-case time step, update tiles
+// -- with a grid-wide barrier between the phases where the separate launches of
+// launch_step have their kernel boundaries, so every value is the one those launches
+// compute (same bodies, same order within each item).  The grid is sized by the host
+// to the blocks that fit the device at once; every barrier wait is bounded in time and
+// a timeout sets `err` (the host reports it) and lets every block run to its end.
+// ---------------------------------------------------------------------------------
+struct GridBar {
+  unsigned* count;   // arrivals at the current barrier
+  unsigned* gen;     // barriers completed
+  int* err;          // a wait timed out
+};
+
+// Arrivals back to zero before a launch (the error word stays set once a wait timed out).
+__global__ void k_bar_reset(GridBar gb) { *gb.count = 0u; }
+
+// Bounded grid barrier: block 0..n-1 arrive, the last one opens the next generation.
+// Returns false when a wait timed out (now or at an earlier barrier).
+__device__ __forceinline__ bool grid_sync(const GridBar& gb) {
+  __shared__ int ok;
+  // every wave's own stores (not only thread 0's) complete and released at device scope
+  // before its block arrives; thread 0's acquire after the wait covers the block (the
+  // CU's caches)
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned g0 = __hip_atomic_load(gb.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned n = gridDim.x;
+    if (atomicAdd(gb.count, 1u) == n - 1) {
+      __hip_atomic_store(gb.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(gb.gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      // the constant-rate wall clock (100 MHz on MI355X): give up after ~5 s, or after
+      // 2^26 polls whatever the clock says
+      const unsigned long long t0 = (unsigned long long)wall_clock64();
+      for (unsigned polls = 0;
+           __hip_atomic_load(gb.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g0; ++polls) {
+        if (__hip_atomic_load(gb.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if ((unsigned long long)wall_clock64() - t0 > 500000000ull || polls > (1u << 26)) {
+          atomicExch(gb.err, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __threadfence();
+    ok = __hip_atomic_load(gb.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+struct PersistentPlan {
+  int n_grad_tiles;    // 256-cell gradient tiles
+  int n_face_tiles;    // face tiles of FTl.ts cells (the update runs over the same tiles)
+  int upd_per_iter;    // update tiles per 256-thread block iteration (256 / FTl.ts)
+  int upd_lds_stride;  // bytes of LDS per update tile
+  int n_cell_blocks;   // 256-cell blocks of the time-step gather
+  int flowthrough;     // cases have inflow/outflow patches: stage 1 counts their mass
+};
+
+template <typename GT, int DISCR, bool VENKAT, bool FAST>
+__global__ __launch_bounds__(256) void k_step_persistent(Mesh M, State S, Gas G, Cases C,
+                                                         Patches P, Tiles T, FaceTiles FTl,
+                                                         GT* __restrict__ grad,
+                                                         double* __restrict__ flux,
+                                                         double* __restrict__ srad,
+                                                         PersistentPlan pl, GridBar gb) {
+  extern __shared__ double lds[];
+  const int nb = (int)gridDim.x, b0 = (int)blockIdx.x;
+  for (int stage = 1; stage <= 3; ++stage) {
+    for (int t = b0; t < pl.n_grad_tiles; t += nb) {
+      if (FAST) grad_mom_body<GT, VENKAT, 4, 256, 0>(M, S, T, grad, t, lds);
+      else grad_lds_body<GT, VENKAT, 4, 256, 0, false>(M, S, T, nullptr, grad, t, lds);
+      __syncthreads();
+    }
+    if (!grid_sync(gb)) return;
+    for (int t = b0; t < pl.n_face_tiles; t += nb) {
+      if (stage == 1) face_tile_body<DISCR, double, GT, double, true>(M, S, G, C, FTl, grad, flux, srad, t, lds);
+      else face_tile_body<DISCR, double, GT, double, false>(M, S, G, C, FTl, grad, flux, srad, t, lds);
+      __syncthreads();
+    }
+    for (int b = b0 * 256 + (int)threadIdx.x; b < M.B; b += nb * 256) {
+      if (stage == 1) bface_body<2, GT, double, true>(M, S, G, C, P, grad, flux, srad, b);
+      else bface_body<2, GT, double, false>(M, S, G, C, P, grad, flux, srad, b);
+    }
+    if (!grid_sync(gb)) return;
+    if (stage == 1) {
+      for (int k = b0; k < pl.n_cell_blocks; k += nb) {
+        dt_gather_cases_body(M, C, srad, k);
+        __syncthreads();
+      }
+      if (!grid_sync(gb)) return;
+      for (int c = b0 * 256 + (int)threadIdx.x; c < C.K; c += nb * 256) dt_finalize_case(C, c);
+      if (!grid_sync(gb)) return;
+      if (pl.flowthrough)
+        for (int c = b0; c < C.K; c += nb) {
+          mass_inout_body<double>(M, C, P, flux, c);
+          __syncthreads();
+        }
+    }
+    const int per = pl.upd_per_iter, ts = FTl.ts;
+    const int part = (int)threadIdx.x / ts, lane = (int)threadIdx.x - part * ts;
+    for (int t2 = b0; t2 * per < pl.n_face_tiles; t2 += nb) {
+      const int t = t2 * per + part;
+      update_tile_body<4, double, true, true>(M, S, G, C, FTl, flux, C.sc, stage, stage == 3,
+                                              -1.0, t, lane, ts, t < pl.n_face_tiles,
+                                              (char*)lds + (size_t)part * pl.upd_lds_stride);
+      __syncthreads();
+    }
+    if (stage < 3 && !grid_sync(gb)) return;
+  }
 }
 
 }  // namespace rflu

@@ -309,6 +309,7 @@ class GpuRegion:
     OPT_FACE_TILE = 7
     OPT_UPDATE_TILE = 8
     OPT_GRAD_FAST = 9
+    OPT_PERSISTENT = 10
 
     def set_option(self, option: int, value: int):
         nat.check(self._lib.rflu_gpu_set_option(self.handle, int(option), int(value)),

@@ -98,3 +98,88 @@ def test_batch_end_times_and_a_failing_case(gpu):
         assert stats[i]["time"] == end and stats[i]["finished"] == 1
         assert stats[i]["steps"] == st["steps"]
         assert np.array_equal(outs[i], ref), f"case {i}"
+
+
+def _batch_run(ens, nsteps, persistent, fast=0, end_time=None, cvs=None):
+    b = GpuBatch([(m, num) for m, num, _ in ens])
+    try:
+        b.set_option(GpuRegion.OPT_FACE_TILE, 128)
+        b.set_option(GpuRegion.OPT_UPDATE_TILE, 1)
+        b.set_option(GpuRegion.OPT_GRAD_FAST, fast)
+        if persistent:
+            b.set_option(GpuRegion.OPT_PERSISTENT, 1)
+            assert b.get_option(GpuRegion.OPT_PERSISTENT) == 1
+        b.upload(cvs if cvs is not None else [cv0 for _, _, cv0 in ens])
+        if end_time is not None:
+            b.set_end_time(end_time)
+        err = None
+        try:
+            b.time_steps(nsteps, graph=True)
+        except nat.RfluError as e:
+            err = e.code
+        outs, _ = b.download()
+        return outs, [b.case_stats(i) for i in range(len(ens))], err
+    finally:
+        b.close()
+
+
+@pytest.mark.parametrize("precision,reconst,fast", [(0, 11, 0), (1, 11, 0), (1, 11, 1), (0, 0, 0)])
+def test_persistent_step_matches_separate_launches(gpu, precision, reconst, fast):
+    """One persistent launch per time step (grid barriers between the phases) gives the
+    bits, clocks and step counts of the separate launches, every case."""
+    ens = small_ensemble(precision=precision, reconst=reconst)
+    print(f"case: persistent batch prec={precision} reconst={reconst} fast={fast} sizes",
+          [m.n_cells for m, _, _ in ens], flush=True)
+    ref, st_ref, e1 = _batch_run(ens, 12, False, fast)
+    out, st, e2 = _batch_run(ens, 12, True, fast)
+    assert e1 is None and e2 is None
+    for i in range(len(ens)):
+        assert np.array_equal(out[i], ref[i]), f"case {i}"
+        assert st[i]["time"] == st_ref[i]["time"] and st[i]["steps"] == st_ref[i]["steps"] == 12
+
+
+def test_persistent_step_end_times_and_a_failing_case(gpu):
+    """The persistent step keeps each case's end-time clamp and stop-at-first-failure:
+    the same states, clocks and counters as the separate launches."""
+    ens = small_ensemble(k=4)
+    cvs = [cv0.copy() for _, _, cv0 in ens]
+    cvs[2][0, 0] = -1.0
+    ref, st_ref, e1 = _batch_run(ens, 400, False, end_time=0.5, cvs=cvs)
+    out, st, e2 = _batch_run(ens, 400, True, end_time=0.5, cvs=cvs)
+    assert e1 is not None and e1 == e2
+    for i in range(len(ens)):
+        # the failed case holds NaNs where its negative density reached the fluxes
+        assert np.array_equal(out[i], ref[i], equal_nan=True), f"case {i}"
+        for k in ("time", "steps", "finished", "n_negative", "n_invalid"):
+            assert st[i][k] == st_ref[i][k], (i, k)
+
+
+def test_persistent_step_refuses_what_it_does_not_run(gpu):
+    """Single regions, WENO-XYZ, other integrators and missing face tiles are refused."""
+    ens = small_ensemble(k=2)
+    g = GpuRegion(ens[0][0], ens[0][1])
+    try:
+        with pytest.raises(nat.RfluError):
+            g.set_option(GpuRegion.OPT_PERSISTENT, 1)
+    finally:
+        g.close()
+    b = GpuBatch([(m, num) for m, num, _ in ens])
+    try:
+        with pytest.raises(nat.RfluError):   # no face tiles yet
+            b.set_option(GpuRegion.OPT_PERSISTENT, 1)
+        b.set_option(GpuRegion.OPT_FACE_TILE, 128)
+        b.set_option(GpuRegion.OPT_GRAD_TILE, 512)
+        with pytest.raises(nat.RfluError):
+            b.set_option(GpuRegion.OPT_PERSISTENT, 1)
+        assert b.get_option(GpuRegion.OPT_PERSISTENT) == 0
+    finally:
+        b.close()
+    for kw in (dict(reconst=2), dict(rk=2)):
+        ens2 = small_ensemble(k=2, **kw)
+        b = GpuBatch([(m, num) for m, num, _ in ens2])
+        try:
+            b.set_option(GpuRegion.OPT_FACE_TILE, 128)
+            with pytest.raises(nat.RfluError):
+                b.set_option(GpuRegion.OPT_PERSISTENT, 1)
+        finally:
+            b.close()

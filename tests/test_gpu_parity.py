@@ -304,6 +304,9 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
                 dict(lds=1, unroll=2, xcd=1, tile=512, weights=1),
                 # weights formed for 256-cell tiles, then the tiles rebuilt at 128
                 dict(lds=1, unroll=2, xcd=8, weights=1, tile_after=128),
+                # stored inverse distances, at two tile sizes and rebuilt with the tiles
+                dict(lds=1, unroll=4, xcd=8, weights=2), dict(lds=1, unroll=4, xcd=1, tile=512, weights=2),
+                dict(lds=1, unroll=4, xcd=8, weights=2, tile_after=128),
                 # interior fluxes from LDS face tiles
                 dict(lds=1, unroll=4, xcd=8, face_tile=128), dict(lds=1, unroll=4, xcd=1, face_tile=64),
                 dict(lds=0, unroll=2, xcd=8, face_tile=128, tile=128)]
@@ -319,8 +322,8 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
             if "tile" in v:
                 g.set_option(GpuRegion.OPT_GRAD_TILE, v["tile"])
             if v.get("weights"):
-                g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, 1)
-                assert g.get_option(GpuRegion.OPT_GRAD_WEIGHTS) == 1
+                g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, v["weights"])
+                assert g.get_option(GpuRegion.OPT_GRAD_WEIGHTS) == v["weights"]
             if "face_tile" in v:
                 with pytest.raises(Exception):   # 256-cell FP64 face tiles exceed 64 KiB here
                     g.set_option(GpuRegion.OPT_FACE_TILE, 256)
@@ -330,7 +333,7 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
             if "tile_after" in v:
                 g.set_option(GpuRegion.OPT_GRAD_TILE, v["tile_after"])
                 assert g.get_option(GpuRegion.OPT_GRAD_TILE) == v["tile_after"]
-                assert g.get_option(GpuRegion.OPT_GRAD_WEIGHTS) == 1
+                assert g.get_option(GpuRegion.OPT_GRAD_WEIGHTS) == v["weights"]
             g.upload(cv0)
             g.time_steps(8, graph=True)
             cvg, _ = g.download()
