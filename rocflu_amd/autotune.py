@@ -206,6 +206,17 @@ def tune(g: GpuRegion, reps: int = 10, allow_weights: bool = True, use_cache: bo
         g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, 0)
         g.set_option(GpuRegion.OPT_GRAD_FAST, 0)
         apply(g, choice)
+        # the block order again, now over the chosen tiles and with the gradient: the
+        # first pass timed the one-thread-per-face flux without tiles
+        best = None
+        for xcd in (8, 1):
+            g.set_option(GpuRegion.OPT_XCD, xcd)
+            t = sum(_time(g, k, reps) for k in (K_GRAD, K_FACE, K_UPDATE))
+            timings[f"stage_xcd{xcd}"] = t
+            if best is None or t < best[0]:
+                best = (t, xcd)
+        choice["xcd"] = best[1]
+        g.set_option(GpuRegion.OPT_XCD, best[1])
     _cache[sig] = dict(choice)
     _store_file(sig, choice)
     return dict(choice, timings_ms=timings, cached=False)

@@ -663,6 +663,21 @@ int launch_blend_t(Region* R) {
   if (R->reconst != RFLU_RECONST_WENO_XYZ) return 0;
   Mesh M = R->mesh();
   State S = R->state();
+  const size_t shm = (size_t)8 * R->tile_cap * sizeof(GT);
+  if (R->use_lds && R->grad_lds && shm <= 160 * 1024) {
+    // over the gradient's stencil tiles, 8 components of every union record in LDS
+    Tiles T = region_tiles(R);
+#define RFLU_WENO(TS)                                                                          \
+  hipLaunchKernelGGL((k_weno_clip_lds<GT, TS>), dim3(R->n_tiles), dim3(TS), shm, R->stream, M, \
+                     S, T, (const GT*)R->grad, (GT*)R->grad2, R->gate_sc())
+    switch (R->grad_tile) {
+      case 128: RFLU_WENO(128); break;
+      case 512: RFLU_WENO(512); break;
+      default: RFLU_WENO(256); break;
+    }
+#undef RFLU_WENO
+    return 0;
+  }
   hipLaunchKernelGGL((k_weno_clip<GT>), dim3(nblk(R->N, R->grad_block)), dim3(R->grad_block), 0,
                      R->stream, M, S, (const GT*)R->grad, (GT*)R->grad2, R->gate_sc());
   return 0;
@@ -2155,7 +2170,7 @@ int form_weights(Region* R, int kind) {
 // weights, whose layout follows the tiles).  Tiles that would not fit LDS leave the
 // region as it was.
 // Face tiles of ts cells (0: none).  Needs the faces ordered by their lower cell, as
-// the renumbering leaves them; a tile's records must fit 64 KiB of LDS so that
+// the renumbering leaves them; a tile's records must fit 80 KiB of LDS so that
 // several blocks share a CU.
 void free_face_tiles(Region* R) {
   void* old[] = {R->ft_fptr, R->ft_eptr, R->ft_eidx, R->ft_fslot};
@@ -2210,7 +2225,7 @@ int set_face_tiles(Region* R, int ts) {
   }
   const bool fp32 = R->prec == RFLU_PREC_MIXED || R->prec == RFLU_PREC_FP32_GRAD;
   const size_t bytes = fp32 ? face_tile_lds<float>(cap, true) : face_tile_lds<double>(cap, true);
-  if (cap > 65535 || bytes > 64 * 1024)
+  if (cap > 65535 || bytes > 80 * 1024)   // two blocks per CU at most 80 KiB each
     return fail(RFLU_ERR_ARG, "face tiles of that size do not fit LDS");
   free_face_tiles(R);
   int rc;

@@ -2707,6 +2707,35 @@ __global__ void k_barth_jesp(Mesh M, State S, GT* __restrict__ grad,
   for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = GT(g[k]);
 }
 
+// Positivity clip of rho and p (modflu/RFLU_ModLimiters.F90:1199-1267): a variable's
+// gradient is zeroed when its extrapolation to any face of the cell is not positive.
+template <typename T>
+__device__ __forceinline__ void weno_clip_cell(const Mesh& M, const State& S, int i, T* g) {
+  const int N = M.N;
+  double4 ci = M.cell[i];
+  const int pos[2] = {0, 4};
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    int iv = pos[q];
+    double varc = S.prim[(size_t)iv * N + i];
+    int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
+    for (int e = e0; e < e1; ++e) {
+      int sgn = M.c2f_idx[e];
+      int f = (sgn > 0 ? sgn : -sgn) - 1;
+      double4 xf = f < M.F ? M.fc[f] : M.bfc[f - M.F];
+      double dx = xf.x - ci.x, dy = xf.y - ci.y, dz = xf.z - ci.z;
+      double varf = varc + double(g[3 * iv]) * dx + double(g[3 * iv + 1]) * dy +
+                    double(g[3 * iv + 2]) * dz;
+      if (varf <= 0.0) {
+        g[3 * iv] = T(0);
+        g[3 * iv + 1] = T(0);
+        g[3 * iv + 2] = T(0);
+        break;
+      }
+    }
+  }
+}
+
 // WENO-XYZ blend (modflu/RFLU_ModWENO.F90:1437-1508) followed by the positivity clip
 // of rho and p (modflu/RFLU_ModLimiters.F90:1199-1267).
 template <typename GT>
@@ -2744,28 +2773,78 @@ __global__ void k_weno_clip(Mesh M, State S, const GT* __restrict__ gin, GT* __r
   }
 #pragma unroll
   for (int k = 0; k < 15; ++k) g[k] = loc[k] / sumw[k];
-  double4 ci = M.cell[i];
-  const int pos[2] = {0, 4};
+  weno_clip_cell<T>(M, S, i, g);
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    int iv = pos[q];
-    double varc = S.prim[(size_t)iv * N + i];
-    int e0 = M.c2f_ptr[i], e1 = M.c2f_ptr[i + 1];
-    for (int e = e0; e < e1; ++e) {
-      int sgn = M.c2f_idx[e];
-      int f = (sgn > 0 ? sgn : -sgn) - 1;
-      double4 xf = f < M.F ? M.fc[f] : M.bfc[f - M.F];
-      double dx = xf.x - ci.x, dy = xf.y - ci.y, dz = xf.z - ci.z;
-      double varf = varc + double(g[3 * iv]) * dx + double(g[3 * iv + 1]) * dy +
-                    double(g[3 * iv + 2]) * dz;
-      if (varf <= 0.0) {
-        g[3 * iv] = T(0);
-        g[3 * iv + 1] = T(0);
-        g[3 * iv + 2] = T(0);
-        break;
+  for (int k = 0; k < 15; ++k) gout[(size_t)k * N + i] = g[k];
+}
+
+
+// The blend over the gradient kernel's stencil tiles: the raw gradients of a tile's
+// union records are staged in LDS, components 0-7 and then 8-14 (8 values per record
+// fit the gradient's LDS budget in FP64), and every lane blends its own stencil in
+// member order through the tile's slot table -- the same sums in the same order.
+template <typename GT, int TS>
+__global__ __launch_bounds__(TS) void k_weno_clip_lds(Mesh M, State S, Tiles T,
+                                                      const GT* __restrict__ gin,
+                                                      GT* __restrict__ gout,
+                                                      const Scalars* __restrict__ sc) {
+  extern __shared__ double lds_raw[];
+  if (gate(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
+  GT* lds = (GT*)lds_raw;
+  const int N = M.N, cap = T.cap;
+  const int t = xcd_block(M.nxcd);
+  const int i = t * TS + (int)threadIdx.x;
+  const bool live = i < N;
+  const int ic = live ? i : N - 1;
+  const int cnt = live ? M.sten_ptr[ic + 1] - M.sten_ptr[ic] : 0;
+  const unsigned short* L = T.loc + T.lptr[t] + threadIdx.x;
+  const int u0 = T.uptr[t], nu = T.uptr[t + 1] - u0;
+  const GT eps = GT(1.0E-15);
+  GT g[15];
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    constexpr int NQ = 8;
+    const int k0 = half ? 8 : 0, nq = half ? 7 : 8;
+    if (half) __syncthreads();   // the first half's reads are done
+    for (int k = threadIdx.x; k < nu; k += TS) {
+      const int j = T.uidx[u0 + k];
+      GT v[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) v[q] = q < nq ? gin[(size_t)(k0 + q) * N + j] : GT(0);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if (q < nq) lds[q * cap + k] = v[q];
+    }
+    __syncthreads();
+    if (live) {
+      GT sumw[NQ], loc[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        if (q >= nq) break;
+        GT term = gin[(size_t)(k0 + q) * N + i];
+        sumw[q] = GT(1) / (term * term + eps);
+        loc[q] = sumw[q] * term;
       }
+      int kn = cnt > 0 ? (int)L[0] : 0;
+      for (int s = 0; s < cnt; ++s) {
+        const int k = kn;
+        if (s + 1 < cnt) kn = (int)L[(size_t)(s + 1) * TS];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          if (q >= nq) break;
+          const GT tq = lds[q * cap + k];
+          GT w = GT(1) / (tq * tq + eps);
+          sumw[q] = sumw[q] + w;
+          loc[q] = loc[q] + w * tq;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if (q < nq) g[k0 + q] = loc[q] / sumw[q];
     }
   }
+  if (!live) return;
+  weno_clip_cell<GT>(M, S, i, g);
 #pragma unroll
   for (int k = 0; k < 15; ++k) gout[(size_t)k * N + i] = g[k];
 }

@@ -325,8 +325,8 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
                 g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, v["weights"])
                 assert g.get_option(GpuRegion.OPT_GRAD_WEIGHTS) == v["weights"]
             if "face_tile" in v:
-                with pytest.raises(Exception):   # 256-cell FP64 face tiles exceed 64 KiB here
-                    g.set_option(GpuRegion.OPT_FACE_TILE, 256)
+                with pytest.raises(Exception):   # tiles of 64, 128 or 256 cells only
+                    g.set_option(GpuRegion.OPT_FACE_TILE, 100)
                 assert g.get_option(GpuRegion.OPT_FACE_TILE) == 0
                 g.set_option(GpuRegion.OPT_FACE_TILE, v["face_tile"])
                 assert g.get_option(GpuRegion.OPT_FACE_TILE) == v["face_tile"]
