@@ -128,6 +128,27 @@ def cpu_model():
     return "unknown"
 
 
+def cpu_topology() -> dict:
+    """lscpu's model, sockets, cores, threads and caches (empty where it is missing)."""
+    keep = ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)",
+            "CPU max MHz", "L2 cache", "L3 cache", "NUMA node(s)", "Flags")
+    out = {}
+    try:
+        p = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                           text=True, timeout=20)
+        for line in p.stdout.splitlines():
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in keep:
+                v = v.strip()
+                if k == "Flags":   # the vector extensions only
+                    v = " ".join(f for f in v.split() if f.startswith(("avx", "fma", "sse4")))
+                out[k] = v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return out
+
+
 def host_threads():
     """Threads the host grants this process (OMP_NUM_THREADS, else the affinity mask)."""
     n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
@@ -166,6 +187,7 @@ def cpu_baseline(args):
     return {"value": rates[tmax], "unit": "cell-updates/s", "cores": tmax, "kind": "port",
             "value_1thread": rates[1], "thread_speedup": rates[tmax] / rates[1],
             "threads_same_bits": bool(same), "cpu_model": cpu_model(),
+            "lscpu": cpu_topology(),
             "sample": f"{n}^3x6 = {m.n_cells} tets, {steps} SSP-RK3 steps of the same scheme: "
                       f"{times[1]:.1f} s on 1 thread, {times[tmax]:.1f} s on {tmax} threads"}
 
@@ -534,7 +556,7 @@ def main():
     ap.add_argument("--c2-n", type=int, default=80, help="the 80^3 x 6 configuration")
     ap.add_argument("--c2-err-steps", type=int, default=200)
     ap.add_argument("--no-c2", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=32)
+    ap.add_argument("--cpu-n", type=int, default=48)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-cases", type=int, default=64)

@@ -329,7 +329,8 @@ int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
 
 /* Tuning options (set by the autotuner, rocflu_amd/autotune.py): whether the gradient
  * kernel stages each tile's stencil members in LDS (1) or reads them from global
- * memory (0), how many members it handles together (1, 2 or 4), (read only) the LDS
+ * memory (0), how many members it handles together (1, 2 or 4; 8 with 256-cell tiles
+ * and weights formed in the pass, otherwise read as 4), (read only) the LDS
  * records per tile, over how many XCDs the cell and face kernels spread contiguous
  * block ranges (1 = hardware order; 8 on MI355X), whether the stencil weights are
  * formed once and streamed from HBM (1; 3 weights per stencil entry, 24 B in FP64,

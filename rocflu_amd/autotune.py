@@ -109,7 +109,7 @@ def gradient_candidates(g: GpuRegion, allow_weights: bool = True,
     c = []
     if g.get_option(GpuRegion.OPT_TILE_CAP) > 0:
         for ts in TILE_SIZES:
-            for u in (2, 4):
+            for u in ((2, 4, 8) if ts == 256 else (2, 4)):
                 c.append(dict(grad_tile=ts, grad_lds=1, grad_unroll=u, grad_weights=0,
                               grad_fast=0))
             if allow_weights:

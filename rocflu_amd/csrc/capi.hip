@@ -650,6 +650,23 @@ int launch_gradients_t(Region* R) {
   State S = R->state();
   // with constrained cells the limiter runs after them, as a pass of its own
   bool v = R->reconst == RFLU_RECONST_LIM_VENKAT && R->con_n == 0;
+  // groups of 8 members: formed weights over 256-cell LDS tiles only (else groups of 4)
+  if (R->grad_unroll == 8 && R->use_lds && R->grad_lds && R->grad_tile == 256 &&
+      !R->grad_fast && !R->gw) {
+    Tiles T = region_tiles(R);
+    const int cap = R->tile_cap;
+    auto per_cu = [](int c) { return (160 * 1024) / (8 * c * (int)sizeof(double)); };
+    int capc = 0;
+    for (int c : {1024, 1280, 1536, 2048})
+      if (cap <= c) {
+        if (per_cu(c) == per_cu(cap)) capc = c;
+        break;
+      }
+    const size_t shm = (size_t)8 * (capc ? capc : cap) * sizeof(double);
+    if (v) launch_gradient_lds<GT, true, 8, 256, false>(R, M, S, T, capc, shm);
+    else launch_gradient_lds<GT, false, 8, 256, false>(R, M, S, T, capc, shm);
+    return 0;
+  }
   switch (R->grad_unroll) {
     case 1: v ? launch_gradient_kernel<GT, true, 1>(R, M, S) : launch_gradient_kernel<GT, false, 1>(R, M, S); break;
     case 2: v ? launch_gradient_kernel<GT, true, 2>(R, M, S) : launch_gradient_kernel<GT, false, 2>(R, M, S); break;
@@ -2340,7 +2357,8 @@ int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
       R->grad_lds = value != 0;
       break;
     case RFLU_OPT_GRAD_UNROLL:
-      if (value != 1 && value != 2 && value != 4) return fail(RFLU_ERR_ARG, "GRAD_UNROLL is 1, 2 or 4");
+      if (value != 1 && value != 2 && value != 4 && value != 8)
+        return fail(RFLU_ERR_ARG, "GRAD_UNROLL is 1, 2, 4 or 8");
       R->grad_unroll = value;
       break;
     case RFLU_OPT_XCD:

@@ -300,6 +300,7 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
                 dict(lds=1, unroll=4, xcd=8, weights=1), dict(lds=1, unroll=1, xcd=1, weights=1),
                 dict(lds=1, unroll=2, xcd=8, weights=1),
                 dict(lds=1, unroll=4, xcd=8, tile=128), dict(lds=1, unroll=2, xcd=8, tile=512),
+                dict(lds=1, unroll=8, xcd=8), dict(lds=1, unroll=8, xcd=1, tile=512),
                 dict(lds=1, unroll=1, xcd=8, tile=128, weights=1),
                 dict(lds=1, unroll=2, xcd=1, tile=512, weights=1),
                 # weights formed for 256-cell tiles, then the tiles rebuilt at 128
