@@ -274,9 +274,31 @@ def final_state(m, num, steps, fast=False):
         g.close()
 
 
+def one_stage_state(m, num):
+    """State after the first RK stage from the headline initial state, through the
+    per-callee entry points (RungeKuttaMP's order within a stage, the FP64 time step)."""
+    from rocflu_amd import cases
+    from rocflu_amd.solver import GpuRegion
+    cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
+    g = GpuRegion(m, num)
+    try:
+        g.upload(cv0)
+        dt = g.time_step()
+        g.rk_init(1)
+        g.cell_gradients()
+        g.convective_fluxes(1)
+        g.rk_update(1, dt)
+        g.set_dependent_vars()
+        g.after_update(1)
+        return g.download()[0]
+    finally:
+        g.close()
+
+
 def mixed_accuracy(args, m, final64):
-    """The mixed run's error per conserved variable beside what sets it: after one step
-    (the stage's own rounding), after err_steps without a modifier (no amplification),
+    """The mixed run's error per conserved variable beside what sets it: after one stage
+    and one step (the stage's own rounding), after err_steps without a modifier (no
+    amplification),
     and after err_steps with the limiter next to an FP64 run whose gradient only sums in
     another order (the moment form) -- the limiter amplifies rounding-level differences
     of the gradients in the transverse momenta whatever causes them."""
@@ -285,6 +307,8 @@ def mixed_accuracy(args, m, final64):
     mk = lambda prec, reconst: Numerics(discr=args.discr, order=2, reconst=reconst, rk=4,
                                         cfl=0.5, dt_imposed=1e-4, dt_min_limit=0.0,
                                         precision=prec)
+    a, b = one_stage_state(m, mk(1, args.reconst)), one_stage_state(m, mk(0, args.reconst))
+    out["one_stage"] = rel_l2_per_variable(a, b)
     a, b = final_state(m, mk(1, args.reconst), 1), final_state(m, mk(0, args.reconst), 1)
     out["one_step"] = rel_l2_per_variable(a, b)
     r = final_state(m, mk(0, args.reconst), args.err_steps, fast=True)
