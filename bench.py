@@ -635,7 +635,11 @@ def main():
     kt = fp["kernels"]
     dom = max(KERNELS, key=lambda k: kt[k][0])
     ms_launch = kt[dom][0] / max(kt[dom][1], 1)
-    ab = algorithmic_bytes(m, fp["num"], dom, fp["tuning"])
+    # algorithmic bytes without stored stencil data (weights or inverse distances, a
+    # design choice that trades HBM bytes for arithmetic); what the variant stores is
+    # reported beside them
+    ab = algorithmic_bytes(m, fp["num"], dom, fp["tuning"], compulsory=True)
+    stored = algorithmic_bytes(m, fp["num"], dom, fp["tuning"]) - ab
     achieved = ab / (ms_launch * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_tet{args.n}_prec0.json")
@@ -643,8 +647,9 @@ def main():
         with open(tfile) as f:
             tj = json.load(f)
         # measured for the same kernel variant only
-        if dom != "gradient" or tj.get("variant") == grad_variant(fp["tuning"]):
-            traffic = tj.get("bytes_per_launch")
+        rec = tj.get("variants", {}).get(grad_variant(fp["tuning"]) if dom == "gradient" else dom)
+        if rec:
+            traffic = rec.get("bytes_per_launch")
 
     def step_hbm(mesh, rr):
         """Compulsory bytes of a whole step (three stages of gradient, fluxes and
@@ -679,7 +684,8 @@ def main():
         "tuning": fp["tuning"],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": ab, "ms_per_launch": ms_launch,
+                     "algorithmic_bytes_per_launch": ab,
+                     "stored_stencil_bytes_per_launch": stored, "ms_per_launch": ms_launch,
                      "variant": grad_variant(fp["tuning"]) if dom == "gradient" else dom},
         "step_hbm": step_hbm(m, fp),
         "dropin": fp["dropin"],

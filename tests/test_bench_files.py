@@ -17,11 +17,13 @@ def test_traffic_record_names_a_variant_the_bench_can_match():
     assert any("tet160_prec0" in f for f in files)      # the headline configuration
     for p in files:
         d = json.load(open(p))
-        m = re.fullmatch(r"grad_t(\d+)_lds([01])_u(\d)_w([01])(_fast)?", d["variant"])
-        assert m, d["variant"]
-        t = {"grad_tile": int(m.group(1)), "grad_lds": int(m.group(2)),
-             "grad_unroll": int(m.group(3)), "grad_weights": int(m.group(4)),
-             "grad_fast": 1 if m.group(5) else 0}
-        assert bench.grad_variant(t) == d["variant"]
-        assert d["bytes_per_launch"] > 0
+        assert d["variants"]
+        for v, rec in d["variants"].items():
+            m = re.fullmatch(r"grad_t(\d+)_lds([01])_u(\d)_w([012])(_fast)?", v)
+            assert m, v
+            t = {"grad_tile": int(m.group(1)), "grad_lds": int(m.group(2)),
+                 "grad_unroll": int(m.group(3)), "grad_weights": int(m.group(4)),
+                 "grad_fast": 1 if m.group(5) else 0}
+            assert bench.grad_variant(t) == v
+            assert rec["bytes_per_launch"] > 0
         assert os.path.exists(os.path.join(ROOT, d["source"]))
