@@ -219,8 +219,11 @@ __device__ __forceinline__ float tabs(float x) { return fabsf(x); }
 // selects and quotient fix-up, all of which leave such arguments unchanged -- so the
 // bits are those of 1.0 / sqrt(x), in 9 fewer instructions; other arguments take the
 // library path.  The stencil weights use it once per member.
-__device__ __forceinline__ double inv_sqrt_cr(double x) {
-  if (!(x >= 0x1p-700 && x <= 0x1p700)) return 1.0 / sqrt(x);
+// Whether inv_sqrt_fast gives the bits of 1.0 / sqrt(x).
+__device__ __forceinline__ bool inv_sqrt_fast_ok(double x) { return x >= 0x1p-700 && x <= 0x1p700; }
+
+// The sequences of inv_sqrt_cr without its range check (arguments in [2^-700, 2^700]).
+__device__ __forceinline__ double inv_sqrt_fast(double x) {
   const double y = __builtin_amdgcn_rsq(x);
   const double s0 = x * y;
   const double h0 = y * 0.5;
@@ -239,7 +242,35 @@ __device__ __forceinline__ double inv_sqrt_cr(double x) {
   const double f = __builtin_fma(-q, r, 1.0);
   return __builtin_fma(f, r, r);                  // 1.0 / q
 }
+
+__device__ __forceinline__ double inv_sqrt_cr(double x) {
+  if (!inv_sqrt_fast_ok(x)) return 1.0 / sqrt(x);
+  return inv_sqrt_fast(x);
+}
 __device__ __forceinline__ float inv_sqrt_cr(float x) { return 1.0f / sqrtf(x); }
+
+// inv_sqrt_cr of n arguments at once: the fast sequences of all of them first as
+// straight-line code (independent chains the scheduler can interleave; a branch per
+// argument would serialize them), then the library path for any out of range.
+template <int n>
+__device__ __forceinline__ void inv_sqrt_cr_n(const double* x, double* r) {
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < n; ++k) {
+    r[k] = inv_sqrt_fast(x[k]);
+    ok = ok && inv_sqrt_fast_ok(x[k]);
+  }
+  if (!ok) {
+#pragma unroll
+    for (int k = 0; k < n; ++k)
+      if (!inv_sqrt_fast_ok(x[k])) r[k] = 1.0 / sqrt(x[k]);
+  }
+}
+template <int n>
+__device__ __forceinline__ void inv_sqrt_cr_n(const float* x, float* r) {
+#pragma unroll
+  for (int k = 0; k < n; ++k) r[k] = 1.0f / sqrtf(x[k]);
+}
 
 // ---------------------------------------------------------------------------------
 // Flux functions (interior faces).  See the oracle for the line-by-line citations:
@@ -2090,10 +2121,17 @@ __device__ __forceinline__ void grad_lds_body(Mesh M, State S, Tiles T,
   // Slot numbers (and stored terms) are loaded one group of U members ahead of their use.
   int nk[U];
   GT nt[U];
+  // prefetch loads are unconditional (row 0 for rows past the lane's members: the
+  // tables carry padding rows after the last tile), their values selected after
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    nk[u] = u < cnt ? (int)L[(size_t)u * T.ts] : 0;
-    if (TW) nt[u] = u < cnt ? TWp[(size_t)u * T.ts] : GT(1);
+    const size_t ro = (size_t)(u < cnt ? u : 0) * T.ts;
+    const int kv = (int)L[ro];
+    nk[u] = u < cnt ? kv : 0;
+    if (TW) {
+      const GT tv = TWp[ro];
+      nt[u] = u < cnt ? tv : GT(1);
+    }
   }
   const int u0 = T.uptr[t];
   stage_tile<true>(M, S, T.uidx + u0, T.uptr[t + 1] - u0, cap, lds);
@@ -2111,26 +2149,42 @@ __device__ __forceinline__ void grad_lds_body(Mesh M, State S, Tiles T,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       kk[u] = nk[u];
-      int sn = s + U + u;
-      nk[u] = sn < cnt ? (int)L[(size_t)sn * T.ts] : 0;
+      const int sn = s + U + u;
+      const size_t ro = (size_t)(sn < cnt ? sn : 0) * T.ts;
+      const int kv = (int)L[ro];
+      nk[u] = sn < cnt ? kv : 0;
       if (TW) {
         tt[u] = nt[u];
-        nt[u] = sn < cnt ? TWp[(size_t)sn * T.ts] : GT(1);
+        const GT tv = TWp[ro];
+        nt[u] = sn < cnt ? tv : GT(1);
       }
     }
     GT w[U][3];
     double dvar[U][5];
+    double ddx[U], ddy[U], ddz[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       int k = kk[u];
       double xj, yj, zj, vj[5];
       rec_load(lds, cap, k, xj, yj, zj, vj);
-      const double dx = xj - ci.x, dy = yj - ci.y, dz = zj - ci.z;
-      if (TW) grad_weights_t<GT>(c, tt[u], dx, dy, dz, w[u]);
-      else grad_weights<GT>(c, dx, dy, dz, w[u]);
+      ddx[u] = xj - ci.x;
+      ddy[u] = yj - ci.y;
+      ddz[u] = zj - ci.z;
 #pragma unroll
       for (int q = 0; q < 5; ++q) dvar[u][q] = vj[q] - vi[q];
     }
+    if (!TW) {
+      // the group's inverse distances as independent chains (grad_term's arithmetic)
+      GT d2[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const GT dx = GT(ddx[u]), dy = GT(ddy[u]), dz = GT(ddz[u]);
+        d2[u] = dx * dx + dy * dy + dz * dz;
+      }
+      inv_sqrt_cr_n<U>(d2, tt);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) grad_weights_t<GT>(c, tt[u], ddx[u], ddy[u], ddz[u], w[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) grad_accum<GT>(w[u], dvar[u], g);
   }
