@@ -3186,9 +3186,10 @@ __global__ void k_bar_reset(GridBar gb) { *gb.count = 0u; }
 __device__ __forceinline__ bool grid_sync(const GridBar& gb) {
   __shared__ int ok;
   // every wave's own stores (not only thread 0's) complete and released at device scope
-  // before its block arrives; thread 0's acquire after the wait covers the block (the
-  // CU's caches)
-  __threadfence();
+  // before its block arrives (a release fence: write-back, no invalidation); thread 0
+  // polls with relaxed loads and acquires once after the wait, which covers the block
+  // (the CU's caches)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned g0 = __hip_atomic_load(gb.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -3201,7 +3202,7 @@ __device__ __forceinline__ bool grid_sync(const GridBar& gb) {
       // 2^26 polls whatever the clock says
       const unsigned long long t0 = (unsigned long long)wall_clock64();
       for (unsigned polls = 0;
-           __hip_atomic_load(gb.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g0; ++polls) {
+           __hip_atomic_load(gb.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0; ++polls) {
         if (__hip_atomic_load(gb.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
         if ((unsigned long long)wall_clock64() - t0 > 500000000ull || polls > (1u << 26)) {
           atomicExch(gb.err, 1);
