@@ -851,24 +851,41 @@ __device__ __forceinline__ void face_tile_body(Mesh M, State S, Gas G, Cases C,
   const int e0 = FTl.eptr[t], nrec = nown + (FTl.eptr[t + 1] - e0);
   constexpr int ND = SRAD ? 9 : 8;   // x y z rho u v w p (c)
   GT* lg = (GT*)(lds + (size_t)ND * cap);
-  for (int k = threadIdx.x; k < nrec; k += blockDim.x) {
-    const int c = k < nown ? c0 + k : FTl.eidx[e0 + k - nown];
-    const double4 cc = M.cell[c];
-    double pv[5];
+  // two records per thread and pass, every load of both issued before the first store
+  for (int k0 = threadIdx.x; k0 < nrec; k0 += 2 * blockDim.x) {
+    int kr[2], cr[2];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) pv[q] = S.prim[(size_t)q * N + c];
-    GT gv[15];
+    for (int r = 0; r < 2; ++r) {
+      kr[r] = k0 + r * (int)blockDim.x;
+      const int k = kr[r];
+      cr[r] = k < nown ? c0 + k : k < nrec ? FTl.eidx[e0 + k - nown] : c0;
+    }
+    double4 cc[2];
+    double pv[2][5], a[2];
+    GT gv[2][15];
 #pragma unroll
-    for (int q = 0; q < 15; ++q) gv[q] = grad[(size_t)q * N + c];
-    double a = SRAD ? S.dv[2 * (size_t)N + c] : 0.0;
-    lds[k] = cc.x;
-    lds[cap + k] = cc.y;
-    lds[2 * cap + k] = cc.z;
+    for (int r = 0; r < 2; ++r) {
+      const int c = cr[r];
+      cc[r] = M.cell[c];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) lds[(size_t)(3 + q) * cap + k] = pv[q];
-    if (SRAD) lds[(size_t)8 * cap + k] = a;
+      for (int q = 0; q < 5; ++q) pv[r][q] = S.prim[(size_t)q * N + c];
 #pragma unroll
-    for (int q = 0; q < 15; ++q) lg[(size_t)q * cap + k] = gv[q];
+      for (int q = 0; q < 15; ++q) gv[r][q] = grad[(size_t)q * N + c];
+      a[r] = SRAD ? S.dv[2 * (size_t)N + c] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int k = kr[r];
+      if (k >= nrec) continue;
+      lds[k] = cc[r].x;
+      lds[cap + k] = cc[r].y;
+      lds[2 * cap + k] = cc[r].z;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) lds[(size_t)(3 + q) * cap + k] = pv[r][q];
+      if (SRAD) lds[(size_t)8 * cap + k] = a[r];
+#pragma unroll
+      for (int q = 0; q < 15; ++q) lg[(size_t)q * cap + k] = gv[r][q];
+    }
   }
   __syncthreads();
   const size_t FB = (size_t)M.F + M.B;
