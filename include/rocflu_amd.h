@@ -353,7 +353,9 @@ int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
  * gradient tiles and face tiles) advances every case by one time step in one
  * persistent launch whose resident blocks take the gradient tiles, face tiles,
  * boundary faces, per-case time steps and update tiles of all cases phase by phase
- * between grid barriers; the same bits as the separate launches (0, the default). */
+ * between grid barriers; the same bits as the separate launches (0, the default).  A
+ * barrier wait that times out (5 s) makes rflu_gpu_time_steps return RFLU_ERR_HIP until
+ * PERSISTENT is set to 1 again. */
 #define RFLU_OPT_GRAD_LDS 1
 #define RFLU_OPT_GRAD_UNROLL 2
 #define RFLU_OPT_TILE_CAP 3

@@ -2412,6 +2412,9 @@ int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
           if (int rc = dalloc(&R->pbar, 4)) return rc;
           HIP_TRY(hipMemset(R->pbar, 0, 4 * sizeof(unsigned)));
         }
+        // a barrier wait that timed out stays reported until the option is set again
+        HIP_TRY(hipStreamSynchronize(R->stream));
+        HIP_TRY(hipMemset(R->pbar + 2, 0, sizeof(unsigned)));
       }
       R->persistent = value == 1;
       break;
