@@ -192,9 +192,10 @@ def cpu_baseline(args):
                       f"{times[1]:.1f} s on 1 thread, {times[tmax]:.1f} s on {tmax} threads"}
 
 
-def time_region(args, m, num, exact: bool, steps: int, err_steps: int, dropin: bool = False):
+def time_region(args, m, num, exact: bool, steps: int, err_steps: int, dropin: bool = False,
+                fp32_records: bool = False):
     """One region of mesh m with numerics num: autotuned (exact: bit-identical variants
-    only), then `steps` fused steps timed three ways -- eager (the value), eager with HIP
+    only; fp32_records: the FP32 gradient tile records among the candidates), then `steps` fused steps timed three ways -- eager (the value), eager with HIP
     events around every launch (per-kernel milliseconds), and as a replayed graph -- and
     with dropin the same steps through the per-callee entry points (the reference
     host's call sequence); finally err_steps graph steps from the initial state for the
@@ -206,7 +207,7 @@ def time_region(args, m, num, exact: bool, steps: int, err_steps: int, dropin: b
     g = GpuRegion(m, num)
     try:
         g.upload(cv0)
-        tuning = autotune.tune(g, exact=exact)
+        tuning = autotune.tune(g, exact=exact, fp32_records=fp32_records)
         tuning = {k: v for k, v in tuning.items() if k != "timings_ms"}
 
         def timed(fn, n):
@@ -257,14 +258,17 @@ def time_region(args, m, num, exact: bool, steps: int, err_steps: int, dropin: b
     return res
 
 
-def final_state(m, num, steps, fast=False):
-    """State after `steps` fused steps from the headline initial state, default kernel
-    variants (fast: the FP64 moment-form gradient, which only sums in another order)."""
-    from rocflu_amd import cases
+def final_state(m, num, steps, fast=False, tuning=None):
+    """State after `steps` fused steps from the headline initial state, with the kernel
+    variants of `tuning` (default ones without; fast: the FP64 moment-form gradient,
+    which only sums in another order)."""
+    from rocflu_amd import cases, autotune
     from rocflu_amd.solver import GpuRegion
     cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
     g = GpuRegion(m, num)
     try:
+        if tuning:
+            autotune.apply(g, tuning)
         if fast:
             g.set_option(GpuRegion.OPT_GRAD_FAST, 1)
         g.upload(cv0)
@@ -274,14 +278,16 @@ def final_state(m, num, steps, fast=False):
         g.close()
 
 
-def one_stage_state(m, num):
+def one_stage_state(m, num, tuning=None):
     """State after the first RK stage from the headline initial state, through the
     per-callee entry points (RungeKuttaMP's order within a stage, the FP64 time step)."""
-    from rocflu_amd import cases
+    from rocflu_amd import cases, autotune
     from rocflu_amd.solver import GpuRegion
     cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
     g = GpuRegion(m, num)
     try:
+        if tuning:
+            autotune.apply(g, tuning)
         g.upload(cv0)
         dt = g.time_step()
         g.rk_init(1)
@@ -295,7 +301,7 @@ def one_stage_state(m, num):
         g.close()
 
 
-def mixed_accuracy(args, m, final64):
+def mixed_accuracy(args, m, final64, tuning):
     """The mixed run's error per conserved variable beside what sets it: after one stage
     and one step (the stage's own rounding), after err_steps without a modifier (no
     amplification),
@@ -307,13 +313,17 @@ def mixed_accuracy(args, m, final64):
     mk = lambda prec, reconst: Numerics(discr=args.discr, order=2, reconst=reconst, rk=4,
                                         cfl=0.5, dt_imposed=1e-4, dt_min_limit=0.0,
                                         precision=prec)
-    a, b = one_stage_state(m, mk(1, args.reconst)), one_stage_state(m, mk(0, args.reconst))
+    # the mixed runs use the timed mixed run's kernel variants (its error is the one that
+    # goes with its speed)
+    a = one_stage_state(m, mk(1, args.reconst), tuning)
+    b = one_stage_state(m, mk(0, args.reconst))
     out["one_stage"] = rel_l2_per_variable(a, b)
-    a, b = final_state(m, mk(1, args.reconst), 1), final_state(m, mk(0, args.reconst), 1)
+    a = final_state(m, mk(1, args.reconst), 1, tuning=tuning)
+    b = final_state(m, mk(0, args.reconst), 1)
     out["one_step"] = rel_l2_per_variable(a, b)
     r = final_state(m, mk(0, args.reconst), args.err_steps, fast=True)
     out["fp64_reorder_drift"] = rel_l2_per_variable(r, final64)
-    a = final_state(m, mk(1, 0), args.err_steps)
+    a = final_state(m, mk(1, 0), args.err_steps, tuning=tuning)
     b = final_state(m, mk(0, 0), args.err_steps)
     out["no_modifier"] = rel_l2_per_variable(a, b)
     print(f"# mixed accuracy: {out}", file=sys.stderr, flush=True)
@@ -332,6 +342,7 @@ def persistent_vs_launches(b, cvs, args, n_tot):
     from rocflu_amd import autotune
     from rocflu_amd.solver import GpuRegion
     autotune._set_tile(b, 256)
+    b.set_option(GpuRegion.OPT_GRAD_LDS, 1)
     b.set_option(GpuRegion.OPT_GRAD_WEIGHTS, 0)
     b.set_option(GpuRegion.OPT_FACE_TILE, 128)
     b.set_option(GpuRegion.OPT_UPDATE_TILE, 1)
@@ -707,9 +718,21 @@ def main():
                                  "state/residual/limiter/positivity",
                         "err_steps": args.err_steps,
                         "rel_l2_per_variable": rel_l2_per_variable(mx["final"], fp["final"]),
-                        "accuracy": mixed_accuracy(args, m, fp["final"]),
+                        "accuracy": mixed_accuracy(args, m, fp["final"], mx["tuning"]),
                         "kernel_ms": {k: [v[0], v[1]] for k, v in mx["kernels"].items()},
                         "tuning": mx["tuning"]}
+        # the FP32 tile records (GRAD_LDS 2): faster gradients, further from FP64
+        f32 = time_region(args, m, build_case(args, 1), exact=False, steps=args.steps,
+                          err_steps=args.err_steps, fp32_records=True)
+        out["mixed"]["fp32_records"] = {
+            "value": f32["value"], "ms_per_step": f32["ms_per_step"],
+            "speedup_vs_fp64": f32["value"] / fp["value"],
+            "rel_l2_per_variable": rel_l2_per_variable(f32["final"], fp["final"]),
+            "one_stage": rel_l2_per_variable(
+                one_stage_state(m, build_case(args, 1), f32["tuning"]),
+                one_stage_state(m, build_case(args, 0))),
+            "kernel_ms": {k: [v[0], v[1]] for k, v in f32["kernels"].items()},
+            "tuning": f32["tuning"]}
     # the 80^3 x 6 configuration: HLLC and AUSM+ in FP64, HLLC mixed
     if not args.no_c2:
         try:

@@ -344,7 +344,13 @@ int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
  * thread per face gathering its two cells from global memory), and whether the
  * update then reads the fluxes of each tile's own faces from an LDS copy
  * (UPDATE_TILE 1; only while face tiles exist; reads as 0 otherwise).  None of these
- * changes results.  GRAD_FAST 1 replaces the gradient's member loop by its moment form
+ * changes results.  GRAD_LDS 2 (FP32 gradients only: the mixed mode or FP32_GRAD;
+ * groups of 4 or 8 members) stages each union record as eight floats, its centroid and
+ * primitive state less those of the tile's first cell rounded once from FP64, and runs
+ * the member loop in FP32 with the hardware reciprocal square root: half the LDS of
+ * the FP64 records, FP32 rounding in the gradients (within the mixed mode's accuracy,
+ * not the bits of GRAD_LDS 1); with GRAD_FAST or stored weights the FP64-layout
+ * records are staged and GRAD_LDS reads 1.  GRAD_FAST 1 replaces the gradient's member loop by its moment form
  * (per member one reciprocal of |d|^2 and fused multiply-adds, the cell's inverse
  * moment matrix applied once; LDS tiles only, ahead of stored weights): the same least
  * squares gradient with other rounding, so results differ from the reference's in the

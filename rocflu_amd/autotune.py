@@ -102,10 +102,12 @@ TILE_SIZES = (256, 128, 512)
 
 
 def gradient_candidates(g: GpuRegion, allow_weights: bool = True,
-                        exact: bool = True) -> List[dict]:
+                        exact: bool = True, fp32_records: bool = False) -> List[dict]:
     """Gradient variants, grouped by tile size (a size change rebuilds the tiles) and,
     within a size, formed weights before stored ones (and, unless exact, the moment
-    form last)."""
+    form last; with fp32_records and FP32 gradients, the FP32 tile records, whose
+    differences carry the rounding of each value's deviation from its tile's first cell
+    and so drift further from the FP64 run over limited steps)."""
     c = []
     if g.get_option(GpuRegion.OPT_TILE_CAP) > 0:
         for ts in TILE_SIZES:
@@ -122,17 +124,22 @@ def gradient_candidates(g: GpuRegion, allow_weights: bool = True,
                 for u in (1, 2, 4):
                     c.append(dict(grad_tile=ts, grad_lds=1, grad_unroll=u, grad_weights=0,
                                   grad_fast=1))
+                if fp32_records and g.num.precision in (1, 2):
+                    for u in (4, 8):
+                        c.append(dict(grad_tile=ts, grad_lds=2, grad_unroll=u, grad_weights=0,
+                                      grad_fast=0))
     c.append(dict(grad_tile=g.get_option(GpuRegion.OPT_GRAD_TILE), grad_lds=0, grad_unroll=4,
                   grad_weights=0, grad_fast=0))
     return c
 
 
 def tune(g: GpuRegion, reps: int = 10, allow_weights: bool = True, use_cache: bool = True,
-         verbose: bool = False, exact: bool = True) -> dict:
+         verbose: bool = False, exact: bool = True, fp32_records: bool = False) -> dict:
     """Chooses and applies the fastest variant of each kernel of the stage for region g
-    (exact=False: also the gradient's moment form, not bit-identical).  Returns the
-    choice with the measured milliseconds of every candidate."""
-    sig = signature(g) + ("" if exact else "|fast")
+    (exact=False: also the gradient's moment form, not bit-identical; fp32_records with
+    exact=False: also the FP32 tile records of FP32 gradients).  Returns the choice with
+    the measured milliseconds of every candidate."""
+    sig = signature(g) + ("" if exact else "|fast") + ("|f32rec" if fp32_records else "")
     if use_cache:
         if not _cache:
             _cache.update(_load_file())
@@ -179,7 +186,7 @@ def tune(g: GpuRegion, reps: int = 10, allow_weights: bool = True, use_cache: bo
             choice["update_tile"] = best[1]
             g.set_option(GpuRegion.OPT_UPDATE_TILE, best[1])
         best = None
-        for cand in gradient_candidates(g, allow_weights, exact):
+        for cand in gradient_candidates(g, allow_weights, exact, fp32_records):
             if g.get_option(GpuRegion.OPT_GRAD_TILE) != cand["grad_tile"]:
                 g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, 0)   # no weights rebuilt in vain
                 if not _set_tile(g, cand["grad_tile"]):

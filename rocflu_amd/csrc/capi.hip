@@ -124,7 +124,8 @@ struct Region {
   int *ft_fptr = nullptr, *ft_eptr = nullptr, *ft_eidx = nullptr;
   unsigned* ft_fslot = nullptr;
   bool use_lds = false;       // tiles were built (every tile fits in LDS)
-  bool grad_lds = true;       // tuning: stage stencils in LDS when tiles exist
+  int grad_lds = 1;           // tuning: stage stencils in LDS when tiles exist (2: the
+                              // mixed mode's FP32 records)
   int grad_unroll = 4;        // tuning: stencil members whose weights are formed together
   bool grad_fast = false;     // tuning: moment form of the gradient (LDS tiles; not bit-identical)
   // per-callee path: the update asked for by rflu_gpu_rk_update (stage, dt) and the
@@ -647,12 +648,35 @@ void launch_gradient_kernel(Region* R, const Mesh& M, const State& S) {
   }
 }
 
+// The mixed mode's FP32-record tiles (GRAD_LDS 2).
+template <bool V, int U>
+void launch_gradient_lds32(Region* R, const Mesh& M, const State& S) {
+  Tiles T = region_tiles(R);
+  const size_t shm = (size_t)2 * R->tile_cap * sizeof(float4);
+#define RFLU_L32(TS)                                                                           \
+  hipLaunchKernelGGL((k_gradient_lds32<V, U, TS>), dim3(R->n_tiles), dim3(TS), shm, R->stream, M, \
+                     S, T, (float*)R->grad, R->gate_sc())
+  switch (R->grad_tile) {
+    case 128: RFLU_L32(128); break;
+    case 512: RFLU_L32(512); break;
+    default: RFLU_L32(256); break;
+  }
+#undef RFLU_L32
+}
+
 template <typename GT>
 int launch_gradients_t(Region* R) {
   Mesh M = R->mesh();
   State S = R->state();
   // with constrained cells the limiter runs after them, as a pass of its own
   bool v = R->reconst == RFLU_RECONST_LIM_VENKAT && R->con_n == 0;
+  if constexpr (sizeof(GT) == 4) {
+    if (R->grad_lds == 2 && R->use_lds && !R->grad_fast && !R->gw) {
+      if (R->grad_unroll == 8) v ? launch_gradient_lds32<true, 8>(R, M, S) : launch_gradient_lds32<false, 8>(R, M, S);
+      else v ? launch_gradient_lds32<true, 4>(R, M, S) : launch_gradient_lds32<false, 4>(R, M, S);
+      return 0;
+    }
+  }
   // groups of 8 members: formed weights over 256-cell LDS tiles only (else groups of 4)
   if (R->grad_unroll == 8 && R->use_lds && R->grad_lds && R->grad_tile == 256 &&
       !R->grad_fast && !R->gw) {
@@ -918,7 +942,7 @@ std::string persistent_unsupported(const Region* R) {
     return "PERSISTENT supports no modifier or the Venkatakrishnan limiter";
   if (R->con_n > 0) return "PERSISTENT does not run constrained gradients";
   if (R->prec == RFLU_PREC_FP32_FLUX) return "PERSISTENT runs FP64 or mixed precision";
-  if (!R->use_lds || !R->grad_lds || R->grad_tile != 256)
+  if (!R->use_lds || R->grad_lds != 1 || R->grad_tile != 256)
     return "PERSISTENT needs 256-cell gradient tiles in LDS (GRAD_TILE 256, GRAD_LDS 1)";
   if (R->face_tile <= 0 || 256 % R->face_tile) return "PERSISTENT needs face tiles (FACE_TILE)";
   return "";
@@ -2356,8 +2380,10 @@ int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
   if (int frc = flush_pending(R)) return frc;
   switch (option) {
     case RFLU_OPT_GRAD_LDS:
-      if (value != 0 && value != 1) return fail(RFLU_ERR_ARG, "GRAD_LDS is 0 or 1");
-      R->grad_lds = value != 0;
+      if (value < 0 || value > 2) return fail(RFLU_ERR_ARG, "GRAD_LDS is 0, 1 or 2");
+      if (value == 2 && !grad32(R))
+        return fail(RFLU_ERR_ARG, "GRAD_LDS 2 (FP32 records) needs FP32 gradients");
+      R->grad_lds = value;
       break;
     case RFLU_OPT_GRAD_UNROLL:
       if (value != 1 && value != 2 && value != 4 && value != 8)
@@ -2440,7 +2466,10 @@ int rflu_gpu_get_option(int32_t h, int32_t option, int32_t* value) {
   if (!R) return fail(RFLU_ERR_HANDLE, "bad region handle");
   if (!value) return fail(RFLU_ERR_ARG, "null value");
   switch (option) {
-    case RFLU_OPT_GRAD_LDS: *value = (R->use_lds && R->grad_lds) ? 1 : 0; break;
+    case RFLU_OPT_GRAD_LDS:
+      // the moment form and stored weights stage FP64-layout records (GRAD_LDS 1)
+      *value = !R->use_lds ? 0 : (R->grad_lds == 2 && (R->grad_fast || R->gw)) ? 1 : R->grad_lds;
+      break;
     case RFLU_OPT_GRAD_UNROLL: *value = R->grad_unroll; break;
     case RFLU_OPT_TILE_CAP: *value = R->tile_cap; break;
     case RFLU_OPT_XCD: *value = R->nxcd; break;

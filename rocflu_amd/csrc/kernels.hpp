@@ -2244,6 +2244,116 @@ __global__ __launch_bounds__(TS) void k_gradient_lds(Mesh M, State S, Tiles T,
 }
 
 
+// Gradient of the mixed mode over the same tiles with FP32 records: each union record is
+// staged as eight floats -- its centroid and primitive state less those of the tile's
+// first cell, rounded once from FP64 -- so the member differences the loop forms keep
+// the accuracy of those deviations; half the LDS of the FP64 records (512-cell tiles fit
+// twice per CU) and FP32 arithmetic throughout, the inverse distance from the hardware
+// reciprocal square root.  Not bit-identical to the FP64-record kernel (the mixed
+// mode's gradients are FP32 either way).  The Venkatakrishnan limiter stays FP64 and
+// reads its face neighbours' values from memory.
+template <bool VENKAT, int U, int TS>
+__global__ __launch_bounds__(TS) void k_gradient_lds32(Mesh M, State S, Tiles T,
+                                                        float* __restrict__ grad,
+                                                        const Scalars* __restrict__ sc) {
+  extern __shared__ double lds_raw[];
+  if (gate(sc)) return;  // uniform over the grid: no lane reaches the barrier alone
+  const int N = M.N, cap = T.cap;
+  float4* la = (float4*)lds_raw;   // x, y, z, rho (less the reference cell's)
+  float4* lb = la + cap;           // u, v, w, p
+  const int t = xcd_block(M.nxcd);
+  const int r0 = t * TS;           // the tile's first cell: the reference
+  const int i = r0 + (int)threadIdx.x;
+  const bool live = i < N;
+  const int ic = live ? i : N - 1;
+  const double4 cr = M.cell[r0];
+  double vr[5], vi[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    vr[q] = S.prim[(size_t)q * N + r0];
+    vi[q] = S.prim[(size_t)q * N + ic];
+  }
+  GradCoef<float> c = grad_coef<float>(M, ic);
+  const double4 ci = M.cell[ic];
+  const float xi = float(ci.x - cr.x), yi = float(ci.y - cr.y), zi = float(ci.z - cr.z);
+  float vd[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) vd[q] = float(vi[q] - vr[q]);
+  const int cnt = live ? M.sten_ptr[ic + 1] - M.sten_ptr[ic] : 0;
+  const unsigned short* L = T.loc + T.lptr[t] + threadIdx.x;
+  int nk[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int kv = (int)L[(size_t)(u < cnt ? u : 0) * TS];   // padded tables: in bounds
+    nk[u] = u < cnt ? kv : 0;
+  }
+  const int u0 = T.uptr[t], nu = T.uptr[t + 1] - u0;
+  for (int k0 = threadIdx.x; k0 < nu; k0 += 2 * TS) {
+    int jj[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int k = k0 + r * TS;
+      jj[r] = k < nu ? T.uidx[u0 + k] : r0;
+    }
+    double4 cc[2];
+    double pp[2][5];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      cc[r] = M.cell[jj[r]];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) pp[r][q] = S.prim[(size_t)q * N + jj[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int k = k0 + r * TS;
+      if (k >= nu) continue;
+      la[k] = make_float4(float(cc[r].x - cr.x), float(cc[r].y - cr.y), float(cc[r].z - cr.z),
+                          float(pp[r][0] - vr[0]));
+      lb[k] = make_float4(float(pp[r][1] - vr[1]), float(pp[r][2] - vr[2]),
+                          float(pp[r][3] - vr[3]), float(pp[r][4] - vr[4]));
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  float g[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) g[k] = 0.f;
+  auto member = [&](int k, float* gg) {
+    const float4 a = la[k], b = lb[k];
+    const float dx = a.x - xi, dy = a.y - yi, dz = a.z - zi;
+    const float term = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
+    float w[3];
+    grad_weights_t<float>(c, term, dx, dy, dz, w);
+    const double dvar[5] = {double(a.w - vd[0]), double(b.x - vd[1]), double(b.y - vd[2]),
+                            double(b.z - vd[3]), double(b.w - vd[4])};
+    grad_accum<float>(w, dvar, gg);
+  };
+  int s = 0;
+  for (; s + U <= cnt; s += U) {
+    int kk[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      kk[u] = nk[u];
+      const int sn = s + U + u;
+      const int kv = (int)L[(size_t)(sn < cnt ? sn : 0) * TS];
+      nk[u] = sn < cnt ? kv : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) member(kk[u], g);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (s + u < cnt) member(nk[u], g);
+  if (VENKAT)
+    venkat_limit<float>(M, i, vi, g, [&](int q) { return M.vnb[(size_t)q * N + i]; },
+                        [&](int o, double* vo) {
+#pragma unroll
+                          for (int k = 0; k < 5; ++k) vo[k] = S.prim[(size_t)k * N + o];
+                        });
+#pragma unroll
+  for (int k = 0; k < 15; ++k) grad[(size_t)k * N + i] = g[k];
+}
+
 // Moment form of the same least-squares gradient.  The member loop's weights are
 // w_j = term^2 (A d_j) with term = 1/|d_j|, d_j = x_j - x_i and A the cell's symmetric
 // matrix built from the R-factor coefficients (A = (sum_j d_j d_j^T / |d_j|^2)^-1), so

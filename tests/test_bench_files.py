@@ -19,7 +19,7 @@ def test_traffic_record_names_a_variant_the_bench_can_match():
         d = json.load(open(p))
         assert d["variants"]
         for v, rec in d["variants"].items():
-            m = re.fullmatch(r"grad_t(\d+)_lds([01])_u(\d)_w([012])(_fast)?", v)
+            m = re.fullmatch(r"grad_t(\d+)_lds([012])_u(\d)_w([012])(_fast)?", v)
             assert m, v
             t = {"grad_tile": int(m.group(1)), "grad_lds": int(m.group(2)),
                  "grad_unroll": int(m.group(3)), "grad_weights": int(m.group(4)),

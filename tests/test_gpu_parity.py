@@ -165,14 +165,16 @@ def test_time_step_matches_oracle(gpu):
     g.close()
 
 
-def _mixed_vs_fp64(m, reconst, steps, floor=False):
+def _mixed_vs_fp64(m, reconst, steps, floor=False, mixed_opts=(), keys=("fp64", "mixed")):
     """Final states of the FP64 and mixed runs (and, with floor, of an FP64 run whose
     gradient sums in another order: the moment form) from the same Mach-2 shock with
-    1% density noise, HLLC order 2, SSP-RK3."""
+    1% density noise, HLLC order 2, SSP-RK3.  mixed_opts: (option, value) pairs set on
+    the mixed run; keys: the runs to make ("fp64", "mixed", "mixed_opts")."""
     out = {}
-    for key, prec, fast in (("fp64", 0, 0), ("mixed", 1, 0), ("reorder", 0, 1)):
-        if key == "reorder" and not floor:
-            continue
+    runs = [(k, 0 if k == "fp64" else 1, 0) for k in keys]
+    if floor:
+        runs.append(("reorder", 0, 1))
+    for key, prec, fast in runs:
         num = Numerics(discr=3, order=2, reconst=reconst, rk=4, cfl=0.5, dt_imposed=1e-3,
                        precision=prec)
         cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=7)
@@ -180,11 +182,18 @@ def _mixed_vs_fp64(m, reconst, steps, floor=False):
         try:
             if fast:
                 g.set_option(GpuRegion.OPT_GRAD_FAST, 1)
+            if key == "mixed_opts":
+                for opt, val in mixed_opts:
+                    g.set_option(opt, val)
+                    assert g.get_option(opt) == val, (opt, val)
             g.upload(cv0)
             g.time_steps(steps)
             out[key] = g.download()[0]
         finally:
             g.close()
+        if key != "reorder":
+            out[key + "_totals"] = [(float(np.dot(cv0[:, k], m.vol)),
+                                     float(np.dot(out[key][:, k], m.vol))) for k in (0, 4)]
     return out
 
 
@@ -220,6 +229,47 @@ def test_mixed_precision_with_the_limiter_over_steps(gpu):
     assert np.all(err[[0, 1, 4]] <= 1e-6), err
     assert np.all(err[[2, 3]] <= 1e-4), err
     assert np.all(flo > 0.0), flo
+
+
+@pytest.mark.parametrize("tile,unroll,reconst,steps",
+                         [(256, 4, 11, 1), (512, 8, 11, 1), (128, 4, 0, 100), (256, 8, 2, 100)])
+def test_mixed_precision_fp32_records(gpu, tile, unroll, reconst, steps):
+    """GRAD_LDS 2, the mixed mode's gradient over FP32 tile records (values less the
+    tile's first cell's, FP32 member loop): within 1e-6 relative L2 of the FP64 run
+    under the same bars as the FP64-record mixed mode (one step with the
+    Venkatakrishnan limiter; 100 steps without a modifier and with WENO-XYZ); the
+    closed box keeps its totals of rho and rho E to 1e-12; the records change the bits
+    of the FP64-record mixed run (the kernel ran), and the option is refused in FP64."""
+    m = cases.tet_box(n=12, jitter=0.08, seed=2)
+    print(f"case: mixed FP32 records, 12^3 x 6, tile {tile}, unroll {unroll}, "
+          f"reconst {reconst}, {steps} steps", flush=True)
+    opts = ((GpuRegion.OPT_GRAD_TILE, tile), (GpuRegion.OPT_GRAD_LDS, 2),
+            (GpuRegion.OPT_GRAD_UNROLL, unroll))
+    r = _mixed_vs_fp64(m, reconst, steps, mixed_opts=opts,
+                       keys=("fp64", "mixed", "mixed_opts"))
+    err = rel_l2(r["mixed_opts"], r["fp64"])
+    print("FP32 records vs FP64:", err, "FP64 records vs FP64:",
+          rel_l2(r["mixed"], r["fp64"]), flush=True)
+    assert np.all(err <= 1e-6), err
+    assert not np.array_equal(r["mixed_opts"], r["mixed"])
+    for t0, t in r["mixed_opts_totals"]:
+        assert abs(t - t0) <= 1e-12 * abs(t0), (t0, t)
+    g = GpuRegion(m, Numerics(discr=3, order=2, reconst=reconst, rk=4, precision=0))
+    try:
+        with pytest.raises(Exception):
+            g.set_option(GpuRegion.OPT_GRAD_LDS, 2)
+        assert g.get_option(GpuRegion.OPT_GRAD_LDS) == 1
+    finally:
+        g.close()
+    g = GpuRegion(m, Numerics(discr=3, order=2, reconst=reconst, rk=4, precision=1))
+    try:
+        g.set_option(GpuRegion.OPT_GRAD_LDS, 2)
+        g.set_option(GpuRegion.OPT_GRAD_FAST, 1)   # the moment form stages FP64 records
+        assert g.get_option(GpuRegion.OPT_GRAD_LDS) == 1
+        g.set_option(GpuRegion.OPT_GRAD_FAST, 0)
+        assert g.get_option(GpuRegion.OPT_GRAD_LDS) == 2
+    finally:
+        g.close()
 
 
 def test_fp32_flux_split(gpu):
