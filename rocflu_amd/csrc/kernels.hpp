@@ -247,7 +247,9 @@ __device__ __forceinline__ double inv_sqrt_cr(double x) {
   if (!inv_sqrt_fast_ok(x)) return 1.0 / sqrt(x);
   return inv_sqrt_fast(x);
 }
-__device__ __forceinline__ float inv_sqrt_cr(float x) { return 1.0f / sqrtf(x); }
+// FP32 gradients (the mixed mode): the hardware reciprocal square root (within 1 ulp;
+// the gradients are FP32 approximations of the FP64 ones either way).
+__device__ __forceinline__ float inv_sqrt_cr(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // inv_sqrt_cr of n arguments at once: the fast sequences of all of them first as
 // straight-line code (independent chains the scheduler can interleave; a branch per
@@ -269,7 +271,7 @@ __device__ __forceinline__ void inv_sqrt_cr_n(const double* x, double* r) {
 template <int n>
 __device__ __forceinline__ void inv_sqrt_cr_n(const float* x, float* r) {
 #pragma unroll
-  for (int k = 0; k < n; ++k) r[k] = 1.0f / sqrtf(x[k]);
+  for (int k = 0; k < n; ++k) r[k] = inv_sqrt_cr(x[k]);
 }
 
 // ---------------------------------------------------------------------------------
@@ -2321,7 +2323,7 @@ __global__ __launch_bounds__(TS) void k_gradient_lds32(Mesh M, State S, Tiles T,
   auto member = [&](int k, float* gg) {
     const float4 a = la[k], b = lb[k];
     const float dx = a.x - xi, dy = a.y - yi, dz = a.z - zi;
-    const float term = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
+    const float term = inv_sqrt_cr(dx * dx + dy * dy + dz * dz);
     float w[3];
     grad_weights_t<float>(c, term, dx, dy, dz, w);
     const double dvar[5] = {double(a.w - vd[0]), double(b.x - vd[1]), double(b.y - vd[2]),
