@@ -588,6 +588,8 @@ def main():
     ap.add_argument("--err-steps", type=int, default=100,
                     help="steps of the mixed-vs-FP64 comparison on the headline mesh")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="time the mixed legs without their error runs (profiling)")
     ap.add_argument("--c2-n", type=int, default=80, help="the 80^3 x 6 configuration")
     ap.add_argument("--c2-err-steps", type=int, default=200)
     ap.add_argument("--no-c2", action="store_true")
@@ -642,7 +644,8 @@ def main():
           f"{m.c2cs_idx.size} stencil entries, built in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
     fp = time_region(args, m, build_case(args, 0), exact=True, steps=args.steps,
-                     err_steps=0 if args.no_mixed else args.err_steps, dropin=True)
+                     err_steps=0 if args.no_mixed or args.no_accuracy else args.err_steps,
+                     dropin=True)
     kt = fp["kernels"]
     dom = max(KERNELS, key=lambda k: kt[k][0])
     ms_launch = kt[dom][0] / max(kt[dom][1], 1)
@@ -709,28 +712,32 @@ def main():
                                       "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                                       "frac": tf / FP64_PEAK_TFLOPS}
     if not args.no_mixed:
+        acc = not args.no_accuracy
         mx = time_region(args, m, build_case(args, 1), exact=False, steps=args.steps,
-                         err_steps=args.err_steps)
+                         err_steps=args.err_steps if acc else 0)
         out["mixed"] = {"value": mx["value"], "ms_per_step": mx["ms_per_step"],
                         "graph_value": mx["graph_value"], "step_hbm": step_hbm(m, mx),
                         "speedup_vs_fp64": mx["value"] / fp["value"],
                         "dtype": "f32 gradients and reconstruction increments, f64 fluxes/"
                                  "state/residual/limiter/positivity",
                         "err_steps": args.err_steps,
-                        "rel_l2_per_variable": rel_l2_per_variable(mx["final"], fp["final"]),
-                        "accuracy": mixed_accuracy(args, m, fp["final"], mx["tuning"]),
+                        "rel_l2_per_variable": rel_l2_per_variable(mx["final"], fp["final"])
+                        if acc else None,
+                        "accuracy": mixed_accuracy(args, m, fp["final"], mx["tuning"])
+                        if acc else None,
                         "kernel_ms": {k: [v[0], v[1]] for k, v in mx["kernels"].items()},
                         "tuning": mx["tuning"]}
         # the FP32 tile records (GRAD_LDS 2): faster gradients, further from FP64
         f32 = time_region(args, m, build_case(args, 1), exact=False, steps=args.steps,
-                          err_steps=args.err_steps, fp32_records=True)
+                          err_steps=args.err_steps if acc else 0, fp32_records=True)
         out["mixed"]["fp32_records"] = {
             "value": f32["value"], "ms_per_step": f32["ms_per_step"],
             "speedup_vs_fp64": f32["value"] / fp["value"],
-            "rel_l2_per_variable": rel_l2_per_variable(f32["final"], fp["final"]),
+            "rel_l2_per_variable": rel_l2_per_variable(f32["final"], fp["final"])
+            if acc else None,
             "one_stage": rel_l2_per_variable(
                 one_stage_state(m, build_case(args, 1), f32["tuning"]),
-                one_stage_state(m, build_case(args, 0))),
+                one_stage_state(m, build_case(args, 0))) if acc else None,
             "kernel_ms": {k: [v[0], v[1]] for k, v in f32["kernels"].items()},
             "tuning": f32["tuning"]}
     # the 80^3 x 6 configuration: HLLC and AUSM+ in FP64, HLLC mixed
