@@ -6,9 +6,11 @@ made on time alone: each candidate is launched on the region's own stream and ti
 with HIP events (``rflu_gpu_time_kernel``), and the fastest is kept.  ``exact=False``
 adds the moment form of the gradient (``GRAD_FAST``), whose results differ from the
 reference's in the last bits (held to 1e-12 relative L2 per variable after 100 steps
-by tests/test_gpu_parity.py::test_fast_gradient_within_fp64_bar).  Choices are cached
-per mesh/scheme signature and that flag, in memory and optionally in a JSON file named
-by ``ROCFLU_AMD_TUNE_CACHE``.
+by tests/test_gpu_parity.py::test_fast_gradient_within_fp64_bar); with FP32 gradients,
+``fp32_records=True`` also adds the FP32 tile records (``GRAD_LDS 2``), which drift
+further from the FP64 run over many limited steps.  Choices are cached per mesh/scheme
+signature and those flags, in memory and optionally in a JSON file named by
+``ROCFLU_AMD_TUNE_CACHE``.
 
 Timing launches run the kernels on the region's current state (the update kernel with a
 negligible dt), so upload the initial state after tuning.
