@@ -461,9 +461,10 @@ void build_tiles(Region* R, const std::vector<int>& sp, const std::vector<int>& 
     std::vector<int>().swap(tu[t]);
     std::vector<unsigned short>().swap(tl[t]);
   }
-  // rows of zeros after the last tile: the kernels' prefetch loads read row 0 of a
-  // lane's column whatever its member count, so every such read stays in bounds
-  loc.resize(loc.size() + 512, (unsigned short)0);
+  // rows of zeros after the last tile: the kernels' prefetch loads read up to a group
+  // (at most 8 members) past a lane's last member, or row 0 whatever its member count,
+  // so every such read stays in bounds
+  loc.resize(loc.size() + 8 * 512, (unsigned short)0);
   R->n_tiles = nt;
   R->tile_cap = cap;
   R->use_lds = true;

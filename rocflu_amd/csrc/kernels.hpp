@@ -2138,18 +2138,22 @@ __device__ __forceinline__ void grad_lds_body(Mesh M, State S, Tiles T,
   const unsigned short* L = T.loc + lbase;
   const GT* TWp = TW ? tw + lbase : nullptr;
   // Slot numbers (and stored terms) are loaded one group of U members ahead of their use.
+  // Slot rows are read past the lane's last member too (the next tile's rows, or the
+  // padding rows after the last tile) and those values are never used, so the loads
+  // take a running row pointer with immediate offsets; stored terms keep row 0 there.
   int nk[U];
   GT nt[U];
-  // prefetch loads are unconditional (row 0 for rows past the lane's members: the
-  // tables carry padding rows after the last tile), their values selected after
+  const unsigned short* Ln = L + (size_t)U * TS;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const size_t ro = (size_t)(u < cnt ? u : 0) * T.ts;
-    const int kv = (int)L[ro];
-    nk[u] = u < cnt ? kv : 0;
     if (TW) {
+      const size_t ro = (size_t)(u < cnt ? u : 0) * TS;
+      const int kv = (int)L[ro];
+      nk[u] = u < cnt ? kv : 0;
       const GT tv = TWp[ro];
       nt[u] = u < cnt ? tv : GT(1);
+    } else {
+      nk[u] = (int)L[u * TS];
     }
   }
   const int u0 = T.uptr[t];
@@ -2168,16 +2172,19 @@ __device__ __forceinline__ void grad_lds_body(Mesh M, State S, Tiles T,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       kk[u] = nk[u];
-      const int sn = s + U + u;
-      const size_t ro = (size_t)(sn < cnt ? sn : 0) * T.ts;
-      const int kv = (int)L[ro];
-      nk[u] = sn < cnt ? kv : 0;
       if (TW) {
+        const int sn = s + U + u;
+        const size_t ro = (size_t)(sn < cnt ? sn : 0) * TS;
+        const int kv = (int)L[ro];
+        nk[u] = sn < cnt ? kv : 0;
         tt[u] = nt[u];
         const GT tv = TWp[ro];
         nt[u] = sn < cnt ? tv : GT(1);
+      } else {
+        nk[u] = (int)Ln[u * TS];
       }
     }
+    Ln += (size_t)U * TS;
     GT w[U][3];
     double dvar[U][5];
     double ddx[U], ddy[U], ddz[U];
@@ -2283,12 +2290,11 @@ __global__ __launch_bounds__(TS) void k_gradient_lds32(Mesh M, State S, Tiles T,
   for (int q = 0; q < 5; ++q) vd[q] = float(vi[q] - vr[q]);
   const int cnt = live ? M.sten_ptr[ic + 1] - M.sten_ptr[ic] : 0;
   const unsigned short* L = T.loc + T.lptr[t] + threadIdx.x;
+  // slot rows one group ahead, past the lane's last member too (never used; padded)
   int nk[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int kv = (int)L[(size_t)(u < cnt ? u : 0) * TS];   // padded tables: in bounds
-    nk[u] = u < cnt ? kv : 0;
-  }
+  for (int u = 0; u < U; ++u) nk[u] = (int)L[u * TS];
+  const unsigned short* Ln = L + (size_t)U * TS;
   const int u0 = T.uptr[t], nu = T.uptr[t + 1] - u0;
   for (int k0 = threadIdx.x; k0 < nu; k0 += 2 * TS) {
     int jj[2];
@@ -2336,10 +2342,9 @@ __global__ __launch_bounds__(TS) void k_gradient_lds32(Mesh M, State S, Tiles T,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       kk[u] = nk[u];
-      const int sn = s + U + u;
-      const int kv = (int)L[(size_t)(sn < cnt ? sn : 0) * TS];
-      nk[u] = sn < cnt ? kv : 0;
+      nk[u] = (int)Ln[u * TS];
     }
+    Ln += (size_t)U * TS;
 #pragma unroll
     for (int u = 0; u < U; ++u) member(kk[u], g);
   }
@@ -2438,9 +2443,11 @@ __device__ __forceinline__ void grad_mom_body(Mesh M, State S, Tiles T,
   for (int k = 0; k < 5; ++k) vi[k] = S.prim[(size_t)k * N + ic];
   const int cnt = live ? M.sten_ptr[ic + 1] - M.sten_ptr[ic] : 0;
   const unsigned short* L = T.loc + T.lptr[t] + threadIdx.x;
+  // slot rows one group ahead, past the lane's last member too (never used; padded)
   int nk[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u) nk[u] = u < cnt ? (int)L[(size_t)u * T.ts] : 0;
+  for (int u = 0; u < U; ++u) nk[u] = (int)L[u * TS];
+  const unsigned short* Ln = L + (size_t)U * TS;
   const int u0 = T.uptr[t];
   stage_tile<true>(M, S, T.uidx + u0, T.uptr[t + 1] - u0, cap, lds);
   __syncthreads();
@@ -2454,9 +2461,9 @@ __device__ __forceinline__ void grad_mom_body(Mesh M, State S, Tiles T,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       kk[u] = nk[u];
-      int sn = s + U + u;
-      nk[u] = sn < cnt ? (int)L[(size_t)sn * T.ts] : 0;
+      nk[u] = (int)Ln[u * TS];
     }
+    Ln += (size_t)U * TS;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int k = kk[u];
@@ -3008,10 +3015,12 @@ __global__ __launch_bounds__(TS) void k_weno_clip_lds(Mesh M, State S, Tiles T,
         sumw[q] = GT(1) / (term * term + eps);
         loc[q] = sumw[q] * term;
       }
-      int kn = cnt > 0 ? (int)L[0] : 0;
+      // the next member's slot read one row ahead (the row after the last member is the
+      // next tile's or the padding: read, never used)
+      int kn = (int)L[0];
       for (int s = 0; s < cnt; ++s) {
         const int k = kn;
-        if (s + 1 < cnt) kn = (int)L[(size_t)(s + 1) * TS];
+        kn = (int)L[(size_t)(s + 1) * TS];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           if (q >= nq) break;
