@@ -343,7 +343,9 @@ int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
  * and evaluates each tile's faces from there (FACE_TILE; 0, the default, is one
  * thread per face gathering its two cells from global memory), and whether the
  * update then reads the fluxes of each tile's own faces from an LDS copy
- * (UPDATE_TILE 1; only while face tiles exist; reads as 0 otherwise).  None of these
+ * (UPDATE_TILE 1; only while face tiles exist; reads as 0 otherwise), and how many
+ * threads evaluate a face tile (FACE_THREADS 256, the default, or 512: twice the
+ * resident waves where the tiles' LDS holds a CU to two blocks).  None of these
  * changes results.  GRAD_LDS 2 (FP32 gradients only: the mixed mode or FP32_GRAD;
  * groups of 4 or 8 members) stages each union record as eight floats, its centroid and
  * primitive state less those of the tile's first cell rounded once from FP64, and runs
@@ -372,6 +374,7 @@ int rflu_gpu_set_launch(int32_t handle, int32_t face_block, int32_t cell_block,
 #define RFLU_OPT_UPDATE_TILE 8
 #define RFLU_OPT_GRAD_FAST 9
 #define RFLU_OPT_PERSISTENT 10
+#define RFLU_OPT_FACE_THREADS 11
 int rflu_gpu_set_option(int32_t handle, int32_t option, int32_t value);
 int rflu_gpu_get_option(int32_t handle, int32_t option, int32_t* value);
 

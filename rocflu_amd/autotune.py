@@ -63,6 +63,7 @@ def apply(g: GpuRegion, choice: dict):
         g.set_option(GpuRegion.OPT_GRAD_WEIGHTS, choice["grad_weights"])
         g.set_option(GpuRegion.OPT_GRAD_FAST, choice.get("grad_fast", 0))
         _set_face_tile(g, choice.get("face_tile", 0))
+        g.set_option(GpuRegion.OPT_FACE_THREADS, choice.get("face_threads", 256))
         g.set_option(GpuRegion.OPT_UPDATE_TILE, choice.get("update_tile", 0))
 
 
@@ -149,7 +150,8 @@ def tune(g: GpuRegion, reps: int = 10, allow_weights: bool = True, use_cache: bo
             apply(g, _cache[sig])
             return dict(_cache[sig], cached=True)
     choice = dict(xcd=8, face_block=256, cell_block=256, grad_tile=256, grad_lds=0,
-                  grad_unroll=4, grad_weights=0, grad_fast=0, face_tile=0, update_tile=0)
+                  grad_unroll=4, grad_weights=0, grad_fast=0, face_tile=0, face_threads=256,
+                  update_tile=0)
     timings = {}
     # XCD-aware block order and block sizes of the face and cell kernels
     best = None
@@ -165,17 +167,21 @@ def tune(g: GpuRegion, reps: int = 10, allow_weights: bool = True, use_cache: bo
     g.set_option(GpuRegion.OPT_XCD, choice["xcd"])
     g.set_launch(face_block=choice["face_block"], cell_block=choice["cell_block"])
     if g.num.order == 2:
-        # interior flux: one thread per face, or face tiles staged in LDS
+        # interior flux: one thread per face, or face tiles staged in LDS by blocks of
+        # 256 or 512 threads
         best = None
         for ft in FACE_TILES:
             if not _set_face_tile(g, ft):
                 continue
-            t = _time(g, K_FACE, reps)
-            timings[f"face_tile{ft}"] = t
-            if best is None or t < best[0]:
-                best = (t, ft)
-        choice["face_tile"] = best[1]
+            for nt in ((256, 512) if ft else (256,)):
+                g.set_option(GpuRegion.OPT_FACE_THREADS, nt)
+                t = _time(g, K_FACE, reps)
+                timings[f"face_tile{ft}" + (f"_nt{nt}" if ft else "")] = t
+                if best is None or t < best[0]:
+                    best = (t, ft, nt)
+        choice["face_tile"], choice["face_threads"] = best[1], best[2]
         _set_face_tile(g, best[1])
+        g.set_option(GpuRegion.OPT_FACE_THREADS, best[2])
         if best[1] > 0:
             # the update over the same tiles, its own faces' fluxes from LDS
             best = None

@@ -127,6 +127,7 @@ struct Region {
   int grad_lds = 1;           // tuning: stage stencils in LDS when tiles exist (2: the
                               // mixed mode's FP32 records)
   int grad_unroll = 4;        // tuning: stencil members whose weights are formed together
+  int face_threads = 256;     // tuning: threads per face-tile block (256 or 512)
   bool grad_fast = false;     // tuning: moment form of the gradient (LDS tiles; not bit-identical)
   // per-callee path: the update asked for by rflu_gpu_rk_update (stage, dt) and the
   // dependent variables of rflu_gpu_set_dependent_vars, not launched yet; the next
@@ -524,12 +525,18 @@ int launch_fluxes_t(Region* R) {
     FTl.fptr = R->ft_fptr; FTl.eptr = R->ft_eptr; FTl.eidx = R->ft_eidx; FTl.fslot = R->ft_fslot;
     FTl.ts = R->face_tile; FTl.cap = R->face_cap; FTl.fcap = R->face_fcap;
     const size_t shm = face_tile_lds<GT>(R->face_cap, SR);
-#define RFLU_FACE_TILE(D)                                                                          \
-  hipLaunchKernelGGL((k_face_flux_tile<D, T, GT, FT, SR>), dim3(R->n_face_tiles), dim3(256), shm, \
-                     R->stream, M, S, R->gas, R->cases(), FTl, g, fl, R->srad, R->gate_sc())
-    if (R->discr == 1) RFLU_FACE_TILE(1);
-    else if (R->discr == 3) RFLU_FACE_TILE(3);
-    else RFLU_FACE_TILE(4);
+#define RFLU_FACE_TILE(D, NT)                                                                    \
+  hipLaunchKernelGGL((k_face_flux_tile<D, T, GT, FT, SR, NT>), dim3(R->n_face_tiles), dim3(NT),    \
+                     shm, R->stream, M, S, R->gas, R->cases(), FTl, g, fl, R->srad, R->gate_sc())
+    if (R->face_threads == 512) {
+      if (R->discr == 1) RFLU_FACE_TILE(1, 512);
+      else if (R->discr == 3) RFLU_FACE_TILE(3, 512);
+      else RFLU_FACE_TILE(4, 512);
+    } else {
+      if (R->discr == 1) RFLU_FACE_TILE(1, 256);
+      else if (R->discr == 3) RFLU_FACE_TILE(3, 256);
+      else RFLU_FACE_TILE(4, 256);
+    }
 #undef RFLU_FACE_TILE
     return 0;
   }
@@ -2430,6 +2437,10 @@ int rflu_gpu_set_option(int32_t h, int32_t option, int32_t value) {
       if (rc) return rc;
       break;
     }
+    case RFLU_OPT_FACE_THREADS:
+      if (value != 256 && value != 512) return fail(RFLU_ERR_ARG, "FACE_THREADS is 256 or 512");
+      R->face_threads = value;
+      break;
     case RFLU_OPT_PERSISTENT: {
       if (value != 0 && value != 1) return fail(RFLU_ERR_ARG, "PERSISTENT is 0 or 1");
       if (value == 1) {
@@ -2480,6 +2491,7 @@ int rflu_gpu_get_option(int32_t h, int32_t option, int32_t* value) {
     case RFLU_OPT_UPDATE_TILE: *value = (R->update_tile && R->face_tile > 0) ? 1 : 0; break;
     case RFLU_OPT_GRAD_FAST: *value = (R->grad_fast && R->use_lds && R->grad_lds) ? 1 : 0; break;
     case RFLU_OPT_PERSISTENT: *value = R->persistent ? 1 : 0; break;
+    case RFLU_OPT_FACE_THREADS: *value = R->face_threads; break;
     default: return fail(RFLU_ERR_ARG, "unknown option");
   }
   return RFLU_OK;

@@ -924,8 +924,10 @@ __device__ __forceinline__ void face_tile_body(Mesh M, State S, Gas G, Cases C,
   }
 }
 
-template <int DISCR, typename T, typename GT, typename FT, bool SRAD>
-__global__ __launch_bounds__(256) void k_face_flux_tile(Mesh M, State S, Gas G, Cases C,
+// NT threads per block: 256, or 512 (twice the resident waves for tiles whose records
+// hold a CU to two blocks).
+template <int DISCR, typename T, typename GT, typename FT, bool SRAD, int NT>
+__global__ __launch_bounds__(NT) void k_face_flux_tile(Mesh M, State S, Gas G, Cases C,
                                                         FaceTiles FTl, const GT* __restrict__ grad,
                                                         FT* __restrict__ flux,
                                                         double* __restrict__ srad,

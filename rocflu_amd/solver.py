@@ -310,6 +310,7 @@ class GpuRegion:
     OPT_UPDATE_TILE = 8
     OPT_GRAD_FAST = 9
     OPT_PERSISTENT = 10
+    OPT_FACE_THREADS = 11
 
     def set_option(self, option: int, value: int):
         nat.check(self._lib.rflu_gpu_set_option(self.handle, int(option), int(value)),

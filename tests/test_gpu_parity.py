@@ -360,7 +360,10 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
                 dict(lds=1, unroll=4, xcd=8, weights=2, tile_after=128),
                 # interior fluxes from LDS face tiles
                 dict(lds=1, unroll=4, xcd=8, face_tile=128), dict(lds=1, unroll=4, xcd=1, face_tile=64),
-                dict(lds=0, unroll=2, xcd=8, face_tile=128, tile=128)]
+                dict(lds=0, unroll=2, xcd=8, face_tile=128, tile=128),
+                # face tiles by blocks of 512 threads
+                dict(lds=1, unroll=4, xcd=8, face_tile=256, face_threads=512),
+                dict(lds=1, unroll=4, xcd=1, face_tile=64, face_threads=512)]
     for v in variants:
         print("variant", v, flush=True)
         g = GpuRegion(m, num)
@@ -381,6 +384,11 @@ def test_tuning_options_do_not_change_results(gpu, reconst):
                 assert g.get_option(GpuRegion.OPT_FACE_TILE) == 0
                 g.set_option(GpuRegion.OPT_FACE_TILE, v["face_tile"])
                 assert g.get_option(GpuRegion.OPT_FACE_TILE) == v["face_tile"]
+            if "face_threads" in v:
+                with pytest.raises(Exception):   # blocks of 256 or 512 threads only
+                    g.set_option(GpuRegion.OPT_FACE_THREADS, 384)
+                g.set_option(GpuRegion.OPT_FACE_THREADS, v["face_threads"])
+                assert g.get_option(GpuRegion.OPT_FACE_THREADS) == v["face_threads"]
             if "tile_after" in v:
                 g.set_option(GpuRegion.OPT_GRAD_TILE, v["tile_after"])
                 assert g.get_option(GpuRegion.OPT_GRAD_TILE) == v["tile_after"]
@@ -417,11 +425,12 @@ def test_face_tiles_give_the_same_bits(gpu, discr, prec):
                    precision=prec)
     cv0 = cases.planar_shock(m, num, 0.3 * m.dims[0], mach=2.0, noise=0.01, seed=5)
     outs = {}
-    for ft, ut in ((0, 0), (128, 0), (64, 1), (128, 1)):
+    for ft, ut, nt in ((0, 0, 256), (128, 0, 256), (64, 1, 256), (128, 1, 256), (256, 1, 512)):
         for mode in ("fused", "callee"):
-            print("face_tile", ft, "update_tile", ut, mode, flush=True)
+            print("face_tile", ft, "update_tile", ut, "threads", nt, mode, flush=True)
             g = GpuRegion(m, num)
             try:
+                g.set_option(GpuRegion.OPT_FACE_THREADS, nt)
                 g.set_option(GpuRegion.OPT_FACE_TILE, ft)
                 g.set_option(GpuRegion.OPT_UPDATE_TILE, ut)
                 assert g.get_option(GpuRegion.OPT_UPDATE_TILE) == (1 if ft and ut else 0)
