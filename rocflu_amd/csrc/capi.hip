@@ -921,10 +921,13 @@ int launch_dt_reduce(Region* R, double time_in, double end_time) {
   }
   CaseParams cp = R->case0();
   cp.end_time = end_time;
-  hipLaunchKernelGGL(k_dt_gather, dim3(R->n_dt_parts), dim3(256), 0, R->stream, M,
+  // at most 2048 blocks, each over a grid-strided share of the cells: the single-block
+  // minimum then reads 8 partial minima per thread
+  const int nparts = std::min(R->n_dt_parts, 2048);
+  hipLaunchKernelGGL(k_dt_gather, dim3(nparts), dim3(256), 0, R->stream, M,
                      (const double*)R->srad, R->dt_part, R->sc);
   hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, R->stream, (const double*)R->dt_part,
-                     R->n_dt_parts, R->sc, cp, time_in);
+                     nparts, R->sc, cp, time_in);
   return 0;
 }
 

@@ -3100,9 +3100,18 @@ __global__ void k_dt_gather(Mesh M, const double* __restrict__ srad, double* __r
                             const Scalars* __restrict__ sc) {
   __shared__ double red[16];
   const int blk = xcd_block(M.nxcd);
-  int i = blk * blockDim.x + threadIdx.x;
   double dtc = HUGE_VAL;
-  if (i < M.nact && !stopped(sc)) dtc = M.cell[i].w / cell_srad_sum(M, srad, i);
+  if (!stopped(sc)) {
+    // the minimum does not depend on the order: each block takes a grid-strided share
+    // (a lane's first cell is taken as it is, so a NaN stays one as with a cell per lane)
+    const int stride = (int)(gridDim.x * blockDim.x);
+    bool first = true;
+    for (int i = blk * (int)blockDim.x + (int)threadIdx.x; i < M.nact; i += stride) {
+      const double d = M.cell[i].w / cell_srad_sum(M, srad, i);
+      dtc = first ? d : fmin(dtc, d);
+      first = false;
+    }
+  }
   dtc = wave_min(dtc);
   int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) red[wv] = dtc;
