@@ -301,7 +301,7 @@ def one_stage_state(m, num, tuning=None):
         g.close()
 
 
-def mixed_accuracy(args, m, final64, tuning):
+def mixed_accuracy(args, m, final64, tuning, keep=None):
     """The mixed run's error per conserved variable beside what sets it: after one stage
     and one step (the stage's own rounding), after err_steps without a modifier (no
     amplification),
@@ -317,6 +317,8 @@ def mixed_accuracy(args, m, final64, tuning):
     # goes with its speed)
     a = one_stage_state(m, mk(1, args.reconst), tuning)
     b = one_stage_state(m, mk(0, args.reconst))
+    if keep is not None:
+        keep["one_stage64"] = b   # the FP64 side of the FP32-record line's one-stage error
     out["one_stage"] = rel_l2_per_variable(a, b)
     a = final_state(m, mk(1, args.reconst), 1, tuning=tuning)
     b = final_state(m, mk(0, args.reconst), 1)
@@ -713,6 +715,7 @@ def main():
                                       "frac": tf / FP64_PEAK_TFLOPS}
     if not args.no_mixed:
         acc = not args.no_accuracy
+        keep = {}
         mx = time_region(args, m, build_case(args, 1), exact=False, steps=args.steps,
                          err_steps=args.err_steps if acc else 0)
         out["mixed"] = {"value": mx["value"], "ms_per_step": mx["ms_per_step"],
@@ -723,7 +726,7 @@ def main():
                         "err_steps": args.err_steps,
                         "rel_l2_per_variable": rel_l2_per_variable(mx["final"], fp["final"])
                         if acc else None,
-                        "accuracy": mixed_accuracy(args, m, fp["final"], mx["tuning"])
+                        "accuracy": mixed_accuracy(args, m, fp["final"], mx["tuning"], keep)
                         if acc else None,
                         "kernel_ms": {k: [v[0], v[1]] for k, v in mx["kernels"].items()},
                         "tuning": mx["tuning"]}
@@ -737,7 +740,7 @@ def main():
             if acc else None,
             "one_stage": rel_l2_per_variable(
                 one_stage_state(m, build_case(args, 1), f32["tuning"]),
-                one_stage_state(m, build_case(args, 0))) if acc else None,
+                keep["one_stage64"]) if acc else None,
             "kernel_ms": {k: [v[0], v[1]] for k, v in f32["kernels"].items()},
             "tuning": f32["tuning"]}
     # the 80^3 x 6 configuration: HLLC and AUSM+ in FP64, HLLC mixed
