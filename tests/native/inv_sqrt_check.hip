@@ -1,8 +1,9 @@
 // Checks on the GPU that rflu::inv_sqrt_cr (the stencil weights' 1/sqrt) returns the
 // bits of 1.0 / sqrt(x): log-uniform arguments over its fast range [2^-700, 2^700],
-// arguments in [2^-27, 2^27] (cell distances), and perfect squares k^2 with their two
-// neighbouring doubles (the hard cases of a square root's rounding), plus arguments
-// outside the fast range (the library path).  Usage: inv_sqrt_check [log2 samples
+// arguments in [2^-27, 2^27] (cell distances), perfect squares k^2 with their two
+// neighbouring doubles (the hard cases of a square root's rounding), arguments outside
+// the fast range (the library path), and squares of values whose significand is near
+// all ones or near one, with their neighbours (the hard cases of the quotient 1/y).  Usage: inv_sqrt_check [log2 samples
 // per set]; exit status 0 when every result matches.
 #include <hip/hip_runtime.h>
 
@@ -37,6 +38,15 @@ __global__ void k_check(int set, uint64_t n, uint64_t seed, unsigned long long* 
       const int w = (int)(i % 3);
       x = __longlong_as_double((long long)(w == 0 ? b : w == 1 ? b + 1 : b - 1));
       x = ldexp(x, 2 * (int)((r >> 40) % 201) - 200);
+    } else if (set == 4) {
+      // the quotient's hard cases: square roots whose significand is within 1024 ulps of
+      // all ones or of one, with the squares' two neighbouring doubles
+      const double j = (double)(r % 1024);
+      const double q = (r >> 10) & 1 ? 2.0 - j * 0x1p-52 : 1.0 + j * 0x1p-52;
+      const uint64_t b = (uint64_t)__double_as_longlong(q * q);
+      const int w = (int)((r >> 11) % 3);
+      x = __longlong_as_double((long long)(w == 0 ? b : w == 1 ? b + 1 : b - 1));
+      x = ldexp(x, 2 * (int)((r >> 20) % 601) - 600);
     } else {
       x = ldexp(m, (r & 1) ? 700 + (int)((r >> 1) % 300) : -700 - (int)((r >> 1) % 370));
     }
@@ -56,7 +66,7 @@ int main(int argc, char** argv) {
   if (hipMalloc(&bad, sizeof(unsigned long long)) != hipSuccess) return 2;
   if (hipMalloc(&ex, sizeof(double)) != hipSuccess) return 2;
   int rc = 0;
-  for (int set = 0; set < 4; ++set) {
+  for (int set = 0; set < 5; ++set) {
     const uint64_t n = 1ULL << lg;
     if (hipMemset(bad, 0, sizeof(unsigned long long)) != hipSuccess) return 2;
     hipLaunchKernelGGL(k_check, dim3(4096), dim3(256), 0, 0, set, n, 977ULL + set, bad, ex);
