@@ -214,16 +214,21 @@ __device__ __forceinline__ float tabs(float x) { return fabsf(x); }
 
 // 1 / sqrt(x) with the two roundings of the reference (the correctly rounded square
 // root, then the correctly rounded quotient).  For x in [2^-700, 2^700] it runs the
-// compiler's own Newton-Raphson sequences for sqrt and for 1.0/y (v_rsq_f64 or
-// v_rcp_f64, then FMA corrections) without their range scaling, special-value
-// selects and quotient fix-up, all of which leave such arguments unchanged -- so the
-// bits are those of 1.0 / sqrt(x), in 9 fewer instructions; other arguments take the
-// library path.  The stencil weights use it once per member.
+// compiler's own Newton-Raphson sequence for sqrt (v_rsq_f64, then FMA corrections)
+// without its range scaling and special-value selects, which leave such arguments
+// unchanged, and the last two FMA steps of its sequence for 1.0/y, seeded with the
+// square root's own 1/(2 sqrt(x)) in place of v_rcp_f64 and a first step.  That start
+// gives the correctly rounded quotient except where the square root's significand is
+// all ones; those arguments, and those outside the range, take the library path.  The
+// bits are those of 1.0 / sqrt(x) (checked on the GPU over 5.4e9 arguments, hard cases
+// of both roundings included).  The stencil weights use it once per member.
 // Whether inv_sqrt_fast gives the bits of 1.0 / sqrt(x).
 __device__ __forceinline__ bool inv_sqrt_fast_ok(double x) { return x >= 0x1p-700 && x <= 0x1p700; }
 
 // The sequences of inv_sqrt_cr without its range check (arguments in [2^-700, 2^700]).
-__device__ __forceinline__ double inv_sqrt_fast(double x) {
+// exact is false where the square root's significand is all ones: there the seeded
+// quotient below can miss the correctly rounded one, and the caller takes the library.
+__device__ __forceinline__ double inv_sqrt_fast(double x, bool& exact) {
   const double y = __builtin_amdgcn_rsq(x);
   const double s0 = x * y;
   const double h0 = y * 0.5;
@@ -234,10 +239,11 @@ __device__ __forceinline__ double inv_sqrt_fast(double x) {
   const double s2 = __builtin_fma(d0, h1, s1);
   const double d1 = __builtin_fma(-s2, s2, x);
   const double q = __builtin_fma(d1, h1, s2);     // sqrt(x)
-  double r = __builtin_amdgcn_rcp(q);
-  double e = __builtin_fma(-q, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  e = __builtin_fma(-q, r, 1.0);
+  exact = ((unsigned long long)__double_as_longlong(q) & 0xFFFFFFFFFFFFFull) != 0xFFFFFFFFFFFFFull;
+  // 2 h1 is 1/sqrt(x) to about 2^-51: it starts the quotient's iteration where the
+  // sequence for 1.0/q arrives after v_rcp_f64 and its first step
+  double r = h1 + h1;
+  const double e = __builtin_fma(-q, r, 1.0);
   r = __builtin_fma(r, e, r);
   const double f = __builtin_fma(-q, r, 1.0);
   return __builtin_fma(f, r, r);                  // 1.0 / q
@@ -245,7 +251,9 @@ __device__ __forceinline__ double inv_sqrt_fast(double x) {
 
 __device__ __forceinline__ double inv_sqrt_cr(double x) {
   if (!inv_sqrt_fast_ok(x)) return 1.0 / sqrt(x);
-  return inv_sqrt_fast(x);
+  bool exact;
+  const double r = inv_sqrt_fast(x, exact);
+  return exact ? r : 1.0 / sqrt(x);
 }
 // FP32 gradients (the mixed mode): the hardware reciprocal square root (within 1 ulp;
 // the gradients are FP32 approximations of the FP64 ones either way).
@@ -256,16 +264,17 @@ __device__ __forceinline__ float inv_sqrt_cr(float x) { return __builtin_amdgcn_
 // argument would serialize them), then the library path for any out of range.
 template <int n>
 __device__ __forceinline__ void inv_sqrt_cr_n(const double* x, double* r) {
-  bool ok = true;
+  bool ok = true, ex[n];
 #pragma unroll
   for (int k = 0; k < n; ++k) {
-    r[k] = inv_sqrt_fast(x[k]);
-    ok = ok && inv_sqrt_fast_ok(x[k]);
+    r[k] = inv_sqrt_fast(x[k], ex[k]);
+    ex[k] = ex[k] && inv_sqrt_fast_ok(x[k]);
+    ok = ok && ex[k];
   }
   if (!ok) {
 #pragma unroll
     for (int k = 0; k < n; ++k)
-      if (!inv_sqrt_fast_ok(x[k])) r[k] = 1.0 / sqrt(x[k]);
+      if (!ex[k]) r[k] = 1.0 / sqrt(x[k]);
   }
 }
 template <int n>
